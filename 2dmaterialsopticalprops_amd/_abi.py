@@ -1,0 +1,277 @@
+"""ctypes binding of librpa_hip.so (include/rpa_hip.h).
+
+The library is the product: loading fails loudly when it has not been built;
+there is no host fallback for any device entry point.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "librpa_hip.so")
+
+RPA_OK, RPA_E_ARG, RPA_E_STATE, RPA_E_HIP, RPA_E_NOTFOUND, RPA_E_LIMIT, RPA_E_SINGULAR = range(7)
+RPA_SLOSS, RPA_OPTABS = 0, 1
+RPA_FP64, RPA_MIXED = 0, 1
+RPA_MAP_DIRECT, RPA_MAP_IGK = 0, 1
+POL = {"xx": 1, "x": 1, "yy": 2, "y": 2, "zz": 3, "z": 3, "yz": 4}
+
+EXPORTS = [
+    "rpa_ctx_create", "rpa_ctx_destroy", "rpa_last_error", "rpa_last_run_ms",
+    "rpa_set_gvectors", "rpa_upload_basis", "rpa_set_bands", "rpa_set_local_fields",
+    "rpa_set_params", "rpa_set_q", "rpa_vertices", "rpa_accumulate_s0", "rpa_kk",
+    "rpa_coulomb", "rpa_dyson_w", "rpa_epilogue", "rpa_run", "rpa_run_graph",
+    "rpa_get_vertices", "rpa_gen_fbz", "rpa_find_min_q", "rpa_find_kq", "rpa_gen_glf",
+    "rpa_gk_sort", "rpa_gcart",
+]
+
+
+class RpaDims(C.Structure):
+    _fields_ = [("nki", C.c_int), ("nband", C.c_int), ("npwx", C.c_int), ("nlfd", C.c_int),
+                ("no", C.c_int), ("program", C.c_int), ("precision", C.c_int), ("device", C.c_int)]
+
+
+class RpaParams(C.Structure):
+    _fields_ = [("nval", C.c_int), ("vertex_map", C.c_int), ("ngd", C.c_int), ("pol", C.c_int),
+                ("eps", C.c_double), ("gcar", C.c_double), ("domega", C.c_double),
+                ("eta", C.c_double), ("lor_cut", C.c_double), ("df_cut", C.c_double),
+                ("efermi", C.c_double), ("temp", C.c_double), ("vcell", C.c_double),
+                ("c0", C.c_double), ("eta_intra", C.c_double)]
+
+
+class RpaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rpa error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+P = C.c_void_p
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.rpa_ctx_create.argtypes = [C.POINTER(RpaDims), C.POINTER(P)]
+        L.rpa_ctx_destroy.argtypes = [P]
+        L.rpa_last_error.argtypes = [P]
+        L.rpa_last_error.restype = C.c_char_p
+        L.rpa_last_run_ms.argtypes = [P]
+        L.rpa_last_run_ms.restype = C.c_double
+        L.rpa_set_gvectors.argtypes = [P, C.c_int, P, P]
+        L.rpa_upload_basis.argtypes = [P, C.c_int, C.c_int, P, P]
+        L.rpa_set_bands.argtypes = [P, P]
+        L.rpa_set_local_fields.argtypes = [P, C.c_int, P, P]
+        L.rpa_set_params.argtypes = [P, C.POINTER(RpaParams)]
+        L.rpa_set_q.argtypes = [P, C.c_int, P, P, P, P, P]
+        L.rpa_vertices.argtypes = [P]
+        L.rpa_accumulate_s0.argtypes = [P, P, P]
+        L.rpa_kk.argtypes = [P, P]
+        L.rpa_coulomb.argtypes = [P, P]
+        L.rpa_dyson_w.argtypes = [P, P, P, P]
+        L.rpa_epilogue.argtypes = [P, P, P, P]
+        L.rpa_run.argtypes = [P]
+        L.rpa_run_graph.argtypes = [P]
+        L.rpa_get_vertices.argtypes = [P, P, P, C.POINTER(C.c_longlong)]
+        L.rpa_gen_fbz.argtypes = [C.c_int, C.c_int, C.c_double, P, P, C.POINTER(C.c_int), P]
+        L.rpa_find_min_q.argtypes = [C.c_int, P, C.c_int, P]
+        L.rpa_find_kq.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P, P, C.c_double, P, P, P]
+        L.rpa_gen_glf.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_int, P, P, C.c_int,
+                                  C.POINTER(C.c_int), P, P, P]
+        L.rpa_gk_sort.argtypes = [P, C.c_int, P, C.c_double, C.c_int, C.POINTER(C.c_int), P]
+        L.rpa_gcart.argtypes = [C.c_int, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    """Address of a numpy array.  The caller must hold a reference to `a`
+    until the C call returns (never pass the address of a temporary)."""
+    return a.ctypes.data if a is not None else None
+
+
+def _f(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+# ---------------------------------------------------------------------------
+# host setup entry points (no device needed)
+
+def gcart(mill, kc):
+    mill = np.ascontiguousarray(mill, dtype=np.int32)
+    g = np.zeros((len(mill), 3))
+    kcf = _f(kc, np.float64)  # keep the converted array alive across the call
+    _check(None, lib().rpa_gcart(len(mill), _ptr(mill), _ptr(kcf), _ptr(g)))
+    return g
+
+
+def find_min_q(ktot, iq):
+    k = np.ascontiguousarray(ktot, dtype=np.float64)
+    q = np.zeros(3)
+    _check(None, lib().rpa_find_min_q(len(k), _ptr(k), iq, _ptr(q)))
+    return q
+
+
+def find_kq(ktot, ki, g, q, eps):
+    k = np.ascontiguousarray(ktot, dtype=np.float64)
+    kI = np.ascontiguousarray(ki, dtype=np.float64)
+    gg = np.ascontiguousarray(g, dtype=np.float64)
+    qq = np.ascontiguousarray(q, dtype=np.float64)
+    n = len(k)
+    K1, K2, iG0 = (np.zeros(n, dtype=np.int32) for _ in range(3))
+    _check(None, lib().rpa_find_kq(n, _ptr(k), len(kI), _ptr(kI), len(gg), _ptr(gg), _ptr(qq), eps,
+                                   _ptr(K1), _ptr(K2), _ptr(iG0)))
+    return K1, K2, iG0
+
+
+def gen_glf(lf, ecut, gcar, g, mill, nlfd):
+    gg = np.ascontiguousarray(g, dtype=np.float64)
+    mm = np.ascontiguousarray(mill, dtype=np.int32)
+    glf = np.zeros((max(nlfd, 1), 3))
+    parg = np.zeros(max(nlfd, 1), dtype=np.int32)
+    idx = np.zeros(max(nlfd, 1), dtype=np.int32)
+    n = C.c_int()
+    rc = lib().rpa_gen_glf(lf.encode(), ecut, gcar, len(gg), _ptr(gg), _ptr(mm), nlfd, C.byref(n),
+                           _ptr(glf), _ptr(parg), _ptr(idx))
+    m = min(n.value, nlfd)
+    return rc, n.value, glf[:m].copy(), parg[:m].copy(), idx[:m].copy()
+
+
+def gk_sort(k, g, gcutw, npwx):
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    gg = np.ascontiguousarray(g, dtype=np.float64)
+    igk = np.zeros(npwx, dtype=np.int32)
+    n = C.c_int()
+    rc = lib().rpa_gk_sort(_ptr(kk), len(gg), _ptr(gg), gcutw, npwx, C.byref(n), _ptr(igk))
+    return rc, igk[:n.value].copy()
+
+
+def _check(ctx, rc):
+    if rc != RPA_OK:
+        msg = lib().rpa_last_error(ctx).decode() if ctx else ""
+        raise RpaError(rc, msg)
+
+
+class Context:
+    """One device context (rpa_ctx) with numpy-facing methods."""
+
+    def __init__(self, nki, nband, npwx, nlfd, no, program, precision=RPA_FP64, device=0):
+        self.dims = RpaDims(nki, nband, npwx, nlfd, no, program, precision, device)
+        h = P()
+        rc = lib().rpa_ctx_create(C.byref(self.dims), C.byref(h))
+        if rc != RPA_OK:
+            raise RpaError(rc, "rpa_ctx_create failed")
+        self.h = h
+        self.nlf = 0
+        self.no = no
+        self.program = program
+
+    def close(self):
+        if self.h:
+            lib().rpa_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        _check(self.h, rc)
+
+    def set_gvectors(self, mill, kc):
+        mill = np.ascontiguousarray(mill, dtype=np.int32)
+        kcf = _f(kc, np.float64)
+        self._c(lib().rpa_set_gvectors(self.h, len(mill), _ptr(mill), _ptr(kcf)))
+
+    def upload_basis(self, ik, mill, evc):
+        """evc: (ngk, nband) complex128 (the file's band records as columns)."""
+        mill = np.ascontiguousarray(mill, dtype=np.int32)
+        e = np.asfortranarray(evc, dtype=np.complex128)
+        self._c(lib().rpa_upload_basis(self.h, ik, len(mill), _ptr(mill), _ptr(e)))
+
+    def set_bands(self, E):
+        Ef = _f(E, np.float64)
+        self._c(lib().rpa_set_bands(self.h, _ptr(Ef)))
+
+    def set_local_fields(self, lf_idx, parg):
+        li = np.ascontiguousarray(lf_idx, dtype=np.int32)
+        pg = np.ascontiguousarray(parg, dtype=np.int32)
+        self.nlf = len(li)
+        self._c(lib().rpa_set_local_fields(self.h, len(li), _ptr(li), _ptr(pg)))
+
+    def set_params(self, **kw):
+        p = RpaParams()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self.params = p
+        self._c(lib().rpa_set_params(self.h, C.byref(p)))
+
+    def set_q(self, K1, K2, iG0, ktot, q):
+        K1 = np.ascontiguousarray(K1, dtype=np.int32)
+        K2 = np.ascontiguousarray(K2, dtype=np.int32)
+        iG0 = np.ascontiguousarray(iG0, dtype=np.int32)
+        kt = np.ascontiguousarray(ktot, dtype=np.float64)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        self._c(lib().rpa_set_q(self.h, len(K1), _ptr(K1), _ptr(K2), _ptr(iG0), _ptr(kt), _ptr(qq)))
+
+    # ---- hot path
+    def vertices(self):
+        self._c(lib().rpa_vertices(self.h))
+
+    def accumulate_s0(self, want=True):
+        nlf = self.nlf
+        S0 = Q = None
+        if want:
+            if self.program == RPA_SLOSS:
+                S0 = np.zeros((self.no, nlf, nlf), order="F")
+            else:
+                S0 = np.zeros((2 * self.no + 1, nlf, nlf), dtype=np.complex128, order="F")
+                Q = np.zeros((nlf, nlf), dtype=np.complex128, order="F")
+        self._c(lib().rpa_accumulate_s0(self.h, _ptr(S0), _ptr(Q)))
+        return S0, Q
+
+    def kk(self, want=True):
+        out = np.zeros((self.no - 1, self.nlf, self.nlf), dtype=np.complex128, order="F") if want else None
+        self._c(lib().rpa_kk(self.h, _ptr(out)))
+        return out
+
+    def coulomb(self):
+        V = np.zeros((self.nlf, self.nlf), order="F")
+        self._c(lib().rpa_coulomb(self.h, _ptr(V)))
+        return V
+
+    def dyson_w(self, want=True):
+        shp = (self.no - 1, self.nlf, self.nlf)
+        outs = [np.zeros(shp, dtype=np.complex128, order="F") for _ in range(3)] if want else [None] * 3
+        self._c(lib().rpa_dyson_w(self.h, *[_ptr(o) for o in outs]))
+        return outs
+
+    def epilogue(self):
+        nf = self.no - 1
+        loss = np.zeros(nf)
+        pi11 = np.zeros(nf, dtype=np.complex128)
+        sig = np.zeros((2, nf))
+        self._c(lib().rpa_epilogue(self.h, _ptr(loss), _ptr(pi11), _ptr(sig)))
+        return loss, pi11, sig
+
+    def run(self, graph=False):
+        self._c((lib().rpa_run_graph if graph else lib().rpa_run)(self.h))
+        return lib().rpa_last_run_ms(self.h)
+
+    def get_vertices(self, both=False):
+        n = C.c_longlong()
+        self._c(lib().rpa_get_vertices(self.h, None, None, C.byref(n)))
+        r1 = np.zeros((self.nlf, n.value), dtype=np.complex128, order="F")
+        r2 = np.zeros_like(r1) if both else None
+        self._c(lib().rpa_get_vertices(self.h, _ptr(r1), _ptr(r2), C.byref(n)))
+        return r1, r2

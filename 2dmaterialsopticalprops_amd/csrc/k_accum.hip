@@ -1,0 +1,514 @@
+// Spectral-function accumulation S0(w,G,G') = sum_t W(w,t) rho_t(G) conj(rho'_t(G'))
+// (Sloss.f90:407-430 real part; Optabs.f90:386-461 complex) as a banded
+// grouped GEMM on the FP64 matrix cores.
+//
+//  1. k_transitions   per transition: energies, occupation filter, and the
+//                     first frequency tile ("home") its Lorentzian window
+//                     reaches (-1 if none; intraband Qeff pairs get the
+//                     pseudo tile ntiles).
+//  2. k_tile_hist / k_tile_scan / k_tile_scatter
+//                     stable counting sort by home tile -> perm[t] (natural ->
+//                     sorted position).  The vertex kernel writes its rows at
+//                     perm[t], so no data moves for the sort.
+//  3. k_worklist      per tile b the candidates are the contiguous sorted
+//                     range of homes [b-span, b]; cut into chunks of kChunkT.
+//  4. k_accum         persistent kernel over (chunk, column block): a 32 x 256
+//                     tile of S0 += W (32 x K) P (K x 256), W computed from the
+//                     energies with the reference's exact cutoff test, P from
+//                     the vertices staged in LDS.  Partials per chunk.
+//  5. k_reduce        fixed-order Neumaier sum of the chunk partials of each
+//                     tile -> S0 (deterministic, FP64).
+//  6. k_qeff          Optabs intraband Qeff (Optabs.f90:430-442).
+#include "rpa_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace rpa {
+
+constexpr int kKS = 16;          // transitions per LDS stage
+constexpr int kMaxTiles = 1024;
+
+struct TransArgs {
+  const int *k1, *k2;
+  const double *E;
+  int nki, nband;
+  PairTable pairs;
+  long long ntrans;
+  int program;
+  double domega, hw;
+  int shift;  // frequency index offset (Optabs: no)
+  int nw, ntiles;
+  double efermi, temp, df_cut;
+  int *key;
+};
+
+__device__ inline void trans_decode(const PairTable &pt, long long t, int &k, int &n, int &m) {
+  k = (int)(t / pt.tpk);
+  int r = (int)(t - (long long)k * pt.tpk);
+  int pb = 0;
+#pragma unroll
+  for (int b = 1; b < kMaxPairBlocks; ++b)
+    if (b < pt.nblocks && r >= pt.off[b]) pb = b;
+  const PairBlock &B = pt.b[pb];
+  int loc = r - pt.off[pb];
+  int nm = B.m_hi - B.m_lo;
+  n = B.n_lo + loc / nm;
+  m = B.m_lo + loc % nm;
+}
+
+// Optabs genOccupation (Optabs.f90:935-951)
+__device__ inline void occupation(double ei, double ef, double T, double &expo, double &f) {
+#pragma clang fp contract(off)
+  double x = (ei - ef) / T;
+  if (x < -20.0) {
+    expo = 0.0;
+    f = 1.0;
+  } else if (x > 20.0) {
+    expo = x;
+    f = 0.0;
+  } else {
+    expo = exp(x);
+    f = 1.0 / (expo + 1.0);
+  }
+}
+
+__global__ void k_transitions(TransArgs a) {
+#pragma clang fp contract(off)
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.ntrans) return;
+  int k, n, m;
+  trans_decode(a.pairs, t, k, n, m);
+  const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
+  const double e1 = a.E[K1 + (size_t)a.nki * n];
+  const double e2 = a.E[K2 + (size_t)a.nki * m];
+  bool inter = true, intra = false;
+  if (a.program == RPA_OPTABS) {
+    double x1, f1, x2, f2;
+    occupation(e1, a.efermi, a.temp, x1, f1);
+    occupation(e2, a.efermi, a.temp, x2, f2);
+    double df = f1 - f2;
+    inter = (n != m) && (fabs(df) >= a.df_cut);
+    intra = (n == m) && (fabs(e1 - a.efermi) <= 10.0 * a.temp);
+  }
+  int key = -1;
+  if (inter) {
+    int wlo, whi;
+    if (isinf(a.hw)) {
+      wlo = 0;
+      whi = a.nw - 1;
+    } else {
+      double res = e2 - e1;
+      double lo = ceil((res - a.hw) / a.domega) - 2.0 + a.shift;
+      double hi = floor((res + a.hw) / a.domega) + 2.0 + a.shift;
+      lo = fmax(lo, 0.0);
+      hi = fmin(hi, (double)(a.nw - 1));
+      wlo = (int)lo;
+      whi = (int)hi;
+      if (!(lo <= hi)) wlo = 1, whi = 0;
+    }
+    if (wlo <= whi) key = wlo / kTileW;
+  } else if (intra) {
+    key = a.ntiles;
+  }
+  a.key[t] = key;
+}
+
+__global__ __launch_bounds__(256) void k_tile_hist(const int *key, long long ntrans, int nkeys,
+                                                    int nblk, int *bcnt) {
+  __shared__ int h[kMaxTiles + 1];
+  for (int i = threadIdx.x; i < nkeys; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntrans) {
+    int k = key[t];
+    if (k >= 0) atomicAdd(&h[k], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nkeys; i += blockDim.x) bcnt[(size_t)i * nblk + blockIdx.x] = h[i];
+}
+
+// Exclusive scan of bcnt (key-major) in place by one workgroup; toff[key] =
+// first sorted position of key, toff[nkeys] = total.
+__global__ __launch_bounds__(1024) void k_tile_scan(int *bcnt, long long n, int nkeys, int nblk,
+                                                     int *toff) {
+  __shared__ long long part[1024];
+  const int tid = threadIdx.x;
+  long long seg = (n + 1023) / 1024;
+  long long lo = tid * seg, hi = lo + seg < n ? lo + seg : n;
+  long long s = 0;
+  for (long long i = lo; i < hi; ++i) s += bcnt[i];
+  part[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    long long run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      long long v = part[i];
+      part[i] = run;
+      run += v;
+    }
+    toff[nkeys] = (int)run;
+  }
+  __syncthreads();
+  long long run = part[tid];
+  for (long long i = lo; i < hi; ++i) {
+    int v = bcnt[i];
+    bcnt[i] = (int)run;
+    run += v;
+  }
+  __syncthreads();
+  for (int kk = tid; kk < nkeys; kk += 1024) toff[kk] = bcnt[(size_t)kk * nblk];
+}
+
+struct ScatterArgs {
+  const int *key;
+  const int *bcnt;
+  long long ntrans;
+  int nblk;
+  const int *k1, *k2;
+  const double *E;
+  int nki;
+  PairTable pairs;
+  int program;
+  int ntiles;
+  double efermi, temp;
+  int *perm, *nat;
+  double *te;
+};
+
+__global__ __launch_bounds__(256) void k_tile_scatter(ScatterArgs a) {
+#pragma clang fp contract(off)
+  __shared__ int keys[256];
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int key = (t < a.ntrans) ? a.key[t] : -1;
+  keys[threadIdx.x] = key;
+  __syncthreads();
+  if (t >= a.ntrans) return;
+  if (key < 0) {
+    a.perm[t] = -1;
+    return;
+  }
+  int rank = 0;
+  for (int j = 0; j < (int)threadIdx.x; ++j) rank += (keys[j] == key);
+  int pos = a.bcnt[(size_t)key * a.nblk + blockIdx.x] + rank;
+  a.perm[t] = pos;
+  a.nat[pos] = (int)t;
+  int k, n, m;
+  trans_decode(a.pairs, t, k, n, m);
+  const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
+  const double e1 = a.E[K1 + (size_t)a.nki * n];
+  const double e2 = a.E[K2 + (size_t)a.nki * m];
+  double z = 1.0;
+  if (a.program == RPA_OPTABS) {
+    double x1, f1, x2, f2;
+    occupation(e1, a.efermi, a.temp, x1, f1);
+    occupation(e2, a.efermi, a.temp, x2, f2);
+    if (key == a.ntiles) {
+      // intraband: fact = -expo/((expo+1)^2 T)  (Optabs.f90:432-433)
+      double fact = x1 / ((x1 + 1.0) * (x1 + 1.0));
+      z = -fact / a.temp;
+    } else {
+      z = f1 - f2;
+    }
+  }
+  a.te[3 * (size_t)pos + 0] = e1;
+  a.te[3 * (size_t)pos + 1] = e2;
+  a.te[3 * (size_t)pos + 2] = z;
+}
+
+__global__ __launch_bounds__(1024) void k_worklist(const int *toff, int ntiles, int span, int kc,
+                                                    int *items, int *first_item, int *nitems) {
+  __shared__ int cnt[kMaxTiles + 1];
+  for (int b = threadIdx.x; b < ntiles; b += blockDim.x) {
+    int lo = toff[b - span > 0 ? b - span : 0], hi = toff[b + 1];
+    cnt[b] = hi > lo ? (hi - lo + kc - 1) / kc : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int b = 0; b < ntiles; ++b) {
+      int v = cnt[b];
+      cnt[b] = run;
+      run += v;
+    }
+    cnt[ntiles] = run;
+    nitems[0] = run;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b <= ntiles; b += blockDim.x) first_item[b] = cnt[b];
+  for (int b = threadIdx.x; b < ntiles; b += blockDim.x) {
+    int lo = toff[b - span > 0 ? b - span : 0], hi = toff[b + 1];
+    int it = cnt[b];
+    for (int s = lo; s < hi; s += kc, ++it) {
+      items[3 * it + 0] = b;
+      items[3 * it + 1] = s;
+      items[3 * it + 2] = s + kc < hi ? s + kc : hi;
+    }
+  }
+}
+
+struct AccArgs {
+  const double *te;
+  const double2 *rho1, *rho2;
+  const int *items;
+  const int *nitems;
+  int npb;  // column blocks of 256
+  int nlf, nw, ncol, ncol_pad;
+  int program;
+  int shift;
+  double domega, eta, cut, den;
+  double *part;
+};
+
+// Weight of transition (e1, e2, z) at frequency row w, with the reference's
+// exact arithmetic and cutoff test (Sloss.f90:408-411, Optabs.f90:391-402).
+__device__ inline double weight(int program, int w, int shift, double domega, double eta,
+                                double cut, double den, double e1, double e2, double z) {
+#pragma clang fp contract(off)
+  double o = (double)(w - shift) * domega;
+  double dE = (o + e1) - e2;
+  if (program == RPA_SLOSS) {
+    double lor = -eta / (dE * dE + eta * eta);
+    if (!(fabs(lor) >= cut)) return 0.0;
+    return -(2.0 * lor) / den;
+  }
+  double lor = eta / (dE * dE + eta * eta);
+  if (!(fabs(lor) >= cut)) return 0.0;
+  return -((2.0 * z) * lor) / den;
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_accum(AccArgs a) {
+  extern __shared__ double smem[];
+  double *sW = smem;                                   // [32][kKS+1]
+  double2 *sR1 = (double2 *)(smem + kTileW * (kKS + 1));  // [kKS][nlf]
+  double2 *sR2 = sR1 + kKS * a.nlf;                    // [kKS][nlf] (CPLX)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, q4 = lane >> 4;
+  const int total = a.nitems[0] * a.npb;
+
+  for (int wk = blockIdx.x; wk < total; wk += gridDim.x) {
+    const int item = wk / a.npb, pbk = wk - item * a.npb;
+    const int b = a.items[3 * item], t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
+
+    // columns owned by this lane in the four 16-wide blocks of its wave
+    int cg_i[4], cg_j[4], cg_p[4];
+    bool cg_ok[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      int col = pbk * kColsWG + wave * 64 + cb * 16 + r;
+      int pair = CPLX ? (col >> 1) : col;
+      cg_p[cb] = CPLX ? (col & 1) : 0;
+      cg_i[cb] = pair / a.nlf;
+      cg_j[cb] = pair - cg_i[cb] * a.nlf;
+      cg_ok[cb] = col < a.ncol;
+      if (!cg_ok[cb]) cg_i[cb] = cg_j[cb] = 0;
+    }
+    rpa_d4 acc[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = rpa_d4{0.0, 0.0, 0.0, 0.0};
+
+    for (int t0 = t_lo; t0 < t_hi; t0 += kKS) {
+      __syncthreads();
+      for (int x = tid; x < kTileW * kKS; x += 256) {
+        int row = x / kKS, kt = x - row * kKS;
+        int t = t0 + kt, w = b * kTileW + row;
+        double v = 0.0;
+        if (t < t_hi && w < a.nw) {
+          const double *e = a.te + 3 * (size_t)t;
+          v = weight(a.program, w, a.shift, a.domega, a.eta, a.cut, a.den, e[0], e[1], e[2]);
+        }
+        sW[row * (kKS + 1) + kt] = v;
+      }
+      const int nst = kKS * a.nlf;
+      for (int x = tid; x < nst; x += 256) {
+        int kt = x / a.nlf;
+        int t = t0 + kt;
+        bool ok = t < t_hi;
+        sR1[x] = ok ? a.rho1[(size_t)t0 * a.nlf + x] : make_double2(0.0, 0.0);
+        if (CPLX) sR2[x] = ok ? a.rho2[(size_t)t0 * a.nlf + x] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int st = 0; st < kKS / 4; ++st) {
+        const int kt = st * 4 + q4;
+        double av[2];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) av[rb] = sW[(rb * 16 + r) * (kKS + 1) + kt];
+        const double2 *r1 = sR1 + kt * a.nlf;
+        const double2 *r2 = (CPLX ? sR2 : sR1) + kt * a.nlf;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          double2 x = r1[cg_i[cb]];
+          double2 y = r2[cg_j[cb]];
+          double bv;
+          if (!CPLX) {
+            bv = x.x * y.x + x.y * y.y;  // Re(x conj y)
+          } else {
+            bv = cg_p[cb] == 0 ? (x.x * y.x + x.y * y.y) : (x.y * y.x - x.x * y.y);
+          }
+          bv = cg_ok[cb] ? bv : 0.0;
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) acc[rb][cb] = mfma_f64_16x16x4(av[rb], bv, acc[rb][cb]);
+        }
+      }
+    }
+    double *dst = a.part + ((size_t)item * a.npb + pbk) * (kTileW * kColsWG);
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          int row = rb * 16 + 4 * e + q4;
+          int col = wave * 64 + cb * 16 + r;
+          dst[row * kColsWG + col] = acc[rb][cb][e];
+        }
+  }
+}
+
+// S0[w][col] = sum over the chunks of tile w/32 of their partials (Neumaier).
+__global__ void k_reduce(const double *part, const int *first_item, int npb, int nw, int ncol_pad,
+                         double *s0) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)nw * ncol_pad) return;
+  const int w = (int)(idx / ncol_pad), col = (int)(idx - (long long)w * ncol_pad);
+  const int b = w / kTileW, row = w - b * kTileW;
+  const int pbk = col / kColsWG, cc = col - pbk * kColsWG;
+  const int i0 = first_item[b], i1 = first_item[b + 1];
+  double s = 0.0, comp = 0.0;
+  for (int it = i0; it < i1; ++it) {
+    double v = part[((size_t)it * npb + pbk) * (kTileW * kColsWG) + row * kColsWG + cc];
+    double tt = s + v;
+    if (fabs(s) >= fabs(v))
+      comp += (s - tt) + v;
+    else
+      comp += (v - tt) + s;
+    s = tt;
+  }
+  s0[idx] = s + comp;
+}
+
+// Qeff(iG,jG) = sum_intra 2 fact j1(iG) conj(j2(jG)) / (Ntot Vcell)
+__global__ void k_qeff(const double *te, const double2 *rho1, const double2 *rho2, const int *toff,
+                       int ntiles, int nlf, double dq, double *qeff) {
+#pragma clang fp contract(off)
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nlf * nlf) return;
+  int iG = idx / nlf, jG = idx - iG * nlf;
+  double sr = 0.0, si = 0.0;
+  for (int t = toff[ntiles]; t < toff[ntiles + 1]; ++t) {
+    double f = 2.0 * te[3 * (size_t)t + 2];
+    double2 x = rho1[(size_t)t * nlf + iG], y = rho2[(size_t)t * nlf + jG];
+    double xr = f * x.x, xi = f * x.y;
+    sr += (xr * y.x + xi * y.y) / dq;
+    si += (xi * y.x - xr * y.y) / dq;
+  }
+  qeff[2 * idx] = sr;
+  qeff[2 * idx + 1] = si;
+}
+
+// ---------------------------------------------------------------------------
+
+int launch_transitions(rpa_ctx *c) {
+  const long long nt = c->ntrans;
+  const int nblk = (int)((nt + 255) / 256);
+  const int nkeys = c->ntiles + 1;
+  if (nkeys > kMaxTiles) return fail(c, RPA_E_LIMIT, "too many frequency tiles");
+  if (nt > (long long)INT32_MAX) return fail(c, RPA_E_LIMIT, "too many transitions");
+  TransArgs ta{};
+  ta.k1 = c->d_k1;
+  ta.k2 = c->d_k2;
+  ta.E = c->d_E;
+  ta.nki = c->d.nki;
+  ta.nband = c->d.nband;
+  ta.pairs = c->pairs;
+  ta.ntrans = nt;
+  ta.program = c->d.program;
+  ta.domega = c->p.domega;
+  ta.shift = (c->d.program == RPA_OPTABS) ? c->d.no : 0;
+  ta.nw = c->nw;
+  ta.ntiles = c->ntiles;
+  ta.efermi = c->p.efermi;
+  ta.temp = c->p.temp;
+  ta.df_cut = c->p.df_cut;
+  ta.key = c->d_key;
+  // window half-width: |Lor| >= cut  <=>  dE^2 <= eta/cut - eta^2
+  double h2 = c->p.eta / c->wcut - c->p.eta * c->p.eta;
+  ta.hw = (c->wcut > 0.0) ? std::sqrt(std::max(h2, 0.0)) : INFINITY;
+  if (nt == 0) return 0;
+  hipLaunchKernelGGL(k_transitions, dim3(nblk), dim3(256), 0, c->stream, ta);
+  hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(256), 0, c->stream, (const int *)c->d_key, nt,
+                     nkeys, nblk, c->d_bcnt);
+  hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, c->stream, c->d_bcnt,
+                     (long long)nkeys * nblk, nkeys, nblk, c->d_toff);
+  ScatterArgs sa{};
+  sa.key = c->d_key;
+  sa.bcnt = c->d_bcnt;
+  sa.ntrans = nt;
+  sa.nblk = nblk;
+  sa.k1 = c->d_k1;
+  sa.k2 = c->d_k2;
+  sa.E = c->d_E;
+  sa.nki = c->d.nki;
+  sa.pairs = c->pairs;
+  sa.program = c->d.program;
+  sa.ntiles = c->ntiles;
+  sa.efermi = c->p.efermi;
+  sa.temp = c->p.temp;
+  sa.perm = c->d_perm;
+  sa.nat = c->d_nat;
+  sa.te = c->d_te;
+  hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(256), 0, c->stream, sa);
+  hipLaunchKernelGGL(k_worklist, dim3(1), dim3(1024), 0, c->stream, (const int *)c->d_toff,
+                     c->ntiles, c->span, kChunkT, c->d_items, c->d_first_item, c->d_nitems);
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int launch_accumulate(rpa_ctx *c) {
+  AccArgs a{};
+  a.te = c->d_te;
+  a.rho1 = (const double2 *)c->d_rho1;
+  a.rho2 = (const double2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
+  a.items = c->d_items;
+  a.nitems = c->d_nitems;
+  a.npb = c->ncol_pad / kColsWG;
+  a.nlf = c->nlf;
+  a.nw = c->nw;
+  a.ncol = c->ncol;
+  a.ncol_pad = c->ncol_pad;
+  a.program = c->d.program;
+  a.shift = (c->d.program == RPA_OPTABS) ? c->d.no : 0;
+  a.domega = c->p.domega;
+  a.eta = c->p.eta;
+  a.cut = c->wcut;
+  a.den = (M_PI * (double)c->ntot) * c->p.vcell;
+  a.part = c->d_part;
+  const bool cplx = (c->d.program == RPA_OPTABS);
+  const size_t lds = sizeof(double) * kTileW * (kKS + 1) + sizeof(double2) * kKS * c->nlf * (cplx ? 2 : 1);
+  long long work = (long long)c->max_items * a.npb;
+  int grid = (int)std::min<long long>(std::max<long long>(work, 1), 4096);
+  if (cplx)
+    hipLaunchKernelGGL(k_accum<true>, dim3(grid), dim3(256), lds, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_accum<false>, dim3(grid), dim3(256), lds, c->stream, a);
+  const long long n = (long long)c->nw * c->ncol_pad;
+  hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     (const double *)c->d_part, (const int *)c->d_first_item, a.npb, c->nw,
+                     c->ncol_pad, c->d_s0);
+  if (cplx) {
+    const int nn = c->nlf * c->nlf;
+    hipLaunchKernelGGL(k_qeff, dim3((nn + 255) / 256), dim3(256), 0, c->stream,
+                       (const double *)c->d_te, (const double2 *)c->d_rho1,
+                       (const double2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1),
+                       (const int *)c->d_toff, c->ntiles, c->nlf,
+                       (double)c->ntot * c->p.vcell, c->d_qeff);
+  }
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace rpa

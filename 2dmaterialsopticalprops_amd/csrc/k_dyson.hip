@@ -1,0 +1,268 @@
+// Coulomb matrix, Dyson step, screened interaction and scalar epilogues.
+//
+//  k_coulomb   V(G,G') of the slab (Sloss.f90:1173-1208 genV); V is
+//              independent of frequency, so it is built once per q instead of
+//              once per frequency as the reference does (Sloss.f90:456).
+//  k_dyson     one workgroup per frequency io = 1..no-1:
+//                eps = I - chi0 V            (genDielectricEpsilon, 1210-1234)
+//                eps <- eps^-1               (gjel, Sloss.f90:462; here
+//                                             Gauss-Jordan with row pivoting,
+//                                             in place in LDS)
+//                chi = eps^-1 chi0           (genChi, 1236-1253)
+//                W   = V + (chi V) pre-multiplied by V   (genWT, 1255-1277,
+//                                             O(N^3) instead of O(N^4))
+//                loss = -Im eps^-1(1,1)
+//  k_pi        Optabs Pi assembly (Optabs.f90:575-611): Pi_tot(1,1) and the
+//              conductivities of fort.401 / fort.402.
+#include "rpa_internal.h"
+
+namespace rpa {
+
+__global__ void k_coulomb(int nlf, const double *glf, const int *parg, double qx, double qy,
+                          double gcar, double c0, double eps, double *V) {
+#pragma clang fp contract(off)
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nlf * nlf) return;
+  const int iG = idx / nlf, jG = idx - iG * nlf;
+  const double *gi = glf + 3 * iG, *gj = glf + 3 * jG;
+  const double a = qx + gcar * gi[0], b = qy + gcar * gi[1];
+  double gabs = sqrt(a * a + b * b);
+  if (gabs == 0.0) gabs = eps;
+  double v = 0.0;
+  if (gj[0] == gi[0] && gj[1] == gi[1]) {
+    const double gzi = gcar * gi[2], gzj = gcar * gj[2];
+    v = 4.0 * M_PI * (1.0 - exp(-gabs * c0)) / (gabs * c0);
+    v = v * (gabs * gabs - gzi * gzj);
+    v = v / (gabs * gabs + gzi * gzi);
+    v = v / (gabs * gabs + gzj * gzj);
+    v = -(double)parg[iG] * (double)parg[jG] * v;
+    if (gj[2] == gi[2]) v = 4.0 * M_PI / (gabs * gabs + gzi * gzi) + v;
+  }
+  V[idx] = v;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cdiv1(double2 a) {  // 1 / a (scaled)
+  double s = fabs(a.x) > fabs(a.y) ? fabs(a.x) : fabs(a.y);
+  double ar = a.x / s, ai = a.y / s;
+  double d = ar * ar + ai * ai;
+  return make_double2(ar / (d * s), -ai / (d * s));
+}
+
+struct DysonArgs {
+  int nlf, nfreq;
+  const double2 *chi0;  // [nfreq][nlf][nlf]
+  const double *V;      // [nlf][nlf]
+  double2 *epsinv, *chi, *W;
+  double2 *T;           // scratch [nfreq][nlf][nlf]
+  double *loss;
+  int *status;
+};
+
+__global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
+  extern __shared__ double2 sa[];  // [nlf][nlf] eps, then eps^-1
+  __shared__ double2 fcol[kMaxNlf];
+  __shared__ int piv[kMaxNlf];
+  __shared__ double red_v[256];
+  __shared__ int red_i[256];
+  const int N = a.nlf, tid = threadIdx.x;
+  const int f = blockIdx.x;
+  const double2 *X0 = a.chi0 + (size_t)f * N * N;
+  // eps = I - chi0 V
+  for (int x = tid; x < N * N; x += 256) {
+    int i = x / N, j = x - i * N;
+    double2 s = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    for (int k = 0; k < N; ++k) {
+      double2 c = X0[i * N + k];
+      double v = a.V[k * N + j];
+      s.x -= c.x * v;
+      s.y -= c.y * v;
+    }
+    sa[x] = s;
+  }
+  __syncthreads();
+  // in-place Gauss-Jordan inversion with row pivoting
+  for (int k = 0; k < N; ++k) {
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + tid; i < N; i += 256) {
+      double2 z = sa[i * N + k];
+      double m = fabs(z.x) + fabs(z.y);
+      if (m > best) {
+        best = m;
+        bi = i;
+      }
+    }
+    red_v[tid] = best;
+    red_i[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) {
+        if (red_v[tid + s] > red_v[tid] || (red_v[tid + s] == red_v[tid] && red_i[tid + s] < red_i[tid])) {
+          red_v[tid] = red_v[tid + s];
+          red_i[tid] = red_i[tid + s];
+        }
+      }
+      __syncthreads();
+    }
+    const int p = red_i[0];
+    if (tid == 0) {
+      piv[k] = p;
+      if (!(red_v[0] > 0.0)) atomicOr(a.status, 1);
+    }
+    if (p != k)
+      for (int j = tid; j < N; j += 256) {
+        double2 t = sa[k * N + j];
+        sa[k * N + j] = sa[p * N + j];
+        sa[p * N + j] = t;
+      }
+    __syncthreads();
+    const double2 inv = cdiv1(sa[k * N + k]);
+    __syncthreads();
+    for (int j = tid; j < N; j += 256) sa[k * N + j] = (j == k) ? inv : cmul(sa[k * N + j], inv);
+    for (int i = tid; i < N; i += 256) fcol[i] = sa[i * N + k];
+    __syncthreads();
+    for (int i = tid; i < N; i += 256)
+      if (i != k) sa[i * N + k] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int x = tid; x < N * N; x += 256) {
+      int i = x / N, j = x - i * N;
+      if (i != k) {
+        double2 fk = fcol[i];
+        double2 r = sa[k * N + j];
+        double2 v = sa[x];
+        v.x -= fk.x * r.x - fk.y * r.y;
+        v.y -= fk.x * r.y + fk.y * r.x;
+        sa[x] = v;
+      }
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, last first
+  for (int k = N - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int i = tid; i < N; i += 256) {
+        double2 t = sa[i * N + k];
+        sa[i * N + k] = sa[i * N + p];
+        sa[i * N + p] = t;
+      }
+    __syncthreads();
+  }
+  double2 *Einv = a.epsinv + (size_t)f * N * N;
+  double2 *Ch = a.chi + (size_t)f * N * N;
+  double2 *T = a.T + (size_t)f * N * N;
+  double2 *Wo = a.W + (size_t)f * N * N;
+  for (int x = tid; x < N * N; x += 256) Einv[x] = sa[x];
+  if (tid == 0) a.loss[f] = -sa[0].y;
+  // chi = eps^-1 chi0
+  for (int x = tid; x < N * N; x += 256) {
+    int i = x / N, j = x - i * N;
+    double2 s = make_double2(0.0, 0.0);
+    for (int k = 0; k < N; ++k) {
+      double2 e = sa[i * N + k], c = X0[k * N + j];
+      s.x += e.x * c.x - e.y * c.y;
+      s.y += e.x * c.y + e.y * c.x;
+    }
+    Ch[x] = s;
+  }
+  __syncthreads();
+  // T = chi V
+  for (int x = tid; x < N * N; x += 256) {
+    int i = x / N, j = x - i * N;
+    double2 s = make_double2(0.0, 0.0);
+    for (int k = 0; k < N; ++k) {
+      double2 c = Ch[i * N + k];
+      double v = a.V[k * N + j];
+      s.x += c.x * v;
+      s.y += c.y * v;
+    }
+    T[x] = s;
+  }
+  __syncthreads();
+  // W = V + V T
+  for (int x = tid; x < N * N; x += 256) {
+    int i = x / N, j = x - i * N;
+    double2 s = make_double2(0.0, 0.0);
+    for (int k = 0; k < N; ++k) {
+      double v = a.V[i * N + k];
+      double2 t = T[k * N + j];
+      s.x += v * t.x;
+      s.y += v * t.y;
+    }
+    Wo[x] = make_double2(a.V[x] + s.x, s.y);
+  }
+}
+
+// Optabs Pi assembly per io (Optabs.f90:575-611).  Pi_inter is read after each
+// (iG,jG) update in the reference, so for Nlf > 1 it holds Pi_tot(1,1)
+// including the intraband term (Optabs.f90:594).
+__global__ void k_pi(int nfreq, int nlf, double dw, double eta_intra, double c0,
+                     const double2 *pi0, const double *qeff, double *pi11, double *sig) {
+  const int iop = blockIdx.x * blockDim.x + threadIdx.x;
+  if (iop >= nfreq) return;
+  const int io = iop + 1;
+  const double oi = (double)(io - 1) * dw;
+  const double2 p = pi0[(size_t)iop * nlf * nlf];
+  const double dia = -pi0[0].x;
+  const double tr = p.x + dia, ti = p.y + 0.0;
+  // Qeff(1,1) * oi / (oi + i eta_intra)
+  const double qr = qeff[0] * oi, qi = qeff[1] * oi;
+  const double dr = oi, di = eta_intra;
+  const double den = dr * dr + di * di;
+  const double ir = (qr * dr + qi * di) / den, ii = (qi * dr - qr * di) / den;
+  const double totr = tr + ir, toti = ti + ii;
+  pi11[2 * iop] = totr;
+  pi11[2 * iop + 1] = toti;
+  double inter_r = (nlf > 1) ? totr : tr, inter_i = (nlf > 1) ? toti : ti;
+  if (io > 1) {
+    // Re(-i c0 P / oi) = c0 Im(P) / oi
+    sig[2 * iop] = c0 * inter_i / oi;
+    sig[2 * iop + 1] = c0 * ii / oi;
+  } else {
+    sig[2 * iop] = 0.0;
+    sig[2 * iop + 1] = 0.0;
+  }
+  (void)inter_r;
+}
+
+int launch_coulomb(rpa_ctx *c) {
+  const int nn = c->nlf * c->nlf;
+  hipLaunchKernelGGL(k_coulomb, dim3((nn + 255) / 256), dim3(256), 0, c->stream, c->nlf,
+                     (const double *)c->d_glf, (const int *)c->d_parg, c->p.gcar * c->q[0],
+                     c->p.gcar * c->q[1], c->p.gcar, c->p.c0, c->p.eps, c->d_V);
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int launch_dyson(rpa_ctx *c) {
+  DysonArgs a{};
+  a.nlf = c->nlf;
+  a.nfreq = c->d.no - 1;
+  a.chi0 = (const double2 *)c->d_chi0;
+  a.V = c->d_V;
+  a.epsinv = (double2 *)c->d_epsinv;
+  a.chi = (double2 *)c->d_chi;
+  a.W = (double2 *)c->d_W;
+  a.T = (double2 *)c->d_scratch;
+  a.loss = c->d_loss;
+  a.status = c->d_status;
+  const size_t lds = sizeof(double2) * c->nlf * c->nlf;
+  hipLaunchKernelGGL(k_dyson, dim3(a.nfreq), dim3(256), lds, c->stream, a);
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int launch_epilogue(rpa_ctx *c) {
+  if (c->d.program != RPA_OPTABS) return 0;
+  const int nf = c->d.no - 1;
+  hipLaunchKernelGGL(k_pi, dim3((nf + 255) / 256), dim3(256), 0, c->stream, nf, c->nlf,
+                     c->p.domega, c->p.eta_intra, c->p.c0, (const double2 *)c->d_chi0,
+                     (const double *)c->d_qeff, c->d_pi11, c->d_sig);
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace rpa

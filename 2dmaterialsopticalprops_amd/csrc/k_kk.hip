@@ -1,0 +1,202 @@
+// Kramers-Kronig transform of S0 as a GEMM with the reference's discrete
+// stencil:  X[io'][col] = sum_w K[io'][w] S0[w][col],  io = io'+1 = 1..no-1.
+//
+//  K2 (Sloss genChi0, Sloss.f90:954-1059): if / else-if chain, so the
+//      corrections act only at jo == io (coefficient 0); io = 1 is the static
+//      limit -2 sum f_j S0(j) with f_2 = 3/2 and a half weight at jo = no.
+//  K1 (Optabs genReChi0/genImChi0, Optabs.f90:1112-1318; equal to the complex
+//      variant of Modules/response_function_KK.f90:74-178): the same c+ and c-
+//      weights with the +-3/2, 1, -1 and half-weight corrections, applied to
+//      S0(jo-1) and S0(-jo+1) of the (-no:no) grid; the +-pi S0(io-1) terms
+//      are added in the epilogue.
+// Each stencil entry is evaluated independently with the reference's chain,
+// so the matrix is exactly the coefficients the reference multiplies with.
+#include "rpa_internal.h"
+
+namespace rpa {
+
+// c- weight of the Optabs stencil for output io at input jo (io >= 2)
+__device__ inline double k1_cminus(int io, int jo, int no, double oi, double oj, double dw) {
+#pragma clang fp contract(off)
+  double fact = 0.0;
+  if (jo != io) fact = dw / (oi - oj);
+  if (io == 2) {
+    if (jo == 1) fact = 1.0;
+    else if (jo == 2) fact = 0.0;
+    else if (jo == 3) fact = -3.0 / 2.0;
+    else if (jo == no) fact = 0.5 * dw / (oi - oj);
+  } else if (io == no - 1) {
+    if (jo == 1) fact = 0.5 * dw / (oi - oj);
+    else if (jo == no - 2) fact = 3.0 / 2.0;
+    else if (jo == no - 1) fact = 0.0;
+    else if (jo == no) fact = -1.0;
+  } else {
+    if (jo == 1) fact = 0.5 * dw / (oi - oj);
+    else if (jo == io - 1) fact = 3.0 / 2.0;
+    else if (jo == io) fact = 0.0;
+    else if (jo == io + 1) fact = -3.0 / 2.0;
+    else if (jo == no) fact = 0.5 * dw / (oi - oj);
+  }
+  return fact;
+}
+
+// c- weight of the Sloss stencil (io >= 2): plain 1/(oi-oj), zero at jo == io
+__device__ inline double k2_cminus(int io, int jo, double oi, double oj, double dw) {
+#pragma clang fp contract(off)
+  return (jo != io) ? dw / (oi - oj) : 0.0;
+}
+
+__device__ inline double c_plus(int jo, int no, double oi, double oj, double dw) {
+#pragma clang fp contract(off)
+  double fact = dw / (oi + oj);
+  if (jo == 1 || jo == no) fact = 0.5 * dw / (oi + oj);
+  return fact;
+}
+
+// static-limit weight f_j (io = 1), both stencils
+__device__ inline double f_static(int jo, int no, double oj, double dw) {
+#pragma clang fp contract(off)
+  double fact = dw / oj;
+  if (jo == 2) fact = 3.0 / 2.0;
+  else if (jo == no) fact = 0.5 * dw / oj;
+  return fact;
+}
+
+__global__ void k_kstencil(int program, int no, int nw, double dw, double *K) {
+#pragma clang fp contract(off)
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)(no - 1) * nw) return;
+  const int iop = (int)(idx / nw), w = (int)(idx - (long long)iop * nw);
+  const int io = iop + 1;
+  const double oi = (double)(io - 1) * dw;
+  double coef = 0.0;
+  if (program == RPA_SLOSS) {
+    const int jo = w + 1;  // S0(jo), jo = 1..no
+    const double oj = (double)(jo - 1) * dw;
+    if (io == 1) {
+      coef = (jo >= 2) ? -2.0 * f_static(jo, no, oj, dw) : 0.0;
+    } else {
+      coef = k2_cminus(io, jo, oi, oj, dw) - c_plus(jo, no, oi, oj, dw);
+    }
+  } else {
+    const int f = w - no;  // frequency index of S0(f), f = -no..no
+    if (io == 1) {
+      if (f > 0 && f + 1 <= no) {
+        int jo = f + 1;
+        coef = -f_static(jo, no, (double)(jo - 1) * dw, dw);
+      } else if (f < 0 && -f + 1 <= no) {
+        int jo = -f + 1;
+        coef = f_static(jo, no, (double)(jo - 1) * dw, dw);
+      }
+    } else {
+      if (f >= 0 && f + 1 <= no) {
+        int jo = f + 1;
+        coef += k1_cminus(io, jo, no, oi, (double)(jo - 1) * dw, dw);
+      }
+      if (f <= 0 && -f + 1 <= no) {
+        int jo = -f + 1;
+        coef += c_plus(jo, no, oi, (double)(jo - 1) * dw, dw);
+      }
+    }
+  }
+  K[idx] = coef;
+}
+
+// C[M][N] = A[M][K] B[K][N], row-major, FP64 matrix cores, 64x64 tiles.
+__global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const double *__restrict__ A,
+                                                   const double *__restrict__ B, double *__restrict__ C) {
+  __shared__ double sA[64][17];
+  __shared__ double sB[16][65];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, q4 = lane >> 4;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  rpa_d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = rpa_d4{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    __syncthreads();
+    for (int x = tid; x < 64 * 16; x += 256) {
+      int i = x >> 4, k = x & 15;
+      int gm = m0 + i, gk = k0 + k;
+      sA[i][k] = (gm < M && gk < K) ? A[(size_t)gm * K + gk] : 0.0;
+      int kb = x >> 6, j = x & 63;
+      int gk2 = k0 + kb, gn = n0 + j;
+      sB[kb][j] = (gk2 < K && gn < N) ? B[(size_t)gk2 * N + gn] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = kk * 4 + q4;
+      double av[2], bv[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) av[i] = sA[wm + 16 * i + r][k];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bv[j] = sB[k][wn + 16 * j + r];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = mfma_f64_16x16x4(av[i], bv[j], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int gm = m0 + wm + 16 * i + 4 * e + q4, gn = n0 + wn + 16 * j + r;
+        if (gm < M && gn < N) C[(size_t)gm * N + gn] = acc[i][j][e];
+      }
+}
+
+// chi0[io'][iG][jG] (complex) from X = K S0 and S0:
+//   Sloss:  (X, -pi S0(io))                          Sloss.f90:1053-1054
+//   Optabs: (X_re + pi Im S0(io-1), X_im - pi Re S0(io-1))  Optabs.f90:1209,1316
+__global__ void k_chi0_epilogue(int program, int no, int nlf, int ncol_pad, const double *X,
+                                const double *s0, double *chi0) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nn = (long long)nlf * nlf;
+  if (idx >= (long long)(no - 1) * nn) return;
+  const int iop = (int)(idx / nn);
+  const int pr = (int)(idx - (long long)iop * nn);
+  double re, im;
+  if (program == RPA_SLOSS) {
+    re = X[(size_t)iop * ncol_pad + pr];
+    im = -M_PI * s0[(size_t)iop * ncol_pad + pr];
+  } else {
+    const size_t w = (size_t)iop + no;  // frequency index io-1 = iop
+    re = X[(size_t)iop * ncol_pad + 2 * pr] + M_PI * s0[w * ncol_pad + 2 * pr + 1];
+    im = X[(size_t)iop * ncol_pad + 2 * pr + 1] - M_PI * s0[w * ncol_pad + 2 * pr];
+  }
+  chi0[2 * idx] = re;
+  chi0[2 * idx + 1] = im;
+}
+
+int launch_kstencil(rpa_ctx *c) {
+  const int no = c->d.no, nw = c->nw;
+  const long long n = (long long)(no - 1) * nw;
+  hipLaunchKernelGGL(k_kstencil, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->d.program, no, nw, c->p.domega, c->d_kmat);
+  RPA_HIP(c, hipGetLastError());
+  c->kmat_no = no;
+  c->kmat_dw = c->p.domega;
+  return 0;
+}
+
+int launch_kk(rpa_ctx *c) {
+  const int no = c->d.no, nw = c->nw;
+  dim3 grid((c->ncol_pad + 63) / 64, (no - 1 + 63) / 64);
+  hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, c->stream, no - 1, c->ncol_pad, nw,
+                     (const double *)c->d_kmat, (const double *)c->d_s0, c->d_x);
+  const long long n = (long long)(no - 1) * c->nlf * c->nlf;
+  hipLaunchKernelGGL(k_chi0_epilogue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     c->d.program, no, c->nlf, c->ncol_pad, (const double *)c->d_x,
+                     (const double *)c->d_s0, c->d_chi0);
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace rpa

@@ -1,0 +1,622 @@
+// C ABI of librpa_hip.so: context, inputs, orchestration and copy-outs.
+// Kernels live in k_vertex.hip, k_accum.hip, k_kk.hip and k_dyson.hip.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "rpa_internal.h"
+
+namespace rpa {
+
+int fail(rpa_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(rpa_ctx *c, hipError_t e, const char *what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (c) c->err = m;
+  return RPA_E_HIP;
+}
+
+static void invalidate_graph(rpa_ctx *c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  c->graph_exec = nullptr;
+  c->graph_valid = false;
+}
+
+// Which band pairs can carry weight: Sloss n <= Nocc < m (Sloss.f90:355,378);
+// Optabs every (n,m) with |f_n - f_m| >= df_cut or n == m (Optabs.f90:360).
+// When T = 0 and the first Nval bands lie below E_F at every k, those pairs
+// are exactly (occupied, empty) and (empty, occupied); otherwise all pairs
+// are enumerated and the device filter decides.
+static void build_pairs(rpa_ctx *c) {
+  PairTable &pt = c->pairs;
+  const int nb = c->d.nband, nv = c->p.nval;
+  pt.nblocks = 0;
+  if (c->d.program == RPA_SLOSS) {
+    pt.b[pt.nblocks++] = PairBlock{0, nv, nv, nb};
+  } else {
+    bool clean = (c->p.temp == 0.0) && c->p.df_cut > 0.0 && c->p.df_cut <= 1.0;
+    for (int k = 0; clean && k < c->d.nki; ++k)
+      for (int n = 0; n < nb; ++n) {
+        double e = c->E[k + (size_t)c->d.nki * n];
+        if ((n < nv && !(e < c->p.efermi)) || (n >= nv && !(e > c->p.efermi))) {
+          clean = false;
+          break;
+        }
+      }
+    if (clean) {
+      pt.b[pt.nblocks++] = PairBlock{0, nv, nv, nb};
+      pt.b[pt.nblocks++] = PairBlock{nv, nb, 0, nv};
+    } else {
+      pt.b[pt.nblocks++] = PairBlock{0, nb, 0, nb};
+    }
+  }
+  int off = 0;
+  for (int i = 0; i < pt.nblocks; ++i) {
+    pt.off[i] = off;
+    off += (pt.b[i].n_hi - pt.b[i].n_lo) * (pt.b[i].m_hi - pt.b[i].m_lo);
+  }
+  pt.tpk = off;
+}
+
+static int prepare(rpa_ctx *c) {
+  if (c->prepared) return 0;
+  if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors not called");
+  for (int k = 0; k < c->d.nki; ++k)
+    if (!c->uploaded[k]) return fail(c, RPA_E_STATE, "basis of k-point " + std::to_string(k + 1) + " not uploaded");
+  if (c->E.empty()) return fail(c, RPA_E_STATE, "rpa_set_bands not called");
+  if (c->nlf == 0) return fail(c, RPA_E_STATE, "rpa_set_local_fields not called");
+  if (!c->have_params) return fail(c, RPA_E_STATE, "rpa_set_params not called");
+  if (!c->have_q) return fail(c, RPA_E_STATE, "rpa_set_q not called");
+  const rpa_params &p = c->p;
+  if (p.vertex_map == RPA_MAP_DIRECT) {
+    for (int k = 0; k < c->d.nki; ++k)
+      if (p.ngd > c->ngk[k])  // Optabs.f90:349-352, 377-380
+        return fail(c, RPA_E_ARG, "NGd is bigger than the plane waves of k-point " + std::to_string(k + 1));
+    if (p.ngd > c->ngm) return fail(c, RPA_E_ARG, "NGd is bigger than NG");
+  }
+  invalidate_graph(c);
+  if (p.vertex_map == RPA_MAP_IGK && !c->kbox_ready) {
+    for (int a = 0; a < 3; ++a) {
+      c->kbox.lo[a] = c->mill_min[a];
+      c->kbox.n[a] = c->mill_max[a] - c->mill_min[a] + 1;
+    }
+    if ((long long)c->kbox.size() * c->d.nki > (1LL << 31))
+      return fail(c, RPA_E_LIMIT, "Miller box too large");
+    if (int e = launch_build_kbox(c)) return e;
+    c->kbox_ready = true;
+  }
+  build_pairs(c);
+  const int no = c->d.no;
+  c->nw = (c->d.program == RPA_OPTABS) ? 2 * no + 1 : no;
+  c->ntiles = (c->nw + kTileW - 1) / kTileW;
+  c->wcut = (c->d.program == RPA_OPTABS) ? p.lor_cut / p.eta : 1.0e-3 / p.eta;
+  double hw = INFINITY;
+  if (c->wcut > 0.0) hw = std::sqrt(std::max(p.eta / c->wcut - p.eta * p.eta, 0.0));
+  if (std::isinf(hw)) {
+    c->span = c->ntiles;
+  } else {
+    double bins = 2.0 * hw / p.domega + 6.0;
+    c->span = std::min(c->ntiles, (int)std::ceil(bins / kTileW));
+  }
+  c->ntrans = (long long)c->ntot * c->pairs.tpk;
+  const size_t nt = (size_t)std::max<long long>(c->ntrans, 1);
+  const size_t nblk = (nt + 255) / 256;
+  c->max_items = (size_t)(c->span + 1) * ((nt + kChunkT - 1) / kChunkT) + c->ntiles + 1;
+  const bool two = (c->d.program == RPA_OPTABS && p.pol == RPA_POL_YZ);
+  c->ncol = (c->d.program == RPA_OPTABS ? 2 : 1) * c->nlf * c->nlf;
+  c->ncol_pad = (c->ncol + kColsWG - 1) / kColsWG * kColsWG;
+  const size_t npb = (size_t)c->ncol_pad / kColsWG;
+  const size_t nf = (size_t)(no - 1);
+  const size_t nn = (size_t)c->nlf * c->nlf;
+  int e = 0;
+  size_t cap_perm = c->cap_key, cap_nat = c->cap_key;
+  if (!e) e = ensure(c, c->d_key, c->cap_key, nt);
+  if (!e) e = ensure(c, c->d_perm, cap_perm, nt);
+  if (!e) e = ensure(c, c->d_nat, cap_nat, nt);
+  if (!e) e = ensure(c, c->d_te, c->cap_te, 3 * nt);
+  if (!e) e = ensure(c, c->d_bcnt, c->cap_bcnt, (size_t)(c->ntiles + 1) * nblk);
+  if (!e) e = ensure(c, c->d_toff, c->cap_toff, (size_t)c->ntiles + 2);
+  if (!e) e = ensure(c, c->d_items, c->cap_items, 3 * c->max_items);
+  if (!e) e = ensure(c, c->d_first_item, c->cap_first, (size_t)c->ntiles + 1);
+  if (!e) e = ensure(c, c->d_nitems, c->cap_nitems, 2);
+  if (!e) e = ensure(c, (double2 *&)c->d_rho1, c->cap_rho, nt * c->nlf);
+  if (!e && two) e = ensure(c, (double2 *&)c->d_rho2, c->cap_rho2, nt * c->nlf);
+  if (!e && !two && c->d_rho2) {
+    (void)hipFree(c->d_rho2);
+    c->d_rho2 = nullptr;
+    c->cap_rho2 = 0;
+  }
+  if (!e) e = ensure(c, c->d_s0, c->cap_s0, (size_t)c->nw * c->ncol_pad);
+  if (!e) e = ensure(c, c->d_part, c->cap_part, c->max_items * npb * kTileW * kColsWG);
+  if (!e) e = ensure(c, c->d_kmat, c->cap_kmat, nf * c->nw);
+  if (!e) e = ensure(c, c->d_x, c->cap_x, nf * c->ncol_pad);
+  if (!e) e = ensure(c, c->d_chi0, c->cap_chi0, nf * nn * 2);
+  if (!e) e = ensure(c, c->d_epsinv, c->cap_epsinv, nf * nn * 2);
+  if (!e) e = ensure(c, c->d_chi, c->cap_chi, nf * nn * 2);
+  if (!e) e = ensure(c, c->d_W, c->cap_W, nf * nn * 2);
+  if (!e) e = ensure(c, c->d_scratch, c->cap_scratch, nf * nn * 2);
+  if (!e) e = ensure(c, c->d_V, c->cap_V, nn);
+  if (!e) e = ensure(c, c->d_loss, c->cap_loss, nf);
+  if (!e) e = ensure(c, c->d_pi11, c->cap_pi11, 2 * nf);
+  if (!e) e = ensure(c, c->d_sig, c->cap_sig, 2 * nf);
+  if (!e) e = ensure(c, c->d_qeff, c->cap_qeff, 2 * nn);
+  if (!e) e = ensure(c, c->d_status, c->cap_status, 1);
+  if (e) return e;
+  RPA_HIP(c, hipMemsetAsync(c->d_qeff, 0, sizeof(double) * 2 * nn, c->stream));
+  RPA_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
+  if (int e2 = launch_kstencil(c)) return e2;
+  RPA_HIP(c, hipStreamSynchronize(c->stream));
+  c->prepared = true;
+  return 0;
+}
+
+// The per-q chain after prepare(): bookkeeping, vertices, S0, KK, Dyson/W or Pi.
+static int chain(rpa_ctx *c) {
+  if (int e = launch_transitions(c)) return e;
+  if (int e = launch_vertices(c, c->d_rho1, c->d_rho2, c->d_perm)) return e;
+  if (int e = launch_accumulate(c)) return e;
+  if (int e = launch_kk(c)) return e;
+  if (c->d.program == RPA_SLOSS) {
+    if (int e = launch_coulomb(c)) return e;
+    if (int e = launch_dyson(c)) return e;
+  } else {
+    if (int e = launch_epilogue(c)) return e;
+  }
+  return 0;
+}
+
+static int sync(rpa_ctx *c) {
+  RPA_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+template <class T>
+static int d2h(rpa_ctx *c, T *dst, const T *src, size_t n) {
+  RPA_HIP(c, hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+  RPA_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// [f][i][j] complex (device) -> (f, iG, jG) column-major complex (reference)
+static int copy_freq_mats(rpa_ctx *c, const double *dev, double *out) {
+  const size_t nf = (size_t)(c->d.no - 1), N = (size_t)c->nlf;
+  std::vector<double> h(nf * N * N * 2);
+  if (int e = d2h(c, h.data(), dev, h.size())) return e;
+  for (size_t f = 0; f < nf; ++f)
+    for (size_t i = 0; i < N; ++i)
+      for (size_t j = 0; j < N; ++j) {
+        size_t src = ((f * N + i) * N + j) * 2, dst = (f + nf * (i + N * j)) * 2;
+        out[dst] = h[src];
+        out[dst + 1] = h[src + 1];
+      }
+  return 0;
+}
+
+}  // namespace rpa
+
+using namespace rpa;
+
+extern "C" {
+
+int rpa_ctx_create(const rpa_dims *dims, rpa_ctx **out) {
+  if (!dims || !out) return RPA_E_ARG;
+  *out = nullptr;
+  const rpa_dims &d = *dims;
+  if (d.nki <= 0 || d.nband <= 0 || d.npwx <= 0 || d.nlfd <= 0 || d.no < 4) return RPA_E_ARG;
+  if (d.program != RPA_SLOSS && d.program != RPA_OPTABS) return RPA_E_ARG;
+  if (d.precision != RPA_FP64 && d.precision != RPA_MIXED) return RPA_E_ARG;
+  if (d.npwx > 32767) return RPA_E_LIMIT;
+  if (d.nlfd > kMaxNlf) return RPA_E_LIMIT;
+  rpa_ctx *c = new rpa_ctx();
+  c->d = d;
+  if (d.precision == RPA_MIXED) {
+    delete c;
+    return RPA_E_ARG;  // mixed precision path is selected per run, see rpa_params
+  }
+  hipError_t e = hipSetDevice(d.device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  const size_t nel = (size_t)d.nki * d.npwx;
+  if (e == hipSuccess) e = hipMalloc(&c->d_evc, sizeof(double2) * nel * d.nband);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_mill_k, sizeof(int) * nel);
+  if (e == hipSuccess) e = hipMemset(c->d_mill_k, 0, sizeof(int) * nel);
+  if (e == hipSuccess) e = hipMemset(c->d_evc, 0, sizeof(double2) * nel * d.nband);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_ngk, sizeof(int) * d.nki);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_E, sizeof(double) * d.nki * d.nband);
+  if (e != hipSuccess) {
+    rpa_ctx_destroy(c);
+    return RPA_E_HIP;
+  }
+  c->ngk.assign(d.nki, 0);
+  c->uploaded.assign(d.nki, 0);
+  *out = c;
+  return RPA_OK;
+}
+
+int rpa_ctx_destroy(rpa_ctx *c) {
+  if (!c) return RPA_OK;
+  (void)hipSetDevice(c->d.device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  invalidate_graph(c);
+  void *bufs[] = {c->d_mill_g, c->d_gbox, c->d_evc, c->d_mill_k, c->d_kbox, c->d_ngk, c->d_E,
+                  c->d_lf_key, c->d_glf, c->d_parg, c->d_k1, c->d_k2, c->d_m0, c->d_ktot,
+                  c->d_key, c->d_perm, c->d_bcnt, c->d_toff, c->d_nat, c->d_te, c->d_items,
+                  c->d_nitems, c->d_first_item, c->d_intra, c->d_nintra, c->d_rho1, c->d_rho2,
+                  c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
+                  c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig, c->d_scratch,
+                  c->d_status};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return RPA_OK;
+}
+
+const char *rpa_last_error(const rpa_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+double rpa_last_run_ms(const rpa_ctx *c) { return c ? c->last_ms : 0.0; }
+
+int rpa_set_gvectors(rpa_ctx *c, int ngm, const int *mill, const double *kc) {
+  if (!c || ngm <= 0 || !mill || !kc) return fail(c, RPA_E_ARG, "rpa_set_gvectors: bad arguments");
+  (void)hipSetDevice(c->d.device);
+  if (mill[0] != 0 || mill[1] != 0 || mill[2] != 0)  // Optabs_qe6.f90:1153-1157
+    return fail(c, RPA_E_ARG, "G vectors input is wrong: G(1) is not (0,0,0)");
+  c->ngm = ngm;
+  c->mill_g.assign(mill, mill + 3 * (size_t)ngm);
+  std::memcpy(c->kc, kc, sizeof(double) * 9);
+  int lo[3] = {1 << 20, 1 << 20, 1 << 20}, hi[3] = {-(1 << 20), -(1 << 20), -(1 << 20)};
+  std::vector<int> keys(ngm);
+  for (int i = 0; i < ngm; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      int v = mill[3 * i + a];
+      if (v < -kMillBias || v >= kMillBias) return fail(c, RPA_E_LIMIT, "Miller index out of range");
+      lo[a] = std::min(lo[a], v);
+      hi[a] = std::max(hi[a], v);
+    }
+    keys[i] = mill_pack(mill[3 * i], mill[3 * i + 1], mill[3 * i + 2]);
+  }
+  for (int a = 0; a < 3; ++a) {
+    c->gbox.lo[a] = lo[a];
+    c->gbox.n[a] = hi[a] - lo[a] + 1;
+  }
+  std::vector<int> box((size_t)c->gbox.size(), -1);
+  for (int i = 0; i < ngm; ++i) {
+    int bi = c->gbox.index(mill[3 * i], mill[3 * i + 1], mill[3 * i + 2]);
+    if (box[bi] < 0) box[bi] = i;
+  }
+  if (c->d_mill_g) (void)hipFree(c->d_mill_g);
+  if (c->d_gbox) (void)hipFree(c->d_gbox);
+  c->d_mill_g = c->d_gbox = nullptr;
+  RPA_HIP(c, hipMalloc((void **)&c->d_mill_g, sizeof(int) * ngm));
+  RPA_HIP(c, hipMalloc((void **)&c->d_gbox, sizeof(int) * box.size()));
+  RPA_HIP(c, hipMemcpy(c->d_mill_g, keys.data(), sizeof(int) * ngm, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_gbox, box.data(), sizeof(int) * box.size(), hipMemcpyHostToDevice));
+  c->prepared = false;
+  invalidate_graph(c);
+  return RPA_OK;
+}
+
+int rpa_upload_basis(rpa_ctx *c, int ik, int ngk, const int *mill, const double *evc) {
+  if (!c || !mill || !evc) return fail(c, RPA_E_ARG, "rpa_upload_basis: null argument");
+  if (ik < 1 || ik > c->d.nki) return fail(c, RPA_E_ARG, "rpa_upload_basis: ik out of range");
+  if (ngk < 1 || ngk > c->d.npwx) return fail(c, RPA_E_ARG, "rpa_upload_basis: ngk exceeds npwx");
+  (void)hipSetDevice(c->d.device);
+  const int k = ik - 1, nb = c->d.nband;
+  std::vector<int> keys(c->d.npwx, 0);
+  for (int i = 0; i < ngk; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      int v = mill[3 * i + a];
+      if (v < -kMillBias || v >= kMillBias) return fail(c, RPA_E_LIMIT, "Miller index out of range");
+      c->mill_min[a] = std::min(c->mill_min[a], v);
+      c->mill_max[a] = std::max(c->mill_max[a], v);
+    }
+    keys[i] = mill_pack(mill[3 * i], mill[3 * i + 1], mill[3 * i + 2]);
+  }
+  // evc(ngk, nband) column-major -> rows of all bands per plane wave
+  std::vector<double> rows((size_t)c->d.npwx * nb * 2, 0.0);
+  for (int n = 0; n < nb; ++n)
+    for (int i = 0; i < ngk; ++i) {
+      rows[((size_t)i * nb + n) * 2] = evc[((size_t)i + (size_t)ngk * n) * 2];
+      rows[((size_t)i * nb + n) * 2 + 1] = evc[((size_t)i + (size_t)ngk * n) * 2 + 1];
+    }
+  RPA_HIP(c, hipMemcpy((double2 *)c->d_evc + (size_t)k * c->d.npwx * nb, rows.data(),
+                       sizeof(double) * rows.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_mill_k + (size_t)k * c->d.npwx, keys.data(), sizeof(int) * c->d.npwx,
+                       hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_ngk + k, &ngk, sizeof(int), hipMemcpyHostToDevice));
+  c->ngk[k] = ngk;
+  c->uploaded[k] = 1;
+  c->kbox_ready = false;
+  c->prepared = false;
+  invalidate_graph(c);
+  return RPA_OK;
+}
+
+int rpa_set_bands(rpa_ctx *c, const double *E) {
+  if (!c || !E) return fail(c, RPA_E_ARG, "rpa_set_bands: null argument");
+  (void)hipSetDevice(c->d.device);
+  const size_t n = (size_t)c->d.nki * c->d.nband;
+  c->E.assign(E, E + n);
+  RPA_HIP(c, hipMemcpy(c->d_E, E, sizeof(double) * n, hipMemcpyHostToDevice));
+  c->prepared = false;
+  invalidate_graph(c);
+  return RPA_OK;
+}
+
+int rpa_set_local_fields(rpa_ctx *c, int nlf, const int *lf_idx, const int *parg) {
+  if (!c || !lf_idx || !parg || nlf <= 0) return fail(c, RPA_E_ARG, "rpa_set_local_fields: bad arguments");
+  if (nlf > c->d.nlfd) return fail(c, RPA_E_LIMIT, "Nlf is bigger than Nlfd");  // Optabs.f90:1054-1057
+  if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors must precede rpa_set_local_fields");
+  (void)hipSetDevice(c->d.device);
+  std::vector<int> keys(nlf);
+  std::vector<double> glf(3 * (size_t)nlf);
+  for (int i = 0; i < nlf; ++i) {
+    int g = lf_idx[i] - 1;
+    if (g < 0 || g >= c->ngm) return fail(c, RPA_E_ARG, "local-field index outside the G list");
+    if (parg[i] != 1 && parg[i] != -1) return fail(c, RPA_E_ARG, "parity must be +1 or -1");
+    const int *m = &c->mill_g[3 * (size_t)g];
+    keys[i] = mill_pack(m[0], m[1], m[2]);
+    for (int a = 0; a < 3; ++a) {
+      double s = 0.0;
+      for (int b = 0; b < 3; ++b) s = s + c->kc[a + 3 * b] * (double)m[b];
+      glf[3 * i + a] = s;
+    }
+  }
+  c->nlf = nlf;
+  c->lf_idx.assign(lf_idx, lf_idx + nlf);
+  c->parg.assign(parg, parg + nlf);
+  for (void *p : {(void *)c->d_lf_key, (void *)c->d_glf, (void *)c->d_parg})
+    if (p) (void)hipFree(p);
+  c->d_lf_key = nullptr;
+  c->d_glf = nullptr;
+  c->d_parg = nullptr;
+  RPA_HIP(c, hipMalloc((void **)&c->d_lf_key, sizeof(int) * nlf));
+  RPA_HIP(c, hipMalloc((void **)&c->d_glf, sizeof(double) * 3 * nlf));
+  RPA_HIP(c, hipMalloc((void **)&c->d_parg, sizeof(int) * nlf));
+  RPA_HIP(c, hipMemcpy(c->d_lf_key, keys.data(), sizeof(int) * nlf, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_glf, glf.data(), sizeof(double) * 3 * nlf, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_parg, parg, sizeof(int) * nlf, hipMemcpyHostToDevice));
+  c->prepared = false;
+  invalidate_graph(c);
+  return RPA_OK;
+}
+
+int rpa_set_params(rpa_ctx *c, const rpa_params *p) {
+  if (!c || !p) return fail(c, RPA_E_ARG, "rpa_set_params: null argument");
+  if (p->nval < 0 || p->nval > c->d.nband) return fail(c, RPA_E_ARG, "nval outside 0..Nband");
+  if (p->vertex_map != RPA_MAP_DIRECT && p->vertex_map != RPA_MAP_IGK)
+    return fail(c, RPA_E_ARG, "unknown vertex map");
+  if (p->vertex_map == RPA_MAP_DIRECT && p->ngd <= 0) return fail(c, RPA_E_ARG, "NGd must be positive");
+  if (!(p->eta > 0.0) || !(p->domega > 0.0) || !(p->vcell > 0.0))
+    return fail(c, RPA_E_ARG, "eta, domega and Vcell must be positive");
+  if (c->d.program == RPA_OPTABS && (p->pol < RPA_POL_X || p->pol > RPA_POL_YZ))
+    return fail(c, RPA_E_ARG, "unknown polarisation");
+  if (c->d.program == RPA_SLOSS && (p->nval == 0 || p->nval == c->d.nband))
+    return fail(c, RPA_E_ARG, "Sloss needs occupied and empty bands");
+  c->p = *p;
+  c->have_params = true;
+  c->prepared = false;
+  invalidate_graph(c);
+  return RPA_OK;
+}
+
+int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0, const double *ktot,
+              const double *q) {
+  if (!c || ntot <= 0 || !K1 || !K2 || !iG0 || !ktot || !q) return fail(c, RPA_E_ARG, "rpa_set_q: bad arguments");
+  if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors must precede rpa_set_q");
+  (void)hipSetDevice(c->d.device);
+  std::vector<int> m0(ntot);
+  for (int k = 0; k < ntot; ++k) {
+    if (K1[k] < 1 || K1[k] > c->d.nki || K2[k] < 1 || K2[k] > c->d.nki)
+      return fail(c, RPA_E_ARG, "K1/K2 outside 1..NkI");
+    if (iG0[k] < 1 || iG0[k] > c->ngm) return fail(c, RPA_E_ARG, "iG0 outside 1..NG");
+    const int *m = &c->mill_g[3 * (size_t)(iG0[k] - 1)];
+    m0[k] = mill_pack(m[0], m[1], m[2]);
+  }
+  c->ntot = ntot;
+  c->K1.assign(K1, K1 + ntot);
+  c->K2.assign(K2, K2 + ntot);
+  c->iG0.assign(iG0, iG0 + ntot);
+  c->ktot.assign(ktot, ktot + 3 * (size_t)ntot);
+  for (int a = 0; a < 3; ++a) c->q[a] = q[a];
+  for (void *p : {(void *)c->d_k1, (void *)c->d_k2, (void *)c->d_m0, (void *)c->d_ktot})
+    if (p) (void)hipFree(p);
+  c->d_k1 = c->d_k2 = c->d_m0 = nullptr;
+  c->d_ktot = nullptr;
+  RPA_HIP(c, hipMalloc((void **)&c->d_k1, sizeof(int) * ntot));
+  RPA_HIP(c, hipMalloc((void **)&c->d_k2, sizeof(int) * ntot));
+  RPA_HIP(c, hipMalloc((void **)&c->d_m0, sizeof(int) * ntot));
+  RPA_HIP(c, hipMalloc((void **)&c->d_ktot, sizeof(double) * 3 * ntot));
+  RPA_HIP(c, hipMemcpy(c->d_k1, K1, sizeof(int) * ntot, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_k2, K2, sizeof(int) * ntot, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_m0, m0.data(), sizeof(int) * ntot, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_ktot, ktot, sizeof(double) * 3 * ntot, hipMemcpyHostToDevice));
+  c->have_q = true;
+  c->prepared = false;
+  invalidate_graph(c);
+  return RPA_OK;
+}
+
+int rpa_vertices(rpa_ctx *c) {
+  if (!c) return RPA_E_ARG;
+  (void)hipSetDevice(c->d.device);
+  if (int e = prepare(c)) return e;
+  if (int e = launch_transitions(c)) return e;
+  if (int e = launch_vertices(c, c->d_rho1, c->d_rho2, c->d_perm)) return e;
+  return sync(c);
+}
+
+int rpa_accumulate_s0(rpa_ctx *c, double *S0_out, double *qeff_out) {
+  if (!c) return RPA_E_ARG;
+  if (!c->d_rho1 || c->ntrans == 0) return fail(c, RPA_E_STATE, "rpa_vertices must precede rpa_accumulate_s0");
+  (void)hipSetDevice(c->d.device);
+  if (int e = launch_accumulate(c)) return e;
+  if (int e = sync(c)) return e;
+  const int nw = c->nw, N = c->nlf;
+  if (S0_out) {
+    std::vector<double> h((size_t)nw * c->ncol_pad);
+    if (int e = d2h(c, h.data(), (const double *)c->d_s0, h.size())) return e;
+    for (int w = 0; w < nw; ++w)
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+          size_t pr = (size_t)i * N + j;
+          size_t dst = (size_t)w + (size_t)nw * (i + (size_t)N * j);
+          if (c->d.program == RPA_SLOSS) {
+            S0_out[dst] = h[(size_t)w * c->ncol_pad + pr];
+          } else {
+            S0_out[2 * dst] = h[(size_t)w * c->ncol_pad + 2 * pr];
+            S0_out[2 * dst + 1] = h[(size_t)w * c->ncol_pad + 2 * pr + 1];
+          }
+        }
+  }
+  if (qeff_out && c->d.program == RPA_OPTABS) {
+    std::vector<double> h(2 * (size_t)N * N);
+    if (int e = d2h(c, h.data(), (const double *)c->d_qeff, h.size())) return e;
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        qeff_out[2 * (i + (size_t)N * j)] = h[2 * ((size_t)i * N + j)];
+        qeff_out[2 * (i + (size_t)N * j) + 1] = h[2 * ((size_t)i * N + j) + 1];
+      }
+  }
+  return RPA_OK;
+}
+
+int rpa_kk(rpa_ctx *c, double *chi0_out) {
+  if (!c) return RPA_E_ARG;
+  if (!c->d_s0) return fail(c, RPA_E_STATE, "rpa_accumulate_s0 must precede rpa_kk");
+  (void)hipSetDevice(c->d.device);
+  if (int e = launch_kk(c)) return e;
+  if (int e = sync(c)) return e;
+  if (chi0_out) return copy_freq_mats(c, c->d_chi0, chi0_out);
+  return RPA_OK;
+}
+
+int rpa_coulomb(rpa_ctx *c, double *V_out) {
+  if (!c) return RPA_E_ARG;
+  if (!c->d_V) return fail(c, RPA_E_STATE, "rpa_vertices must precede rpa_coulomb");
+  (void)hipSetDevice(c->d.device);
+  if (int e = launch_coulomb(c)) return e;
+  if (V_out) {
+    const int N = c->nlf;
+    std::vector<double> h((size_t)N * N);
+    if (int e = d2h(c, h.data(), (const double *)c->d_V, h.size())) return e;
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) V_out[i + (size_t)N * j] = h[(size_t)i * N + j];
+    return RPA_OK;
+  }
+  return sync(c);
+}
+
+int rpa_dyson_w(rpa_ctx *c, double *epsinv_out, double *chi_out, double *W_out) {
+  if (!c) return RPA_E_ARG;
+  if (c->d.program != RPA_SLOSS) return fail(c, RPA_E_STATE, "the Dyson step belongs to the Sloss path");
+  if (!c->d_chi0) return fail(c, RPA_E_STATE, "rpa_kk must precede rpa_dyson_w");
+  (void)hipSetDevice(c->d.device);
+  if (int e = launch_coulomb(c)) return e;
+  if (int e = launch_dyson(c)) return e;
+  if (int e = sync(c)) return e;
+  int status = 0;
+  if (int e = d2h(c, &status, (const int *)c->d_status, 1)) return e;
+  if (status) return fail(c, RPA_E_SINGULAR, "singular matrix in the Dyson step");
+  if (epsinv_out)
+    if (int e = copy_freq_mats(c, c->d_epsinv, epsinv_out)) return e;
+  if (chi_out)
+    if (int e = copy_freq_mats(c, c->d_chi, chi_out)) return e;
+  if (W_out)
+    if (int e = copy_freq_mats(c, c->d_W, W_out)) return e;
+  return RPA_OK;
+}
+
+int rpa_epilogue(rpa_ctx *c, double *loss, double *pi11, double *sig) {
+  if (!c) return RPA_E_ARG;
+  if (!c->d_chi0) return fail(c, RPA_E_STATE, "rpa_kk must precede rpa_epilogue");
+  (void)hipSetDevice(c->d.device);
+  const size_t nf = (size_t)(c->d.no - 1);
+  if (c->d.program == RPA_OPTABS) {
+    if (int e = launch_epilogue(c)) return e;
+    if (pi11)
+      if (int e = d2h(c, pi11, (const double *)c->d_pi11, 2 * nf)) return e;
+    if (sig) {
+      std::vector<double> h(2 * nf);
+      if (int e = d2h(c, h.data(), (const double *)c->d_sig, 2 * nf)) return e;
+      for (size_t f = 0; f < nf; ++f) {
+        sig[f] = h[2 * f];
+        sig[nf + f] = h[2 * f + 1];
+      }
+    }
+  } else if (loss) {
+    if (int e = d2h(c, loss, (const double *)c->d_loss, nf)) return e;
+  }
+  return sync(c);
+}
+
+int rpa_run(rpa_ctx *c) {
+  if (!c) return RPA_E_ARG;
+  (void)hipSetDevice(c->d.device);
+  if (int e = prepare(c)) return e;
+  RPA_HIP(c, hipEventRecord(c->ev0, c->stream));
+  if (int e = chain(c)) return e;
+  RPA_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (int e = sync(c)) return e;
+  float ms = 0.f;
+  RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  return RPA_OK;
+}
+
+int rpa_run_graph(rpa_ctx *c) {
+  if (!c) return RPA_E_ARG;
+  (void)hipSetDevice(c->d.device);
+  if (int e = prepare(c)) return e;
+  if (!c->graph_exec) {
+    hipGraph_t g = nullptr;
+    RPA_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int e = chain(c);
+    hipError_t he = hipStreamEndCapture(c->stream, &g);
+    if (e) {
+      if (g) (void)hipGraphDestroy(g);
+      return e;
+    }
+    if (he != hipSuccess) return hip_fail(c, he, "hipStreamEndCapture");
+    he = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (he != hipSuccess) return hip_fail(c, he, "hipGraphInstantiate");
+    c->graph_valid = true;
+  }
+  RPA_HIP(c, hipEventRecord(c->ev0, c->stream));
+  RPA_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+  RPA_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (int e = sync(c)) return e;
+  float ms = 0.f;
+  RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  return RPA_OK;
+}
+
+int rpa_get_vertices(rpa_ctx *c, double *rho1, double *rho2, long long *ntrans_out) {
+  if (!c) return RPA_E_ARG;
+  (void)hipSetDevice(c->d.device);
+  if (int e = prepare(c)) return e;
+  const size_t n = (size_t)c->ntrans * c->nlf;
+  if (ntrans_out) *ntrans_out = c->ntrans;
+  if (!rho1) return RPA_OK;
+  double2 *t1 = nullptr, *t2 = nullptr;
+  RPA_HIP(c, hipMalloc((void **)&t1, sizeof(double2) * std::max<size_t>(n, 1)));
+  hipError_t he = hipSuccess;
+  if (rho2) he = hipMalloc((void **)&t2, sizeof(double2) * std::max<size_t>(n, 1));
+  int e = (he == hipSuccess) ? launch_vertices(c, t1, t2, nullptr) : hip_fail(c, he, "hipMalloc");
+  if (!e) e = d2h(c, rho1, (const double *)t1, 2 * n);
+  if (!e && rho2) e = d2h(c, rho2, (const double *)t2, 2 * n);
+  (void)hipFree(t1);
+  if (t2) (void)hipFree(t2);
+  return e;
+}
+
+}  // extern "C"

@@ -1,0 +1,225 @@
+// Internal declarations of librpa_hip.so (MI355X / gfx950).
+//
+// Device data layout (HBM):
+//   basis    evc[nki][npwx][nband]  complex  (G-major, bands fastest: one
+//            plane wave of all bands is one contiguous 16*nband-byte row, so
+//            the gathered vertex operand C2[map(i)][m] is a coalesced row read)
+//   mill     mill_k[nki][npwx]      packed Miller key (10 bits per component)
+//   kbox     kbox[nki][box]         int16 position of each Miller triple in
+//            that k's coefficient list (-1 = absent); gbox[box] int32 position
+//            in the global G list (direct map)
+//   vertices rho[pos][nlf]          complex, pos = transition position sorted
+//            by the first frequency tile its Lorentzian window reaches
+//   S0       s0[nw][ncol]           real (Sloss) or interleaved complex
+//            (Optabs) columns col = iG*NGP + jG (x2 for complex)
+//   chi0     chi0[no-1][nlf][nlf]   complex (row-major iG, jG)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rpa_hip.h"
+
+namespace rpa {
+
+constexpr int kMillBias = 512;
+__host__ __device__ inline int mill_pack(int a, int b, int c) {
+  return ((a + kMillBias) << 20) | ((b + kMillBias) << 10) | (c + kMillBias);
+}
+__host__ __device__ inline void mill_unpack(int key, int &a, int &b, int &c) {
+  a = ((key >> 20) & 1023) - kMillBias;
+  b = ((key >> 10) & 1023) - kMillBias;
+  c = (key & 1023) - kMillBias;
+}
+
+struct Box {
+  int lo[3];
+  int n[3];
+  __host__ __device__ int size() const { return n[0] * n[1] * n[2]; }
+  // flattened index of a Miller triple, -1 if outside
+  __host__ __device__ int index(int a, int b, int c) const {
+    a -= lo[0];
+    b -= lo[1];
+    c -= lo[2];
+    if ((unsigned)a >= (unsigned)n[0] || (unsigned)b >= (unsigned)n[1] || (unsigned)c >= (unsigned)n[2])
+      return -1;
+    return (a * n[1] + b) * n[2] + c;
+  }
+};
+
+// A rectangular block of band pairs (n in [n_lo,n_hi), m in [m_lo,m_hi)).
+struct PairBlock {
+  int n_lo, n_hi, m_lo, m_hi;
+};
+
+constexpr int kMaxPairBlocks = 4;
+constexpr int kTileW = 32;     // frequency rows per accumulation tile
+constexpr int kColsWG = 256;   // S0 columns per accumulation work item
+constexpr int kChunkT = 4096;  // transitions per accumulation work item
+constexpr int kMaxNlf = 128;   // local fields supported by the kernels
+
+struct PairTable {
+  int nblocks;
+  int tpk;  // transitions per k
+  int off[kMaxPairBlocks];
+  PairBlock b[kMaxPairBlocks];
+};
+
+}  // namespace rpa
+
+struct rpa_ctx {
+  rpa_dims d{};
+  rpa_params p{};
+  bool have_params = false;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+
+  // G list
+  int ngm = 0;
+  std::vector<int> mill_g;  // 3*ngm
+  double kc[9] = {0};
+  int *d_mill_g = nullptr;  // packed keys [ngm]
+  int *d_gbox = nullptr;    // global box -> index
+  rpa::Box gbox{};
+
+  // basis
+  void *d_evc = nullptr;
+  int *d_mill_k = nullptr;
+  int16_t *d_kbox = nullptr;
+  int *d_ngk = nullptr;
+  std::vector<int> ngk;
+  std::vector<char> uploaded;
+  int mill_min[3] = {1 << 20, 1 << 20, 1 << 20}, mill_max[3] = {-(1 << 20), -(1 << 20), -(1 << 20)};
+  rpa::Box kbox{};
+  bool kbox_ready = false;
+
+  // bands
+  std::vector<double> E;  // E(nki, nband)
+  double *d_E = nullptr;
+
+  // local fields
+  int nlf = 0;
+  int ngp = 0;  // nlf padded to a multiple of 8
+  std::vector<int> lf_idx, parg;
+  int *d_lf_key = nullptr;
+  double *d_glf = nullptr;  // cartesian (2pi/a0) 3*nlf
+  int *d_parg = nullptr;
+
+  // q
+  int ntot = 0;
+  double q[3] = {0, 0, 0};
+  std::vector<int> K1, K2, iG0;
+  std::vector<double> ktot;
+  int *d_k1 = nullptr, *d_k2 = nullptr, *d_m0 = nullptr;
+  double *d_ktot = nullptr;
+  bool have_q = false;
+
+  // transitions
+  rpa::PairTable pairs{};
+  long long ntrans = 0;
+  int nw = 0;       // frequency rows of S0
+  int ntiles = 0;   // frequency tiles
+  int span = 0;     // extra tiles a Lorentzian window can reach
+  double wcut = 0;  // |Lor| threshold
+  int nthread_blocks = 0;
+  int *d_key = nullptr;      // home tile per natural transition (-1 none)
+  int *d_perm = nullptr;     // natural -> sorted position (-1 none)
+  int *d_bcnt = nullptr;     // per-block per-tile counts / bases
+  int *d_toff = nullptr;     // tile offsets [ntiles+1]
+  int *d_nat = nullptr;      // sorted -> natural
+  double *d_te = nullptr;    // sorted (E1, E2, df) triples
+  int *d_items = nullptr;    // accumulation work items
+  int *d_nitems = nullptr;   // [0] = item count, [1] = total transitions kept
+  int *d_first_item = nullptr;
+  int *d_intra = nullptr;    // intraband natural transitions [count]
+  int *d_nintra = nullptr;
+  size_t max_items = 0;
+  size_t cap_trans = 0;
+
+  // vertices (sorted order), complex
+  void *d_rho1 = nullptr, *d_rho2 = nullptr;
+  size_t cap_rho = 0;
+
+  // S0
+  int ncol = 0, ncol_pad = 0;
+  double *d_s0 = nullptr;
+  double *d_part = nullptr;
+  double *d_qeff = nullptr;
+  size_t cap_part = 0;
+
+  // KK, Dyson
+  double *d_kmat = nullptr;  // stencil (no-1) x nw
+  int kmat_no = 0;
+  double kmat_dw = 0;
+  double *d_x = nullptr;     // K * S0 (no-1) x ncol_pad
+  double *d_chi0 = nullptr;  // complex (no-1) nlf nlf
+  double *d_V = nullptr;     // nlf x nlf
+  double *d_epsinv = nullptr, *d_chi = nullptr, *d_W = nullptr;
+  double *d_loss = nullptr, *d_pi11 = nullptr, *d_sig = nullptr;
+  double *d_scratch = nullptr;
+  size_t cap_scratch = 0;
+  int *d_status = nullptr;
+
+  // graph
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_valid = false;
+  bool prepared = false;
+  // capacities (elements) of the buffers sized by prepare()
+  size_t cap_key = 0, cap_bcnt = 0, cap_toff = 0, cap_items = 0, cap_first = 0, cap_nitems = 0;
+  size_t cap_te = 0, cap_s0 = 0, cap_kmat = 0, cap_x = 0, cap_chi0 = 0, cap_epsinv = 0;
+  size_t cap_chi = 0, cap_W = 0, cap_V = 0, cap_loss = 0, cap_pi11 = 0, cap_sig = 0;
+  size_t cap_qeff = 0, cap_status = 0, cap_rho2 = 0;
+};
+
+namespace rpa {
+
+int fail(rpa_ctx *c, int code, const std::string &msg);
+int hip_fail(rpa_ctx *c, hipError_t e, const char *what);
+
+#define RPA_HIP(ctx, call)                                 \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return rpa::hip_fail(ctx, e_, #call); \
+  } while (0)
+
+template <class T>
+int ensure(rpa_ctx *c, T *&ptr, size_t &cap, size_t n) {
+  if (n <= cap && ptr) return 0;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  hipError_t e = hipMalloc((void **)&ptr, sizeof(T) * (n ? n : 1));
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
+  cap = n;
+  return 0;
+}
+
+// kernel launchers (defined in the .hip files)
+int launch_build_kbox(rpa_ctx *c);
+int launch_transitions(rpa_ctx *c);
+int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm);
+int launch_accumulate(rpa_ctx *c);
+int launch_kstencil(rpa_ctx *c);
+int launch_kk(rpa_ctx *c);
+int launch_coulomb(rpa_ctx *c);
+int launch_dyson(rpa_ctx *c);
+int launch_epilogue(rpa_ctx *c);
+
+}  // namespace rpa
+
+// ---- device helpers -------------------------------------------------------
+#if defined(__HIPCC__)
+typedef double rpa_d4 __attribute__((ext_vector_type(4)));
+
+// D(16x16) += A(16x4) B(4x16) in FP64 on the matrix cores.
+// Operand layout (wave64, measured on gfx950): lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; element e of lane l's result is D[4*e + l/16][l%16].
+__device__ inline rpa_d4 mfma_f64_16x16x4(double a, double b, rpa_d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+#endif

@@ -1,0 +1,1135 @@
+/*
+ * CPU oracle: FP64 restatement of the RPA response path of the reference
+ * programs Optabs.x (Chi_RPA/Optabs.f90) and Sloss.x (W/Sloss.f90) and of the
+ * reference modules (the Modules directory).  Every function cites the reference
+ * lines it follows.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load this library, and only as the checker
+ * or as the timed CPU baseline.  The product library never uses it.
+ *
+ * Restrictions (documented, not silent):
+ *   - symmetry operations are the identity (Nsym = 1): R = RI = I, R1 = R2 = 1;
+ *   - the G-vector lookup of the vertex map uses exact Miller arithmetic with a
+ *     hash (orc_gfast_miller); orc_gfast_linear restates the reference's
+ *     linear search with the eps tolerance and the tests check they agree.
+ */
+#include "rpa_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static const double ORC_PI = 3.14159265358979323846; /* 4*atan(1), Optabs.f90:44 */
+
+/* ------------------------------------------------------------------------ */
+/* a4: local fields.  Optabs.f90:1016-1059 (genGlf), Sloss.f90:824-881
+ * (genGlfandParity), Modules/local_field.f90:11-94.  parity from the Miller z
+ * index as Sloss.f90:1349 stores it.  Returns -1 when Nlf > Nlfd (the
+ * reference stops, Optabs.f90:1054-1057).                                    */
+int orc_gen_glf(const char *lf, double ecut, double gcar, int ng, const double *g,
+                const int *mill_g, int nlfd, int *nlf, double *glf, int *parg,
+                int *idx) {
+  int n = 0;
+  int is_z = (strcmp(lf, "z") == 0);
+  int is_xyz = (strcmp(lf, "xyz") == 0);
+  if (!is_z && !is_xyz) return -2;
+  for (int iG = 0; iG < ng; ++iG) {
+    const double *G = g + 3 * iG;
+    double eref;
+    int take = 0;
+    if (is_z) {
+      if (G[0] == 0.0 && G[1] == 0.0) {
+        eref = gcar * gcar * (G[2] * G[2]) / 2.0;
+        take = (eref <= ecut);
+      }
+    } else {
+      eref = gcar * gcar * ((G[0] * G[0] + G[1] * G[1]) + G[2] * G[2]) / 2.0;
+      take = (eref <= ecut);
+    }
+    if (!take) continue;
+    if (n < nlfd) {
+      if (glf) {
+        glf[3 * n + 0] = is_z ? 0.0 : G[0];
+        glf[3 * n + 1] = is_z ? 0.0 : G[1];
+        glf[3 * n + 2] = G[2];
+      }
+      if (parg) {
+        int pz = mill_g ? mill_g[3 * iG + 2] : 0;
+        parg[n] = ((pz / 2) * 2 == pz) ? 1 : -1;
+      }
+      if (idx) idx[n] = iG + 1;
+    }
+    ++n;
+  }
+  *nlf = n;
+  return (n > nlfd) ? -1 : 0;
+}
+
+/* a5: Fermi-Dirac occupation, Optabs.f90:935-951 (genOccupation). */
+void orc_occupation(double ei, double ef, double T, double *expo, double *f) {
+  double x = (ei - ef) / T;
+  if (x < -20) {
+    *expo = 0.0;
+    *f = 1.0;
+  } else if (x > 20) {
+    *expo = x;
+    *f = 0.0;
+  } else {
+    *expo = exp(x);
+    *f = 1.0 / (*expo + 1.0);
+  }
+}
+
+/* a3: FBZ generation, Optabs.f90:697-759 (genFBZ, Sloss.f90 same).  R is the
+ * reference's R(48,3,3).  Writing fbz_check.dat is left to the caller.       */
+int orc_gen_fbz(int nki, int nsym, double eps, const double *ki, const double *R,
+                int *ntot, double *ktot) {
+  int nk = 48 * nki;
+  double *k = (double *)malloc(sizeof(double) * 3 * (size_t)nk);
+  if (!k) return -1;
+  int jk = 0, nt = 0;
+  for (int i = 0; i < nsym; ++i) {
+    for (int ik = 0; ik < nki; ++ik) {
+      int it = 1;
+      for (int n = 0; n < 3; ++n) {
+        double s = 0.0;
+        for (int m = 0; m < 3; ++m) s = s + R[i + 48 * (n + 3 * m)] * ki[3 * ik + m];
+        k[3 * jk + n] = s;
+      }
+      for (int lk = 0; lk < jk; ++lk)
+        if (fabs(k[3 * jk] - k[3 * lk]) <= eps && fabs(k[3 * jk + 1] - k[3 * lk + 1]) <= eps &&
+            fabs(k[3 * jk + 2] - k[3 * lk + 2]) <= eps)
+          it = 2;
+      if (it == 1) {
+        memcpy(ktot + 3 * nt, k + 3 * jk, 3 * sizeof(double));
+        ++nt;
+      }
+      ++jk;
+    }
+  }
+  free(k);
+  *ntot = nt;
+  return 0;
+}
+
+/* a3: q = (iq-1) * k_min, Optabs.f90:743-773 (findMinQ). */
+void orc_find_min_q(int ntot, const double *ktot, int iq, double q[3]) {
+  double kmin = 1.0;
+  int ikmin = 0;
+  for (int i = 0; i < ntot; ++i) {
+    const double *k = ktot + 3 * i;
+    double kref = sqrt((k[0] * k[0] + k[1] * k[1]) + k[2] * k[2]);
+    if (kref == 0.0) continue;
+    if (kref < kmin) {
+      kmin = kref;
+      ikmin = i;
+    }
+  }
+  for (int c = 0; c < 3; ++c) q[c] = (double)(iq - 1) * ktot[3 * ikmin + c];
+}
+
+/* a3: k+q = G0 + K2, Optabs.f90:820-869 / Sloss.f90:646-695 (findKQinBZ)
+ * with Nsym = 1.  Returns 1 (not in FBZ) or 2 (not in IBZ) as the reference's
+ * two stop paths, 0 on success.  iG0, K2 are 1-based.                        */
+int orc_find_kq(const double kq[3], double eps, int nki, int ntot, int ng,
+                const double *ktot, const double *ki, const double *g, int *iG0,
+                int *K2) {
+  int it = 1;
+  for (int iG = 0; iG < ng; ++iG) {
+    for (int jk = 0; jk < ntot; ++jk) {
+      if (fabs(kq[0] - g[3 * iG] - ktot[3 * jk]) <= eps &&
+          fabs(kq[1] - g[3 * iG + 1] - ktot[3 * jk + 1]) <= eps &&
+          fabs(kq[2] - g[3 * iG + 2] - ktot[3 * jk + 2]) <= eps) {
+        it = 2;
+        *iG0 = iG + 1;
+        /* identity symmetry: K11 = sum(RI(1,1,1:3)*ktot(1:3,jk)) */
+        double K11 = (1.0 * ktot[3 * jk] + 0.0 * ktot[3 * jk + 1]) + 0.0 * ktot[3 * jk + 2];
+        double K22 = (0.0 * ktot[3 * jk] + 1.0 * ktot[3 * jk + 1]) + 0.0 * ktot[3 * jk + 2];
+        double K33 = (0.0 * ktot[3 * jk] + 0.0 * ktot[3 * jk + 1]) + 1.0 * ktot[3 * jk + 2];
+        for (int j = 0; j < nki; ++j) {
+          if (fabs(K11 - ki[3 * j]) <= eps && fabs(K22 - ki[3 * j + 1]) <= eps &&
+              fabs(K33 - ki[3 * j + 2]) <= eps) {
+            *K2 = j + 1;
+            return 0;
+          }
+        }
+      }
+    }
+  }
+  return it == 1 ? 1 : 2;
+}
+
+/* G(n) = sum_m KC(n,m) * mill(m), Optabs_qe6.f90:1162-1167 (loadG_QE6). */
+void orc_gcart(int n, const int *mill, const double *kc, double *g) {
+  for (int i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      double s = 0.0;
+      for (int b = 0; b < 3; ++b) s = s + kc[a + 3 * b] * (double)mill[3 * i + b];
+      g[3 * i + a] = s;
+    }
+}
+
+/* a2: |k+G|^2 <= gcutw, sorted by |k+G| with index tie-break within eps8.
+ * Modules/qe_gksort.f90:44-94 (gk_sort) and 96-203 (hpsort_eps).  igk is
+ * 1-based.  Returns -1 on npwx overflow, -2 when the G list ends before the
+ * |G| > |k| + sqrt(ecut) exit (both are error stops in the reference).      */
+typedef struct {
+  double q;
+  int ind;
+} orc_gk_entry;
+static int orc_gk_cmp(const void *a, const void *b) {
+  const orc_gk_entry *x = (const orc_gk_entry *)a, *y = (const orc_gk_entry *)b;
+  if (fabs(x->q - y->q) >= 1.0e-8) return (x->q < y->q) ? -1 : 1;
+  return (x->ind < y->ind) ? -1 : (x->ind > y->ind);
+}
+int orc_gk_sort(const double k[3], int ngm, const double *g, double gcutw, int npwx,
+                int *ngk, int *igk) {
+  const double eps8 = 1.0e-8;
+  double kk = sqrt((k[0] * k[0] + k[1] * k[1]) + k[2] * k[2]);
+  double q2x = (kk + sqrt(gcutw)) * (kk + sqrt(gcutw));
+  orc_gk_entry *e = (orc_gk_entry *)malloc(sizeof(orc_gk_entry) * (size_t)(npwx + 1));
+  int n = 0, ng;
+  for (ng = 0; ng < ngm; ++ng) {
+    const double *G = g + 3 * ng;
+    double a0 = k[0] + G[0], a1 = k[1] + G[1], a2 = k[2] + G[2];
+    double q = (a0 * a0 + a1 * a1) + a2 * a2;
+    if (q <= eps8) q = 0.0;
+    if (q <= gcutw) {
+      if (n >= npwx) {
+        free(e);
+        return -1;
+      }
+      e[n].q = q;
+      e[n].ind = ng + 1;
+      ++n;
+    } else if ((G[0] * G[0] + G[1] * G[1]) + G[2] * G[2] > q2x + eps8) {
+      break;
+    }
+  }
+  if (ng >= ngm) {
+    free(e);
+    return -2;
+  }
+  qsort(e, (size_t)n, sizeof(orc_gk_entry), orc_gk_cmp);
+  for (int i = 0; i < n; ++i) igk[i] = e[i].ind;
+  *ngk = n;
+  free(e);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a6: vertex index map with the reference's linear search and tolerance,
+ * Sloss.f90:907-946 / Optabs.f90:1399-1449 / vertices.f90:120-168, R = RI = I.
+ * gfast[iG*n1 + i] is the 0-based position in list 2, or -1.                */
+void orc_gfast_linear(int n1, const double *l1, int n2, const double *l2, int nlf,
+                      const double *glf, const double g0[3], double eps, int *gfast) {
+  for (int iG = 0; iG < nlf; ++iG)
+    for (int i = 0; i < n1; ++i) {
+      double t0 = (l1[3 * i] + glf[3 * iG]) + g0[0];
+      double t1 = (l1[3 * i + 1] + glf[3 * iG + 1]) + g0[1];
+      double t2 = (l1[3 * i + 2] + glf[3 * iG + 2]) + g0[2];
+      int hit = -1;
+      for (int j = 0; j < n2; ++j)
+        if (fabs(l2[3 * j] - t0) < eps && fabs(l2[3 * j + 1] - t1) < eps &&
+            fabs(l2[3 * j + 2] - t2) < eps) {
+          hit = j;
+          break;
+        }
+      gfast[iG * n1 + i] = hit;
+    }
+}
+
+/* Same map by exact Miller arithmetic (first match kept), O(n1 + n2). */
+static inline unsigned orc_hkey(int a, int b, int c) {
+  unsigned h = (unsigned)(a * 73856093) ^ (unsigned)(b * 19349663) ^ (unsigned)(c * 83492791);
+  return h * 2654435761u;
+}
+int orc_gfast_miller(int n1, const int *m1, int n2, const int *m2, int nlf,
+                     const int *mlf, const int m0[3], int *gfast) {
+  unsigned cap = 1;
+  while (cap < 2u * (unsigned)n2 + 2u) cap <<= 1;
+  int *slot = (int *)malloc(sizeof(int) * cap);
+  if (!slot) return -1;
+  for (unsigned s = 0; s < cap; ++s) slot[s] = -1;
+  for (int j = 0; j < n2; ++j) {
+    unsigned h = orc_hkey(m2[3 * j], m2[3 * j + 1], m2[3 * j + 2]) & (cap - 1);
+    for (;;) {
+      int o = slot[h];
+      if (o < 0) {
+        slot[h] = j;
+        break;
+      }
+      if (m2[3 * o] == m2[3 * j] && m2[3 * o + 1] == m2[3 * j + 1] &&
+          m2[3 * o + 2] == m2[3 * j + 2])
+        break; /* keep first occurrence */
+      h = (h + 1) & (cap - 1);
+    }
+  }
+  for (int iG = 0; iG < nlf; ++iG)
+    for (int i = 0; i < n1; ++i) {
+      int a = m1[3 * i] + mlf[3 * iG] + m0[0];
+      int b = m1[3 * i + 1] + mlf[3 * iG + 1] + m0[1];
+      int c = m1[3 * i + 2] + mlf[3 * iG + 2] + m0[2];
+      unsigned h = orc_hkey(a, b, c) & (cap - 1);
+      int hit = -1;
+      for (;;) {
+        int o = slot[h];
+        if (o < 0) break;
+        if (m2[3 * o] == a && m2[3 * o + 1] == b && m2[3 * o + 2] == c) {
+          hit = o;
+          break;
+        }
+        h = (h + 1) & (cap - 1);
+      }
+      gfast[iG * n1 + i] = hit;
+    }
+  free(slot);
+  return 0;
+}
+
+/* Per-k lists of the two coefficient sets (a6 inputs).  Direct mode:
+ * coefficient i <-> global G(i) for i < NGd (C1) and i < ngk(K2) (C2),
+ * Sloss.f90:907-946.  IGK mode: coefficient i <-> mill_k(:,i,K),
+ * vertices.f90:139,162.                                                     */
+typedef struct {
+  int n1, n2;
+  const int *m1, *m2;   /* Miller lists */
+  const double *c1base; /* evc(:,:,K1) */
+  const double *c2base; /* evc(:,:,K2) */
+  int K1, K2;           /* 0-based */
+  int m0[3];
+} orc_kpair;
+
+static int orc_make_pair(const orc_problem *p, int ik, orc_kpair *kp) {
+  int K1 = p->K1[ik] - 1, K2 = p->K2[ik] - 1, ig0 = p->iG0[ik] - 1;
+  kp->K1 = K1;
+  kp->K2 = K2;
+  for (int c = 0; c < 3; ++c) kp->m0[c] = p->mill_g[3 * ig0 + c];
+  size_t kstride = (size_t)2 * p->npwx * p->nband;
+  kp->c1base = p->evc + kstride * K1;
+  kp->c2base = p->evc + kstride * K2;
+  if (p->map_mode == ORC_MAP_DIRECT) {
+    if (p->ngd > p->ngk[K1] || p->ngd > p->ngk[K2]) return -3; /* Optabs.f90:349-352,377-380 */
+    kp->n1 = p->ngd;
+    kp->n2 = p->ngk[K2];
+    if (kp->n2 > p->ngm) return -4;
+    kp->m1 = p->mill_g;
+    kp->m2 = p->mill_g;
+  } else {
+    kp->n1 = p->ngk[K1];
+    kp->n2 = p->ngk[K2];
+    kp->m1 = p->mill_k + (size_t)3 * p->npwx * K1;
+    kp->m2 = p->mill_k + (size_t)3 * p->npwx * K2;
+  }
+  return 0;
+}
+
+static void orc_lf_miller(const orc_problem *p, int *mlf) {
+  for (int iG = 0; iG < p->nlf; ++iG)
+    for (int c = 0; c < 3; ++c) mlf[3 * iG + c] = p->mill_g[3 * (p->lf_idx[iG] - 1) + c];
+}
+
+/* a7: charge vertex rho(iG) = sum_i conj(C1(i)) C2(gfast), Sloss.f90:905-951. */
+static void orc_charge_vertex(int nlf, int n1, const int *gfast, const double *c1,
+                              const double *c2, double *rr, double *ri) {
+  for (int iG = 0; iG < nlf; ++iG) {
+    double sr = 0.0, si = 0.0;
+    const int *gf = gfast + (size_t)iG * n1;
+    for (int i = 0; i < n1; ++i) {
+      int j = gf[i];
+      if (j < 0) continue;
+      double ar = c1[2 * i], ai = -c1[2 * i + 1];
+      double br = c2[2 * j], bi = c2[2 * j + 1];
+      sr = sr + (ar * br - ai * bi);
+      si = si + (ar * bi + ai * br);
+    }
+    rr[iG] = sr;
+    ri[iG] = si;
+  }
+}
+
+/* a8: current vertex, Optabs.f90:1399-1469 (direct) and vertices.f90:343-390
+ * (_v2): j(iG) += ((0.5*conj(C1)) * J) * C2 with
+ * J = (q + 2k + Glf(iG) + 2 R G(iG1))_pol * Gcar.                           */
+static void orc_current_vertex(int pol, int nlf, int n1, const int *gfast,
+                               const double *l1, const double *glf, const double q[3],
+                               const double kv[3], double gcar, const double *c1,
+                               const double *c2, double *rr, double *ri) {
+  for (int iG = 0; iG < nlf; ++iG) {
+    double sr = 0.0, si = 0.0;
+    const int *gf = gfast + (size_t)iG * n1;
+    for (int i = 0; i < n1; ++i) {
+      int j = gf[i];
+      if (j < 0) continue;
+      double J = (((q[pol] + 2.0 * kv[pol]) + glf[3 * iG + pol]) + 2.0 * l1[3 * i + pol]) * gcar;
+      double ar = 0.5 * c1[2 * i], ai = 0.5 * (-c1[2 * i + 1]);
+      ar = ar * J;
+      ai = ai * J;
+      double br = c2[2 * j], bi = c2[2 * j + 1];
+      sr = sr + (ar * br - ai * bi);
+      si = si + (ar * bi + ai * br);
+    }
+    rr[iG] = sr;
+    ri[iG] = si;
+  }
+}
+
+int orc_vertices(const orc_problem *p, int kind, double gcar, int n_lo, int n_hi,
+                 int m_lo, int m_hi, orc_z *rho1, orc_z *rho2) {
+  int nlf = p->nlf;
+  int nn = n_hi - n_lo, nm = m_hi - m_lo;
+  size_t ntrans = (size_t)p->ntot * nn * nm;
+  double *glf = (double *)malloc(sizeof(double) * 3 * nlf);
+  int *mlf = (int *)malloc(sizeof(int) * 3 * nlf);
+  orc_lf_miller(p, mlf);
+  orc_gcart(nlf, mlf, p->kc, glf);
+  int err = 0;
+#pragma omp parallel
+  {
+    int *gfast = (int *)malloc(sizeof(int) * (size_t)nlf * (p->ngd > p->npwx ? p->ngd : p->npwx));
+    double *l1 = (double *)malloc(sizeof(double) * 3 * (size_t)(p->ngd > p->npwx ? p->ngd : p->npwx));
+    double *vr = (double *)malloc(sizeof(double) * nlf * 2);
+    double *vi = (double *)malloc(sizeof(double) * nlf * 2);
+#pragma omp for schedule(static)
+    for (int ik = 0; ik < p->ntot; ++ik) {
+      orc_kpair kp;
+      if (orc_make_pair(p, ik, &kp)) {
+        err = -3;
+        continue;
+      }
+      orc_gfast_miller(kp.n1, kp.m1, kp.n2, kp.m2, nlf, mlf, kp.m0, gfast);
+      orc_gcart(kp.n1, kp.m1, p->kc, l1);
+      for (int n = n_lo; n < n_hi; ++n) {
+        const double *c1 = kp.c1base + (size_t)2 * p->npwx * n;
+        for (int m = m_lo; m < m_hi; ++m) {
+          const double *c2 = kp.c2base + (size_t)2 * p->npwx * m;
+          size_t t = ((size_t)ik * nn + (n - n_lo)) * nm + (m - m_lo);
+          if (kind == 0) {
+            orc_charge_vertex(nlf, kp.n1, gfast, c1, c2, vr, vi);
+            for (int iG = 0; iG < nlf; ++iG) rho1[(size_t)iG * ntrans + t] = vr[iG] + I * vi[iG];
+          } else {
+            const double *kv = p->ktot + 3 * ik;
+            int pol1 = (kind == 4) ? 1 : kind - 1;
+            orc_current_vertex(pol1, nlf, kp.n1, gfast, l1, glf, p->q, kv, gcar, c1, c2, vr, vi);
+            for (int iG = 0; iG < nlf; ++iG) rho1[(size_t)iG * ntrans + t] = vr[iG] + I * vi[iG];
+            if (rho2) {
+              if (kind == 4)
+                orc_current_vertex(2, nlf, kp.n1, gfast, l1, glf, p->q, kv, gcar, c1, c2, vr, vi);
+              for (int iG = 0; iG < nlf; ++iG) rho2[(size_t)iG * ntrans + t] = vr[iG] + I * vi[iG];
+            }
+          }
+        }
+      }
+    }
+    free(gfast);
+    free(l1);
+    free(vr);
+    free(vi);
+  }
+  free(glf);
+  free(mlf);
+  return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a9/a10 Sloss: real S0(no,Nlf,Nlf), Sloss.f90:355-430.
+ *   dE = o + E(K1,n) - E(K2,m);  Lor = -eta/(dE^2+eta^2);
+ *   if |Lor| >= 1e-3/eta:  S0 -= Re[2 Lor rho(iG) conj(rho(jG))] / (pi Ntot Vcell)
+ * The per-k partial sums are added in k order within each thread's static
+ * block, then the thread blocks in thread order (deterministic for a fixed
+ * thread count).                                                           */
+int orc_sloss_s0(const orc_problem *p, const orc_sloss_params *sp, double *S0,
+                 int nthreads) {
+  const int nlf = p->nlf, no = sp->no, nband = p->nband, nocc = sp->nocc;
+  const size_t s0n = (size_t)no * nlf * nlf;
+  const double den = (ORC_PI * (double)p->ntot) * sp->vcell;
+  const double cut = 1.0e-3 / sp->eta;
+  int *mlf = (int *)malloc(sizeof(int) * 3 * nlf);
+  orc_lf_miller(p, mlf);
+  if (nthreads <= 0) nthreads = 1;
+  double *acc = (double *)calloc((size_t)nthreads * s0n, sizeof(double));
+  int err = 0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double *mine = acc + (size_t)tid * s0n;
+    /* internal layout [io][iG][jG] for locality; transposed at the end */
+    double *part = (double *)malloc(sizeof(double) * s0n);
+    int nmax = p->ngd > p->npwx ? p->ngd : p->npwx;
+    int *gfast = (int *)malloc(sizeof(int) * (size_t)nlf * nmax);
+    double *rr = (double *)malloc(sizeof(double) * nlf);
+    double *ri = (double *)malloc(sizeof(double) * nlf);
+#pragma omp for schedule(static)
+    for (int ik = 0; ik < p->ntot; ++ik) {
+      orc_kpair kp;
+      if (orc_make_pair(p, ik, &kp)) {
+        err = -3;
+        continue;
+      }
+      orc_gfast_miller(kp.n1, kp.m1, kp.n2, kp.m2, nlf, mlf, kp.m0, gfast);
+      memset(part, 0, sizeof(double) * s0n);
+      for (int n = 0; n < nocc; ++n) {
+        const double *c1 = kp.c1base + (size_t)2 * p->npwx * n;
+        const double e1 = p->E[kp.K1 + p->nki * n];
+        for (int m = nocc; m < nband; ++m) {
+          const double *c2 = kp.c2base + (size_t)2 * p->npwx * m;
+          const double e2 = p->E[kp.K2 + p->nki * m];
+          orc_charge_vertex(nlf, kp.n1, gfast, c1, c2, rr, ri);
+          for (int io = 0; io < no; ++io) {
+            double o = (double)io * sp->domega;
+            double dE = (o + e1) - e2;
+            double lor = -sp->eta / (dE * dE + sp->eta * sp->eta);
+            if (!(fabs(lor) >= cut)) continue;
+            double t = 2.0 * lor;
+            double *row = part + (size_t)io * nlf * nlf;
+            for (int iG = 0; iG < nlf; ++iG) {
+              double ar = t * rr[iG], ai = t * ri[iG];
+              double *dst = row + (size_t)iG * nlf;
+              for (int jG = 0; jG < nlf; ++jG) dst[jG] -= (ar * rr[jG] + ai * ri[jG]) / den;
+            }
+          }
+        }
+      }
+      for (size_t x = 0; x < s0n; ++x) mine[x] += part[x];
+    }
+    free(part);
+    free(gfast);
+    free(rr);
+    free(ri);
+  }
+  for (int io = 0; io < no; ++io)
+    for (int iG = 0; iG < nlf; ++iG)
+      for (int jG = 0; jG < nlf; ++jG) {
+        double s = 0.0;
+        size_t x = ((size_t)io * nlf + iG) * nlf + jG;
+        for (int t = 0; t < nthreads; ++t) s += acc[(size_t)t * s0n + x];
+        S0[io + (size_t)no * (iG + (size_t)nlf * jG)] = s;
+      }
+  free(acc);
+  free(mlf);
+  return err;
+}
+
+/* a9/a10 Optabs: complex S0(-no:no,Nlf,Nlf) and Qeff(Nlf,Nlf),
+ * Optabs.f90:327-461:
+ *   pair kept if |f_n - f_m| >= df_cut or n == m (360);
+ *   n /= m: dE = o + E(K1,n) - E(K2,m), Lor = eta/(dE^2+eta^2),
+ *           if |Lor| >= Lor_cut/eta: S0 -= 2 df Lor j1(iG) conj(j2(jG))/(pi Ntot V);
+ *   n == m and |E-Ef| <= 10 T: Qeff += 2 fact j1 conj(j2)/(Ntot V),
+ *           fact = -expo/((expo+1)^2 T) (430-442).                          */
+int orc_optabs_s0(const orc_problem *p, const orc_optabs_params *op, orc_z *S0,
+                  orc_z *Qeff, int nthreads) {
+  const int nlf = p->nlf, no = op->no, nband = p->nband, nw = 2 * no + 1;
+  const size_t s0n = (size_t)nw * nlf * nlf;
+  const double den = (ORC_PI * (double)p->ntot) * op->vcell;
+  const double dq = (double)p->ntot * op->vcell;
+  const double cut = op->lor_cut / op->eta_inter;
+  int *mlf = (int *)malloc(sizeof(int) * 3 * nlf);
+  double *glf = (double *)malloc(sizeof(double) * 3 * nlf);
+  orc_lf_miller(p, mlf);
+  orc_gcart(nlf, mlf, p->kc, glf);
+  if (nthreads <= 0) nthreads = 1;
+  double *acc = (double *)calloc((size_t)nthreads * 2 * s0n, sizeof(double));
+  double *qacc = (double *)calloc((size_t)nthreads * 2 * nlf * nlf, sizeof(double));
+  int pol1 = (op->pol == 4) ? 1 : op->pol - 1;
+  int pol2 = (op->pol == 4) ? 2 : op->pol - 1;
+  int err = 0;
+#pragma omp parallel num_threads(nthreads)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double *mine = acc + (size_t)tid * 2 * s0n;
+    double *qmine = qacc + (size_t)tid * 2 * nlf * nlf;
+    double *part = (double *)malloc(sizeof(double) * 2 * s0n);
+    double *qpart = (double *)malloc(sizeof(double) * 2 * nlf * nlf);
+    int nmax = p->ngd > p->npwx ? p->ngd : p->npwx;
+    int *gfast = (int *)malloc(sizeof(int) * (size_t)nlf * nmax);
+    double *l1 = (double *)malloc(sizeof(double) * 3 * (size_t)nmax);
+    double *ar = (double *)malloc(sizeof(double) * nlf), *ai = (double *)malloc(sizeof(double) * nlf);
+    double *br = (double *)malloc(sizeof(double) * nlf), *bi = (double *)malloc(sizeof(double) * nlf);
+#pragma omp for schedule(static)
+    for (int ik = 0; ik < p->ntot; ++ik) {
+      orc_kpair kp;
+      if (orc_make_pair(p, ik, &kp)) {
+        err = -3;
+        continue;
+      }
+      orc_gfast_miller(kp.n1, kp.m1, kp.n2, kp.m2, nlf, mlf, kp.m0, gfast);
+      orc_gcart(kp.n1, kp.m1, p->kc, l1);
+      const double *kv = p->ktot + 3 * ik;
+      memset(part, 0, sizeof(double) * 2 * s0n);
+      memset(qpart, 0, sizeof(double) * 2 * nlf * nlf);
+      for (int n = 0; n < nband; ++n) {
+        double expo1, f1, expo2, f2;
+        const double e1 = p->E[kp.K1 + p->nki * n];
+        orc_occupation(e1, op->efermi, op->T, &expo1, &f1);
+        const double *c1 = kp.c1base + (size_t)2 * p->npwx * n;
+        for (int m = 0; m < nband; ++m) {
+          const double e2 = p->E[kp.K2 + p->nki * m];
+          orc_occupation(e2, op->efermi, op->T, &expo2, &f2);
+          double df = f1 - f2;
+          if (!((fabs(df) >= op->df_cut) || (n == m))) continue;
+          int intra = (n == m) && (fabs(e1 - op->efermi) <= 10.0 * op->T);
+          if (n == m && !intra) continue; /* vertex computed but unused (430) */
+          const double *c2 = kp.c2base + (size_t)2 * p->npwx * m;
+          orc_current_vertex(pol1, nlf, kp.n1, gfast, l1, glf, p->q, kv, op->gcar, c1, c2, ar, ai);
+          if (pol2 != pol1)
+            orc_current_vertex(pol2, nlf, kp.n1, gfast, l1, glf, p->q, kv, op->gcar, c1, c2, br, bi);
+          else {
+            memcpy(br, ar, sizeof(double) * nlf);
+            memcpy(bi, ai, sizeof(double) * nlf);
+          }
+          if (n != m) {
+            for (int io = -no; io <= no; ++io) {
+              double o = (double)io * op->domega;
+              double dE = (o + e1) - e2;
+              double lor = op->eta_inter / (dE * dE + op->eta_inter * op->eta_inter);
+              if (!(fabs(lor) >= cut)) continue;
+              double t = (2.0 * df) * lor;
+              double *row = part + (size_t)2 * (io + no) * nlf * nlf;
+              for (int iG = 0; iG < nlf; ++iG) {
+                double xr = t * ar[iG], xi = t * ai[iG];
+                double *dst = row + (size_t)2 * iG * nlf;
+                for (int jG = 0; jG < nlf; ++jG) {
+                  /* (xr + i xi) * (br - i bi) */
+                  double pr = xr * br[jG] + xi * bi[jG];
+                  double pi_ = xi * br[jG] - xr * bi[jG];
+                  dst[2 * jG] -= pr / den;
+                  dst[2 * jG + 1] -= pi_ / den;
+                }
+              }
+            }
+          } else {
+            double fact = expo1 / ((expo1 + 1.0) * (expo1 + 1.0));
+            fact = -fact / op->T;
+            double t = 2.0 * fact;
+            for (int iG = 0; iG < nlf; ++iG)
+              for (int jG = 0; jG < nlf; ++jG) {
+                double xr = t * ar[iG], xi = t * ai[iG];
+                double pr = xr * br[jG] + xi * bi[jG];
+                double pi_ = xi * br[jG] - xr * bi[jG];
+                qpart[2 * (iG * nlf + jG)] += pr / dq;
+                qpart[2 * (iG * nlf + jG) + 1] += pi_ / dq;
+              }
+          }
+        }
+      }
+      for (size_t x = 0; x < 2 * s0n; ++x) mine[x] += part[x];
+      for (int x = 0; x < 2 * nlf * nlf; ++x) qmine[x] += qpart[x];
+    }
+    free(part);
+    free(qpart);
+    free(gfast);
+    free(l1);
+    free(ar);
+    free(ai);
+    free(br);
+    free(bi);
+  }
+  for (int w = 0; w < nw; ++w)
+    for (int iG = 0; iG < nlf; ++iG)
+      for (int jG = 0; jG < nlf; ++jG) {
+        double sr = 0.0, si = 0.0;
+        size_t x = 2 * (((size_t)w * nlf + iG) * nlf + jG);
+        for (int t = 0; t < nthreads; ++t) {
+          sr += acc[(size_t)t * 2 * s0n + x];
+          si += acc[(size_t)t * 2 * s0n + x + 1];
+        }
+        S0[w + (size_t)nw * (iG + (size_t)nlf * jG)] = sr + I * si;
+      }
+  if (Qeff)
+    for (int iG = 0; iG < nlf; ++iG)
+      for (int jG = 0; jG < nlf; ++jG) {
+        double sr = 0.0, si = 0.0;
+        for (int t = 0; t < nthreads; ++t) {
+          sr += qacc[(size_t)t * 2 * nlf * nlf + 2 * (iG * nlf + jG)];
+          si += qacc[(size_t)t * 2 * nlf * nlf + 2 * (iG * nlf + jG) + 1];
+        }
+        Qeff[iG + (size_t)nlf * jG] = sr + I * si;
+      }
+  free(acc);
+  free(qacc);
+  free(mlf);
+  free(glf);
+  return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a12 K1: Optabs genReChi0/genImChi0, Optabs.f90:1112-1318 (== the complex
+ * variant of Modules/response_function_KK.f90:74-178).  S0(-no:no,Nlf,Nlf),
+ * Pi0(no-1,Nlf,Nlf) for io = 1..no-1.                                       */
+static double orc_k1_part(int io, int no, double oi, double domega, const orc_z *s,
+                          int want_re) {
+  /* s points at S0(0,iG,jG) (frequency index 0) */
+#define S0P(j) (want_re ? creal(s[(j)]) : cimag(s[(j)]))
+  double acc = 0.0, fact = 0.0, oj;
+  if (io == 1) {
+    for (int jo = 2; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      fact = domega / oj;
+      if (jo == 2)
+        fact = 3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / oj;
+      acc = acc + fact * (S0P(-jo + 1) - S0P(jo - 1));
+    }
+  } else if (io == 2) {
+    for (int jo = 1; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      if (jo != io) fact = domega / (oi - oj);
+      if (jo == 1)
+        fact = 1.0;
+      else if (jo == 2)
+        fact = 0.0;
+      else if (jo == 3)
+        fact = -3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / (oi - oj);
+      acc = acc + fact * S0P(jo - 1);
+      fact = domega / (oi + oj);
+      if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+      acc = acc + fact * S0P(-jo + 1);
+    }
+  } else if (io == no - 1) {
+    for (int jo = 1; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      if (jo != io) fact = domega / (oi - oj);
+      if (jo == 1)
+        fact = 0.5 * domega / (oi - oj);
+      else if (jo == no - 2)
+        fact = 3.0 / 2.0;
+      else if (jo == no - 1)
+        fact = 0.0;
+      else if (jo == no)
+        fact = -1.0;
+      acc = acc + fact * S0P(jo - 1);
+      fact = domega / (oi + oj);
+      if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+      acc = acc + fact * S0P(-jo + 1);
+    }
+  } else {
+    for (int jo = 1; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      if (jo != io) fact = domega / (oi - oj);
+      if (jo == 1)
+        fact = 0.5 * domega / (oi - oj);
+      else if (jo == io - 1)
+        fact = 3.0 / 2.0;
+      else if (jo == io)
+        fact = 0.0;
+      else if (jo == io + 1)
+        fact = -3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / (oi - oj);
+      acc = acc + fact * S0P(jo - 1);
+      fact = domega / (oi + oj);
+      if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+      acc = acc + fact * S0P(-jo + 1);
+    }
+  }
+#undef S0P
+  return acc;
+}
+
+void orc_kk_optabs(int no, int nlf, double domega, const orc_z *S0, orc_z *Pi0) {
+  const int nw = 2 * no + 1;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int jG = 0; jG < nlf; ++jG)
+    for (int iG = 0; iG < nlf; ++iG) {
+      const orc_z *s = S0 + (size_t)nw * (iG + (size_t)nlf * jG) + no; /* frequency 0 */
+      for (int io = 1; io <= no - 1; ++io) {
+        double oi = (double)(io - 1) * domega;
+        double re = orc_k1_part(io, no, oi, domega, s, 1);
+        re = re + ORC_PI * cimag(s[io - 1]);
+        double im = orc_k1_part(io, no, oi, domega, s, 0);
+        im = im - ORC_PI * creal(s[io - 1]);
+        Pi0[(io - 1) + (size_t)(no - 1) * (iG + (size_t)nlf * jG)] = re + I * im;
+      }
+    }
+}
+
+/* a14 K2: Sloss genChi0, Sloss.f90:954-1059 (if / else-if chain: the
+ * overrides only act at jo == io).  S0(no,Nlf,Nlf) real, chi0(no-1,Nlf,Nlf). */
+static double orc_k2_re(int io, int no, double oi, double domega, const double *s) {
+  double re = 0.0, fact = 0.0, oj;
+  if (io == 1) {
+    for (int jo = 2; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      fact = domega / oj;
+      if (jo == 2)
+        fact = 3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / oj;
+      re = re + fact * s[jo - 1];
+    }
+    re = -2.0 * re;
+    return re;
+  }
+  for (int jo = 1; jo <= no; ++jo) {
+    oj = (double)(jo - 1) * domega;
+    if (jo != io) {
+      fact = domega / (oi - oj);
+    } else if (io == 2) {
+      if (jo == 1)
+        fact = 1.0;
+      else if (jo == 2)
+        fact = 0.0;
+      else if (jo == 3)
+        fact = -3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / (oi - oj);
+    } else if (io == no - 1) {
+      if (jo == 1)
+        fact = 0.5 * domega / (oi - oj);
+      else if (jo == no - 2)
+        fact = 3.0 / 2.0;
+      else if (jo == no - 1)
+        fact = 0.0;
+      else if (jo == no)
+        fact = -1.0;
+    } else {
+      if (jo == 1)
+        fact = 0.5 * domega / (oi - oj);
+      else if (jo == io - 1)
+        fact = 3.0 / 2.0;
+      else if (jo == io)
+        fact = 0.0;
+      else if (jo == io + 1)
+        fact = -3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / (oi - oj);
+    }
+    re = re + fact * s[jo - 1];
+    fact = domega / (oi + oj);
+    if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+    re = re - fact * s[jo - 1];
+  }
+  return re;
+}
+
+void orc_kk_sloss(int no, int nlf, double domega, const double *S0, orc_z *chi0) {
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int jG = 0; jG < nlf; ++jG)
+    for (int iG = 0; iG < nlf; ++iG) {
+      const double *s = S0 + (size_t)no * (iG + (size_t)nlf * jG);
+      for (int io = 1; io <= no - 1; ++io) {
+        double oi = (double)(io - 1) * domega;
+        double re = orc_k2_re(io, no, oi, domega, s);
+        double im = -ORC_PI * s[io - 1];
+        chi0[(io - 1) + (size_t)(no - 1) * (iG + (size_t)nlf * jG)] = re + I * im;
+      }
+    }
+}
+
+/* a20: genW1W2, Sloss.f90:1062-1171 (same if / else-if chain). */
+void orc_w1w2(int io, int no, int nlf, int iG, int jG, double domega, const double *S0,
+              double *W1, double *W2) {
+  const double *s = S0 + (size_t)no * ((iG - 1) + (size_t)nlf * (jG - 1));
+  double w1 = 0.0, w2 = 0.0, fact = 0.0, oj;
+  double oi = (double)(io - 1) * domega;
+  if (io == 1) {
+    for (int jo = 2; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      fact = domega / oj;
+      if (jo == 2)
+        fact = 3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / oj;
+      w1 = w1 - fact * s[jo - 1];
+    }
+    w2 = -w1;
+  } else {
+    for (int jo = 1; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      if (jo != io) {
+        fact = domega / (oi - oj);
+      } else if (io == 2) {
+        if (jo == 1)
+          fact = 1.0;
+        else if (jo == 2)
+          fact = 0.0;
+        else if (jo == 3)
+          fact = -3.0 / 2.0;
+        else if (jo == no)
+          fact = 0.5 * domega / (oi - oj);
+      } else if (io == no - 1) {
+        if (jo == 1)
+          fact = 0.5 * domega / (oi - oj);
+        else if (jo == no - 2)
+          fact = 3.0 / 2.0;
+        else if (jo == no - 1)
+          fact = 0.0;
+        else if (jo == no)
+          fact = -1.0;
+      } else {
+        if (jo == 1)
+          fact = 0.5 * domega / (oi - oj);
+        else if (jo == io - 1)
+          fact = 3.0 / 2.0;
+        else if (jo == io)
+          fact = 0.0;
+        else if (jo == io + 1)
+          fact = -3.0 / 2.0;
+        else if (jo == no)
+          fact = 0.5 * domega / (oi - oj);
+      }
+      w1 = w1 + fact * s[jo - 1];
+      fact = domega / (oi + oj);
+      if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+      w2 = w2 + fact * s[jo - 1];
+    }
+  }
+  *W1 = w1;
+  *W2 = w2;
+}
+
+/* K3: real-S0 variant of Modules/response_function_KK.f90:181-273.  S0m is
+ * the module's S0(-No:No,Nlf,Nlf) array (real).                             */
+void orc_kk_module_real(int io, int no, int nlf, int iG, int jG, double domega,
+                        const double *S0m, double *re_out, double *im_out) {
+  const int nw = 2 * no + 1;
+  const double *s = S0m + (size_t)nw * ((iG - 1) + (size_t)nlf * (jG - 1)) + no; /* index 0 */
+  double re = 0.0, fact = 0.0, oj;
+  double oi = (double)(io - 1) * domega;
+  if (io == 1) {
+    for (int jo = 2; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      fact = domega / oj;
+      if (jo == 2) fact = 3.0 / 2.0;
+      if (jo == no) fact = 0.5 * domega / oj;
+      re = re + fact * s[jo];
+    }
+    re = -2.0 * re;
+  } else if (io == 2) {
+    for (int jo = 1; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      if (jo != io)
+        fact = domega / (oi - oj);
+      else if (jo == 1)
+        fact = 1.0;
+      else if (jo == 2)
+        fact = 0.0;
+      else if (jo == 3)
+        fact = -3.0 / 2.0;
+      else if (jo == no)
+        fact = 0.5 * domega / (oi - oj);
+      re = re + fact * s[jo];
+      fact = domega / (oi + oj);
+      if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+      re = re - fact * s[jo];
+    }
+  } else if (io == no - 1) {
+    for (int jo = 1; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      if (jo != io) fact = domega / (oi - oj);
+      if (jo == 1) fact = 0.5 * domega / (oi - oj);
+      if (jo == no - 2) fact = 3.0 / 2.0;
+      if (jo == no - 1) fact = 0.0;
+      if (jo == no) fact = -1.0;
+      re = re + fact * s[jo];
+      fact = domega / (oi + oj);
+      if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+      re = re - fact * s[jo];
+    }
+  } else {
+    for (int jo = 1; jo <= no; ++jo) {
+      oj = (double)(jo - 1) * domega;
+      if (jo != io) fact = domega / (oi - oj);
+      if (jo == 1) fact = 0.5 * domega / (oi - oj);
+      if (jo == io - 1) fact = 3.0 / 2.0;
+      if (jo == io) fact = 0.0;
+      if (jo == io + 1) fact = -3.0 / 2.0;
+      if (jo == no) fact = 0.5 * domega / (oi - oj);
+      re = re + fact * s[jo];
+      fact = domega / (oi + oj);
+      if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+      re = re - fact * s[jo];
+    }
+  }
+  *re_out = re;
+  *im_out = -ORC_PI * s[io];
+}
+
+/* ------------------------------------------------------------------------ */
+/* a15: slab Coulomb V(G,G'), Sloss.f90:1173-1208.  qx, qy and GlfV are
+ * cartesian (a.u.); glf in 2pi/a0 units (compared exactly as the reference). */
+void orc_genV(double eps, double qx, double qy, double c0, int nlf, const int *parg,
+              const double *glf, const double *glfv, double *V) {
+  for (int iG = 0; iG < nlf; ++iG) {
+    double a = qx + glfv[3 * iG], b = qy + glfv[3 * iG + 1];
+    double gabs = sqrt(a * a + b * b);
+    if (gabs == 0.0) gabs = eps;
+    for (int jG = 0; jG < nlf; ++jG) {
+      double v = 0.0;
+      if (glf[3 * jG] == glf[3 * iG] && glf[3 * jG + 1] == glf[3 * iG + 1]) {
+        double gz_i = glfv[3 * iG + 2], gz_j = glfv[3 * jG + 2];
+        v = 4.0 * ORC_PI * (1.0 - exp(-gabs * c0)) / (gabs * c0);
+        v = v * (gabs * gabs - gz_i * gz_j);
+        v = v / (gabs * gabs + gz_i * gz_i);
+        v = v / (gabs * gabs + gz_j * gz_j);
+        v = -(double)parg[iG] * (double)parg[jG] * v;
+        if (glf[3 * jG + 2] == glf[3 * iG + 2]) v = 4.0 * ORC_PI / (gabs * gabs + gz_i * gz_i) + v;
+      }
+      V[iG + (size_t)nlf * jG] = v;
+    }
+  }
+}
+
+/* a17: Gauss-Jordan inversion with full pivoting, in place (A <- A^-1, B
+ * transformed alongside), Chi_RPA/gjel.f:1-91.  Column-major a(np,np),
+ * b(np,mp).  A singular pivot is reported by the reference and then divided
+ * through (gjel.f:60-63); the repeated-pivot case stops (gjel.f:40-43),
+ * here the routine returns early leaving A partially reduced.              */
+void orc_gjel(orc_z *a, int n, int np, orc_z *b, int m, int mp) {
+  (void)mp;
+  int *indxc = (int *)malloc(sizeof(int) * n), *indxr = (int *)malloc(sizeof(int) * n);
+  int *ipiv = (int *)calloc((size_t)n, sizeof(int));
+  int irow = 0, icol = 0;
+#define A(i, j) a[(i) + (size_t)np * (j)]
+#define B(i, j) b[(i) + (size_t)np * (j)]
+  for (int i = 0; i < n; ++i) {
+    double big = 0.0;
+    for (int j = 0; j < n; ++j) {
+      if (ipiv[j] != 1) {
+        for (int k = 0; k < n; ++k) {
+          if (ipiv[k] == 0) {
+            if (cabs(A(j, k)) >= big) {
+              big = cabs(A(j, k));
+              irow = j;
+              icol = k;
+            }
+          } else if (ipiv[k] > 1) {
+            goto done; /* 'singular matrix in gjel' + stop */
+          }
+        }
+      }
+    }
+    ipiv[icol] += 1;
+    if (irow != icol) {
+      for (int l = 0; l < n; ++l) {
+        orc_z d = A(irow, l);
+        A(irow, l) = A(icol, l);
+        A(icol, l) = d;
+      }
+      for (int l = 0; l < m; ++l) {
+        orc_z d = B(irow, l);
+        B(irow, l) = B(icol, l);
+        B(icol, l) = d;
+      }
+    }
+    indxr[i] = irow;
+    indxc[i] = icol;
+    orc_z pivinv = 1.0 / A(icol, icol);
+    A(icol, icol) = 1.0;
+    for (int l = 0; l < n; ++l) A(icol, l) = A(icol, l) * pivinv;
+    for (int l = 0; l < m; ++l) B(icol, l) = B(icol, l) * pivinv;
+    for (int ll = 0; ll < n; ++ll) {
+      if (ll != icol) {
+        orc_z d = A(ll, icol);
+        A(ll, icol) = 0.0;
+        for (int l = 0; l < n; ++l) A(ll, l) = A(ll, l) - A(icol, l) * d;
+        for (int l = 0; l < m; ++l) B(ll, l) = B(ll, l) - B(icol, l) * d;
+      }
+    }
+  }
+  for (int l = n - 1; l >= 0; --l) {
+    if (indxr[l] != indxc[l]) {
+      for (int k = 0; k < n; ++k) {
+        orc_z d = A(k, indxr[l]);
+        A(k, indxr[l]) = A(k, indxc[l]);
+        A(k, indxc[l]) = d;
+      }
+    }
+  }
+done:
+#undef A
+#undef B
+  free(indxc);
+  free(indxr);
+  free(ipiv);
+}
+
+/* a16-a19: eps = I - chi0 V (Sloss.f90:1210-1234), eps <- eps^-1 by gjel
+ * (Sloss.f90:462), chi = eps^-1 chi0 (1236-1253), W = V + V chi V
+ * (1255-1277).  All (Nlf,Nlf) column-major.                                 */
+void orc_dyson_w(int nlf, const orc_z *chi0, const double *V, orc_z *epsinv,
+                 orc_z *chi, orc_z *W) {
+  size_t nn = (size_t)nlf * nlf;
+  orc_z *e = (orc_z *)malloc(sizeof(orc_z) * nn);
+  orc_z *bm = (orc_z *)calloc(nn, sizeof(orc_z));
+  orc_z *c = (orc_z *)malloc(sizeof(orc_z) * nn);
+  for (int iG = 0; iG < nlf; ++iG)
+    for (int jG = 0; jG < nlf; ++jG) {
+      orc_z d = (iG == jG) ? 1.0 : 0.0;
+      for (int k = 0; k < nlf; ++k) d = d - chi0[iG + (size_t)nlf * k] * V[k + (size_t)nlf * jG];
+      e[iG + (size_t)nlf * jG] = d;
+    }
+  orc_gjel(e, nlf, nlf, bm, nlf, nlf);
+  for (int iG = 0; iG < nlf; ++iG)
+    for (int jG = 0; jG < nlf; ++jG) {
+      orc_z s = 0.0;
+      for (int k = 0; k < nlf; ++k) s = s + e[iG + (size_t)nlf * k] * chi0[k + (size_t)nlf * jG];
+      c[iG + (size_t)nlf * jG] = s;
+    }
+  if (W)
+    for (int iG = 0; iG < nlf; ++iG)
+      for (int jG = 0; jG < nlf; ++jG) {
+        orc_z s = 0.0;
+        for (int k1 = 0; k1 < nlf; ++k1)
+          for (int k2 = 0; k2 < nlf; ++k2)
+            s = s + V[iG + (size_t)nlf * k1] * c[k1 + (size_t)nlf * k2] * V[k2 + (size_t)nlf * jG];
+        W[iG + (size_t)nlf * jG] = V[iG + (size_t)nlf * jG] + s;
+      }
+  if (epsinv) memcpy(epsinv, e, sizeof(orc_z) * nn);
+  if (chi) memcpy(chi, c, sizeof(orc_z) * nn);
+  free(e);
+  free(bm);
+  free(c);
+}
+
+/* a13: Pi assembly, Optabs.f90:575-611.  Pi0(no-1,Nlf,Nlf) from orc_kk_optabs;
+ * pi11[io-1] = Pi_tot(1,1); sig_inter/sig_intra = Re(-i c0 Pi/oi) (io > 1,
+ * entry 0 left 0).  Pi_inter is read back after each (iG,jG) update exactly as
+ * the reference does (Optabs.f90:594).                                     */
+void orc_pi_assembly(int no, int nlf, double domega, double eta_intra, double c0,
+                     const orc_z *Pi0, const orc_z *Qeff, orc_z *pi11,
+                     double *sig_inter, double *sig_intra) {
+  size_t nn = (size_t)nlf * nlf;
+  orc_z *dia = (orc_z *)calloc(nn, sizeof(orc_z));
+  orc_z *tot = (orc_z *)calloc(nn, sizeof(orc_z));
+  for (int io = 1; io <= no - 1; ++io) {
+    double oi = (double)(io - 1) * domega;
+    orc_z pinter = 0.0, pintra = 0.0;
+    for (int iG = 0; iG < nlf; ++iG)
+      for (int jG = 0; jG < nlf; ++jG) {
+        orc_z p0 = Pi0[(io - 1) + (size_t)(no - 1) * (iG + (size_t)nlf * jG)];
+        size_t x = iG + (size_t)nlf * jG;
+        if (io == 1) dia[x] = -(creal(p0) + I * 0.0);
+        tot[x] = p0;
+        tot[x] = tot[x] + dia[x];
+        pinter = tot[0];
+        orc_z den = oi + I * eta_intra;
+        pintra = Qeff[0] * oi / den;
+        tot[x] = tot[x] + Qeff[x] * oi / den;
+      }
+    pi11[io - 1] = tot[0];
+    if (io > 1) {
+      sig_inter[io - 1] = creal(-I * c0 * pinter / oi);
+      sig_intra[io - 1] = creal(-I * c0 * pintra / oi);
+    } else {
+      sig_inter[0] = 0.0;
+      sig_intra[0] = 0.0;
+    }
+  }
+  free(dia);
+  free(tot);
+}

@@ -1,0 +1,121 @@
+/*
+ * CPU oracle for the RPA response path (FP64 restatement of the reference
+ * Fortran loops).  TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg use it as the checker.  The product library
+ * (librpa_hip.so) never links, loads or calls anything in this directory.
+ *
+ * Conventions (shared with include/rpa_hip.h):
+ *   - arrays use Fortran column-major layout with the reference's shapes,
+ *     e.g. E(NkI,Nband) is E[ik + NkI*n];
+ *   - index maps that cross this interface are 1-based (K1, K2, iG0, lf_idx);
+ *   - complex numbers are interleaved (re, im) doubles.
+ */
+#ifndef RPA_ORACLE_H
+#define RPA_ORACLE_H
+
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double _Complex orc_z;
+
+/* Vertex index-map semantics.
+ *   ORC_MAP_DIRECT : drivers (Sloss.f90:907-946, Optabs.f90:1399-1449): the
+ *                    first NGd coefficients, coefficient index == global G index.
+ *   ORC_MAP_IGK    : Modules/vertices.f90:88-189 (_v2): the whole k-sphere,
+ *                    coefficient i of k has G = G(igk_k(i,k)).               */
+enum { ORC_MAP_DIRECT = 0, ORC_MAP_IGK = 1 };
+
+typedef struct orc_problem {
+  int nki;            /* IBZ k-points (== Ntot, Nsym = 1)                     */
+  int nband;
+  int npwx;           /* stride of the coefficient arrays                      */
+  const int *ngk;     /* [nki] plane waves of each k                          */
+  const int *mill_k;  /* (3, npwx, nki) Miller indices of each coefficient     */
+  const double *evc;  /* (2, npwx, nband, nki) coefficients, complex128        */
+  int ngm;            /* density G vectors                                     */
+  const int *mill_g;  /* (3, ngm)                                              */
+  const double *kc;   /* KC(3,3): cartesian = KC * crystal                     */
+  const double *E;    /* E(nki, nband), Hartree, scissor already applied       */
+  int ntot;           /* FBZ k-points                                          */
+  const double *ktot; /* (3, ntot), 2pi/a0 units                               */
+  const int *K1;      /* [ntot] 1-based IBZ index of k                         */
+  const int *K2;      /* [ntot] 1-based IBZ index of k+q                       */
+  const int *iG0;     /* [ntot] 1-based global G index of G0                   */
+  double q[3];        /* 2pi/a0 units                                          */
+  int nlf;
+  const int *lf_idx;  /* [nlf] 1-based global G index of each local field      */
+  int map_mode;       /* ORC_MAP_*                                             */
+  int ngd;            /* NGd (direct mode)                                     */
+  double eps;         /* G-vector match tolerance                              */
+} orc_problem;
+
+/* ---- host setup restatements (a3-a5) ---------------------------------- */
+int  orc_gen_glf(const char *lf, double ecut, double gcar, int ng, const double *g,
+                 const int *mill_g, int nlfd, int *nlf, double *glf, int *parg,
+                 int *idx);
+void orc_occupation(double ei, double ef, double T, double *expo, double *f);
+int  orc_gen_fbz(int nki, int nsym, double eps, const double *ki, const double *R,
+                 int *ntot, double *ktot);
+void orc_find_min_q(int ntot, const double *ktot, int iq, double q[3]);
+int  orc_find_kq(const double kq[3], double eps, int nki, int ntot, int ng,
+                 const double *ktot, const double *ki, const double *g,
+                 int *iG0, int *K2);
+void orc_gcart(int n, const int *mill, const double *kc, double *g);
+int  orc_gk_sort(const double k[3], int ngm, const double *g, double gcutw,
+                 int npwx, int *ngk, int *igk);
+
+/* ---- a6: vertex index map --------------------------------------------- */
+void orc_gfast_linear(int n1, const double *l1, int n2, const double *l2, int nlf,
+                      const double *glf, const double g0[3], double eps, int *gfast);
+int  orc_gfast_miller(int n1, const int *m1, int n2, const int *m2, int nlf,
+                      const int *mlf, const int m0[3], int *gfast);
+
+/* ---- a7/a8: vertices for every transition ------------------------------ */
+/* kind: 0 charge (Sloss), 1..3 current along x/y/z (Optabs), 4 = yz pair.
+ * Transition order: k (FBZ), then n, then m over the band windows
+ * [n_lo,n_hi) x [m_lo,m_hi).  rho1/rho2 are (nlf, ntrans) complex.          */
+int  orc_vertices(const orc_problem *p, int kind, double gcar, int n_lo, int n_hi,
+                  int m_lo, int m_hi, orc_z *rho1, orc_z *rho2);
+
+/* ---- a9/a10: spectral-function accumulation ----------------------------- */
+typedef struct orc_sloss_params {
+  int nocc, no;
+  double domega, eta, vcell;
+} orc_sloss_params;
+int  orc_sloss_s0(const orc_problem *p, const orc_sloss_params *sp, double *S0,
+                  int nthreads);
+
+typedef struct orc_optabs_params {
+  int nval, no, pol;   /* pol: 1 x, 2 y, 3 z, 4 yz */
+  double domega, eta_inter, lor_cut, df_cut, efermi, T, vcell, gcar;
+} orc_optabs_params;
+int  orc_optabs_s0(const orc_problem *p, const orc_optabs_params *op, orc_z *S0,
+                   orc_z *Qeff, int nthreads);
+
+/* ---- a12/a14/a20: Kramers-Kronig stencils -------------------------------- */
+void orc_kk_optabs(int no, int nlf, double domega, const orc_z *S0, orc_z *Pi0);
+void orc_kk_sloss(int no, int nlf, double domega, const double *S0, orc_z *chi0);
+void orc_w1w2(int io, int no, int nlf, int iG, int jG, double domega,
+              const double *S0, double *W1, double *W2);
+void orc_kk_module_real(int io, int no, int nlf, int iG, int jG, double domega,
+                        const double *S0m, double *re, double *im);
+
+/* ---- a15-a19: Coulomb, Dyson, W ---------------------------------------- */
+void orc_genV(double eps, double qx, double qy, double c0, int nlf, const int *parg,
+              const double *glf, const double *glfv, double *V);
+void orc_gjel(orc_z *a, int n, int np, orc_z *b, int m, int mp);
+void orc_dyson_w(int nlf, const orc_z *chi0, const double *V, orc_z *epsinv,
+                 orc_z *chi, orc_z *W);
+
+/* ---- a13: Pi assembly (Optabs calc=2) ----------------------------------- */
+void orc_pi_assembly(int no, int nlf, double domega, double eta_intra, double c0,
+                     const orc_z *Pi0, const orc_z *Qeff, orc_z *pi11,
+                     double *sig_inter, double *sig_intra);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
