@@ -26,8 +26,9 @@ EXPORTS = [
     "rpa_set_params", "rpa_set_q", "rpa_vertices", "rpa_accumulate_s0", "rpa_kk",
     "rpa_coulomb", "rpa_dyson_w", "rpa_epilogue", "rpa_run", "rpa_run_graph",
     "rpa_get_vertices", "rpa_gen_fbz", "rpa_find_min_q", "rpa_find_kq", "rpa_gen_glf",
-    "rpa_gk_sort", "rpa_gcart",
+    "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms",
 ]
+STAGES = ["sort", "vertices", "accumulate", "reduce", "kk", "dyson_or_pi"]
 
 
 class RpaDims(C.Structure):
@@ -88,6 +89,8 @@ def lib() -> C.CDLL:
                                   C.POINTER(C.c_int), P, P, P]
         L.rpa_gk_sort.argtypes = [P, C.c_int, P, C.c_double, C.c_int, C.POINTER(C.c_int), P]
         L.rpa_gcart.argtypes = [C.c_int, P, P, P]
+        L.rpa_set_profiling.argtypes = [P, C.c_int]
+        L.rpa_stage_ms.argtypes = [P, P, C.c_int]
         _lib = L
     return _lib
 
@@ -267,6 +270,14 @@ class Context:
     def run(self, graph=False):
         self._c((lib().rpa_run_graph if graph else lib().rpa_run)(self.h))
         return lib().rpa_last_run_ms(self.h)
+
+    def set_profiling(self, on=True):
+        self._c(lib().rpa_set_profiling(self.h, 1 if on else 0))
+
+    def stage_ms(self):
+        ms = np.zeros(6)
+        self._c(lib().rpa_stage_ms(self.h, _ptr(ms), 6))
+        return dict(zip(STAGES, ms.tolist()))
 
     def get_vertices(self, both=False):
         n = C.c_longlong()

@@ -495,6 +495,7 @@ int launch_accumulate(rpa_ctx *c) {
     hipLaunchKernelGGL(k_accum<true>, dim3(grid), dim3(256), lds, c->stream, a);
   else
     hipLaunchKernelGGL(k_accum<false>, dim3(grid), dim3(256), lds, c->stream, a);
+  if (c->profiling) RPA_HIP(c, hipEventRecord(c->prof_ev[3], c->stream));
   const long long n = (long long)c->nw * c->ncol_pad;
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                      (const double *)c->d_part, (const int *)c->d_first_item, a.npb, c->nw,

@@ -155,16 +155,35 @@ static int prepare(rpa_ctx *c) {
 }
 
 // The per-q chain after prepare(): bookkeeping, vertices, S0, KK, Dyson/W or Pi.
+// With profiling on, events mark the stage boundaries (prof_ev[3] is
+// recorded inside launch_accumulate between the matrix-core kernel and the
+// reduction).
 static int chain(rpa_ctx *c) {
+  const bool pr = c->profiling;
+  if (pr) RPA_HIP(c, hipEventRecord(c->prof_ev[0], c->stream));
   if (int e = launch_transitions(c)) return e;
+  if (pr) RPA_HIP(c, hipEventRecord(c->prof_ev[1], c->stream));
   if (int e = launch_vertices(c, c->d_rho1, c->d_rho2, c->d_perm)) return e;
+  if (pr) RPA_HIP(c, hipEventRecord(c->prof_ev[2], c->stream));
   if (int e = launch_accumulate(c)) return e;
+  if (pr) RPA_HIP(c, hipEventRecord(c->prof_ev[4], c->stream));
   if (int e = launch_kk(c)) return e;
+  if (pr) RPA_HIP(c, hipEventRecord(c->prof_ev[5], c->stream));
   if (c->d.program == RPA_SLOSS) {
     if (int e = launch_coulomb(c)) return e;
     if (int e = launch_dyson(c)) return e;
   } else {
     if (int e = launch_epilogue(c)) return e;
+  }
+  if (pr) RPA_HIP(c, hipEventRecord(c->prof_ev[6], c->stream));
+  return 0;
+}
+
+static int collect_stage_times(rpa_ctx *c) {
+  for (int s = 0; s < 6; ++s) {
+    float ms = 0.f;
+    RPA_HIP(c, hipEventElapsedTime(&ms, c->prof_ev[s], c->prof_ev[s + 1]));
+    c->stage_ms[s] = ms;
   }
   return 0;
 }
@@ -252,6 +271,8 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_status};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  for (hipEvent_t ev : c->prof_ev)
+    if (ev) (void)hipEventDestroy(ev);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -568,6 +589,24 @@ int rpa_run(rpa_ctx *c) {
   float ms = 0.f;
   RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
+  if (c->profiling)
+    if (int e = collect_stage_times(c)) return e;
+  return RPA_OK;
+}
+
+int rpa_set_profiling(rpa_ctx *c, int on) {
+  if (!c) return RPA_E_ARG;
+  (void)hipSetDevice(c->d.device);
+  if (on && !c->prof_ev[0])
+    for (int i = 0; i < 8; ++i) RPA_HIP(c, hipEventCreate(&c->prof_ev[i]));
+  c->profiling = on != 0;
+  invalidate_graph(c);  // a captured graph would hold the event nodes
+  return RPA_OK;
+}
+
+int rpa_stage_ms(const rpa_ctx *c, double *ms, int n) {
+  if (!c || !ms || n < 0) return RPA_E_ARG;
+  for (int i = 0; i < n && i < 6; ++i) ms[i] = c->stage_ms[i];
   return RPA_OK;
 }
 

@@ -169,6 +169,10 @@ struct rpa_ctx {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
   bool prepared = false;
+  // stage profiling (rpa_set_profiling)
+  bool profiling = false;
+  hipEvent_t prof_ev[8] = {nullptr};
+  double stage_ms[6] = {0};
   // capacities (elements) of the buffers sized by prepare()
   size_t cap_key = 0, cap_bcnt = 0, cap_toff = 0, cap_items = 0, cap_first = 0, cap_nitems = 0;
   size_t cap_te = 0, cap_s0 = 0, cap_kmat = 0, cap_x = 0, cap_chi0 = 0, cap_epsinv = 0;

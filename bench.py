@@ -1,0 +1,237 @@
+#!/usr/bin/env python3
+"""Throughput of the RPA response chain chi(q, omega) on one MI355X.
+
+One step = one pass of the hot path for one q over all frequencies of the
+workload: transition bookkeeping, vertices, S0 accumulation, Kramers-Kronig,
+Coulomb, Dyson/W and the loss epilogue (Sloss semantics), inputs resident in
+HBM.  value = (no-1) output frequencies per step / seconds per step.
+
+Default workload (production size, FP64): 64x64x1 k-grid (4096 k), 20 bands
+(8 valence / 12 conduction), ~1500 plane waves per k, 31 local fields,
+701 frequencies over 0-30 eV, eta = 50 meV, q = one k-grid step.
+
+Prints one JSON line (the last line of stdout).
+"""
+
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ω-pts/s for χ(q,ω), mixed vs FP64 GPU at ε⁻¹ rel err ≤1e-6; % HBM/MFMA peak"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def nnz_weights(E, K1, K2, nv, nb, no, dw, eta, program="sloss"):
+    """Number of (transition, frequency) pairs with nonzero Lorentzian weight
+    under the reference cutoff |Lor| >= 1e-3/eta (Sloss.f90:411)."""
+    o = np.arange(no) * dw
+    cut = 1.0e-3 / eta
+    total = 0
+    nk = len(K1)
+    for k0 in range(0, nk, 256):
+        ks = np.arange(k0, min(nk, k0 + 256))
+        e1 = E[K1[ks] - 1][:, :nv]            # (k, nv)
+        e2 = E[K2[ks] - 1][:, nv:nb]          # (k, nc)
+        d = (o[None, None, None, :] + e1[:, :, None, None]) - e2[:, None, :, None]
+        lor = -eta / (d * d + eta * eta)
+        total += int(np.count_nonzero(np.abs(lor) >= cut))
+    return total
+
+
+def cpu_baseline(sys_, r, threads, target_s=12.0):
+    """The oracle (FP64 restatement of the reference loops) on a k-subset,
+    scaled to the full k-grid; KK and Dyson/W at full size."""
+    from oracle import oracle as orc
+    cfg = sys_.cfg
+    eta = cfg.eta_ev / 27.2113846
+
+    def problem(nk):
+        return orc.Problem(sys_, r.K1[:nk], r.K2[:nk], r.iG0[:nk], r.ktot[:nk], r.q, r.lf_idx,
+                           map_mode=0, ngd=r.ngd, E_hartree=r.E)
+
+    nk_full = len(r.K1)
+    nk = min(nk_full, 64)
+    t0 = time.perf_counter()
+    orc.sloss_s0(problem(nk), cfg.nval, cfg.no, r.domega, eta, sys_.vcell, nthreads=threads)
+    t_probe = time.perf_counter() - t0
+    per_k = t_probe / nk
+    nk_s = int(min(nk_full, max(nk, target_s / max(per_k, 1e-9))))
+    nk_s = max(8, (nk_s // 8) * 8)
+    pb = problem(nk_s)
+    t0 = time.perf_counter()
+    S0 = orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, eta, sys_.vcell, nthreads=threads)
+    t_s0 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    chi0 = orc.kk_sloss(S0, r.domega)
+    V = orc.genV(r.rc.eps, r.q[0] * sys_.gcar, r.q[1] * sys_.gcar, cfg.c, r.parg, r.glf, sys_.gcar)
+    outs = [orc.dyson_w(chi0[f], V) for f in range(cfg.no - 1)]
+    t_tail = time.perf_counter() - t0
+    t_full = t_s0 * nk_full / nk_s + t_tail
+    return {
+        "t_full_s": t_full, "nk_sample": nk_s, "t_s0_sample": t_s0, "t_tail": t_tail,
+        "S0": S0, "chi0": chi0, "epsinv": np.array([o[0] for o in outs]),
+        "W": np.array([o[2] for o in outs]),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cfg1")
+    ap.add_argument("--program", default="sloss", choices=["sloss", "optabs"])
+    ap.add_argument("--vertex-map", default="direct", choices=["direct", "igk"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy check")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path "
+              "(q points would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    synth = importlib.import_module("2dmaterialsopticalprops_amd.synth")
+    host = importlib.import_module("2dmaterialsopticalprops_amd.host")
+    cfg = synth.CONFIGS[args.config]
+    t0 = time.perf_counter()
+    s = synth.make_system(cfg)
+    log(f"[bench] synthetic {args.config}: nk={len(s.ngk)} nband={cfg.nband} ngk={s.ngk.min()}-{s.ngk.max()} "
+        f"({time.perf_counter() - t0:.1f}s)")
+    t0 = time.perf_counter()
+    r = host.Runner(s, host.RunConfig(program=args.program, vertex_map=args.vertex_map))
+    log(f"[bench] setup+upload {time.perf_counter() - t0:.1f}s nlf={r.nlf} no={cfg.no} ntrans={r.ntransitions()}")
+    ctx = r.ctx
+    for _ in range(max(args.warmup, 1)):
+        ctx.run()
+    # timed region: K eager runs with stage events on the context's stream
+    ctx.set_profiling(True)
+    stages = {k: 0.0 for k in host._abi.STAGES}
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev_ms += ctx.run()
+        for k, v in ctx.stage_ms().items():
+            stages[k] += v
+    wall = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    stages = {k: v / args.steps for k, v in stages.items()}
+    ms_step = wall * 1e3 / args.steps
+    # graph replay of the same chain
+    ctx.run(graph=True)
+    tg = time.perf_counter()
+    gms = 0.0
+    for _ in range(args.steps):
+        gms += ctx.run(graph=True)
+    gwall = (time.perf_counter() - tg) * 1e3 / args.steps
+    nfreq = cfg.no - 1
+    value = nfreq / (ms_step * 1e-3)
+    log(f"[bench] {ms_step:.3f} ms/step (device {dev_ms / args.steps:.3f} ms), graph {gwall:.3f} ms/step; "
+        f"stages {json.dumps({k: round(v, 4) for k, v in stages.items()})}")
+
+    # algorithmic work of the two matrix-core kernels
+    nv, nb = cfg.nval, cfg.nband
+    eta = cfg.eta_ev / host.HARTREE
+    nnz = nnz_weights(r.E, r.K1, r.K2, nv, nb, cfg.no, r.domega, eta)
+    ntr = len(r.K1) * nv * (nb - nv)
+    flops_acc = 2.0 * r.nlf * r.nlf * nnz
+    flops_vtx = 8.0 * r.nlf * r.ngd * ntr
+    acc_ms, vtx_ms = stages["accumulate"], stages["vertices"]
+    if acc_ms >= vtx_ms:
+        dom, fl, ms = "k_accum (S0 accumulation)", flops_acc, acc_ms
+    else:
+        dom, fl, ms = "k_vertex_f64 (vertices)", flops_vtx, vtx_ms
+    achieved = fl / (ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
+        "kernel_ms": round(ms, 4), "algorithmic_flop": fl,
+        "other": {
+            "k_vertex_f64": {"ms": round(vtx_ms, 4), "flop": flops_vtx,
+                             "tflops": round(flops_vtx / (vtx_ms * 1e-3) / 1e12, 3) if vtx_ms > 0 else None},
+            "k_accum": {"ms": round(acc_ms, 4), "flop": flops_acc, "nnz_weights": nnz,
+                        "tflops": round(flops_acc / (acc_ms * 1e-3) / 1e12, 3) if acc_ms > 0 else None},
+        },
+    }
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "omega-points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: {program_name(args.program)} chain, FP64, {len(s.ngk)} k "
+                        f"({cfg.nk}x{cfg.nk}x1), {nb} bands ({nv}v/{nb - nv}c), ngk {s.ngk.min()}-{s.ngk.max()}, "
+                        f"N_G={r.nlf}, {cfg.no} omega 0-{cfg.omax_ev:g} eV, eta={cfg.eta_ev * 1e3:g} meV, "
+                        f"q=k-grid step, vertex map {args.vertex_map}",
+            "omega_points_per_step": nfreq, "transitions": ntr, "graph_ms_per_step": round(gwall, 4),
+            "device_ms_per_step": round(dev_ms / args.steps, 4),
+            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        },
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        try:
+            cb = cpu_baseline(s, r, threads, args.cpu_seconds)
+            out["cpu_baseline"] = {
+                "value": round(nfreq / cb["t_full_s"], 2), "unit": "omega-points/s", "cores": threads,
+                "kind": "port",
+                "sample": f"oracle S0 on {cb['nk_sample']} of {len(r.K1)} k-points "
+                          f"({cb['t_s0_sample']:.1f}s, scaled x{len(r.K1) / cb['nk_sample']:.2f}) "
+                          f"+ full KK/Dyson/W ({cb['t_tail']:.1f}s)",
+            }
+            out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+            # accuracy on the same k-subset: GPU chain vs oracle
+            acc = accuracy(s, r, cb)
+            out["accuracy_vs_fp64_oracle"] = acc
+        except Exception as e:  # the baseline is reported, never silently replaced
+            out["cpu_baseline"] = {"error": repr(e)}
+    r.close()
+    print(json.dumps(out))
+
+
+def program_name(p):
+    return "Sloss (chi0 -> eps^-1 -> chi -> W -> loss)" if p == "sloss" else "Optabs (Pi0 -> Pi, sigma)"
+
+
+def accuracy(s, r, cb):
+    nk = cb["nk_sample"]
+    ctx = r.ctx
+    ctx.set_q(r.K1[:nk], r.K2[:nk], r.iG0[:nk], r.ktot[:nk], r.q)
+    ctx.vertices()
+    S0, _ = ctx.accumulate_s0()
+    chi0 = ctx.kk()
+    ctx.coulomb()
+    epsinv, chi, W = ctx.dyson_w()
+
+    def fro(a, b):
+        return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+    loss_g = -epsinv[:, 0, 0].imag
+    loss_o = -cb["epsinv"][:, 0, 0].imag
+    return {
+        "sample": f"{nk} of {len(r.K1)} k-points, full bands/plane waves/N_G/omega",
+        "S0_rel_fro": fro(S0, cb["S0"]), "chi0_rel_fro": fro(chi0, cb["chi0"]),
+        "epsinv_rel_fro": fro(epsinv, cb["epsinv"]), "W_rel_fro": fro(W, cb["W"]),
+        "loss_peak_shift_bins": int(np.argmax(loss_g) - np.argmax(loss_o)),
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -167,6 +167,15 @@ int rpa_run_graph(rpa_ctx *ctx);
  * complex; ntrans_out receives the count.  rho2 (pol yz) may be NULL.      */
 int rpa_get_vertices(rpa_ctx *ctx, double *rho1, double *rho2, long long *ntrans_out);
 
+/* Diagnostics with no counterpart in the reference (which has no timers):
+ * with profiling on, rpa_run records HIP events between its launches on the
+ * context's stream; rpa_stage_ms then gives the device milliseconds of the
+ * last run per stage: [0] transition bookkeeping and sort, [1] vertices,
+ * [2] S0 accumulation (matrix-core kernel), [3] chunk reduction and Qeff,
+ * [4] Kramers-Kronig, [5] Coulomb + Dyson/W (Sloss) or Pi assembly (Optabs). */
+int rpa_set_profiling(rpa_ctx *ctx, int on);
+int rpa_stage_ms(const rpa_ctx *ctx, double *ms, int n);
+
 /* ---- host setup helpers (the reference's setup subroutines) ------------ */
 /* genFBZ (Optabs.f90:697-759): R(48,3,3); ktot must hold 3*48*nki doubles.   */
 int rpa_gen_fbz(int nki, int nsym, double eps, const double *ki, const double *R,
