@@ -7,8 +7,11 @@
 //           (Optabs.f90:1354-1472 genStrujniVrhovi; vertices.f90:293-394)
 //
 // map(G,i) is the reference's Gfast: the position of G(i) + Glf(G) + G0 in the
-// k+q coefficient list (Sloss.f90:907-946), found here with exact Miller
-// arithmetic in a dense Miller box instead of a linear search (R = RI = I).
+// k+q coefficient list (Sloss.f90:907-946), found with exact Miller arithmetic
+// in a dense Miller box instead of a linear search (R = RI = I).  In the direct
+// map the target is a global G index that depends on k only through G0, so
+// k_direct_map tabulates it once per distinct G0 of the q; each k then only
+// checks it against ngk(K2).
 //
 // For one (k, G) the sums over all band pairs form a complex product
 // [conj(C1)]_(n x i) [C2 o map]_(i x m).  It runs on the FP64 matrix cores as
@@ -16,13 +19,20 @@
 // rows 8..15 give Im -- the K index is (i, re/im), columns are 16 bands m:
 //   A[n][(i,0)] =  C1r   A[n][(i,1)] = C1i   A[8+n][(i,0)] = -C1i  A[8+n][(i,1)] = C1r
 //   B[(i,0)][m] =  C2r(map(i))*J           B[(i,1)][m] = C2i(map(i))*J
-// One wave owns one G; a workgroup owns 4 G of one k.  Workgroups of one k are
-// placed on one XCD so the k+q coefficient rows they gather stay in its L2.
+// A workgroup owns 4 G of one k (one wave each).  It walks the plane waves in
+// chunks: the C1 rows and the gathered C2 rows of chunk c+1 are loaded into
+// registers while chunk c runs on the matrix cores from LDS.  Workgroups of
+// one k are placed on one XCD so the k+q rows they gather stay in its L2.
+#include <algorithm>
+#include <cstdlib>
+
 #include "rpa_internal.h"
 
 namespace rpa {
 
 constexpr int kVertexMaxBlocks = 8;
+constexpr int kPfC1 = 4;   // prefetched C1 elements per thread
+constexpr int kPfB = 8;    // prefetched gathered elements per thread
 
 struct VBlock {
   int n0, nrows;  // 8 bands n per block
@@ -36,6 +46,8 @@ struct VertexArgs {
   const int *mill_g;
   const int16_t *kbox;
   const int *gbox;
+  const int *dmap;   // direct map [ng0][nlf][ngd] global G index or -1
+  const int *g0idx;  // per k: index of its G0 in the direct map
   Box kb, gb;
   const int *ngk;
   const int *k1, *k2, *m0;
@@ -51,16 +63,49 @@ struct VertexArgs {
   int nvb;
   VBlock vb[kVertexMaxBlocks];
   int groups;      // G groups of 4 per k
+  int ch;          // plane waves per LDS chunk
+  int na0, nna;    // staged C1 bands [na0, na0+nna)
+  int mb0, nmb;    // staged C2 bands [mb0, mb0+nmb)
   int kslots;      // ntot rounded up to a multiple of 8
   const int *perm;
   double2 *rho1, *rho2;
 };
 
+// Target global index of G(i) + Glf(g) + G0 for every distinct G0 of the q.
+__global__ void k_direct_map(const int *mill_g, const int *gbox, Box gb, const int *lf_key,
+                             const int *g0keys, int ng0, int nlf, int ngd, int *dmap) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)ng0 * nlf * ngd;
+  if (idx >= n) return;
+  const int i = (int)(idx % ngd);
+  const int g = (int)((idx / ngd) % nlf);
+  const int z = (int)(idx / ((long long)ngd * nlf));
+  int s0, s1, s2, l0, l1, l2, o0, o1, o2;
+  mill_unpack(mill_g[i], s0, s1, s2);
+  mill_unpack(lf_key[g], l0, l1, l2);
+  mill_unpack(g0keys[z], o0, o1, o2);
+  const int bi = gb.index(s0 + l0 + o0, s1 + l1 + o1, s2 + l2 + o2);
+  dmap[idx] = bi >= 0 ? gbox[bi] : -1;
+}
+
 // KIND: 0 charge, 1 current (one polarisation), 2 current (pol yz: two)
-template <int KIND>
+// MAXB: band blocks compiled in; DUAL: two accumulator chains (even/odd steps)
+// CH: plane waves per chunk.  Thread t stages the gathered row (w, c) =
+// divmod(t / TPR, CH) for its G and columns m = t % TPR + TPR u, and the C1
+// row c = t / TPA, bands n = t % TPA + TPA u; the assignment is fixed for the
+// whole walk, so a chunk costs one map lookup per thread and no division.
+template <int KIND, int MAXB, bool DUAL, int CH>
 __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  constexpr int TPR = 256 / (4 * CH);  // threads per gathered row
+  constexpr int TPA = 256 / CH;        // threads per C1 row
+  extern __shared__ double2 vsm[];
+  const int nna = a.nna, nmb = a.nmb;
+  double2 *sC1 = vsm;                           // [CH][nna]
+  double2 *sB = sC1 + CH * nna;                 // [4][CH][nmb]
+  double *sJ1 = (double *)(sB + 4 * CH * nmb);  // [4][CH]
+  double *sJ2 = sJ1 + 4 * CH;                   // [4][CH]
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware placement: linear id L runs on XCD L % 8; all G groups of one
   // k get ids with the same residue.
   const int L = blockIdx.x;
@@ -68,106 +113,181 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
   const int slot = L >> 3;
   const int k = xcd + 8 * (slot / a.groups);
   const int grp = slot % a.groups;
-  if (k >= a.ntot) return;
+  if (k >= a.ntot) return;  // whole workgroup leaves together
   const int iG = grp * 4 + wave;
-  if (iG >= a.nlf) return;  // whole wave leaves; no block barrier below
+  const bool active = iG < a.nlf;
 
   const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
-  int d0, d1, d2, l0, l1, l2;
+  int d0, d1, d2;
   mill_unpack(a.m0[k], d0, d1, d2);
-  mill_unpack(a.lf_key[iG], l0, l1, l2);
-  d0 += l0;
-  d1 += l1;
-  d2 += l2;
 
   const bool direct = (a.map_mode == RPA_MAP_DIRECT);
   const int n1 = direct ? a.ngd : a.ngk[K1];
   const int lim2 = a.ngk[K2];
   const int *src = direct ? a.mill_g : a.mill_k + (size_t)K1 * a.npwx;
   const int16_t *box2 = a.kbox + (size_t)K2 * a.kb.size();
-  const double2 *c1 = a.evc + (size_t)K1 * a.npwx * a.nband;
-  const double2 *c2 = a.evc + (size_t)K2 * a.npwx * a.nband;
+  const double2 *c1 = a.evc + (size_t)K1 * a.npwx * a.nband + a.na0;
+  const double2 *c2 = a.evc + (size_t)K2 * a.npwx * a.nband + a.mb0;
+  const double *kv = a.ktot + 3 * k;
+
+  // this thread's staging assignment
+  const int brow = tid / TPR, bsub = tid % TPR;  // gathered row (w, c)
+  const int bw = brow / CH, bc = brow % CH;
+  const int bg = grp * 4 + bw;
+  const bool bg_ok = bg < a.nlf;
+  const int *dmap = direct ? a.dmap + ((size_t)a.g0idx[k] * a.nlf + (bg_ok ? bg : 0)) * a.ngd : nullptr;
+  int lf0 = 0, lf1 = 0, lf2 = 0;
+  if (!direct && bg_ok) mill_unpack(a.lf_key[bg], lf0, lf1, lf2);
+  const int arow = tid / TPA, asub = tid % TPA;  // C1 row c, first band
 
   const int r = lane & 15, q4 = lane >> 4;
   const int ii = q4 >> 1, part = q4 & 1;
 
-  double jbase1 = 0.0, jbase2 = 0.0;
-  if (KIND >= 1) {
-    const double *kv = a.ktot + 3 * k;
-    jbase1 = (a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * iG + a.pol1];
-    jbase2 = (a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * iG + a.pol2];
-  }
+  double2 pc1[kPfC1], pb[kPfB];
+  double pj1 = 0.0, pj2 = 0.0;
 
-  rpa_d4 acc1[kVertexMaxBlocks], acc2[kVertexMaxBlocks];
+  auto prefetch = [&](int i0) {
+    {
+      const int i = i0 + arow;
+      const double2 *row = c1 + (size_t)i * a.nband;
 #pragma unroll
-  for (int b = 0; b < kVertexMaxBlocks; ++b) {
-    acc1[b] = rpa_d4{0.0, 0.0, 0.0, 0.0};
-    acc2[b] = rpa_d4{0.0, 0.0, 0.0, 0.0};
-  }
-
-  for (int i0 = 0; i0 < n1; i0 += 2) {
-    const int i = i0 + ii;
+      for (int u = 0; u < kPfC1; ++u) {
+        const int n = asub + TPA * u;
+        pc1[u] = (i < n1 && n < nna) ? row[n] : make_double2(0.0, 0.0);
+      }
+    }
+    const int i = i0 + bc;
     int j = -1;
-    double w1 = 1.0, w2 = 1.0;
-    if (i < n1) {
-      int s0, s1, s2;
-      mill_unpack(src[i], s0, s1, s2);
+    if (bg_ok && i < n1) {
       if (direct) {
-        int bi = a.gb.index(s0 + d0, s1 + d1, s2 + d2);
-        int pos = bi >= 0 ? a.gbox[bi] : -1;
-        j = (pos >= 0 && pos < lim2) ? pos : -1;
+        const int p = dmap[i];
+        j = (p >= 0 && p < lim2) ? p : -1;
       } else {
-        int bi = a.kb.index(s0 + d0, s1 + d1, s2 + d2);
+        int s0, s1, s2;
+        mill_unpack(src[i], s0, s1, s2);
+        const int bi = a.kb.index(s0 + lf0 + d0, s1 + lf1 + d1, s2 + lf2 + d2);
         j = bi >= 0 ? (int)box2[bi] : -1;
       }
-      if (KIND >= 1) {
-        // G(i) cartesian = KC * mill (left to right), then J * 0.5
-        double g1 = ((0.0 + a.kc[a.pol1 + 0] * (double)s0) + a.kc[a.pol1 + 3] * (double)s1) +
-                    a.kc[a.pol1 + 6] * (double)s2;
-        w1 = 0.5 * ((jbase1 + 2.0 * g1) * a.gcar);
+    }
+    const double2 *row = c2 + (size_t)(j >= 0 ? j : 0) * a.nband;
+#pragma unroll
+    for (int u = 0; u < kPfB; ++u) {
+      const int m = bsub + TPR * u;
+      pb[u] = (j >= 0 && m < nmb) ? row[m] : make_double2(0.0, 0.0);
+    }
+    if (KIND >= 1 && bsub == 0) {
+      pj1 = pj2 = 0.0;
+      if (bg_ok && i < n1) {
+        int s0, s1, s2;
+        mill_unpack(src[i], s0, s1, s2);
+        // J = Gcar (q + 2k + Glf(G) + 2 G(i))_pol with G(i) = KC mill (left to right)
+        const double jb1 = (a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * bg + a.pol1];
+        const double g1 = ((0.0 + a.kc[a.pol1 + 0] * (double)s0) + a.kc[a.pol1 + 3] * (double)s1) +
+                          a.kc[a.pol1 + 6] * (double)s2;
+        pj1 = 0.5 * ((jb1 + 2.0 * g1) * a.gcar);
         if (KIND == 2) {
-          double g2 = ((0.0 + a.kc[a.pol2 + 0] * (double)s0) + a.kc[a.pol2 + 3] * (double)s1) +
-                      a.kc[a.pol2 + 6] * (double)s2;
-          w2 = 0.5 * ((jbase2 + 2.0 * g2) * a.gcar);
+          const double jb2 = (a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * bg + a.pol2];
+          const double g2 = ((0.0 + a.kc[a.pol2 + 0] * (double)s0) + a.kc[a.pol2 + 3] * (double)s1) +
+                            a.kc[a.pol2 + 6] * (double)s2;
+          pj2 = 0.5 * ((jb2 + 2.0 * g2) * a.gcar);
         }
       }
     }
-    const double *c1row = (const double *)(c1 + (size_t)(i < n1 ? i : 0) * a.nband);
-    const double *c2row = (const double *)(c2 + (size_t)(j >= 0 ? j : 0) * a.nband);
+  };
+
+  auto stage = [&]() {
 #pragma unroll
-    for (int b = 0; b < kVertexMaxBlocks; ++b) {
+    for (int u = 0; u < kPfC1; ++u) {
+      const int n = asub + TPA * u;
+      if (n < nna) sC1[arow * nna + n] = pc1[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kPfB; ++u) {
+      const int m = bsub + TPR * u;
+      if (m < nmb) sB[brow * nmb + m] = pb[u];
+    }
+    if (KIND >= 1 && bsub == 0) {
+      sJ1[brow] = pj1;
+      sJ2[brow] = pj2;
+    }
+  };
+
+  rpa_d4 acc1[MAXB], acc2[MAXB], bcc1[MAXB], bcc2[MAXB];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    acc1[b] = acc2[b] = bcc1[b] = bcc2[b] = rpa_d4{0.0, 0.0, 0.0, 0.0};
+  }
+
+  // one K step (two plane waves) of every band block into accumulators x1/x2
+  auto step = [&](int c, rpa_d4 *x1, rpa_d4 *x2) {
+    const double wj1 = (KIND >= 1) ? sJ1[wave * CH + c] : 1.0;
+    const double wj2 = (KIND == 2) ? sJ2[wave * CH + c] : 1.0;
+    const double2 *ar = sC1 + c * nna;
+    const double2 *br = sB + (wave * CH + c) * nmb;
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
       if (b < a.nvb) {
         const VBlock vb = a.vb[b];
         const int nr = r & 7;
         double av = 0.0;
-        if (i < n1 && nr < vb.nrows) {
-          const double *z = c1row + 2 * (vb.n0 + nr);
+        if (nr < vb.nrows) {
+          const double2 z = ar[vb.n0 - a.na0 + nr];
           // rows 0..7: (C1r, C1i); rows 8..15: (-C1i, C1r)
-          av = (r < 8) ? z[part] : (part == 0 ? -z[1] : z[0]);
+          av = (r < 8) ? (part ? z.y : z.x) : (part ? z.x : -z.y);
         }
         double bv = 0.0;
-        if (j >= 0 && r < vb.ncols) bv = c2row[2 * (vb.m0 + r) + part];
-        acc1[b] = mfma_f64_16x16x4(av, bv * w1, acc1[b]);
-        if (KIND == 2) acc2[b] = mfma_f64_16x16x4(av, bv * w2, acc2[b]);
+        if (r < vb.ncols) {
+          const double2 z = br[vb.m0 - a.mb0 + r];
+          bv = part ? z.y : z.x;
+        }
+        x1[b] = mfma_f64_16x16x4(av, bv * wj1, x1[b]);
+        if (KIND == 2) x2[b] = mfma_f64_16x16x4(av, bv * wj2, x2[b]);
       }
+    }
+  };
+
+  prefetch(0);
+  for (int i0 = 0; i0 < n1; i0 += CH) {
+    __syncthreads();  // the previous chunk's LDS reads are done
+    stage();
+    __syncthreads();
+    if (i0 + CH < n1) prefetch(i0 + CH);
+    if (active) {
+      const int cmax = min(CH, n1 - i0);
+      int c0 = 0;
+      if (DUAL) {
+        for (; c0 + 2 < cmax; c0 += 4) {
+          step(c0 + ii, acc1, acc2);
+          step(c0 + 2 + ii, bcc1, bcc2);
+        }
+      }
+      for (; c0 < cmax; c0 += 2) step(c0 + ii, acc1, acc2);
+    }
+  }
+  if (!active) return;
+  if (DUAL) {
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      acc1[b] += bcc1[b];
+      acc2[b] += bcc2[b];
     }
   }
 
   // result element e of lane (q4, r) is row 4e + q4: e = 0,1 give Re of band
   // n = 4e + q4, e = 2,3 the matching Im, all in the same lane
 #pragma unroll
-  for (int b = 0; b < kVertexMaxBlocks; ++b) {
+  for (int b = 0; b < MAXB; ++b) {
     if (b < a.nvb) {
       const VBlock vb = a.vb[b];
-      const PairBlock &pb = a.pairs.b[vb.pb];
-      const int nm = pb.m_hi - pb.m_lo;
+      const PairBlock &pbk = a.pairs.b[vb.pb];
+      const int nm = pbk.m_hi - pbk.m_lo;
 #pragma unroll
       for (int e = 0; e < 2; ++e) {
         const int nr = 4 * e + q4;
         if (nr < vb.nrows && r < vb.ncols) {
           const int n = vb.n0 + nr, m = vb.m0 + r;
           const long long t = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
-                              (long long)(n - pb.n_lo) * nm + (m - pb.m_lo);
+                              (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
           const long long pos = a.perm ? (long long)a.perm[t] : t;
           if (pos >= 0) {
             a.rho1[pos * a.nlf + iG] = make_double2(acc1[b][e], acc1[b][e + 2]);
@@ -180,29 +300,21 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
   }
 }
 
-// kbox[k][box(mill)] = position of mill in k's list (first occurrence wins as
-// the reference's search stops at the first match).
+// kbox[k][box(mill)] = position of mill in k's list.  Miller triples of one
+// list are distinct, so each slot is written at most once.
 __global__ void k_build_kbox(const int *mill_k, const int *ngk, int npwx, Box kb, int16_t *kbox) {
   const int k = blockIdx.y;
   const int n = ngk[k];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int a0, a1, a2;
     mill_unpack(mill_k[(size_t)k * npwx + i], a0, a1, a2);
-    int bi = kb.index(a0, a1, a2);
-    if (bi >= 0) {
-      // atomicMin on the 32-bit word holding the 16-bit slot is avoided by
-      // construction: Miller triples of one list are distinct.
-      kbox[(size_t)k * kb.size() + bi] = (int16_t)i;
-    }
+    const int bi = kb.index(a0, a1, a2);
+    if (bi >= 0) kbox[(size_t)k * kb.size() + bi] = (int16_t)i;
   }
 }
 
 int launch_build_kbox(rpa_ctx *c) {
   const size_t n = (size_t)c->d.nki * c->kbox.size();
-  int16_t *dummy = nullptr;
-  size_t cap = 0;
-  (void)dummy;
-  (void)cap;
   if (c->d_kbox) (void)hipFree(c->d_kbox);
   c->d_kbox = nullptr;
   RPA_HIP(c, hipMalloc((void **)&c->d_kbox, sizeof(int16_t) * (n ? n : 1)));
@@ -214,6 +326,40 @@ int launch_build_kbox(rpa_ctx *c) {
   return 0;
 }
 
+int launch_direct_map(rpa_ctx *c) {
+  if (c->p.vertex_map != RPA_MAP_DIRECT) return 0;
+  const long long n = (long long)c->ng0 * c->nlf * c->p.ngd;
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_direct_map, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     (const int *)c->d_mill_g, (const int *)c->d_gbox, c->gbox,
+                     (const int *)c->d_lf_key, (const int *)c->d_g0keys, c->ng0, c->nlf, c->p.ngd,
+                     c->d_dmap);
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+template <int KIND, int MAXB, bool DUAL>
+static void launch_one(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
+  if (a.ch == 32)
+    hipLaunchKernelGGL((k_vertex_f64<KIND, MAXB, DUAL, 32>), grid, dim3(256), lds, c->stream, a);
+  else if (a.ch == 16)
+    hipLaunchKernelGGL((k_vertex_f64<KIND, MAXB, DUAL, 16>), grid, dim3(256), lds, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_vertex_f64<KIND, MAXB, DUAL, 8>), grid, dim3(256), lds, c->stream, a);
+}
+
+template <int KIND>
+static void launch_kind(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
+  if (a.nvb <= 1)
+    launch_one<KIND, 1, true>(c, a, grid, lds);
+  else if (a.nvb <= 2)
+    launch_one<KIND, 2, true>(c, a, grid, lds);
+  else if (a.nvb <= 4)
+    launch_one<KIND, 4, false>(c, a, grid, lds);
+  else
+    launch_one<KIND, 8, false>(c, a, grid, lds);
+}
+
 int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   VertexArgs a{};
   a.evc = (const double2 *)c->d_evc;
@@ -221,6 +367,8 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   a.mill_g = c->d_mill_g;
   a.kbox = c->d_kbox;
   a.gbox = c->d_gbox;
+  a.dmap = c->d_dmap;
+  a.g0idx = c->d_g0idx;
   a.kb = c->kbox;
   a.gb = c->gbox;
   a.ngk = c->d_ngk;
@@ -263,6 +411,35 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
       }
   }
   a.nvb = nvb;
+  int nlo = c->d.nband, nhi = 0, mlo = c->d.nband, mhi = 0;
+  for (int b = 0; b < nvb; ++b) {
+    nlo = std::min(nlo, a.vb[b].n0);
+    nhi = std::max(nhi, a.vb[b].n0 + a.vb[b].nrows);
+    mlo = std::min(mlo, a.vb[b].m0);
+    mhi = std::max(mhi, a.vb[b].m0 + a.vb[b].ncols);
+  }
+  a.na0 = nlo;
+  a.nna = std::max(nhi - nlo, 1);
+  a.mb0 = mlo;
+  a.nmb = std::max(mhi - mlo, 1);
+  // chunk of 32, 16 or 8 plane waves: the staged bands of one row fit the
+  // prefetch registers (256/CH threads per C1 row, 64/CH per gathered row)
+  // and the staging fits in 64 KB of LDS
+  auto lds_of = [&](int ch) {
+    return sizeof(double2) * (size_t)ch * (a.nna + 4 * a.nmb) + sizeof(double) * 2 * 4 * ch;
+  };
+  auto fits = [&](int ch) {
+    return a.nna <= (256 / ch) * kPfC1 && a.nmb <= (64 / ch) * kPfB && lds_of(ch) <= 64 * 1024;
+  };
+  int ch = 32;
+  if (const char *env = std::getenv("RPA_VERTEX_CH")) {
+    const int v = std::atoi(env);
+    ch = v >= 32 ? 32 : v >= 16 ? 16 : 8;
+  }
+  while (ch > 8 && !fits(ch)) ch /= 2;
+  if (!fits(ch)) return fail(c, RPA_E_LIMIT, "band count too large for the vertex staging");
+  a.ch = ch;
+  const size_t lds = lds_of(ch);
   a.groups = (c->nlf + 3) / 4;
   a.kslots = (c->ntot + 7) / 8 * 8;
   a.perm = perm;
@@ -270,13 +447,14 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   a.rho2 = (double2 *)rho2;
   const long long nblocks = (long long)a.kslots * a.groups;
   if (nblocks <= 0) return 0;
+  if (int e = launch_direct_map(c)) return e;
   dim3 grid((unsigned)nblocks);
   if (kind == 0)
-    hipLaunchKernelGGL(k_vertex_f64<0>, grid, dim3(256), 0, c->stream, a);
+    launch_kind<0>(c, a, grid, lds);
   else if (kind == 1)
-    hipLaunchKernelGGL(k_vertex_f64<1>, grid, dim3(256), 0, c->stream, a);
+    launch_kind<1>(c, a, grid, lds);
   else
-    hipLaunchKernelGGL(k_vertex_f64<2>, grid, dim3(256), 0, c->stream, a);
+    launch_kind<2>(c, a, grid, lds);
   RPA_HIP(c, hipGetLastError());
   return 0;
 }

@@ -145,6 +145,8 @@ static int prepare(rpa_ctx *c) {
   if (!e) e = ensure(c, c->d_sig, c->cap_sig, 2 * nf);
   if (!e) e = ensure(c, c->d_qeff, c->cap_qeff, 2 * nn);
   if (!e) e = ensure(c, c->d_status, c->cap_status, 1);
+  if (!e && p.vertex_map == RPA_MAP_DIRECT)
+    e = ensure(c, c->d_dmap, c->cap_dmap, (size_t)c->ng0 * c->nlf * p.ngd);
   if (e) return e;
   RPA_HIP(c, hipMemsetAsync(c->d_qeff, 0, sizeof(double) * 2 * nn, c->stream));
   RPA_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
@@ -268,7 +270,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_nitems, c->d_first_item, c->d_intra, c->d_nintra, c->d_rho1, c->d_rho2,
                   c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig, c->d_scratch,
-                  c->d_status};
+                  c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t ev : c->prof_ev)
@@ -433,13 +435,21 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   if (!c || ntot <= 0 || !K1 || !K2 || !iG0 || !ktot || !q) return fail(c, RPA_E_ARG, "rpa_set_q: bad arguments");
   if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors must precede rpa_set_q");
   (void)hipSetDevice(c->d.device);
-  std::vector<int> m0(ntot);
+  std::vector<int> m0(ntot), g0idx(ntot), g0keys;
+  std::vector<int> g0seen;  // distinct iG0 in order of appearance
   for (int k = 0; k < ntot; ++k) {
     if (K1[k] < 1 || K1[k] > c->d.nki || K2[k] < 1 || K2[k] > c->d.nki)
       return fail(c, RPA_E_ARG, "K1/K2 outside 1..NkI");
     if (iG0[k] < 1 || iG0[k] > c->ngm) return fail(c, RPA_E_ARG, "iG0 outside 1..NG");
     const int *m = &c->mill_g[3 * (size_t)(iG0[k] - 1)];
     m0[k] = mill_pack(m[0], m[1], m[2]);
+    int z = 0;
+    while (z < (int)g0seen.size() && g0seen[z] != iG0[k]) ++z;
+    if (z == (int)g0seen.size()) {
+      g0seen.push_back(iG0[k]);
+      g0keys.push_back(m0[k]);
+    }
+    g0idx[k] = z;
   }
   c->ntot = ntot;
   c->K1.assign(K1, K1 + ntot);
@@ -447,9 +457,10 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   c->iG0.assign(iG0, iG0 + ntot);
   c->ktot.assign(ktot, ktot + 3 * (size_t)ntot);
   for (int a = 0; a < 3; ++a) c->q[a] = q[a];
-  for (void *p : {(void *)c->d_k1, (void *)c->d_k2, (void *)c->d_m0, (void *)c->d_ktot})
+  for (void *p : {(void *)c->d_k1, (void *)c->d_k2, (void *)c->d_m0, (void *)c->d_ktot,
+                  (void *)c->d_g0keys, (void *)c->d_g0idx})
     if (p) (void)hipFree(p);
-  c->d_k1 = c->d_k2 = c->d_m0 = nullptr;
+  c->d_k1 = c->d_k2 = c->d_m0 = c->d_g0keys = c->d_g0idx = nullptr;
   c->d_ktot = nullptr;
   RPA_HIP(c, hipMalloc((void **)&c->d_k1, sizeof(int) * ntot));
   RPA_HIP(c, hipMalloc((void **)&c->d_k2, sizeof(int) * ntot));
@@ -459,6 +470,11 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   RPA_HIP(c, hipMemcpy(c->d_k2, K2, sizeof(int) * ntot, hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(c->d_m0, m0.data(), sizeof(int) * ntot, hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(c->d_ktot, ktot, sizeof(double) * 3 * ntot, hipMemcpyHostToDevice));
+  c->ng0 = (int)g0keys.size();
+  RPA_HIP(c, hipMalloc((void **)&c->d_g0keys, sizeof(int) * c->ng0));
+  RPA_HIP(c, hipMalloc((void **)&c->d_g0idx, sizeof(int) * ntot));
+  RPA_HIP(c, hipMemcpy(c->d_g0keys, g0keys.data(), sizeof(int) * c->ng0, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_g0idx, g0idx.data(), sizeof(int) * ntot, hipMemcpyHostToDevice));
   c->have_q = true;
   c->prepared = false;
   invalidate_graph(c);

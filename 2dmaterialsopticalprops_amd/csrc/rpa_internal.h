@@ -118,6 +118,10 @@ struct rpa_ctx {
   int *d_k1 = nullptr, *d_k2 = nullptr, *d_m0 = nullptr;
   double *d_ktot = nullptr;
   bool have_q = false;
+  // distinct G0 of the q (direct vertex map): keys, per-k index, map table
+  int ng0 = 0;
+  int *d_g0keys = nullptr, *d_g0idx = nullptr, *d_dmap = nullptr;
+  size_t cap_dmap = 0;
 
   // transitions
   rpa::PairTable pairs{};
@@ -207,6 +211,7 @@ int ensure(rpa_ctx *c, T *&ptr, size_t &cap, size_t n) {
 int launch_build_kbox(rpa_ctx *c);
 int launch_transitions(rpa_ctx *c);
 int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm);
+int launch_direct_map(rpa_ctx *c);
 int launch_accumulate(rpa_ctx *c);
 int launch_kstencil(rpa_ctx *c);
 int launch_kk(rpa_ctx *c);
