@@ -1,0 +1,474 @@
+"""Pins the CPU oracle before it is trusted as the checker of the HIP path.
+
+Each oracle routine is checked against something that does not restate it:
+  - the reference's own routines (tests/golden/ref_golden.npz, written by
+    tests/golden/make_golden.py from oracle/_ref/libref.so, which is compiled
+    from Modules/response_function_KK.f90, qe_gksort.f90, local_field.f90 and
+    Chi_RPA/gjel.f where they lie), and the live library when it is built;
+  - closed forms: orthonormality of the Bloch coefficients, the slab-averaged
+    2D Coulomb integral, the continuum principal value on a uniform grid;
+  - brute force in numpy on tiny inputs (Miller-index dictionaries instead of
+    the oracle's index arithmetic);
+  - identities: eps^-1 = I + chi V, Hermiticity, LAPACK inverses.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+import helpers as H
+from oracle import oracle as orc
+
+synth = H.synth
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_golden.npz"))
+NOS = (5, 6, 9, 17)
+
+
+# ---------------------------------------------------------------------------
+# Kramers-Kronig stencils
+
+def _k1_table(no, dw):
+    """Optabs K1 stencil of the oracle as a table (io, w, part_in, part_out)."""
+    nw = 2 * no + 1
+    t = np.zeros((no - 1, nw, 2, 2))
+    for w in range(nw):
+        for part in (0, 1):
+            s = np.zeros((nw, 1, 1), dtype=np.complex128)
+            s[w, 0, 0] = 1.0 if part == 0 else 1.0j
+            out = orc.kk_optabs(s, dw)[:, 0, 0]
+            t[:, w, part, 0] = out.real
+            t[:, w, part, 1] = out.imag
+    return t
+
+
+def _k2_table(no, dw):
+    """Sloss K2 stencil of the oracle: (io, j) -> chi0 for a unit S0(j)."""
+    t = np.zeros((no - 1, no), dtype=np.complex128)
+    for j in range(no):
+        s = np.zeros((no, 1, 1))
+        s[j, 0, 0] = 1.0
+        t[:, j] = orc.kk_sloss(s, dw)[:, 0, 0]
+    return t
+
+
+@pytest.mark.parametrize("no", NOS)
+def test_kk_optabs_matches_reference_golden(no):
+    dw = float(GOLD["kk_dw"])
+    ref = GOLD[f"kk_cmplx_No{no}"]
+    got = _k1_table(no, dw)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-14 * scale
+
+
+@pytest.mark.parametrize("no", NOS)
+def test_kk_module_real_matches_reference_golden(no):
+    dw = float(GOLD["kk_dw"])
+    ref = GOLD[f"kk_real_No{no}"]
+    nw = 2 * no + 1
+    got = np.zeros_like(ref)
+    for w in range(nw):
+        s = np.zeros((nw, 1, 1))
+        s[w, 0, 0] = 1.0
+        for io in range(1, no):
+            got[io - 1, w] = orc.kk_module_real(io, 1, 1, s, dw)
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_kk_against_live_reference_on_random_spectra():
+    ref = orc.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref/libref.so not built (needs the reference sources)")
+    import ctypes as C
+    rng = np.random.default_rng(7)
+    no, nlf, dw = 23, 3, 0.0123
+    s = np.asfortranarray(rng.standard_normal((2 * no + 1, nlf, nlf))
+                          + 1j * rng.standard_normal((2 * no + 1, nlf, nlf)))
+    pi0 = orc.kk_optabs(s, dw)
+    a, b = C.c_double(), C.c_double()
+    for io in (1, 2, 7, no - 2, no - 1):
+        for iG, jG in ((1, 1), (2, 3), (3, 1)):
+            ref.ref_kk_cmplx(io, no, nlf, iG, jG, (io - 1) * dw, dw, s, C.byref(a), C.byref(b))
+            z = pi0[io - 1, iG - 1, jG - 1]
+            assert abs(z.real - a.value) <= 1e-13 * max(1.0, abs(a.value))
+            assert abs(z.imag - b.value) <= 1e-13 * max(1.0, abs(b.value))
+
+
+@pytest.mark.parametrize("no", NOS)
+def test_kk_sloss_static_and_second_rows_from_reference(no):
+    """K2 rows io = 1 and io = 2 equal the module's real variant at the same
+    rows, read one frequency index higher (the module indexes S0 on -No:No)."""
+    dw = float(GOLD["kk_dw"])
+    ref = GOLD[f"kk_real_No{no}"]          # (io, w, re/im) for unit S0m(w - no)
+    k2 = _k2_table(no, dw)
+    for io in (1, 2):
+        for j in range(1, no + 1):          # Sloss S0(j), frequency (j-1) dw
+            want = ref[io - 1, no + j, 0] if no + j < 2 * no + 1 else 0.0
+            got = k2[io - 1, j - 1].real
+            assert abs(got - want) <= 1e-14 * max(1.0, abs(want)), (io, j)
+
+
+@pytest.mark.parametrize("no", NOS)
+def test_kk_sloss_interior_rows(no):
+    """For io >= 3: Re chi0 coefficient on S0(j) is the uniform-grid principal
+    value 1/(io-j) (0 at j = io) minus the K1 'plus' weight, which the
+    reference golden table holds at frequency -(j-1); Im chi0 = -pi S0(io)."""
+    dw = float(GOLD["kk_dw"])
+    ref = GOLD[f"kk_cmplx_No{no}"]
+    k2 = _k2_table(no, dw)
+    for io in range(3, no):
+        for j in range(1, no + 1):
+            plus = ref[io - 1, no - (j - 1), 0, 0]
+            if j == 1:
+                plus = 0.5 / (io - 1)        # S0(0) carries both halves in K1
+            minus = 0.0 if j == io else 1.0 / (io - j)
+            assert abs(k2[io - 1, j - 1].real - (minus - plus)) <= 1e-13, (io, j)
+            want_im = -np.pi if j == io else 0.0
+            assert k2[io - 1, j - 1].imag == pytest.approx(want_im, abs=1e-15)
+
+
+def test_kk_k1_plus_weight_at_zero_frequency_is_half():
+    """K1 puts c+_1 = c-_1 = dw/(2 w_i) on S0(0) for io >= 3 (half weights), so
+    the golden coefficient on S0(0) is 1/(io-1) in grid units."""
+    no = 17
+    ref = GOLD[f"kk_cmplx_No{no}"]
+    for io in range(3, no - 1):
+        assert ref[io - 1, no, 0, 0] == pytest.approx(1.0 / (io - 1), rel=1e-14)
+
+
+def test_w1w2_consistent_with_k2():
+    rng = np.random.default_rng(3)
+    no, nlf, dw = 19, 2, 0.02
+    s = np.asfortranarray(rng.standard_normal((no, nlf, nlf)))
+    chi0 = orc.kk_sloss(s, dw)
+    for io in range(1, no):
+        w1, w2 = orc.w1w2(io, 2, 1, s, dw)
+        if io == 1:
+            assert w1 == pytest.approx(0.5 * chi0[0, 1, 0].real, rel=1e-13)
+            assert w2 == -w1
+        else:
+            assert w1 - w2 == pytest.approx(chi0[io - 1, 1, 0].real, rel=1e-12, abs=1e-12)
+
+
+def test_kk_continuum_limit_of_a_box_spectrum():
+    """Loose check of the discretisation: S0 = 1 on [a, b] gives
+    Re chi0(w) -> -(ln|(w-b)/(w-a)| + ln((w+b)/(w+a))) as dw -> 0 (K2 form
+    Re chi0 = P int S0(w')[1/(w-w') - 1/(w+w')] dw')."""
+    no = 2001
+    w = np.arange(no) * (4.0 / (no - 1))
+    a, b = 1.0, 2.0
+    s = ((w >= a) & (w <= b)).astype(float)
+    dw = w[1] - w[0]
+    chi0 = orc.kk_sloss(s.reshape(no, 1, 1), dw)[:, 0, 0]
+    for x in (0.5, 2.6, 3.1):
+        io = int(round(x / dw)) + 1
+        wi = (io - 1) * dw
+        exact = np.log(abs((wi - a) / (wi - b))) - np.log((wi + b) / (wi + a))
+        assert chi0[io - 1].real == pytest.approx(exact, abs=5e-3)
+
+
+# ---------------------------------------------------------------------------
+# gjel, gk_sort, genGlf
+
+@pytest.mark.parametrize("n", (1, 2, 3, 5, 11))
+def test_gjel_matches_reference_golden_and_lapack(n):
+    a = GOLD[f"gjel_in_{n}"]
+    got = orc.gjel(a)
+    ref = GOLD[f"gjel_out_{n}"]
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+    assert np.abs(got - np.linalg.inv(a)).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_gk_sort_matches_reference_golden():
+    ks, g, gcutw, igk = GOLD["igk_k"], GOLD["igk_g"], float(GOLD["igk_gcutw"]), GOLD["igk_out"]
+    for ik in range(len(ks)):
+        rc, lst = orc.gk_sort(ks[ik], g, gcutw, igk.shape[0])
+        assert rc == 0
+        want = igk[:, ik]
+        want = want[want > 0]
+        np.testing.assert_array_equal(lst, want)
+
+
+def test_gk_sort_brute_force_rule():
+    """|k+G|^2 <= gcutw, stable by |k+G|^2 with index tie-break (eps 1e-8)."""
+    ks, g, gcutw = GOLD["igk_k"], GOLD["igk_g"], float(GOLD["igk_gcutw"])
+    for ik in range(len(ks)):
+        q2 = ((ks[ik] + g) ** 2).sum(1)
+        sel = np.nonzero(q2 <= gcutw)[0]
+        # hpsort_eps compares within 1e-8: bucket equal keys, keep index order
+        key = np.round(q2[sel] / 1e-8).astype(np.int64)
+        order = sel[np.lexsort((sel, key))] + 1
+        rc, lst = orc.gk_sort(ks[ik], g, gcutw, 400)
+        np.testing.assert_array_equal(lst, order)
+
+
+@pytest.mark.parametrize("name", ["tiny", "cfg0"])
+@pytest.mark.parametrize("lf", ["z", "xyz"])
+def test_gen_glf_matches_reference_golden(name, lf):
+    s = synth.make_system(synth.CONFIGS[name], with_coeffs=False)
+    g = synth.gcart(s.mill_g, s.kc)
+    ecut = float(GOLD[f"glf_{name}_{lf}_ecut"])
+    rc, n, glf, parg, idx = orc.gen_glf(lf, ecut, s.gcar, g, s.mill_g, 4000)
+    assert rc == 0
+    assert n == int(GOLD[f"glf_{name}_{lf}_n"])
+    np.testing.assert_array_equal(glf, GOLD[f"glf_{name}_{lf}_glf"])
+    np.testing.assert_array_equal(parg, GOLD[f"glf_{name}_{lf}_parg"])
+    np.testing.assert_array_equal(g[idx - 1], glf)
+
+
+# ---------------------------------------------------------------------------
+# k / k+q mapping
+
+def test_find_kq_brute_force():
+    s = H.system("tiny")
+    g = synth.gcart(s.mill_g, s.kc)
+    for iq in (1, 2, 3):
+        q = orc.find_min_q(s.kcart, iq)
+        K2, iG0 = orc.find_kq_all(s.kcart, s.kcart, g, q, 1e-4)
+        for ik in range(len(s.kcart)):
+            kq = s.kcart[ik] + q
+            # k+q = G0 + k2 with G0 the first G (in list order) that works
+            np.testing.assert_allclose(kq, g[iG0[ik] - 1] + s.kcart[K2[ik] - 1], atol=1e-4)
+            d = np.abs(kq[None, None, :] - g[:, None, :] - s.kcart[None, :, :]).max(2)
+            hits = np.argwhere(d < 1e-4)
+            assert len(hits) >= 1
+            assert hits[0, 0] + 1 == iG0[ik]
+
+
+# ---------------------------------------------------------------------------
+# vertices
+
+def _vertices_brute(pb, kind, gcar, n_lo, n_hi, m_lo, m_hi):
+    s = pb.sys
+    mlf = s.mill_g[pb.lf_idx - 1]
+    glf = synth.gcart(mlf, s.kc)
+    gpos = {tuple(m): i for i, m in enumerate(s.mill_g)}
+    nn, nm = n_hi - n_lo, m_hi - m_lo
+    out = np.zeros((len(pb.K1) * nn * nm, pb.nlf), dtype=np.complex128)
+    for ik in range(len(pb.K1)):
+        K1, K2 = pb.K1[ik] - 1, pb.K2[ik] - 1
+        m0 = s.mill_g[pb.iG0[ik] - 1]
+        if pb.map_mode == 0:
+            l1 = s.mill_g[:pb.ngd]
+            look = gpos
+            n2 = s.ngk[K2]
+        else:
+            l1 = s.mill_k[K1, :s.ngk[K1]]
+            look = {tuple(m): i for i, m in enumerate(s.mill_k[K2, :s.ngk[K2]])}
+            n2 = s.ngk[K2]
+        g1 = synth.gcart(l1, s.kc)
+        for iG in range(pb.nlf):
+            idx = np.array([look.get(tuple(m), -1) for m in (l1 + mlf[iG] + m0)])
+            ok = (idx >= 0) & (idx < n2)
+            c1 = s.evc[K1, n_lo:n_hi, :len(l1)][:, ok]
+            c2 = s.evc[K2, m_lo:m_hi][:, idx[ok]]
+            if kind == 0:
+                wgt = np.ones(ok.sum())
+            else:
+                p = kind - 1
+                wgt = 0.5 * gcar * (pb.q[p] + 2 * pb.ktot[ik, p] + glf[iG, p] + 2 * g1[ok, p])
+            v = np.conj(c1) @ (c2 * wgt).T                  # (nn, nm)
+            out[ik * nn * nm:(ik + 1) * nn * nm, iG] = v.ravel()
+    return out
+
+
+@pytest.mark.parametrize("map_mode", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 3])
+def test_vertices_brute_force_tiny(map_mode, kind):
+    s = H.system("tiny")
+    pb = H.setup(s, "sloss" if kind == 0 else "optabs", map_mode=map_mode)
+    nv, nb = s.cfg.nval, s.cfg.nband
+    got = orc.vertices(pb, kind, s.gcar, 0, nv, 0, nb)
+    want = _vertices_brute(pb, kind, s.gcar, 0, nv, 0, nb)
+    assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_vertices_q0_orthonormality_and_current_closed_form():
+    """q = 0, G = 0, whole sphere: rho_nm = delta_nm and
+    j_nn = Gcar sum_i |C_n(i)|^2 (k + G_i)_pol."""
+    s = H.system("tiny")
+    pb = H.setup(s, "optabs", map_mode=1, iq=1)
+    assert np.all(pb.K2 == pb.K1) and np.allclose(s.mill_g[pb.iG0 - 1], 0)
+    g0 = int(np.nonzero((s.mill_g[pb.lf_idx - 1] == 0).all(1))[0][0])
+    nb = s.cfg.nband
+    rho = orc.vertices(pb, 0, s.gcar, 0, nb, 0, nb)[:, g0].reshape(len(pb.K1), nb, nb)
+    eye = np.broadcast_to(np.eye(nb), rho.shape)
+    assert np.abs(rho - eye).max() < 1e-13
+    for kind in (1, 2, 3):
+        j = orc.vertices(pb, kind, s.gcar, 0, nb, 0, nb)[:, g0].reshape(len(pb.K1), nb, nb)
+        p = kind - 1
+        for ik in range(len(pb.K1)):
+            n1 = s.ngk[ik]
+            kg = s.kcart[ik, p] + synth.gcart(s.mill_k[ik, :n1], s.kc)[:, p]
+            want = s.gcar * (np.abs(s.evc[ik, :, :n1]) ** 2 @ kg)
+            np.testing.assert_allclose(np.diag(j[ik]).real, want, rtol=1e-12, atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# S0 accumulation
+
+def _sloss_s0_brute(pb, nocc, no, dw, eta, vcell):
+    s = pb.sys
+    nb = s.cfg.nband
+    rho = orc.vertices(pb, 0, s.gcar, 0, nocc, nocc, nb)
+    nt = len(pb.K1)
+    S0 = np.zeros((no, pb.nlf, pb.nlf))
+    o = np.arange(no) * dw
+    t = 0
+    for ik in range(nt):
+        for n in range(nocc):
+            for m in range(nocc, nb):
+                dE = o + pb.E[pb.K1[ik] - 1, n] - pb.E[pb.K2[ik] - 1, m]
+                lor = -eta / (dE ** 2 + eta ** 2)
+                lor[np.abs(lor) < 1e-3 / eta] = 0.0
+                r = rho[t]
+                S0 -= (2 * lor)[:, None, None] * np.real(np.outer(r, np.conj(r)))[None] / (np.pi * nt * vcell)
+                t += 1
+    return S0
+
+
+def test_sloss_s0_brute_force_and_symmetry():
+    s = H.system("tiny")
+    pb = H.setup(s, "sloss")
+    dw = H.sloss_grid(s.cfg)
+    eta = s.cfg.eta_ev / H.HARTREE
+    S0 = orc.sloss_s0(pb, s.cfg.nval, s.cfg.no, dw, eta, s.vcell, nthreads=2)
+    want = _sloss_s0_brute(pb, s.cfg.nval, s.cfg.no, dw, eta, s.vcell)
+    assert np.abs(S0 - want).max() <= 1e-12 * np.abs(want).max()
+    assert np.allclose(S0, S0.transpose(0, 2, 1), rtol=0, atol=1e-14 * np.abs(S0).max())
+    assert (np.einsum("wii->wi", S0) >= 0).all()
+
+
+def test_sloss_s0_thread_count_only_reorders_sums():
+    s = H.system("tiny")
+    pb = H.setup(s, "sloss")
+    dw = H.sloss_grid(s.cfg)
+    eta = s.cfg.eta_ev / H.HARTREE
+    a = orc.sloss_s0(pb, s.cfg.nval, s.cfg.no, dw, eta, s.vcell, nthreads=1)
+    b = orc.sloss_s0(pb, s.cfg.nval, s.cfg.no, dw, eta, s.vcell, nthreads=3)
+    assert np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+
+
+def test_optabs_s0_brute_force_hermitian_and_qeff():
+    s = H.system("tiny")
+    pb = H.setup(s, "optabs")
+    cfg = s.cfg
+    dw = H.optabs_grid(cfg)
+    eta = cfg.eta_ev / H.HARTREE
+    T = 0.05 / H.HARTREE
+    ef = pb.E[:, cfg.nval - 1].max() - 0.1 / H.HARTREE  # Drude terms near the valence top
+    lor_cut, df_cut = 1e-3, 1e-3
+    nb, no = cfg.nband, cfg.no
+    S0, Q = orc.optabs_s0(pb, cfg.nval, no, 1, dw, eta, lor_cut, df_cut, ef, T, s.vcell, s.gcar,
+                          nthreads=2)
+    j = orc.vertices(pb, 1, s.gcar, 0, nb, 0, nb)
+    nt = len(pb.K1)
+    o = np.arange(-no, no + 1) * dw
+    want = np.zeros_like(S0)
+    wq = np.zeros_like(Q)
+
+    def occ(e):  # Fermi function, exactly 1 / 0 beyond |x| = 20
+        x = (e - ef) / T
+        if x < -20:
+            return 1.0, 0.0
+        if x > 20:
+            return 0.0, x
+        return 1.0 / (np.exp(x) + 1.0), np.exp(x)
+
+    t = 0
+    for ik in range(nt):
+        for n in range(nb):
+            for m in range(nb):
+                e1 = pb.E[pb.K1[ik] - 1, n]
+                e2 = pb.E[pb.K2[ik] - 1, m]
+                f1, x1 = occ(e1)
+                f2, _ = occ(e2)
+                v = j[t]
+                t += 1
+                if n == m:
+                    if abs(e1 - ef) <= 10 * T:
+                        fact = -x1 / ((x1 + 1) ** 2 * T)
+                        wq += 2 * fact * np.outer(v, np.conj(v)) / (nt * s.vcell)
+                    continue
+                if abs(f1 - f2) < df_cut:
+                    continue
+                dE = o + e1 - e2
+                lor = eta / (dE ** 2 + eta ** 2)
+                lor[lor < lor_cut / eta] = 0.0
+                want -= (2 * (f1 - f2) * lor)[:, None, None] * np.outer(v, np.conj(v))[None] / (np.pi * nt * s.vcell)
+    scale = np.abs(want).max()
+    assert np.abs(S0 - want).max() <= 1e-12 * scale
+    assert np.abs(S0 - np.conj(S0.transpose(0, 2, 1))).max() <= 1e-14 * scale
+    assert np.abs(wq).max() > 0
+    assert np.abs(Q - wq).max() <= 1e-12 * np.abs(wq).max()
+
+
+# ---------------------------------------------------------------------------
+# Coulomb, Dyson, Pi
+
+def test_genV_is_the_slab_averaged_coulomb_integral():
+    """V(Gz, Gz') = (1/c) int int (2 pi/Q) e^{-Q|z-z'|} e^{-i Gz z + i Gz' z'}
+    over the slab |z|, |z'| <= c/2 (lf = z local fields, Q = |q|)."""
+    from scipy import integrate
+    c0, a0 = 20.0, 4.65
+    gcar = 2 * np.pi / a0
+    ns = [0, 1, -1, 2]
+    glf = np.array([[0.0, 0.0, n * a0 / c0] for n in ns])
+    parg = np.array([1 if n % 2 == 0 else -1 for n in ns], dtype=np.int32)
+    qx, qy = 0.04, 0.03
+    V = orc.genV(1e-4, qx, qy, c0, parg, glf, gcar)
+    Q = np.hypot(qx, qy)
+    for i in range(len(ns)):
+        for j in range(len(ns)):
+            gi, gj = glf[i, 2] * gcar, glf[j, 2] * gcar
+            f = lambda z, zp: (2 * np.pi / Q) * np.exp(-Q * abs(z - zp)) * np.cos(-gi * z + gj * zp)
+            r, _ = integrate.dblquad(f, -c0 / 2, c0 / 2, -c0 / 2, c0 / 2, epsabs=1e-9, epsrel=1e-9)
+            assert V[i, j] == pytest.approx(r / c0, rel=2e-7, abs=1e-7)
+
+
+def test_genV_zero_between_different_in_plane_fields():
+    c0, a0 = 20.0, 4.65
+    gcar = 2 * np.pi / a0
+    glf = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [1.0, 0.0, 0.2325], [0.0, 0.0, 0.2325]])
+    parg = np.array([1, 1, -1, -1], dtype=np.int32)
+    V = orc.genV(1e-4, 0.01, 0.0, c0, parg, glf, gcar)
+    for i in range(4):
+        for j in range(4):
+            if glf[i, 0] != glf[j, 0]:
+                assert V[i, j] == 0.0
+    # the in-plane field G = (1,0) sees Q = |q + G Gcar|
+    Q = (0.01 + gcar)
+    want = 4 * np.pi / Q ** 2 * (1 - (1 - np.exp(-Q * c0)) / (Q * c0))
+    assert V[1, 1] == pytest.approx(want, rel=1e-12)
+
+
+@pytest.mark.parametrize("nlf", [1, 4, 9])
+def test_dyson_w_identities_and_lapack(nlf):
+    rng = np.random.default_rng(nlf)
+    chi0 = 0.05 * (rng.standard_normal((nlf, nlf)) + 1j * rng.standard_normal((nlf, nlf)))
+    V = rng.standard_normal((nlf, nlf))
+    V = V + V.T
+    e, c, w = orc.dyson_w(chi0, V)
+    eps = np.eye(nlf) - chi0 @ V
+    ref_inv = np.linalg.inv(eps)
+    assert np.abs(e - ref_inv).max() <= 1e-12 * np.abs(ref_inv).max()
+    assert np.abs(e - (np.eye(nlf) + c @ V)).max() <= 1e-12 * np.abs(e).max()
+    assert np.abs(c - ref_inv @ chi0).max() <= 1e-12 * np.abs(c).max()
+    assert np.abs(w - (V + V @ c @ V)).max() <= 1e-12 * np.abs(w).max()
+    if nlf == 1:
+        assert c[0, 0] == pytest.approx(chi0[0, 0] / (1 - V[0, 0] * chi0[0, 0]), rel=1e-14)
+
+
+def test_pi_assembly_single_field_closed_form():
+    rng = np.random.default_rng(11)
+    no, dw, eta_i, c0 = 12, 0.01, 0.004, 20.0
+    pi0 = (rng.standard_normal(no - 1) + 1j * rng.standard_normal(no - 1)).reshape(no - 1, 1, 1)
+    qeff = np.array([[0.3 - 0.1j]])
+    pi11, si, sa = orc.pi_assembly(pi0, qeff, dw, eta_i, c0)
+    w = np.arange(no - 1) * dw
+    intra = qeff[0, 0] * w / (w + 1j * eta_i)
+    inter = pi0[:, 0, 0] - pi0[0, 0, 0].real
+    np.testing.assert_allclose(pi11, inter + intra, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(si[1:], (-1j * c0 * inter[1:] / w[1:]).real, rtol=1e-13)
+    np.testing.assert_allclose(sa[1:], (-1j * c0 * intra[1:] / w[1:]).real, rtol=1e-13)
+    assert si[0] == 0.0 and sa[0] == 0.0
