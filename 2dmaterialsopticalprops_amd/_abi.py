@@ -26,8 +26,12 @@ EXPORTS = [
     "rpa_set_params", "rpa_set_q", "rpa_vertices", "rpa_accumulate_s0", "rpa_kk",
     "rpa_coulomb", "rpa_dyson_w", "rpa_epilogue", "rpa_run", "rpa_run_graph",
     "rpa_get_vertices", "rpa_gen_fbz", "rpa_find_min_q", "rpa_find_kq", "rpa_gen_glf",
-    "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms",
+    "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms", "rpa_get_result",
+    "rpa_load_s0", "rpa_w_kk",
 ]
+# enum rpa_result
+RES = {"S0": 0, "QEFF": 1, "CHI0": 2, "V": 3, "EPSINV": 4, "CHI": 5, "W": 6, "LOSS": 7,
+       "PI11": 8, "SIG": 9}
 STAGES = ["sort", "vertices", "accumulate", "reduce", "kk", "dyson_or_pi"]
 
 
@@ -91,6 +95,9 @@ def lib() -> C.CDLL:
         L.rpa_gcart.argtypes = [C.c_int, P, P, P]
         L.rpa_set_profiling.argtypes = [P, C.c_int]
         L.rpa_stage_ms.argtypes = [P, P, C.c_int]
+        L.rpa_get_result.argtypes = [P, C.c_int, P]
+        L.rpa_load_s0.argtypes = [P, P, P]
+        L.rpa_w_kk.argtypes = [P, P, P]
         _lib = L
     return _lib
 
@@ -278,6 +285,34 @@ class Context:
         ms = np.zeros(6)
         self._c(lib().rpa_stage_ms(self.h, _ptr(ms), 6))
         return dict(zip(STAGES, ms.tolist()))
+
+    def result(self, name):
+        """A result of the last run, downloaded without recomputation."""
+        nlf, nf = self.nlf, self.no - 1
+        shapes = {
+            "S0": ((self.no, nlf, nlf), np.float64) if self.program == RPA_SLOSS
+            else ((2 * self.no + 1, nlf, nlf), np.complex128),
+            "QEFF": ((nlf, nlf), np.complex128), "CHI0": ((nf, nlf, nlf), np.complex128),
+            "V": ((nlf, nlf), np.float64), "EPSINV": ((nf, nlf, nlf), np.complex128),
+            "CHI": ((nf, nlf, nlf), np.complex128), "W": ((nf, nlf, nlf), np.complex128),
+            "LOSS": ((nf,), np.float64), "PI11": ((nf,), np.complex128), "SIG": ((nf, 2), np.float64),
+        }
+        shp, dt = shapes[name]
+        out = np.zeros(shp, dtype=dt, order="F")
+        self._c(lib().rpa_get_result(self.h, RES[name], _ptr(out)))
+        return out
+
+    def load_s0(self, S0, qeff=None):
+        dt = np.float64 if self.program == RPA_SLOSS else np.complex128
+        s = np.asfortranarray(S0, dtype=dt)
+        q = None if qeff is None else np.asfortranarray(qeff, dtype=np.complex128)
+        self._c(lib().rpa_load_s0(self.h, _ptr(s), _ptr(q)))
+
+    def w_kk(self):
+        nf = self.no - 1
+        w1, w2 = np.zeros(nf), np.zeros(nf)
+        self._c(lib().rpa_w_kk(self.h, _ptr(w1), _ptr(w2)))
+        return w1, w2
 
     def get_vertices(self, both=False):
         n = C.c_longlong()

@@ -56,6 +56,40 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(PKG, "fortran")
+BIN = os.path.join(PKG, "bin")
+F_MODULES = ["rpa_hip_f.f90", "rpa_files.f90"]
+F_PROGRAMS = {"sloss_hip.x": "sloss_hip.f90", "optabs_hip.x": "optabs_hip.f90"}
+F_FLAGS = ["-O2"]
+
+
+def build_fortran(verbose: bool = False) -> dict:
+    """The Fortran drivers (ISO_C_BINDING over librpa_hip.so) -> bin/*.x.
+    They find the library next to the package through an $ORIGIN rpath."""
+    if not os.path.exists(LIB):
+        build_lib(verbose)
+    fb = os.path.join(BUILD, "fortran")
+    os.makedirs(fb, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    objs = []
+    for src in F_MODULES:
+        obj = os.path.join(fb, os.path.splitext(src)[0] + ".o")
+        _run([FLANG] + F_FLAGS + ["-module-dir", fb, "-c", os.path.join(FORTRAN, src), "-o", obj])
+        objs.append(obj)
+    out = {}
+    for exe, src in F_PROGRAMS.items():
+        obj = os.path.join(fb, os.path.splitext(src)[0] + ".o")
+        _run([FLANG] + F_FLAGS + ["-module-dir", fb, "-c", os.path.join(FORTRAN, src), "-o", obj])
+        dst = os.path.join(BIN, exe)
+        _run([FLANG, "-o", dst + ".tmp", obj] + objs
+             + ["-L", PKG, "-lrpa_hip", "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(dst + ".tmp", dst)
+        out[exe] = dst
+    return out
+
+
 if __name__ == "__main__":
     build_lib(verbose="-v" in sys.argv)
     print(LIB)
+    for path in build_fortran(verbose="-v" in sys.argv).values():
+        print(path)

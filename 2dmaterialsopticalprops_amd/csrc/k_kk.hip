@@ -199,4 +199,49 @@ int launch_kk(rpa_ctx *c) {
   return 0;
 }
 
+// Kramers-Kronig self-check of the screened interaction (Sloss.f90:489-534,
+// genW1W2 1062-1171) at (G, G') = (1, 1): the spectral function
+// S_W(io) = -(1/pi) Im W(io,1,1) for io = 1..no-1 (S0(no,1,1) keeps its
+// chi0 value, the reference overwrites only 1..no-1), then per io
+//   io = 1:  W1 = -sum_{jo>=2} f_jo S_W(jo),  W2 = -W1
+//   io >= 2: W1 = sum_jo c-(io,jo) S_W(jo),  W2 = sum_jo c+(io,jo) S_W(jo)
+// with the K2 weights.  One workgroup; S_W staged in LDS.
+__global__ void k_w_kk(int no, double dw, int nlf, const double *W, const double *s0_last,
+                       double *w12) {
+#pragma clang fp contract(off)
+  extern __shared__ double sw[];
+  const double pi = 3.14159265358979323846;
+  const size_t stride = 2 * (size_t)nlf * nlf;  // complex [f][i][j]
+  for (int j = threadIdx.x; j < no; j += blockDim.x)
+    sw[j] = (j < no - 1) ? -((1.0 / pi) * W[(size_t)j * stride + 1]) : s0_last[0];
+  __syncthreads();
+  for (int io = 1 + threadIdx.x; io <= no - 1; io += blockDim.x) {
+    const double oi = (double)(io - 1) * dw;
+    double w1 = 0.0, w2 = 0.0;
+    if (io == 1) {
+      for (int jo = 2; jo <= no; ++jo) w1 = w1 - f_static(jo, no, (double)(jo - 1) * dw, dw) * sw[jo - 1];
+      w2 = -w1;
+    } else {
+      for (int jo = 1; jo <= no; ++jo) {
+        const double oj = (double)(jo - 1) * dw;
+        w1 = w1 + k2_cminus(io, jo, oi, oj, dw) * sw[jo - 1];
+        w2 = w2 + c_plus(jo, no, oi, oj, dw) * sw[jo - 1];
+      }
+    }
+    w12[io - 1] = w1;
+    w12[no - 1 + io - 1] = w2;
+  }
+}
+
+int launch_w_kk(rpa_ctx *c) {
+  const int no = c->d.no;
+  const size_t lds = sizeof(double) * (size_t)no;
+  if (lds > 64 * 1024) return fail(c, RPA_E_LIMIT, "too many frequencies for the W Kramers-Kronig check");
+  hipLaunchKernelGGL(k_w_kk, dim3(1), dim3(256), lds, c->stream, no, c->p.domega, c->nlf,
+                     (const double *)c->d_W, (const double *)c->d_s0 + (size_t)(no - 1) * c->ncol_pad,
+                     c->d_wkk);
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
 }  // namespace rpa

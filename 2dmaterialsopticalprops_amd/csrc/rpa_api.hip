@@ -144,6 +144,7 @@ static int prepare(rpa_ctx *c) {
   if (!e) e = ensure(c, c->d_pi11, c->cap_pi11, 2 * nf);
   if (!e) e = ensure(c, c->d_sig, c->cap_sig, 2 * nf);
   if (!e) e = ensure(c, c->d_qeff, c->cap_qeff, 2 * nn);
+  if (!e) e = ensure(c, c->d_wkk, c->cap_wkk, 2 * nf);
   if (!e) e = ensure(c, c->d_status, c->cap_status, 1);
   if (!e && p.vertex_map == RPA_MAP_DIRECT)
     e = ensure(c, c->d_dmap, c->cap_dmap, (size_t)c->ng0 * c->nlf * p.ngd);
@@ -217,6 +218,50 @@ static int copy_freq_mats(rpa_ctx *c, const double *dev, double *out) {
   return 0;
 }
 
+// S0 in the reference layout: Sloss S0(no,Nlf,Nlf) real, Optabs
+// S0(-no:no,Nlf,Nlf) complex; device [w][pair] with pair = i*Nlf + j
+static int get_s0(rpa_ctx *c, double *out) {
+  const int nw = c->nw, N = c->nlf;
+  std::vector<double> h((size_t)nw * c->ncol_pad);
+  if (int e = d2h(c, h.data(), (const double *)c->d_s0, h.size())) return e;
+  for (int w = 0; w < nw; ++w)
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        const size_t pr = (size_t)i * N + j;
+        const size_t dst = (size_t)w + (size_t)nw * (i + (size_t)N * j);
+        if (c->d.program == RPA_SLOSS) {
+          out[dst] = h[(size_t)w * c->ncol_pad + pr];
+        } else {
+          out[2 * dst] = h[(size_t)w * c->ncol_pad + 2 * pr];
+          out[2 * dst + 1] = h[(size_t)w * c->ncol_pad + 2 * pr + 1];
+        }
+      }
+  return 0;
+}
+
+// square (Nlf,Nlf) matrix, device row-major [i][j] -> column-major
+static int get_square(rpa_ctx *c, const double *dev, int ncomp, double *out) {
+  const size_t N = (size_t)c->nlf;
+  std::vector<double> h(N * N * ncomp);
+  if (int e = d2h(c, h.data(), dev, h.size())) return e;
+  for (size_t i = 0; i < N; ++i)
+    for (size_t j = 0; j < N; ++j)
+      for (int q = 0; q < ncomp; ++q) out[(i + N * j) * ncomp + q] = h[(i * N + j) * ncomp + q];
+  return 0;
+}
+
+// sig(no-1, 2): device interleaves (inter, intra) per frequency
+static int get_sig(rpa_ctx *c, double *sig) {
+  const size_t nf = (size_t)(c->d.no - 1);
+  std::vector<double> h(2 * nf);
+  if (int e = d2h(c, h.data(), (const double *)c->d_sig, 2 * nf)) return e;
+  for (size_t f = 0; f < nf; ++f) {
+    sig[f] = h[2 * f];
+    sig[nf + f] = h[2 * f + 1];
+  }
+  return 0;
+}
+
 }  // namespace rpa
 
 using namespace rpa;
@@ -270,7 +315,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_nitems, c->d_first_item, c->d_intra, c->d_nintra, c->d_rho1, c->d_rho2,
                   c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig, c->d_scratch,
-                  c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap};
+                  c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t ev : c->prof_ev)
@@ -496,32 +541,10 @@ int rpa_accumulate_s0(rpa_ctx *c, double *S0_out, double *qeff_out) {
   (void)hipSetDevice(c->d.device);
   if (int e = launch_accumulate(c)) return e;
   if (int e = sync(c)) return e;
-  const int nw = c->nw, N = c->nlf;
-  if (S0_out) {
-    std::vector<double> h((size_t)nw * c->ncol_pad);
-    if (int e = d2h(c, h.data(), (const double *)c->d_s0, h.size())) return e;
-    for (int w = 0; w < nw; ++w)
-      for (int i = 0; i < N; ++i)
-        for (int j = 0; j < N; ++j) {
-          size_t pr = (size_t)i * N + j;
-          size_t dst = (size_t)w + (size_t)nw * (i + (size_t)N * j);
-          if (c->d.program == RPA_SLOSS) {
-            S0_out[dst] = h[(size_t)w * c->ncol_pad + pr];
-          } else {
-            S0_out[2 * dst] = h[(size_t)w * c->ncol_pad + 2 * pr];
-            S0_out[2 * dst + 1] = h[(size_t)w * c->ncol_pad + 2 * pr + 1];
-          }
-        }
-  }
-  if (qeff_out && c->d.program == RPA_OPTABS) {
-    std::vector<double> h(2 * (size_t)N * N);
-    if (int e = d2h(c, h.data(), (const double *)c->d_qeff, h.size())) return e;
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) {
-        qeff_out[2 * (i + (size_t)N * j)] = h[2 * ((size_t)i * N + j)];
-        qeff_out[2 * (i + (size_t)N * j) + 1] = h[2 * ((size_t)i * N + j) + 1];
-      }
-  }
+  if (S0_out)
+    if (int e = get_s0(c, S0_out)) return e;
+  if (qeff_out && c->d.program == RPA_OPTABS)
+    if (int e = get_square(c, (const double *)c->d_qeff, 2, qeff_out)) return e;
   return RPA_OK;
 }
 
@@ -540,14 +563,7 @@ int rpa_coulomb(rpa_ctx *c, double *V_out) {
   if (!c->d_V) return fail(c, RPA_E_STATE, "rpa_vertices must precede rpa_coulomb");
   (void)hipSetDevice(c->d.device);
   if (int e = launch_coulomb(c)) return e;
-  if (V_out) {
-    const int N = c->nlf;
-    std::vector<double> h((size_t)N * N);
-    if (int e = d2h(c, h.data(), (const double *)c->d_V, h.size())) return e;
-    for (int i = 0; i < N; ++i)
-      for (int j = 0; j < N; ++j) V_out[i + (size_t)N * j] = h[(size_t)i * N + j];
-    return RPA_OK;
-  }
+  if (V_out) return get_square(c, (const double *)c->d_V, 1, V_out);
   return sync(c);
 }
 
@@ -580,14 +596,8 @@ int rpa_epilogue(rpa_ctx *c, double *loss, double *pi11, double *sig) {
     if (int e = launch_epilogue(c)) return e;
     if (pi11)
       if (int e = d2h(c, pi11, (const double *)c->d_pi11, 2 * nf)) return e;
-    if (sig) {
-      std::vector<double> h(2 * nf);
-      if (int e = d2h(c, h.data(), (const double *)c->d_sig, 2 * nf)) return e;
-      for (size_t f = 0; f < nf; ++f) {
-        sig[f] = h[2 * f];
-        sig[nf + f] = h[2 * f + 1];
-      }
-    }
+    if (sig)
+      if (int e = get_sig(c, sig)) return e;
   } else if (loss) {
     if (int e = d2h(c, loss, (const double *)c->d_loss, nf)) return e;
   }
@@ -608,6 +618,88 @@ int rpa_run(rpa_ctx *c) {
   if (c->profiling)
     if (int e = collect_stage_times(c)) return e;
   return RPA_OK;
+}
+
+int rpa_load_s0(rpa_ctx *c, const double *S0, const double *qeff) {
+  if (!c || !S0) return RPA_E_ARG;
+  (void)hipSetDevice(c->d.device);
+  if (int e = prepare(c)) return e;
+  const int nw = c->nw, N = c->nlf;
+  std::vector<double> h((size_t)nw * c->ncol_pad, 0.0);
+  for (int w = 0; w < nw; ++w)
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) {
+        const size_t pr = (size_t)i * N + j;
+        const size_t src = (size_t)w + (size_t)nw * (i + (size_t)N * j);
+        if (c->d.program == RPA_SLOSS) {
+          h[(size_t)w * c->ncol_pad + pr] = S0[src];
+        } else {
+          h[(size_t)w * c->ncol_pad + 2 * pr] = S0[2 * src];
+          h[(size_t)w * c->ncol_pad + 2 * pr + 1] = S0[2 * src + 1];
+        }
+      }
+  RPA_HIP(c, hipMemcpyAsync(c->d_s0, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+  if (c->d.program == RPA_OPTABS) {
+    std::vector<double> q(2 * (size_t)N * N, 0.0);
+    if (qeff)
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+          for (int r = 0; r < 2; ++r) q[2 * ((size_t)i * N + j) + r] = qeff[2 * (i + (size_t)N * j) + r];
+    RPA_HIP(c, hipMemcpyAsync(c->d_qeff, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice, c->stream));
+  }
+  if (int e = sync(c)) return e;
+  c->prepared = false;  // the next full run starts from a cleared Qeff again
+  return RPA_OK;
+}
+
+int rpa_get_result(rpa_ctx *c, int which, double *out) {
+  if (!c || !out) return RPA_E_ARG;
+  if (!c->d_s0) return fail(c, RPA_E_STATE, "nothing computed yet");
+  (void)hipSetDevice(c->d.device);
+  if (int e = sync(c)) return e;
+  const size_t nf = (size_t)(c->d.no - 1);
+  const bool sloss = c->d.program == RPA_SLOSS;
+  switch (which) {
+    case RPA_RES_S0: return get_s0(c, out);
+    case RPA_RES_QEFF:
+      if (sloss) break;
+      return get_square(c, (const double *)c->d_qeff, 2, out);
+    case RPA_RES_CHI0: return copy_freq_mats(c, c->d_chi0, out);
+    case RPA_RES_V:
+      if (!sloss) break;
+      return get_square(c, (const double *)c->d_V, 1, out);
+    case RPA_RES_EPSINV:
+      if (!sloss) break;
+      return copy_freq_mats(c, c->d_epsinv, out);
+    case RPA_RES_CHI:
+      if (!sloss) break;
+      return copy_freq_mats(c, c->d_chi, out);
+    case RPA_RES_W:
+      if (!sloss) break;
+      return copy_freq_mats(c, c->d_W, out);
+    case RPA_RES_LOSS:
+      if (!sloss) break;
+      return d2h(c, out, (const double *)c->d_loss, nf);
+    case RPA_RES_PI11:
+      if (sloss) break;
+      return d2h(c, out, (const double *)c->d_pi11, 2 * nf);
+    case RPA_RES_SIG:
+      if (sloss) break;
+      return get_sig(c, out);
+    default: return fail(c, RPA_E_ARG, "unknown result selector");
+  }
+  return fail(c, RPA_E_STATE, "result does not belong to this program");
+}
+
+int rpa_w_kk(rpa_ctx *c, double *w1, double *w2) {
+  if (!c || !w1 || !w2) return RPA_E_ARG;
+  if (c->d.program != RPA_SLOSS) return fail(c, RPA_E_STATE, "W belongs to the Sloss path");
+  if (!c->d_W) return fail(c, RPA_E_STATE, "rpa_run or rpa_dyson_w must precede rpa_w_kk");
+  (void)hipSetDevice(c->d.device);
+  if (int e = launch_w_kk(c)) return e;
+  const size_t nf = (size_t)(c->d.no - 1);
+  if (int e = d2h(c, w1, (const double *)c->d_wkk, nf)) return e;
+  return d2h(c, w2, (const double *)c->d_wkk + nf, nf);
 }
 
 int rpa_set_profiling(rpa_ctx *c, int on) {

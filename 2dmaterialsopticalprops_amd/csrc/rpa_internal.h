@@ -122,6 +122,8 @@ struct rpa_ctx {
   int ng0 = 0;
   int *d_g0keys = nullptr, *d_g0idx = nullptr, *d_dmap = nullptr;
   size_t cap_dmap = 0;
+  double *d_wkk = nullptr;  // W1(no-1), W2(no-1) of the W Kramers-Kronig check
+  size_t cap_wkk = 0;
 
   // transitions
   rpa::PairTable pairs{};
@@ -212,6 +214,7 @@ int launch_build_kbox(rpa_ctx *c);
 int launch_transitions(rpa_ctx *c);
 int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm);
 int launch_direct_map(rpa_ctx *c);
+int launch_w_kk(rpa_ctx *c);
 int launch_accumulate(rpa_ctx *c);
 int launch_kstencil(rpa_ctx *c);
 int launch_kk(rpa_ctx *c);

@@ -336,14 +336,14 @@ def write_config(path: str, sys: SynthSystem, program: str, savedir: str,
  pol = '{x.get("pol", "xx")}'
  calc = {x.get("calc", 3)}
  jump = 1
- omin = {x.get("omin", 0.0):.10e}
- omax = {cfg.omax_ev:.10e}
+ omin = {x.get("omin", 0.0):.17e}
+ omax = {cfg.omax_ev:.17e}
  qmin = {cfg.iq}
  qmax = {cfg.iq}
- eta_intra = {x.get("eta_intra", 0.01):.10e}
- eta_inter = {cfg.eta_ev:.10e}
- Lor_cut = {x.get("lor_cut", 1.0e-3):.10e}
- df_cut = {x.get("df_cut", 1.0e-3):.10e}
+ eta_intra = {x.get("eta_intra", 0.01):.17e}
+ eta_inter = {cfg.eta_ev:.17e}
+ Lor_cut = {x.get("lor_cut", 1.0e-3):.17e}
+ df_cut = {x.get("df_cut", 1.0e-3):.17e}
  vertex_map = '{x.get("vertex_map", "direct")}'
 /
 &config
@@ -358,13 +358,13 @@ def write_config(path: str, sys: SynthSystem, program: str, savedir: str,
 /
 &parameters
  Efermi = 0.0
- dGW = {x.get("dgw", 0.0):.10e}
+ dGW = {x.get("dgw", 0.0):.17e}
  eps = 1.0e-4
- T = {x.get("T", 0.0):.10e}
- Ecut = {ecut_lf:.10e}
- a0 = {cfg.a:.10e}
- c0 = {cfg.c:.10e}
- Vcell = {sys.vcell:.10e}
+ T = {x.get("T", 0.0):.17e}
+ Ecut = {ecut_lf:.17e}
+ a0 = {cfg.a:.17e}
+ c0 = {cfg.c:.17e}
+ Vcell = {sys.vcell:.17e}
 /
 &parallel
  Nthreads = 1
@@ -399,13 +399,13 @@ def write_config(path: str, sys: SynthSystem, program: str, savedir: str,
 /
 &parameters
  Efermi = 0.0
- a0 = {cfg.a:.10e}
- c0 = {cfg.c:.10e}
+ a0 = {cfg.a:.17e}
+ c0 = {cfg.c:.17e}
  eps = 1.0e-4
  T = 0.0
- eta = {cfg.eta_ev:.10e}
- Ecut = {ecut_lf:.10e}
- Vcell = {sys.vcell:.10e}
+ eta = {cfg.eta_ev:.17e}
+ Ecut = {ecut_lf:.17e}
+ Vcell = {sys.vcell:.17e}
 /
 &parallel
  Nthreads = 1

@@ -90,6 +90,21 @@ def cpu_baseline(sys_, r, threads, target_s=12.0):
     }
 
 
+def hbm_calibration(mib):
+    """A plain copy of known size through one elementwise kernel (read n bytes,
+    write n bytes), three times, so HBM counters can be scaled against it."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"[bench] hbm calibration: 3 x elementwise copy of {mib} MiB")
+    del src, dst
+    torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,12 +115,17 @@ def main():
     ap.add_argument("--vertex-map", default="direct", choices=["direct", "igk"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy check")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--hbm-calibration-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (x1.0 elementwise kernel), "
+                         "the reference point for FETCH_SIZE / WRITE_SIZE counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
               "(q points would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.hbm_calibration_mib > 0:
+        hbm_calibration(args.hbm_calibration_mib)
     synth = importlib.import_module("2dmaterialsopticalprops_amd.synth")
     host = importlib.import_module("2dmaterialsopticalprops_amd.host")
     cfg = synth.CONFIGS[args.config]

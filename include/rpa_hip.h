@@ -52,6 +52,20 @@ enum rpa_stencil {
   RPA_KK_SLOSS = 2   /* genChi0, Sloss.f90:954-1059                          */
 };
 
+/* Results resident on the device after rpa_run, for rpa_get_result. */
+enum rpa_result {
+  RPA_RES_S0 = 0,     /* Sloss S0(no,Nlf,Nlf) real; Optabs S0(-no:no,Nlf,Nlf) complex */
+  RPA_RES_QEFF = 1,   /* Optabs Qeff(Nlf,Nlf) complex                          */
+  RPA_RES_CHI0 = 2,   /* chi0 (Sloss) / Pi0 (Optabs), (no-1,Nlf,Nlf) complex     */
+  RPA_RES_V = 3,      /* Sloss V(Nlf,Nlf) real                                  */
+  RPA_RES_EPSINV = 4, /* Sloss eps^-1, (no-1,Nlf,Nlf) complex                   */
+  RPA_RES_CHI = 5,    /* Sloss chi, (no-1,Nlf,Nlf) complex                      */
+  RPA_RES_W = 6,      /* Sloss WT, (no-1,Nlf,Nlf) complex                       */
+  RPA_RES_LOSS = 7,   /* Sloss -Im eps^-1(1,1), (no-1) real                     */
+  RPA_RES_PI11 = 8,   /* Optabs Pi_tot(1,1), (no-1) complex                     */
+  RPA_RES_SIG = 9     /* Optabs (no-1,2): fort.401 and fort.402 values           */
+};
+
 /* Sizes fixed for the lifetime of a context. */
 typedef struct rpa_dims {
   int nki;       /* k-points with wavefunctions (NkI; == Ntot with Nsym = 1)   */
@@ -151,7 +165,7 @@ int rpa_dyson_w(rpa_ctx *ctx, double *epsinv_out, double *chi_out, double *W_out
 /* Epilogue scalars per io = 1..no-1:
  *   loss(no-1)   = -Im eps^-1(1,1)                      (Sloss path)
  *   pi11(no-1)   = Pi_tot(1,1) complex                   (Optabs.f90:575-603)
- *   sig(2,no-1)  = Re(-i c0 Pi_inter/oi), Re(-i c0 Pi_intra/oi) (609-610)
+ *   sig(no-1,2)  = Re(-i c0 Pi_inter/oi), Re(-i c0 Pi_intra/oi) (609-610)
  * Any pointer may be NULL; quantities of the other program are ignored.    */
 int rpa_epilogue(rpa_ctx *ctx, double *loss, double *pi11, double *sig);
 
@@ -161,6 +175,25 @@ int rpa_epilogue(rpa_ctx *ctx, double *loss, double *pi11, double *sig);
  * replays it.                                                              */
 int rpa_run(rpa_ctx *ctx);
 int rpa_run_graph(rpa_ctx *ctx);
+
+/* Copies a result of the last rpa_run (or of the stage calls) to the host
+ * without recomputing it; out must hold the size given by enum rpa_result.
+ * Replaces the reference's in-memory reads of S0/Qeff/WT/V for its output
+ * files (Optabs.f90:480-510, Sloss.f90:474-489, 1572-1601).                 */
+int rpa_get_result(rpa_ctx *ctx, int which, double *out);
+
+/* Places a host S0 (reference layout, as rpa_get_result RPA_RES_S0) and, for
+ * Optabs, Qeff(Nlf,Nlf) (may be NULL: zero) on the device as the input of
+ * rpa_kk / rpa_epilogue: the calc = 2 path that reads Corrfun_{pol} and
+ * Qeff_{pol} back from the text files (Optabs.f90:513-547).  rpa_set_q must
+ * have been called (the reference sets q before reading, Optabs.f90:291-296). */
+int rpa_load_s0(rpa_ctx *ctx, const double *S0, const double *qeff);
+
+/* Kramers-Kronig check of the screened interaction (Sloss.f90:489-534 with
+ * genW1W2, Sloss.f90:1062-1171) at (G,G') = (1,1): S_W = -Im WT/pi over
+ * io = 1..no-1 (S0(no,1,1) unchanged), w1(no-1) and w2(no-1) as genW1W2
+ * returns them.  Needs the W of the last run.                               */
+int rpa_w_kk(rpa_ctx *ctx, double *w1, double *w2);
 
 /* Diagnostics: vertices of every transition in natural order
  * t = ((k*Npairs) + pair) over the program's band-pair blocks, (Nlf, ntrans)

@@ -170,3 +170,77 @@ def test_rerun_is_deterministic():
     np.testing.assert_array_equal(a["S0"], b["S0"])
     np.testing.assert_array_equal(a["W"], b["W"])
     r.close()
+
+
+def test_get_result_matches_stage_outputs_sloss():
+    s = H.system("tiny")
+    r = _runner(s, program="sloss")
+    out = r.full()
+    r.ctx.run()
+    for name, key in (("S0", "S0"), ("CHI0", "chi0"), ("V", "V"), ("EPSINV", "epsinv"),
+                      ("CHI", "chi"), ("W", "W"), ("LOSS", "loss")):
+        np.testing.assert_array_equal(r.ctx.result(name), out[key])
+    r.close()
+
+
+def test_get_result_matches_stage_outputs_optabs():
+    s = H.system("tiny")
+    r = _runner(s, program="optabs", temp_ev=0.5)
+    out = r.full()
+    r.ctx.run()
+    np.testing.assert_array_equal(r.ctx.result("S0"), out["S0"])
+    np.testing.assert_array_equal(r.ctx.result("QEFF"), out["Qeff"])
+    np.testing.assert_array_equal(r.ctx.result("CHI0"), out["chi0"])
+    np.testing.assert_array_equal(r.ctx.result("PI11"), out["pi11"])
+    np.testing.assert_array_equal(r.ctx.result("SIG").T, out["sig"])
+    r.close()
+
+
+@pytest.mark.parametrize("program", ["sloss", "optabs"])
+def test_load_s0_then_kk_matches_oracle(program):
+    """The calc = 2 path: an S0 (here the oracle's, quantised like the
+    Corrfun text file) is placed on the device and transformed there."""
+    s = H.system("tiny")
+    cfg = s.cfg
+    r = _runner(s, program=program, temp_ev=0.5 if program == "optabs" else 0.0)
+    pb = _oracle_problem(s, r, 0)
+    if program == "sloss":
+        S0 = orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, cfg.eta_ev / H.HARTREE, s.vcell)
+        Q = None
+        ref = orc.kk_sloss(S0, r.domega)
+    else:
+        S0, Q = orc.optabs_s0(pb, cfg.nval, cfg.no, 1, r.domega, cfg.eta_ev / H.HARTREE, r.rc.lor_cut,
+                              r.rc.df_cut, 0.0, 0.5 / H.HARTREE, s.vcell, s.gcar)
+        S0 = np.round(S0.real, 10) + 1j * np.round(S0.imag, 10)
+        Q = np.round(Q.real, 10) + 1j * np.round(Q.imag, 10)
+        ref = orc.kk_optabs(S0, r.domega)
+    r.ctx.load_s0(S0, Q)
+    chi0 = r.ctx.kk()
+    assert H.rel_max(chi0, ref) < 1e-12
+    if program == "optabs":
+        _, pi11, sig = r.ctx.epilogue()
+        p11, si, sa = orc.pi_assembly(ref, Q, r.domega, r.rc.eta_intra_ev / H.HARTREE, cfg.c)
+        assert H.rel_max(pi11, p11) < 1e-12
+        assert H.rel_max(sig[1], sa) < 1e-12
+    # a full run afterwards starts again from its own S0 and a cleared Qeff
+    r.ctx.run()
+    np.testing.assert_array_equal(r.ctx.result("S0").shape, S0.shape)
+    r.close()
+
+
+def test_w_kk_check_matches_oracle_genw1w2():
+    s = H.system("tiny")
+    cfg = s.cfg
+    r = _runner(s, program="sloss")
+    r.ctx.run()
+    W = r.ctx.result("W")
+    S0 = r.ctx.result("S0")
+    w1, w2 = r.ctx.w_kk()
+    # Sloss.f90:489: S0(1:no-1) = -(1/pi) Im WT; S0(no) keeps its value
+    SW = S0.copy()
+    SW[:cfg.no - 1] = -(1.0 / np.pi) * W.imag
+    for io in range(1, cfg.no):
+        a, b = orc.w1w2(io, 1, 1, SW, r.domega)
+        assert w1[io - 1] == pytest.approx(a, rel=1e-12, abs=1e-12 * np.abs(w1).max())
+        assert w2[io - 1] == pytest.approx(b, rel=1e-12, abs=1e-12 * np.abs(w2).max())
+    r.close()
