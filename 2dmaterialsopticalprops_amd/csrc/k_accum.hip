@@ -19,6 +19,11 @@
 //  5. k_reduce        fixed-order Neumaier sum of the chunk partials of each
 //                     tile -> S0 (deterministic, FP64).
 //  6. k_qeff          Optabs intraband Qeff (Optabs.f90:430-442).
+//
+// Mixed precision replaces step 4 by k_accum_mx: the FP64 weights times the
+// FP32 vertices, split into BF16 hi + lo parts, on the BF16 matrix cores as
+// hi*hi + hi*lo + lo*hi with FP32 accumulation over a chunk of kChunkTMx
+// transitions; step 5 still sums the chunks in FP64 with compensation.
 #include "rpa_internal.h"
 
 #include <algorithm>
@@ -369,6 +374,160 @@ __global__ __launch_bounds__(256) void k_accum(AccArgs a) {
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Mixed precision: S0 tile (32 frequencies x all G') of one local field G per
+// GEMM,  S0(w, G, G') += sum_(t,c) A_G[w][(t,c)] B[(t,c)][G'],
+//   A_G = W(w,t) x_t(G)_c       (x = rho1, c = Re/Im; W from FP64)
+//   B   = y_t(G')_c             (Re S0: y = rho2;  Im S0 (Optabs):
+//                                B' = (-Im y, Re y) for c = (Re, Im))
+// with K = 2 x kMxKS = 32 per MFMA, each operand split as hi + lo BF16.
+constexpr int kMxKS = 16;   // transitions per stage (K = 32)
+constexpr int kMxRow = 40;  // bf16 per staged B row (32 + pad, 80 bytes)
+
+struct AccMxArgs {
+  const double *te;
+  const float2 *rho1, *rho2;
+  const int *items;
+  const int *nitems;
+  int nlf, nw, ncol, ncol_pad, npb, nib;
+  int program, shift;
+  double domega, eta, cut, den;
+  double *part;
+};
+
+__device__ inline void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)v;
+  lo = (__bf16)(v - (float)hi);
+}
+
+// IPW local fields G per wave (4*IPW per workgroup), NCB blocks of 16 G'.
+template <bool CPLX, int NCB, int IPW>
+__global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
+  constexpr int NB = CPLX ? 2 : 1;  // B operands (Re, Im)
+  constexpr int NJ = 16 * NCB;
+  __shared__ float sW[kTileW][kMxKS];
+  __shared__ float2 sX[kMxKS][kMaxNlf];
+  __shared__ __align__(16) __bf16 sB[NB][2][NJ][kMxRow];  // [re/im][hi/lo][G'][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  const int total = a.nitems[0] * a.nib;
+
+  for (int wk = blockIdx.x; wk < total; wk += gridDim.x) {
+    const int item = wk / a.nib, ib = wk - item * a.nib;
+    const int b = a.items[3 * item], t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
+    const int i0 = ib * (4 * IPW) + wave * IPW;
+
+    rpa_f4 acc[NB][IPW][2][NCB];
+#pragma unroll
+    for (int s = 0; s < NB; ++s)
+#pragma unroll
+      for (int q = 0; q < IPW; ++q)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < NCB; ++cb) acc[s][q][rb][cb] = rpa_f4{0.f, 0.f, 0.f, 0.f};
+
+    for (int t0 = t_lo; t0 < t_hi; t0 += kMxKS) {
+      __syncthreads();
+      for (int x = tid; x < kTileW * kMxKS; x += 256) {
+        const int row = x / kMxKS, kt = x - row * kMxKS;
+        const int t = t0 + kt, w = b * kTileW + row;
+        double v = 0.0;
+        if (t < t_hi && w < a.nw) {
+          const double *e = a.te + 3 * (size_t)t;
+          v = weight(a.program, w, a.shift, a.domega, a.eta, a.cut, a.den, e[0], e[1], e[2]);
+        }
+        sW[row][kt] = (float)v;
+      }
+      for (int x = tid; x < kMxKS * a.nlf; x += 256) {
+        const int kt = x / a.nlf, g = x - kt * a.nlf;
+        sX[kt][g] = (t0 + kt < t_hi) ? a.rho1[(size_t)t0 * a.nlf + x] : make_float2(0.f, 0.f);
+      }
+      for (int x = tid; x < kMxKS * NJ; x += 256) {
+        const int kt = x / NJ, j = x - kt * NJ;
+        float2 y = make_float2(0.f, 0.f);
+        if (t0 + kt < t_hi && j < a.nlf) y = a.rho2[(size_t)(t0 + kt) * a.nlf + j];
+        split_bf16(y.x, sB[0][0][j][2 * kt], sB[0][1][j][2 * kt]);
+        split_bf16(y.y, sB[0][0][j][2 * kt + 1], sB[0][1][j][2 * kt + 1]);
+        if (CPLX) {
+          split_bf16(-y.y, sB[NB - 1][0][j][2 * kt], sB[NB - 1][1][j][2 * kt]);
+          split_bf16(y.x, sB[NB - 1][0][j][2 * kt + 1], sB[NB - 1][1][j][2 * kt + 1]);
+        }
+      }
+      __syncthreads();
+      // B fragments: column G' = 16 cb + r, K slice 8 kg .. 8 kg + 7
+      rpa_bf8 bh[NB][NCB], bl[NB][NCB];
+#pragma unroll
+      for (int s = 0; s < NB; ++s)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+          bh[s][cb] = *(const rpa_bf8 *)&sB[s][0][16 * cb + r][8 * kg];
+          bl[s][cb] = *(const rpa_bf8 *)&sB[s][1][16 * cb + r][8 * kg];
+        }
+#pragma unroll
+      for (int q = 0; q < IPW; ++q) {
+        const int i = i0 + q;
+        if (i >= a.nlf) continue;  // uniform over the wave
+        float2 xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xv[u] = sX[4 * kg + u][i];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const float *wrow = &sW[16 * rb + r][4 * kg];
+          rpa_bf8 ah, al;
+#pragma unroll
+          for (int v = 0; v < 8; ++v) {
+            const float w = wrow[v >> 1];
+            const float xc = (v & 1) ? xv[v >> 1].y : xv[v >> 1].x;
+            __bf16 h, l;
+            split_bf16(w * xc, h, l);
+            ah[v] = h;
+            al[v] = l;
+          }
+#pragma unroll
+          for (int s = 0; s < NB; ++s)
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+              rpa_f4 c = acc[s][q][rb][cb];
+              c = mfma_bf16_16x16x32(ah, bh[s][cb], c);
+              c = mfma_bf16_16x16x32(ah, bl[s][cb], c);
+              c = mfma_bf16_16x16x32(al, bh[s][cb], c);
+              acc[s][q][rb][cb] = c;
+            }
+        }
+      }
+    }
+    // partials in the FP64 layout of k_accum: part[item][col/256][row][col%256]
+    double *dst = a.part + (size_t)item * a.npb * (kTileW * kColsWG);
+#pragma unroll
+    for (int q = 0; q < IPW; ++q) {
+      const int i = i0 + q;
+      if (i >= a.nlf) continue;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+          const int j = 16 * cb + r;
+          if (j >= a.nlf) continue;
+#pragma unroll
+          for (int s = 0; s < NB; ++s) {
+            const int col = CPLX ? 2 * (i * a.nlf + j) + s : i * a.nlf + j;
+            double *d = dst + (size_t)(col / kColsWG) * (kTileW * kColsWG) + col % kColsWG;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) d[(16 * rb + 4 * kg + e) * kColsWG] = (double)acc[s][q][rb][cb][e];
+          }
+        }
+    }
+    if (ib == 0) {  // padded columns stay zero
+      for (int x = tid; x < kTileW * (a.ncol_pad - a.ncol); x += 256) {
+        const int row = x / (a.ncol_pad - a.ncol), col = a.ncol + x % (a.ncol_pad - a.ncol);
+        dst[(size_t)(col / kColsWG) * (kTileW * kColsWG) + row * kColsWG + col % kColsWG] = 0.0;
+      }
+    }
+  }
+}
+
 // S0[w][col] = sum over the chunks of tile w/32 of their partials (Neumaier).
 __global__ void k_reduce(const double *part, const int *first_item, int npb, int nw, int ncol_pad,
                          double *s0) {
@@ -392,7 +551,8 @@ __global__ void k_reduce(const double *part, const int *first_item, int npb, int
 }
 
 // Qeff(iG,jG) = sum_intra 2 fact j1(iG) conj(j2(jG)) / (Ntot Vcell)
-__global__ void k_qeff(const double *te, const double2 *rho1, const double2 *rho2, const int *toff,
+template <class C2>
+__global__ void k_qeff(const double *te, const C2 *rho1, const C2 *rho2, const int *toff,
                        int ntiles, int nlf, double dq, double *qeff) {
 #pragma clang fp contract(off)
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,10 +561,11 @@ __global__ void k_qeff(const double *te, const double2 *rho1, const double2 *rho
   double sr = 0.0, si = 0.0;
   for (int t = toff[ntiles]; t < toff[ntiles + 1]; ++t) {
     double f = 2.0 * te[3 * (size_t)t + 2];
-    double2 x = rho1[(size_t)t * nlf + iG], y = rho2[(size_t)t * nlf + jG];
-    double xr = f * x.x, xi = f * x.y;
-    sr += (xr * y.x + xi * y.y) / dq;
-    si += (xi * y.x - xr * y.y) / dq;
+    const C2 x = rho1[(size_t)t * nlf + iG], y = rho2[(size_t)t * nlf + jG];
+    const double xr = f * (double)x.x, xi = f * (double)x.y;
+    const double yr = y.x, yi = y.y;
+    sr += (xr * yr + xi * yi) / dq;
+    si += (xi * yr - xr * yi) / dq;
   }
   qeff[2 * idx] = sr;
   qeff[2 * idx + 1] = si;
@@ -463,50 +624,108 @@ int launch_transitions(rpa_ctx *c) {
   sa.te = c->d_te;
   hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(256), 0, c->stream, sa);
   hipLaunchKernelGGL(k_worklist, dim3(1), dim3(1024), 0, c->stream, (const int *)c->d_toff,
-                     c->ntiles, c->span, kChunkT, c->d_items, c->d_first_item, c->d_nitems);
+                     c->ntiles, c->span, c->d.precision == RPA_MIXED ? kChunkTMx : kChunkT,
+                     c->d_items, c->d_first_item, c->d_nitems);
   RPA_HIP(c, hipGetLastError());
   return 0;
 }
 
+template <bool CPLX, int NCB, int IPW>
+static void launch_mx(rpa_ctx *c, const AccMxArgs &a, int grid) {
+  hipLaunchKernelGGL((k_accum_mx<CPLX, NCB, IPW>), dim3(grid), dim3(256), 0, c->stream, a);
+}
+
+template <bool CPLX>
+static int launch_mx_ncb(rpa_ctx *c, AccMxArgs &a, long long items) {
+  constexpr int IPW = CPLX ? 1 : 2;
+  const int ncb = (c->nlf + 15) / 16;
+  a.nib = (c->nlf + 4 * IPW - 1) / (4 * IPW);
+  const int grid = (int)std::min<long long>(std::max<long long>(items * a.nib, 1), 8192);
+  switch (ncb) {
+    case 1: launch_mx<CPLX, 1, IPW>(c, a, grid); break;
+    case 2: launch_mx<CPLX, 2, IPW>(c, a, grid); break;
+    case 3: launch_mx<CPLX, 3, IPW>(c, a, grid); break;
+    case 4: launch_mx<CPLX, 4, IPW>(c, a, grid); break;
+    case 5: launch_mx<CPLX, 5, IPW>(c, a, grid); break;
+    case 6: launch_mx<CPLX, 6, IPW>(c, a, grid); break;
+    default: return fail(c, RPA_E_LIMIT, "mixed precision accumulation supports Nlf <= 96");
+  }
+  return 0;
+}
+
 int launch_accumulate(rpa_ctx *c) {
-  AccArgs a{};
-  a.te = c->d_te;
-  a.rho1 = (const double2 *)c->d_rho1;
-  a.rho2 = (const double2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
-  a.items = c->d_items;
-  a.nitems = c->d_nitems;
-  a.npb = c->ncol_pad / kColsWG;
-  a.nlf = c->nlf;
-  a.nw = c->nw;
-  a.ncol = c->ncol;
-  a.ncol_pad = c->ncol_pad;
-  a.program = c->d.program;
-  a.shift = (c->d.program == RPA_OPTABS) ? c->d.no : 0;
-  a.domega = c->p.domega;
-  a.eta = c->p.eta;
-  a.cut = c->wcut;
-  a.den = (M_PI * (double)c->ntot) * c->p.vcell;
-  a.part = c->d_part;
   const bool cplx = (c->d.program == RPA_OPTABS);
-  const size_t lds = sizeof(double) * kTileW * (kKS + 1) + sizeof(double2) * kKS * c->nlf * (cplx ? 2 : 1);
-  long long work = (long long)c->max_items * a.npb;
-  int grid = (int)std::min<long long>(std::max<long long>(work, 1), 4096);
-  if (cplx)
-    hipLaunchKernelGGL(k_accum<true>, dim3(grid), dim3(256), lds, c->stream, a);
-  else
-    hipLaunchKernelGGL(k_accum<false>, dim3(grid), dim3(256), lds, c->stream, a);
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  const int npb = c->ncol_pad / kColsWG;
+  const double den = (M_PI * (double)c->ntot) * c->p.vcell;
+  const int shift = cplx ? c->d.no : 0;
+  if (mixed) {
+    AccMxArgs a{};
+    a.te = c->d_te;
+    a.rho1 = (const float2 *)c->d_rho1;
+    a.rho2 = (const float2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
+    a.items = c->d_items;
+    a.nitems = c->d_nitems;
+    a.nlf = c->nlf;
+    a.nw = c->nw;
+    a.ncol = c->ncol;
+    a.ncol_pad = c->ncol_pad;
+    a.npb = npb;
+    a.program = c->d.program;
+    a.shift = shift;
+    a.domega = c->p.domega;
+    a.eta = c->p.eta;
+    a.cut = c->wcut;
+    a.den = den;
+    a.part = c->d_part;
+    const int e = cplx ? launch_mx_ncb<true>(c, a, (long long)c->max_items)
+                       : launch_mx_ncb<false>(c, a, (long long)c->max_items);
+    if (e) return e;
+  } else {
+    AccArgs a{};
+    a.te = c->d_te;
+    a.rho1 = (const double2 *)c->d_rho1;
+    a.rho2 = (const double2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
+    a.items = c->d_items;
+    a.nitems = c->d_nitems;
+    a.npb = npb;
+    a.nlf = c->nlf;
+    a.nw = c->nw;
+    a.ncol = c->ncol;
+    a.ncol_pad = c->ncol_pad;
+    a.program = c->d.program;
+    a.shift = shift;
+    a.domega = c->p.domega;
+    a.eta = c->p.eta;
+    a.cut = c->wcut;
+    a.den = den;
+    a.part = c->d_part;
+    const size_t lds = sizeof(double) * kTileW * (kKS + 1) + sizeof(double2) * kKS * c->nlf * (cplx ? 2 : 1);
+    long long work = (long long)c->max_items * a.npb;
+    int grid = (int)std::min<long long>(std::max<long long>(work, 1), 4096);
+    if (cplx)
+      hipLaunchKernelGGL(k_accum<true>, dim3(grid), dim3(256), lds, c->stream, a);
+    else
+      hipLaunchKernelGGL(k_accum<false>, dim3(grid), dim3(256), lds, c->stream, a);
+  }
   if (c->profiling) RPA_HIP(c, hipEventRecord(c->prof_ev[3], c->stream));
   const long long n = (long long)c->nw * c->ncol_pad;
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     (const double *)c->d_part, (const int *)c->d_first_item, a.npb, c->nw,
+                     (const double *)c->d_part, (const int *)c->d_first_item, npb, c->nw,
                      c->ncol_pad, c->d_s0);
   if (cplx) {
     const int nn = c->nlf * c->nlf;
-    hipLaunchKernelGGL(k_qeff, dim3((nn + 255) / 256), dim3(256), 0, c->stream,
-                       (const double *)c->d_te, (const double2 *)c->d_rho1,
-                       (const double2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1),
-                       (const int *)c->d_toff, c->ntiles, c->nlf,
-                       (double)c->ntot * c->p.vcell, c->d_qeff);
+    const dim3 g((nn + 255) / 256);
+    if (mixed)
+      hipLaunchKernelGGL(k_qeff<float2>, g, dim3(256), 0, c->stream, (const double *)c->d_te,
+                         (const float2 *)c->d_rho1, (const float2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1),
+                         (const int *)c->d_toff, c->ntiles, c->nlf, (double)c->ntot * c->p.vcell,
+                         c->d_qeff);
+    else
+      hipLaunchKernelGGL(k_qeff<double2>, g, dim3(256), 0, c->stream, (const double *)c->d_te,
+                         (const double2 *)c->d_rho1, (const double2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1),
+                         (const int *)c->d_toff, c->ntiles, c->nlf, (double)c->ntot * c->p.vcell,
+                         c->d_qeff);
   }
   RPA_HIP(c, hipGetLastError());
   return 0;

@@ -19,6 +19,9 @@
 // rows 8..15 give Im -- the K index is (i, re/im), columns are 16 bands m:
 //   A[n][(i,0)] =  C1r   A[n][(i,1)] = C1i   A[8+n][(i,0)] = -C1i  A[8+n][(i,1)] = C1r
 //   B[(i,0)][m] =  C2r(map(i))*J           B[(i,1)][m] = C2i(map(i))*J
+// In mixed precision the same contraction runs in complex FP32 on the FP32
+// matrix cores (P = PrecF32); only the row order of A and of the result
+// differs, because the FP32 instruction returns rows 4*(l/16)+e.
 // A workgroup owns 4 G of one k (one wave each).  It walks the plane waves in
 // chunks: the C1 rows and the gathered C2 rows of chunk c+1 are loaded into
 // registers while chunk c runs on the matrix cores from LDS.  Workgroups of
@@ -41,7 +44,7 @@ struct VBlock {
 };
 
 struct VertexArgs {
-  const double2 *evc;
+  const void *evc;  // double2 (FP64) or float2 (mixed)
   const int *mill_k;
   const int *mill_g;
   const int16_t *kbox;
@@ -68,7 +71,38 @@ struct VertexArgs {
   int mb0, nmb;    // staged C2 bands [mb0, mb0+nmb)
   int kslots;      // ntot rounded up to a multiple of 8
   const int *perm;
-  double2 *rho1, *rho2;
+  void *rho1, *rho2;
+};
+
+// Precision traits of the vertex contraction.  arow_*: which band and part
+// row r of the A operand carries; out_*: for result pair p of lane group q4,
+// the band and the result elements holding (Re, Im).
+struct PrecF64 {
+  using real = double;
+  using cplx = double2;
+  using acc = rpa_d4;
+  static __device__ inline acc mma(real a, real b, acc c) { return mfma_f64_16x16x4(a, b, c); }
+  static __device__ inline int arow_band(int r) { return r & 7; }
+  static __device__ inline bool arow_im(int r) { return r >= 8; }
+  static __device__ inline int out_band(int q4, int p) { return 4 * p + q4; }
+  static __device__ inline int out_re(int p) { return p; }
+  static __device__ inline int out_im(int p) { return p + 2; }
+  static __device__ inline cplx make(real x, real y) { return make_double2(x, y); }
+  static __device__ inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+};
+
+struct PrecF32 {
+  using real = float;
+  using cplx = float2;
+  using acc = rpa_f4;
+  static __device__ inline acc mma(real a, real b, acc c) { return mfma_f32_16x16x4(a, b, c); }
+  static __device__ inline int arow_band(int r) { return r >> 1; }
+  static __device__ inline bool arow_im(int r) { return (r & 1) != 0; }
+  static __device__ inline int out_band(int q4, int p) { return 2 * q4 + p; }
+  static __device__ inline int out_re(int p) { return 2 * p; }
+  static __device__ inline int out_im(int p) { return 2 * p + 1; }
+  static __device__ inline cplx make(real x, real y) { return make_float2(x, y); }
+  static __device__ inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
 };
 
 // Target global index of G(i) + Glf(g) + G0 for every distinct G0 of the q.
@@ -94,16 +128,19 @@ __global__ void k_direct_map(const int *mill_g, const int *gbox, Box gb, const i
 // divmod(t / TPR, CH) for its G and columns m = t % TPR + TPR u, and the C1
 // row c = t / TPA, bands n = t % TPA + TPA u; the assignment is fixed for the
 // whole walk, so a chunk costs one map lookup per thread and no division.
-template <int KIND, int MAXB, bool DUAL, int CH>
-__global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
+template <class P, int KIND, int MAXB, bool DUAL, int CH>
+__global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
+  using real = typename P::real;
+  using cplx = typename P::cplx;
+  using acc_t = typename P::acc;
   constexpr int TPR = 256 / (4 * CH);  // threads per gathered row
   constexpr int TPA = 256 / CH;        // threads per C1 row
-  extern __shared__ double2 vsm[];
+  extern __shared__ __align__(16) unsigned char vsm_raw[];
   const int nna = a.nna, nmb = a.nmb;
-  double2 *sC1 = vsm;                           // [CH][nna]
-  double2 *sB = sC1 + CH * nna;                 // [4][CH][nmb]
-  double *sJ1 = (double *)(sB + 4 * CH * nmb);  // [4][CH]
-  double *sJ2 = sJ1 + 4 * CH;                   // [4][CH]
+  cplx *sC1 = (cplx *)vsm_raw;              // [CH][nna]
+  cplx *sB = sC1 + CH * nna;                // [4][CH][nmb]
+  real *sJ1 = (real *)(sB + 4 * CH * nmb);  // [4][CH]
+  real *sJ2 = sJ1 + 4 * CH;                 // [4][CH]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware placement: linear id L runs on XCD L % 8; all G groups of one
@@ -126,8 +163,9 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
   const int lim2 = a.ngk[K2];
   const int *src = direct ? a.mill_g : a.mill_k + (size_t)K1 * a.npwx;
   const int16_t *box2 = a.kbox + (size_t)K2 * a.kb.size();
-  const double2 *c1 = a.evc + (size_t)K1 * a.npwx * a.nband + a.na0;
-  const double2 *c2 = a.evc + (size_t)K2 * a.npwx * a.nband + a.mb0;
+  const cplx *evc = (const cplx *)a.evc;
+  const cplx *c1 = evc + (size_t)K1 * a.npwx * a.nband + a.na0;
+  const cplx *c2 = evc + (size_t)K2 * a.npwx * a.nband + a.mb0;
   const double *kv = a.ktot + 3 * k;
 
   // this thread's staging assignment
@@ -143,17 +181,17 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
   const int r = lane & 15, q4 = lane >> 4;
   const int ii = q4 >> 1, part = q4 & 1;
 
-  double2 pc1[kPfC1], pb[kPfB];
-  double pj1 = 0.0, pj2 = 0.0;
+  cplx pc1[kPfC1], pb[kPfB];
+  real pj1 = 0, pj2 = 0;
 
   auto prefetch = [&](int i0) {
     {
       const int i = i0 + arow;
-      const double2 *row = c1 + (size_t)i * a.nband;
+      const cplx *row = c1 + (size_t)i * a.nband;
 #pragma unroll
       for (int u = 0; u < kPfC1; ++u) {
         const int n = asub + TPA * u;
-        pc1[u] = (i < n1 && n < nna) ? row[n] : make_double2(0.0, 0.0);
+        pc1[u] = (i < n1 && n < nna) ? row[n] : P::make(0, 0);
       }
     }
     const int i = i0 + bc;
@@ -169,14 +207,14 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
         j = bi >= 0 ? (int)box2[bi] : -1;
       }
     }
-    const double2 *row = c2 + (size_t)(j >= 0 ? j : 0) * a.nband;
+    const cplx *row = c2 + (size_t)(j >= 0 ? j : 0) * a.nband;
 #pragma unroll
     for (int u = 0; u < kPfB; ++u) {
       const int m = bsub + TPR * u;
-      pb[u] = (j >= 0 && m < nmb) ? row[m] : make_double2(0.0, 0.0);
+      pb[u] = (j >= 0 && m < nmb) ? row[m] : P::make(0, 0);
     }
     if (KIND >= 1 && bsub == 0) {
-      pj1 = pj2 = 0.0;
+      pj1 = pj2 = 0;
       if (bg_ok && i < n1) {
         int s0, s1, s2;
         mill_unpack(src[i], s0, s1, s2);
@@ -184,12 +222,12 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
         const double jb1 = (a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * bg + a.pol1];
         const double g1 = ((0.0 + a.kc[a.pol1 + 0] * (double)s0) + a.kc[a.pol1 + 3] * (double)s1) +
                           a.kc[a.pol1 + 6] * (double)s2;
-        pj1 = 0.5 * ((jb1 + 2.0 * g1) * a.gcar);
+        pj1 = (real)(0.5 * ((jb1 + 2.0 * g1) * a.gcar));
         if (KIND == 2) {
           const double jb2 = (a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * bg + a.pol2];
           const double g2 = ((0.0 + a.kc[a.pol2 + 0] * (double)s0) + a.kc[a.pol2 + 3] * (double)s1) +
                             a.kc[a.pol2 + 6] * (double)s2;
-          pj2 = 0.5 * ((jb2 + 2.0 * g2) * a.gcar);
+          pj2 = (real)(0.5 * ((jb2 + 2.0 * g2) * a.gcar));
         }
       }
     }
@@ -212,36 +250,37 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
     }
   };
 
-  rpa_d4 acc1[MAXB], acc2[MAXB], bcc1[MAXB], bcc2[MAXB];
+  acc_t acc1[MAXB], acc2[MAXB], bcc1[MAXB], bcc2[MAXB];
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
-    acc1[b] = acc2[b] = bcc1[b] = bcc2[b] = rpa_d4{0.0, 0.0, 0.0, 0.0};
+    acc1[b] = acc2[b] = bcc1[b] = bcc2[b] = P::zero();
   }
 
   // one K step (two plane waves) of every band block into accumulators x1/x2
-  auto step = [&](int c, rpa_d4 *x1, rpa_d4 *x2) {
-    const double wj1 = (KIND >= 1) ? sJ1[wave * CH + c] : 1.0;
-    const double wj2 = (KIND == 2) ? sJ2[wave * CH + c] : 1.0;
-    const double2 *ar = sC1 + c * nna;
-    const double2 *br = sB + (wave * CH + c) * nmb;
+  const int nr = P::arow_band(r);
+  const bool im_row = P::arow_im(r);
+  auto step = [&](int c, acc_t *x1, acc_t *x2) {
+    const real wj1 = (KIND >= 1) ? sJ1[wave * CH + c] : real(1);
+    const real wj2 = (KIND == 2) ? sJ2[wave * CH + c] : real(1);
+    const cplx *ar = sC1 + c * nna;
+    const cplx *br = sB + (wave * CH + c) * nmb;
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
       if (b < a.nvb) {
         const VBlock vb = a.vb[b];
-        const int nr = r & 7;
-        double av = 0.0;
+        real av = 0;
         if (nr < vb.nrows) {
-          const double2 z = ar[vb.n0 - a.na0 + nr];
-          // rows 0..7: (C1r, C1i); rows 8..15: (-C1i, C1r)
-          av = (r < 8) ? (part ? z.y : z.x) : (part ? z.x : -z.y);
+          const cplx z = ar[vb.n0 - a.na0 + nr];
+          // Re rows: (C1r, C1i); Im rows: (-C1i, C1r)
+          av = im_row ? (part ? z.x : -z.y) : (part ? z.y : z.x);
         }
-        double bv = 0.0;
+        real bv = 0;
         if (r < vb.ncols) {
-          const double2 z = br[vb.m0 - a.mb0 + r];
+          const cplx z = br[vb.m0 - a.mb0 + r];
           bv = part ? z.y : z.x;
         }
-        x1[b] = mfma_f64_16x16x4(av, bv * wj1, x1[b]);
-        if (KIND == 2) x2[b] = mfma_f64_16x16x4(av, bv * wj2, x2[b]);
+        x1[b] = P::mma(av, bv * wj1, x1[b]);
+        if (KIND == 2) x2[b] = P::mma(av, bv * wj2, x2[b]);
       }
     }
   };
@@ -273,8 +312,8 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
     }
   }
 
-  // result element e of lane (q4, r) is row 4e + q4: e = 0,1 give Re of band
-  // n = 4e + q4, e = 2,3 the matching Im, all in the same lane
+  // the two (Re, Im) result pairs of this lane: band out_band(q4, p), column r
+  cplx *rho1 = (cplx *)a.rho1, *rho2 = (cplx *)a.rho2;
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     if (b < a.nvb) {
@@ -282,17 +321,18 @@ __global__ __launch_bounds__(256) void k_vertex_f64(VertexArgs a) {
       const PairBlock &pbk = a.pairs.b[vb.pb];
       const int nm = pbk.m_hi - pbk.m_lo;
 #pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const int nr = 4 * e + q4;
-        if (nr < vb.nrows && r < vb.ncols) {
-          const int n = vb.n0 + nr, m = vb.m0 + r;
+      for (int p = 0; p < 2; ++p) {
+        const int onr = P::out_band(q4, p);
+        if (onr < vb.nrows && r < vb.ncols) {
+          const int n = vb.n0 + onr, m = vb.m0 + r;
           const long long t = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
                               (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
           const long long pos = a.perm ? (long long)a.perm[t] : t;
           if (pos >= 0) {
-            a.rho1[pos * a.nlf + iG] = make_double2(acc1[b][e], acc1[b][e + 2]);
-            if (KIND == 2) a.rho2[pos * a.nlf + iG] = make_double2(acc2[b][e], acc2[b][e + 2]);
-            else if (KIND == 1 && a.rho2) a.rho2[pos * a.nlf + iG] = make_double2(acc1[b][e], acc1[b][e + 2]);
+            const cplx v1 = P::make(acc1[b][P::out_re(p)], acc1[b][P::out_im(p)]);
+            rho1[pos * a.nlf + iG] = v1;
+            if (KIND == 2) rho2[pos * a.nlf + iG] = P::make(acc2[b][P::out_re(p)], acc2[b][P::out_im(p)]);
+            else if (KIND == 1 && rho2) rho2[pos * a.nlf + iG] = v1;
           }
         }
       }
@@ -338,31 +378,42 @@ int launch_direct_map(rpa_ctx *c) {
   return 0;
 }
 
-template <int KIND, int MAXB, bool DUAL>
+template <class P, int KIND, int MAXB, bool DUAL>
 static void launch_one(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
   if (a.ch == 32)
-    hipLaunchKernelGGL((k_vertex_f64<KIND, MAXB, DUAL, 32>), grid, dim3(256), lds, c->stream, a);
+    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 32>), grid, dim3(256), lds, c->stream, a);
   else if (a.ch == 16)
-    hipLaunchKernelGGL((k_vertex_f64<KIND, MAXB, DUAL, 16>), grid, dim3(256), lds, c->stream, a);
+    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 16>), grid, dim3(256), lds, c->stream, a);
   else
-    hipLaunchKernelGGL((k_vertex_f64<KIND, MAXB, DUAL, 8>), grid, dim3(256), lds, c->stream, a);
+    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 8>), grid, dim3(256), lds, c->stream, a);
 }
 
-template <int KIND>
+template <class P, int KIND>
 static void launch_kind(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
   if (a.nvb <= 1)
-    launch_one<KIND, 1, true>(c, a, grid, lds);
+    launch_one<P, KIND, 1, true>(c, a, grid, lds);
   else if (a.nvb <= 2)
-    launch_one<KIND, 2, true>(c, a, grid, lds);
+    launch_one<P, KIND, 2, true>(c, a, grid, lds);
   else if (a.nvb <= 4)
-    launch_one<KIND, 4, false>(c, a, grid, lds);
+    launch_one<P, KIND, 4, false>(c, a, grid, lds);
   else
-    launch_one<KIND, 8, false>(c, a, grid, lds);
+    launch_one<P, KIND, 8, false>(c, a, grid, lds);
+}
+
+template <class P>
+static void launch_prec(rpa_ctx *c, int kind, const VertexArgs &a, dim3 grid, size_t lds) {
+  if (kind == 0)
+    launch_kind<P, 0>(c, a, grid, lds);
+  else if (kind == 1)
+    launch_kind<P, 1>(c, a, grid, lds);
+  else
+    launch_kind<P, 2>(c, a, grid, lds);
 }
 
 int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   VertexArgs a{};
-  a.evc = (const double2 *)c->d_evc;
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  a.evc = mixed ? (const void *)c->d_evc32 : (const void *)c->d_evc;
   a.mill_k = c->d_mill_k;
   a.mill_g = c->d_mill_g;
   a.kbox = c->d_kbox;
@@ -425,9 +476,8 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   // chunk of 32, 16 or 8 plane waves: the staged bands of one row fit the
   // prefetch registers (256/CH threads per C1 row, 64/CH per gathered row)
   // and the staging fits in 64 KB of LDS
-  auto lds_of = [&](int ch) {
-    return sizeof(double2) * (size_t)ch * (a.nna + 4 * a.nmb) + sizeof(double) * 2 * 4 * ch;
-  };
+  const size_t rsz = mixed ? sizeof(float) : sizeof(double);
+  auto lds_of = [&](int ch) { return 2 * rsz * (size_t)ch * (a.nna + 4 * a.nmb) + rsz * 2 * 4 * ch; };
   auto fits = [&](int ch) {
     return a.nna <= (256 / ch) * kPfC1 && a.nmb <= (64 / ch) * kPfB && lds_of(ch) <= 64 * 1024;
   };
@@ -443,18 +493,16 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   a.groups = (c->nlf + 3) / 4;
   a.kslots = (c->ntot + 7) / 8 * 8;
   a.perm = perm;
-  a.rho1 = (double2 *)rho1;
-  a.rho2 = (double2 *)rho2;
+  a.rho1 = rho1;
+  a.rho2 = rho2;
   const long long nblocks = (long long)a.kslots * a.groups;
   if (nblocks <= 0) return 0;
   if (int e = launch_direct_map(c)) return e;
   dim3 grid((unsigned)nblocks);
-  if (kind == 0)
-    launch_kind<0>(c, a, grid, lds);
-  else if (kind == 1)
-    launch_kind<1>(c, a, grid, lds);
+  if (mixed)
+    launch_prec<PrecF32>(c, kind, a, grid, lds);
   else
-    launch_kind<2>(c, a, grid, lds);
+    launch_prec<PrecF64>(c, kind, a, grid, lds);
   RPA_HIP(c, hipGetLastError());
   return 0;
 }

@@ -105,7 +105,8 @@ static int prepare(rpa_ctx *c) {
   c->ntrans = (long long)c->ntot * c->pairs.tpk;
   const size_t nt = (size_t)std::max<long long>(c->ntrans, 1);
   const size_t nblk = (nt + 255) / 256;
-  c->max_items = (size_t)(c->span + 1) * ((nt + kChunkT - 1) / kChunkT) + c->ntiles + 1;
+  const size_t chunk = (c->d.precision == RPA_MIXED) ? kChunkTMx : kChunkT;
+  c->max_items = (size_t)(c->span + 1) * ((nt + chunk - 1) / chunk) + c->ntiles + 1;
   const bool two = (c->d.program == RPA_OPTABS && p.pol == RPA_POL_YZ);
   c->ncol = (c->d.program == RPA_OPTABS ? 2 : 1) * c->nlf * c->nlf;
   c->ncol_pad = (c->ncol + kColsWG - 1) / kColsWG * kColsWG;
@@ -123,8 +124,10 @@ static int prepare(rpa_ctx *c) {
   if (!e) e = ensure(c, c->d_items, c->cap_items, 3 * c->max_items);
   if (!e) e = ensure(c, c->d_first_item, c->cap_first, (size_t)c->ntiles + 1);
   if (!e) e = ensure(c, c->d_nitems, c->cap_nitems, 2);
-  if (!e) e = ensure(c, (double2 *&)c->d_rho1, c->cap_rho, nt * c->nlf);
-  if (!e && two) e = ensure(c, (double2 *&)c->d_rho2, c->cap_rho2, nt * c->nlf);
+  // vertices: complex FP64, or complex FP32 in mixed precision
+  const size_t rho_n = nt * c->nlf * (c->d.precision == RPA_MIXED ? 1 : 2);
+  if (!e) e = ensure(c, (float2 *&)c->d_rho1, c->cap_rho, rho_n);
+  if (!e && two) e = ensure(c, (float2 *&)c->d_rho2, c->cap_rho2, rho_n);
   if (!e && !two && c->d_rho2) {
     (void)hipFree(c->d_rho2);
     c->d_rho2 = nullptr;
@@ -279,10 +282,6 @@ int rpa_ctx_create(const rpa_dims *dims, rpa_ctx **out) {
   if (d.nlfd > kMaxNlf) return RPA_E_LIMIT;
   rpa_ctx *c = new rpa_ctx();
   c->d = d;
-  if (d.precision == RPA_MIXED) {
-    delete c;
-    return RPA_E_ARG;  // mixed precision path is selected per run, see rpa_params
-  }
   hipError_t e = hipSetDevice(d.device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -292,6 +291,10 @@ int rpa_ctx_create(const rpa_dims *dims, rpa_ctx **out) {
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_mill_k, sizeof(int) * nel);
   if (e == hipSuccess) e = hipMemset(c->d_mill_k, 0, sizeof(int) * nel);
   if (e == hipSuccess) e = hipMemset(c->d_evc, 0, sizeof(double2) * nel * d.nband);
+  if (e == hipSuccess && d.precision == RPA_MIXED) {
+    e = hipMalloc((void **)&c->d_evc32, sizeof(float2) * nel * d.nband);
+    if (e == hipSuccess) e = hipMemset(c->d_evc32, 0, sizeof(float2) * nel * d.nband);
+  }
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_ngk, sizeof(int) * d.nki);
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_E, sizeof(double) * d.nki * d.nband);
   if (e != hipSuccess) {
@@ -315,7 +318,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_nitems, c->d_first_item, c->d_intra, c->d_nintra, c->d_rho1, c->d_rho2,
                   c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig, c->d_scratch,
-                  c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk};
+                  c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (hipEvent_t ev : c->prof_ev)
@@ -396,6 +399,12 @@ int rpa_upload_basis(rpa_ctx *c, int ik, int ngk, const int *mill, const double 
     }
   RPA_HIP(c, hipMemcpy((double2 *)c->d_evc + (size_t)k * c->d.npwx * nb, rows.data(),
                        sizeof(double) * rows.size(), hipMemcpyHostToDevice));
+  if (c->d.precision == RPA_MIXED) {
+    std::vector<float> rows32(rows.size());
+    for (size_t x = 0; x < rows.size(); ++x) rows32[x] = (float)rows[x];
+    RPA_HIP(c, hipMemcpy(c->d_evc32 + (size_t)k * c->d.npwx * nb, rows32.data(),
+                         sizeof(float) * rows32.size(), hipMemcpyHostToDevice));
+  }
   RPA_HIP(c, hipMemcpy(c->d_mill_k + (size_t)k * c->d.npwx, keys.data(), sizeof(int) * c->d.npwx,
                        hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(c->d_ngk + k, &ngk, sizeof(int), hipMemcpyHostToDevice));
@@ -759,8 +768,18 @@ int rpa_get_vertices(rpa_ctx *c, double *rho1, double *rho2, long long *ntrans_o
   hipError_t he = hipSuccess;
   if (rho2) he = hipMalloc((void **)&t2, sizeof(double2) * std::max<size_t>(n, 1));
   int e = (he == hipSuccess) ? launch_vertices(c, t1, t2, nullptr) : hip_fail(c, he, "hipMalloc");
-  if (!e) e = d2h(c, rho1, (const double *)t1, 2 * n);
-  if (!e && rho2) e = d2h(c, rho2, (const double *)t2, 2 * n);
+  if (c->d.precision == RPA_MIXED) {  // complex FP32 results, widened here
+    std::vector<float> h(2 * n);
+    for (int s = 0; s < 2 && !e; ++s) {
+      double *dst = s ? rho2 : rho1;
+      if (!dst) continue;
+      e = d2h(c, h.data(), (const float *)(s ? t2 : t1), 2 * n);
+      for (size_t x = 0; x < 2 * n && !e; ++x) dst[x] = h[x];
+    }
+  } else {
+    if (!e) e = d2h(c, rho1, (const double *)t1, 2 * n);
+    if (!e && rho2) e = d2h(c, rho2, (const double *)t2, 2 * n);
+  }
   (void)hipFree(t1);
   if (t2) (void)hipFree(t2);
   return e;

@@ -58,7 +58,8 @@ struct PairBlock {
 constexpr int kMaxPairBlocks = 4;
 constexpr int kTileW = 32;     // frequency rows per accumulation tile
 constexpr int kColsWG = 256;   // S0 columns per accumulation work item
-constexpr int kChunkT = 4096;  // transitions per accumulation work item
+constexpr int kChunkT = 4096;    // transitions per accumulation work item (FP64)
+constexpr int kChunkTMx = 1024;  // the same for the mixed path (FP32 partial sums)
 constexpr int kMaxNlf = 128;   // local fields supported by the kernels
 
 struct PairTable {
@@ -89,6 +90,7 @@ struct rpa_ctx {
 
   // basis
   void *d_evc = nullptr;
+  float2 *d_evc32 = nullptr;  // mixed precision: FP32 copy of the basis
   int *d_mill_k = nullptr;
   int16_t *d_kbox = nullptr;
   int *d_ngk = nullptr;
@@ -233,5 +235,23 @@ typedef double rpa_d4 __attribute__((ext_vector_type(4)));
 // and B[l/16][l%16]; element e of lane l's result is D[4*e + l/16][l%16].
 __device__ inline rpa_d4 mfma_f64_16x16x4(double a, double b, rpa_d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+typedef float rpa_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 rpa_bf8 __attribute__((ext_vector_type(8)));
+
+// D(16x16) += A(16x4) B(4x16) in FP32.  Lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; element e of lane l's result is D[4*(l/16) + e][l%16]
+// (measured on gfx950; note the different row order from the FP64 form).
+__device__ inline rpa_f4 mfma_f32_16x16x4(float a, float b, rpa_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// D(16x16) += A(16x32) B(32x16), BF16 inputs, FP32 accumulation.  Lane l
+// supplies row l%16 of A and column l%16 of B, 8 values each for the K
+// slice 8*(l/16) .. 8*(l/16)+7 (the same slice in A and B); element e of
+// lane l's result is D[4*(l/16) + e][l%16] (measured on gfx950).
+__device__ inline rpa_f4 mfma_bf16_16x16x32(rpa_bf8 a, rpa_bf8 b, rpa_f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 #endif

@@ -34,6 +34,8 @@ class RunConfig:
     eps: float = 1.0e-4
     ngd: Optional[int] = None
     device: int = 0
+    precision: str = "fp64"          # 'fp64' | 'mixed'
+
 
 
 def energies(sys_, rc: RunConfig) -> np.ndarray:
@@ -88,7 +90,9 @@ class Runner:
         self.nlf, self.glf, self.parg, self.lf_idx = nlf, glf, parg, idx
         self.E = energies(sys_, rc)
         self.domega = frequency_step(cfg, rc)
-        self.ctx = _abi.Context(nki, cfg.nband, sys_.npwx, nlfd, cfg.no, prog, device=rc.device)
+        prec = _abi.RPA_MIXED if rc.precision == "mixed" else _abi.RPA_FP64
+        self.ctx = _abi.Context(nki, cfg.nband, sys_.npwx, nlfd, cfg.no, prog, precision=prec,
+                                device=rc.device)
         ctx = self.ctx
         ctx.set_gvectors(sys_.mill_g, sys_.kc)
         for ik in range(nki):
