@@ -237,6 +237,17 @@ int launch_coulomb(rpa_ctx *c) {
   return 0;
 }
 
+// Done once per context outside any stream capture: the in-LDS solve holds
+// Nlf^2 complex values, beyond the default 64 KB when Nlf > 64.
+int configure_dyson(rpa_ctx *c) {
+  const size_t lds = sizeof(double2) * c->nlf * c->nlf;
+  if (lds > 64 * 1024 && lds <= 160 * 1024 && !c->dyson_lds_set) {
+    RPA_HIP(c, hipFuncSetAttribute((const void *)k_dyson, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    c->dyson_lds_set = true;
+  }
+  return 0;
+}
+
 int launch_dyson(rpa_ctx *c) {
   DysonArgs a{};
   a.nlf = c->nlf;
@@ -250,6 +261,9 @@ int launch_dyson(rpa_ctx *c) {
   a.loss = c->d_loss;
   a.status = c->d_status;
   const size_t lds = sizeof(double2) * c->nlf * c->nlf;
+  if (lds > 160 * 1024) return fail(c, RPA_E_LIMIT, "Nlf too large for the in-LDS Dyson solve (Nlf <= 101)");
+  if (lds > 64 * 1024 && !c->dyson_lds_set)
+    return fail(c, RPA_E_STATE, "Dyson LDS limit not configured");
   hipLaunchKernelGGL(k_dyson, dim3(a.nfreq), dim3(256), lds, c->stream, a);
   RPA_HIP(c, hipGetLastError());
   return 0;

@@ -84,11 +84,18 @@ struct PrecF64 {
   static __device__ inline acc mma(real a, real b, acc c) { return mfma_f64_16x16x4(a, b, c); }
   static __device__ inline int arow_band(int r) { return r & 7; }
   static __device__ inline bool arow_im(int r) { return r >= 8; }
+  // staged row of band offset n (rows of all blocks: Re then Im halves of 8)
+  static __device__ inline int arow_index(int n, bool im) { return 2 * (n & ~7) + (im ? 8 : 0) + (n & 7); }
   static __device__ inline int out_band(int q4, int p) { return 4 * p + q4; }
   static __device__ inline int out_re(int p) { return p; }
   static __device__ inline int out_im(int p) { return p + 2; }
   static __device__ inline cplx make(real x, real y) { return make_double2(x, y); }
   static __device__ inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+  // four consecutive reals from 16-byte aligned LDS
+  static __device__ inline void load4(const real *p, real v[4]) {
+    const double2 x = *(const double2 *)p, y = *(const double2 *)(p + 2);
+    v[0] = x.x, v[1] = x.y, v[2] = y.x, v[3] = y.y;
+  }
 };
 
 struct PrecF32 {
@@ -98,11 +105,16 @@ struct PrecF32 {
   static __device__ inline acc mma(real a, real b, acc c) { return mfma_f32_16x16x4(a, b, c); }
   static __device__ inline int arow_band(int r) { return r >> 1; }
   static __device__ inline bool arow_im(int r) { return (r & 1) != 0; }
+  static __device__ inline int arow_index(int n, bool im) { return 2 * n + (im ? 1 : 0); }
   static __device__ inline int out_band(int q4, int p) { return 2 * q4 + p; }
   static __device__ inline int out_re(int p) { return 2 * p; }
   static __device__ inline int out_im(int p) { return 2 * p + 1; }
   static __device__ inline cplx make(real x, real y) { return make_float2(x, y); }
   static __device__ inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ inline void load4(const real *p, real v[4]) {
+    const float4 x = *(const float4 *)p;
+    v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w;
+  }
 };
 
 // Target global index of G(i) + Glf(g) + G0 for every distinct G0 of the q.
@@ -128,19 +140,35 @@ __global__ void k_direct_map(const int *mill_g, const int *gbox, Box gb, const i
 // divmod(t / TPR, CH) for its G and columns m = t % TPR + TPR u, and the C1
 // row c = t / TPA, bands n = t % TPA + TPA u; the assignment is fixed for the
 // whole walk, so a chunk costs one map lookup per thread and no division.
+//
+// The operands are staged in LDS already in matrix-core form, transposed so
+// that consecutive K steps of one lane are contiguous:
+//   sA[ii][part][row][s]      A value of plane wave c = 2s + ii (the sign and
+//                             Re/Im choice of the complex embedding applied)
+//   sB[w][ii][part][m][s]     Re or Im of the gathered C2 (times J for
+//                             current vertices)
+// so a lane loads four K steps of A and of B with two 16-byte reads.  Row
+// 2*nna (and column nmb) stay zero and serve the lanes outside a band block.
 template <class P, int KIND, int MAXB, bool DUAL, int CH>
 __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   using real = typename P::real;
   using cplx = typename P::cplx;
   using acc_t = typename P::acc;
-  constexpr int TPR = 256 / (4 * CH);  // threads per gathered row
-  constexpr int TPA = 256 / CH;        // threads per C1 row
+  constexpr int TPR = 256 / (4 * CH);      // threads per gathered row
+  constexpr int TPA = 256 / CH;            // threads per C1 row
+  constexpr int S = CH / 2;                // K steps per chunk
+  constexpr int SP = S + 16 / sizeof(real);  // padded step stride (16-byte multiple)
+  constexpr int NBS = (KIND == 2) ? 2 : 1;   // B sets (J1, J2)
   extern __shared__ __align__(16) unsigned char vsm_raw[];
   const int nna = a.nna, nmb = a.nmb;
-  cplx *sC1 = (cplx *)vsm_raw;              // [CH][nna]
-  cplx *sB = sC1 + CH * nna;                // [4][CH][nmb]
-  real *sJ1 = (real *)(sB + 4 * CH * nmb);  // [4][CH]
-  real *sJ2 = sJ1 + 4 * CH;                 // [4][CH]
+  const int arows = 2 * ((nna + 7) / 8) * 8 + 1, bcols = nmb + 1;  // + zero row / column
+  const int zrow = arows - 1;
+  real *sA = (real *)vsm_raw;                      // [2][2][arows][SP]
+  real *sB = sA + 4 * arows * SP;                  // [NBS][4][2][2][bcols][SP]
+  auto A_at = [&](int ii_, int part_, int row) { return sA + ((ii_ * 2 + part_) * arows + row) * SP; };
+  auto B_at = [&](int set, int w, int ii_, int part_, int m) {
+    return sB + ((((set * 4 + w) * 2 + ii_) * 2 + part_) * bcols + m) * SP;
+  };
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // XCD-aware placement: linear id L runs on XCD L % 8; all G groups of one
@@ -168,6 +196,13 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   const cplx *c2 = evc + (size_t)K2 * a.npwx * a.nband + a.mb0;
   const double *kv = a.ktot + 3 * k;
 
+  // zero rows / columns for masked lanes (never overwritten by staging)
+  for (int x = tid; x < 4 * SP; x += 256) A_at(x / SP / 2, (x / SP) & 1, zrow)[x % SP] = 0;
+  for (int x = tid; x < NBS * 16 * SP; x += 256) {
+    const int q = x / SP;  // (set, w, ii, part)
+    B_at(q >> 4, (q >> 2) & 3, (q >> 1) & 1, q & 1, nmb)[x % SP] = 0;
+  }
+
   // this thread's staging assignment
   const int brow = tid / TPR, bsub = tid % TPR;  // gathered row (w, c)
   const int bw = brow / CH, bc = brow % CH;
@@ -182,9 +217,12 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   const int ii = q4 >> 1, part = q4 & 1;
 
   cplx pc1[kPfC1], pb[kPfB];
-  real pj1 = 0, pj2 = 0;
+  real pj1 = 1, pj2 = 1;
 
-  auto prefetch = [&](int i0) {
+  // Loads of one chunk into registers: the C1 rows, the gathered k+q rows
+  // (row j of this thread's G, -1 = absent) and, for current vertices, J
+  // from the Miller key mkey of the plane wave.
+  auto prefetch = [&](int i0, int j, int mkey) {
     {
       const int i = i0 + arow;
       const cplx *row = c1 + (size_t)i * a.nband;
@@ -194,30 +232,17 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
         pc1[u] = (i < n1 && n < nna) ? row[n] : P::make(0, 0);
       }
     }
-    const int i = i0 + bc;
-    int j = -1;
-    if (bg_ok && i < n1) {
-      if (direct) {
-        const int p = dmap[i];
-        j = (p >= 0 && p < lim2) ? p : -1;
-      } else {
-        int s0, s1, s2;
-        mill_unpack(src[i], s0, s1, s2);
-        const int bi = a.kb.index(s0 + lf0 + d0, s1 + lf1 + d1, s2 + lf2 + d2);
-        j = bi >= 0 ? (int)box2[bi] : -1;
-      }
-    }
     const cplx *row = c2 + (size_t)(j >= 0 ? j : 0) * a.nband;
 #pragma unroll
     for (int u = 0; u < kPfB; ++u) {
       const int m = bsub + TPR * u;
       pb[u] = (j >= 0 && m < nmb) ? row[m] : P::make(0, 0);
     }
-    if (KIND >= 1 && bsub == 0) {
+    if (KIND >= 1) {
       pj1 = pj2 = 0;
-      if (bg_ok && i < n1) {
+      if (bg_ok && i0 + bc < n1) {
         int s0, s1, s2;
-        mill_unpack(src[i], s0, s1, s2);
+        mill_unpack(mkey, s0, s1, s2);
         // J = Gcar (q + 2k + Glf(G) + 2 G(i))_pol with G(i) = KC mill (left to right)
         const double jb1 = (a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * bg + a.pol1];
         const double g1 = ((0.0 + a.kc[a.pol1 + 0] * (double)s0) + a.kc[a.pol1 + 3] * (double)s1) +
@@ -233,20 +258,56 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
     }
   };
 
+  // Gather-index pipeline: the row gathered for chunk c+1 is known before
+  // its data loads issue, so they never wait on an index load.
+  //   direct: raw map value of chunk c+1 in jn (loaded while chunk c-1 ran)
+  //   igk:    Miller key of chunk c+2 in kn, its box lookup -> jn one chunk
+  //           later (two dependent loads, each hidden behind a chunk)
+  constexpr int kNone = -2147483647 - 1;
+  auto load_key = [&](int i) -> int {
+    return (bg_ok && i < n1) ? (direct ? dmap[i] : src[i]) : kNone;
+  };
+  auto lookup_igk = [&](int key) -> int {
+    if (key == kNone) return -1;
+    int s0, s1, s2;
+    mill_unpack(key, s0, s1, s2);
+    const int bi = a.kb.index(s0 + lf0 + d0, s1 + lf1 + d1, s2 + lf2 + d2);
+    return bi >= 0 ? (int)box2[bi] : -1;
+  };
+  auto row_of = [&](int jraw) -> int { return (jraw >= 0 && jraw < lim2) ? jraw : -1; };
+  auto load_mkey = [&](int i) -> int { return (KIND >= 1 && bg_ok && i < n1) ? src[i] : 0; };
+
+  // registers -> LDS in matrix-core form
   auto stage = [&]() {
+    {
+      const int ai = arow & 1, as = arow >> 1;
 #pragma unroll
-    for (int u = 0; u < kPfC1; ++u) {
-      const int n = asub + TPA * u;
-      if (n < nna) sC1[arow * nna + n] = pc1[u];
+      for (int u = 0; u < kPfC1; ++u) {
+        const int n = asub + TPA * u;
+        if (n < nna) {
+          const cplx z = pc1[u];
+          const int rre = P::arow_index(n, false), rim = P::arow_index(n, true);
+          // Re rows: (C1r, C1i); Im rows: (-C1i, C1r) for part (0, 1)
+          A_at(ai, 0, rre)[as] = z.x;
+          A_at(ai, 1, rre)[as] = z.y;
+          A_at(ai, 0, rim)[as] = -z.y;
+          A_at(ai, 1, rim)[as] = z.x;
+        }
+      }
     }
+    const int bi_ = bc & 1, bs = bc >> 1;
 #pragma unroll
     for (int u = 0; u < kPfB; ++u) {
       const int m = bsub + TPR * u;
-      if (m < nmb) sB[brow * nmb + m] = pb[u];
-    }
-    if (KIND >= 1 && bsub == 0) {
-      sJ1[brow] = pj1;
-      sJ2[brow] = pj2;
+      if (m < nmb) {
+        const cplx z = pb[u];
+        B_at(0, bw, bi_, 0, m)[bs] = z.x * pj1;
+        B_at(0, bw, bi_, 1, m)[bs] = z.y * pj1;
+        if (KIND == 2) {
+          B_at(NBS - 1, bw, bi_, 0, m)[bs] = z.x * pj2;
+          B_at(NBS - 1, bw, bi_, 1, m)[bs] = z.y * pj2;
+        }
+      }
     }
   };
 
@@ -255,52 +316,69 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   for (int b = 0; b < MAXB; ++b) {
     acc1[b] = acc2[b] = bcc1[b] = bcc2[b] = P::zero();
   }
-
-  // one K step (two plane waves) of every band block into accumulators x1/x2
-  const int nr = P::arow_band(r);
-  const bool im_row = P::arow_im(r);
-  auto step = [&](int c, acc_t *x1, acc_t *x2) {
-    const real wj1 = (KIND >= 1) ? sJ1[wave * CH + c] : real(1);
-    const real wj2 = (KIND == 2) ? sJ2[wave * CH + c] : real(1);
-    const cplx *ar = sC1 + c * nna;
-    const cplx *br = sB + (wave * CH + c) * nmb;
+  // per band block: this lane's A row and B column (zero row/column if masked)
+  const real *arowp[MAXB];
+  const real *bcolp[MAXB], *bcolp2[MAXB];
+  {
+    const int nr = P::arow_band(r);
+    const bool im_row = P::arow_im(r);
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
-      if (b < a.nvb) {
-        const VBlock vb = a.vb[b];
-        real av = 0;
-        if (nr < vb.nrows) {
-          const cplx z = ar[vb.n0 - a.na0 + nr];
-          // Re rows: (C1r, C1i); Im rows: (-C1i, C1r)
-          av = im_row ? (part ? z.x : -z.y) : (part ? z.y : z.x);
-        }
-        real bv = 0;
-        if (r < vb.ncols) {
-          const cplx z = br[vb.m0 - a.mb0 + r];
-          bv = part ? z.y : z.x;
-        }
-        x1[b] = P::mma(av, bv * wj1, x1[b]);
-        if (KIND == 2) x2[b] = P::mma(av, bv * wj2, x2[b]);
-      }
+      const VBlock vb = a.vb[b < a.nvb ? b : 0];
+      const int arix = (nr < vb.nrows) ? P::arow_index(vb.n0 - a.na0 + nr, im_row) : zrow;
+      const int bcix = (r < vb.ncols) ? vb.m0 - a.mb0 + r : nmb;
+      arowp[b] = A_at(ii, part, arix);
+      bcolp[b] = B_at(0, wave, ii, part, bcix);
+      bcolp2[b] = B_at(NBS - 1, wave, ii, part, bcix);
     }
-  };
+  }
 
-  prefetch(0);
+  int jn, kn = kNone, mkn;
+  {
+    const int key0 = load_key(bc);
+    const int j0 = row_of(direct ? key0 : lookup_igk(key0));
+    prefetch(0, j0, load_mkey(bc));
+    const int key1 = load_key(CH + bc);
+    jn = direct ? key1 : lookup_igk(key1);
+    mkn = load_mkey(CH + bc);
+    if (!direct) kn = load_key(2 * CH + bc);
+  }
   for (int i0 = 0; i0 < n1; i0 += CH) {
     __syncthreads();  // the previous chunk's LDS reads are done
     stage();
     __syncthreads();
-    if (i0 + CH < n1) prefetch(i0 + CH);
+    if (i0 + CH < n1) prefetch(i0 + CH, row_of(jn), mkn);
+    {
+      const int i2 = i0 + 2 * CH + bc;
+      if (direct) {
+        jn = load_key(i2);
+      } else {
+        jn = lookup_igk(kn);
+        kn = load_key(i2 + CH);
+      }
+      mkn = load_mkey(i2);
+    }
     if (active) {
-      const int cmax = min(CH, n1 - i0);
-      int c0 = 0;
-      if (DUAL) {
-        for (; c0 + 2 < cmax; c0 += 4) {
-          step(c0 + ii, acc1, acc2);
-          step(c0 + 2 + ii, bcc1, bcc2);
+      // four K steps per pass: two 16-byte reads per operand
+#pragma unroll
+      for (int s4 = 0; s4 < S; s4 += 4) {
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+          if (b < a.nvb) {
+            real av[4], bv[4], bw2[4];
+            P::load4(arowp[b] + s4, av);
+            P::load4(bcolp[b] + s4, bv);
+            if (KIND == 2) P::load4(bcolp2[b] + s4, bw2);
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+              acc_t *x1 = (DUAL && (x & 1)) ? bcc1 : acc1;
+              acc_t *x2 = (DUAL && (x & 1)) ? bcc2 : acc2;
+              x1[b] = P::mma(av[x], bv[x], x1[b]);
+              if (KIND == 2) x2[b] = P::mma(av[x], bw2[x], x2[b]);
+            }
+          }
         }
       }
-      for (; c0 < cmax; c0 += 2) step(c0 + ii, acc1, acc2);
     }
   }
   if (!active) return;
@@ -477,7 +555,12 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   // prefetch registers (256/CH threads per C1 row, 64/CH per gathered row)
   // and the staging fits in 64 KB of LDS
   const size_t rsz = mixed ? sizeof(float) : sizeof(double);
-  auto lds_of = [&](int ch) { return 2 * rsz * (size_t)ch * (a.nna + 4 * a.nmb) + rsz * 2 * 4 * ch; };
+  const int nbs = (kind == 2) ? 2 : 1;
+  auto lds_of = [&](int ch) {
+    const size_t sp = (size_t)ch / 2 + 16 / rsz;
+    const size_t arows = 2 * (((size_t)a.nna + 7) / 8) * 8 + 1;
+    return rsz * sp * (4 * arows + (size_t)nbs * 16 * (a.nmb + 1));
+  };
   auto fits = [&](int ch) {
     return a.nna <= (256 / ch) * kPfC1 && a.nmb <= (64 / ch) * kPfB && lds_of(ch) <= 64 * 1024;
   };

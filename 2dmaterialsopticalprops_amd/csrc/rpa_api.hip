@@ -155,6 +155,8 @@ static int prepare(rpa_ctx *c) {
   RPA_HIP(c, hipMemsetAsync(c->d_qeff, 0, sizeof(double) * 2 * nn, c->stream));
   RPA_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
   if (int e2 = launch_kstencil(c)) return e2;
+  if (c->d.program == RPA_SLOSS)
+    if (int e2 = configure_dyson(c)) return e2;
   RPA_HIP(c, hipStreamSynchronize(c->stream));
   c->prepared = true;
   return 0;

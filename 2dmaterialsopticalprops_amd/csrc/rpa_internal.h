@@ -91,6 +91,7 @@ struct rpa_ctx {
   // basis
   void *d_evc = nullptr;
   float2 *d_evc32 = nullptr;  // mixed precision: FP32 copy of the basis
+  bool dyson_lds_set = false; // k_dyson allowed more than 64 KB of LDS
   int *d_mill_k = nullptr;
   int16_t *d_kbox = nullptr;
   int *d_ngk = nullptr;
@@ -222,6 +223,7 @@ int launch_kstencil(rpa_ctx *c);
 int launch_kk(rpa_ctx *c);
 int launch_coulomb(rpa_ctx *c);
 int launch_dyson(rpa_ctx *c);
+int configure_dyson(rpa_ctx *c);
 int launch_epilogue(rpa_ctx *c);
 
 }  // namespace rpa

@@ -6,9 +6,13 @@ workload: transition bookkeeping, vertices, S0 accumulation, Kramers-Kronig,
 Coulomb, Dyson/W and the loss epilogue (Sloss semantics), inputs resident in
 HBM.  value = (no-1) output frequencies per step / seconds per step.
 
-Default workload (production size, FP64): 64x64x1 k-grid (4096 k), 20 bands
-(8 valence / 12 conduction), ~1500 plane waves per k, 31 local fields,
-701 frequencies over 0-30 eV, eta = 50 meV, q = one k-grid step.
+Default workload (production size): 64x64x1 k-grid (4096 k), 20 bands
+(8 valence / 12 conduction), ~1500 plane waves per k, 51 local fields,
+701 frequencies over 0-30 eV, eta = 50 meV, q = one k-grid step.  value is
+the mixed precision chain (complex FP32 vertices, BF16 hi/lo split products
+into FP32 for S0, FP64 weights, chunk reduction, KK and Dyson/W); the FP64
+chain is timed on the same inputs next to it, and both are compared with the
+FP64 oracle on a k-subset.
 
 Prints one JSON line (the last line of stdout).
 """
@@ -30,6 +34,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ω-pts/s for χ(q,ω), mixed vs FP64 GPU at ε⁻¹ rel err ≤1e-6; % HBM/MFMA peak"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+BF16X3_PEAK_TFLOPS = round(2500.0 / 3.0, 1)  # dense BF16 peak over three split products
 HBM_PEAK_GBS = 8000.0
 
 
@@ -105,14 +111,69 @@ def hbm_calibration(mib):
     torch.cuda.empty_cache()
 
 
+def time_chain(r, steps, warmup):
+    """Warmup, then `steps` eager runs of the whole chain with stage events on
+    the context's stream, then the same number of graph replays."""
+    host = importlib.import_module("2dmaterialsopticalprops_amd.host")
+    ctx = r.ctx
+    for _ in range(max(warmup, 1)):
+        ctx.run()
+    ctx.set_profiling(True)
+    stages = {k: 0.0 for k in host._abi.STAGES}
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev_ms += ctx.run()
+        for k, v in ctx.stage_ms().items():
+            stages[k] += v
+    wall = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    ctx.run(graph=True)
+    tg = time.perf_counter()
+    for _ in range(steps):
+        ctx.run(graph=True)
+    gwall = (time.perf_counter() - tg) * 1e3 / steps
+    return {"ms_step": wall * 1e3 / steps, "device_ms": dev_ms / steps, "graph_ms": gwall,
+            "stages": {k: v / steps for k, v in stages.items()}}
+
+
+def roofline_of(res, prec, flops_acc, flops_vtx):
+    """Roofline of the dominant matrix-core kernel of one run."""
+    acc_ms, vtx_ms = res["stages"]["accumulate"], res["stages"]["vertices"]
+    if prec == "mixed":
+        names = ("k_accum_mx (S0, BF16 hi/lo x3 into FP32)", "k_vertex<PrecF32> (vertices, FP32)")
+        peaks = (BF16X3_PEAK_TFLOPS, FP32_PEAK_TFLOPS)
+    else:
+        names = ("k_accum (S0, FP64)", "k_vertex<PrecF64> (vertices, FP64)")
+        peaks = (FP64_PEAK_TFLOPS, FP64_PEAK_TFLOPS)
+    if acc_ms >= vtx_ms:
+        dom, fl, ms, peak = names[0], flops_acc, acc_ms, peaks[0]
+    else:
+        dom, fl, ms, peak = names[1], flops_vtx, vtx_ms, peaks[1]
+    achieved = fl / (ms * 1e-3) / 1e12
+    return {
+        "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+        "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom, "kernel_ms": round(ms, 4),
+        "algorithmic_flop": fl,
+        "other": {
+            names[1]: {"ms": round(vtx_ms, 4), "flop": flops_vtx, "peak": peaks[1],
+                       "tflops": round(flops_vtx / (vtx_ms * 1e-3) / 1e12, 3) if vtx_ms > 0 else None},
+            names[0]: {"ms": round(acc_ms, 4), "flop": flops_acc, "peak": peaks[0],
+                       "tflops": round(flops_acc / (acc_ms * 1e-3) / 1e12, 3) if acc_ms > 0 else None},
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg1")
+    ap.add_argument("--config", default="cfg2")
     ap.add_argument("--program", default="sloss", choices=["sloss", "optabs"])
     ap.add_argument("--vertex-map", default="direct", choices=["direct", "igk"])
+    ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"],
+                    help="'both': value is the mixed path, timed next to the FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy check")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--hbm-calibration-mib", type=int, default=0,
@@ -123,7 +184,6 @@ def main():
         print("bench.py runs on one GPU: this build has no multi-GPU path "
               "(q points would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
-
     if args.hbm_calibration_mib > 0:
         hbm_calibration(args.hbm_calibration_mib)
     synth = importlib.import_module("2dmaterialsopticalprops_amd.synth")
@@ -133,78 +193,60 @@ def main():
     s = synth.make_system(cfg)
     log(f"[bench] synthetic {args.config}: nk={len(s.ngk)} nband={cfg.nband} ngk={s.ngk.min()}-{s.ngk.max()} "
         f"({time.perf_counter() - t0:.1f}s)")
-    t0 = time.perf_counter()
-    r = host.Runner(s, host.RunConfig(program=args.program, vertex_map=args.vertex_map))
-    log(f"[bench] setup+upload {time.perf_counter() - t0:.1f}s nlf={r.nlf} no={cfg.no} ntrans={r.ntransitions()}")
-    ctx = r.ctx
-    for _ in range(max(args.warmup, 1)):
-        ctx.run()
-    # timed region: K eager runs with stage events on the context's stream
-    ctx.set_profiling(True)
-    stages = {k: 0.0 for k in host._abi.STAGES}
-    dev_ms = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dev_ms += ctx.run()
-        for k, v in ctx.stage_ms().items():
-            stages[k] += v
-    wall = time.perf_counter() - t0
-    ctx.set_profiling(False)
-    stages = {k: v / args.steps for k, v in stages.items()}
-    ms_step = wall * 1e3 / args.steps
-    # graph replay of the same chain
-    ctx.run(graph=True)
-    tg = time.perf_counter()
-    gms = 0.0
-    for _ in range(args.steps):
-        gms += ctx.run(graph=True)
-    gwall = (time.perf_counter() - tg) * 1e3 / args.steps
+    precs = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    runs, runners = {}, {}
+    for prec in precs:
+        t0 = time.perf_counter()
+        r = host.Runner(s, host.RunConfig(program=args.program, vertex_map=args.vertex_map, precision=prec))
+        log(f"[bench] {prec}: setup+upload {time.perf_counter() - t0:.1f}s nlf={r.nlf} no={cfg.no} "
+            f"ntrans={r.ntransitions()}")
+        res = time_chain(r, args.steps, args.warmup)
+        log(f"[bench] {prec}: {res['ms_step']:.3f} ms/step (device {res['device_ms']:.3f} ms), graph "
+            f"{res['graph_ms']:.3f} ms/step; stages {json.dumps({k: round(v, 4) for k, v in res['stages'].items()})}")
+        runs[prec], runners[prec] = res, r
+    head = precs[0]
+    r = runners[head]
     nfreq = cfg.no - 1
-    value = nfreq / (ms_step * 1e-3)
-    log(f"[bench] {ms_step:.3f} ms/step (device {dev_ms / args.steps:.3f} ms), graph {gwall:.3f} ms/step; "
-        f"stages {json.dumps({k: round(v, 4) for k, v in stages.items()})}")
+    value = nfreq / (runs[head]["ms_step"] * 1e-3)
 
-    # algorithmic work of the two matrix-core kernels
+    # algorithmic work of the two matrix-core kernels (nonzero weights only)
     nv, nb = cfg.nval, cfg.nband
     eta = cfg.eta_ev / host.HARTREE
     nnz = nnz_weights(r.E, r.K1, r.K2, nv, nb, cfg.no, r.domega, eta)
     ntr = len(r.K1) * nv * (nb - nv)
     flops_acc = 2.0 * r.nlf * r.nlf * nnz
     flops_vtx = 8.0 * r.nlf * r.ngd * ntr
-    acc_ms, vtx_ms = stages["accumulate"], stages["vertices"]
-    if acc_ms >= vtx_ms:
-        dom, fl, ms = "k_accum (S0 accumulation)", flops_acc, acc_ms
-    else:
-        dom, fl, ms = "k_vertex_f64 (vertices)", flops_vtx, vtx_ms
-    achieved = fl / (ms * 1e-3) / 1e12
-    roofline = {
-        "bound": "mfma", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-        "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
-        "kernel_ms": round(ms, 4), "algorithmic_flop": fl,
-        "other": {
-            "k_vertex_f64": {"ms": round(vtx_ms, 4), "flop": flops_vtx,
-                             "tflops": round(flops_vtx / (vtx_ms * 1e-3) / 1e12, 3) if vtx_ms > 0 else None},
-            "k_accum": {"ms": round(acc_ms, 4), "flop": flops_acc, "nnz_weights": nnz,
-                        "tflops": round(flops_acc / (acc_ms * 1e-3) / 1e12, 3) if acc_ms > 0 else None},
-        },
-    }
+    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx)
+    roofline["other"]["nnz_weights"] = nnz
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "omega-points/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(runs[head]["ms_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": ("fp32 vertices + bf16 hi/lo x3 S0 into fp32, fp64 weights/reduction/KK/Dyson"
+                  if head == "mixed" else "f64"),
         "data": "synthetic",
         "config": {
-            "workload": f"{args.config}: {program_name(args.program)} chain, FP64, {len(s.ngk)} k "
+            "workload": f"{args.config}: {program_name(args.program)} chain, {len(s.ngk)} k "
                         f"({cfg.nk}x{cfg.nk}x1), {nb} bands ({nv}v/{nb - nv}c), ngk {s.ngk.min()}-{s.ngk.max()}, "
                         f"N_G={r.nlf}, {cfg.no} omega 0-{cfg.omax_ev:g} eV, eta={cfg.eta_ev * 1e3:g} meV, "
                         f"q=k-grid step, vertex map {args.vertex_map}",
-            "omega_points_per_step": nfreq, "transitions": ntr, "graph_ms_per_step": round(gwall, 4),
-            "device_ms_per_step": round(dev_ms / args.steps, 4),
-            "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+            "precision": head, "omega_points_per_step": nfreq, "transitions": ntr,
+            "graph_ms_per_step": round(runs[head]["graph_ms"], 4),
+            "device_ms_per_step": round(runs[head]["device_ms"], 4),
+            "stage_ms": {k: round(v, 4) for k, v in runs[head]["stages"].items()},
         },
         "roofline": roofline,
     }
+    if "fp64" in runs and head != "fp64":
+        f = runs["fp64"]
+        fv = nfreq / (f["ms_step"] * 1e-3)
+        out["fp64_gpu"] = {
+            "value": round(fv, 1), "unit": "omega-points/s", "ms_per_step": round(f["ms_step"], 4),
+            "stage_ms": {k: round(v, 4) for k, v in f["stages"].items()},
+            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx),
+        }
+        out["mixed_vs_fp64"] = round(value / fv, 2)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         try:
@@ -217,12 +259,11 @@ def main():
                           f"+ full KK/Dyson/W ({cb['t_tail']:.1f}s)",
             }
             out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
-            # accuracy on the same k-subset: GPU chain vs oracle
-            acc = accuracy(s, r, cb)
-            out["accuracy_vs_fp64_oracle"] = acc
+            out["accuracy_vs_fp64_oracle"] = {p: accuracy(s, runners[p], cb) for p in precs}
         except Exception as e:  # the baseline is reported, never silently replaced
             out["cpu_baseline"] = {"error": repr(e)}
-    r.close()
+    for rr in runners.values():
+        rr.close()
     print(json.dumps(out))
 
 
