@@ -91,6 +91,8 @@ struct PrecF64 {
   static __device__ inline int out_im(int p) { return p + 2; }
   static __device__ inline cplx make(real x, real y) { return make_double2(x, y); }
   static __device__ inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+  // two consecutive complex values from 16-byte aligned LDS
+  static __device__ inline void load2c(const cplx *p, cplx v[2]) { v[0] = p[0], v[1] = p[1]; }
   // four consecutive reals from 16-byte aligned LDS
   static __device__ inline void load4(const real *p, real v[4]) {
     const double2 x = *(const double2 *)p, y = *(const double2 *)(p + 2);
@@ -111,6 +113,10 @@ struct PrecF32 {
   static __device__ inline int out_im(int p) { return 2 * p + 1; }
   static __device__ inline cplx make(real x, real y) { return make_float2(x, y); }
   static __device__ inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+  static __device__ inline void load2c(const cplx *p, cplx v[2]) {
+    const float4 x = *(const float4 *)p;
+    v[0] = make_float2(x.x, x.y), v[1] = make_float2(x.z, x.w);
+  }
   static __device__ inline void load4(const real *p, real v[4]) {
     const float4 x = *(const float4 *)p;
     v[0] = x.x, v[1] = x.y, v[2] = x.z, v[3] = x.w;
@@ -418,6 +424,258 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   }
 }
 
+// Gather-direct form for up to two band blocks.  The K index of one MFMA is
+// four plane waves c = 4t + kg (kg = lane / 16) of one part: Re(conj(C1) C2)
+// = C1r C2r + C1i C2i and Im = C1r C2i - C1i C2r are two MFMAs sharing the
+// loaded values.  Each lane loads its B value (the C2 row of its plane wave,
+// column m = lane % 16) straight from L2 -- 32 lanes read one 128-byte row
+// segment -- one chunk ahead in registers, so B never passes through LDS.
+// The C1 chunk, shared by the four waves, is staged in LDS ([kg][n][t],
+// double-buffered, one barrier per chunk).  Row indices of the gathered
+// rows are resolved by lanes 0..31 (plane wave i0 + lane) and broadcast
+// with a lane shuffle.
+constexpr int kGdCH = 32;           // plane waves per chunk
+constexpr int kGdNT = kGdCH / 4;    // K groups of four per chunk
+
+template <class P, int KIND, int MAXB, bool DUAL>
+__global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
+  using real = typename P::real;
+  using cplx = typename P::cplx;
+  using acc_t = typename P::acc;
+  constexpr int CH = kGdCH, NT = kGdNT;
+  constexpr int UA = 2 * MAXB;  // C1 elements per staging thread: nna <= 8 UA
+  extern __shared__ __align__(16) unsigned char vsm_raw[];
+  const int nna = a.nna;
+  const int nap = nna + 1;  // + zero row
+  cplx *sA = (cplx *)vsm_raw;  // [2][4][nap][NT]
+  auto A_at = [&](int buf, int q, int n) { return sA + ((buf * 4 + q) * nap + n) * NT; };
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int L = blockIdx.x;
+  const int xcd = L & 7;
+  const int slot = L >> 3;
+  const int k = xcd + 8 * (slot / a.groups);
+  const int grp = slot % a.groups;
+  if (k >= a.ntot) return;  // whole workgroup leaves together
+  const int iG = grp * 4 + wave;
+  const bool active = iG < a.nlf;
+
+  const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
+  int d0, d1, d2;
+  mill_unpack(a.m0[k], d0, d1, d2);
+  const bool direct = (a.map_mode == RPA_MAP_DIRECT);
+  const int n1 = direct ? a.ngd : a.ngk[K1];
+  const int lim2 = a.ngk[K2];
+  const int *src = direct ? a.mill_g : a.mill_k + (size_t)K1 * a.npwx;
+  const int16_t *box2 = a.kbox + (size_t)K2 * a.kb.size();
+  const cplx *evc = (const cplx *)a.evc;
+  const cplx *c1 = evc + (size_t)K1 * a.npwx * a.nband + a.na0;
+  const cplx *c2 = evc + (size_t)K2 * a.npwx * a.nband + a.mb0;
+  const double *kv = a.ktot + 3 * k;
+  const int *dmap = (direct && active) ? a.dmap + ((size_t)a.g0idx[k] * a.nlf + iG) * a.ngd : nullptr;
+  int lf0 = 0, lf1 = 0, lf2 = 0;
+  if (!direct && active) mill_unpack(a.lf_key[iG], lf0, lf1, lf2);
+
+  const int r = lane & 15, kg = lane >> 4;
+  // zero rows of both buffers
+  for (int x = tid; x < 2 * 4 * NT; x += 256) A_at(x / (4 * NT), (x / NT) & 3, nna)[x % NT] = P::make(0, 0);
+
+  // ---- per-lane operand positions of each band block
+  int arow[MAXB], bcol[MAXB];
+  {
+    const int nr = P::arow_band(r);
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      const VBlock vb = a.vb[b < a.nvb ? b : 0];
+      arow[b] = (nr < vb.nrows) ? vb.n0 - a.na0 + nr : nna;
+      bcol[b] = (r < vb.ncols) ? vb.m0 - a.mb0 + r : -1;
+    }
+  }
+  const bool im_row = P::arow_im(r);
+
+  // ---- C1 staging: thread -> plane wave ca = tid / 8, bands (tid % 8) + 8u
+  const int ca = tid >> 3, na = tid & 7;
+  cplx pa[UA];
+  auto load_a = [&](int i0) {
+    const int i = i0 + ca;
+    const cplx *row = c1 + (size_t)i * a.nband;
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const int n = na + 8 * u;
+      if (8 * u < nna) pa[u] = (i < n1 && n < nna) ? row[n] : P::make(0, 0);
+    }
+  };
+  auto store_a = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < UA; ++u) {
+      const int n = na + 8 * u;
+      if (8 * u < nna && n < nna) A_at(buf, ca & 3, n)[ca >> 2] = pa[u];
+    }
+  };
+
+  // ---- gathered rows: lane l < 32 resolves plane wave i0 + l of its G
+  constexpr int kNone = -2147483647 - 1;
+  auto load_key = [&](int i) -> int {
+    return (active && lane < CH && i < n1) ? (direct ? dmap[i] : src[i]) : kNone;
+  };
+  auto lookup_igk = [&](int key) -> int {
+    if (key == kNone) return -1;
+    int s0, s1, s2;
+    mill_unpack(key, s0, s1, s2);
+    const int bi = a.kb.index(s0 + lf0 + d0, s1 + lf1 + d1, s2 + lf2 + d2);
+    return bi >= 0 ? (int)box2[bi] : -1;
+  };
+  auto row_of = [&](int jraw) -> int { return (jraw >= 0 && jraw < lim2) ? jraw : -1; };
+  auto load_mkey = [&](int i) -> int { return (KIND >= 1 && active && lane < CH && i < n1) ? src[i] : 0; };
+  // J of plane wave i for current vertices (pol p), lanes < 32
+  auto jweight = [&](int mkey, int i, int pol) -> real {
+    if (!(active && lane < CH && i < n1)) return 0;
+    int s0, s1, s2;
+    mill_unpack(mkey, s0, s1, s2);
+    const double jb = (a.q[pol] + 2.0 * kv[pol]) + a.glf[3 * iG + pol];
+    const double g = ((0.0 + a.kc[pol + 0] * (double)s0) + a.kc[pol + 3] * (double)s1) + a.kc[pol + 6] * (double)s2;
+    return (real)(0.5 * ((jb + 2.0 * g) * a.gcar));
+  };
+
+  // B registers of one chunk: value of plane wave 4t + kg, column bcol[b]
+  cplx bz[NT][MAXB];
+  real jw1 = 1, jw2 = 1;  // J of plane wave i0 + lane (lanes < 32) for the chunk in bz
+  auto load_b = [&](int jrow) {  // jrow: resolved row of plane wave i0 + lane
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int j = __shfl(jrow, 4 * t + kg);
+#pragma unroll
+      for (int b = 0; b < MAXB; ++b)
+        bz[t][b] = (b < a.nvb && j >= 0 && bcol[b] >= 0) ? c2[(size_t)j * a.nband + bcol[b]] : P::make(0, 0);
+    }
+  };
+
+  acc_t acc1[MAXB], acc2[MAXB], bcc1[MAXB], bcc2[MAXB];
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) acc1[b] = acc2[b] = bcc1[b] = bcc2[b] = P::zero();
+
+  // ---- prologue: A(0) in LDS, A(1) in registers, B(0) in registers,
+  //      rows of chunk 1 resolved (jn), keys of chunk 2 (kn, igk)
+  int jn, kn = kNone, mkn = 0;
+  real jwn1 = 1, jwn2 = 1;
+  load_a(0);
+  store_a(0);
+  {
+    const int key0 = load_key(lane);
+    const int j0 = row_of(direct ? key0 : lookup_igk(key0));
+    if (KIND >= 1) {
+      const int mk0 = load_mkey(lane);
+      jw1 = jweight(mk0, lane, a.pol1);
+      if (KIND == 2) jw2 = jweight(mk0, lane, a.pol2);
+    }
+    if (active) load_b(j0);
+    const int key1 = load_key(CH + lane);
+    jn = direct ? key1 : lookup_igk(key1);
+    if (KIND >= 1) mkn = load_mkey(CH + lane);
+    if (!direct) kn = load_key(2 * CH + lane);
+  }
+  if (CH < n1) load_a(CH);
+
+  int buf = 0;
+  for (int i0 = 0; i0 < n1; i0 += CH, buf ^= 1) {
+    __syncthreads();  // A(i0) visible; reads of the other buffer finished
+    if (i0 + CH < n1) {
+      store_a(buf ^ 1);
+      if (i0 + 2 * CH < n1) load_a(i0 + 2 * CH);
+    }
+    // B values and J of this chunk; then start the next chunk's loads
+    cplx bc[NT][MAXB];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int b = 0; b < MAXB; ++b) bc[t][b] = bz[t][b];
+    const real w1 = jw1, w2 = jw2;
+    if (active && i0 + CH < n1) {
+      load_b(row_of(jn));
+      if (KIND >= 1) {
+        jw1 = jweight(mkn, i0 + CH + lane, a.pol1);
+        if (KIND == 2) jw2 = jweight(mkn, i0 + CH + lane, a.pol2);
+      }
+      const int i2 = i0 + 2 * CH + lane;
+      if (direct) {
+        jn = load_key(i2);
+      } else {
+        jn = lookup_igk(kn);
+        kn = load_key(i2 + CH);
+      }
+      if (KIND >= 1) mkn = load_mkey(i2);
+    }
+    if (active) {
+#pragma unroll
+      for (int t = 0; t < NT; t += 2) {
+#pragma unroll
+        for (int b = 0; b < MAXB; ++b) {
+          if (b < a.nvb) {
+            // C1 of plane waves 4t + kg and 4(t+1) + kg, band arow[b]
+            cplx za[2];
+            P::load2c(A_at(buf, kg, arow[b]) + t, za);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const cplx z1 = za[h];
+              const real are = im_row ? -z1.y : z1.x;
+              const real aim = im_row ? z1.x : z1.y;
+              const cplx z2 = bc[t + h][b];
+              acc_t *x1 = (DUAL && h) ? bcc1 : acc1;
+              acc_t *x2 = (DUAL && h) ? bcc2 : acc2;
+              if (KIND >= 1) {
+                const real wa = __shfl(w1, 4 * (t + h) + kg);
+                x1[b] = P::mma(are, z2.x * wa, x1[b]);
+                x1[b] = P::mma(aim, z2.y * wa, x1[b]);
+                if (KIND == 2) {
+                  const real wb = __shfl(w2, 4 * (t + h) + kg);
+                  x2[b] = P::mma(are, z2.x * wb, x2[b]);
+                  x2[b] = P::mma(aim, z2.y * wb, x2[b]);
+                }
+              } else {
+                x1[b] = P::mma(are, z2.x, x1[b]);
+                x1[b] = P::mma(aim, z2.y, x1[b]);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+  if (!active) return;
+  if (DUAL) {
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+      acc1[b] += bcc1[b];
+      acc2[b] += bcc2[b];
+    }
+  }
+  cplx *rho1 = (cplx *)a.rho1, *rho2 = (cplx *)a.rho2;
+#pragma unroll
+  for (int b = 0; b < MAXB; ++b) {
+    if (b < a.nvb) {
+      const VBlock vb = a.vb[b];
+      const PairBlock &pbk = a.pairs.b[vb.pb];
+      const int nm = pbk.m_hi - pbk.m_lo;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int onr = P::out_band(kg, p);
+        if (onr < vb.nrows && r < vb.ncols) {
+          const int n = vb.n0 + onr, m = vb.m0 + r;
+          const long long tt = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
+                               (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
+          const long long pos = a.perm ? (long long)a.perm[tt] : tt;
+          if (pos >= 0) {
+            const cplx v1 = P::make(acc1[b][P::out_re(p)], acc1[b][P::out_im(p)]);
+            rho1[pos * a.nlf + iG] = v1;
+            if (KIND == 2) rho2[pos * a.nlf + iG] = P::make(acc2[b][P::out_re(p)], acc2[b][P::out_im(p)]);
+            else if (KIND == 1 && rho2) rho2[pos * a.nlf + iG] = v1;
+          }
+        }
+      }
+    }
+  }
+}
+
 // kbox[k][box(mill)] = position of mill in k's list.  Miller triples of one
 // list are distinct, so each slot is written at most once.
 __global__ void k_build_kbox(const int *mill_k, const int *ngk, int npwx, Box kb, int16_t *kbox) {
@@ -476,6 +734,24 @@ static void launch_kind(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) 
     launch_one<P, KIND, 4, false>(c, a, grid, lds);
   else
     launch_one<P, KIND, 8, false>(c, a, grid, lds);
+}
+
+template <class P, int KIND>
+static void launch_gd(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
+  if (a.nvb <= 1)
+    hipLaunchKernelGGL((k_vertex_gd<P, KIND, 1, true>), grid, dim3(256), lds, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_vertex_gd<P, KIND, 2, true>), grid, dim3(256), lds, c->stream, a);
+}
+
+template <class P>
+static void launch_prec_gd(rpa_ctx *c, int kind, const VertexArgs &a, dim3 grid, size_t lds) {
+  if (kind == 0)
+    launch_gd<P, 0>(c, a, grid, lds);
+  else if (kind == 1)
+    launch_gd<P, 1>(c, a, grid, lds);
+  else
+    launch_gd<P, 2>(c, a, grid, lds);
 }
 
 template <class P>
@@ -582,10 +858,22 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   if (nblocks <= 0) return 0;
   if (int e = launch_direct_map(c)) return e;
   dim3 grid((unsigned)nblocks);
-  if (mixed)
+  // gather-direct kernel (FP32, at most two band blocks spanning <= 16 bands
+  // of n): measured faster than the staged form in FP32 and slower in FP64,
+  // where its larger register set lowers the occupancy
+  const bool gd = mixed && a.nvb <= 2 && a.nna <= 16 && !std::getenv("RPA_VERTEX_STAGED");
+  if (gd) {
+    const size_t lds_gd = 2 * 4 * (size_t)(a.nna + 1) * kGdNT * (mixed ? sizeof(float2) : sizeof(double2));
+    a.ch = kGdCH;
+    if (mixed)
+      launch_prec_gd<PrecF32>(c, kind, a, grid, lds_gd);
+    else
+      launch_prec_gd<PrecF64>(c, kind, a, grid, lds_gd);
+  } else if (mixed) {
     launch_prec<PrecF32>(c, kind, a, grid, lds);
-  else
+  } else {
     launch_prec<PrecF64>(c, kind, a, grid, lds);
+  }
   RPA_HIP(c, hipGetLastError());
   return 0;
 }

@@ -244,3 +244,42 @@ def test_w_kk_check_matches_oracle_genw1w2():
         assert w1[io - 1] == pytest.approx(a, rel=1e-12, abs=1e-12 * np.abs(w1).max())
         assert w2[io - 1] == pytest.approx(b, rel=1e-12, abs=1e-12 * np.abs(w2).max())
     r.close()
+
+
+@pytest.mark.parametrize("program,pol", [("sloss", "xx"), ("optabs", "yz")])
+def test_staged_vertex_form_matches_oracle(program, pol, monkeypatch):
+    """The LDS-staged vertex kernel (used for more than two band blocks) on
+    the cases the gather-direct kernel normally takes."""
+    monkeypatch.setenv("RPA_VERTEX_STAGED", "1")
+    s = H.system("cfg0")
+    r = _runner(s, program=program, vertex_map="direct", pol=pol)
+    pb = _oracle_problem(s, r, 0)
+    nv, nb = s.cfg.nval, s.cfg.nband
+    if program == "sloss":
+        ref = orc.vertices(pb, 0, s.gcar, 0, nv, nv, nb)
+        got, _ = r.vertices()
+        assert H.rel_max(got.T, ref) < TOL_V
+    else:
+        a1, a2 = orc.vertices(pb, 4, s.gcar, 0, nv, nv, nb, both=True)
+        g1, g2 = r.vertices(both=True)
+        nk, tv = len(r.K1), nv * (nb - nv)
+        g1 = g1.T.reshape(nk, 2 * tv, r.nlf)
+        g2 = g2.T.reshape(nk, 2 * tv, r.nlf)
+        assert H.rel_max(g1[:, :tv], a1.reshape(nk, tv, r.nlf)) < TOL_V
+        assert H.rel_max(g2[:, :tv], a2.reshape(nk, tv, r.nlf)) < TOL_V
+    r.close()
+
+
+def test_many_band_blocks_optabs_full_pairs():
+    """Finite temperature keeps every band pair: nband = 20 gives more than
+    two 8 x 16 blocks, so the staged kernel runs (cfg0 bands widened)."""
+    s = H.system("tiny", nval=6, ncond=14)
+    cfg = s.cfg
+    r = _runner(s, program="optabs", vertex_map="direct", pol="xx", temp_ev=0.5)
+    pb = _oracle_problem(s, r, 0)
+    out = r.full()
+    S0, Q = orc.optabs_s0(pb, cfg.nval, cfg.no, 1, r.domega, cfg.eta_ev / H.HARTREE, r.rc.lor_cut,
+                          r.rc.df_cut, r.rc.efermi_ev / H.HARTREE, 0.5 / H.HARTREE, s.vcell, s.gcar)
+    assert H.rel_max(out["S0"], S0) < TOL
+    assert H.rel_max(out["Qeff"], Q) < TOL
+    r.close()
