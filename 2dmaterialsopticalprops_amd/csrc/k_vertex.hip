@@ -676,6 +676,232 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   }
 }
 
+// LDS-resident form for charge vertices of one band-pair block with at most
+// 8 bands n (every Sloss case).  A workgroup owns one k and a group of four bands m: the k+q
+// coefficients of those bands, all ngk(K2) rows, live in LDS for the whole
+// walk (plus a zero row for absent targets), so every local field gathers
+// from LDS and L2 supplies each k+q row once per band group.  MFMA columns
+// are (4 bands m) x (4 local fields G); rows are (8 bands n, Re/Im); the K
+// index is four plane waves of one part (Re and Im of the product are two
+// MFMAs).  The gathered row of every (G, plane wave) of a chunk is resolved
+// one chunk ahead and staged as int16 in LDS, as is the C1 chunk
+// (double-buffered, one barrier per chunk).  The four waves split K: wave w
+// takes the K groups t = w and w + 4 of each chunk for all local fields, and
+// the partial sums are added through LDS at the end.
+constexpr int kLdsCH = 32;             // plane waves per chunk
+constexpr int kLdsNT = kLdsCH / 4;     // K groups per chunk
+constexpr int kLdsMaxGQ = 24;          // local-field quads (Nlf <= 96)
+
+constexpr int kLdsRowStride = 5;       // complex values per staged k+q row (4 + pad)
+
+struct LdsLayout {
+  int rows2;     // staged k+q rows + 1 (zero row)
+  int ngp;       // local fields padded to 4
+  size_t bytes;  // dynamic LDS
+};
+
+// local-field quads compiled for a given Nlf (extra fields gather the zero row)
+__host__ __device__ inline int lds_ngq(int nlf) {
+  const int q = (nlf + 3) / 4;
+  return q <= 2 ? 2 : q <= 4 ? 4 : q <= 8 ? 8 : q <= 13 ? 13 : q <= 16 ? 16 : q <= 20 ? 20 : 24;
+}
+
+template <class P>
+__host__ __device__ inline LdsLayout lds_layout(int npwx, int nlf) {
+  LdsLayout L;
+  L.rows2 = npwx + 1;
+  L.ngp = 4 * lds_ngq(nlf);
+  const size_t c2 = sizeof(typename P::cplx) * kLdsRowStride * (size_t)L.rows2;
+  const size_t aa = sizeof(typename P::cplx) * 2 * 4 * 9 * kLdsNT;  // [2][4][8+1][NT]
+  const size_t jj = sizeof(int16_t) * 2 * (size_t)L.ngp * kLdsCH;
+  const size_t red = sizeof(typename P::real) * (size_t)L.ngp * 256;  // final wave sum
+  const size_t st = (c2 + aa + jj + 15) / 16 * 16;
+  L.bytes = st > red ? st : red;
+  return L;
+}
+
+template <class P, int NGQ>
+__global__ __launch_bounds__(256) void k_vertex_lds(VertexArgs a) {
+  using real = typename P::real;
+  using cplx = typename P::cplx;
+  using acc_t = typename P::acc;
+  constexpr int CH = kLdsCH, NT = kLdsNT;
+  extern __shared__ __align__(16) unsigned char vsm_raw[];
+  const LdsLayout lay = lds_layout<P>(a.npwx, a.nlf);
+  constexpr int RS = kLdsRowStride;
+  cplx *sC2 = (cplx *)vsm_raw;                         // [rows2][RS], 4 used
+  cplx *sA = sC2 + RS * (size_t)lay.rows2;             // [2][4][9][NT]
+  int16_t *sJ = (int16_t *)(sA + 2 * 4 * 9 * NT);      // [2][ngp][CH]
+  auto A_at = [&](int buf, int q, int n) { return sA + ((buf * 4 + q) * 9 + n) * NT; };
+  auto J_at = [&](int buf, int g) { return sJ + ((size_t)buf * lay.ngp + g) * CH; };
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nmg = (a.pairs.b[0].m_hi - a.pairs.b[0].m_lo + 3) / 4;  // band groups of m
+  // XCD-aware placement: linear id L runs on XCD L % 8; the band groups of
+  // one k share the XCD (and its copy of the k / k+q rows in L2)
+  const int Lid = blockIdx.x;
+  const int xcd = Lid & 7;
+  const int slot = Lid >> 3;
+  const int k = xcd + 8 * (slot / nmg);
+  const int mg = slot % nmg;
+  if (k >= a.ntot) return;  // whole workgroup leaves together
+
+  const PairBlock &pbk = a.pairs.b[0];  // the single band-pair block
+  const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
+  int d0, d1, d2;
+  mill_unpack(a.m0[k], d0, d1, d2);
+  const bool direct = (a.map_mode == RPA_MAP_DIRECT);
+  const int n1 = direct ? a.ngd : a.ngk[K1];
+  const int lim2 = a.ngk[K2];
+  const int zrow = lim2;  // zero row of the staged k+q block
+  const int *src = direct ? a.mill_g : a.mill_k + (size_t)K1 * a.npwx;
+  const int16_t *box2 = a.kbox + (size_t)K2 * a.kb.size();
+  const cplx *evc = (const cplx *)a.evc;
+  const cplx *c1 = evc + (size_t)K1 * a.npwx * a.nband + pbk.n_lo;
+  const int mlo = pbk.m_lo + 4 * mg;              // first band m of the group
+  const int nmv = min(4, pbk.m_hi - mlo);         // valid bands in the group
+  const cplx *c2 = evc + (size_t)K2 * a.npwx * a.nband + mlo;
+  const int nlf = a.nlf, nrows = pbk.n_hi - pbk.n_lo;
+
+  // ---- the k+q block of the band group, rows 0..lim2-1, and the zero row
+  for (int x = tid; x < 4 * (lim2 + 1); x += 256) {
+    const int j = x >> 2, m = x & 3;
+    sC2[RS * j + m] = (j < lim2 && m < nmv) ? c2[(size_t)j * a.nband + m] : P::make(0, 0);
+  }
+  for (int x = tid; x < 2 * 4 * NT; x += 256) A_at(x / (4 * NT), (x / NT) & 3, 8)[x % NT] = P::make(0, 0);
+
+  // ---- staging assignments: C1 (plane wave ca, band na) and gather rows
+  const int ca = tid >> 3, na = tid & 7;
+  constexpr int NJ = (kLdsMaxGQ * 4 * CH + 255) / 256;  // rows resolved per thread (max)
+  const int njt = (lay.ngp * CH + 255) / 256;           // used
+  cplx pa;
+  int pj[NJ];
+  auto load_a = [&](int i0) {
+    const int i = i0 + ca;
+    pa = (i < n1 && na < nrows) ? c1[(size_t)i * a.nband + na] : P::make(0, 0);
+  };
+  // gather row of entry x = g * CH + c of the chunk at i0: (G g, plane wave i0 + c)
+  auto raw_key = [&](int x, int i0) -> int {
+    const int g = x / CH, i = i0 + (x % CH);
+    if (g >= nlf || i >= n1) return -1;
+    return direct ? a.dmap[((size_t)a.g0idx[k] * nlf + g) * a.ngd + i] : src[i];
+  };
+  auto resolve = [&](int x, int v) -> int {
+    if (v < 0 && direct) return zrow;
+    const int g = x / CH, i = x % CH;
+    (void)i;
+    if (g >= nlf) return zrow;
+    if (direct) return (v >= 0 && v < lim2) ? v : zrow;
+    if (v == -1) return zrow;  // past the end of the list
+    int s0, s1, s2, l0, l1, l2;
+    mill_unpack(v, s0, s1, s2);
+    mill_unpack(a.lf_key[g], l0, l1, l2);
+    const int bi = a.kb.index(s0 + l0 + d0, s1 + l1 + d1, s2 + l2 + d2);
+    const int j = bi >= 0 ? (int)box2[bi] : -1;
+    return j >= 0 ? j : zrow;
+  };
+  auto load_j = [&](int i0) {
+#pragma unroll
+    for (int u = 0; u < NJ; ++u) {
+      const int x = tid + 256 * u;
+      if (u < njt) pj[u] = (x < lay.ngp * CH) ? raw_key(x, i0) : -1;
+    }
+  };
+  auto store = [&](int buf) {
+    if (na < 8) A_at(buf, ca & 3, na)[ca >> 2] = pa;
+#pragma unroll
+    for (int u = 0; u < NJ; ++u) {
+      const int x = tid + 256 * u;
+      if (u < njt && x < lay.ngp * CH) J_at(buf, x / CH)[x % CH] = (int16_t)resolve(x, pj[u]);
+    }
+  };
+
+  const int r = lane & 15, kg = lane >> 4;
+  const int nr = P::arow_band(r);
+  const bool im_row = P::arow_im(r);
+  const int arow = nr < nrows ? nr : 8;
+  const int ml = r & 3, gl = r >> 2;  // column = (band m ml, local field 4 gq + gl)
+
+  acc_t acc1[NGQ];
+#pragma unroll
+  for (int q = 0; q < NGQ; ++q) acc1[q] = P::zero();
+  constexpr int ngq = NGQ;  // == lay.ngp / 4
+
+  load_a(0);
+  load_j(0);
+  store(0);
+  if (CH < n1) {
+    load_a(CH);
+    load_j(CH);
+  }
+  int buf = 0;
+  for (int i0 = 0; i0 < n1; i0 += CH, buf ^= 1) {
+    __syncthreads();  // chunk i0 staged; the other buffer is free
+    if (i0 + CH < n1) {
+      store(buf ^ 1);
+      if (i0 + 2 * CH < n1) {
+        load_a(i0 + 2 * CH);
+        load_j(i0 + 2 * CH);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int t = wave + 4 * h;  // this wave's K groups of the chunk
+      const cplx za = A_at(buf, kg, arow)[t];
+      const real are = im_row ? -za.y : za.x;
+      const real aim = im_row ? za.x : za.y;
+      const int16_t *jrow = J_at(buf, 0) + (size_t)gl * CH + 4 * t + kg;  // K slot of this lane
+      // all row indices, then all gathered values, then the MFMAs
+      int jq[NGQ];
+#pragma unroll
+      for (int q = 0; q < NGQ; ++q) jq[q] = jrow[(size_t)(4 * q) * CH];
+      cplx bq[NGQ];
+#pragma unroll
+      for (int q = 0; q < NGQ; ++q) bq[q] = sC2[RS * jq[q] + ml];
+#pragma unroll
+      for (int q = 0; q < NGQ; ++q) {
+        acc1[q] = P::mma(are, bq[q].x, acc1[q]);
+        acc1[q] = P::mma(aim, bq[q].y, acc1[q]);
+      }
+    }
+  }
+
+  // ---- add the four waves' partial sums through LDS (the k+q block is done)
+  __syncthreads();
+  real *red = (real *)vsm_raw;  // [wave][q][4 elems][64 lanes]
+#pragma unroll
+  for (int q = 0; q < NGQ; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[((size_t)(wave * ngq + q) * 4 + e) * 64 + lane] = acc1[q][e];
+  __syncthreads();
+  cplx *rho1 = (cplx *)a.rho1;
+  const int nm = pbk.m_hi - pbk.m_lo;
+  // wave w finishes the quads q = w, w + 4, ...
+  for (int q = wave; q < ngq; q += 4) {
+    real v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      v[e] = 0;
+#pragma unroll
+      for (int w = 0; w < 4; ++w) v[e] += red[((size_t)(w * ngq + q) * 4 + e) * 64 + lane];
+    }
+    const int g = 4 * q + gl, m = mlo + ml;
+    if (g < nlf && ml < nmv) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int onr = P::out_band(kg, p);
+        if (onr < nrows) {
+          const int n = pbk.n_lo + onr;
+          const long long tt = (long long)k * a.pairs.tpk + a.pairs.off[0] +
+                               (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
+          const long long pos = a.perm ? (long long)a.perm[tt] : tt;
+          if (pos >= 0) rho1[pos * nlf + g] = P::make(v[P::out_re(p)], v[P::out_im(p)]);
+        }
+      }
+    }
+  }
+}
+
 // kbox[k][box(mill)] = position of mill in k's list.  Miller triples of one
 // list are distinct, so each slot is written at most once.
 __global__ void k_build_kbox(const int *mill_k, const int *ngk, int npwx, Box kb, int16_t *kbox) {
@@ -734,6 +960,58 @@ static void launch_kind(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) 
     launch_one<P, KIND, 4, false>(c, a, grid, lds);
   else
     launch_one<P, KIND, 8, false>(c, a, grid, lds);
+}
+
+// ---- LDS-resident charge-vertex kernel: selection and launch
+template <class P>
+static const void *lds_kernel(int ngq) {
+  switch (ngq) {
+    case 2: return (const void *)k_vertex_lds<P, 2>;
+    case 4: return (const void *)k_vertex_lds<P, 4>;
+    case 8: return (const void *)k_vertex_lds<P, 8>;
+    case 13: return (const void *)k_vertex_lds<P, 13>;
+    case 16: return (const void *)k_vertex_lds<P, 16>;
+    case 20: return (const void *)k_vertex_lds<P, 20>;
+    default: return (const void *)k_vertex_lds<P, 24>;
+  }
+}
+
+template <class P>
+static void launch_lds_p(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds, int ngq) {
+  switch (ngq) {
+    case 2: hipLaunchKernelGGL((k_vertex_lds<P, 2>), grid, dim3(256), lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_vertex_lds<P, 4>), grid, dim3(256), lds, c->stream, a); break;
+    case 8: hipLaunchKernelGGL((k_vertex_lds<P, 8>), grid, dim3(256), lds, c->stream, a); break;
+    case 13: hipLaunchKernelGGL((k_vertex_lds<P, 13>), grid, dim3(256), lds, c->stream, a); break;
+    case 16: hipLaunchKernelGGL((k_vertex_lds<P, 16>), grid, dim3(256), lds, c->stream, a); break;
+    case 20: hipLaunchKernelGGL((k_vertex_lds<P, 20>), grid, dim3(256), lds, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_vertex_lds<P, 24>), grid, dim3(256), lds, c->stream, a); break;
+  }
+}
+
+// Whether the LDS-resident kernel serves this context, and its LDS bytes.
+static bool lds_form(const rpa_ctx *c, size_t &bytes) {
+  if (c->d.program != RPA_SLOSS || c->pairs.nblocks != 1) return false;
+  const PairBlock &pb = c->pairs.b[0];
+  if (pb.n_hi - pb.n_lo > 8 || c->nlf > 4 * kLdsMaxGQ) return false;
+  // opt-in: measured at N_G = 51 slower than the gather-direct (FP32) and
+  // staged (FP64) forms, whose occupancy it cannot reach
+  const char *env = std::getenv("RPA_VERTEX_LDS");
+  if (!env || env[0] != '1') return false;
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  const LdsLayout L = mixed ? lds_layout<PrecF32>(c->d.npwx, c->nlf) : lds_layout<PrecF64>(c->d.npwx, c->nlf);
+  bytes = L.bytes;
+  return bytes <= 160 * 1024;
+}
+
+// Outside any stream capture: allow the LDS-resident kernel its dynamic LDS.
+int configure_vertex(rpa_ctx *c) {
+  size_t bytes = 0;
+  if (!lds_form(c, bytes) || bytes <= 64 * 1024) return 0;
+  const int ngq = lds_ngq(c->nlf);
+  const void *fn = (c->d.precision == RPA_MIXED) ? lds_kernel<PrecF32>(ngq) : lds_kernel<PrecF64>(ngq);
+  RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return 0;
 }
 
 template <class P, int KIND>
@@ -857,6 +1135,19 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   const long long nblocks = (long long)a.kslots * a.groups;
   if (nblocks <= 0) return 0;
   if (int e = launch_direct_map(c)) return e;
+  size_t lds_bytes = 0;
+  if (lds_form(c, lds_bytes)) {
+    // one workgroup per (k, group of four bands m), all local fields
+    const int nmg = (c->pairs.b[0].m_hi - c->pairs.b[0].m_lo + 3) / 4;
+    const dim3 g((unsigned)((long long)a.kslots * nmg));
+    const int ngq = lds_ngq(c->nlf);
+    if (mixed)
+      launch_lds_p<PrecF32>(c, a, g, lds_bytes, ngq);
+    else
+      launch_lds_p<PrecF64>(c, a, g, lds_bytes, ngq);
+    RPA_HIP(c, hipGetLastError());
+    return 0;
+  }
   dim3 grid((unsigned)nblocks);
   // gather-direct kernel (FP32, at most two band blocks spanning <= 16 bands
   // of n): measured faster than the staged form in FP32 and slower in FP64,

@@ -157,6 +157,7 @@ static int prepare(rpa_ctx *c) {
   if (int e2 = launch_kstencil(c)) return e2;
   if (c->d.program == RPA_SLOSS)
     if (int e2 = configure_dyson(c)) return e2;
+  if (int e2 = configure_vertex(c)) return e2;
   RPA_HIP(c, hipStreamSynchronize(c->stream));
   c->prepared = true;
   return 0;

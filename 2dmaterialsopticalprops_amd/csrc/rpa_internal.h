@@ -224,6 +224,7 @@ int launch_kk(rpa_ctx *c);
 int launch_coulomb(rpa_ctx *c);
 int launch_dyson(rpa_ctx *c);
 int configure_dyson(rpa_ctx *c);
+int configure_vertex(rpa_ctx *c);
 int launch_epilogue(rpa_ctx *c);
 
 }  // namespace rpa

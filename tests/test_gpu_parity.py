@@ -249,8 +249,9 @@ def test_w_kk_check_matches_oracle_genw1w2():
 @pytest.mark.parametrize("program,pol", [("sloss", "xx"), ("optabs", "yz")])
 def test_staged_vertex_form_matches_oracle(program, pol, monkeypatch):
     """The LDS-staged vertex kernel (used for more than two band blocks) on
-    the cases the gather-direct kernel normally takes."""
+    cases other kernels normally take."""
     monkeypatch.setenv("RPA_VERTEX_STAGED", "1")
+    monkeypatch.setenv("RPA_VERTEX_LDS", "0")
     s = H.system("cfg0")
     r = _runner(s, program=program, vertex_map="direct", pol=pol)
     pb = _oracle_problem(s, r, 0)
@@ -282,4 +283,22 @@ def test_many_band_blocks_optabs_full_pairs():
                           r.rc.df_cut, r.rc.efermi_ev / H.HARTREE, 0.5 / H.HARTREE, s.vcell, s.gcar)
     assert H.rel_max(out["S0"], S0) < TOL
     assert H.rel_max(out["Qeff"], Q) < TOL
+    r.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("form", ["lds", "gather", "staged"])
+@pytest.mark.parametrize("mode", ["direct", "igk"])
+def test_vertex_kernel_forms(precision, form, mode, monkeypatch):
+    """Every vertex kernel form on the Sloss charge vertices of cfg0."""
+    monkeypatch.setenv("RPA_VERTEX_LDS", "1" if form == "lds" else "0")
+    if form == "staged":
+        monkeypatch.setenv("RPA_VERTEX_STAGED", "1")
+    s = H.system("cfg0")
+    r = host.Runner(s, host.RunConfig(program="sloss", vertex_map=mode, precision=precision))
+    pb = _oracle_problem(s, r, 0 if mode == "direct" else 1)
+    nv, nb = s.cfg.nval, s.cfg.nband
+    ref = orc.vertices(pb, 0, s.gcar, 0, nv, nv, nb)
+    got, _ = r.vertices()
+    assert H.rel_max(got.T, ref) < (TOL_V if precision == "fp64" else 2e-6)
     r.close()
