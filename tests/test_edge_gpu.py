@@ -1,0 +1,118 @@
+"""Edge cases and error paths of the HIP path through the C ABI.
+
+Covers what the reference's own checks imply: a single local field, q = 0
+(iq = 1), NGd beyond a k-point's plane waves (the reference stops with
+"NGd is bigger than NG1", Sloss.f90:339-342), calls out of order, compiled
+limits, and a second q on the same context.  Tolerances as in
+test_gpu_parity.py.
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+
+import helpers as H
+from oracle import oracle as orc
+
+host = importlib.import_module("2dmaterialsopticalprops_amd.host")
+abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+def _runner(s, **kw):
+    return host.Runner(s, host.RunConfig(**kw))
+
+
+def _sloss_ref(s, r):
+    cfg = s.cfg
+    pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+    S0 = orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, cfg.eta_ev / H.HARTREE, s.vcell)
+    chi0 = orc.kk_sloss(S0, r.domega)
+    V = orc.genV(r.rc.eps, r.q[0] * s.gcar, r.q[1] * s.gcar, cfg.c, r.parg, r.glf, s.gcar)
+    E = np.array([orc.dyson_w(chi0[f], V)[0] for f in range(cfg.no - 1)])
+    return S0, chi0, E
+
+
+def test_single_local_field():
+    s = H.system("tiny", nlf_target=1)
+    r = _runner(s, program="sloss")
+    assert r.nlf == 1
+    out = r.full()
+    S0, chi0, E = _sloss_ref(s, r)
+    assert H.rel_max(out["S0"], S0) < TOL
+    assert H.rel_max(out["epsinv"], E) < TOL
+    # N_G = 1: chi = chi0 / (1 - V chi0)
+    V = out["V"][0, 0]
+    np.testing.assert_allclose(out["chi"][:, 0, 0], chi0[:, 0, 0] / (1 - V * chi0[:, 0, 0]), rtol=1e-9)
+    r.close()
+
+
+def test_q_zero_is_an_optical_limit_case():
+    """iq = 1 gives q = 0: K2 = K1 and G0 = 0 for every k."""
+    s = H.system("tiny")
+    r = _runner(s, program="optabs", iq=1, pol="xx")
+    assert np.all(r.K2 == r.K1)
+    out = r.full()
+    cfg = s.cfg
+    pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+    S0, Q = orc.optabs_s0(pb, cfg.nval, cfg.no, 1, r.domega, cfg.eta_ev / H.HARTREE, r.rc.lor_cut,
+                          r.rc.df_cut, 0.0, 0.0, s.vcell, s.gcar)
+    assert H.rel_max(out["S0"], S0) < TOL
+    r.close()
+
+
+def test_ngd_beyond_a_k_point_is_an_error():
+    s = H.system("tiny")
+    r = _runner(s, program="sloss")
+    p = {f: getattr(r.ctx.params, f) for f, _ in abi.RpaParams._fields_}
+    p["ngd"] = int(s.ngk.min()) + 1
+    r.ctx.set_params(**p)
+    with pytest.raises(abi.RpaError) as e:
+        r.ctx.run()
+    assert e.value.code == abi.RPA_E_ARG and "NGd" in str(e.value)
+    r.close()
+
+
+def test_calls_out_of_order_report_state_errors():
+    s = H.system("tiny")
+    cfg = s.cfg
+    ctx = abi.Context(len(s.ngk), cfg.nband, s.npwx, 10, cfg.no, abi.RPA_SLOSS)
+    with pytest.raises(abi.RpaError) as e:
+        ctx.run()
+    assert e.value.code == abi.RPA_E_STATE
+    with pytest.raises(abi.RpaError) as e:
+        ctx.kk()
+    assert e.value.code == abi.RPA_E_STATE
+    ctx.close()
+
+
+def test_compiled_limits():
+    s = H.system("tiny")
+    cfg = s.cfg
+    with pytest.raises(abi.RpaError) as e:
+        abi.Context(len(s.ngk), cfg.nband, s.npwx, 1000, cfg.no, abi.RPA_SLOSS)   # Nlfd > 128
+    assert e.value.code == abi.RPA_E_LIMIT
+    with pytest.raises(abi.RpaError) as e:
+        abi.Context(len(s.ngk), cfg.nband, s.npwx, 10, 3, abi.RPA_SLOSS)          # no < 4
+    assert e.value.code == abi.RPA_E_ARG
+
+
+def test_second_q_on_the_same_context():
+    """The Sloss q loop: set_q again on a live context gives the fresh q's
+    result (buffers are reused, nothing of the first q leaks)."""
+    s = H.system("tiny")
+    r = _runner(s, program="sloss")
+    first = r.full()
+    q3 = abi.find_min_q(r.ktot, 3)
+    K1, K2, iG0 = abi.find_kq(r.ktot, s.kcart, r.g, q3, 1e-4)
+    r.ctx.set_q(K1, K2, iG0, r.ktot, q3)
+    r.ctx.run()
+    S0 = r.ctx.result("S0")
+    cfg = s.cfg
+    pb = orc.Problem(s, K1, K2, iG0, r.ktot, q3, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+    ref = orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, cfg.eta_ev / H.HARTREE, s.vcell)
+    assert H.rel_max(S0, ref) < TOL
+    assert H.rel_max(first["S0"], ref) > 1e-3  # a different q
+    r.close()
