@@ -290,10 +290,16 @@ __global__ __launch_bounds__(256) void k_accum(AccArgs a) {
   double2 *sR2 = sR1 + kKS * a.nlf;                    // [kKS][nlf] (CPLX)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r = lane & 15, q4 = lane >> 4;
-  const int total = a.nitems[0] * a.npb;
+  // XCD-aware: workgroup b runs on XCD b % 8 and takes the items with
+  // item % 8 == b % 8, so every column block of one chunk reads the chunk's
+  // vertices from the same L2 (the grid is a multiple of 8)
+  const int xcd = blockIdx.x & 7, nsl = gridDim.x >> 3;
+  const int nix = (a.nitems[0] - xcd + 7) / 8;
+  const int total = nix * a.npb;
 
-  for (int wk = blockIdx.x; wk < total; wk += gridDim.x) {
-    const int item = wk / a.npb, pbk = wk - item * a.npb;
+  for (int wk = blockIdx.x >> 3; wk < total; wk += nsl) {
+    const int il = wk / a.npb, pbk = wk - il * a.npb;
+    const int item = 8 * il + xcd;
     const int b = a.items[3 * item], t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
 
     // columns owned by this lane in the four 16-wide blocks of its wave
@@ -411,10 +417,14 @@ __global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
   __shared__ __align__(16) __bf16 sB[NB][2][NJ][kMxRow];  // [re/im][hi/lo][G'][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r = lane & 15, kg = lane >> 4;
-  const int total = a.nitems[0] * a.nib;
+  // XCD-aware as in k_accum: all local-field blocks of one chunk on one XCD
+  const int xcd = blockIdx.x & 7, nsl = gridDim.x >> 3;
+  const int nix = (a.nitems[0] - xcd + 7) / 8;
+  const int total = nix * a.nib;
 
-  for (int wk = blockIdx.x; wk < total; wk += gridDim.x) {
-    const int item = wk / a.nib, ib = wk - item * a.nib;
+  for (int wk = blockIdx.x >> 3; wk < total; wk += nsl) {
+    const int il = wk / a.nib, ib = wk - il * a.nib;
+    const int item = 8 * il + xcd;
     const int b = a.items[3 * item], t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
     const int i0 = ib * (4 * IPW) + wave * IPW;
 
@@ -640,7 +650,7 @@ static int launch_mx_ncb(rpa_ctx *c, AccMxArgs &a, long long items) {
   constexpr int IPW = CPLX ? 1 : 2;
   const int ncb = (c->nlf + 15) / 16;
   a.nib = (c->nlf + 4 * IPW - 1) / (4 * IPW);
-  const int grid = (int)std::min<long long>(std::max<long long>(items * a.nib, 1), 8192);
+  const int grid = (int)std::min<long long>(std::max<long long>((items * a.nib + 7) / 8 * 8, 8), 8192);
   switch (ncb) {
     case 1: launch_mx<CPLX, 1, IPW>(c, a, grid); break;
     case 2: launch_mx<CPLX, 2, IPW>(c, a, grid); break;
@@ -702,7 +712,7 @@ int launch_accumulate(rpa_ctx *c) {
     a.part = c->d_part;
     const size_t lds = sizeof(double) * kTileW * (kKS + 1) + sizeof(double2) * kKS * c->nlf * (cplx ? 2 : 1);
     long long work = (long long)c->max_items * a.npb;
-    int grid = (int)std::min<long long>(std::max<long long>(work, 1), 4096);
+    int grid = (int)std::min<long long>(std::max<long long>((work + 7) / 8 * 8, 8), 4096);
     if (cplx)
       hipLaunchKernelGGL(k_accum<true>, dim3(grid), dim3(256), lds, c->stream, a);
     else

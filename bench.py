@@ -137,7 +137,36 @@ def time_chain(r, steps, warmup):
             "stages": {k: v / steps for k, v in stages.items()}}
 
 
-def roofline_of(res, prec, flops_acc, flops_vtx):
+KERNEL_PREFIX = {  # (precision, stage) -> kernel name in the rocprof tables
+    ("mixed", "vertices"): "rpa::k_vertex_gd<rpa::PrecF32", ("fp64", "vertices"): "rpa::k_vertex<rpa::PrecF64",
+    ("mixed", "accumulate"): "rpa::k_accum_mx<", ("fp64", "accumulate"): "rpa::k_accum<",
+}
+
+
+def hbm_traffic(workload, prec, stage):
+    """HBM bytes per launch of a kernel from profiles/hbm_traffic.txt (two
+    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, scaled by a 1 GiB copy),
+    if that table was measured on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    rows, wl = {}, None
+    for line in open(path):
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0] == "workload":
+            wl = " ".join(f[1:])
+        else:
+            rows[f[0]] = float(f[1]) + float(f[2])
+    if wl != workload:
+        return None
+    pre = KERNEL_PREFIX[(prec, stage)]
+    hits = [v for k, v in rows.items() if k.startswith(pre)]
+    return hits[0] if len(hits) == 1 else None
+
+
+def roofline_of(res, prec, flops_acc, flops_vtx, workload=""):
     """Roofline of the dominant matrix-core kernel of one run."""
     acc_ms, vtx_ms = res["stages"]["accumulate"], res["stages"]["vertices"]
     if prec == "mixed":
@@ -147,13 +176,17 @@ def roofline_of(res, prec, flops_acc, flops_vtx):
         names = ("k_accum (S0, FP64)", "k_vertex<PrecF64> (vertices, FP64)")
         peaks = (FP64_PEAK_TFLOPS, FP64_PEAK_TFLOPS)
     if acc_ms >= vtx_ms:
-        dom, fl, ms, peak = names[0], flops_acc, acc_ms, peaks[0]
+        dom, fl, ms, peak, stage = names[0], flops_acc, acc_ms, peaks[0], "accumulate"
     else:
-        dom, fl, ms, peak = names[1], flops_vtx, vtx_ms, peaks[1]
+        dom, fl, ms, peak, stage = names[1], flops_vtx, vtx_ms, peaks[1], "vertices"
     achieved = fl / (ms * 1e-3) / 1e12
+    traffic = hbm_traffic(workload, prec, stage)
     return {
         "bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-        "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom, "kernel_ms": round(ms, 4),
+        "frac": round(achieved / peak, 4), "traffic": traffic,
+        "traffic_note": ("HBM bytes per launch, rocprofv3 FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy "
+                         "(profiles/hbm_traffic.txt)") if traffic else "not measured for this workload",
+        "kernel": dom, "kernel_ms": round(ms, 4),
         "algorithmic_flop": fl,
         "other": {
             names[1]: {"ms": round(vtx_ms, 4), "flop": flops_vtx, "peak": peaks[1],
@@ -216,7 +249,8 @@ def main():
     ntr = len(r.K1) * nv * (nb - nv)
     flops_acc = 2.0 * r.nlf * r.nlf * nnz
     flops_vtx = 8.0 * r.nlf * r.ngd * ntr
-    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx)
+    workload_key = f"{args.config} {args.program} {args.vertex_map}"
+    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key)
     roofline["other"]["nnz_weights"] = nnz
 
     out = {
@@ -244,7 +278,7 @@ def main():
         out["fp64_gpu"] = {
             "value": round(fv, 1), "unit": "omega-points/s", "ms_per_step": round(f["ms_step"], 4),
             "stage_ms": {k: round(v, 4) for k, v in f["stages"].items()},
-            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx),
+            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx, workload_key),
         }
         out["mixed_vs_fp64"] = round(value / fv, 2)
     if not args.no_cpu:
