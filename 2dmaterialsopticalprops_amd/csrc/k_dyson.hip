@@ -9,8 +9,9 @@
 //                                             Gauss-Jordan with row pivoting,
 //                                             in place in LDS)
 //                chi = eps^-1 chi0           (genChi, 1236-1253)
-//                W   = V + (chi V) pre-multiplied by V   (genWT, 1255-1277,
-//                                             O(N^3) instead of O(N^4))
+//                W   = V + V chi V            (genWT, 1255-1277) evaluated
+//                                             as V eps^-1 (chi V = eps^-1 - I),
+//                                             one O(N^3) product
 //                loss = -Im eps^-1(1,1)
 //  k_pi        Optabs Pi assembly (Optabs.f90:575-611): Pi_tot(1,1) and the
 //              conductivities of fort.401 / fort.402.
@@ -56,27 +57,29 @@ struct DysonArgs {
   const double2 *chi0;  // [nfreq][nlf][nlf]
   const double *V;      // [nlf][nlf]
   double2 *epsinv, *chi, *W;
-  double2 *T;           // scratch [nfreq][nlf][nlf]
   double *loss;
   int *status;
 };
 
 __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
-  extern __shared__ double2 sa[];  // [nlf][nlf] eps, then eps^-1
+  extern __shared__ double2 sa[];  // [nlf][nlf] eps, then eps^-1; then V (real) behind it
   __shared__ double2 fcol[kMaxNlf];
   __shared__ int piv[kMaxNlf];
-  __shared__ double red_v[256];
-  __shared__ int red_i[256];
-  const int N = a.nlf, tid = threadIdx.x;
+  __shared__ double red_v[4];
+  __shared__ int red_i[4];
+  const int N = a.nlf, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double *sv = (double *)(sa + N * N);  // [nlf][nlf] V
   const int f = blockIdx.x;
   const double2 *X0 = a.chi0 + (size_t)f * N * N;
+  for (int x = tid; x < N * N; x += 256) sv[x] = a.V[x];
+  __syncthreads();
   // eps = I - chi0 V
   for (int x = tid; x < N * N; x += 256) {
     int i = x / N, j = x - i * N;
     double2 s = make_double2(i == j ? 1.0 : 0.0, 0.0);
     for (int k = 0; k < N; ++k) {
       double2 c = X0[i * N + k];
-      double v = a.V[k * N + j];
+      double v = sv[k * N + j];
       s.x -= c.x * v;
       s.y -= c.y * v;
     }
@@ -85,8 +88,9 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
   __syncthreads();
   // in-place Gauss-Jordan inversion with row pivoting
   for (int k = 0; k < N; ++k) {
+    // pivot: largest |Re| + |Im| in column k at or below row k, lowest row on ties
     double best = -1.0;
-    int bi = k;
+    int bi = N;
     for (int i = k + tid; i < N; i += 256) {
       double2 z = sa[i * N + k];
       double m = fabs(z.x) + fabs(z.y);
@@ -95,23 +99,32 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
         bi = i;
       }
     }
-    red_v[tid] = best;
-    red_i[tid] = bi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) {
-        if (red_v[tid + s] > red_v[tid] || (red_v[tid + s] == red_v[tid] && red_i[tid + s] < red_i[tid])) {
-          red_v[tid] = red_v[tid + s];
-          red_i[tid] = red_i[tid + s];
-        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
       }
-      __syncthreads();
     }
-    const int p = red_i[0];
-    if (tid == 0) {
-      piv[k] = p;
-      if (!(red_v[0] > 0.0)) atomicOr(a.status, 1);
+    if (lane == 0) {
+      red_v[wave] = best;
+      red_i[wave] = bi;
     }
+    __syncthreads();
+    double pv = red_v[0];
+    int p = red_i[0];
+    for (int w = 1; w < 4; ++w)
+      if (red_v[w] > pv || (red_v[w] == pv && red_i[w] < p)) {
+        pv = red_v[w];
+        p = red_i[w];
+      }
+    if (!(pv > 0.0)) {  // singular: flag it and go on without a swap
+      if (tid == 0) atomicOr(a.status, 1);
+      p = k;
+    }
+    if (tid == 0) piv[k] = p;
     if (p != k)
       for (int j = tid; j < N; j += 256) {
         double2 t = sa[k * N + j];
@@ -153,13 +166,12 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
   }
   double2 *Einv = a.epsinv + (size_t)f * N * N;
   double2 *Ch = a.chi + (size_t)f * N * N;
-  double2 *T = a.T + (size_t)f * N * N;
   double2 *Wo = a.W + (size_t)f * N * N;
-  for (int x = tid; x < N * N; x += 256) Einv[x] = sa[x];
   if (tid == 0) a.loss[f] = -sa[0].y;
-  // chi = eps^-1 chi0
   for (int x = tid; x < N * N; x += 256) {
     int i = x / N, j = x - i * N;
+    Einv[x] = sa[x];
+    // chi = eps^-1 chi0
     double2 s = make_double2(0.0, 0.0);
     for (int k = 0; k < N; ++k) {
       double2 e = sa[i * N + k], c = X0[k * N + j];
@@ -167,32 +179,15 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
       s.y += e.x * c.y + e.y * c.x;
     }
     Ch[x] = s;
-  }
-  __syncthreads();
-  // T = chi V
-  for (int x = tid; x < N * N; x += 256) {
-    int i = x / N, j = x - i * N;
-    double2 s = make_double2(0.0, 0.0);
+    // W = V + V (chi V) = V eps^-1, since chi V = eps^-1 chi0 V = eps^-1 (I - eps)
+    double2 w = make_double2(0.0, 0.0);
     for (int k = 0; k < N; ++k) {
-      double2 c = Ch[i * N + k];
-      double v = a.V[k * N + j];
-      s.x += c.x * v;
-      s.y += c.y * v;
+      double v = sv[i * N + k];
+      double2 e = sa[k * N + j];
+      w.x += v * e.x;
+      w.y += v * e.y;
     }
-    T[x] = s;
-  }
-  __syncthreads();
-  // W = V + V T
-  for (int x = tid; x < N * N; x += 256) {
-    int i = x / N, j = x - i * N;
-    double2 s = make_double2(0.0, 0.0);
-    for (int k = 0; k < N; ++k) {
-      double v = a.V[i * N + k];
-      double2 t = T[k * N + j];
-      s.x += v * t.x;
-      s.y += v * t.y;
-    }
-    Wo[x] = make_double2(a.V[x] + s.x, s.y);
+    Wo[x] = w;
   }
 }
 
@@ -240,7 +235,7 @@ int launch_coulomb(rpa_ctx *c) {
 // Done once per context outside any stream capture: the in-LDS solve holds
 // Nlf^2 complex values, beyond the default 64 KB when Nlf > 64.
 int configure_dyson(rpa_ctx *c) {
-  const size_t lds = sizeof(double2) * c->nlf * c->nlf;
+  const size_t lds = (sizeof(double2) + sizeof(double)) * c->nlf * c->nlf;
   if (lds > 64 * 1024 && lds <= 160 * 1024 && !c->dyson_lds_set) {
     RPA_HIP(c, hipFuncSetAttribute((const void *)k_dyson, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     c->dyson_lds_set = true;
@@ -257,11 +252,10 @@ int launch_dyson(rpa_ctx *c) {
   a.epsinv = (double2 *)c->d_epsinv;
   a.chi = (double2 *)c->d_chi;
   a.W = (double2 *)c->d_W;
-  a.T = (double2 *)c->d_scratch;
   a.loss = c->d_loss;
   a.status = c->d_status;
-  const size_t lds = sizeof(double2) * c->nlf * c->nlf;
-  if (lds > 160 * 1024) return fail(c, RPA_E_LIMIT, "Nlf too large for the in-LDS Dyson solve (Nlf <= 101)");
+  const size_t lds = (sizeof(double2) + sizeof(double)) * c->nlf * c->nlf;
+  if (lds > 160 * 1024) return fail(c, RPA_E_LIMIT, "Nlf too large for the in-LDS Dyson solve (Nlf <= 82)");
   if (lds > 64 * 1024 && !c->dyson_lds_set)
     return fail(c, RPA_E_STATE, "Dyson LDS limit not configured");
   hipLaunchKernelGGL(k_dyson, dim3(a.nfreq), dim3(256), lds, c->stream, a);

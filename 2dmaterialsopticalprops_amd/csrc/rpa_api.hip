@@ -141,7 +141,6 @@ static int prepare(rpa_ctx *c) {
   if (!e) e = ensure(c, c->d_epsinv, c->cap_epsinv, nf * nn * 2);
   if (!e) e = ensure(c, c->d_chi, c->cap_chi, nf * nn * 2);
   if (!e) e = ensure(c, c->d_W, c->cap_W, nf * nn * 2);
-  if (!e) e = ensure(c, c->d_scratch, c->cap_scratch, nf * nn * 2);
   if (!e) e = ensure(c, c->d_V, c->cap_V, nn);
   if (!e) e = ensure(c, c->d_loss, c->cap_loss, nf);
   if (!e) e = ensure(c, c->d_pi11, c->cap_pi11, 2 * nf);
