@@ -319,7 +319,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_key, c->d_perm, c->d_bcnt, c->d_toff, c->d_nat, c->d_te, c->d_items,
                   c->d_nitems, c->d_first_item, c->d_intra, c->d_nintra, c->d_rho1, c->d_rho2,
                   c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
-                  c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig, c->d_scratch,
+                  c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
                   c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -451,14 +451,10 @@ int rpa_set_local_fields(rpa_ctx *c, int nlf, const int *lf_idx, const int *parg
   c->nlf = nlf;
   c->lf_idx.assign(lf_idx, lf_idx + nlf);
   c->parg.assign(parg, parg + nlf);
-  for (void *p : {(void *)c->d_lf_key, (void *)c->d_glf, (void *)c->d_parg})
-    if (p) (void)hipFree(p);
-  c->d_lf_key = nullptr;
-  c->d_glf = nullptr;
-  c->d_parg = nullptr;
-  RPA_HIP(c, hipMalloc((void **)&c->d_lf_key, sizeof(int) * nlf));
-  RPA_HIP(c, hipMalloc((void **)&c->d_glf, sizeof(double) * 3 * nlf));
-  RPA_HIP(c, hipMalloc((void **)&c->d_parg, sizeof(int) * nlf));
+  int e = ensure(c, c->d_lf_key, c->cap_lf_key, (size_t)nlf);
+  if (!e) e = ensure(c, c->d_glf, c->cap_glf, 3 * (size_t)nlf);
+  if (!e) e = ensure(c, c->d_parg, c->cap_parg, (size_t)nlf);
+  if (e) return e;
   RPA_HIP(c, hipMemcpy(c->d_lf_key, keys.data(), sizeof(int) * nlf, hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(c->d_glf, glf.data(), sizeof(double) * 3 * nlf, hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(c->d_parg, parg, sizeof(int) * nlf, hipMemcpyHostToDevice));
@@ -513,24 +509,21 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   c->iG0.assign(iG0, iG0 + ntot);
   c->ktot.assign(ktot, ktot + 3 * (size_t)ntot);
   for (int a = 0; a < 3; ++a) c->q[a] = q[a];
-  for (void *p : {(void *)c->d_k1, (void *)c->d_k2, (void *)c->d_m0, (void *)c->d_ktot,
-                  (void *)c->d_g0keys, (void *)c->d_g0idx})
-    if (p) (void)hipFree(p);
-  c->d_k1 = c->d_k2 = c->d_m0 = c->d_g0keys = c->d_g0idx = nullptr;
-  c->d_ktot = nullptr;
-  RPA_HIP(c, hipMalloc((void **)&c->d_k1, sizeof(int) * ntot));
-  RPA_HIP(c, hipMalloc((void **)&c->d_k2, sizeof(int) * ntot));
-  RPA_HIP(c, hipMalloc((void **)&c->d_m0, sizeof(int) * ntot));
-  RPA_HIP(c, hipMalloc((void **)&c->d_ktot, sizeof(double) * 3 * ntot));
-  RPA_HIP(c, hipMemcpy(c->d_k1, K1, sizeof(int) * ntot, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_k2, K2, sizeof(int) * ntot, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_m0, m0.data(), sizeof(int) * ntot, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_ktot, ktot, sizeof(double) * 3 * ntot, hipMemcpyHostToDevice));
   c->ng0 = (int)g0keys.size();
-  RPA_HIP(c, hipMalloc((void **)&c->d_g0keys, sizeof(int) * c->ng0));
-  RPA_HIP(c, hipMalloc((void **)&c->d_g0idx, sizeof(int) * ntot));
+  const size_t nt = (size_t)ntot;
+  int e = ensure(c, c->d_k1, c->cap_k1, nt);
+  if (!e) e = ensure(c, c->d_k2, c->cap_k2, nt);
+  if (!e) e = ensure(c, c->d_m0, c->cap_m0, nt);
+  if (!e) e = ensure(c, c->d_ktot, c->cap_ktot, 3 * nt);
+  if (!e) e = ensure(c, c->d_g0keys, c->cap_g0keys, (size_t)c->ng0);
+  if (!e) e = ensure(c, c->d_g0idx, c->cap_g0idx, nt);
+  if (e) return e;
+  RPA_HIP(c, hipMemcpy(c->d_k1, K1, sizeof(int) * nt, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_k2, K2, sizeof(int) * nt, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_m0, m0.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_ktot, ktot, sizeof(double) * 3 * nt, hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(c->d_g0keys, g0keys.data(), sizeof(int) * c->ng0, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_g0idx, g0idx.data(), sizeof(int) * ntot, hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(c->d_g0idx, g0idx.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
   c->have_q = true;
   c->prepared = false;
   invalidate_graph(c);

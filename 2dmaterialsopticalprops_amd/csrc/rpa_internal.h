@@ -112,6 +112,7 @@ struct rpa_ctx {
   int *d_lf_key = nullptr;
   double *d_glf = nullptr;  // cartesian (2pi/a0) 3*nlf
   int *d_parg = nullptr;
+  size_t cap_lf_key = 0, cap_glf = 0, cap_parg = 0;
 
   // q
   int ntot = 0;
@@ -120,6 +121,7 @@ struct rpa_ctx {
   std::vector<double> ktot;
   int *d_k1 = nullptr, *d_k2 = nullptr, *d_m0 = nullptr;
   double *d_ktot = nullptr;
+  size_t cap_k1 = 0, cap_k2 = 0, cap_m0 = 0, cap_ktot = 0, cap_g0keys = 0, cap_g0idx = 0;
   bool have_q = false;
   // distinct G0 of the q (direct vertex map): keys, per-k index, map table
   int ng0 = 0;
@@ -170,8 +172,6 @@ struct rpa_ctx {
   double *d_V = nullptr;     // nlf x nlf
   double *d_epsinv = nullptr, *d_chi = nullptr, *d_W = nullptr;
   double *d_loss = nullptr, *d_pi11 = nullptr, *d_sig = nullptr;
-  double *d_scratch = nullptr;
-  size_t cap_scratch = 0;
   int *d_status = nullptr;
 
   // graph

@@ -49,6 +49,57 @@ def energies(sys_, rc: RunConfig) -> np.ndarray:
     return E
 
 
+def local_fields_by_qg(g: np.ndarray, mill_g: np.ndarray, gcar: float, q: np.ndarray, nlf: int):
+    """The nlf local fields of smallest |q + G| (ties by G index), for a q loop
+    whose local-field count follows a |q+G| cutoff.
+
+    g is the density G list (2pi/a units, G(1) = 0), q in the same units.
+    Returns (lf_idx 1-based, parG) with the parity rule of genGlfandParity
+    (Sloss.f90:850-854, 865-869: +1 for even Miller z, -1 for odd).
+    """
+    qg = (np.asarray(q, dtype=np.float64)[None, :] + g) * gcar
+    d2 = np.einsum("ij,ij->i", qg, qg)
+    order = np.lexsort((np.arange(len(g)), np.round(d2, 12)))[:nlf]
+    order = np.sort(order)                   # keep the G-list order, as genGlf does
+    mz = np.asarray(mill_g)[order, 2]
+    parg = np.where((mz // 2) * 2 == mz, 1, -1).astype(np.int32)
+    return (order + 1).astype(np.int32), parg
+
+
+def nonzero_weights(E, K1, K2, nv: int, nb: int, no: int, dw: float, eta: float) -> int:
+    """Number of (transition, frequency) pairs whose Sloss weight passes the
+    cutoff |L| >= 1e-3/eta, with L = -eta/(dE^2 + eta^2) and
+    dE = (io-1) dw + E(K1,n) - E(K2,m), n <= Nocc < m (Sloss.f90:407-412).
+
+    The passing bins of one transition are contiguous around the resonance;
+    their range comes from |dE| <= sqrt(eta/cut - eta^2), and both ends are
+    settled by the exact test, so the count equals a bin-by-bin evaluation.
+    """
+    cut = 1.0e-3 / eta
+    hw = math.sqrt(max(eta / cut - eta * eta, 0.0))
+    K1 = np.asarray(K1)
+    K2 = np.asarray(K2)
+    total = 0
+    for k0 in range(0, len(K1), 4096):
+        e1 = E[K1[k0:k0 + 4096] - 1][:, :nv]
+        e2 = E[K2[k0:k0 + 4096] - 1][:, nv:nb]
+        a = np.broadcast_to(e1[:, :, None], (len(e1), nv, nb - nv)).ravel()
+        b = np.broadcast_to(e2[:, None, :], (len(e1), nv, nb - nv)).ravel()
+
+        def ok(io):
+            d = (io * dw + a) - b
+            return np.abs(-eta / (d * d + eta * eta)) >= cut
+
+        c = (b - a) / dw
+        lo = np.ceil(c - hw / dw).astype(np.int64)
+        hi = np.floor(c + hw / dw).astype(np.int64)
+        lo = np.where(ok(lo - 1), lo - 1, np.where(ok(lo), lo, lo + 1))
+        hi = np.where(ok(hi + 1), hi + 1, np.where(ok(hi), hi, hi - 1))
+        n = np.minimum(hi, no - 1) - np.maximum(lo, 0) + 1
+        total += int(np.maximum(n, 0).sum())
+    return total
+
+
 def frequency_step(cfg, rc: RunConfig) -> float:
     if rc.program == "sloss":                          # Sloss.f90:212
         return (cfg.omax_ev / HARTREE - 0.0) / (cfg.no - 1)
@@ -110,6 +161,27 @@ class Runner:
                        efermi=rc.efermi_ev / HARTREE, temp=rc.temp_ev / HARTREE,
                        vcell=sys_.vcell, c0=cfg.c, eta_intra=rc.eta_intra_ev / HARTREE)
         ctx.set_q(self.K1, self.K2, self.iG0, self.ktot, self.q)
+
+    def q_point(self, iq: int, nlf: Optional[int] = None) -> dict:
+        """Host setup of one q of the Sloss q loop (findMinQ, findKQinBZ,
+        Sloss.f90:292-299): the k+q map and, with nlf given, the nlf local
+        fields of smallest |q+G|.  Nothing is sent to the device."""
+        q = _abi.find_min_q(self.ktot, iq)
+        K1, K2, iG0 = _abi.find_kq(self.ktot, self.sys.kcart, self.g, q, self.rc.eps)
+        if nlf is None:
+            idx, parg = self.lf_idx, self.parg
+        else:
+            idx, parg = local_fields_by_qg(self.g, self.sys.mill_g, self.sys.gcar, q, nlf)
+        return {"iq": iq, "q": q, "K1": K1, "K2": K2, "iG0": iG0, "lf_idx": idx, "parg": parg}
+
+    def set_q_point(self, pt: dict) -> None:
+        """Make pt (from q_point) the context's current q."""
+        if pt["lf_idx"] is not self.lf_idx:
+            self.ctx.set_local_fields(pt["lf_idx"], pt["parg"])
+            self.lf_idx, self.parg, self.nlf = pt["lf_idx"], pt["parg"], len(pt["lf_idx"])
+            self.glf = self.g[self.lf_idx - 1].copy()      # Glf(:, n) = G(:, iG)
+        self.ctx.set_q(pt["K1"], pt["K2"], pt["iG0"], self.ktot, pt["q"])
+        self.q, self.K1, self.K2, self.iG0 = pt["q"], pt["K1"], pt["K2"], pt["iG0"]
 
     def close(self):
         self.ctx.close()

@@ -1,7 +1,6 @@
 """Seeded synthetic inputs in Quantum ESPRESSO format for the RPA response path.
 
-The reference reads a QE save directory and text outputs (SURVEY Appendix B of
-the build notes restates the formats):
+The reference reads a QE save directory and these text outputs:
 
 * ``charge-density.dat``  -- Fortran unformatted: (gamma_only, ngm, nspin),
   (b1, b2, b3), mill(3, ngm)                      (Optabs_qe6.f90:1139-1157)
@@ -70,7 +69,29 @@ CONFIGS = {
                         no=701, omax_ev=30.0, seed=20261202 + 1),
     "cfg2": SynthConfig("cfg2", nk=64, nval=8, ncond=12, ngk_target=1500.0, nlf_target=51,
                         no=701, omax_ev=30.0, seed=20261202 + 2),
+    # batched W(q, omega): 100 q, per-q N_G from 21 to 61 chosen by |q+G|
+    # (host.local_fields_by_qg); nlf_target is the largest N_G
+    "cfg3": SynthConfig("cfg3", nk=48, nval=8, ncond=8, ngk_target=1000.0, lf="xyz", nlf_target=61,
+                        no=700, omax_ev=25.0, eta_ev=0.08, seed=20261202 + 3),
 }
+
+# q points of the batched-W workload: the Sloss q loop iq = 2 .. 101
+# (q = (iq-1) k_min, Sloss.f90:292, 595-597), N_G rising linearly from 21 to 61.
+BATCHED_Q = {"cfg3": dict(iq=list(range(2, 102)), nlf_min=21, nlf_max=61)}
+
+
+def batched_q_points(name: str, nq: Optional[int] = None):
+    """(iq, N_G) pairs of a batched-W workload, the first nq of them (spread
+    over the whole N_G range when nq is smaller than the list)."""
+    b = BATCHED_Q[name]
+    iqs = b["iq"]
+    n = len(iqs)
+    ngs = [b["nlf_min"] + int(round((b["nlf_max"] - b["nlf_min"]) * j / max(n - 1, 1))) for j in range(n)]
+    pts = list(zip(iqs, ngs))
+    if nq is None or nq >= n:
+        return pts
+    pick = sorted({int(round(j * (n - 1) / max(nq - 1, 1))) for j in range(nq)})
+    return [pts[j] for j in pick]
 
 
 def reciprocal_axes(a: float, c: float) -> np.ndarray:

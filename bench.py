@@ -43,23 +43,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def nnz_weights(E, K1, K2, nv, nb, no, dw, eta, program="sloss"):
-    """Number of (transition, frequency) pairs with nonzero Lorentzian weight
-    under the reference cutoff |Lor| >= 1e-3/eta (Sloss.f90:411)."""
-    o = np.arange(no) * dw
-    cut = 1.0e-3 / eta
-    total = 0
-    nk = len(K1)
-    for k0 in range(0, nk, 256):
-        ks = np.arange(k0, min(nk, k0 + 256))
-        e1 = E[K1[ks] - 1][:, :nv]            # (k, nv)
-        e2 = E[K2[ks] - 1][:, nv:nb]          # (k, nc)
-        d = (o[None, None, None, :] + e1[:, :, None, None]) - e2[:, None, :, None]
-        lor = -eta / (d * d + eta * eta)
-        total += int(np.count_nonzero(np.abs(lor) >= cut))
-    return total
-
-
 def cpu_baseline(sys_, r, threads, target_s=12.0):
     """The oracle (FP64 restatement of the reference loops) on a k-subset,
     scaled to the full k-grid; KK and Dyson/W at full size."""
@@ -197,6 +180,132 @@ def roofline_of(res, prec, flops_acc, flops_vtx, workload=""):
     }
 
 
+def time_q_loop(r, plans, steps, warmup):
+    """Warmup passes, then `steps` timed passes of the q loop: per q the
+    context takes the q point (local fields, k+q map) and runs the chain."""
+    from importlib import import_module
+    host = import_module("2dmaterialsopticalprops_amd.host")
+    ctx = r.ctx
+    for _ in range(max(warmup, 1)):
+        for pt in plans:
+            r.set_q_point(pt)
+            ctx.run()
+    ctx.set_profiling(True)
+    stages = {k: 0.0 for k in host._abi.STAGES}
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for pt in plans:
+            r.set_q_point(pt)
+            dev_ms += ctx.run()
+            for k, v in ctx.stage_ms().items():
+                stages[k] += v
+    wall = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    return {"ms_step": wall * 1e3 / steps, "device_ms": dev_ms / steps,
+            "stages": {k: v / steps for k, v in stages.items()}}
+
+
+def bench_batched(args, s, cfg, synth, host):
+    """(q, omega) W-solves/s over a q loop with per-q local-field counts."""
+    spec = synth.batched_q_points(args.config, args.nq)
+    precs = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    runs, runners, plans = {}, {}, {}
+    for prec in precs:
+        t0 = time.perf_counter()
+        r = host.Runner(s, host.RunConfig(program="sloss", vertex_map=args.vertex_map, precision=prec,
+                                          iq=spec[0][0]))
+        t1 = time.perf_counter()
+        plans[prec] = [r.q_point(iq, nlf) for iq, nlf in spec]
+        log(f"[bench] {prec}: setup+upload {t1 - t0:.1f}s, {len(spec)} q points planned "
+            f"({time.perf_counter() - t1:.1f}s), N_G {spec[0][1]}..{spec[-1][1]}")
+        res = time_q_loop(r, plans[prec], args.steps, args.warmup)
+        log(f"[bench] {prec}: {res['ms_step']:.3f} ms per q loop (device {res['device_ms']:.3f} ms); "
+            f"stages {json.dumps({k: round(v, 4) for k, v in res['stages'].items()})}")
+        runs[prec], runners[prec] = res, r
+    head = precs[0]
+    r = runners[head]
+    nsolve = len(spec) * (cfg.no - 1)
+    value = nsolve / (runs[head]["ms_step"] * 1e-3)
+
+    # algorithmic work summed over the q points
+    nv, nb = cfg.nval, cfg.nband
+    eta = cfg.eta_ev / host.HARTREE
+    flops_acc = flops_vtx = 0.0
+    ntr = len(r.K1) * nv * (nb - nv)
+    nnz_total = 0
+    for pt, (iq, nlf) in zip(plans[head], spec):
+        nnz = host.nonzero_weights(r.E, pt["K1"], pt["K2"], nv, nb, cfg.no, r.domega, eta)
+        nnz_total += nnz
+        flops_acc += 2.0 * nlf * nlf * nnz
+        flops_vtx += 8.0 * nlf * r.ngd * ntr
+    workload_key = f"{args.config} batched sloss {args.vertex_map}"
+    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key)
+    roofline["other"]["nnz_weights"] = nnz_total
+    dyson_ms = runs[head]["stages"]["dyson_or_pi"]
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "(q,omega) W-solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(runs[head]["ms_step"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": ("fp32 vertices + bf16 hi/lo x3 S0 into fp32, fp64 weights/reduction/KK/Dyson"
+                  if head == "mixed" else "f64"),
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.config}: batched W(q, omega), Sloss q loop iq={spec[0][0]}..{spec[-1][0]} "
+                        f"({len(spec)} q), {len(s.ngk)} k ({cfg.nk}x{cfg.nk}x1), {nb} bands "
+                        f"({nv}v/{nb - nv}c), ngk {s.ngk.min()}-{s.ngk.max()}, N_G {spec[0][1]}-{spec[-1][1]} "
+                        f"by |q+G|, {cfg.no} omega 0-{cfg.omax_ev:g} eV, eta={cfg.eta_ev * 1e3:g} meV",
+            "precision": head, "q_points": len(spec), "w_solves_per_step": nsolve, "transitions_per_q": ntr,
+            "device_ms_per_step": round(runs[head]["device_ms"], 4),
+            "stage_ms": {k: round(v, 4) for k, v in runs[head]["stages"].items()},
+            "dyson_w_solves_per_s": round(nsolve / (dyson_ms * 1e-3), 1) if dyson_ms > 0 else None,
+        },
+        "roofline": roofline,
+    }
+    if "fp64" in runs and head != "fp64":
+        f = runs["fp64"]
+        fv = nsolve / (f["ms_step"] * 1e-3)
+        out["fp64_gpu"] = {
+            "value": round(fv, 1), "unit": "(q,omega) W-solves/s", "ms_per_step": round(f["ms_step"], 4),
+            "stage_ms": {k: round(v, 4) for k, v in f["stages"].items()},
+            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx, workload_key),
+        }
+        out["mixed_vs_fp64"] = round(value / fv, 2)
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        try:
+            # 5 q points spanning the N_G range, each timed on a k-subset,
+            # then scaled by sum N_G^2 over all q
+            sub = sorted({int(round(j * (len(spec) - 1) / 4)) for j in range(5)})
+            t_sub, acc = 0.0, {p: [] for p in precs}
+            notes = []
+            for j in sub:
+                for p in precs:
+                    runners[p].set_q_point(plans[p][j])
+                cb = cpu_baseline(s, runners[head], threads, args.cpu_seconds / len(sub))
+                t_sub += cb["t_full_s"]
+                notes.append(f"iq={spec[j][0]} N_G={spec[j][1]}: {cb['nk_sample']} k")
+                for p in precs:
+                    acc[p].append(dict(accuracy(s, runners[p], cb), iq=spec[j][0], N_G=spec[j][1]))
+            w_all = sum(n * n for _, n in spec)
+            w_sub = sum(spec[j][1] ** 2 for j in sub)
+            t_full = t_sub * w_all / w_sub
+            out["cpu_baseline"] = {
+                "value": round(nsolve / t_full, 2), "unit": "(q,omega) W-solves/s", "cores": threads,
+                "kind": "port",
+                "sample": f"oracle chain on {len(sub)} of {len(spec)} q ({'; '.join(notes)}), S0 on a k-subset "
+                          f"scaled to {len(r.K1)} k, total scaled by sum N_G^2 over all q "
+                          f"(x{w_all / w_sub:.2f}); {t_sub:.1f}s measured",
+            }
+            out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+            out["accuracy_vs_fp64_oracle"] = acc
+        except Exception as e:  # the baseline is reported, never silently replaced
+            out["cpu_baseline"] = {"error": repr(e)}
+    for rr in runners.values():
+        rr.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,13 +318,15 @@ def main():
                     help="'both': value is the mixed path, timed next to the FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy check")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--nq", type=int, default=None,
+                    help="batched-W workloads: use this many of the q points (spread over the N_G range)")
     ap.add_argument("--hbm-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (x1.0 elementwise kernel), "
                          "the reference point for FETCH_SIZE / WRITE_SIZE counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
-              "(q points would shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(q points would shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
     if args.hbm_calibration_mib > 0:
         hbm_calibration(args.hbm_calibration_mib)
@@ -226,6 +337,8 @@ def main():
     s = synth.make_system(cfg)
     log(f"[bench] synthetic {args.config}: nk={len(s.ngk)} nband={cfg.nband} ngk={s.ngk.min()}-{s.ngk.max()} "
         f"({time.perf_counter() - t0:.1f}s)")
+    if args.config in synth.BATCHED_Q:
+        return bench_batched(args, s, cfg, synth, host)
     precs = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
     runs, runners = {}, {}
     for prec in precs:
@@ -245,7 +358,7 @@ def main():
     # algorithmic work of the two matrix-core kernels (nonzero weights only)
     nv, nb = cfg.nval, cfg.nband
     eta = cfg.eta_ev / host.HARTREE
-    nnz = nnz_weights(r.E, r.K1, r.K2, nv, nb, cfg.no, r.domega, eta)
+    nnz = host.nonzero_weights(r.E, r.K1, r.K2, nv, nb, cfg.no, r.domega, eta)
     ntr = len(r.K1) * nv * (nb - nv)
     flops_acc = 2.0 * r.nlf * r.nlf * nnz
     flops_vtx = 8.0 * r.nlf * r.ngd * ntr

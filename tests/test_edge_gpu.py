@@ -116,3 +116,40 @@ def test_second_q_on_the_same_context():
     assert H.rel_max(S0, ref) < TOL
     assert H.rel_max(first["S0"], ref) > 1e-3  # a different q
     r.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_q_loop_with_per_q_local_fields(precision):
+    """The batched-W q loop on one context: every q brings a local-field set of
+    its own size (the nearest |q+G|) and its own k+q map.  Each q gives
+    bitwise what a fresh context gives for that q alone, and matches the
+    oracle: FP64 to 1e-10; mixed to 2e-5 here, where a frequency bin holds
+    only a few of the 64 transitions, so the 2^-16 relative error of a BF16
+    hi/lo product barely averages out (the 1e-6 target is checked at
+    workload sizes by the mixed chain tests)."""
+    s = H.system("tiny", lf="xyz", nlf_target=9)
+    r = _runner(s, program="sloss", precision=precision)
+    for iq, nlf in [(2, 5), (3, 9), (5, 3), (2, 7)]:
+        pt = r.q_point(iq, nlf)
+        r.set_q_point(pt)
+        r.ctx.run()
+        assert r.nlf == nlf and r.ctx.nlf == nlf
+        got = {k: r.ctx.result(k) for k in ("S0", "EPSINV", "W")}
+        fresh = _runner(s, program="sloss", precision=precision)
+        fresh.set_q_point(pt)
+        fresh.ctx.run()
+        for k, v in got.items():
+            np.testing.assert_array_equal(v, fresh.ctx.result(k), err_msg=f"iq={iq} {k}")
+        fresh.close()
+        S0o, chi0o, Eo = _sloss_ref(s, r)
+        V = orc.genV(r.rc.eps, r.q[0] * s.gcar, r.q[1] * s.gcar, s.cfg.c, r.parg, r.glf, s.gcar)
+        Wo = np.array([orc.dyson_w(c, V)[2] for c in chi0o])
+        errs = {"S0": H.rel_fro(got["S0"], S0o), "epsinv": H.rel_fro(got["EPSINV"], Eo),
+                "W": H.rel_fro(got["W"], Wo)}
+        print(f"{precision} iq={iq} N_G={nlf}", {k: f"{v:.2e}" for k, v in errs.items()})
+        if precision == "fp64":
+            assert H.rel_max(got["S0"], S0o) < TOL and H.rel_max(got["EPSINV"], Eo) < TOL
+            assert H.rel_max(got["W"], Wo) < TOL
+        else:
+            assert max(errs.values()) < 2e-5
+    r.close()

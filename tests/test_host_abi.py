@@ -79,3 +79,57 @@ def test_gk_sort_matches_oracle_and_file_lists(name):
         np.testing.assert_array_equal(igk, igko)
         # the wavefunction file's Miller list is G(igk)
         np.testing.assert_array_equal(s.mill_g[igk - 1], s.mill_k[ik, :s.ngk[ik]])
+
+
+host = importlib.import_module("2dmaterialsopticalprops_amd.host")
+
+
+def test_nonzero_weight_count_matches_bin_by_bin():
+    """The per-transition range count equals testing every bin with the
+    reference's expression (Sloss.f90:407-412), including edge bins."""
+    rng = np.random.default_rng(7)
+    for trial in range(6):
+        nk, nv, nb, no = 30, 3, 7, 257
+        E = np.sort(rng.normal(size=(nk, nb)) * 0.3, axis=1)
+        K1 = rng.integers(1, nk + 1, size=25)
+        K2 = rng.integers(1, nk + 1, size=25)
+        dw = 0.9 / (no - 1) * (1 + 0.37 * trial)
+        eta = 0.0015 * (1 + trial)
+        want = 0
+        for k1, k2 in zip(K1, K2):
+            for n in range(nv):
+                for m in range(nv, nb):
+                    for io in range(no):
+                        dE = io * dw + E[k1 - 1, n] - E[k2 - 1, m]
+                        if abs(-eta / (dE * dE + eta * eta)) >= 1.0e-3 / eta:
+                            want += 1
+        assert host.nonzero_weights(E, K1, K2, nv, nb, no, dw, eta) == want
+
+
+def test_local_fields_by_qg():
+    s = H.system("cfg0", lf="xyz")
+    g = synth.gcart(s.mill_g, s.kc)
+    # q = 0 and a count that closes a |G| shell: the genGlf 'xyz' set itself
+    rc, n, glf, parg, idx = abi.gen_glf("xyz", 2.0, s.gcar, g, s.mill_g, 400)
+    assert rc == 0 and n > 5
+    li, pg = host.local_fields_by_qg(g, s.mill_g, s.gcar, np.zeros(3), n)
+    np.testing.assert_array_equal(li, idx)
+    np.testing.assert_array_equal(pg, parg)
+    # finite q: nearest |q+G| first, G-list order kept, parity from Miller z
+    q = abi.find_min_q(s.kcart, 4)
+    for nlf in (1, 7, 21, 40):
+        li, pg = host.local_fields_by_qg(g, s.mill_g, s.gcar, q, nlf)
+        assert len(li) == nlf and np.all(np.diff(li) > 0)
+        d = np.linalg.norm((q[None, :] + g) * s.gcar, axis=1)
+        rest = np.setdiff1d(np.arange(len(g)), li - 1)
+        assert d[li - 1].max() <= d[rest].min() + 1e-12
+        mz = s.mill_g[li - 1, 2]
+        np.testing.assert_array_equal(pg, np.where(mz % 2 == 0, 1, -1))
+
+
+def test_batched_q_points():
+    pts = synth.batched_q_points("cfg3")
+    assert len(pts) == 100 and pts[0] == (2, 21) and pts[-1] == (101, 61)
+    assert all(b >= a for (_, a), (_, b) in zip(pts, pts[1:]))
+    sub = synth.batched_q_points("cfg3", 5)
+    assert sub[0] == pts[0] and sub[-1] == pts[-1] and len(sub) == 5
