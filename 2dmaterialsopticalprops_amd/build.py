@@ -26,7 +26,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["rpa_api.hip", "k_vertex.hip", "k_accum.hip", "k_kk.hip", "k_dyson.hip", "rpa_host.cpp"]
+SOURCES = ["rpa_api.hip", "k_vertex.hip", "k_vertex_col.hip", "k_accum.hip", "k_kk.hip", "k_dyson.hip", "rpa_host.cpp"]
 
 
 def _run(cmd):

@@ -28,20 +28,14 @@
 // one k are placed on one XCD so the k+q rows they gather stay in its L2.
 #include <algorithm>
 #include <cstdlib>
+#include <string>
 
 #include "rpa_internal.h"
 
 namespace rpa {
 
-constexpr int kVertexMaxBlocks = 8;
 constexpr int kPfC1 = 4;   // prefetched C1 elements per thread
 constexpr int kPfB = 8;    // prefetched gathered elements per thread
-
-struct VBlock {
-  int n0, nrows;  // 8 bands n per block
-  int m0, ncols;  // 16 bands m per block
-  int pb;         // pair block index
-};
 
 struct VertexArgs {
   const void *evc;  // double2 (FP64) or float2 (mixed)
@@ -990,14 +984,19 @@ static void launch_lds_p(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds,
 }
 
 // Whether the LDS-resident kernel serves this context, and its LDS bytes.
+// Vertex form asked for by RPA_VERTEX_FORM ("col", "gather", "staged",
+// "lds"); empty: chosen by shape.
+static std::string vertex_form() {
+  const char *env = std::getenv("RPA_VERTEX_FORM");
+  return env ? std::string(env) : std::string();
+}
+
 static bool lds_form(const rpa_ctx *c, size_t &bytes) {
   if (c->d.program != RPA_SLOSS || c->pairs.nblocks != 1) return false;
   const PairBlock &pb = c->pairs.b[0];
   if (pb.n_hi - pb.n_lo > 8 || c->nlf > 4 * kLdsMaxGQ) return false;
-  // opt-in: measured at N_G = 51 slower than the gather-direct (FP32) and
-  // staged (FP64) forms, whose occupancy it cannot reach
-  const char *env = std::getenv("RPA_VERTEX_LDS");
-  if (!env || env[0] != '1') return false;
+  // on request only: measured at N_G = 51 slower than the other forms
+  if (vertex_form() != "lds") return false;
   const bool mixed = (c->d.precision == RPA_MIXED);
   const LdsLayout L = mixed ? lds_layout<PrecF32>(c->d.npwx, c->nlf) : lds_layout<PrecF64>(c->d.npwx, c->nlf);
   bytes = L.bytes;
@@ -1006,6 +1005,12 @@ static bool lds_form(const rpa_ctx *c, size_t &bytes) {
 
 // Outside any stream capture: allow the LDS-resident kernel its dynamic LDS.
 int configure_vertex(rpa_ctx *c) {
+  const std::string form = vertex_form();
+  if (form.empty() || form == "col") {
+    if (int e = build_col_plan(c)) return e;
+  } else {
+    c->col.ok = false;
+  }
   size_t bytes = 0;
   if (!lds_form(c, bytes) || bytes <= 64 * 1024) return 0;
   const int ngq = lds_ngq(c->nlf);
@@ -1134,6 +1139,9 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   a.rho2 = rho2;
   const long long nblocks = (long long)a.kslots * a.groups;
   if (nblocks <= 0) return 0;
+  const std::string form = vertex_form();
+  if (kind == 0 && c->col.ok && (form.empty() || form == "col"))
+    return launch_vertices_col(c, a.vb, a.nvb, rho1, perm);
   if (int e = launch_direct_map(c)) return e;
   size_t lds_bytes = 0;
   if (lds_form(c, lds_bytes)) {
@@ -1152,7 +1160,7 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   // gather-direct kernel (FP32, at most two band blocks spanning <= 16 bands
   // of n): measured faster than the staged form in FP32 and slower in FP64,
   // where its larger register set lowers the occupancy
-  const bool gd = mixed && a.nvb <= 2 && a.nna <= 16 && !std::getenv("RPA_VERTEX_STAGED");
+  const bool gd = mixed && a.nvb <= 2 && a.nna <= 16 && (form.empty() || form == "gather");
   if (gd) {
     const size_t lds_gd = 2 * 4 * (size_t)(a.nna + 1) * kGdNT * (mixed ? sizeof(float2) : sizeof(double2));
     a.ch = kGdCH;

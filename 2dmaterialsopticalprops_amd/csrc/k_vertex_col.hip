@@ -1,0 +1,557 @@
+// Charge vertices of the direct map by column correlation (Sloss).
+//
+//   rho_nm(k,q,G) = sum_{i < NGd} conj(C1_n(i)) C2_m(map(G,i))   (Sloss.f90:905-951)
+//
+// In the direct map plane wave i of C1 is the global G(i), and map(G,i) is the
+// global position of G(i) + Glf(G) + G0, used when it lies below ngk(K2)
+// (Sloss.f90:928-944).  Group the Miller triples of G(0..NGd-1) into columns
+// of equal (mx, my).  For a local field G = (lx, ly, lz) the target of
+// (mx, my, z) is (mx + lx + ox, my + ly + oy, z + lz + oz): the local fields of
+// one in-plane class (lx, ly) differ only by a shift lz along the column.  So
+// the k+q rows of a column -- the target column, z from zlo + lzmin to
+// zhi + lzmax -- are staged in LDS once and serve every local field of the
+// class at an offset of lz - lzmin rows.  The sum becomes a correlation along
+// z: each staged k+q value enters up to (fields of the class) products, and
+// L2 delivers a k+q row once per class instead of once per local field.
+//
+// Matrix cores: rows are (8 bands n) x (Re, Im) of the real embedding of
+// conj(C1), K runs over positions (plane waves), columns are (4 shifts) x
+// (4 bands m); Re and Im of C2 are two MFMAs.  One accumulator per tile
+// (group of 4 shifts, group of 4 bands m); the four waves split the tiles and
+// each walks every position of the chunk, loading the position's A value and
+// B row offset once and reusing them for all of its tiles.  The summation
+// runs over positions in column order instead of the reference's i order.
+//
+// Workgroup = (k, in-plane class, band block); workgroups of one k on one XCD.
+#include <algorithm>
+#include <cstdlib>
+#include <numeric>
+#include <string>
+
+#include "rpa_internal.h"
+
+namespace rpa {
+
+namespace {
+
+constexpr int kColInfo = 2 + kColMaxShift / 4;  // per class: rowoff, nsg, sg list
+constexpr size_t kColLdsBudget = 80 * 1024; // default: two workgroups per CU
+
+struct ColArgs {
+  const void *evc;
+  const int *ngk, *k1, *k2, *g0idx;
+  const int *slot;   // [npos] C1 slot (global index) of position p
+  const int *prow;   // [ncls][npos] B row of p relative to its chunk
+  const int *chunk;  // [nchunk + 1] first position of each chunk
+  const int *crow;   // [ncls][nchunk + 1] first B row of each chunk in the class
+  const int *bmap;   // [ng0][nrows] global index of each B row's target, -1 none
+  const int *shift;  // [ncls][kColMaxShift] local field of each shift slot, -1 none
+  const int *cls;    // [ncls][kColInfo]
+  int npos, nchunk, ncls, nrows;
+  int nband, npwx, nlf, ntot;
+  int blocks;        // ncls * nvb
+  int ms;            // complex values per staged k+q row (bank-conflict-free stride)
+  int nvb;
+  VBlock vb[kVertexMaxBlocks];
+  PairTable pairs;
+  const int *perm;
+  void *rho;
+};
+
+struct ColF64 {
+  using real = double;
+  using cplx = double2;
+  using acc = rpa_d4;
+  static __device__ inline acc mma(real a, real b, acc c) { return mfma_f64_16x16x4(a, b, c); }
+  static __device__ inline int arow_band(int r) { return r & 7; }
+  static __device__ inline bool arow_im(int r) { return r >= 8; }
+  static __device__ inline int out_band(int q4, int p) { return 4 * p + q4; }
+  static __device__ inline int out_re(int p) { return p; }
+  static __device__ inline int out_im(int p) { return p + 2; }
+  static __device__ inline cplx make(real x, real y) { return make_double2(x, y); }
+  static __device__ inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+};
+
+struct ColF32 {
+  using real = float;
+  using cplx = float2;
+  using acc = rpa_f4;
+  static __device__ inline acc mma(real a, real b, acc c) { return mfma_f32_16x16x4(a, b, c); }
+  static __device__ inline int arow_band(int r) { return r >> 1; }
+  static __device__ inline bool arow_im(int r) { return (r & 1) != 0; }
+  static __device__ inline int out_band(int q4, int p) { return 2 * q4 + p; }
+  static __device__ inline int out_re(int p) { return 2 * p; }
+  static __device__ inline int out_im(int p) { return 2 * p + 1; }
+  static __device__ inline cplx make(real x, real y) { return make_float2(x, y); }
+  static __device__ inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+};
+
+// Staging of one chunk in registers (loaded one chunk ahead, stored to LDS
+// after the barrier).  Thread t: C1 rows of positions t/8 + 32u, band t%8;
+// k+q rows t/4 + 64u, bands m 4(t%4)..+3; B row of position t (t < KC).
+template <class P, int KC, int NRU>
+struct ColStage {
+  typename P::cplx a[KC / 32];
+  typename P::cplx b[NRU][4];
+  int prow;
+};
+
+// PIPE: the next chunk is loaded into registers while the current one runs
+// (fewer waves, hidden latency) instead of loaded between the barriers.
+template <class P, int TPW, int KC, int NRU, bool PIPE>
+__global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
+  using real = typename P::real;
+  using cplx = typename P::cplx;
+  using acc_t = typename P::acc;
+  extern __shared__ __align__(16) unsigned char col_raw[];
+  cplx *sA = (cplx *)col_raw;         // [KC][8]   C1 of position p, band n
+  int *sP = (int *)(sA + KC * 8);     // [KC]      B row of position p
+  cplx *sB = (cplx *)(sP + KC);       // [rows][ms] C2 of the targets, bands m
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int L = blockIdx.x;
+  const int xcd = L & 7;
+  const int slot = L >> 3;
+  const int k = xcd + 8 * (slot / a.blocks);
+  if (k >= a.ntot) return;  // whole workgroup leaves together
+  const int rest = slot % a.blocks;
+  const int cl = rest % a.ncls;
+  const VBlock vb = a.vb[rest / a.ncls];
+
+  const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
+  const int lim2 = a.ngk[K2];
+  const cplx *evc = (const cplx *)a.evc;
+  const cplx *c1 = evc + (size_t)K1 * a.npwx * a.nband + vb.n0;
+  const cplx *c2 = evc + (size_t)K2 * a.npwx * a.nband + vb.m0;
+  const int *info = a.cls + (size_t)cl * kColInfo;
+  const int *bmap = a.bmap + (size_t)a.g0idx[k] * a.nrows + info[0];
+  const int *prow = a.prow + (size_t)cl * a.npos;
+  const int *crow = a.crow + (size_t)cl * (a.nchunk + 1);
+  const int nmg = (vb.ncols + 3) >> 2;
+  const int nt = info[1] * nmg;  // tiles of this class and block
+
+  // ---- staging roles
+  const int sa_p = tid >> 3, sa_n = tid & 7;       // C1: position sa_p + 32u, band sa_n
+  const int sb_r = tid >> 2, sb_m = 4 * (tid & 3);  // k+q: row sb_r + 64u, bands sb_m..+3
+  ColStage<P, KC, NRU> st;
+  auto load = [&](int ch) {
+    const int p0 = a.chunk[ch], np = a.chunk[ch + 1] - p0;
+    const int r0 = crow[ch], nr = crow[ch + 1] - r0;
+#pragma unroll
+    for (int u = 0; u < KC / 32; ++u) {
+      const int p = sa_p + 32 * u;
+      st.a[u] = (p < np && sa_n < vb.nrows) ? c1[(size_t)a.slot[p0 + p] * a.nband + sa_n] : P::make(0, 0);
+    }
+    st.prow = (tid < KC && tid < np) ? prow[p0 + tid] : 0;
+    int j[NRU];
+#pragma unroll
+    for (int u = 0; u < NRU; ++u) {
+      const int rr = sb_r + 64 * u;
+      const int jj = (rr < nr) ? bmap[r0 + rr] : -1;
+      j[u] = (jj >= 0 && jj < lim2) ? jj : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < NRU; ++u)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        st.b[u][e] = (j[u] >= 0 && sb_m + e < vb.ncols) ? c2[(size_t)j[u] * a.nband + sb_m + e] : P::make(0, 0);
+  };
+  auto store = [&](int ch) {
+    const int nr = crow[ch + 1] - crow[ch];
+#pragma unroll
+    for (int u = 0; u < KC / 32; ++u) sA[(sa_p + 32 * u) * 8 + sa_n] = st.a[u];
+    if (tid < KC) sP[tid] = st.prow;
+#pragma unroll
+    for (int u = 0; u < NRU; ++u) {
+      const int rr = sb_r + 64 * u;
+      if (rr < nr) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (sb_m + e < a.ms) sB[rr * a.ms + sb_m + e] = st.b[u][e];
+      }
+    }
+  };
+
+  // ---- matrix-core roles
+  const int r = lane & 15, kg = lane >> 4;
+  const int band_r = P::arow_band(r);
+  const bool im_row = P::arow_im(r);
+  // Row stride ms: the 7 rows x 4 bands one step reads fall in distinct banks
+  const int lane_off = (r & 3) * a.ms + (r >> 2);  // shift r&3, band m r>>2
+  int toff[TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int T = wave + 4 * tt;
+    toff[tt] = (T < nt) ? 4 * info[2 + T / nmg] * a.ms + 4 * (T % nmg) : 0;
+  }
+  acc_t acc[TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) acc[tt] = P::zero();
+
+  if (PIPE) load(0);
+  for (int ch = 0; ch < a.nchunk; ++ch) {
+    if (!PIPE) load(ch);
+    __syncthreads();  // the previous chunk's reads are done
+    store(ch);
+    __syncthreads();
+    if (PIPE && ch + 1 < a.nchunk) load(ch + 1);  // in flight during this chunk's products
+    const int nsteps = (a.chunk[ch + 1] - a.chunk[ch] + 3) >> 2;
+    // the position's A value and B row are read one step ahead
+    int prow_c = sP[kg];
+    cplx za_c = sA[kg * 8 + band_r];
+    for (int t = 0; t < nsteps; ++t) {
+      const int pn = min(4 * (t + 1) + kg, KC - 1);
+      const int prow_n = sP[pn];
+      const cplx za_n = sA[pn * 8 + band_r];
+      const cplx *brow = sB + prow_c * a.ms + lane_off;
+      const real are = im_row ? -za_c.y : za_c.x;
+      const real aim = im_row ? za_c.x : za_c.y;
+      // every wave runs TPW tiles; tiles past nt read row group 0 and their
+      // results are dropped (no branch in the loop, so the LDS reads of all
+      // tiles are issued ahead of the products)
+#pragma unroll
+      for (int tt = 0; tt < TPW; ++tt) {
+        const cplx zb = brow[toff[tt]];
+        acc[tt] = P::mma(are, zb.x, acc[tt]);
+        acc[tt] = P::mma(aim, zb.y, acc[tt]);
+      }
+      prow_c = prow_n;
+      za_c = za_n;
+    }
+  }
+
+  // results: tile (sg, mg), lane column r -> shift 4 sg + (r & 3), band m 4 mg + (r >> 2)
+  const PairBlock &pbk = a.pairs.b[vb.pb];
+  const int nm = pbk.m_hi - pbk.m_lo;
+  cplx *rho = (cplx *)a.rho;
+  const int *shift = a.shift + (size_t)cl * kColMaxShift;
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int T = wave + 4 * tt;
+    if (T >= nt) continue;
+    const int sg = info[2 + T / nmg];
+    const int iG = shift[4 * sg + (r & 3)];
+    const int mo = 4 * (T % nmg) + (r >> 2);
+    if (iG < 0 || mo >= vb.ncols) continue;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int onr = P::out_band(kg, p);
+      if (onr >= vb.nrows) continue;
+      const int n = vb.n0 + onr, m = vb.m0 + mo;
+      const long long tt_idx = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
+                               (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
+      const long long pos = a.perm ? (long long)a.perm[tt_idx] : tt_idx;
+      if (pos >= 0) rho[pos * a.nlf + iG] = P::make(acc[tt][P::out_re(p)], acc[tt][P::out_im(p)]);
+    }
+  }
+}
+
+template <class P, int KC, int NRU, bool PIPE>
+const void *col_kernel(int tpw) {
+  switch (tpw) {
+    case 1: return (const void *)k_vertex_col<P, 1, KC, NRU, PIPE>;
+    case 2: return (const void *)k_vertex_col<P, 2, KC, NRU, PIPE>;
+    case 4: return (const void *)k_vertex_col<P, 4, KC, NRU, PIPE>;
+    case 6: return (const void *)k_vertex_col<P, 6, KC, NRU, PIPE>;
+    case 8: return (const void *)k_vertex_col<P, 8, KC, NRU, PIPE>;
+    case 10: return (const void *)k_vertex_col<P, 10, KC, NRU, PIPE>;
+    case 13: return (const void *)k_vertex_col<P, 13, KC, NRU, PIPE>;
+    default: return (const void *)k_vertex_col<P, 16, KC, NRU, PIPE>;
+  }
+}
+
+template <class P, int KC, int NRU, bool PIPE>
+void launch_col_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
+  const size_t lds = c->col.lds;
+  switch (tpw) {
+    case 1: hipLaunchKernelGGL((k_vertex_col<P, 1, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_vertex_col<P, 2, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_vertex_col<P, 4, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 6: hipLaunchKernelGGL((k_vertex_col<P, 6, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 8: hipLaunchKernelGGL((k_vertex_col<P, 8, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 10: hipLaunchKernelGGL((k_vertex_col<P, 10, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 13: hipLaunchKernelGGL((k_vertex_col<P, 13, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_vertex_col<P, 16, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+  }
+}
+
+// 1 when RPA_VERTEX_FORM=col asks for this form regardless of shape
+int vertex_form_forced() {
+  const char *env = std::getenv("RPA_VERTEX_FORM");
+  return (env && std::string(env) == "col") ? 1 : 0;
+}
+
+int tpw_of(int tiles) {
+  const int w = (tiles + 3) / 4;
+  for (int t : {1, 2, 4, 6, 8, 10, 13, 16})
+    if (w <= t) return t;
+  return 0;
+}
+
+// positions per chunk and staged k+q rows per thread (rows <= 64 NRU)
+constexpr int kKcF32 = 128, kKcF64 = 64;
+constexpr int kNruF32 = 8, kNruF64 = 4;
+
+}  // namespace
+
+void free_col_plan(rpa_ctx *c) {
+  ColPlan &P = c->col;
+  for (void *b : {(void *)P.d_slot, (void *)P.d_prow, (void *)P.d_chunk, (void *)P.d_crow, (void *)P.d_bmap,
+                  (void *)P.d_shift, (void *)P.d_cls})
+    if (b) (void)hipFree(b);
+  P = ColPlan{};
+}
+
+// The plan of the current q: positions in column order, chunks that fit the
+// LDS budget, in-plane classes of the local fields with their shift slots,
+// and for every distinct G0 the global index of each staged k+q row.
+// Leaves col.ok false (the other vertex forms run) where it does not apply.
+int build_col_plan(rpa_ctx *c) {
+  ColPlan &P = c->col;
+  P.ok = false;
+  if (c->d.program != RPA_SLOSS || c->p.vertex_map != RPA_MAP_DIRECT) return 0;
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  const int kc = mixed ? kKcF32 : kKcF64;
+  const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
+  const int ngd = c->p.ngd;
+  const int *mg = c->mill_g.data();
+
+  // positions: G(i), i < NGd, ordered by (mx, my, mz, i); segments: runs of
+  // one column, at most kc long
+  std::vector<int> ord(ngd);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+    for (int d = 0; d < 3; ++d)
+      if (mg[3 * x + d] != mg[3 * y + d]) return mg[3 * x + d] < mg[3 * y + d];
+    return false;
+  });
+  struct Seg {
+    int mx, my, zlo, zhi, p0, p1;
+  };
+  std::vector<Seg> segs;
+  for (int p = 0; p < ngd; ++p) {
+    const int *m = mg + 3 * ord[p];
+    if (segs.empty() || segs.back().mx != m[0] || segs.back().my != m[1] || segs.back().p1 - segs.back().p0 >= kc)
+      segs.push_back(Seg{m[0], m[1], m[2], m[2], p, p + 1});
+    else {
+      segs.back().zhi = m[2];
+      segs.back().p1 = p + 1;
+    }
+  }
+
+  // in-plane classes of the local fields
+  struct Cls {
+    int lx, ly, lzmin, lzmax;
+    std::vector<int> fields;
+  };
+  std::vector<Cls> cls;
+  for (int g = 0; g < c->nlf; ++g) {
+    const int *m = mg + 3 * (c->lf_idx[g] - 1);
+    size_t u = 0;
+    while (u < cls.size() && (cls[u].lx != m[0] || cls[u].ly != m[1])) ++u;
+    if (u == cls.size()) cls.push_back(Cls{m[0], m[1], m[2], m[2], {}});
+    cls[u].lzmin = std::min(cls[u].lzmin, m[2]);
+    cls[u].lzmax = std::max(cls[u].lzmax, m[2]);
+    cls[u].fields.push_back(g);
+  }
+  const int ncls = (int)cls.size();
+  // many classes of few fields (|q+G| sets in all directions) stage a k+q
+  // row set per class for little reuse; there the gather forms are faster
+  // (measured at 4-9 fields per class)
+  if (c->nlf < 12 * ncls && vertex_form_forced() != 1) return 0;
+  std::vector<int> shift((size_t)ncls * kColMaxShift, -1), info((size_t)ncls * kColInfo, 0);
+  int spanmax = 0, padmax = 0, tilemax = 0;
+  for (int u = 0; u < ncls; ++u) {
+    const int span = cls[u].lzmax - cls[u].lzmin;
+    const int nslot = 4 * ((span + 4) / 4);
+    if (nslot > kColMaxShift) return 0;
+    for (int g : cls[u].fields) {
+      const int s = mg[3 * (c->lf_idx[g] - 1) + 2] - cls[u].lzmin;
+      if (shift[(size_t)u * kColMaxShift + s] >= 0) return 0;  // repeated local field
+      shift[(size_t)u * kColMaxShift + s] = g;
+    }
+    int nsg = 0;
+    for (int sg = 0; sg < nslot / 4; ++sg) {
+      bool used = false;
+      for (int s = 4 * sg; s < 4 * sg + 4; ++s) used = used || shift[(size_t)u * kColMaxShift + s] >= 0;
+      if (used) info[(size_t)u * kColInfo + 2 + nsg++] = sg;
+    }
+    info[(size_t)u * kColInfo + 1] = nsg;
+    spanmax = std::max(spanmax, span);
+    padmax = std::max(padmax, nslot - 1 - span);
+    tilemax = std::max(tilemax, nsg * 4);  // at most 4 groups of bands m
+  }
+  const int tpw = tpw_of(tilemax);
+  if (tpw == 0) return 0;
+  // k+q row stride: the widest band block's m groups, padded so that 7
+  // consecutive rows of 4 bands map to distinct LDS banks
+  int nmg = 1;
+  for (int b = 0; b < c->pairs.nblocks; ++b)
+    nmg = std::max(nmg, (std::min(16, c->pairs.b[b].m_hi - c->pairs.b[b].m_lo) + 3) / 4);
+  const int ms = (nmg == 1) ? 4 : (nmg <= 3) ? 12 : 20;
+  // loading the next chunk during the products measured faster in both
+  // precisions at N_G = 51 (FP32 3.2 vs 4.6 ms, FP64 5.3 vs 5.4 ms)
+  bool pipe = true;
+  if (const char *env = std::getenv("RPA_VERTEX_COL_PIPE")) pipe = env[0] == '1';
+
+  // chunks: consecutive segments, at most kc positions and rmax rows
+  const size_t fixed = (size_t)kc * (8 * csz + sizeof(int));
+  size_t budget = kColLdsBudget;
+  if (const char *env = std::getenv("RPA_VERTEX_COL_LDS")) budget = std::max<size_t>(fixed + 4096, 1024 * (size_t)std::atoi(env));
+  budget = std::min<size_t>(budget, 160 * 1024);
+  const int rmax = std::min((int)((budget - fixed) / (ms * csz)), 64 * (mixed ? kNruF32 : kNruF64));
+  std::vector<int> chunk{0};
+  std::vector<std::pair<int, int>> chunk_segs;  // [first seg, end seg)
+  {
+    int s0 = 0, npos = 0, rows = padmax;
+    for (int s = 0; s < (int)segs.size(); ++s) {
+      const int cnt = segs[s].p1 - segs[s].p0;
+      const int ext = segs[s].zhi - segs[s].zlo + 1 + spanmax;
+      if (ext + padmax > rmax) return 0;
+      if (s > s0 && (npos + cnt > kc || rows + ext > rmax)) {
+        chunk_segs.push_back({s0, s});
+        chunk.push_back(segs[s].p0);
+        s0 = s;
+        npos = 0;
+        rows = padmax;
+      }
+      npos += cnt;
+      rows += ext;
+    }
+    chunk_segs.push_back({s0, (int)segs.size()});
+    chunk.push_back(ngd);
+  }
+  const int nchunk = (int)chunk_segs.size();
+
+  // per class: B rows of each chunk, chunk-relative row of each position
+  std::vector<int> prow((size_t)ncls * ngd), crow((size_t)ncls * (nchunk + 1));
+  std::vector<int> rowoff(ncls);
+  int nrows = 0, rows_used = 0;
+  for (int u = 0; u < ncls; ++u) {
+    const int span = cls[u].lzmax - cls[u].lzmin;
+    const int pad = 4 * ((span + 4) / 4) - 1 - span;
+    rowoff[u] = nrows;
+    info[(size_t)u * kColInfo] = nrows;
+    int r = 0;
+    for (int ch = 0; ch < nchunk; ++ch) {
+      crow[(size_t)u * (nchunk + 1) + ch] = r;
+      int rs = 0;
+      for (int s = chunk_segs[ch].first; s < chunk_segs[ch].second; ++s) {
+        for (int p = segs[s].p0; p < segs[s].p1; ++p)
+          prow[(size_t)u * ngd + p] = rs + (mg[3 * ord[p] + 2] - segs[s].zlo);
+        rs += segs[s].zhi - segs[s].zlo + 1 + span;
+      }
+      rs += pad;
+      rows_used = std::max(rows_used, rs);
+      r += rs;
+    }
+    crow[(size_t)u * (nchunk + 1) + nchunk] = r;
+    nrows += r;
+  }
+  // target global index of every B row, per distinct G0
+  const int ng0 = (int)c->g0_list.size();
+  std::vector<int> bmap((size_t)ng0 * nrows, -1);
+  for (int z = 0; z < ng0; ++z) {
+    const int *o = mg + 3 * (c->g0_list[z] - 1);
+    for (int u = 0; u < ncls; ++u) {
+      const int span = cls[u].lzmax - cls[u].lzmin;
+      for (int ch = 0; ch < nchunk; ++ch) {
+        int row = rowoff[u] + crow[(size_t)u * (nchunk + 1) + ch];
+        for (int s = chunk_segs[ch].first; s < chunk_segs[ch].second; ++s) {
+          const int ext = segs[s].zhi - segs[s].zlo + 1 + span;
+          for (int e = 0; e < ext; ++e, ++row) {
+            const int bi = c->gbox.index(segs[s].mx + cls[u].lx + o[0], segs[s].my + cls[u].ly + o[1],
+                                         segs[s].zlo + cls[u].lzmin + e + o[2]);
+            bmap[(size_t)z * nrows + row] = bi >= 0 ? c->h_gbox[bi] : -1;
+          }
+        }
+      }
+    }
+  }
+  std::vector<int> slot(ngd);
+  for (int p = 0; p < ngd; ++p) slot[p] = ord[p];
+
+  int e = ensure(c, P.d_slot, P.cap_slot, (size_t)ngd);
+  if (!e) e = ensure(c, P.d_prow, P.cap_prow, prow.size());
+  if (!e) e = ensure(c, P.d_chunk, P.cap_chunk, chunk.size());
+  if (!e) e = ensure(c, P.d_crow, P.cap_crow, crow.size());
+  if (!e) e = ensure(c, P.d_bmap, P.cap_bmap, std::max<size_t>(bmap.size(), 1));
+  if (!e) e = ensure(c, P.d_shift, P.cap_shift, shift.size());
+  if (!e) e = ensure(c, P.d_cls, P.cap_cls, info.size());
+  if (e) return e;
+  RPA_HIP(c, hipMemcpy(P.d_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(P.d_prow, prow.data(), sizeof(int) * prow.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(P.d_chunk, chunk.data(), sizeof(int) * chunk.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(P.d_crow, crow.data(), sizeof(int) * crow.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(P.d_bmap, bmap.data(), sizeof(int) * bmap.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(P.d_shift, shift.data(), sizeof(int) * shift.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(P.d_cls, info.data(), sizeof(int) * info.size(), hipMemcpyHostToDevice));
+  P.npos = ngd;
+  P.nchunk = nchunk;
+  P.ncls = ncls;
+  P.nrows = nrows;
+  P.kc = kc;
+  P.tpw = tpw;
+  P.ms = ms;
+  P.pipe = pipe;
+  P.lds = fixed + (size_t)rows_used * ms * csz;
+  if (P.lds > 64 * 1024) {
+    const void *fn = mixed ? (pipe ? col_kernel<ColF32, kKcF32, kNruF32, true>(tpw)
+                                   : col_kernel<ColF32, kKcF32, kNruF32, false>(tpw))
+                           : (pipe ? col_kernel<ColF64, kKcF64, kNruF64, true>(tpw)
+                                   : col_kernel<ColF64, kKcF64, kNruF64, false>(tpw));
+    RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds));
+  }
+  P.ok = true;
+  return 0;
+}
+
+int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, const int *perm) {
+  const ColPlan &P = c->col;
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  ColArgs a{};
+  a.evc = mixed ? (const void *)c->d_evc32 : (const void *)c->d_evc;
+  a.ngk = c->d_ngk;
+  a.k1 = c->d_k1;
+  a.k2 = c->d_k2;
+  a.g0idx = c->d_g0idx;
+  a.slot = P.d_slot;
+  a.prow = P.d_prow;
+  a.chunk = P.d_chunk;
+  a.crow = P.d_crow;
+  a.bmap = P.d_bmap;
+  a.shift = P.d_shift;
+  a.cls = P.d_cls;
+  a.npos = P.npos;
+  a.nchunk = P.nchunk;
+  a.ncls = P.ncls;
+  a.nrows = P.nrows;
+  a.nband = c->d.nband;
+  a.npwx = c->d.npwx;
+  a.nlf = c->nlf;
+  a.ntot = c->ntot;
+  a.nvb = nvb;
+  for (int b = 0; b < nvb; ++b) a.vb[b] = vb[b];
+  a.blocks = P.ncls * nvb;
+  a.pairs = c->pairs;
+  a.perm = perm;
+  a.rho = rho;
+  const long long kslots = (c->ntot + 7) / 8 * 8;
+  const dim3 grid((unsigned)(kslots * a.blocks));
+  a.ms = P.ms;
+  if (mixed) {
+    if (P.pipe)
+      launch_col_p<ColF32, kKcF32, kNruF32, true>(c, a, P.tpw, grid);
+    else
+      launch_col_p<ColF32, kKcF32, kNruF32, false>(c, a, P.tpw, grid);
+  } else {
+    if (P.pipe)
+      launch_col_p<ColF64, kKcF64, kNruF64, true>(c, a, P.tpw, grid);
+    else
+      launch_col_p<ColF64, kKcF64, kNruF64, false>(c, a, P.tpw, grid);
+  }
+  RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace rpa

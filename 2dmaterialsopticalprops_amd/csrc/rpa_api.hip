@@ -323,6 +323,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
+  free_col_plan(c);
   for (hipEvent_t ev : c->prof_ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -364,6 +365,7 @@ int rpa_set_gvectors(rpa_ctx *c, int ngm, const int *mill, const double *kc) {
     int bi = c->gbox.index(mill[3 * i], mill[3 * i + 1], mill[3 * i + 2]);
     if (box[bi] < 0) box[bi] = i;
   }
+  c->h_gbox = box;
   if (c->d_mill_g) (void)hipFree(c->d_mill_g);
   if (c->d_gbox) (void)hipFree(c->d_gbox);
   c->d_mill_g = c->d_gbox = nullptr;
@@ -510,6 +512,7 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   c->ktot.assign(ktot, ktot + 3 * (size_t)ntot);
   for (int a = 0; a < 3; ++a) c->q[a] = q[a];
   c->ng0 = (int)g0keys.size();
+  c->g0_list = g0seen;
   const size_t nt = (size_t)ntot;
   int e = ensure(c, c->d_k1, c->cap_k1, nt);
   if (!e) e = ensure(c, c->d_k2, c->cap_k2, nt);

@@ -69,6 +69,28 @@ struct PairTable {
   PairBlock b[kMaxPairBlocks];
 };
 
+// One vertex work block: up to 8 bands n by up to 16 bands m of a pair block.
+constexpr int kVertexMaxBlocks = 8;
+struct VBlock {
+  int n0, nrows;  // 8 bands n per block
+  int m0, ncols;  // 16 bands m per block
+  int pb;         // pair block index
+};
+
+// Column-correlation plan of the charge vertices in the direct map (built
+// per q on the host, see k_vertex_col.hip).
+constexpr int kColMaxShift = 128;  // shift slots of one in-plane class
+struct ColPlan {
+  bool ok = false;
+  int npos = 0, nchunk = 0, ncls = 0, nrows = 0;  // nrows: B rows of all classes (one G0)
+  int kc = 0, tpw = 0, ms = 0;
+  bool pipe = false;
+  size_t lds = 0;
+  int *d_slot = nullptr, *d_prow = nullptr, *d_chunk = nullptr, *d_crow = nullptr;
+  int *d_bmap = nullptr, *d_shift = nullptr, *d_cls = nullptr;
+  size_t cap_slot = 0, cap_prow = 0, cap_chunk = 0, cap_crow = 0, cap_bmap = 0, cap_shift = 0, cap_cls = 0;
+};
+
 }  // namespace rpa
 
 struct rpa_ctx {
@@ -87,6 +109,7 @@ struct rpa_ctx {
   int *d_mill_g = nullptr;  // packed keys [ngm]
   int *d_gbox = nullptr;    // global box -> index
   rpa::Box gbox{};
+  std::vector<int> h_gbox;  // host copy of the global box
 
   // basis
   void *d_evc = nullptr;
@@ -125,7 +148,9 @@ struct rpa_ctx {
   bool have_q = false;
   // distinct G0 of the q (direct vertex map): keys, per-k index, map table
   int ng0 = 0;
+  std::vector<int> g0_list;  // distinct iG0 (1-based) in order of appearance
   int *d_g0keys = nullptr, *d_g0idx = nullptr, *d_dmap = nullptr;
+  rpa::ColPlan col;          // column-correlation vertex plan
   size_t cap_dmap = 0;
   double *d_wkk = nullptr;  // W1(no-1), W2(no-1) of the W Kramers-Kronig check
   size_t cap_wkk = 0;
@@ -217,6 +242,9 @@ int launch_build_kbox(rpa_ctx *c);
 int launch_transitions(rpa_ctx *c);
 int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm);
 int launch_direct_map(rpa_ctx *c);
+int build_col_plan(rpa_ctx *c);
+int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, const int *perm);
+void free_col_plan(rpa_ctx *c);
 int launch_w_kk(rpa_ctx *c);
 int launch_accumulate(rpa_ctx *c);
 int launch_kstencil(rpa_ctx *c);

@@ -120,9 +120,9 @@ def time_chain(r, steps, warmup):
             "stages": {k: v / steps for k, v in stages.items()}}
 
 
-KERNEL_PREFIX = {  # (precision, stage) -> kernel name in the rocprof tables
-    ("mixed", "vertices"): "rpa::k_vertex_gd<rpa::PrecF32", ("fp64", "vertices"): "rpa::k_vertex<rpa::PrecF64",
-    ("mixed", "accumulate"): "rpa::k_accum_mx<", ("fp64", "accumulate"): "rpa::k_accum<",
+KERNEL_NAME = {  # (precision, stage) -> substrings of the kernel name in the rocprof tables
+    ("mixed", "vertices"): ("k_vertex_col<", "ColF32"), ("fp64", "vertices"): ("k_vertex_col<", "ColF64"),
+    ("mixed", "accumulate"): ("k_accum_mx<",), ("fp64", "accumulate"): ("k_accum<",),
 }
 
 
@@ -144,19 +144,30 @@ def hbm_traffic(workload, prec, stage):
             rows[f[0]] = float(f[1]) + float(f[2])
     if wl != workload:
         return None
-    pre = KERNEL_PREFIX[(prec, stage)]
-    hits = [v for k, v in rows.items() if k.startswith(pre)]
+    parts = KERNEL_NAME[(prec, stage)]
+    hits = [v for k, v in rows.items() if all(p in k for p in parts)]
     return hits[0] if len(hits) == 1 else None
 
 
-def roofline_of(res, prec, flops_acc, flops_vtx, workload=""):
+def vertex_kernel(r, prec):
+    """Name of the vertex kernel the library picks for the runner's local
+    fields (the column form when the in-plane classes hold at least 12 fields
+    on average, else the gather-direct FP32 or staged FP64 form)."""
+    m = r.sys.mill_g[np.asarray(r.lf_idx) - 1]
+    ncls = len({(int(x[0]), int(x[1])) for x in m})
+    if r.rc.vertex_map == "direct" and len(m) >= 12 * ncls:
+        return "k_vertex_col<ColF32>" if prec == "mixed" else "k_vertex_col<ColF64>"
+    return "k_vertex_gd<PrecF32>" if prec == "mixed" else "k_vertex<PrecF64> (staged)"
+
+
+def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None):
     """Roofline of the dominant matrix-core kernel of one run."""
     acc_ms, vtx_ms = res["stages"]["accumulate"], res["stages"]["vertices"]
     if prec == "mixed":
-        names = ("k_accum_mx (S0, BF16 hi/lo x3 into FP32)", "k_vertex<PrecF32> (vertices, FP32)")
+        names = ("k_accum_mx (S0, BF16 hi/lo x3 into FP32)", f"{vname or 'k_vertex_col<ColF32>'} (vertices, FP32)")
         peaks = (BF16X3_PEAK_TFLOPS, FP32_PEAK_TFLOPS)
     else:
-        names = ("k_accum (S0, FP64)", "k_vertex<PrecF64> (vertices, FP64)")
+        names = ("k_accum (S0, FP64)", f"{vname or 'k_vertex_col<ColF64>'} (vertices, FP64)")
         peaks = (FP64_PEAK_TFLOPS, FP64_PEAK_TFLOPS)
     if acc_ms >= vtx_ms:
         dom, fl, ms, peak, stage = names[0], flops_acc, acc_ms, peaks[0], "accumulate"
@@ -240,7 +251,7 @@ def bench_batched(args, s, cfg, synth, host):
         flops_acc += 2.0 * nlf * nlf * nnz
         flops_vtx += 8.0 * nlf * r.ngd * ntr
     workload_key = f"{args.config} batched sloss {args.vertex_map}"
-    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key)
+    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key, vertex_kernel(r, head))
     roofline["other"]["nnz_weights"] = nnz_total
     dyson_ms = runs[head]["stages"]["dyson_or_pi"]
     out = {
@@ -268,7 +279,8 @@ def bench_batched(args, s, cfg, synth, host):
         out["fp64_gpu"] = {
             "value": round(fv, 1), "unit": "(q,omega) W-solves/s", "ms_per_step": round(f["ms_step"], 4),
             "stage_ms": {k: round(v, 4) for k, v in f["stages"].items()},
-            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx, workload_key),
+            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx, workload_key,
+                                    vertex_kernel(runners["fp64"], "fp64")),
         }
         out["mixed_vs_fp64"] = round(value / fv, 2)
     if not args.no_cpu:
@@ -363,7 +375,7 @@ def main():
     flops_acc = 2.0 * r.nlf * r.nlf * nnz
     flops_vtx = 8.0 * r.nlf * r.ngd * ntr
     workload_key = f"{args.config} {args.program} {args.vertex_map}"
-    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key)
+    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key, vertex_kernel(r, head))
     roofline["other"]["nnz_weights"] = nnz
 
     out = {
@@ -391,7 +403,8 @@ def main():
         out["fp64_gpu"] = {
             "value": round(fv, 1), "unit": "omega-points/s", "ms_per_step": round(f["ms_step"], 4),
             "stage_ms": {k: round(v, 4) for k, v in f["stages"].items()},
-            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx, workload_key),
+            "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx, workload_key,
+                                    vertex_kernel(runners["fp64"], "fp64")),
         }
         out["mixed_vs_fp64"] = round(value / fv, 2)
     if not args.no_cpu:

@@ -153,3 +153,27 @@ def test_q_loop_with_per_q_local_fields(precision):
         else:
             assert max(errs.values()) < 2e-5
     r.close()
+
+
+@pytest.mark.parametrize("pipe", ["0", "1"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_column_form_multi_class(precision, pipe, monkeypatch):
+    """The column-correlation vertex form, asked for on local fields in all
+    directions (several in-plane classes, uneven shift sets) and on q points
+    whose k+q wraps with G0, with the next chunk loaded during the products
+    or between the barriers: vertices against the oracle (FP64 1e-10, mixed
+    2e-6 max-norm as for the other FP32 forms)."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "col")
+    monkeypatch.setenv("RPA_VERTEX_COL_PIPE", pipe)
+    s = H.system("tiny", lf="xyz", nlf_target=9)
+    r = _runner(s, program="sloss", precision=precision)
+    nv, nb = s.cfg.nval, s.cfg.nband
+    for iq, nlf in [(2, 9), (4, 14), (6, 5)]:
+        r.set_q_point(r.q_point(iq, nlf))
+        pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+        ref = orc.vertices(pb, 0, s.gcar, 0, nv, nv, nb)
+        got, _ = r.vertices()
+        err = H.rel_max(got.T, ref)
+        print(f"{precision} iq={iq} N_G={nlf} column-form vertices rel max err {err:.2e}")
+        assert err < (1e-10 if precision == "fp64" else 2e-6)
+    r.close()

@@ -250,8 +250,7 @@ def test_w_kk_check_matches_oracle_genw1w2():
 def test_staged_vertex_form_matches_oracle(program, pol, monkeypatch):
     """The LDS-staged vertex kernel (used for more than two band blocks) on
     cases other kernels normally take."""
-    monkeypatch.setenv("RPA_VERTEX_STAGED", "1")
-    monkeypatch.setenv("RPA_VERTEX_LDS", "0")
+    monkeypatch.setenv("RPA_VERTEX_FORM", "staged")
     s = H.system("cfg0")
     r = _runner(s, program=program, vertex_map="direct", pol=pol)
     pb = _oracle_problem(s, r, 0)
@@ -287,13 +286,12 @@ def test_many_band_blocks_optabs_full_pairs():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("form", ["lds", "gather", "staged"])
+@pytest.mark.parametrize("form", ["col", "lds", "gather", "staged"])
 @pytest.mark.parametrize("mode", ["direct", "igk"])
 def test_vertex_kernel_forms(precision, form, mode, monkeypatch):
-    """Every vertex kernel form on the Sloss charge vertices of cfg0."""
-    monkeypatch.setenv("RPA_VERTEX_LDS", "1" if form == "lds" else "0")
-    if form == "staged":
-        monkeypatch.setenv("RPA_VERTEX_STAGED", "1")
+    """Every vertex kernel form on the Sloss charge vertices of cfg0 (the
+    column form applies to the direct map; with igk the shape rule picks)."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", form)
     s = H.system("cfg0")
     r = host.Runner(s, host.RunConfig(program="sloss", vertex_map=mode, precision=precision))
     pb = _oracle_problem(s, r, 0 if mode == "direct" else 1)
