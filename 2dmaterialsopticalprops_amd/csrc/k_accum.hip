@@ -388,32 +388,87 @@ __global__ __launch_bounds__(256) void k_accum(AccArgs a) {
 //   B   = y_t(G')_c             (Re S0: y = rho2;  Im S0 (Optabs):
 //                                B' = (-Im y, Re y) for c = (Re, Im))
 // with K = 2 x kMxKS = 32 per MFMA, each operand split as hi + lo BF16.
+// Two passes before the products, so that the workgroups sharing a chunk do
+// not each redo them: k_mx_weights tabulates W of every chunk as FP32, and
+// k_mx_split writes the BF16 hi/lo parts of y transposed (local field major,
+// Re/Im of one transition in one 32-bit word) so a stage of B is a row copy.
 constexpr int kMxKS = 16;   // transitions per stage (K = 32)
 constexpr int kMxRow = 40;  // bf16 per staged B row (32 + pad, 80 bytes)
-
-struct AccMxArgs {
-  const double *te;
-  const float2 *rho1, *rho2;
-  const int *items;
-  const int *nitems;
-  int nlf, nw, ncol, ncol_pad, npb, nib;
-  int program, shift;
-  double domega, eta, cut, den;
-  double *part;
-};
 
 __device__ inline void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
   hi = (__bf16)v;
   lo = (__bf16)(v - (float)hi);
 }
 
+__device__ inline uint32_t bf16_pair(__bf16 re, __bf16 im) {
+  return (uint32_t)__builtin_bit_cast(uint16_t, re) | ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
+}
+
+// W of chunk `item`: wmx[(item*32 + row)*kChunkTMx + t - t_lo], FP32, zero
+// for frequencies past nw and for t in [t_hi, t_lo + round16(t_hi - t_lo)).
+__global__ void k_mx_weights(const double *te, const int *items, const int *nitems, int program, int shift,
+                             double domega, double eta, double cut, double den, int nw, float *wmx) {
+  const int item = blockIdx.x >> 5, row = blockIdx.x & 31;
+  if (item >= nitems[0]) return;
+  const int b = items[3 * item], t_lo = items[3 * item + 1], t_hi = items[3 * item + 2];
+  const int w = b * kTileW + row;
+  const int len = (t_hi - t_lo + kMxKS - 1) / kMxKS * kMxKS;
+  float *dst = wmx + ((size_t)item * kTileW + row) * kChunkTMx;
+  for (int x = threadIdx.x; x < len; x += blockDim.x) {
+    const int t = t_lo + x;
+    double v = 0.0;
+    if (t < t_hi && w < nw) {
+      const double *e = te + 3 * (size_t)t;
+      v = weight(program, w, shift, domega, eta, cut, den, e[0], e[1], e[2]);
+    }
+    dst[x] = (float)v;
+  }
+}
+
+// y[t][j] (complex FP32) -> rbt[hl][j][t] = (Re, Im) of the hi (hl = 0) or
+// lo (hl = 1) BF16 part, through an LDS tile of 64 transitions.
+__global__ __launch_bounds__(256) void k_mx_split(const float2 *y, long long nt, int nlf, uint32_t *rbt) {
+  extern __shared__ float2 ytile[];  // [64][nlf]
+  const long long t0 = (long long)blockIdx.x * 64;
+  const int nrow = (int)min(64LL, nt - t0);
+  const float2 *src = y + t0 * nlf;
+  for (int x = threadIdx.x; x < nrow * nlf; x += 256) ytile[x] = src[x];
+  __syncthreads();
+  for (int x = threadIdx.x; x < 64 * nlf; x += 256) {
+    const int j = x >> 6, tt = x & 63;
+    if (tt >= nrow) continue;
+    const float2 v = ytile[tt * nlf + j];
+    __bf16 rh, rl, ih, il;
+    split_bf16(v.x, rh, rl);
+    split_bf16(v.y, ih, il);
+    const size_t o = (size_t)j * nt + t0 + tt;
+    rbt[o] = bf16_pair(rh, ih);
+    rbt[(size_t)nlf * nt + o] = bf16_pair(rl, il);
+  }
+}
+
+struct AccMxArgs {
+  const float *wmx;
+  const float2 *rho1;
+  const uint32_t *rbt;  // [2][nlf][nt]
+  long long nt;
+  const int *items;
+  const int *nitems;
+  int nlf, nw, ncol, ncol_pad, npb, nib;
+  double *part;
+};
+
 // IPW local fields G per wave (4*IPW per workgroup), NCB blocks of 16 G'.
+// The next stage's W, x and B words are loaded into registers while the
+// current stage runs on the matrix cores.
 template <bool CPLX, int NCB, int IPW>
-__global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
+__global__ __launch_bounds__(256, (NCB <= 4) ? 2 : 1) void k_accum_mx(AccMxArgs a) {
   constexpr int NB = CPLX ? 2 : 1;  // B operands (Re, Im)
   constexpr int NJ = 16 * NCB;
+  constexpr int NI = 4 * IPW;       // local fields of the workgroup
+  constexpr int NBV = 2 * NCB;      // staged B words per thread: 2 x NJ x 16 / 256
   __shared__ float sW[kTileW][kMxKS];
-  __shared__ float2 sX[kMxKS][kMaxNlf];
+  __shared__ float2 sX[kMxKS][NI];
   __shared__ __align__(16) __bf16 sB[NB][2][NJ][kMxRow];  // [re/im][hi/lo][G'][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r = lane & 15, kg = lane >> 4;
@@ -425,8 +480,50 @@ __global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
   for (int wk = blockIdx.x >> 3; wk < total; wk += nsl) {
     const int il = wk / a.nib, ib = wk - il * a.nib;
     const int item = 8 * il + xcd;
-    const int b = a.items[3 * item], t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
-    const int i0 = ib * (4 * IPW) + wave * IPW;
+    const int t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
+    const int ig0 = ib * NI;                 // first local field of the workgroup
+    const int i0 = wave * IPW;               // first local field of the wave (local)
+    const float *wsrc = a.wmx + (size_t)item * kTileW * kChunkTMx;
+
+    // staging in registers: W (2 per thread), x (tid < 16 NI), B words
+    float rw[2];
+    float2 rx;
+    uint32_t rb[NBV];
+    auto load = [&](int t0) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = tid + 256 * u, row = e >> 4, kt = e & 15;
+        rw[u] = wsrc[(size_t)row * kChunkTMx + (t0 - t_lo) + kt];
+      }
+      if (tid < kMxKS * NI) {
+        const int kt = tid / NI, ii = tid - kt * NI;
+        const int t = t0 + kt, g = ig0 + ii;
+        rx = (t < t_hi && g < a.nlf) ? a.rho1[(size_t)t * a.nlf + g] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < NBV; ++u) {
+        const int e = tid + 256 * u, kt = e & 15, jh = e >> 4;
+        const int hl = jh / NJ, j = jh - hl * NJ;
+        const int t = t0 + kt;
+        rb[u] = (t < t_hi && j < a.nlf) ? a.rbt[((size_t)hl * a.nlf + j) * a.nt + t] : 0u;
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = tid + 256 * u;
+        sW[e >> 4][e & 15] = rw[u];
+      }
+      if (tid < kMxKS * NI) sX[tid / NI][tid % NI] = rx;
+#pragma unroll
+      for (int u = 0; u < NBV; ++u) {
+        const int e = tid + 256 * u, kt = e & 15, jh = e >> 4;
+        const int hl = jh / NJ, j = jh - hl * NJ;
+        *(uint32_t *)&sB[0][hl][j][2 * kt] = rb[u];
+        if (CPLX)  // (-Im, Re): swap the halves, flip the sign of the new low half
+          *(uint32_t *)&sB[NB - 1][hl][j][2 * kt] = ((rb[u] & 0xffffu) << 16) | ((rb[u] >> 16) ^ 0x8000u);
+      }
+    };
 
     rpa_f4 acc[NB][IPW][2][NCB];
 #pragma unroll
@@ -434,38 +531,16 @@ __global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
 #pragma unroll
       for (int q = 0; q < IPW; ++q)
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+        for (int rb2 = 0; rb2 < 2; ++rb2)
 #pragma unroll
-          for (int cb = 0; cb < NCB; ++cb) acc[s][q][rb][cb] = rpa_f4{0.f, 0.f, 0.f, 0.f};
+          for (int cb = 0; cb < NCB; ++cb) acc[s][q][rb2][cb] = rpa_f4{0.f, 0.f, 0.f, 0.f};
 
+    load(t_lo);
     for (int t0 = t_lo; t0 < t_hi; t0 += kMxKS) {
+      __syncthreads();  // the previous stage's reads are done
+      store();
       __syncthreads();
-      for (int x = tid; x < kTileW * kMxKS; x += 256) {
-        const int row = x / kMxKS, kt = x - row * kMxKS;
-        const int t = t0 + kt, w = b * kTileW + row;
-        double v = 0.0;
-        if (t < t_hi && w < a.nw) {
-          const double *e = a.te + 3 * (size_t)t;
-          v = weight(a.program, w, a.shift, a.domega, a.eta, a.cut, a.den, e[0], e[1], e[2]);
-        }
-        sW[row][kt] = (float)v;
-      }
-      for (int x = tid; x < kMxKS * a.nlf; x += 256) {
-        const int kt = x / a.nlf, g = x - kt * a.nlf;
-        sX[kt][g] = (t0 + kt < t_hi) ? a.rho1[(size_t)t0 * a.nlf + x] : make_float2(0.f, 0.f);
-      }
-      for (int x = tid; x < kMxKS * NJ; x += 256) {
-        const int kt = x / NJ, j = x - kt * NJ;
-        float2 y = make_float2(0.f, 0.f);
-        if (t0 + kt < t_hi && j < a.nlf) y = a.rho2[(size_t)(t0 + kt) * a.nlf + j];
-        split_bf16(y.x, sB[0][0][j][2 * kt], sB[0][1][j][2 * kt]);
-        split_bf16(y.y, sB[0][0][j][2 * kt + 1], sB[0][1][j][2 * kt + 1]);
-        if (CPLX) {
-          split_bf16(-y.y, sB[NB - 1][0][j][2 * kt], sB[NB - 1][1][j][2 * kt]);
-          split_bf16(y.x, sB[NB - 1][0][j][2 * kt + 1], sB[NB - 1][1][j][2 * kt + 1]);
-        }
-      }
-      __syncthreads();
+      if (t0 + kMxKS < t_hi) load(t0 + kMxKS);
       // B fragments: column G' = 16 cb + r, K slice 8 kg .. 8 kg + 7
       rpa_bf8 bh[NB][NCB], bl[NB][NCB];
 #pragma unroll
@@ -477,14 +552,13 @@ __global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
         }
 #pragma unroll
       for (int q = 0; q < IPW; ++q) {
-        const int i = i0 + q;
-        if (i >= a.nlf) continue;  // uniform over the wave
+        if (ig0 + i0 + q >= a.nlf) continue;  // uniform over the wave
         float2 xv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xv[u] = sX[4 * kg + u][i];
+        for (int u = 0; u < 4; ++u) xv[u] = sX[4 * kg + u][i0 + q];
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-          const float *wrow = &sW[16 * rb + r][4 * kg];
+        for (int rb2 = 0; rb2 < 2; ++rb2) {
+          const float *wrow = &sW[16 * rb2 + r][4 * kg];
           rpa_bf8 ah, al;
 #pragma unroll
           for (int v = 0; v < 8; ++v) {
@@ -499,11 +573,11 @@ __global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
           for (int s = 0; s < NB; ++s)
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
-              rpa_f4 c = acc[s][q][rb][cb];
+              rpa_f4 c = acc[s][q][rb2][cb];
               c = mfma_bf16_16x16x32(ah, bh[s][cb], c);
               c = mfma_bf16_16x16x32(ah, bl[s][cb], c);
               c = mfma_bf16_16x16x32(al, bh[s][cb], c);
-              acc[s][q][rb][cb] = c;
+              acc[s][q][rb2][cb] = c;
             }
         }
       }
@@ -512,10 +586,10 @@ __global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
     double *dst = a.part + (size_t)item * a.npb * (kTileW * kColsWG);
 #pragma unroll
     for (int q = 0; q < IPW; ++q) {
-      const int i = i0 + q;
+      const int i = ig0 + i0 + q;
       if (i >= a.nlf) continue;
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
+      for (int rb2 = 0; rb2 < 2; ++rb2)
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
           const int j = 16 * cb + r;
@@ -525,7 +599,7 @@ __global__ __launch_bounds__(256) void k_accum_mx(AccMxArgs a) {
             const int col = CPLX ? 2 * (i * a.nlf + j) + s : i * a.nlf + j;
             double *d = dst + (size_t)(col / kColsWG) * (kTileW * kColsWG) + col % kColsWG;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) d[(16 * rb + 4 * kg + e) * kColsWG] = (double)acc[s][q][rb][cb][e];
+            for (int e = 0; e < 4; ++e) d[(16 * rb2 + 4 * kg + e) * kColsWG] = (double)acc[s][q][rb2][cb][e];
           }
         }
     }
@@ -670,10 +744,19 @@ int launch_accumulate(rpa_ctx *c) {
   const double den = (M_PI * (double)c->ntot) * c->p.vcell;
   const int shift = cplx ? c->d.no : 0;
   if (mixed) {
+    if (c->nlf > 96) return fail(c, RPA_E_LIMIT, "mixed precision accumulation supports Nlf <= 96");
+    const long long nt = std::max<long long>(c->ntrans, 1);
+    const float2 *y = (const float2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
+    hipLaunchKernelGGL(k_mx_weights, dim3((unsigned)(c->max_items * kTileW)), dim3(256), 0, c->stream,
+                       (const double *)c->d_te, (const int *)c->d_items, (const int *)c->d_nitems, c->d.program,
+                       shift, c->p.domega, c->p.eta, c->wcut, den, c->nw, c->d_wmx);
+    hipLaunchKernelGGL(k_mx_split, dim3((unsigned)((nt + 63) / 64)), dim3(256), 64 * c->nlf * sizeof(float2),
+                       c->stream, y, nt, c->nlf, c->d_rbt);
     AccMxArgs a{};
-    a.te = c->d_te;
+    a.wmx = c->d_wmx;
     a.rho1 = (const float2 *)c->d_rho1;
-    a.rho2 = (const float2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
+    a.rbt = c->d_rbt;
+    a.nt = nt;
     a.items = c->d_items;
     a.nitems = c->d_nitems;
     a.nlf = c->nlf;
@@ -681,12 +764,6 @@ int launch_accumulate(rpa_ctx *c) {
     a.ncol = c->ncol;
     a.ncol_pad = c->ncol_pad;
     a.npb = npb;
-    a.program = c->d.program;
-    a.shift = shift;
-    a.domega = c->p.domega;
-    a.eta = c->p.eta;
-    a.cut = c->wcut;
-    a.den = den;
     a.part = c->d_part;
     const int e = cplx ? launch_mx_ncb<true>(c, a, (long long)c->max_items)
                        : launch_mx_ncb<false>(c, a, (long long)c->max_items);

@@ -185,6 +185,9 @@ struct rpa_ctx {
   int ncol = 0, ncol_pad = 0;
   double *d_s0 = nullptr;
   double *d_part = nullptr;
+  float *d_wmx = nullptr;      // mixed: FP32 weights of every chunk [item][32][kChunkTMx]
+  uint32_t *d_rbt = nullptr;   // mixed: BF16 hi/lo of the vertices, [2][nlf][ntrans]
+  size_t cap_wmx = 0, cap_rbt = 0;
   double *d_qeff = nullptr;
   size_t cap_part = 0;
 
