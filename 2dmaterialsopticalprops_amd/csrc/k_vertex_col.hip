@@ -32,8 +32,6 @@
 
 namespace rpa {
 
-namespace {
-
 constexpr int kColInfo = 2 + kColMaxShift / 4;  // per class: rowoff, nsg, sg list
 constexpr size_t kColLdsBudget = 80 * 1024; // default: two workgroups per CU
 
@@ -247,7 +245,7 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
 }
 
 template <class P, int KC, int NRU, bool PIPE>
-const void *col_kernel(int tpw) {
+static const void *col_kernel(int tpw) {
   switch (tpw) {
     case 1: return (const void *)k_vertex_col<P, 1, KC, NRU, PIPE>;
     case 2: return (const void *)k_vertex_col<P, 2, KC, NRU, PIPE>;
@@ -261,7 +259,7 @@ const void *col_kernel(int tpw) {
 }
 
 template <class P, int KC, int NRU, bool PIPE>
-void launch_col_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
+static void launch_col_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
   const size_t lds = c->col.lds;
   switch (tpw) {
     case 1: hipLaunchKernelGGL((k_vertex_col<P, 1, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
@@ -276,12 +274,12 @@ void launch_col_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
 }
 
 // 1 when RPA_VERTEX_FORM=col asks for this form regardless of shape
-int vertex_form_forced() {
+static int vertex_form_forced() {
   const char *env = std::getenv("RPA_VERTEX_FORM");
   return (env && std::string(env) == "col") ? 1 : 0;
 }
 
-int tpw_of(int tiles) {
+static int tpw_of(int tiles) {
   const int w = (tiles + 3) / 4;
   for (int t : {1, 2, 4, 6, 8, 10, 13, 16})
     if (w <= t) return t;
@@ -291,8 +289,6 @@ int tpw_of(int tiles) {
 // positions per chunk and staged k+q rows per thread (rows <= 64 NRU)
 constexpr int kKcF32 = 128, kKcF64 = 64;
 constexpr int kNruF32 = 8, kNruF64 = 4;
-
-}  // namespace
 
 void free_col_plan(rpa_ctx *c) {
   ColPlan &P = c->col;
@@ -379,15 +375,17 @@ int build_col_plan(rpa_ctx *c) {
     info[(size_t)u * kColInfo + 1] = nsg;
     spanmax = std::max(spanmax, span);
     padmax = std::max(padmax, nslot - 1 - span);
-    tilemax = std::max(tilemax, nsg * 4);  // at most 4 groups of bands m
+    tilemax = std::max(tilemax, nsg);
   }
-  const int tpw = tpw_of(tilemax);
-  if (tpw == 0) return 0;
-  // k+q row stride: the widest band block's m groups, padded so that 7
-  // consecutive rows of 4 bands map to distinct LDS banks
+  // groups of 4 bands m of the widest band block (blocks of at most 16 m)
   int nmg = 1;
   for (int b = 0; b < c->pairs.nblocks; ++b)
     nmg = std::max(nmg, (std::min(16, c->pairs.b[b].m_hi - c->pairs.b[b].m_lo) + 3) / 4);
+  // every wave runs TPW tiles, so TPW follows the tiles the shape needs
+  const int tpw = tpw_of(tilemax * nmg);
+  if (tpw == 0) return 0;
+  // k+q row stride: padded so that 7 consecutive rows of 4 bands map to
+  // distinct LDS banks
   const int ms = (nmg == 1) ? 4 : (nmg <= 3) ? 12 : 20;
   // loading the next chunk during the products measured faster in both
   // precisions at N_G = 51 (FP32 3.2 vs 4.6 ms, FP64 5.3 vs 5.4 ms)
