@@ -3,7 +3,8 @@
 //  k_coulomb   V(G,G') of the slab (Sloss.f90:1173-1208 genV); V is
 //              independent of frequency, so it is built once per q instead of
 //              once per frequency as the reference does (Sloss.f90:456).
-//  k_dyson     one workgroup per frequency io = 1..no-1:
+//  k_dyson     one workgroup per frequency io = 1..no-1 (k_dyson_col for
+//              N_G <= 64, the general form k_dyson up to the LDS limit):
 //                eps = I - chi0 V            (genDielectricEpsilon, 1210-1234)
 //                eps <- eps^-1               (gjel, Sloss.f90:462; here
 //                                             Gauss-Jordan with row pivoting,
@@ -61,6 +62,7 @@ struct DysonArgs {
   int *status;
 };
 
+// General form (N_G up to the LDS limit): eps and V in LDS, chi0 from L2.
 __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
   extern __shared__ double2 sa[];  // [nlf][nlf] eps, then eps^-1; then V (real) behind it
   __shared__ double2 fcol[kMaxNlf];
@@ -191,6 +193,194 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
   }
 }
 
+// The same solve for N_G <= 64 with lane = column j and rows i = wave + 4u:
+// no index divisions, and each O(N^3) product reads one operand per k and
+// takes the other as a broadcast.  Only eps (then eps^-1) lives in LDS
+// (16 N_G^2 bytes), so three frequencies share a CU; chi0 and V are read
+// from L2, the broadcast operand through the scalar cache (row addresses are
+// wave-uniform).  Rows past N_G read row N_G-1 and their results are dropped,
+// so the row loops have no branch and their loads are issued together.
+// Gauss-Jordan step k: wave 0 finds the pivot; waves 0 and 1 read the pivot
+// row, row p and column k, then (after a barrier) write the scaled pivot row,
+// the swapped row p and the multipliers; all waves eliminate.
+constexpr int kDyRows = 16;  // rows per wave (N_G <= 64)
+
+__global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
+  extern __shared__ double2 sa[];  // eps -> eps^-1 [N][N]
+  __shared__ double2 fcol[64];
+  __shared__ int piv[64];
+  __shared__ int pidx;
+  const int N = a.nlf, tid = threadIdx.x, j = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int f = blockIdx.x;
+  const double2 *X0 = a.chi0 + (size_t)f * N * N;
+  const double *V = a.V;
+  const bool jok = j < N;
+  const int jc = jok ? j : N - 1;  // lanes past N work on a valid column, results dropped
+  double2 acc[kDyRows];
+  auto ic = [&](int u) { return min(wave + 4 * u, N - 1); };
+
+  // eps = I - chi0 V
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k = 0; k < N; ++k) {
+    const double v = V[k * N + jc];
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const double2 c = X0[ic(u) * N + k];
+      acc[u].x += c.x * v;
+      acc[u].y += c.y * v;
+    }
+  }
+  if (jok) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (i < N) sa[i * N + j] = make_double2((i == j ? 1.0 : 0.0) - acc[u].x, -acc[u].y);
+    }
+  }
+  __syncthreads();
+
+  // in-place Gauss-Jordan inversion with row pivoting
+  for (int k = 0; k < N; ++k) {
+    if (wave == 0) {
+      // pivot: largest |Re| + |Im| in column k at or below row k, lowest row on ties
+      const double2 z = sa[jc * N + k];
+      double best = (j >= k && jok) ? fabs(z.x) + fabs(z.y) : -1.0;
+      int bi = (j >= k && jok) ? j : N;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (j == 0) {
+        int p = bi;
+        if (!(best > 0.0)) {  // singular: flag it and go on without a swap
+          atomicOr(a.status, 1);
+          p = k;
+        }
+        pidx = p;
+        piv[k] = p;
+      }
+    }
+    __syncthreads();
+    const int p = pidx;
+    // read phase
+    double2 rk = make_double2(0.0, 0.0), rp = rk, inv = rk, ck = rk, okk = rk;
+    if (wave == 0) {
+      rk = sa[k * N + jc];
+      rp = sa[p * N + jc];
+      inv = cdiv1(sa[p * N + k]);  // pivot value: column k of the new row k
+    } else if (wave == 1) {
+      ck = sa[jc * N + k];         // column k of row i = j before the swap
+      okk = sa[k * N + k];         // column k of the old row k (the new row p)
+    }
+    __syncthreads();
+    // write phase: scaled pivot row, swapped row p, multipliers, zeroed column k
+    if (wave == 0 && jok) {
+      sa[k * N + j] = (j == k) ? inv : cmul(rp, inv);
+      if (p != k && j != k) sa[p * N + j] = rk;
+    } else if (wave == 1 && jok && j != k) {
+      fcol[j] = (j == p) ? okk : ck;
+      sa[j * N + k] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    {
+      const double2 r = sa[k * N + jc];
+      // two batches of 8 rows keep the live registers low
+#pragma unroll
+      for (int h = 0; h < kDyRows; h += 8) {
+        double2 fk[8], v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          fk[u] = fcol[ic(h + u)];
+          v[u] = sa[ic(h + u) * N + jc];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          v[u].x -= fk[u].x * r.x - fk[u].y * r.y;
+          v[u].y -= fk[u].x * r.y + fk[u].y * r.x;
+        }
+        if (jok) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int i = wave + 4 * (h + u);
+            if (i < N && i != k) sa[i * N + j] = v[u];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, last first: column j of
+  // the result is column c(j) of the eliminated matrix, c = t_(N-1) .. t_0 (j)
+  int cj = jc;
+  for (int k = 0; k < N; ++k) {
+    const int pk = piv[k];
+    cj = (cj == k) ? pk : (cj == pk) ? k : cj;
+  }
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = sa[ic(u) * N + cj];
+  __syncthreads();
+  double2 *Einv = a.epsinv + (size_t)f * N * N;
+  double2 *Ch = a.chi + (size_t)f * N * N;
+  double2 *Wo = a.W + (size_t)f * N * N;
+  if (jok) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (i < N) {
+        sa[i * N + j] = acc[u];
+        Einv[i * N + j] = acc[u];
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) a.loss[f] = -sa[0].y;
+  // chi = eps^-1 chi0
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k = 0; k < N; ++k) {
+    const double2 c = X0[k * N + jc];
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const double2 e = sa[ic(u) * N + k];
+      acc[u].x += e.x * c.x - e.y * c.y;
+      acc[u].y += e.x * c.y + e.y * c.x;
+    }
+  }
+  if (jok) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (i < N) Ch[i * N + j] = acc[u];
+    }
+  }
+  // W = V eps^-1 (= V + V chi V)
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k = 0; k < N; ++k) {
+    const double2 e = sa[k * N + jc];
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const double v = V[ic(u) * N + k];
+      acc[u].x += v * e.x;
+      acc[u].y += v * e.y;
+    }
+  }
+  if (jok) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (i < N) Wo[i * N + j] = acc[u];
+    }
+  }
+}
+
 // Optabs Pi assembly per io (Optabs.f90:575-611).  Pi_inter is read after each
 // (iG,jG) update in the reference, so for Nlf > 1 it holds Pi_tot(1,1)
 // including the intraband term (Optabs.f90:594).
@@ -234,12 +424,26 @@ int launch_coulomb(rpa_ctx *c) {
 
 // Done once per context outside any stream capture: the in-LDS solve holds
 // Nlf^2 complex values, beyond the default 64 KB when Nlf > 64.
+// static LDS of k_dyson (fcol, piv, reductions) next to the dynamic part
+constexpr size_t kDysonStatic = (sizeof(double2) + sizeof(int)) * kMaxNlf + 64;
+constexpr size_t kLdsMax = 160 * 1024;
+
+// k_dyson_col (N_G <= 64): eps only in LDS; k_dyson: eps and V
+static bool dyson_stage(int nlf) { return nlf <= 64; }
+
+static size_t dyson_lds(int nlf) {
+  return (dyson_stage(nlf) ? sizeof(double2) : sizeof(double2) + sizeof(double)) * nlf * nlf;
+}
+
+// Outside any stream capture (prepare): allow the kernel the dynamic LDS of
+// the current N_G when that exceeds the default 64 KB.
 int configure_dyson(rpa_ctx *c) {
-  const size_t lds = (sizeof(double2) + sizeof(double)) * c->nlf * c->nlf;
-  if (lds > 64 * 1024 && lds <= 160 * 1024 && !c->dyson_lds_set) {
-    RPA_HIP(c, hipFuncSetAttribute((const void *)k_dyson, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    c->dyson_lds_set = true;
-  }
+  const size_t lds = dyson_lds(c->nlf);
+  const int st = dyson_stage(c->nlf) ? 1 : 0;
+  if (lds <= 64 * 1024 || lds + kDysonStatic > kLdsMax || lds <= c->dyson_lds_attr[st]) return 0;
+  const void *fn = st ? (const void *)k_dyson_col : (const void *)k_dyson;
+  RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  c->dyson_lds_attr[st] = lds;
   return 0;
 }
 
@@ -254,11 +458,15 @@ int launch_dyson(rpa_ctx *c) {
   a.W = (double2 *)c->d_W;
   a.loss = c->d_loss;
   a.status = c->d_status;
-  const size_t lds = (sizeof(double2) + sizeof(double)) * c->nlf * c->nlf;
-  if (lds > 160 * 1024) return fail(c, RPA_E_LIMIT, "Nlf too large for the in-LDS Dyson solve (Nlf <= 82)");
-  if (lds > 64 * 1024 && !c->dyson_lds_set)
+  const size_t lds = dyson_lds(c->nlf);
+  const int st = dyson_stage(c->nlf) ? 1 : 0;
+  if (lds + kDysonStatic > kLdsMax) return fail(c, RPA_E_LIMIT, "Nlf too large for the in-LDS Dyson solve (Nlf <= 81)");
+  if (lds > 64 * 1024 && lds > c->dyson_lds_attr[st])
     return fail(c, RPA_E_STATE, "Dyson LDS limit not configured");
-  hipLaunchKernelGGL(k_dyson, dim3(a.nfreq), dim3(256), lds, c->stream, a);
+  if (dyson_stage(c->nlf))
+    hipLaunchKernelGGL(k_dyson_col, dim3(a.nfreq), dim3(256), lds, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_dyson, dim3(a.nfreq), dim3(256), lds, c->stream, a);
   RPA_HIP(c, hipGetLastError());
   return 0;
 }

@@ -114,7 +114,7 @@ struct rpa_ctx {
   // basis
   void *d_evc = nullptr;
   float2 *d_evc32 = nullptr;  // mixed precision: FP32 copy of the basis
-  bool dyson_lds_set = false; // k_dyson allowed more than 64 KB of LDS
+  size_t dyson_lds_attr[2] = {0, 0};  // dynamic LDS k_dyson<false/true> is allowed
   int *d_mill_k = nullptr;
   int16_t *d_kbox = nullptr;
   int *d_ngk = nullptr;

@@ -177,3 +177,21 @@ def test_column_form_multi_class(precision, pipe, monkeypatch):
         print(f"{precision} iq={iq} N_G={nlf} column-form vertices rel max err {err:.2e}")
         assert err < (1e-10 if precision == "fp64" else 2e-6)
     r.close()
+
+
+@pytest.mark.parametrize("nlf_target", [60, 70])
+def test_dyson_both_lds_layouts(nlf_target):
+    """N_G up to 64 runs the column form (eps in LDS, chi0 and V from L2);
+    larger N_G (to the in-LDS limit of 81) the general form with eps and V in
+    LDS.  Both against the oracle."""
+    s = H.system("tiny", lf="xyz", nlf_target=nlf_target)
+    r = _runner(s, program="sloss")
+    assert (r.nlf <= 64) == (nlf_target == 60) and r.nlf <= 81
+    out = r.full()
+    S0, chi0, E = _sloss_ref(s, r)
+    V = orc.genV(r.rc.eps, r.q[0] * s.gcar, r.q[1] * s.gcar, s.cfg.c, r.parg, r.glf, s.gcar)
+    W = np.array([orc.dyson_w(c, V)[2] for c in chi0])
+    assert H.rel_max(out["S0"], S0) < TOL
+    assert H.rel_max(out["epsinv"], E) < TOL
+    assert H.rel_max(out["W"], W) < TOL
+    r.close()
