@@ -1140,8 +1140,7 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   const long long nblocks = (long long)a.kslots * a.groups;
   if (nblocks <= 0) return 0;
   const std::string form = vertex_form();
-  if (kind == 0 && c->col.ok && (form.empty() || form == "col"))
-    return launch_vertices_col(c, a.vb, a.nvb, rho1, perm);
+  if (c->col.ok && (form.empty() || form == "col")) return launch_vertices_col(c, a.vb, a.nvb, rho1, rho2, perm);
   if (int e = launch_direct_map(c)) return e;
   size_t lds_bytes = 0;
   if (lds_form(c, lds_bytes)) {

@@ -22,6 +22,14 @@
 // B row offset once and reusing them for all of its tiles.  The summation
 // runs over positions in column order instead of the reference's i order.
 //
+// Current vertices (Optabs, Optabs.f90:1412-1466) use the same correlation:
+// with 0.5 J(G,i) = alpha(G) + beta(i), alpha = 0.5 Gcar (q + 2k + Glf(G))_pol
+// and beta = Gcar G(i)_pol,
+//   j(G) = alpha(G) rho(G) + sum_i conj(C1(i)) beta(i) C2(map(G,i)),
+// so each staged k+q value also feeds a second (pol yz: third) accumulator
+// whose A operand is scaled by beta of the position; alpha is applied when
+// the results are written.
+//
 // Workgroup = (k, in-plane class, band block); workgroups of one k on one XCD.
 #include <algorithm>
 #include <cstdlib>
@@ -53,7 +61,13 @@ struct ColArgs {
   VBlock vb[kVertexMaxBlocks];
   PairTable pairs;
   const int *perm;
-  void *rho;
+  void *rho, *rho2;
+  // current vertices: beta of every position for pol1, pol2; alpha inputs
+  const double *beta;  // [2][npos]
+  const double *ktot, *glf;
+  double q[3];
+  double gcar;
+  int pol1, pol2;
 };
 
 struct ColF64 {
@@ -92,19 +106,23 @@ struct ColStage {
   typename P::cplx a[KC / 32];
   typename P::cplx b[NRU][4];
   int prow;
+  typename P::real beta[2];
 };
 
 // PIPE: the next chunk is loaded into registers while the current one runs
 // (fewer waves, hidden latency) instead of loaded between the barriers.
-template <class P, int TPW, int KC, int NRU, bool PIPE>
+// KIND: 0 charge, 1 current (one polarisation), 2 current (pol yz: two).
+template <class P, int KIND, int TPW, int KC, int NRU, bool PIPE>
 __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
   using real = typename P::real;
   using cplx = typename P::cplx;
   using acc_t = typename P::acc;
+  constexpr int NV = 1 + (KIND >= 1) + (KIND == 2);  // A variants: C1, beta1 C1, beta2 C1
   extern __shared__ __align__(16) unsigned char col_raw[];
   cplx *sA = (cplx *)col_raw;         // [KC][8]   C1 of position p, band n
   int *sP = (int *)(sA + KC * 8);     // [KC]      B row of position p
-  cplx *sB = (cplx *)(sP + KC);       // [rows][ms] C2 of the targets, bands m
+  real *sBeta = (real *)(sP + KC);    // [2][KC]   beta of position p (current vertices)
+  cplx *sB = (cplx *)(sBeta + 2 * KC);  // [rows][ms] C2 of the targets, bands m
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int L = blockIdx.x;
@@ -141,6 +159,11 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
       st.a[u] = (p < np && sa_n < vb.nrows) ? c1[(size_t)a.slot[p0 + p] * a.nband + sa_n] : P::make(0, 0);
     }
     st.prow = (tid < KC && tid < np) ? prow[p0 + tid] : 0;
+    if (KIND >= 1) {
+#pragma unroll
+      for (int v = 0; v < NV - 1; ++v)
+        st.beta[v] = (tid < KC && tid < np) ? (real)a.beta[(size_t)v * a.npos + p0 + tid] : (real)0;
+    }
     int j[NRU];
 #pragma unroll
     for (int u = 0; u < NRU; ++u) {
@@ -159,6 +182,10 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
 #pragma unroll
     for (int u = 0; u < KC / 32; ++u) sA[(sa_p + 32 * u) * 8 + sa_n] = st.a[u];
     if (tid < KC) sP[tid] = st.prow;
+    if (KIND >= 1 && tid < KC) {
+#pragma unroll
+      for (int v = 0; v < NV - 1; ++v) sBeta[v * KC + tid] = st.beta[v];
+    }
 #pragma unroll
     for (int u = 0; u < NRU; ++u) {
       const int rr = sb_r + 64 * u;
@@ -182,9 +209,11 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
     const int T = wave + 4 * tt;
     toff[tt] = (T < nt) ? 4 * info[2 + T / nmg] * a.ms + 4 * (T % nmg) : 0;
   }
-  acc_t acc[TPW];
+  acc_t acc[NV][TPW];
 #pragma unroll
-  for (int tt = 0; tt < TPW; ++tt) acc[tt] = P::zero();
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) acc[v][tt] = P::zero();
 
   if (PIPE) load(0);
   for (int ch = 0; ch < a.nchunk; ++ch) {
@@ -194,35 +223,53 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
     __syncthreads();
     if (PIPE && ch + 1 < a.nchunk) load(ch + 1);  // in flight during this chunk's products
     const int nsteps = (a.chunk[ch + 1] - a.chunk[ch] + 3) >> 2;
-    // the position's A value and B row are read one step ahead
+    // the position's A value, beta and B row are read one step ahead
     int prow_c = sP[kg];
     cplx za_c = sA[kg * 8 + band_r];
+    real bt_c[2] = {(real)0, (real)0};
+#pragma unroll
+    for (int v = 0; v < NV - 1; ++v) bt_c[v] = sBeta[v * KC + kg];
     for (int t = 0; t < nsteps; ++t) {
       const int pn = min(4 * (t + 1) + kg, KC - 1);
       const int prow_n = sP[pn];
       const cplx za_n = sA[pn * 8 + band_r];
+      real bt_n[2] = {(real)0, (real)0};
+#pragma unroll
+      for (int v = 0; v < NV - 1; ++v) bt_n[v] = sBeta[v * KC + pn];
       const cplx *brow = sB + prow_c * a.ms + lane_off;
-      const real are = im_row ? -za_c.y : za_c.x;
-      const real aim = im_row ? za_c.x : za_c.y;
+      real are[NV], aim[NV];
+      are[0] = im_row ? -za_c.y : za_c.x;
+      aim[0] = im_row ? za_c.x : za_c.y;
+#pragma unroll
+      for (int v = 1; v < NV; ++v) {
+        are[v] = bt_c[v - 1] * are[0];
+        aim[v] = bt_c[v - 1] * aim[0];
+      }
       // every wave runs TPW tiles; tiles past nt read row group 0 and their
       // results are dropped (no branch in the loop, so the LDS reads of all
       // tiles are issued ahead of the products)
 #pragma unroll
       for (int tt = 0; tt < TPW; ++tt) {
         const cplx zb = brow[toff[tt]];
-        acc[tt] = P::mma(are, zb.x, acc[tt]);
-        acc[tt] = P::mma(aim, zb.y, acc[tt]);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          acc[v][tt] = P::mma(are[v], zb.x, acc[v][tt]);
+          acc[v][tt] = P::mma(aim[v], zb.y, acc[v][tt]);
+        }
       }
       prow_c = prow_n;
       za_c = za_n;
+#pragma unroll
+      for (int v = 0; v < NV - 1; ++v) bt_c[v] = bt_n[v];
     }
   }
 
   // results: tile (sg, mg), lane column r -> shift 4 sg + (r & 3), band m 4 mg + (r >> 2)
   const PairBlock &pbk = a.pairs.b[vb.pb];
   const int nm = pbk.m_hi - pbk.m_lo;
-  cplx *rho = (cplx *)a.rho;
+  cplx *rho = (cplx *)a.rho, *rho2 = (cplx *)a.rho2;
   const int *shift = a.shift + (size_t)cl * kColMaxShift;
+  const double *kv = a.ktot + 3 * (size_t)k;
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt) {
     const int T = wave + 4 * tt;
@@ -239,38 +286,75 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
       const long long tt_idx = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
                                (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
       const long long pos = a.perm ? (long long)a.perm[tt_idx] : tt_idx;
-      if (pos >= 0) rho[pos * a.nlf + iG] = P::make(acc[tt][P::out_re(p)], acc[tt][P::out_im(p)]);
+      if (pos < 0) continue;
+      const real xr = acc[0][tt][P::out_re(p)], xi = acc[0][tt][P::out_im(p)];
+      if (KIND == 0) {
+        rho[pos * a.nlf + iG] = P::make(xr, xi);
+      } else {
+        // j = alpha(G) rho + rho_beta, alpha = 0.5 Gcar (q + 2k + Glf(G))_pol
+        const real al1 = (real)(0.5 * (((a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * iG + a.pol1]) * a.gcar));
+        const cplx j1 = P::make(al1 * xr + acc[1][tt][P::out_re(p)], al1 * xi + acc[1][tt][P::out_im(p)]);
+        rho[pos * a.nlf + iG] = j1;
+        if (KIND == 2) {
+          const real al2 = (real)(0.5 * (((a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * iG + a.pol2]) * a.gcar));
+          rho2[pos * a.nlf + iG] = P::make(al2 * xr + acc[NV - 1][tt][P::out_re(p)],
+                                           al2 * xi + acc[NV - 1][tt][P::out_im(p)]);
+        } else if (rho2) {
+          rho2[pos * a.nlf + iG] = j1;
+        }
+      }
     }
   }
 }
 
-template <class P, int KC, int NRU, bool PIPE>
+template <class P, int KIND, int KC, int NRU, bool PIPE>
 static const void *col_kernel(int tpw) {
   switch (tpw) {
-    case 1: return (const void *)k_vertex_col<P, 1, KC, NRU, PIPE>;
-    case 2: return (const void *)k_vertex_col<P, 2, KC, NRU, PIPE>;
-    case 4: return (const void *)k_vertex_col<P, 4, KC, NRU, PIPE>;
-    case 6: return (const void *)k_vertex_col<P, 6, KC, NRU, PIPE>;
-    case 8: return (const void *)k_vertex_col<P, 8, KC, NRU, PIPE>;
-    case 10: return (const void *)k_vertex_col<P, 10, KC, NRU, PIPE>;
-    case 13: return (const void *)k_vertex_col<P, 13, KC, NRU, PIPE>;
-    default: return (const void *)k_vertex_col<P, 16, KC, NRU, PIPE>;
+    case 1: return (const void *)k_vertex_col<P, KIND, 1, KC, NRU, PIPE>;
+    case 2: return (const void *)k_vertex_col<P, KIND, 2, KC, NRU, PIPE>;
+    case 4: return (const void *)k_vertex_col<P, KIND, 4, KC, NRU, PIPE>;
+    case 6: return (const void *)k_vertex_col<P, KIND, 6, KC, NRU, PIPE>;
+    case 8: return (const void *)k_vertex_col<P, KIND, 8, KC, NRU, PIPE>;
+    case 10: return (const void *)k_vertex_col<P, KIND, 10, KC, NRU, PIPE>;
+    case 13: return (const void *)k_vertex_col<P, KIND, 13, KC, NRU, PIPE>;
+    default: return (const void *)k_vertex_col<P, KIND, 16, KC, NRU, PIPE>;
   }
 }
 
-template <class P, int KC, int NRU, bool PIPE>
+template <class P, int KIND, int KC, int NRU, bool PIPE>
 static void launch_col_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
   const size_t lds = c->col.lds;
   switch (tpw) {
-    case 1: hipLaunchKernelGGL((k_vertex_col<P, 1, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
-    case 2: hipLaunchKernelGGL((k_vertex_col<P, 2, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
-    case 4: hipLaunchKernelGGL((k_vertex_col<P, 4, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
-    case 6: hipLaunchKernelGGL((k_vertex_col<P, 6, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
-    case 8: hipLaunchKernelGGL((k_vertex_col<P, 8, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
-    case 10: hipLaunchKernelGGL((k_vertex_col<P, 10, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
-    case 13: hipLaunchKernelGGL((k_vertex_col<P, 13, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
-    default: hipLaunchKernelGGL((k_vertex_col<P, 16, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 1: hipLaunchKernelGGL((k_vertex_col<P, KIND, 1, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_vertex_col<P, KIND, 2, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_vertex_col<P, KIND, 4, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 6: hipLaunchKernelGGL((k_vertex_col<P, KIND, 6, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 8: hipLaunchKernelGGL((k_vertex_col<P, KIND, 8, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 10: hipLaunchKernelGGL((k_vertex_col<P, KIND, 10, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 13: hipLaunchKernelGGL((k_vertex_col<P, KIND, 13, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_vertex_col<P, KIND, 16, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
   }
+}
+
+// Kernel of a (precision, kind, staging) choice; current vertices always
+// load the next chunk during the products.
+template <class P, int KC, int NRU>
+static const void *col_kernel_of(int kind, bool pipe, int tpw) {
+  if (kind == 1) return col_kernel<P, 1, KC, NRU, true>(tpw);
+  if (kind == 2) return col_kernel<P, 2, KC, NRU, true>(tpw);
+  return pipe ? col_kernel<P, 0, KC, NRU, true>(tpw) : col_kernel<P, 0, KC, NRU, false>(tpw);
+}
+
+template <class P, int KC, int NRU>
+static void launch_col_of(rpa_ctx *c, const ColArgs &a, int kind, bool pipe, int tpw, dim3 grid) {
+  if (kind == 1)
+    launch_col_p<P, 1, KC, NRU, true>(c, a, tpw, grid);
+  else if (kind == 2)
+    launch_col_p<P, 2, KC, NRU, true>(c, a, tpw, grid);
+  else if (pipe)
+    launch_col_p<P, 0, KC, NRU, true>(c, a, tpw, grid);
+  else
+    launch_col_p<P, 0, KC, NRU, false>(c, a, tpw, grid);
 }
 
 // 1 when RPA_VERTEX_FORM=col asks for this form regardless of shape
@@ -293,7 +377,7 @@ constexpr int kNruF32 = 8, kNruF64 = 4;
 void free_col_plan(rpa_ctx *c) {
   ColPlan &P = c->col;
   for (void *b : {(void *)P.d_slot, (void *)P.d_prow, (void *)P.d_chunk, (void *)P.d_crow, (void *)P.d_bmap,
-                  (void *)P.d_shift, (void *)P.d_cls})
+                  (void *)P.d_shift, (void *)P.d_cls, (void *)P.d_beta})
     if (b) (void)hipFree(b);
   P = ColPlan{};
 }
@@ -305,10 +389,17 @@ void free_col_plan(rpa_ctx *c) {
 int build_col_plan(rpa_ctx *c) {
   ColPlan &P = c->col;
   P.ok = false;
-  if (c->d.program != RPA_SLOSS || c->p.vertex_map != RPA_MAP_DIRECT) return 0;
+  if (c->p.vertex_map != RPA_MAP_DIRECT) return 0;
   const bool mixed = (c->d.precision == RPA_MIXED);
   const int kc = mixed ? kKcF32 : kKcF64;
   const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
+  const size_t rsz = mixed ? sizeof(float) : sizeof(double);
+  int kind = 0, pol1 = 0, pol2 = 0;
+  if (c->d.program == RPA_OPTABS) {
+    kind = (c->p.pol == RPA_POL_YZ) ? 2 : 1;
+    pol1 = (kind == 2) ? 1 : c->p.pol - 1;
+    pol2 = (kind == 2) ? 2 : pol1;
+  }
   const int ngd = c->p.ngd;
   const int *mg = c->mill_g.data();
 
@@ -384,6 +475,7 @@ int build_col_plan(rpa_ctx *c) {
   // every wave runs TPW tiles, so TPW follows the tiles the shape needs
   const int tpw = tpw_of(tilemax * nmg);
   if (tpw == 0) return 0;
+  if (!mixed && kind == 2 && tpw >= 13) return 0;  // three FP64 accumulator sets would spill
   // k+q row stride: padded so that 7 consecutive rows of 4 bands map to
   // distinct LDS banks
   const int ms = (nmg == 1) ? 4 : (nmg <= 3) ? 12 : 20;
@@ -393,7 +485,7 @@ int build_col_plan(rpa_ctx *c) {
   if (const char *env = std::getenv("RPA_VERTEX_COL_PIPE")) pipe = env[0] == '1';
 
   // chunks: consecutive segments, at most kc positions and rmax rows
-  const size_t fixed = (size_t)kc * (8 * csz + sizeof(int));
+  const size_t fixed = (size_t)kc * (8 * csz + sizeof(int) + 2 * rsz);
   size_t budget = kColLdsBudget;
   if (const char *env = std::getenv("RPA_VERTEX_COL_LDS")) budget = std::max<size_t>(fixed + 4096, 1024 * (size_t)std::atoi(env));
   budget = std::min<size_t>(budget, 160 * 1024);
@@ -468,6 +560,19 @@ int build_col_plan(rpa_ctx *c) {
   }
   std::vector<int> slot(ngd);
   for (int p = 0; p < ngd; ++p) slot[p] = ord[p];
+  // beta(i) = Gcar G(i)_pol of every position (current vertices), with the
+  // cartesian component summed as the gather forms do
+  std::vector<double> beta(2 * (size_t)ngd, 0.0);
+  if (kind >= 1)
+    for (int v = 0; v < 2; ++v) {
+      const int pol = v ? pol2 : pol1;
+      for (int p = 0; p < ngd; ++p) {
+        const int *m = mg + 3 * ord[p];
+        const double g = ((0.0 + c->kc[pol + 0] * (double)m[0]) + c->kc[pol + 3] * (double)m[1]) +
+                         c->kc[pol + 6] * (double)m[2];
+        beta[(size_t)v * ngd + p] = g * c->p.gcar;
+      }
+    }
 
   int e = ensure(c, P.d_slot, P.cap_slot, (size_t)ngd);
   if (!e) e = ensure(c, P.d_prow, P.cap_prow, prow.size());
@@ -476,6 +581,7 @@ int build_col_plan(rpa_ctx *c) {
   if (!e) e = ensure(c, P.d_bmap, P.cap_bmap, std::max<size_t>(bmap.size(), 1));
   if (!e) e = ensure(c, P.d_shift, P.cap_shift, shift.size());
   if (!e) e = ensure(c, P.d_cls, P.cap_cls, info.size());
+  if (!e) e = ensure(c, P.d_beta, P.cap_beta, beta.size());
   if (e) return e;
   RPA_HIP(c, hipMemcpy(P.d_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(P.d_prow, prow.data(), sizeof(int) * prow.size(), hipMemcpyHostToDevice));
@@ -484,6 +590,7 @@ int build_col_plan(rpa_ctx *c) {
   RPA_HIP(c, hipMemcpy(P.d_bmap, bmap.data(), sizeof(int) * bmap.size(), hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(P.d_shift, shift.data(), sizeof(int) * shift.size(), hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(P.d_cls, info.data(), sizeof(int) * info.size(), hipMemcpyHostToDevice));
+  RPA_HIP(c, hipMemcpy(P.d_beta, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice));
   P.npos = ngd;
   P.nchunk = nchunk;
   P.ncls = ncls;
@@ -491,20 +598,21 @@ int build_col_plan(rpa_ctx *c) {
   P.kc = kc;
   P.tpw = tpw;
   P.ms = ms;
-  P.pipe = pipe;
+  P.pipe = pipe || kind >= 1;
+  P.kind = kind;
+  P.pol1 = pol1;
+  P.pol2 = pol2;
   P.lds = fixed + (size_t)rows_used * ms * csz;
   if (P.lds > 64 * 1024) {
-    const void *fn = mixed ? (pipe ? col_kernel<ColF32, kKcF32, kNruF32, true>(tpw)
-                                   : col_kernel<ColF32, kKcF32, kNruF32, false>(tpw))
-                           : (pipe ? col_kernel<ColF64, kKcF64, kNruF64, true>(tpw)
-                                   : col_kernel<ColF64, kKcF64, kNruF64, false>(tpw));
+    const void *fn = mixed ? col_kernel_of<ColF32, kKcF32, kNruF32>(kind, pipe, tpw)
+                           : col_kernel_of<ColF64, kKcF64, kNruF64>(kind, pipe, tpw);
     RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds));
   }
   P.ok = true;
   return 0;
 }
 
-int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, const int *perm) {
+int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, void *rho2, const int *perm) {
   const ColPlan &P = c->col;
   const bool mixed = (c->d.precision == RPA_MIXED);
   ColArgs a{};
@@ -534,20 +642,21 @@ int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, const 
   a.pairs = c->pairs;
   a.perm = perm;
   a.rho = rho;
+  a.rho2 = rho2;
+  a.beta = P.d_beta;
+  a.ktot = c->d_ktot;
+  a.glf = c->d_glf;
+  for (int i = 0; i < 3; ++i) a.q[i] = c->q[i];
+  a.gcar = c->p.gcar;
+  a.pol1 = P.pol1;
+  a.pol2 = P.pol2;
   const long long kslots = (c->ntot + 7) / 8 * 8;
   const dim3 grid((unsigned)(kslots * a.blocks));
   a.ms = P.ms;
-  if (mixed) {
-    if (P.pipe)
-      launch_col_p<ColF32, kKcF32, kNruF32, true>(c, a, P.tpw, grid);
-    else
-      launch_col_p<ColF32, kKcF32, kNruF32, false>(c, a, P.tpw, grid);
-  } else {
-    if (P.pipe)
-      launch_col_p<ColF64, kKcF64, kNruF64, true>(c, a, P.tpw, grid);
-    else
-      launch_col_p<ColF64, kKcF64, kNruF64, false>(c, a, P.tpw, grid);
-  }
+  if (mixed)
+    launch_col_of<ColF32, kKcF32, kNruF32>(c, a, P.kind, P.pipe, P.tpw, grid);
+  else
+    launch_col_of<ColF64, kKcF64, kNruF64>(c, a, P.kind, P.pipe, P.tpw, grid);
   RPA_HIP(c, hipGetLastError());
   return 0;
 }

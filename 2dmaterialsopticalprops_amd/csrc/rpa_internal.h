@@ -83,12 +83,14 @@ constexpr int kColMaxShift = 128;  // shift slots of one in-plane class
 struct ColPlan {
   bool ok = false;
   int npos = 0, nchunk = 0, ncls = 0, nrows = 0;  // nrows: B rows of all classes (one G0)
-  int kc = 0, tpw = 0, ms = 0;
+  int kc = 0, tpw = 0, ms = 0, kind = 0, pol1 = 0, pol2 = 0;
   bool pipe = false;
   size_t lds = 0;
   int *d_slot = nullptr, *d_prow = nullptr, *d_chunk = nullptr, *d_crow = nullptr;
   int *d_bmap = nullptr, *d_shift = nullptr, *d_cls = nullptr;
+  double *d_beta = nullptr;  // current vertices: Gcar G(i)_pol per position, [2][npos]
   size_t cap_slot = 0, cap_prow = 0, cap_chunk = 0, cap_crow = 0, cap_bmap = 0, cap_shift = 0, cap_cls = 0;
+  size_t cap_beta = 0;
 };
 
 }  // namespace rpa
@@ -246,7 +248,7 @@ int launch_transitions(rpa_ctx *c);
 int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm);
 int launch_direct_map(rpa_ctx *c);
 int build_col_plan(rpa_ctx *c);
-int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, const int *perm);
+int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, void *rho2, const int *perm);
 void free_col_plan(rpa_ctx *c);
 int launch_w_kk(rpa_ctx *c);
 int launch_accumulate(rpa_ctx *c);

@@ -300,3 +300,27 @@ def test_vertex_kernel_forms(precision, form, mode, monkeypatch):
     got, _ = r.vertices()
     assert H.rel_max(got.T, ref) < (TOL_V if precision == "fp64" else 2e-6)
     r.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("pol", ["xx", "zz", "yz"])
+def test_column_form_current_vertices(precision, pol, monkeypatch):
+    """Optabs current vertices through the column-correlation form
+    (j = alpha(G) rho + rho_beta) against the oracle's direct sums with the
+    reference's J(G, i) per term."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "col")
+    s = H.system("cfg0")
+    r = host.Runner(s, host.RunConfig(program="optabs", vertex_map="direct", pol=pol, precision=precision))
+    pb = _oracle_problem(s, r, 0)
+    nv, nb = s.cfg.nval, s.cfg.nband
+    a1, a2 = orc.vertices(pb, host._abi.POL[pol], s.gcar, 0, nv, nv, nb, both=True)
+    g1, g2 = r.vertices(both=True)
+    nk, tv = len(r.K1), nv * (nb - nv)
+    g1 = g1.T.reshape(nk, 2 * tv, r.nlf)
+    g2 = g2.T.reshape(nk, 2 * tv, r.nlf)
+    tol = TOL_V if precision == "fp64" else 2e-6
+    e1 = H.rel_max(g1[:, :tv], a1.reshape(nk, tv, r.nlf))
+    e2 = H.rel_max(g2[:, :tv], a2.reshape(nk, tv, r.nlf))
+    print(f"{precision} {pol} column-form current vertices rel max err {e1:.2e} {e2:.2e}")
+    assert e1 < tol and e2 < tol
+    r.close()
