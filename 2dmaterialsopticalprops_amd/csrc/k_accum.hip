@@ -12,10 +12,13 @@
 //                     perm[t], so no data moves for the sort.
 //  3. k_worklist      per tile b the candidates are the contiguous sorted
 //                     range of homes [b-span, b]; cut into chunks of kChunkT.
-//  4. k_accum         persistent kernel over (chunk, column block): a 32 x 256
-//                     tile of S0 += W (32 x K) P (K x 256), W computed from the
-//                     energies with the reference's exact cutoff test, P from
-//                     the vertices staged in LDS.  Partials per chunk.
+//  4. k_weight_table  W of every chunk (32 frequencies x its transitions) from
+//                     the energies with the reference's exact arithmetic and
+//                     cutoff test, once per run.
+//     k_accum         persistent kernel over (chunk, column block): a 32 x 256
+//                     tile of S0 += W (32 x K) P (K x 256), W copied from the
+//                     table, P from the vertices staged in LDS.  Partials per
+//                     chunk.
 //  5. k_reduce        fixed-order Neumaier sum of the chunk partials of each
 //                     tile -> S0 (deterministic, FP64).
 //  6. k_qeff          Optabs intraband Qeff (Optabs.f90:430-442).
@@ -254,6 +257,7 @@ __global__ __launch_bounds__(1024) void k_worklist(const int *toff, int ntiles, 
 
 struct AccArgs {
   const double *te;
+  const double *wt;  // weight table [item][32][kChunkT]
   const double2 *rho1, *rho2;
   const int *items;
   const int *nitems;
@@ -282,7 +286,33 @@ __device__ inline double weight(int program, int w, int shift, double domega, do
   return -((2.0 * z) * lor) / den;
 }
 
-template <bool CPLX>
+// W of chunk `item` for its 32 frequency rows: wt[(item*32 + row)*CHUNK +
+// t - t_lo], zero for frequencies past nw and for t in [t_hi, t_lo +
+// round16(t_hi - t_lo)).  Computed once per run so the workgroups that share
+// a chunk copy it instead of each evaluating the Lorentzians.
+template <class T, int CHUNK>
+__global__ void k_weight_table(const double *te, const int *items, const int *nitems, int program, int shift,
+                               double domega, double eta, double cut, double den, int nw, T *wt) {
+  const int item = blockIdx.x >> 5, row = blockIdx.x & 31;
+  if (item >= nitems[0]) return;
+  const int b = items[3 * item], t_lo = items[3 * item + 1], t_hi = items[3 * item + 2];
+  const int w = b * kTileW + row;
+  const int len = (t_hi - t_lo + 15) / 16 * 16;
+  T *dst = wt + ((size_t)item * kTileW + row) * CHUNK;
+  for (int x = threadIdx.x; x < len; x += blockDim.x) {
+    const int t = t_lo + x;
+    double v = 0.0;
+    if (t < t_hi && w < nw) {
+      const double *e = te + 3 * (size_t)t;
+      v = weight(program, w, shift, domega, eta, cut, den, e[0], e[1], e[2]);
+    }
+    dst[x] = (T)v;
+  }
+}
+
+// RU: vertex values staged per thread and stage (16 nlf <= 256 RU); the next
+// stage's vertices and weights are loaded into registers during the products.
+template <bool CPLX, int RU>
 __global__ __launch_bounds__(256) void k_accum(AccArgs a) {
   extern __shared__ double smem[];
   double *sW = smem;                                   // [32][kKS+1]
@@ -300,7 +330,8 @@ __global__ __launch_bounds__(256) void k_accum(AccArgs a) {
   for (int wk = blockIdx.x >> 3; wk < total; wk += nsl) {
     const int il = wk / a.npb, pbk = wk - il * a.npb;
     const int item = 8 * il + xcd;
-    const int b = a.items[3 * item], t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
+    const int t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
+    const double *wsrc = a.wt + (size_t)item * kTileW * kChunkT;
 
     // columns owned by this lane in the four 16-wide blocks of its wave
     int cg_i[4], cg_j[4], cg_p[4];
@@ -321,27 +352,44 @@ __global__ __launch_bounds__(256) void k_accum(AccArgs a) {
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = rpa_d4{0.0, 0.0, 0.0, 0.0};
 
-    for (int t0 = t_lo; t0 < t_hi; t0 += kKS) {
-      __syncthreads();
-      for (int x = tid; x < kTileW * kKS; x += 256) {
-        int row = x / kKS, kt = x - row * kKS;
-        int t = t0 + kt, w = b * kTileW + row;
-        double v = 0.0;
-        if (t < t_hi && w < a.nw) {
-          const double *e = a.te + 3 * (size_t)t;
-          v = weight(a.program, w, a.shift, a.domega, a.eta, a.cut, a.den, e[0], e[1], e[2]);
+    double rw[2];
+    double2 rx[RU], ry[RU];
+    const int nst = kKS * a.nlf;
+    auto load = [&](int t0) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int x = tid + 256 * u, row = x / kKS, kt = x - row * kKS;
+        rw[u] = wsrc[(size_t)row * kChunkT + (t0 - t_lo) + kt];
+      }
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int x = tid + 256 * u;
+        const bool ok = x < nst && t0 + x / a.nlf < t_hi;
+        rx[u] = ok ? a.rho1[(size_t)t0 * a.nlf + x] : make_double2(0.0, 0.0);
+        if (CPLX) ry[u] = ok ? a.rho2[(size_t)t0 * a.nlf + x] : make_double2(0.0, 0.0);
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int x = tid + 256 * u, row = x / kKS, kt = x - row * kKS;
+        sW[row * (kKS + 1) + kt] = rw[u];
+      }
+#pragma unroll
+      for (int u = 0; u < RU; ++u) {
+        const int x = tid + 256 * u;
+        if (x < nst) {
+          sR1[x] = rx[u];
+          if (CPLX) sR2[x] = ry[u];
         }
-        sW[row * (kKS + 1) + kt] = v;
       }
-      const int nst = kKS * a.nlf;
-      for (int x = tid; x < nst; x += 256) {
-        int kt = x / a.nlf;
-        int t = t0 + kt;
-        bool ok = t < t_hi;
-        sR1[x] = ok ? a.rho1[(size_t)t0 * a.nlf + x] : make_double2(0.0, 0.0);
-        if (CPLX) sR2[x] = ok ? a.rho2[(size_t)t0 * a.nlf + x] : make_double2(0.0, 0.0);
-      }
+    };
+    load(t_lo);
+    for (int t0 = t_lo; t0 < t_hi; t0 += kKS) {
+      __syncthreads();  // the previous stage's reads are done
+      store();
       __syncthreads();
+      if (t0 + kKS < t_hi) load(t0 + kKS);
 #pragma unroll
       for (int st = 0; st < kKS / 4; ++st) {
         const int kt = st * 4 + q4;
@@ -389,7 +437,7 @@ __global__ __launch_bounds__(256) void k_accum(AccArgs a) {
 //                                B' = (-Im y, Re y) for c = (Re, Im))
 // with K = 2 x kMxKS = 32 per MFMA, each operand split as hi + lo BF16.
 // Two passes before the products, so that the workgroups sharing a chunk do
-// not each redo them: k_mx_weights tabulates W of every chunk as FP32, and
+// not each redo them: k_weight_table tabulates W of every chunk as FP32, and
 // k_mx_split writes the BF16 hi/lo parts of y transposed (local field major,
 // Re/Im of one transition in one 32-bit word) so a stage of B is a row copy.
 constexpr int kMxKS = 16;   // transitions per stage (K = 32)
@@ -402,27 +450,6 @@ __device__ inline void split_bf16(float v, __bf16 &hi, __bf16 &lo) {
 
 __device__ inline uint32_t bf16_pair(__bf16 re, __bf16 im) {
   return (uint32_t)__builtin_bit_cast(uint16_t, re) | ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
-}
-
-// W of chunk `item`: wmx[(item*32 + row)*kChunkTMx + t - t_lo], FP32, zero
-// for frequencies past nw and for t in [t_hi, t_lo + round16(t_hi - t_lo)).
-__global__ void k_mx_weights(const double *te, const int *items, const int *nitems, int program, int shift,
-                             double domega, double eta, double cut, double den, int nw, float *wmx) {
-  const int item = blockIdx.x >> 5, row = blockIdx.x & 31;
-  if (item >= nitems[0]) return;
-  const int b = items[3 * item], t_lo = items[3 * item + 1], t_hi = items[3 * item + 2];
-  const int w = b * kTileW + row;
-  const int len = (t_hi - t_lo + kMxKS - 1) / kMxKS * kMxKS;
-  float *dst = wmx + ((size_t)item * kTileW + row) * kChunkTMx;
-  for (int x = threadIdx.x; x < len; x += blockDim.x) {
-    const int t = t_lo + x;
-    double v = 0.0;
-    if (t < t_hi && w < nw) {
-      const double *e = te + 3 * (size_t)t;
-      v = weight(program, w, shift, domega, eta, cut, den, e[0], e[1], e[2]);
-    }
-    dst[x] = (float)v;
-  }
 }
 
 // y[t][j] (complex FP32) -> rbt[hl][j][t] = (Re, Im) of the hi (hl = 0) or
@@ -747,9 +774,9 @@ int launch_accumulate(rpa_ctx *c) {
     if (c->nlf > 96) return fail(c, RPA_E_LIMIT, "mixed precision accumulation supports Nlf <= 96");
     const long long nt = std::max<long long>(c->ntrans, 1);
     const float2 *y = (const float2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
-    hipLaunchKernelGGL(k_mx_weights, dim3((unsigned)(c->max_items * kTileW)), dim3(256), 0, c->stream,
-                       (const double *)c->d_te, (const int *)c->d_items, (const int *)c->d_nitems, c->d.program,
-                       shift, c->p.domega, c->p.eta, c->wcut, den, c->nw, c->d_wmx);
+    hipLaunchKernelGGL((k_weight_table<float, kChunkTMx>), dim3((unsigned)(c->max_items * kTileW)), dim3(256), 0,
+                       c->stream, (const double *)c->d_te, (const int *)c->d_items, (const int *)c->d_nitems,
+                       c->d.program, shift, c->p.domega, c->p.eta, c->wcut, den, c->nw, c->d_wmx);
     hipLaunchKernelGGL(k_mx_split, dim3((unsigned)((nt + 63) / 64)), dim3(256), 64 * c->nlf * sizeof(float2),
                        c->stream, y, nt, c->nlf, c->d_rbt);
     AccMxArgs a{};
@@ -769,8 +796,12 @@ int launch_accumulate(rpa_ctx *c) {
                        : launch_mx_ncb<false>(c, a, (long long)c->max_items);
     if (e) return e;
   } else {
+    hipLaunchKernelGGL((k_weight_table<double, kChunkT>), dim3((unsigned)(c->max_items * kTileW)), dim3(256), 0,
+                       c->stream, (const double *)c->d_te, (const int *)c->d_items, (const int *)c->d_nitems,
+                       c->d.program, shift, c->p.domega, c->p.eta, c->wcut, den, c->nw, c->d_wt64);
     AccArgs a{};
     a.te = c->d_te;
+    a.wt = c->d_wt64;
     a.rho1 = (const double2 *)c->d_rho1;
     a.rho2 = (const double2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
     a.items = c->d_items;
@@ -790,10 +821,23 @@ int launch_accumulate(rpa_ctx *c) {
     const size_t lds = sizeof(double) * kTileW * (kKS + 1) + sizeof(double2) * kKS * c->nlf * (cplx ? 2 : 1);
     long long work = (long long)c->max_items * a.npb;
     int grid = (int)std::min<long long>(std::max<long long>((work + 7) / 8 * 8, 8), 4096);
-    if (cplx)
-      hipLaunchKernelGGL(k_accum<true>, dim3(grid), dim3(256), lds, c->stream, a);
-    else
-      hipLaunchKernelGGL(k_accum<false>, dim3(grid), dim3(256), lds, c->stream, a);
+    const int ru = (kKS * c->nlf + 255) / 256;
+    if (ru > 8) return fail(c, RPA_E_LIMIT, "Nlf too large for the accumulation staging");
+    if (cplx) {
+      if (ru <= 2)
+        hipLaunchKernelGGL((k_accum<true, 2>), dim3(grid), dim3(256), lds, c->stream, a);
+      else if (ru <= 4)
+        hipLaunchKernelGGL((k_accum<true, 4>), dim3(grid), dim3(256), lds, c->stream, a);
+      else
+        hipLaunchKernelGGL((k_accum<true, 8>), dim3(grid), dim3(256), lds, c->stream, a);
+    } else {
+      if (ru <= 2)
+        hipLaunchKernelGGL((k_accum<false, 2>), dim3(grid), dim3(256), lds, c->stream, a);
+      else if (ru <= 4)
+        hipLaunchKernelGGL((k_accum<false, 4>), dim3(grid), dim3(256), lds, c->stream, a);
+      else
+        hipLaunchKernelGGL((k_accum<false, 8>), dim3(grid), dim3(256), lds, c->stream, a);
+    }
   }
   if (c->profiling) RPA_HIP(c, hipEventRecord(c->prof_ev[3], c->stream));
   const long long n = (long long)c->nw * c->ncol_pad;

@@ -139,6 +139,7 @@ static int prepare(rpa_ctx *c) {
     e = ensure(c, c->d_wmx, c->cap_wmx, c->max_items * kTileW * kChunkTMx);
     if (!e) e = ensure(c, c->d_rbt, c->cap_rbt, 2 * (size_t)c->nlf * nt);
   }
+  if (!e && c->d.precision != RPA_MIXED) e = ensure(c, c->d_wt64, c->cap_wt64, c->max_items * kTileW * kChunkT);
   if (!e) e = ensure(c, c->d_kmat, c->cap_kmat, nf * c->nw);
   if (!e) e = ensure(c, c->d_x, c->cap_x, nf * c->ncol_pad);
   if (!e) e = ensure(c, c->d_chi0, c->cap_chi0, nf * nn * 2);
@@ -325,7 +326,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
                   c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32,
-                  c->d_wmx, c->d_rbt};
+                  c->d_wmx, c->d_rbt, c->d_wt64};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_col_plan(c);

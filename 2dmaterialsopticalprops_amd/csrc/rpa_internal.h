@@ -189,7 +189,8 @@ struct rpa_ctx {
   double *d_part = nullptr;
   float *d_wmx = nullptr;      // mixed: FP32 weights of every chunk [item][32][kChunkTMx]
   uint32_t *d_rbt = nullptr;   // mixed: BF16 hi/lo of the vertices, [2][nlf][ntrans]
-  size_t cap_wmx = 0, cap_rbt = 0;
+  double *d_wt64 = nullptr;    // FP64: weights of every chunk [item][32][kChunkT]
+  size_t cap_wmx = 0, cap_rbt = 0, cap_wt64 = 0;
   double *d_qeff = nullptr;
   size_t cap_part = 0;
 
