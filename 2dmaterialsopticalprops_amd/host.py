@@ -66,37 +66,60 @@ def local_fields_by_qg(g: np.ndarray, mill_g: np.ndarray, gcar: float, q: np.nda
     return (order + 1).astype(np.int32), parg
 
 
+def _window_count(a, b, io_lo: int, io_hi: int, dw: float, eta: float, cut: float) -> int:
+    """Bins io in [io_lo, io_hi] with eta / (dE^2 + eta^2) >= cut for
+    dE = io dw + a - b, per pair (a, b).  The passing bins are contiguous
+    around the resonance; their range comes from |dE| <= sqrt(eta/cut -
+    eta^2), and both ends are settled by the exact test, so the count equals
+    a bin-by-bin evaluation."""
+    hw = math.sqrt(max(eta / cut - eta * eta, 0.0))
+
+    def ok(io):
+        d = (io * dw + a) - b
+        return np.abs(-eta / (d * d + eta * eta)) >= cut
+
+    c = (b - a) / dw
+    lo = np.ceil(c - hw / dw).astype(np.int64)
+    hi = np.floor(c + hw / dw).astype(np.int64)
+    lo = np.where(ok(lo - 1), lo - 1, np.where(ok(lo), lo, lo + 1))
+    hi = np.where(ok(hi + 1), hi + 1, np.where(ok(hi), hi, hi - 1))
+    n = np.minimum(hi, io_hi) - np.maximum(lo, io_lo) + 1
+    return int(np.maximum(n, 0).sum())
+
+
+def _pairs(E, K1, K2, n_rng, m_rng):
+    e1 = E[K1 - 1][:, n_rng[0]:n_rng[1]]
+    e2 = E[K2 - 1][:, m_rng[0]:m_rng[1]]
+    shp = (len(K1), e1.shape[1], e2.shape[1])
+    return np.broadcast_to(e1[:, :, None], shp).ravel(), np.broadcast_to(e2[:, None, :], shp).ravel()
+
+
 def nonzero_weights(E, K1, K2, nv: int, nb: int, no: int, dw: float, eta: float) -> int:
     """Number of (transition, frequency) pairs whose Sloss weight passes the
     cutoff |L| >= 1e-3/eta, with L = -eta/(dE^2 + eta^2) and
-    dE = (io-1) dw + E(K1,n) - E(K2,m), n <= Nocc < m (Sloss.f90:407-412).
-
-    The passing bins of one transition are contiguous around the resonance;
-    their range comes from |dE| <= sqrt(eta/cut - eta^2), and both ends are
-    settled by the exact test, so the count equals a bin-by-bin evaluation.
-    """
-    cut = 1.0e-3 / eta
-    hw = math.sqrt(max(eta / cut - eta * eta, 0.0))
+    dE = (io-1) dw + E(K1,n) - E(K2,m), n <= Nocc < m (Sloss.f90:407-412)."""
     K1 = np.asarray(K1)
     K2 = np.asarray(K2)
     total = 0
     for k0 in range(0, len(K1), 4096):
-        e1 = E[K1[k0:k0 + 4096] - 1][:, :nv]
-        e2 = E[K2[k0:k0 + 4096] - 1][:, nv:nb]
-        a = np.broadcast_to(e1[:, :, None], (len(e1), nv, nb - nv)).ravel()
-        b = np.broadcast_to(e2[:, None, :], (len(e1), nv, nb - nv)).ravel()
+        a, b = _pairs(E, K1[k0:k0 + 4096], K2[k0:k0 + 4096], (0, nv), (nv, nb))
+        total += _window_count(a, b, 0, no - 1, dw, eta, 1.0e-3 / eta)
+    return total
 
-        def ok(io):
-            d = (io * dw + a) - b
-            return np.abs(-eta / (d * d + eta * eta)) >= cut
 
-        c = (b - a) / dw
-        lo = np.ceil(c - hw / dw).astype(np.int64)
-        hi = np.floor(c + hw / dw).astype(np.int64)
-        lo = np.where(ok(lo - 1), lo - 1, np.where(ok(lo), lo, lo + 1))
-        hi = np.where(ok(hi + 1), hi + 1, np.where(ok(hi), hi, hi - 1))
-        n = np.minimum(hi, no - 1) - np.maximum(lo, 0) + 1
-        total += int(np.maximum(n, 0).sum())
+def nonzero_weights_optabs(E, K1, K2, nv: int, nb: int, no: int, dw: float, eta: float, lor_cut: float) -> int:
+    """The same for the Optabs weight at T = 0: pairs n != m with |f_n - f_m|
+    = 1 (valence to conduction and back), L = eta/(dE^2 + eta^2) >=
+    Lor_cut/eta, dE = io dw + E(K1,n) - E(K2,m), io = -no..no
+    (Optabs.f90:356-402)."""
+    K1 = np.asarray(K1)
+    K2 = np.asarray(K2)
+    total = 0
+    for k0 in range(0, len(K1), 2048):
+        k1, k2 = K1[k0:k0 + 2048], K2[k0:k0 + 2048]
+        for n_rng, m_rng in (((0, nv), (nv, nb)), ((nv, nb), (0, nv))):
+            a, b = _pairs(E, k1, k2, n_rng, m_rng)
+            total += _window_count(a, b, -no, no, dw, eta, lor_cut / eta)
     return total
 
 

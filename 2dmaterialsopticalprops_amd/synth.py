@@ -73,11 +73,18 @@ CONFIGS = {
     # (host.local_fields_by_qg); nlf_target is the largest N_G
     "cfg3": SynthConfig("cfg3", nk=48, nval=8, ncond=8, ngk_target=1000.0, lf="xyz", nlf_target=61,
                         no=700, omax_ev=25.0, eta_ev=0.08, seed=20261202 + 3),
+    # spectral function + KK (Optabs route): 2 no + 1 = 2001 bins over 0-40 eV,
+    # KK to no - 1 = 999 frequencies; a bilayer cell treated as one 2D cell
+    # (a = 4.68 bohr, c = 40 bohr); 24 q, per-q N_G from 61 to 81 by |q+G|
+    "cfg4": SynthConfig("cfg4", a=4.68, c=40.0, nk=36, nval=16, ncond=24, ngk_target=3200.0, lf="xyz",
+                        nlf_target=81, no=1000, omax_ev=40.0, eta_ev=0.05, seed=20261202 + 4),
 }
 
-# q points of the batched-W workload: the Sloss q loop iq = 2 .. 101
-# (q = (iq-1) k_min, Sloss.f90:292, 595-597), N_G rising linearly from 21 to 61.
-BATCHED_Q = {"cfg3": dict(iq=list(range(2, 102)), nlf_min=21, nlf_max=61)}
+# q points of the batched workloads: the drivers' q loop (q = (iq-1) k_min,
+# Sloss.f90:292, 595-597), iq = 2 .. 101 for batched W and 2 .. 25 for the
+# Optabs route, N_G rising linearly over the q points.
+BATCHED_Q = {"cfg3": dict(iq=list(range(2, 102)), nlf_min=21, nlf_max=61, program="sloss"),
+             "cfg4": dict(iq=list(range(2, 26)), nlf_min=61, nlf_max=81, program="optabs")}
 
 
 def batched_q_points(name: str, nq: Optional[int] = None):

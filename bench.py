@@ -45,38 +45,52 @@ def log(*a):
 
 def cpu_baseline(sys_, r, threads, target_s=12.0):
     """The oracle (FP64 restatement of the reference loops) on a k-subset,
-    scaled to the full k-grid; KK and Dyson/W at full size."""
+    scaled to the full k-grid; the tail (KK and Dyson/W, or KK and the Pi
+    assembly) at full size."""
     from oracle import oracle as orc
-    cfg = sys_.cfg
-    eta = cfg.eta_ev / 27.2113846
+    cfg, rc = sys_.cfg, r.rc
+    H = 27.2113846
+    eta = cfg.eta_ev / H
+    optabs = rc.program == "optabs"
 
     def problem(nk):
         return orc.Problem(sys_, r.K1[:nk], r.K2[:nk], r.iG0[:nk], r.ktot[:nk], r.q, r.lf_idx,
                            map_mode=0, ngd=r.ngd, E_hartree=r.E)
 
+    def s0(pb):
+        if optabs:
+            pol = importlib.import_module("2dmaterialsopticalprops_amd._abi").POL[rc.pol]
+            return orc.optabs_s0(pb, cfg.nval, cfg.no, pol, r.domega, eta, rc.lor_cut, rc.df_cut,
+                                 rc.efermi_ev / H, rc.temp_ev / H, sys_.vcell, sys_.gcar, nthreads=threads)
+        return orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, eta, sys_.vcell, nthreads=threads), None
+
     nk_full = len(r.K1)
-    nk = min(nk_full, 64)
+    nk = min(nk_full, 16 if optabs else 64)
     t0 = time.perf_counter()
-    orc.sloss_s0(problem(nk), cfg.nval, cfg.no, r.domega, eta, sys_.vcell, nthreads=threads)
+    s0(problem(nk))
     t_probe = time.perf_counter() - t0
     per_k = t_probe / nk
     nk_s = int(min(nk_full, max(nk, target_s / max(per_k, 1e-9))))
     nk_s = max(8, (nk_s // 8) * 8)
     pb = problem(nk_s)
     t0 = time.perf_counter()
-    S0 = orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, eta, sys_.vcell, nthreads=threads)
+    S0, Q = s0(pb)
     t_s0 = time.perf_counter() - t0
     t0 = time.perf_counter()
-    chi0 = orc.kk_sloss(S0, r.domega)
-    V = orc.genV(r.rc.eps, r.q[0] * sys_.gcar, r.q[1] * sys_.gcar, cfg.c, r.parg, r.glf, sys_.gcar)
-    outs = [orc.dyson_w(chi0[f], V) for f in range(cfg.no - 1)]
+    out = {"S0": S0}
+    if optabs:
+        out["chi0"] = orc.kk_optabs(S0, r.domega)
+        out["pi11"], out["si"], out["sa"] = orc.pi_assembly(out["chi0"], Q, r.domega, rc.eta_intra_ev / H, cfg.c)
+    else:
+        out["chi0"] = orc.kk_sloss(S0, r.domega)
+        V = orc.genV(rc.eps, r.q[0] * sys_.gcar, r.q[1] * sys_.gcar, cfg.c, r.parg, r.glf, sys_.gcar)
+        outs = [orc.dyson_w(out["chi0"][f], V) for f in range(cfg.no - 1)]
+        out["epsinv"] = np.array([o[0] for o in outs])
+        out["W"] = np.array([o[2] for o in outs])
     t_tail = time.perf_counter() - t0
     t_full = t_s0 * nk_full / nk_s + t_tail
-    return {
-        "t_full_s": t_full, "nk_sample": nk_s, "t_s0_sample": t_s0, "t_tail": t_tail,
-        "S0": S0, "chi0": chi0, "epsinv": np.array([o[0] for o in outs]),
-        "W": np.array([o[2] for o in outs]),
-    }
+    out.update({"t_full_s": t_full, "nk_sample": nk_s, "t_s0_sample": t_s0, "t_tail": t_tail})
+    return out
 
 
 def hbm_calibration(mib):
@@ -149,15 +163,28 @@ def hbm_traffic(workload, prec, stage):
     return hits[0] if len(hits) == 1 else None
 
 
-def vertex_kernel(r, prec):
+def vertex_kernel(r, prec, lf_idx=None):
     """Name of the vertex kernel the library picks for the runner's local
-    fields (the column form when the in-plane classes hold at least 12 fields
-    on average, else the gather-direct FP32 or staged FP64 form)."""
-    m = r.sys.mill_g[np.asarray(r.lf_idx) - 1]
+    fields: the column form when the in-plane classes hold at least 12 fields
+    on average (current vertices: one class only), else the gather-direct
+    FP32 form (Sloss) or the staged form."""
+    m = r.sys.mill_g[np.asarray(r.lf_idx if lf_idx is None else lf_idx) - 1]
     ncls = len({(int(x[0]), int(x[1])) for x in m})
-    if r.rc.vertex_map == "direct" and len(m) >= 12 * ncls:
+    optabs = r.rc.program == "optabs"
+    if r.rc.vertex_map == "direct" and len(m) >= 12 * ncls and not (optabs and ncls > 1):
         return "k_vertex_col<ColF32>" if prec == "mixed" else "k_vertex_col<ColF64>"
-    return "k_vertex_gd<PrecF32>" if prec == "mixed" else "k_vertex<PrecF64> (staged)"
+    if prec == "mixed" and not optabs:
+        return "k_vertex_gd<PrecF32>"
+    return "k_vertex<PrecF32> (staged)" if prec == "mixed" else "k_vertex<PrecF64> (staged)"
+
+
+def vertex_kernels(r, prec, plans):
+    """The vertex kernels of a q loop, with the number of q points each serves."""
+    names = {}
+    for pt in plans:
+        n = vertex_kernel(r, prec, pt["lf_idx"])
+        names[n] = names.get(n, 0) + 1
+    return " + ".join(f"{n} on {c} q" for n, c in names.items())
 
 
 def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None):
@@ -191,6 +218,22 @@ def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None):
     }
 
 
+def work_model(r, host, cfg, K1, K2, nlf, ntr):
+    """Algorithmic work of the two matrix-core kernels for one q: nonzero
+    (transition, bin) weights, S0 flops (2 N_G^2 per real Sloss weight, 8 N_G^2
+    per complex Optabs weight) and vertex flops (8 N_G NGd per transition and
+    vertex: rho for Sloss, j for Optabs, j of both polarisations for yz --
+    one complex product per term, as the reference's loop)."""
+    nv, nb = cfg.nval, cfg.nband
+    eta = cfg.eta_ev / host.HARTREE
+    if r.rc.program == "optabs":
+        nnz = host.nonzero_weights_optabs(r.E, K1, K2, nv, nb, cfg.no, r.domega, eta, r.rc.lor_cut)
+        nvtx = 2 if r.rc.pol == "yz" else 1
+        return nnz, 8.0 * nlf * nlf * nnz, 8.0 * nlf * r.ngd * ntr * nvtx
+    nnz = host.nonzero_weights(r.E, K1, K2, nv, nb, cfg.no, r.domega, eta)
+    return nnz, 2.0 * nlf * nlf * nnz, 8.0 * nlf * r.ngd * ntr
+
+
 def time_q_loop(r, plans, steps, warmup):
     """Warmup passes, then `steps` timed passes of the q loop: per q the
     context takes the q point (local fields, k+q map) and runs the chain."""
@@ -218,14 +261,18 @@ def time_q_loop(r, plans, steps, warmup):
 
 
 def bench_batched(args, s, cfg, synth, host):
-    """(q, omega) W-solves/s over a q loop with per-q local-field counts."""
+    """Throughput over a q loop with per-q local-field counts: (q, omega)
+    W-solves/s for the Sloss chain, omega-points/s for the Optabs route."""
     spec = synth.batched_q_points(args.config, args.nq)
+    program = synth.BATCHED_Q[args.config]["program"]
+    optabs = program == "optabs"
+    unit = "omega-points/s" if optabs else "(q,omega) W-solves/s"
     precs = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
     runs, runners, plans = {}, {}, {}
     for prec in precs:
         t0 = time.perf_counter()
-        r = host.Runner(s, host.RunConfig(program="sloss", vertex_map=args.vertex_map, precision=prec,
-                                          iq=spec[0][0]))
+        r = host.Runner(s, host.RunConfig(program=program, vertex_map=args.vertex_map, precision=prec,
+                                          iq=spec[0][0], pol=args.pol))
         t1 = time.perf_counter()
         plans[prec] = [r.q_point(iq, nlf) for iq, nlf in spec]
         log(f"[bench] {prec}: setup+upload {t1 - t0:.1f}s, {len(spec)} q points planned "
@@ -241,46 +288,54 @@ def bench_batched(args, s, cfg, synth, host):
 
     # algorithmic work summed over the q points
     nv, nb = cfg.nval, cfg.nband
-    eta = cfg.eta_ev / host.HARTREE
+    ntr = r.ntransitions()
     flops_acc = flops_vtx = 0.0
-    ntr = len(r.K1) * nv * (nb - nv)
     nnz_total = 0
     for pt, (iq, nlf) in zip(plans[head], spec):
-        nnz = host.nonzero_weights(r.E, pt["K1"], pt["K2"], nv, nb, cfg.no, r.domega, eta)
+        nnz, fa, fv = work_model(r, host, cfg, pt["K1"], pt["K2"], nlf, ntr)
         nnz_total += nnz
-        flops_acc += 2.0 * nlf * nlf * nnz
-        flops_vtx += 8.0 * nlf * r.ngd * ntr
-    workload_key = f"{args.config} batched sloss {args.vertex_map}"
-    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key, vertex_kernel(r, head))
+        flops_acc += fa
+        flops_vtx += fv
+    workload_key = f"{args.config} batched {program} {args.vertex_map}"
+    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key,
+                           vertex_kernels(r, head, plans[head]))
     roofline["other"]["nnz_weights"] = nnz_total
     dyson_ms = runs[head]["stages"]["dyson_or_pi"]
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "(q,omega) W-solves/s", "n_gpus": 1,
+        "metric": METRIC, "value": round(value, 1), "unit": unit, "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(runs[head]["ms_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": ("fp32 vertices + bf16 hi/lo x3 S0 into fp32, fp64 weights/reduction/KK/Dyson"
                   if head == "mixed" else "f64"),
         "data": "synthetic",
         "config": {
-            "workload": f"{args.config}: batched W(q, omega), Sloss q loop iq={spec[0][0]}..{spec[-1][0]} "
-                        f"({len(spec)} q), {len(s.ngk)} k ({cfg.nk}x{cfg.nk}x1), {nb} bands "
-                        f"({nv}v/{nb - nv}c), ngk {s.ngk.min()}-{s.ngk.max()}, N_G {spec[0][1]}-{spec[-1][1]} "
-                        f"by |q+G|, {cfg.no} omega 0-{cfg.omax_ev:g} eV, eta={cfg.eta_ev * 1e3:g} meV",
-            "precision": head, "q_points": len(spec), "w_solves_per_step": nsolve, "transitions_per_q": ntr,
+            "workload": (f"{args.config}: Optabs spectral function + KK, pol {args.pol}, q loop "
+                         f"iq={spec[0][0]}..{spec[-1][0]} ({len(spec)} q), {len(s.ngk)} k ({cfg.nk}x{cfg.nk}x1), "
+                         f"{nb} bands ({nv}v/{nb - nv}c), ngk {s.ngk.min()}-{s.ngk.max()}, "
+                         f"N_G {spec[0][1]}-{spec[-1][1]} by |q+G|, {2 * cfg.no + 1} bins "
+                         f"+-{cfg.omax_ev:g} eV -> KK to {cfg.no - 1} omega, eta={cfg.eta_ev * 1e3:g} meV")
+            if optabs else
+                        (f"{args.config}: batched W(q, omega), Sloss q loop iq={spec[0][0]}..{spec[-1][0]} "
+                         f"({len(spec)} q), {len(s.ngk)} k ({cfg.nk}x{cfg.nk}x1), {nb} bands "
+                         f"({nv}v/{nb - nv}c), ngk {s.ngk.min()}-{s.ngk.max()}, N_G {spec[0][1]}-{spec[-1][1]} "
+                         f"by |q+G|, {cfg.no} omega 0-{cfg.omax_ev:g} eV, eta={cfg.eta_ev * 1e3:g} meV"),
+            "precision": head, "q_points": len(spec),
+            ("omega_points_per_step" if optabs else "w_solves_per_step"): nsolve, "transitions_per_q": ntr,
             "device_ms_per_step": round(runs[head]["device_ms"], 4),
             "stage_ms": {k: round(v, 4) for k, v in runs[head]["stages"].items()},
-            "dyson_w_solves_per_s": round(nsolve / (dyson_ms * 1e-3), 1) if dyson_ms > 0 else None,
         },
         "roofline": roofline,
     }
+    if not optabs and dyson_ms > 0:
+        out["config"]["dyson_w_solves_per_s"] = round(nsolve / (dyson_ms * 1e-3), 1)
     if "fp64" in runs and head != "fp64":
         f = runs["fp64"]
         fv = nsolve / (f["ms_step"] * 1e-3)
         out["fp64_gpu"] = {
-            "value": round(fv, 1), "unit": "(q,omega) W-solves/s", "ms_per_step": round(f["ms_step"], 4),
+            "value": round(fv, 1), "unit": unit, "ms_per_step": round(f["ms_step"], 4),
             "stage_ms": {k: round(v, 4) for k, v in f["stages"].items()},
             "roofline": roofline_of(f, "fp64", flops_acc, flops_vtx, workload_key,
-                                    vertex_kernel(runners["fp64"], "fp64")),
+                                    vertex_kernels(runners["fp64"], "fp64", plans["fp64"])),
         }
         out["mixed_vs_fp64"] = round(value / fv, 2)
     if not args.no_cpu:
@@ -289,13 +344,14 @@ def bench_batched(args, s, cfg, synth, host):
             # 5 q points spanning the N_G range, each timed on a k-subset,
             # then scaled by sum N_G^2 over all q
             sub = sorted({int(round(j * (len(spec) - 1) / 4)) for j in range(5)})
-            t_sub, acc = 0.0, {p: [] for p in precs}
+            t_sub, t_meas, acc = 0.0, 0.0, {p: [] for p in precs}
             notes = []
             for j in sub:
                 for p in precs:
                     runners[p].set_q_point(plans[p][j])
                 cb = cpu_baseline(s, runners[head], threads, args.cpu_seconds / len(sub))
                 t_sub += cb["t_full_s"]
+                t_meas += cb["t_s0_sample"] + cb["t_tail"]
                 notes.append(f"iq={spec[j][0]} N_G={spec[j][1]}: {cb['nk_sample']} k")
                 for p in precs:
                     acc[p].append(dict(accuracy(s, runners[p], cb), iq=spec[j][0], N_G=spec[j][1]))
@@ -303,11 +359,11 @@ def bench_batched(args, s, cfg, synth, host):
             w_sub = sum(spec[j][1] ** 2 for j in sub)
             t_full = t_sub * w_all / w_sub
             out["cpu_baseline"] = {
-                "value": round(nsolve / t_full, 2), "unit": "(q,omega) W-solves/s", "cores": threads,
+                "value": round(nsolve / t_full, 2), "unit": unit, "cores": threads,
                 "kind": "port",
                 "sample": f"oracle chain on {len(sub)} of {len(spec)} q ({'; '.join(notes)}), S0 on a k-subset "
                           f"scaled to {len(r.K1)} k, total scaled by sum N_G^2 over all q "
-                          f"(x{w_all / w_sub:.2f}); {t_sub:.1f}s measured",
+                          f"(x{w_all / w_sub:.2f}); {t_meas:.1f}s measured, {t_full:.1f}s estimated for the loop",
             }
             out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
             out["accuracy_vs_fp64_oracle"] = acc
@@ -326,6 +382,8 @@ def main():
     ap.add_argument("--config", default="cfg2")
     ap.add_argument("--program", default="sloss", choices=["sloss", "optabs"])
     ap.add_argument("--vertex-map", default="direct", choices=["direct", "igk"])
+    ap.add_argument("--pol", default="xx", choices=["xx", "yy", "zz", "yz"],
+                    help="Optabs: polarisation of the current vertices")
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"],
                     help="'both': value is the mixed path, timed next to the FP64 path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and accuracy check")
@@ -355,7 +413,8 @@ def main():
     runs, runners = {}, {}
     for prec in precs:
         t0 = time.perf_counter()
-        r = host.Runner(s, host.RunConfig(program=args.program, vertex_map=args.vertex_map, precision=prec))
+        r = host.Runner(s, host.RunConfig(program=args.program, vertex_map=args.vertex_map, precision=prec,
+                                          pol=args.pol))
         log(f"[bench] {prec}: setup+upload {time.perf_counter() - t0:.1f}s nlf={r.nlf} no={cfg.no} "
             f"ntrans={r.ntransitions()}")
         res = time_chain(r, args.steps, args.warmup)
@@ -369,11 +428,8 @@ def main():
 
     # algorithmic work of the two matrix-core kernels (nonzero weights only)
     nv, nb = cfg.nval, cfg.nband
-    eta = cfg.eta_ev / host.HARTREE
-    nnz = host.nonzero_weights(r.E, r.K1, r.K2, nv, nb, cfg.no, r.domega, eta)
-    ntr = len(r.K1) * nv * (nb - nv)
-    flops_acc = 2.0 * r.nlf * r.nlf * nnz
-    flops_vtx = 8.0 * r.nlf * r.ngd * ntr
+    ntr = r.ntransitions()
+    nnz, flops_acc, flops_vtx = work_model(r, host, cfg, r.K1, r.K2, r.nlf, ntr)
     workload_key = f"{args.config} {args.program} {args.vertex_map}"
     roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key, vertex_kernel(r, head))
     roofline["other"]["nnz_weights"] = nnz
@@ -432,26 +488,33 @@ def program_name(p):
 
 
 def accuracy(s, r, cb):
+    """The device chain on the oracle's k-subset against the oracle: S0, the
+    KK output and the chain's end product (eps^-1, W, loss for Sloss; Pi_11
+    and the conductivities for Optabs)."""
     nk = cb["nk_sample"]
     ctx = r.ctx
     ctx.set_q(r.K1[:nk], r.K2[:nk], r.iG0[:nk], r.ktot[:nk], r.q)
     ctx.vertices()
     S0, _ = ctx.accumulate_s0()
     chi0 = ctx.kk()
-    ctx.coulomb()
-    epsinv, chi, W = ctx.dyson_w()
 
     def fro(a, b):
         return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
+    res = {"sample": f"{nk} of {len(r.K1)} k-points, full bands/plane waves/N_G/omega",
+           "S0_rel_fro": fro(S0, cb["S0"]), "chi0_rel_fro": fro(chi0, cb["chi0"])}
+    if r.rc.program == "optabs":
+        _, pi11, sig = ctx.epilogue()
+        res.update({"pi11_rel_fro": fro(pi11, cb["pi11"]), "sigma_inter_rel_fro": fro(sig[0], cb["si"]),
+                    "sigma_peak_shift_bins": int(np.argmax(sig[0]) - np.argmax(cb["si"]))})
+        return res
+    ctx.coulomb()
+    epsinv, chi, W = ctx.dyson_w()
     loss_g = -epsinv[:, 0, 0].imag
     loss_o = -cb["epsinv"][:, 0, 0].imag
-    return {
-        "sample": f"{nk} of {len(r.K1)} k-points, full bands/plane waves/N_G/omega",
-        "S0_rel_fro": fro(S0, cb["S0"]), "chi0_rel_fro": fro(chi0, cb["chi0"]),
-        "epsinv_rel_fro": fro(epsinv, cb["epsinv"]), "W_rel_fro": fro(W, cb["W"]),
-        "loss_peak_shift_bins": int(np.argmax(loss_g) - np.argmax(loss_o)),
-    }
+    res.update({"epsinv_rel_fro": fro(epsinv, cb["epsinv"]), "W_rel_fro": fro(W, cb["W"]),
+                "loss_peak_shift_bins": int(np.argmax(loss_g) - np.argmax(loss_o))})
+    return res
 
 
 if __name__ == "__main__":

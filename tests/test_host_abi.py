@@ -127,9 +127,39 @@ def test_local_fields_by_qg():
         np.testing.assert_array_equal(pg, np.where(mz % 2 == 0, 1, -1))
 
 
+def test_optabs_nonzero_weight_count_matches_bin_by_bin():
+    """The Optabs count (bins -no..no, both directions of each valence /
+    conduction pair, L = eta/(dE^2 + eta^2) >= Lor_cut/eta, Optabs.f90:356-402)
+    equals a bin-by-bin evaluation."""
+    rng = np.random.default_rng(11)
+    for trial in range(4):
+        nk, nv, nb, no = 20, 3, 6, 120
+        E = np.sort(rng.normal(size=(nk, nb)) * 0.3, axis=1)
+        K1 = rng.integers(1, nk + 1, size=15)
+        K2 = rng.integers(1, nk + 1, size=15)
+        dw = 0.9 / (no - 1) * (1 + 0.5 * trial)
+        eta = 0.002 * (1 + trial)
+        lor_cut = 1.0e-3 * (1 + 3 * trial)
+        want = 0
+        for k1, k2 in zip(K1, K2):
+            for n in range(nb):
+                for m in range(nb):
+                    if (n < nv) == (m < nv):
+                        continue
+                    for io in range(-no, no + 1):
+                        dE = io * dw + E[k1 - 1, n] - E[k2 - 1, m]
+                        if abs(eta / (dE * dE + eta * eta)) >= lor_cut / eta:
+                            want += 1
+        assert want > 0
+        assert host.nonzero_weights_optabs(E, K1, K2, nv, nb, no, dw, eta, lor_cut) == want
+
+
 def test_batched_q_points():
     pts = synth.batched_q_points("cfg3")
     assert len(pts) == 100 and pts[0] == (2, 21) and pts[-1] == (101, 61)
     assert all(b >= a for (_, a), (_, b) in zip(pts, pts[1:]))
     sub = synth.batched_q_points("cfg3", 5)
     assert sub[0] == pts[0] and sub[-1] == pts[-1] and len(sub) == 5
+    pts = synth.batched_q_points("cfg4")
+    assert len(pts) == 24 and pts[0] == (2, 61) and pts[-1] == (25, 81)
+    assert synth.BATCHED_Q["cfg4"]["program"] == "optabs"
