@@ -215,41 +215,83 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
 
   const int r = lane & 15, q4 = lane >> 4;
   const int ii = q4 >> 1, part = q4 & 1;
+  constexpr bool MK = MAXB >= 4;
 
   cplx pc1[kPfC1], pb[kPfB];
+  unsigned pm_a = 0, pm_b = 0;  // presence of each prefetched value
   real pj1 = 1, pj2 = 1;
+  // J without the plane-wave term: Gcar-free (q + 2k + Glf(G))_pol of this thread's G
+  double jb1 = 0.0, jb2 = 0.0;
+  if (KIND >= 1 && bg_ok) {
+    jb1 = (a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * bg + a.pol1];
+    if (KIND == 2) jb2 = (a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * bg + a.pol2];
+  }
 
   // Loads of one chunk into registers: the C1 rows, the gathered k+q rows
   // (row j of this thread's G, -1 = absent) and, for current vertices, J
   // from the Miller key mkey of the plane wave.
+  // MK (many band blocks): every load is issued from a clamped in-bounds
+  // address and its presence kept in a mask, applied when the values are
+  // staged -- a conditional load joined with a zero fill makes the compiler
+  // wait for each load before the next one issues.  With few blocks the
+  // products per chunk are few and the masked form's extra loads cost more
+  // (measured: MAXB 8 7% faster masked, MAXB 1 14% slower), so those use
+  // conditional loads.
   auto prefetch = [&](int i0, int j, int mkey) {
     {
       const int i = i0 + arow;
-      const cplx *row = c1 + (size_t)i * a.nband;
+      const bool iok = i < n1;
+      unsigned m = 0;
+      if constexpr (MK) {
+        const cplx *row = c1 + (size_t)(iok ? i : 0) * a.nband;
 #pragma unroll
-      for (int u = 0; u < kPfC1; ++u) {
-        const int n = asub + TPA * u;
-        pc1[u] = (i < n1 && n < nna) ? row[n] : P::make(0, 0);
+        for (int u = 0; u < kPfC1; ++u) {
+          const int n = asub + TPA * u;
+          const bool ok = iok && n < nna;
+          pc1[u] = row[ok ? n : 0];
+          m |= ok ? (1u << u) : 0u;
+        }
+      } else {
+        const cplx *row = c1 + (size_t)i * a.nband;
+#pragma unroll
+        for (int u = 0; u < kPfC1; ++u) {
+          const int n = asub + TPA * u;
+          pc1[u] = (iok && n < nna) ? row[n] : P::make(0, 0);
+        }
+        m = ~0u;
       }
+      pm_a = m;
     }
-    const cplx *row = c2 + (size_t)(j >= 0 ? j : 0) * a.nband;
+    unsigned m = 0;
+    if constexpr (MK) {
+      const cplx *row = c2 + (size_t)(j >= 0 ? j : 0) * a.nband;
 #pragma unroll
-    for (int u = 0; u < kPfB; ++u) {
-      const int m = bsub + TPR * u;
-      pb[u] = (j >= 0 && m < nmb) ? row[m] : P::make(0, 0);
+      for (int u = 0; u < kPfB; ++u) {
+        const int mb = bsub + TPR * u;
+        const bool ok = j >= 0 && mb < nmb;
+        pb[u] = row[ok ? mb : 0];
+        m |= ok ? (1u << u) : 0u;
+      }
+    } else {
+      const cplx *row = c2 + (size_t)(j >= 0 ? j : 0) * a.nband;
+#pragma unroll
+      for (int u = 0; u < kPfB; ++u) {
+        const int mb = bsub + TPR * u;
+        pb[u] = (j >= 0 && mb < nmb) ? row[mb] : P::make(0, 0);
+      }
+      m = ~0u;
     }
+    pm_b = m;
     if (KIND >= 1) {
       pj1 = pj2 = 0;
       if (bg_ok && i0 + bc < n1) {
         int s0, s1, s2;
         mill_unpack(mkey, s0, s1, s2);
         // J = Gcar (q + 2k + Glf(G) + 2 G(i))_pol with G(i) = KC mill (left to right)
-        const double jb1 = (a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * bg + a.pol1];
         const double g1 = ((0.0 + a.kc[a.pol1 + 0] * (double)s0) + a.kc[a.pol1 + 3] * (double)s1) +
                           a.kc[a.pol1 + 6] * (double)s2;
         pj1 = (real)(0.5 * ((jb1 + 2.0 * g1) * a.gcar));
         if (KIND == 2) {
-          const double jb2 = (a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * bg + a.pol2];
           const double g2 = ((0.0 + a.kc[a.pol2 + 0] * (double)s0) + a.kc[a.pol2 + 3] * (double)s1) +
                             a.kc[a.pol2 + 6] * (double)s2;
           pj2 = (real)(0.5 * ((jb2 + 2.0 * g2) * a.gcar));
@@ -263,19 +305,20 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   //   direct: raw map value of chunk c+1 in jn (loaded while chunk c-1 ran)
   //   igk:    Miller key of chunk c+2 in kn, its box lookup -> jn one chunk
   //           later (two dependent loads, each hidden behind a chunk)
-  constexpr int kNone = -2147483647 - 1;
-  auto load_key = [&](int i) -> int {
-    return (bg_ok && i < n1) ? (direct ? dmap[i] : src[i]) : kNone;
-  };
-  auto lookup_igk = [&](int key) -> int {
-    if (key == kNone) return -1;
+  // Index loads read clamped addresses; whether plane wave i exists is known
+  // from i alone and applied when the value is used.
+  auto key_ok = [&](int i) -> bool { return bg_ok && i < n1; };
+  auto load_key = [&](int i) -> int { return direct ? dmap[min(i, n1 - 1)] : src[min(i, n1 - 1)]; };
+  // box lookup of key (valid when ok): the raw box entry, found = present
+  auto lookup_igk = [&](int key, bool ok, bool &found) -> int {
     int s0, s1, s2;
     mill_unpack(key, s0, s1, s2);
     const int bi = a.kb.index(s0 + lf0 + d0, s1 + lf1 + d1, s2 + lf2 + d2);
-    return bi >= 0 ? (int)box2[bi] : -1;
+    found = ok && bi >= 0;
+    return (int)box2[found ? bi : 0];
   };
   auto row_of = [&](int jraw) -> int { return (jraw >= 0 && jraw < lim2) ? jraw : -1; };
-  auto load_mkey = [&](int i) -> int { return (KIND >= 1 && bg_ok && i < n1) ? src[i] : 0; };
+  auto load_mkey = [&](int i) -> int { return KIND >= 1 ? src[min(i, n1 - 1)] : 0; };
 
   // registers -> LDS in matrix-core form
   auto stage = [&]() {
@@ -285,7 +328,8 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
       for (int u = 0; u < kPfC1; ++u) {
         const int n = asub + TPA * u;
         if (n < nna) {
-          const cplx z = pc1[u];
+          const bool ok = (pm_a >> u) & 1u;
+          const cplx z = P::make(ok ? pc1[u].x : (real)0, ok ? pc1[u].y : (real)0);
           const int rre = P::arow_index(n, false), rim = P::arow_index(n, true);
           // Re rows: (C1r, C1i); Im rows: (-C1i, C1r) for part (0, 1)
           A_at(ai, 0, rre)[as] = z.x;
@@ -300,7 +344,8 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
     for (int u = 0; u < kPfB; ++u) {
       const int m = bsub + TPR * u;
       if (m < nmb) {
-        const cplx z = pb[u];
+        const bool ok = (pm_b >> u) & 1u;
+        const cplx z = P::make(ok ? pb[u].x : (real)0, ok ? pb[u].y : (real)0);
         B_at(0, bw, bi_, 0, m)[bs] = z.x * pj1;
         B_at(0, bw, bi_, 1, m)[bs] = z.y * pj1;
         if (KIND == 2) {
@@ -333,30 +378,76 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
     }
   }
 
-  int jn, kn = kNone, mkn;
-  {
+  // jn: raw index of chunk c+1 (valid when jn_ok); kn: igk key of chunk c+2
+  // (valid when kn_ok).  MK: only loads write jn and kn, so no zero fill has
+  // to wait for a load in flight; else absent keys are kNone.
+  constexpr int kNone = -2147483647 - 1;
+  auto load_key0 = [&](int i) -> int { return key_ok(i) ? load_key(i) : kNone; };
+  auto lookup0 = [&](int key) -> int {
+    bool f;
+    const int v = lookup_igk(key, key != kNone, f);
+    return f ? v : -1;
+  };
+  int jn, kn = 0, mkn;
+  bool jn_ok = true, kn_ok = false;
+  if constexpr (MK) {
+    const bool ok0 = key_ok(bc);
     const int key0 = load_key(bc);
-    const int j0 = row_of(direct ? key0 : lookup_igk(key0));
-    prefetch(0, j0, load_mkey(bc));
+    int j0;
+    if (direct) {
+      j0 = ok0 ? key0 : -1;
+    } else {
+      bool f;
+      const int v = lookup_igk(key0, ok0, f);
+      j0 = f ? v : -1;
+    }
+    prefetch(0, row_of(j0), load_mkey(bc));
+    const bool ok1 = key_ok(CH + bc);
     const int key1 = load_key(CH + bc);
-    jn = direct ? key1 : lookup_igk(key1);
+    if (direct) {
+      jn = key1;
+      jn_ok = ok1;
+    } else {
+      jn = lookup_igk(key1, ok1, jn_ok);
+      kn_ok = key_ok(2 * CH + bc);
+      kn = load_key(2 * CH + bc);
+    }
     mkn = load_mkey(CH + bc);
-    if (!direct) kn = load_key(2 * CH + bc);
+  } else {
+    const int key0 = load_key0(bc);
+    const int j0 = row_of(direct ? key0 : lookup0(key0));
+    prefetch(0, j0, key_ok(bc) ? load_mkey(bc) : 0);
+    const int key1 = load_key0(CH + bc);
+    jn = direct ? key1 : lookup0(key1);
+    mkn = key_ok(CH + bc) ? load_mkey(CH + bc) : 0;
+    if (!direct) kn = load_key0(2 * CH + bc);
   }
   for (int i0 = 0; i0 < n1; i0 += CH) {
     __syncthreads();  // the previous chunk's LDS reads are done
     stage();
     __syncthreads();
-    if (i0 + CH < n1) prefetch(i0 + CH, row_of(jn), mkn);
-    {
+    if constexpr (MK) {
+      prefetch(i0 + CH, jn_ok ? row_of(jn) : -1, mkn);  // past the end: clamped and masked
       const int i2 = i0 + 2 * CH + bc;
       if (direct) {
+        jn_ok = key_ok(i2);
         jn = load_key(i2);
       } else {
-        jn = lookup_igk(kn);
+        jn = lookup_igk(kn, kn_ok, jn_ok);
+        kn_ok = key_ok(i2 + CH);
         kn = load_key(i2 + CH);
       }
       mkn = load_mkey(i2);
+    } else {
+      if (i0 + CH < n1) prefetch(i0 + CH, row_of(jn), mkn);
+      const int i2 = i0 + 2 * CH + bc;
+      if (direct) {
+        jn = load_key0(i2);
+      } else {
+        jn = lookup0(kn);
+        kn = load_key0(i2 + CH);
+      }
+      mkn = key_ok(i2) ? load_mkey(i2) : 0;
     }
     if (active) {
       // four K steps per pass: two 16-byte reads per operand

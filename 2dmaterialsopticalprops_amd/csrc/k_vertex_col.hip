@@ -101,12 +101,19 @@ struct ColF32 {
 // Staging of one chunk in registers (loaded one chunk ahead, stored to LDS
 // after the barrier).  Thread t: C1 rows of positions t/8 + 32u, band t%8;
 // k+q rows t/4 + 64u, bands m 4(t%4)..+3; B row of position t (t < KC).
+// Every load is issued unconditionally from a clamped, in-bounds address and
+// its presence kept in a mask; the zero fill of absent values happens in the
+// LDS store.  (A conditional load joined with a zero fill makes the compiler
+// wait for each load before the next one issues.)
 template <class P, int KC, int NRU>
 struct ColStage {
   typename P::cplx a[KC / 32];
   typename P::cplx b[NRU][4];
   int prow;
   typename P::real beta[2];
+  unsigned amask;  // bit u: C1 value u present
+  unsigned bmask;  // bit 4u + e: k+q value (u, e) present
+  int np, nr;      // positions and B rows of the chunk
 };
 
 // PIPE: the next chunk is loaded into registers while the current one runs
@@ -118,6 +125,9 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
   using cplx = typename P::cplx;
   using acc_t = typename P::acc;
   constexpr int NV = 1 + (KIND >= 1) + (KIND == 2);  // A variants: C1, beta1 C1, beta2 C1
+  // steps per pass of the product loop: two in FP32 (FP64 would pass 256
+  // registers and halve the waves per SIMD)
+  constexpr int SU = (sizeof(real) == 4) ? 2 : 1;
   extern __shared__ __align__(16) unsigned char col_raw[];
   cplx *sA = (cplx *)col_raw;         // [KC][8]   C1 of position p, band n
   int *sP = (int *)(sA + KC * 8);     // [KC]      B row of position p
@@ -153,46 +163,66 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
   auto load = [&](int ch) {
     const int p0 = a.chunk[ch], np = a.chunk[ch + 1] - p0;
     const int r0 = crow[ch], nr = crow[ch + 1] - r0;
+    st.np = np;
+    st.nr = nr;
+    const bool nok = sa_n < vb.nrows;
+    const int nn = nok ? sa_n : 0;
+    int slot_u[KC / 32];
 #pragma unroll
-    for (int u = 0; u < KC / 32; ++u) {
-      const int p = sa_p + 32 * u;
-      st.a[u] = (p < np && sa_n < vb.nrows) ? c1[(size_t)a.slot[p0 + p] * a.nband + sa_n] : P::make(0, 0);
-    }
-    st.prow = (tid < KC && tid < np) ? prow[p0 + tid] : 0;
+    for (int u = 0; u < KC / 32; ++u) slot_u[u] = a.slot[p0 + min(sa_p + 32 * u, np - 1)];
+    const int tp = min(tid, np - 1);
+    st.prow = prow[p0 + tp];
     if (KIND >= 1) {
 #pragma unroll
-      for (int v = 0; v < NV - 1; ++v)
-        st.beta[v] = (tid < KC && tid < np) ? (real)a.beta[(size_t)v * a.npos + p0 + tid] : (real)0;
+      for (int v = 0; v < NV - 1; ++v) st.beta[v] = (real)a.beta[(size_t)v * a.npos + p0 + tp];
     }
-    int j[NRU];
+    int jj[NRU];
+#pragma unroll
+    for (int u = 0; u < NRU; ++u) jj[u] = bmap[r0 + min(sb_r + 64 * u, nr - 1)];
+    unsigned am = 0;
+#pragma unroll
+    for (int u = 0; u < KC / 32; ++u) {
+      st.a[u] = c1[(size_t)slot_u[u] * a.nband + nn];
+      am |= (nok && sa_p + 32 * u < np) ? (1u << u) : 0u;
+    }
+    st.amask = am;
+    // row groups u with no row in the chunk (uniform test) issue no loads;
+    // their registers are masked, never written with a fill
+    unsigned bm = 0;
 #pragma unroll
     for (int u = 0; u < NRU; ++u) {
-      const int rr = sb_r + 64 * u;
-      const int jj = (rr < nr) ? bmap[r0 + rr] : -1;
-      j[u] = (jj >= 0 && jj < lim2) ? jj : -1;
+      {
+        const bool ok = sb_r + 64 * u < nr && jj[u] >= 0 && jj[u] < lim2;
+        const int row = ok ? jj[u] : 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool eok = sb_m + e < vb.ncols;
+          st.b[u][e] = c2[(size_t)row * a.nband + (eok ? sb_m + e : 0)];
+          bm |= (ok && eok) ? (1u << (4 * u + e)) : 0u;
+        }
+      }
     }
-#pragma unroll
-    for (int u = 0; u < NRU; ++u)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        st.b[u][e] = (j[u] >= 0 && sb_m + e < vb.ncols) ? c2[(size_t)j[u] * a.nband + sb_m + e] : P::make(0, 0);
+    st.bmask = bm;
   };
-  auto store = [&](int ch) {
-    const int nr = crow[ch + 1] - crow[ch];
+  // zero fill per component (a select of whole vector values keeps the
+  // staging struct in scratch memory for double2)
+  auto keep = [](bool ok, cplx z) { return P::make(ok ? z.x : (real)0, ok ? z.y : (real)0); };
+  auto store = [&]() {
 #pragma unroll
-    for (int u = 0; u < KC / 32; ++u) sA[(sa_p + 32 * u) * 8 + sa_n] = st.a[u];
-    if (tid < KC) sP[tid] = st.prow;
+    for (int u = 0; u < KC / 32; ++u) sA[(sa_p + 32 * u) * 8 + sa_n] = keep((st.amask >> u) & 1u, st.a[u]);
+    const bool pok = tid < st.np;
+    if (tid < KC) sP[tid] = pok ? st.prow : 0;
     if (KIND >= 1 && tid < KC) {
 #pragma unroll
-      for (int v = 0; v < NV - 1; ++v) sBeta[v * KC + tid] = st.beta[v];
+      for (int v = 0; v < NV - 1; ++v) sBeta[v * KC + tid] = pok ? st.beta[v] : (real)0;
     }
 #pragma unroll
     for (int u = 0; u < NRU; ++u) {
       const int rr = sb_r + 64 * u;
-      if (rr < nr) {
+      if (rr < st.nr) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          if (sb_m + e < a.ms) sB[rr * a.ms + sb_m + e] = st.b[u][e];
+          if (sb_m + e < a.ms) sB[rr * a.ms + sb_m + e] = keep((st.bmask >> (4 * u + e)) & 1u, st.b[u][e]);
       }
     }
   };
@@ -219,73 +249,153 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
   for (int ch = 0; ch < a.nchunk; ++ch) {
     if (!PIPE) load(ch);
     __syncthreads();  // the previous chunk's reads are done
-    store(ch);
+    store();
     __syncthreads();
-    if (PIPE && ch + 1 < a.nchunk) load(ch + 1);  // in flight during this chunk's products
+    // in flight during this chunk's products (after the last chunk a reload
+    // of it, never stored: no branch around the loads)
+    if (PIPE) load(min(ch + 1, a.nchunk - 1));
     const int nsteps = (a.chunk[ch + 1] - a.chunk[ch] + 3) >> 2;
-    // the position's A value, beta and B row are read one step ahead
-    int prow_c = sP[kg];
-    cplx za_c = sA[kg * 8 + band_r];
-    real bt_c[2] = {(real)0, (real)0};
-#pragma unroll
-    for (int v = 0; v < NV - 1; ++v) bt_c[v] = sBeta[v * KC + kg];
-    for (int t = 0; t < nsteps; ++t) {
-      const int pn = min(4 * (t + 1) + kg, KC - 1);
-      const int prow_n = sP[pn];
-      const cplx za_n = sA[pn * 8 + band_r];
-      real bt_n[2] = {(real)0, (real)0};
-#pragma unroll
-      for (int v = 0; v < NV - 1; ++v) bt_n[v] = sBeta[v * KC + pn];
-      const cplx *brow = sB + prow_c * a.ms + lane_off;
-      real are[NV], aim[NV];
-      are[0] = im_row ? -za_c.y : za_c.x;
-      aim[0] = im_row ? za_c.x : za_c.y;
+    // A variants of the position's C1 value: real embedding row, scaled by beta
+    auto avals = [&](cplx z, const real *bt, real *are, real *aim) {
+      are[0] = im_row ? -z.y : z.x;
+      aim[0] = im_row ? z.x : z.y;
 #pragma unroll
       for (int v = 1; v < NV; ++v) {
-        are[v] = bt_c[v - 1] * are[0];
-        aim[v] = bt_c[v - 1] * aim[0];
+        are[v] = bt[v - 1] * are[0];
+        aim[v] = bt[v - 1] * aim[0];
       }
-      // every wave runs TPW tiles; tiles past nt read row group 0 and their
-      // results are dropped (no branch in the loop, so the LDS reads of all
-      // tiles are issued ahead of the products)
+    };
+    // every wave runs TPW tiles; tiles past nt read row group 0 and their
+    // results are dropped (no branch in the loop, so the LDS reads of all
+    // tiles are issued ahead of the products)
+    if constexpr (SU == 2) {
+      // two steps per pass: the LDS reads of both steps' A values and B rows
+      // are issued together, so one latency chain feeds 2 x TPW x NV x 2
+      // products.  An odd last step reads positions past the chunk, whose A
+      // values are zero (at most position 4 * 31 + 3 < KC).
+      for (int t = 0; t < nsteps; t += 2) {
+        int prw[2];
+        cplx za[2];
+        real bt[2][2] = {{(real)0, (real)0}, {(real)0, (real)0}};
 #pragma unroll
-      for (int tt = 0; tt < TPW; ++tt) {
-        const cplx zb = brow[toff[tt]];
+        for (int h = 0; h < 2; ++h) {
+          const int p = 4 * (t + h) + kg;
+          prw[h] = sP[p];
+          za[h] = sA[p * 8 + band_r];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          acc[v][tt] = P::mma(are[v], zb.x, acc[v][tt]);
-          acc[v][tt] = P::mma(aim[v], zb.y, acc[v][tt]);
+          for (int v = 0; v < NV - 1; ++v) bt[h][v] = sBeta[v * KC + p];
+        }
+        cplx zb[2][TPW];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const cplx *brow = sB + prw[h] * a.ms + lane_off;
+#pragma unroll
+          for (int tt = 0; tt < TPW; ++tt) zb[h][tt] = brow[toff[tt]];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          real are[NV], aim[NV];
+          avals(za[h], bt[h], are, aim);
+#pragma unroll
+          for (int tt = 0; tt < TPW; ++tt) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+              acc[v][tt] = P::mma(are[v], zb[h][tt].x, acc[v][tt]);
+              acc[v][tt] = P::mma(aim[v], zb[h][tt].y, acc[v][tt]);
+            }
+          }
         }
       }
-      prow_c = prow_n;
-      za_c = za_n;
+    } else {
+      // one step per pass; the position's A value, beta and B row are read
+      // one step ahead
+      int prow_c = sP[kg];
+      cplx za_c = sA[kg * 8 + band_r];
+      real bt_c[2] = {(real)0, (real)0};
 #pragma unroll
-      for (int v = 0; v < NV - 1; ++v) bt_c[v] = bt_n[v];
+      for (int v = 0; v < NV - 1; ++v) bt_c[v] = sBeta[v * KC + kg];
+      for (int t = 0; t < nsteps; ++t) {
+        const int pn = min(4 * (t + 1) + kg, KC - 1);
+        const int prow_n = sP[pn];
+        const cplx za_n = sA[pn * 8 + band_r];
+        real bt_n[2] = {(real)0, (real)0};
+#pragma unroll
+        for (int v = 0; v < NV - 1; ++v) bt_n[v] = sBeta[v * KC + pn];
+        const cplx *brow = sB + prow_c * a.ms + lane_off;
+        real are[NV], aim[NV];
+        avals(za_c, bt_c, are, aim);
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+          const cplx zb = brow[toff[tt]];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            acc[v][tt] = P::mma(are[v], zb.x, acc[v][tt]);
+            acc[v][tt] = P::mma(aim[v], zb.y, acc[v][tt]);
+          }
+        }
+        prow_c = prow_n;
+        za_c = za_n;
+#pragma unroll
+        for (int v = 0; v < NV - 1; ++v) bt_c[v] = bt_n[v];
+      }
     }
   }
 
-  // results: tile (sg, mg), lane column r -> shift 4 sg + (r & 3), band m 4 mg + (r >> 2)
+  // results: tile (sg, mg), lane column r -> shift 4 sg + (r & 3), band m 4 mg + (r >> 2).
+  // The output slots (perm lookups) of all tiles are loaded first, from
+  // clamped addresses, so that the lookups are in flight together.
   const PairBlock &pbk = a.pairs.b[vb.pb];
   const int nm = pbk.m_hi - pbk.m_lo;
   cplx *rho = (cplx *)a.rho, *rho2 = (cplx *)a.rho2;
   const int *shift = a.shift + (size_t)cl * kColMaxShift;
   const double *kv = a.ktot + 3 * (size_t)k;
+  // (slot indices fit 31 bits: perm holds them as int, and the vertex buffer
+  // of 2^31 transitions would exceed device memory)
+  int out_g[TPW];
+  int out_pos[TPW][2];
+  int sgv[TPW];
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt) {
     const int T = wave + 4 * tt;
-    if (T >= nt) continue;
-    const int sg = info[2 + T / nmg];
-    const int iG = shift[4 * sg + (r & 3)];
-    const int mo = 4 * (T % nmg) + (r >> 2);
-    if (iG < 0 || mo >= vb.ncols) continue;
+    sgv[tt] = info[2 + (T < nt ? T : 0) / nmg];
+  }
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) out_g[tt] = shift[4 * sgv[tt] + (r & 3)];
+  long long idx[TPW][2];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int T = wave + 4 * tt;
+    const int mo = 4 * ((T < nt ? T : 0) % nmg) + (r >> 2);
+    const bool ok = T < nt && out_g[tt] >= 0 && mo < vb.ncols;
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       const int onr = P::out_band(kg, p);
-      if (onr >= vb.nrows) continue;
-      const int n = vb.n0 + onr, m = vb.m0 + mo;
-      const long long tt_idx = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
-                               (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
-      const long long pos = a.perm ? (long long)a.perm[tt_idx] : tt_idx;
+      const bool v = ok && onr < vb.nrows;
+      idx[tt][p] = v ? (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] + (long long)onr * nm + mo +
+                           (long long)(vb.n0 - pbk.n_lo) * nm + (vb.m0 - pbk.m_lo)
+                     : -1;
+    }
+  }
+  if (a.perm) {
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int pv = a.perm[idx[tt][p] >= 0 ? idx[tt][p] : 0];
+        out_pos[tt][p] = idx[tt][p] >= 0 ? pv : -1;
+      }
+  } else {
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) out_pos[tt][p] = (int)idx[tt][p];
+  }
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int iG = out_g[tt];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const long long pos = out_pos[tt][p];
       if (pos < 0) continue;
       const real xr = acc[0][tt][P::out_re(p)], xi = acc[0][tt][P::out_im(p)];
       if (KIND == 0) {
