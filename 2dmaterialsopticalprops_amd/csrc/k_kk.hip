@@ -102,11 +102,16 @@ __global__ void k_kstencil(int program, int no, int nw, double dw, double *K) {
   K[idx] = coef;
 }
 
-// C[M][N] = A[M][K] B[K][N], row-major, FP64 matrix cores, 64x64 tiles.
+// C[M][N] = A[M][K] B[K][N], row-major, FP64 matrix cores, 64x64 tiles,
+// K in chunks of 64.  The next chunk's operands are loaded into registers
+// (from clamped addresses, out-of-range values zeroed when stored to LDS)
+// while the current chunk runs on the matrix cores.
+constexpr int kGemmKC = 64;
 __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const double *__restrict__ A,
                                                    const double *__restrict__ B, double *__restrict__ C) {
-  __shared__ double sA[64][17];
-  __shared__ double sB[16][65];
+  constexpr int KC = kGemmKC, U = 64 * KC / 256;
+  __shared__ double sA[64][KC + 1];
+  __shared__ double sB[KC][64 + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r = lane & 15, q4 = lane >> 4;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
@@ -116,19 +121,32 @@ __global__ __launch_bounds__(256) void k_gemm_f64(int M, int N, int K, const dou
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = rpa_d4{0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    __syncthreads();
-    for (int x = tid; x < 64 * 16; x += 256) {
-      int i = x >> 4, k = x & 15;
-      int gm = m0 + i, gk = k0 + k;
-      sA[i][k] = (gm < M && gk < K) ? A[(size_t)gm * K + gk] : 0.0;
-      int kb = x >> 6, j = x & 63;
-      int gk2 = k0 + kb, gn = n0 + j;
-      sB[kb][j] = (gk2 < K && gn < N) ? B[(size_t)gk2 * N + gn] : 0.0;
-    }
-    __syncthreads();
+  // thread roles: A element (x >> 6 row, x & 63 k), B element (x >> 6 k, x & 63 column), x = tid + 256 u
+  double ra[U], rb[U];
+  auto load = [&](int k0) {
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
+    for (int u = 0; u < U; ++u) {
+      const int x = tid + 256 * u, hi = x >> 6, lo = x & 63;
+      ra[u] = A[(size_t)min(m0 + hi, M - 1) * K + min(k0 + lo, K - 1)];
+      rb[u] = B[(size_t)min(k0 + hi, K - 1) * N + min(n0 + lo, N - 1)];
+    }
+  };
+  auto store = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = tid + 256 * u, hi = x >> 6, lo = x & 63;
+      sA[hi][lo] = (m0 + hi < M && k0 + lo < K) ? ra[u] : 0.0;
+      sB[hi][lo] = (k0 + hi < K && n0 + lo < N) ? rb[u] : 0.0;
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += KC) {
+    __syncthreads();  // the previous chunk's reads are done
+    store(k0);
+    __syncthreads();
+    load(min(k0 + KC, K - 1));  // past the end: a clamped reload, never stored
+#pragma unroll
+    for (int kk = 0; kk < KC / 4; ++kk) {
       const int k = kk * 4 + q4;
       double av[2], bv[2];
 #pragma unroll

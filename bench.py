@@ -4,7 +4,10 @@
 One step = one pass of the hot path for one q over all frequencies of the
 workload: transition bookkeeping, vertices, S0 accumulation, Kramers-Kronig,
 Coulomb, Dyson/W and the loss epilogue (Sloss semantics), inputs resident in
-HBM.  value = (no-1) output frequencies per step / seconds per step.
+HBM, run as one replay of the captured chain (every kernel re-runs; the
+replay's result is checked bitwise against an eager run).
+value = (no-1) output frequencies per step / seconds per step.  The q-loop
+workloads run eagerly (each q changes shapes).
 
 Default workload (production size): 64x64x1 k-grid (4096 k), 20 bands
 (8 valence / 12 conduction), ~1500 plane waves per k, 51 local fields,
@@ -109,28 +112,37 @@ def hbm_calibration(mib):
 
 
 def time_chain(r, steps, warmup):
-    """Warmup, then `steps` eager runs of the whole chain with stage events on
-    the context's stream, then the same number of graph replays."""
+    """Warmup, then `steps` graph replays of the captured chain (the timed
+    steps), `steps` eager runs without stage events, and `steps` eager runs
+    with stage events for the per-stage breakdown.  The graph replays must
+    reproduce the eager result bit for bit."""
     host = importlib.import_module("2dmaterialsopticalprops_amd.host")
     ctx = r.ctx
+    out_name = "LOSS" if r.rc.program == "sloss" else "PI11"
     for _ in range(max(warmup, 1)):
         ctx.run()
-    ctx.set_profiling(True)
-    stages = {k: 0.0 for k in host._abi.STAGES}
-    dev_ms = 0.0
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        dev_ms += ctx.run()
-        for k, v in ctx.stage_ms().items():
-            stages[k] += v
-    wall = time.perf_counter() - t0
-    ctx.set_profiling(False)
+    eager_out = ctx.result(out_name)
     ctx.run(graph=True)
+    for _ in range(max(warmup, 1) - 1):
+        ctx.run(graph=True)
     tg = time.perf_counter()
     for _ in range(steps):
         ctx.run(graph=True)
     gwall = (time.perf_counter() - tg) * 1e3 / steps
-    return {"ms_step": wall * 1e3 / steps, "device_ms": dev_ms / steps, "graph_ms": gwall,
+    graph_same = bool(np.array_equal(ctx.result(out_name), eager_out))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.set_profiling(True)
+    stages = {k: 0.0 for k in host._abi.STAGES}
+    dev_ms = 0.0
+    for _ in range(steps):
+        dev_ms += ctx.run()
+        for k, v in ctx.stage_ms().items():
+            stages[k] += v
+    ctx.set_profiling(False)
+    return {"ms_step": gwall, "eager_ms": wall, "device_ms": dev_ms / steps, "graph_same": graph_same,
             "stages": {k: v / steps for k, v in stages.items()}}
 
 
@@ -235,8 +247,9 @@ def work_model(r, host, cfg, K1, K2, nlf, ntr):
 
 
 def time_q_loop(r, plans, steps, warmup):
-    """Warmup passes, then `steps` timed passes of the q loop: per q the
-    context takes the q point (local fields, k+q map) and runs the chain."""
+    """Warmup passes, then `steps` timed passes of the q loop (per q the
+    context takes the q point -- local fields, k+q map -- and runs the
+    chain), then `steps` passes with stage events for the breakdown."""
     from importlib import import_module
     host = import_module("2dmaterialsopticalprops_amd.host")
     ctx = r.ctx
@@ -244,17 +257,21 @@ def time_q_loop(r, plans, steps, warmup):
         for pt in plans:
             r.set_q_point(pt)
             ctx.run()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for pt in plans:
+            r.set_q_point(pt)
+            ctx.run()
+    wall = time.perf_counter() - t0
     ctx.set_profiling(True)
     stages = {k: 0.0 for k in host._abi.STAGES}
     dev_ms = 0.0
-    t0 = time.perf_counter()
     for _ in range(steps):
         for pt in plans:
             r.set_q_point(pt)
             dev_ms += ctx.run()
             for k, v in ctx.stage_ms().items():
                 stages[k] += v
-    wall = time.perf_counter() - t0
     ctx.set_profiling(False)
     return {"ms_step": wall * 1e3 / steps, "device_ms": dev_ms / steps,
             "stages": {k: v / steps for k, v in stages.items()}}
@@ -418,8 +435,9 @@ def main():
         log(f"[bench] {prec}: setup+upload {time.perf_counter() - t0:.1f}s nlf={r.nlf} no={cfg.no} "
             f"ntrans={r.ntransitions()}")
         res = time_chain(r, args.steps, args.warmup)
-        log(f"[bench] {prec}: {res['ms_step']:.3f} ms/step (device {res['device_ms']:.3f} ms), graph "
-            f"{res['graph_ms']:.3f} ms/step; stages {json.dumps({k: round(v, 4) for k, v in res['stages'].items()})}")
+        log(f"[bench] {prec}: {res['ms_step']:.3f} ms/step as graph replays (eager {res['eager_ms']:.3f} ms, "
+            f"device {res['device_ms']:.3f} ms with stage events, graph = eager bitwise: {res['graph_same']}); "
+            f"stages {json.dumps({k: round(v, 4) for k, v in res['stages'].items()})}")
         runs[prec], runners[prec] = res, r
     head = precs[0]
     r = runners[head]
@@ -447,7 +465,9 @@ def main():
                         f"N_G={r.nlf}, {cfg.no} omega 0-{cfg.omax_ev:g} eV, eta={cfg.eta_ev * 1e3:g} meV, "
                         f"q=k-grid step, vertex map {args.vertex_map}",
             "precision": head, "omega_points_per_step": nfreq, "transitions": ntr,
-            "graph_ms_per_step": round(runs[head]["graph_ms"], 4),
+            "timing": "graph replays of the captured chain (rpa_run_graph), one per step",
+            "eager_ms_per_step": round(runs[head]["eager_ms"], 4),
+            "graph_matches_eager_bitwise": runs[head]["graph_same"],
             "device_ms_per_step": round(runs[head]["device_ms"], 4),
             "stage_ms": {k: round(v, 4) for k, v in runs[head]["stages"].items()},
         },
