@@ -220,18 +220,41 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
   double2 acc[kDyRows];
   auto ic = [&](int u) { return min(wave + 4 * u, N - 1); };
 
-  // eps = I - chi0 V
+  // eps = I - chi0 V.  chi0 of this frequency is first copied to LDS in
+  // bursts of four loads per thread, then read as wave-uniform broadcasts;
+  // V is read four k ahead.  (Read row by row from L2 inside the k loop, the
+  // loads of each k waited a full latency: 70 of the kernel's 290 us.)
+  {
+    const int nn = N * N;
+    for (int x0 = 0; x0 < nn; x0 += 1024) {
+      double2 t[4];
 #pragma unroll
-  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
-  for (int k = 0; k < N; ++k) {
-    const double v = V[k * N + jc];
+      for (int u = 0; u < 4; ++u) t[u] = X0[min(x0 + tid + 256 * u, nn - 1)];
 #pragma unroll
-    for (int u = 0; u < kDyRows; ++u) {
-      const double2 c = X0[ic(u) * N + k];
-      acc[u].x += c.x * v;
-      acc[u].y += c.y * v;
+      for (int u = 0; u < 4; ++u)
+        if (x0 + tid + 256 * u < nn) sa[x0 + tid + 256 * u] = t[u];
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k0 = 0; k0 < N; k0 += 4) {
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = V[min(k0 + q, N - 1) * N + jc];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k0 + q < N) {  // uniform
+#pragma unroll
+        for (int u = 0; u < kDyRows; ++u) {
+          const double2 c = sa[ic(u) * N + k0 + q];
+          acc[u].x += c.x * v[q];
+          acc[u].y += c.y * v[q];
+        }
+      }
+    }
+  }
+  __syncthreads();  // every wave has read chi0
   if (jok) {
 #pragma unroll
     for (int u = 0; u < kDyRows; ++u) {
@@ -245,18 +268,14 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
   for (int k = 0; k < N; ++k) {
     if (wave == 0) {
       // pivot: largest |Re| + |Im| in column k at or below row k, lowest row on ties
+      // (the maximum by a butterfly, then the lowest lane holding it by a ballot:
+      // lane j holds row j)
       const double2 z = sa[jc * N + k];
-      double best = (j >= k && jok) ? fabs(z.x) + fabs(z.y) : -1.0;
-      int bi = (j >= k && jok) ? j : N;
+      const double mine = (j >= k && jok) ? fabs(z.x) + fabs(z.y) : -1.0;
+      double best = mine;
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
-        }
-      }
+      for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off));
+      const int bi = __ffsll((unsigned long long)__ballot(mine == best)) - 1;
       if (j == 0) {
         int p = bi;
         if (!(best > 0.0)) {  // singular: flag it and go on without a swap
@@ -285,7 +304,9 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
       sa[k * N + j] = (j == k) ? inv : cmul(rp, inv);
       if (p != k && j != k) sa[p * N + j] = rk;
     } else if (wave == 1 && jok && j != k) {
-      fcol[j] = (j == p) ? okk : ck;
+      // per component: a select of whole double2 values went through scratch memory
+      const bool sw = j == p;
+      fcol[j] = make_double2(sw ? okk.x : ck.x, sw ? okk.y : ck.y);
       sa[j * N + k] = make_double2(0.0, 0.0);
     }
     __syncthreads();
@@ -341,16 +362,23 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
   }
   __syncthreads();
   if (tid == 0) a.loss[f] = -sa[0].y;
-  // chi = eps^-1 chi0
+  // chi = eps^-1 chi0 (chi0 rows read four k ahead)
 #pragma unroll
   for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
-  for (int k = 0; k < N; ++k) {
-    const double2 c = X0[k * N + jc];
+  for (int k0 = 0; k0 < N; k0 += 4) {
+    double2 c[4];
 #pragma unroll
-    for (int u = 0; u < kDyRows; ++u) {
-      const double2 e = sa[ic(u) * N + k];
-      acc[u].x += e.x * c.x - e.y * c.y;
-      acc[u].y += e.x * c.y + e.y * c.x;
+    for (int q = 0; q < 4; ++q) c[q] = X0[min(k0 + q, N - 1) * N + jc];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k0 + q < N) {  // uniform
+#pragma unroll
+        for (int u = 0; u < kDyRows; ++u) {
+          const double2 e = sa[ic(u) * N + k0 + q];
+          acc[u].x += e.x * c[q].x - e.y * c[q].y;
+          acc[u].y += e.x * c[q].y + e.y * c[q].x;
+        }
+      }
     }
   }
   if (jok) {
@@ -360,16 +388,25 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
       if (i < N) Ch[i * N + j] = acc[u];
     }
   }
-  // W = V eps^-1 (= V + V chi V)
+  // W = V eps^-1 (= V + V chi V); the V elements of two k are read together
 #pragma unroll
   for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
-  for (int k = 0; k < N; ++k) {
-    const double2 e = sa[k * N + jc];
+  for (int k0 = 0; k0 < N; k0 += 2) {
+    double vv[2][kDyRows];
 #pragma unroll
-    for (int u = 0; u < kDyRows; ++u) {
-      const double v = V[ic(u) * N + k];
-      acc[u].x += v * e.x;
-      acc[u].y += v * e.y;
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int u = 0; u < kDyRows; ++u) vv[q][u] = V[ic(u) * N + min(k0 + q, N - 1)];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (k0 + q < N) {  // uniform
+        const double2 e = sa[(k0 + q) * N + jc];
+#pragma unroll
+        for (int u = 0; u < kDyRows; ++u) {
+          acc[u].x += vv[q][u] * e.x;
+          acc[u].y += vv[q][u] * e.y;
+        }
+      }
     }
   }
   if (jok) {
