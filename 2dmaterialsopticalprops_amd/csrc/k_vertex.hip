@@ -522,7 +522,7 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
 constexpr int kGdCH = 32;           // plane waves per chunk
 constexpr int kGdNT = kGdCH / 4;    // K groups of four per chunk
 
-template <class P, int KIND, int MAXB, bool DUAL>
+template <class P, int KIND, int MAXB, bool DUAL, bool PAIR>
 __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   using real = typename P::real;
   using cplx = typename P::cplx;
@@ -542,8 +542,17 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   const int k = xcd + 8 * (slot / a.groups);
   const int grp = slot % a.groups;
   if (k >= a.ntot) return;  // whole workgroup leaves together
-  const int iG = grp * 4 + wave;
-  const bool active = iG < a.nlf;
+  // PAIR (every band block at most 8 bands m wide): a wave computes two
+  // local fields, matrix-core columns 0-7 the first and 8-15 the second, so
+  // the columns an 8-wide block leaves empty carry the second field; its
+  // rows are resolved by lanes 32-63
+  constexpr int GPW = PAIR ? 2 : 1;
+  const int iG0 = (grp * 4 + wave) * GPW;  // first local field of the wave
+  const bool active = iG0 < a.nlf;
+  const int hr = PAIR ? (lane >> 5) : 0;   // field of this resolution lane
+  const int lr = PAIR ? (lane & 31) : lane;  // its plane wave in the chunk
+  const int iGr = iG0 + hr;
+  const bool res_ok = active && iGr < a.nlf && (PAIR || lane < CH);
 
   const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
   int d0, d1, d2;
@@ -557,11 +566,13 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   const cplx *c1 = evc + (size_t)K1 * a.npwx * a.nband + a.na0;
   const cplx *c2 = evc + (size_t)K2 * a.npwx * a.nband + a.mb0;
   const double *kv = a.ktot + 3 * k;
-  const int *dmap = (direct && active) ? a.dmap + ((size_t)a.g0idx[k] * a.nlf + iG) * a.ngd : nullptr;
+  const int *dmap = direct ? a.dmap + ((size_t)a.g0idx[k] * a.nlf + min(iGr, a.nlf - 1)) * a.ngd : nullptr;
   int lf0 = 0, lf1 = 0, lf2 = 0;
-  if (!direct && active) mill_unpack(a.lf_key[iG], lf0, lf1, lf2);
+  if (!direct && iGr < a.nlf) mill_unpack(a.lf_key[iGr], lf0, lf1, lf2);
 
   const int r = lane & 15, kg = lane >> 4;
+  const int hc = PAIR ? (r >> 3) : 0;  // field of this matrix-core column
+  const int mc = PAIR ? (r & 7) : r;   // its band m in the block
   // zero rows of both buffers
   for (int x = tid; x < 2 * 4 * NT; x += 256) A_at(x / (4 * NT), (x / NT) & 3, nna)[x % NT] = P::make(0, 0);
 
@@ -573,7 +584,7 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
     for (int b = 0; b < MAXB; ++b) {
       const VBlock vb = a.vb[b < a.nvb ? b : 0];
       arow[b] = (nr < vb.nrows) ? vb.n0 - a.na0 + nr : nna;
-      bcol[b] = (r < vb.ncols) ? vb.m0 - a.mb0 + r : -1;
+      bcol[b] = (mc < vb.ncols) ? vb.m0 - a.mb0 + mc : -1;
     }
   }
   const bool im_row = P::arow_im(r);
@@ -599,9 +610,10 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   };
 
   // ---- gathered rows: lane l < 32 resolves plane wave i0 + l of its G
+  //      (PAIR: lanes 32-63 those of the second field)
   constexpr int kNone = -2147483647 - 1;
   auto load_key = [&](int i) -> int {
-    return (active && lane < CH && i < n1) ? (direct ? dmap[i] : src[i]) : kNone;
+    return (res_ok && i < n1) ? (direct ? dmap[i] : src[i]) : kNone;
   };
   auto lookup_igk = [&](int key) -> int {
     if (key == kNone) return -1;
@@ -611,13 +623,13 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
     return bi >= 0 ? (int)box2[bi] : -1;
   };
   auto row_of = [&](int jraw) -> int { return (jraw >= 0 && jraw < lim2) ? jraw : -1; };
-  auto load_mkey = [&](int i) -> int { return (KIND >= 1 && active && lane < CH && i < n1) ? src[i] : 0; };
+  auto load_mkey = [&](int i) -> int { return (KIND >= 1 && res_ok && i < n1) ? src[i] : 0; };
   // J of plane wave i for current vertices (pol p), lanes < 32
   auto jweight = [&](int mkey, int i, int pol) -> real {
-    if (!(active && lane < CH && i < n1)) return 0;
+    if (!(res_ok && i < n1)) return 0;
     int s0, s1, s2;
     mill_unpack(mkey, s0, s1, s2);
-    const double jb = (a.q[pol] + 2.0 * kv[pol]) + a.glf[3 * iG + pol];
+    const double jb = (a.q[pol] + 2.0 * kv[pol]) + a.glf[3 * iGr + pol];
     const double g = ((0.0 + a.kc[pol + 0] * (double)s0) + a.kc[pol + 3] * (double)s1) + a.kc[pol + 6] * (double)s2;
     return (real)(0.5 * ((jb + 2.0 * g) * a.gcar));
   };
@@ -628,7 +640,7 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   auto load_b = [&](int jrow) {  // jrow: resolved row of plane wave i0 + lane
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const int j = __shfl(jrow, 4 * t + kg);
+      const int j = __shfl(jrow, 32 * hc + 4 * t + kg);
 #pragma unroll
       for (int b = 0; b < MAXB; ++b)
         bz[t][b] = (b < a.nvb && j >= 0 && bcol[b] >= 0) ? c2[(size_t)j * a.nband + bcol[b]] : P::make(0, 0);
@@ -646,18 +658,18 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   load_a(0);
   store_a(0);
   {
-    const int key0 = load_key(lane);
+    const int key0 = load_key(lr);
     const int j0 = row_of(direct ? key0 : lookup_igk(key0));
     if (KIND >= 1) {
-      const int mk0 = load_mkey(lane);
-      jw1 = jweight(mk0, lane, a.pol1);
-      if (KIND == 2) jw2 = jweight(mk0, lane, a.pol2);
+      const int mk0 = load_mkey(lr);
+      jw1 = jweight(mk0, lr, a.pol1);
+      if (KIND == 2) jw2 = jweight(mk0, lr, a.pol2);
     }
     if (active) load_b(j0);
-    const int key1 = load_key(CH + lane);
+    const int key1 = load_key(CH + lr);
     jn = direct ? key1 : lookup_igk(key1);
-    if (KIND >= 1) mkn = load_mkey(CH + lane);
-    if (!direct) kn = load_key(2 * CH + lane);
+    if (KIND >= 1) mkn = load_mkey(CH + lr);
+    if (!direct) kn = load_key(2 * CH + lr);
   }
   if (CH < n1) load_a(CH);
 
@@ -678,10 +690,10 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
     if (active && i0 + CH < n1) {
       load_b(row_of(jn));
       if (KIND >= 1) {
-        jw1 = jweight(mkn, i0 + CH + lane, a.pol1);
-        if (KIND == 2) jw2 = jweight(mkn, i0 + CH + lane, a.pol2);
+        jw1 = jweight(mkn, i0 + CH + lr, a.pol1);
+        if (KIND == 2) jw2 = jweight(mkn, i0 + CH + lr, a.pol2);
       }
-      const int i2 = i0 + 2 * CH + lane;
+      const int i2 = i0 + 2 * CH + lr;
       if (direct) {
         jn = load_key(i2);
       } else {
@@ -708,11 +720,11 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
               acc_t *x1 = (DUAL && h) ? bcc1 : acc1;
               acc_t *x2 = (DUAL && h) ? bcc2 : acc2;
               if (KIND >= 1) {
-                const real wa = __shfl(w1, 4 * (t + h) + kg);
+                const real wa = __shfl(w1, 32 * hc + 4 * (t + h) + kg);
                 x1[b] = P::mma(are, z2.x * wa, x1[b]);
                 x1[b] = P::mma(aim, z2.y * wa, x1[b]);
                 if (KIND == 2) {
-                  const real wb = __shfl(w2, 4 * (t + h) + kg);
+                  const real wb = __shfl(w2, 32 * hc + 4 * (t + h) + kg);
                   x2[b] = P::mma(are, z2.x * wb, x2[b]);
                   x2[b] = P::mma(aim, z2.y * wb, x2[b]);
                 }
@@ -735,6 +747,8 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
     }
   }
   cplx *rho1 = (cplx *)a.rho1, *rho2 = (cplx *)a.rho2;
+  const int iG = iG0 + hc;
+  if (iG >= a.nlf) return;
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
     if (b < a.nvb) {
@@ -744,8 +758,8 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
         const int onr = P::out_band(kg, p);
-        if (onr < vb.nrows && r < vb.ncols) {
-          const int n = vb.n0 + onr, m = vb.m0 + r;
+        if (onr < vb.nrows && mc < vb.ncols) {
+          const int n = vb.n0 + onr, m = vb.m0 + mc;
           const long long tt = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
                                (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
           const long long pos = a.perm ? (long long)a.perm[tt] : tt;
@@ -1111,21 +1125,31 @@ int configure_vertex(rpa_ctx *c) {
 }
 
 template <class P, int KIND>
-static void launch_gd(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
-  if (a.nvb <= 1)
-    hipLaunchKernelGGL((k_vertex_gd<P, KIND, 1, true>), grid, dim3(256), lds, c->stream, a);
-  else
-    hipLaunchKernelGGL((k_vertex_gd<P, KIND, 2, true>), grid, dim3(256), lds, c->stream, a);
+static void launch_gd(rpa_ctx *c, VertexArgs a, size_t lds) {
+  bool pair = true;  // every band block at most 8 bands m wide
+  for (int b = 0; b < a.nvb; ++b) pair = pair && a.vb[b].ncols <= 8;
+  a.groups = (a.nlf + (pair ? 7 : 3)) / (pair ? 8 : 4);
+  const dim3 grid((unsigned)((long long)a.kslots * a.groups));
+  if (pair) {
+    if (a.nvb <= 1)
+      hipLaunchKernelGGL((k_vertex_gd<P, KIND, 1, true, true>), grid, dim3(256), lds, c->stream, a);
+    else
+      hipLaunchKernelGGL((k_vertex_gd<P, KIND, 2, true, true>), grid, dim3(256), lds, c->stream, a);
+  } else if (a.nvb <= 1) {
+    hipLaunchKernelGGL((k_vertex_gd<P, KIND, 1, true, false>), grid, dim3(256), lds, c->stream, a);
+  } else {
+    hipLaunchKernelGGL((k_vertex_gd<P, KIND, 2, true, false>), grid, dim3(256), lds, c->stream, a);
+  }
 }
 
 template <class P>
-static void launch_prec_gd(rpa_ctx *c, int kind, const VertexArgs &a, dim3 grid, size_t lds) {
+static void launch_prec_gd(rpa_ctx *c, int kind, const VertexArgs &a, size_t lds) {
   if (kind == 0)
-    launch_gd<P, 0>(c, a, grid, lds);
+    launch_gd<P, 0>(c, a, lds);
   else if (kind == 1)
-    launch_gd<P, 1>(c, a, grid, lds);
+    launch_gd<P, 1>(c, a, lds);
   else
-    launch_gd<P, 2>(c, a, grid, lds);
+    launch_gd<P, 2>(c, a, lds);
 }
 
 template <class P>
@@ -1254,10 +1278,7 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   if (gd) {
     const size_t lds_gd = 2 * 4 * (size_t)(a.nna + 1) * kGdNT * (mixed ? sizeof(float2) : sizeof(double2));
     a.ch = kGdCH;
-    if (mixed)
-      launch_prec_gd<PrecF32>(c, kind, a, grid, lds_gd);
-    else
-      launch_prec_gd<PrecF64>(c, kind, a, grid, lds_gd);
+    launch_prec_gd<PrecF32>(c, kind, a, lds_gd);  // gd is an FP32 form
   } else if (mixed) {
     launch_prec<PrecF32>(c, kind, a, grid, lds);
   } else {

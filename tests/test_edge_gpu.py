@@ -195,3 +195,50 @@ def test_dyson_both_lds_layouts(nlf_target):
     assert H.rel_max(out["epsinv"], E) < TOL
     assert H.rel_max(out["W"], W) < TOL
     r.close()
+
+
+@pytest.mark.parametrize("ncond", [2, 12])
+def test_gather_form_paired_and_wide_blocks(ncond, monkeypatch):
+    """The FP32 gather-direct vertex form with two local fields per wave
+    (band blocks at most 8 bands m wide; odd N_G leaves the last wave half
+    used) and with one field per wave (12 bands m): Sloss charge vertices on
+    local fields in all directions against the oracle, 2e-6 max-norm as for
+    the other FP32 forms."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "gather")
+    s = H.system("tiny", lf="xyz", nlf_target=9, ncond=ncond)
+    r = _runner(s, program="sloss", precision="mixed")
+    nv, nb = s.cfg.nval, s.cfg.nband
+    for iq, nlf in [(2, 9), (4, 14), (6, 5), (3, 1)]:
+        r.set_q_point(r.q_point(iq, nlf))
+        pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+        ref = orc.vertices(pb, 0, s.gcar, 0, nv, nv, nb)
+        got, _ = r.vertices()
+        err = H.rel_max(got.T, ref)
+        print(f"ncond={ncond} iq={iq} N_G={nlf} gather-form vertices rel max err {err:.2e}")
+        assert err < 2e-6
+    r.close()
+
+
+@pytest.mark.parametrize("pol", ["xx", "yz"])
+def test_gather_form_paired_current_vertices(pol, monkeypatch):
+    """Optabs current vertices (J weights shuffled from the lanes of each
+    field) through the paired gather-direct form: both band directions of
+    the tiny system are 2-wide blocks."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "gather")
+    s = H.system("tiny", lf="xyz", nlf_target=9)
+    r = _runner(s, program="optabs", precision="mixed", pol=pol)
+    nv, nb = s.cfg.nval, s.cfg.nband
+    for iq, nlf in [(2, 9), (4, 6)]:
+        r.set_q_point(r.q_point(iq, nlf))
+        pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+        a1, a2 = orc.vertices(pb, host._abi.POL[pol], s.gcar, 0, nv, nv, nb, both=True)
+        r.ctx.vertices()
+        g1, g2 = r.vertices(both=True)
+        nk, tv = len(r.K1), nv * (nb - nv)
+        g1 = g1.T.reshape(nk, 2 * tv, r.nlf)
+        g2 = g2.T.reshape(nk, 2 * tv, r.nlf)
+        e1 = H.rel_max(g1[:, :tv], a1.reshape(nk, tv, r.nlf))
+        e2 = H.rel_max(g2[:, :tv], a2.reshape(nk, tv, r.nlf))
+        print(f"{pol} iq={iq} N_G={nlf} paired gather current vertices rel max err {e1:.2e} {e2:.2e}")
+        assert e1 < 2e-6 and e2 < 2e-6
+    r.close()
