@@ -137,35 +137,41 @@ __global__ __launch_bounds__(256) void k_tile_hist(const int *key, long long ntr
 }
 
 // Exclusive scan of bcnt (key-major) in place by one workgroup; toff[key] =
-// first sorted position of key, toff[nkeys] = total.
+// first sorted position of key, toff[nkeys] = total.  Tiles of 1024
+// consecutive counts (coalesced loads), a shuffle scan per wave and the 16
+// wave totals through LDS; the running total of earlier tiles is carried.
 __global__ __launch_bounds__(1024) void k_tile_scan(int *bcnt, long long n, int nkeys, int nblk,
                                                      int *toff) {
-  __shared__ long long part[1024];
-  const int tid = threadIdx.x;
-  long long seg = (n + 1023) / 1024;
-  long long lo = tid * seg, hi = lo + seg < n ? lo + seg : n;
-  long long s = 0;
-  for (long long i = lo; i < hi; ++i) s += bcnt[i];
-  part[tid] = s;
-  __syncthreads();
-  if (tid == 0) {
-    long long run = 0;
-    for (int i = 0; i < 1024; ++i) {
-      long long v = part[i];
-      part[i] = run;
-      run += v;
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  int carry = 0;  // total of the earlier tiles
+  for (long long base = 0; base < n; base += 1024) {
+    const long long i = base + tid;
+    const int v = (i < n) ? bcnt[i] : 0;
+    int x = v;  // inclusive scan within the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
     }
-    toff[nkeys] = (int)run;
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    int wpre = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const int t = wsum[w];
+      wpre += (w < wv) ? t : 0;
+      total += t;
+    }
+    if (i < n) {
+      const int e = carry + wpre + x - v;
+      bcnt[i] = e;
+      if (i % nblk == 0) toff[i / nblk] = e;  // first block of key i / nblk
+    }
+    carry += total;
+    __syncthreads();  // wsum is rewritten by the next tile
   }
-  __syncthreads();
-  long long run = part[tid];
-  for (long long i = lo; i < hi; ++i) {
-    int v = bcnt[i];
-    bcnt[i] = (int)run;
-    run += v;
-  }
-  __syncthreads();
-  for (int kk = tid; kk < nkeys; kk += 1024) toff[kk] = bcnt[(size_t)kk * nblk];
+  if (tid == 0) toff[nkeys] = carry;
 }
 
 struct ScatterArgs {
