@@ -186,7 +186,8 @@ def vertex_kernel(r, prec, lf_idx=None):
     if r.rc.vertex_map == "direct" and len(m) >= 12 * ncls and not (optabs and ncls > 1):
         return "k_vertex_col<ColF32>" if prec == "mixed" else "k_vertex_col<ColF64>"
     if prec == "mixed" and not optabs:
-        return "k_vertex_gd<PrecF32>"
+        nc = r.cfg.nband - r.cfg.nval  # Sloss band block: 8 valence rows x up to 16 conduction columns
+        return "k_vertex_gd<PrecF32> (two local fields per wave)" if nc <= 8 else "k_vertex_gd<PrecF32>"
     return "k_vertex<PrecF32> (staged)" if prec == "mixed" else "k_vertex<PrecF64> (staged)"
 
 
