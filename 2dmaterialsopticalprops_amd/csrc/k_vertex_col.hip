@@ -699,14 +699,15 @@ int build_col_plan(rpa_ctx *c) {
   if (!e) e = ensure(c, P.d_cls, P.cap_cls, info.size());
   if (!e) e = ensure(c, P.d_beta, P.cap_beta, beta.size());
   if (e) return e;
-  RPA_HIP(c, hipMemcpy(P.d_slot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(P.d_prow, prow.data(), sizeof(int) * prow.size(), hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(P.d_chunk, chunk.data(), sizeof(int) * chunk.size(), hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(P.d_crow, crow.data(), sizeof(int) * crow.size(), hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(P.d_bmap, bmap.data(), sizeof(int) * bmap.size(), hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(P.d_shift, shift.data(), sizeof(int) * shift.size(), hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(P.d_cls, info.data(), sizeof(int) * info.size(), hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(P.d_beta, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice));
+  const Upload up[] = {{P.d_slot, slot.data(), sizeof(int) * slot.size()},
+                       {P.d_prow, prow.data(), sizeof(int) * prow.size()},
+                       {P.d_chunk, chunk.data(), sizeof(int) * chunk.size()},
+                       {P.d_crow, crow.data(), sizeof(int) * crow.size()},
+                       {P.d_bmap, bmap.data(), sizeof(int) * bmap.size()},
+                       {P.d_shift, shift.data(), sizeof(int) * shift.size()},
+                       {P.d_cls, info.data(), sizeof(int) * info.size()},
+                       {P.d_beta, beta.data(), sizeof(double) * beta.size()}};
+  if (int e2 = upload(c, up, 8)) return e2;
   P.npos = ngd;
   P.nchunk = nchunk;
   P.ncls = ncls;

@@ -272,6 +272,29 @@ static int get_sig(rpa_ctx *c, double *sig) {
   return 0;
 }
 
+int upload(rpa_ctx *c, const Upload *parts, int n) {
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total += (parts[i].bytes + 255) / 256 * 256;
+  // the previous batch's copies read the staging buffer until they complete
+  RPA_HIP(c, hipStreamSynchronize(c->stream));
+  if (total > c->cap_stage) {
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    c->h_stage = nullptr;
+    c->cap_stage = 0;
+    const size_t cap = std::max<size_t>(total, 1 << 20);
+    RPA_HIP(c, hipHostMalloc((void **)&c->h_stage, cap, hipHostMallocDefault));
+    c->cap_stage = cap;
+  }
+  size_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    if (parts[i].bytes == 0) continue;
+    std::memcpy(c->h_stage + off, parts[i].src, parts[i].bytes);
+    RPA_HIP(c, hipMemcpyAsync(parts[i].dst, c->h_stage + off, parts[i].bytes, hipMemcpyHostToDevice, c->stream));
+    off += (parts[i].bytes + 255) / 256 * 256;
+  }
+  return 0;
+}
+
 }  // namespace rpa
 
 using namespace rpa;
@@ -335,6 +358,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   delete c;
   return RPA_OK;
 }
@@ -463,9 +487,10 @@ int rpa_set_local_fields(rpa_ctx *c, int nlf, const int *lf_idx, const int *parg
   if (!e) e = ensure(c, c->d_glf, c->cap_glf, 3 * (size_t)nlf);
   if (!e) e = ensure(c, c->d_parg, c->cap_parg, (size_t)nlf);
   if (e) return e;
-  RPA_HIP(c, hipMemcpy(c->d_lf_key, keys.data(), sizeof(int) * nlf, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_glf, glf.data(), sizeof(double) * 3 * nlf, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_parg, parg, sizeof(int) * nlf, hipMemcpyHostToDevice));
+  const Upload up[] = {{c->d_lf_key, keys.data(), sizeof(int) * nlf},
+                       {c->d_glf, glf.data(), sizeof(double) * 3 * nlf},
+                       {c->d_parg, parg, sizeof(int) * nlf}};
+  if (int e2 = upload(c, up, 3)) return e2;
   c->prepared = false;
   invalidate_graph(c);
   return RPA_OK;
@@ -527,12 +552,13 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   if (!e) e = ensure(c, c->d_g0keys, c->cap_g0keys, (size_t)c->ng0);
   if (!e) e = ensure(c, c->d_g0idx, c->cap_g0idx, nt);
   if (e) return e;
-  RPA_HIP(c, hipMemcpy(c->d_k1, K1, sizeof(int) * nt, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_k2, K2, sizeof(int) * nt, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_m0, m0.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_ktot, ktot, sizeof(double) * 3 * nt, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_g0keys, g0keys.data(), sizeof(int) * c->ng0, hipMemcpyHostToDevice));
-  RPA_HIP(c, hipMemcpy(c->d_g0idx, g0idx.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+  const Upload up[] = {{c->d_k1, K1, sizeof(int) * nt},
+                       {c->d_k2, K2, sizeof(int) * nt},
+                       {c->d_m0, m0.data(), sizeof(int) * nt},
+                       {c->d_ktot, ktot, sizeof(double) * 3 * nt},
+                       {c->d_g0keys, g0keys.data(), sizeof(int) * c->ng0},
+                       {c->d_g0idx, g0idx.data(), sizeof(int) * nt}};
+  if (int e2 = upload(c, up, 6)) return e2;
   c->have_q = true;
   c->prepared = false;
   invalidate_graph(c);

@@ -102,6 +102,9 @@ struct rpa_ctx {
   std::string err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // pinned staging of the per-q uploads (q point, local fields, column plan)
+  unsigned char *h_stage = nullptr;
+  size_t cap_stage = 0;
   double last_ms = 0.0;
 
   // G list
@@ -242,6 +245,17 @@ int ensure(rpa_ctx *c, T *&ptr, size_t &cap, size_t n) {
   cap = n;
   return 0;
 }
+
+// One host-to-device upload of a staged batch: each part is copied into the
+// context's pinned staging buffer and sent with an asynchronous copy on the
+// context's stream, ahead of the kernels that read it.  The buffer is reused
+// only after the stream has drained.
+struct Upload {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+int upload(rpa_ctx *c, const Upload *parts, int n);
 
 // kernel launchers (defined in the .hip files)
 int launch_build_kbox(rpa_ctx *c);
