@@ -1279,6 +1279,47 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
     const size_t lds_gd = 2 * 4 * (size_t)(a.nna + 1) * kGdNT * (mixed ? sizeof(float2) : sizeof(double2));
     a.ch = kGdCH;
     launch_prec_gd<PrecF32>(c, kind, a, lds_gd);  // gd is an FP32 form
+  } else if (mixed && c->pairs.nblocks > 1) {
+    // FP32 staged form over several pair blocks (Optabs: both band
+    // directions): one launch per pair block, each staging only its own band
+    // ranges, so the chunk can be larger (at configs[4] 16 and 32 plane waves
+    // instead of 8 for the 40 bands of both directions together: vertices
+    // 205 -> 158 ms per 6 q; in FP64 the LDS allows 16 either way, and the
+    // two launches measured 3% slower)
+    int b0 = 0;
+    while (b0 < nvb) {
+      int b1 = b0;
+      while (b1 < nvb && a.vb[b1].pb == a.vb[b0].pb) ++b1;
+      VertexArgs sa = a;
+      sa.nvb = b1 - b0;
+      for (int b = 0; b < sa.nvb; ++b) sa.vb[b] = a.vb[b0 + b];
+      int snlo = c->d.nband, snhi = 0, smlo = c->d.nband, smhi = 0;
+      for (int b = 0; b < sa.nvb; ++b) {
+        snlo = std::min(snlo, sa.vb[b].n0);
+        snhi = std::max(snhi, sa.vb[b].n0 + sa.vb[b].nrows);
+        smlo = std::min(smlo, sa.vb[b].m0);
+        smhi = std::max(smhi, sa.vb[b].m0 + sa.vb[b].ncols);
+      }
+      sa.na0 = snlo;
+      sa.nna = std::max(snhi - snlo, 1);
+      sa.mb0 = smlo;
+      sa.nmb = std::max(smhi - smlo, 1);
+      auto slds = [&](int chs) {
+        const size_t sp = (size_t)chs / 2 + 16 / rsz;
+        const size_t arows = 2 * (((size_t)sa.nna + 7) / 8) * 8 + 1;
+        return rsz * sp * (4 * arows + (size_t)nbs * 16 * (sa.nmb + 1));
+      };
+      auto sfits = [&](int chs) {
+        return sa.nna <= (256 / chs) * kPfC1 && sa.nmb <= (64 / chs) * kPfB && slds(chs) <= 64 * 1024;
+      };
+      int sch = std::getenv("RPA_VERTEX_CH") ? ch : 32;  // RPA_VERTEX_CH caps the chunk
+      while (sch > 8 && !sfits(sch)) sch /= 2;
+      if (!sfits(sch)) return fail(c, RPA_E_LIMIT, "band count too large for the vertex staging");
+      sa.ch = sch;
+      launch_prec<PrecF32>(c, kind, sa, grid, slds(sch));
+      RPA_HIP(c, hipGetLastError());
+      b0 = b1;
+    }
   } else if (mixed) {
     launch_prec<PrecF32>(c, kind, a, grid, lds);
   } else {
