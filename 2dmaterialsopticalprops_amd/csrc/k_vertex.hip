@@ -149,19 +149,24 @@ __global__ void k_direct_map(const int *mill_g, const int *gbox, Box gb, const i
 //                             current vertices)
 // so a lane loads four K steps of A and of B with two 16-byte reads.  Row
 // 2*nna (and column nmb) stay zero and serve the lanes outside a band block.
-template <class P, int KIND, int MAXB, bool DUAL, int CH>
+template <class P, int KIND, int MAXB, bool DUAL, int CH, bool PAIR>
 __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   using real = typename P::real;
   using cplx = typename P::cplx;
   using acc_t = typename P::acc;
-  constexpr int TPR = 256 / (4 * CH);      // threads per gathered row
+  // PAIR (every band block at most 8 bands m wide): a wave computes two
+  // local fields, its 16 staged B columns holding the first field's bands m
+  // in 0-7 and the second's in 8-15, so no matrix-core column stays empty
+  constexpr int GPW = PAIR ? 2 : 1;              // local fields per wave
+  constexpr int TPR = 256 / (4 * GPW * CH);      // threads per gathered row
   constexpr int TPA = 256 / CH;            // threads per C1 row
   constexpr int S = CH / 2;                // K steps per chunk
   constexpr int SP = S + 16 / sizeof(real);  // padded step stride (16-byte multiple)
   constexpr int NBS = (KIND == 2) ? 2 : 1;   // B sets (J1, J2)
   extern __shared__ __align__(16) unsigned char vsm_raw[];
   const int nna = a.nna, nmb = a.nmb;
-  const int arows = 2 * ((nna + 7) / 8) * 8 + 1, bcols = nmb + 1;  // + zero row / column
+  const int zcol = PAIR ? 16 : nmb;  // zero column
+  const int arows = 2 * ((nna + 7) / 8) * 8 + 1, bcols = zcol + 1;  // + zero row / column
   const int zrow = arows - 1;
   real *sA = (real *)vsm_raw;                      // [2][2][arows][SP]
   real *sB = sA + 4 * arows * SP;                  // [NBS][4][2][2][bcols][SP]
@@ -179,8 +184,8 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   const int k = xcd + 8 * (slot / a.groups);
   const int grp = slot % a.groups;
   if (k >= a.ntot) return;  // whole workgroup leaves together
-  const int iG = grp * 4 + wave;
-  const bool active = iG < a.nlf;
+  const int iG0 = (grp * 4 + wave) * GPW;  // first local field of the wave
+  const bool active = iG0 < a.nlf;
 
   const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
   int d0, d1, d2;
@@ -200,13 +205,14 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   for (int x = tid; x < 4 * SP; x += 256) A_at(x / SP / 2, (x / SP) & 1, zrow)[x % SP] = 0;
   for (int x = tid; x < NBS * 16 * SP; x += 256) {
     const int q = x / SP;  // (set, w, ii, part)
-    B_at(q >> 4, (q >> 2) & 3, (q >> 1) & 1, q & 1, nmb)[x % SP] = 0;
+    B_at(q >> 4, (q >> 2) & 3, (q >> 1) & 1, q & 1, zcol)[x % SP] = 0;
   }
 
   // this thread's staging assignment
-  const int brow = tid / TPR, bsub = tid % TPR;  // gathered row (w, c)
-  const int bw = brow / CH, bc = brow % CH;
-  const int bg = grp * 4 + bw;
+  const int brow = tid / TPR, bsub = tid % TPR;  // gathered row (field, c)
+  const int bf = brow / CH, bc = brow % CH;      // field of the workgroup, plane wave
+  const int bw = bf / GPW, bh = bf % GPW;        // its wave and column half
+  const int bg = grp * 4 * GPW + bf;
   const bool bg_ok = bg < a.nlf;
   const int *dmap = direct ? a.dmap + ((size_t)a.g0idx[k] * a.nlf + (bg_ok ? bg : 0)) * a.ngd : nullptr;
   int lf0 = 0, lf1 = 0, lf2 = 0;
@@ -346,11 +352,12 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
       if (m < nmb) {
         const bool ok = (pm_b >> u) & 1u;
         const cplx z = P::make(ok ? pb[u].x : (real)0, ok ? pb[u].y : (real)0);
-        B_at(0, bw, bi_, 0, m)[bs] = z.x * pj1;
-        B_at(0, bw, bi_, 1, m)[bs] = z.y * pj1;
+        const int mcol = 8 * bh + m;
+        B_at(0, bw, bi_, 0, mcol)[bs] = z.x * pj1;
+        B_at(0, bw, bi_, 1, mcol)[bs] = z.y * pj1;
         if (KIND == 2) {
-          B_at(NBS - 1, bw, bi_, 0, m)[bs] = z.x * pj2;
-          B_at(NBS - 1, bw, bi_, 1, m)[bs] = z.y * pj2;
+          B_at(NBS - 1, bw, bi_, 0, mcol)[bs] = z.x * pj2;
+          B_at(NBS - 1, bw, bi_, 1, mcol)[bs] = z.y * pj2;
         }
       }
     }
@@ -371,7 +378,8 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
     for (int b = 0; b < MAXB; ++b) {
       const VBlock vb = a.vb[b < a.nvb ? b : 0];
       const int arix = (nr < vb.nrows) ? P::arow_index(vb.n0 - a.na0 + nr, im_row) : zrow;
-      const int bcix = (r < vb.ncols) ? vb.m0 - a.mb0 + r : nmb;
+      const int hc = PAIR ? (r >> 3) : 0, mc = PAIR ? (r & 7) : r;
+      const int bcix = (mc < vb.ncols) ? 8 * hc + vb.m0 - a.mb0 + mc : zcol;
       arowp[b] = A_at(ii, part, arix);
       bcolp[b] = B_at(0, wave, ii, part, bcix);
       bcolp2[b] = B_at(NBS - 1, wave, ii, part, bcix);
@@ -482,6 +490,10 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   }
 
   // the two (Re, Im) result pairs of this lane: band out_band(q4, p), column r
+  // (PAIR: field iG0 + r / 8, band m r % 8)
+  const int hc = PAIR ? (r >> 3) : 0, mc = PAIR ? (r & 7) : r;
+  const int iG = iG0 + hc;
+  if (iG >= a.nlf) return;
   cplx *rho1 = (cplx *)a.rho1, *rho2 = (cplx *)a.rho2;
 #pragma unroll
   for (int b = 0; b < MAXB; ++b) {
@@ -492,8 +504,8 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
         const int onr = P::out_band(q4, p);
-        if (onr < vb.nrows && r < vb.ncols) {
-          const int n = vb.n0 + onr, m = vb.m0 + r;
+        if (onr < vb.nrows && mc < vb.ncols) {
+          const int n = vb.n0 + onr, m = vb.m0 + mc;
           const long long t = (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] +
                               (long long)(n - pbk.n_lo) * nm + (m - pbk.m_lo);
           const long long pos = a.perm ? (long long)a.perm[t] : t;
@@ -1039,18 +1051,39 @@ int launch_direct_map(rpa_ctx *c) {
   return 0;
 }
 
-template <class P, int KIND, int MAXB, bool DUAL>
+template <class P, int KIND, int MAXB, bool DUAL, bool PAIR = false>
 static void launch_one(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
   if (a.ch == 32)
-    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 32>), grid, dim3(256), lds, c->stream, a);
+    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 32, PAIR>), grid, dim3(256), lds, c->stream, a);
   else if (a.ch == 16)
-    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 16>), grid, dim3(256), lds, c->stream, a);
+    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 16, PAIR>), grid, dim3(256), lds, c->stream, a);
   else
-    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 8>), grid, dim3(256), lds, c->stream, a);
+    hipLaunchKernelGGL((k_vertex<P, KIND, MAXB, DUAL, 8, PAIR>), grid, dim3(256), lds, c->stream, a);
+}
+
+// Whether the staged form runs two local fields per wave: charge vertices
+// whose band blocks are all at most 8 bands m wide
+static bool staged_pair(const VertexArgs &a, int kind) {
+  if (kind != 0 || a.nvb > 2) return false;
+  for (int b = 0; b < a.nvb; ++b)
+    if (a.vb[b].ncols > 8) return false;
+  return true;
 }
 
 template <class P, int KIND>
 static void launch_kind(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds) {
+  if constexpr (KIND == 0) {
+    if (staged_pair(a, KIND)) {
+      VertexArgs pa = a;
+      pa.groups = (a.nlf + 7) / 8;
+      const dim3 pg((unsigned)((long long)a.kslots * pa.groups));
+      if (a.nvb <= 1)
+        launch_one<P, KIND, 1, true, true>(c, pa, pg, lds);
+      else
+        launch_one<P, KIND, 2, true, true>(c, pa, pg, lds);
+      return;
+    }
+  }
   if (a.nvb <= 1)
     launch_one<P, KIND, 1, true>(c, a, grid, lds);
   else if (a.nvb <= 2)
@@ -1233,7 +1266,8 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   auto lds_of = [&](int ch) {
     const size_t sp = (size_t)ch / 2 + 16 / rsz;
     const size_t arows = 2 * (((size_t)a.nna + 7) / 8) * 8 + 1;
-    return rsz * sp * (4 * arows + (size_t)nbs * 16 * (a.nmb + 1));
+    const size_t bcols = staged_pair(a, kind) ? 17 : (size_t)a.nmb + 1;  // paired fields: 16 columns
+    return rsz * sp * (4 * arows + (size_t)nbs * 16 * bcols);
   };
   auto fits = [&](int ch) {
     return a.nna <= (256 / ch) * kPfC1 && a.nmb <= (64 / ch) * kPfB && lds_of(ch) <= 64 * 1024;

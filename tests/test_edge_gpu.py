@@ -242,3 +242,25 @@ def test_gather_form_paired_current_vertices(pol, monkeypatch):
         print(f"{pol} iq={iq} N_G={nlf} paired gather current vertices rel max err {e1:.2e} {e2:.2e}")
         assert e1 < 2e-6 and e2 < 2e-6
     r.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("ncond", [2, 12])
+def test_staged_form_paired_and_wide_blocks(precision, ncond, monkeypatch):
+    """The staged vertex form with two local fields per wave (band blocks at
+    most 8 bands m wide; odd N_G leaves the last wave half used) and with one
+    (12 bands m): Sloss charge vertices on local fields in all directions
+    against the oracle (FP64 1e-10, FP32 2e-6 max-norm)."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "staged")
+    s = H.system("tiny", lf="xyz", nlf_target=9, ncond=ncond)
+    r = _runner(s, program="sloss", precision=precision)
+    nv, nb = s.cfg.nval, s.cfg.nband
+    for iq, nlf in [(2, 9), (4, 14), (6, 5), (3, 1)]:
+        r.set_q_point(r.q_point(iq, nlf))
+        pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+        ref = orc.vertices(pb, 0, s.gcar, 0, nv, nv, nb)
+        got, _ = r.vertices()
+        err = H.rel_max(got.T, ref)
+        print(f"{precision} ncond={ncond} iq={iq} N_G={nlf} staged-form vertices rel max err {err:.2e}")
+        assert err < (1e-10 if precision == "fp64" else 2e-6)
+    r.close()
