@@ -200,16 +200,15 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
 // from L2, the broadcast operand through the scalar cache (row addresses are
 // wave-uniform).  Rows past N_G read row N_G-1 and their results are dropped,
 // so the row loops have no branch and their loads are issued together.
-// Gauss-Jordan step k: wave 0 finds the pivot; waves 0 and 1 read the pivot
-// row, row p and column k, then (after a barrier) write the scaled pivot row,
-// the swapped row p and the multipliers; all waves eliminate.
+// Gauss-Jordan step k: wave 0 finds the pivot, reads the pivot row, row p and
+// column k and writes the scaled pivot row, the swapped row p and the
+// multipliers; after one barrier all waves eliminate.
 constexpr int kDyRows = 16;  // rows per wave (N_G <= 64)
 
 __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
   extern __shared__ double2 sa[];  // eps -> eps^-1 [N][N]
   __shared__ double2 fcol[64];
   __shared__ int piv[64];
-  __shared__ int pidx;
   const int N = a.nlf, tid = threadIdx.x, j = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int f = blockIdx.x;
@@ -264,50 +263,41 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
   }
   __syncthreads();
 
-  // in-place Gauss-Jordan inversion with row pivoting
+  // in-place Gauss-Jordan inversion with row pivoting.  Wave 0 alone finds
+  // the pivot, reads the pivot row, row p and column k, and writes the
+  // scaled pivot row, the swapped row, the multipliers and the zeroed column
+  // k: its LDS operations run in program order, so no barrier separates
+  // those reads from those writes.  All waves then eliminate.
   for (int k = 0; k < N; ++k) {
     if (wave == 0) {
       // pivot: largest |Re| + |Im| in column k at or below row k, lowest row on ties
       // (the maximum by a butterfly, then the lowest lane holding it by a ballot:
       // lane j holds row j)
-      const double2 z = sa[jc * N + k];
+      const double2 z = sa[jc * N + k];  // column k of row j before the swap
       const double mine = (j >= k && jok) ? fabs(z.x) + fabs(z.y) : -1.0;
       double best = mine;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off));
-      const int bi = __ffsll((unsigned long long)__ballot(mine == best)) - 1;
-      if (j == 0) {
-        int p = bi;
-        if (!(best > 0.0)) {  // singular: flag it and go on without a swap
-          atomicOr(a.status, 1);
-          p = k;
-        }
-        pidx = p;
-        piv[k] = p;
+      int p = __ffsll((unsigned long long)__ballot(mine == best)) - 1;
+      if (!(best > 0.0)) {  // singular: flag it and go on without a swap
+        if (j == 0) atomicOr(a.status, 1);
+        p = k;
       }
-    }
-    __syncthreads();
-    const int p = pidx;
-    // read phase
-    double2 rk = make_double2(0.0, 0.0), rp = rk, inv = rk, ck = rk, okk = rk;
-    if (wave == 0) {
-      rk = sa[k * N + jc];
-      rp = sa[p * N + jc];
-      inv = cdiv1(sa[p * N + k]);  // pivot value: column k of the new row k
-    } else if (wave == 1) {
-      ck = sa[jc * N + k];         // column k of row i = j before the swap
-      okk = sa[k * N + k];         // column k of the old row k (the new row p)
-    }
-    __syncthreads();
-    // write phase: scaled pivot row, swapped row p, multipliers, zeroed column k
-    if (wave == 0 && jok) {
-      sa[k * N + j] = (j == k) ? inv : cmul(rp, inv);
-      if (p != k && j != k) sa[p * N + j] = rk;
-    } else if (wave == 1 && jok && j != k) {
-      // per component: a select of whole double2 values went through scratch memory
-      const bool sw = j == p;
-      fcol[j] = make_double2(sw ? okk.x : ck.x, sw ? okk.y : ck.y);
-      sa[j * N + k] = make_double2(0.0, 0.0);
+      if (j == 0) piv[k] = p;
+      const double2 rk = sa[k * N + jc];
+      const double2 rp = sa[p * N + jc];
+      const double2 inv = cdiv1(sa[p * N + k]);  // pivot value: column k of the new row k
+      const double2 okk = sa[k * N + k];         // column k of the old row k (the new row p)
+      if (jok) {
+        sa[k * N + j] = (j == k) ? inv : cmul(rp, inv);
+        if (p != k && j != k) sa[p * N + j] = rk;
+        if (j != k) {
+          // per component: a select of whole double2 values goes through scratch memory
+          const bool sw = j == p;
+          fcol[j] = make_double2(sw ? okk.x : z.x, sw ? okk.y : z.y);
+          sa[j * N + k] = make_double2(0.0, 0.0);
+        }
+      }
     }
     __syncthreads();
     {
