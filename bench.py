@@ -200,7 +200,20 @@ def vertex_kernels(r, prec, plans):
     return " + ".join(f"{n} on {c} q" for n, c in names.items())
 
 
-def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None):
+def vertex_operand_bytes(r, prec):
+    """Bytes the vertex kernel has to read at least once per k (direct map):
+    C1 of the NGd plane waves in the bands n, and the k+q rows of every
+    distinct target G(i) + Glf(G) in the bands m, summed over the FBZ k."""
+    m = r.sys.mill_g[:r.ngd]
+    lf = r.sys.mill_g[np.asarray(r.lf_idx) - 1]
+    ntarget = len(np.unique((m[:, None, :] + lf[None, :, :]).reshape(-1, 3), axis=0))
+    nv, nb = r.cfg.nval, r.cfg.nband
+    nn, nm = (nb, nb) if r.rc.program == "optabs" else (nv, nb - nv)
+    es = 8 if prec == "mixed" else 16
+    return float(len(r.K1)) * (r.ngd * nn + ntarget * nm) * es
+
+
+def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None, vbytes=None):
     """Roofline of the dominant matrix-core kernel of one run."""
     acc_ms, vtx_ms = res["stages"]["accumulate"], res["stages"]["vertices"]
     if prec == "mixed":
@@ -222,6 +235,9 @@ def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None):
                          "(profiles/hbm_traffic.txt)") if traffic else "not measured for this workload",
         "kernel": dom, "kernel_ms": round(ms, 4),
         "algorithmic_flop": fl,
+        **({"algorithmic_bytes": vbytes,
+            "traffic_over_algorithmic_bytes": round(traffic / vbytes, 3) if traffic else None}
+           if stage == "vertices" and vbytes else {}),
         "other": {
             names[1]: {"ms": round(vtx_ms, 4), "flop": flops_vtx, "peak": peaks[1],
                        "tflops": round(flops_vtx / (vtx_ms * 1e-3) / 1e12, 3) if vtx_ms > 0 else None},
@@ -450,7 +466,8 @@ def main():
     ntr = r.ntransitions()
     nnz, flops_acc, flops_vtx = work_model(r, host, cfg, r.K1, r.K2, r.nlf, ntr)
     workload_key = f"{args.config} {args.program} {args.vertex_map}"
-    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key, vertex_kernel(r, head))
+    vbytes = vertex_operand_bytes(r, head) if args.vertex_map == "direct" else None
+    roofline = roofline_of(runs[head], head, flops_acc, flops_vtx, workload_key, vertex_kernel(r, head), vbytes)
     roofline["other"]["nnz_weights"] = nnz
 
     out = {
