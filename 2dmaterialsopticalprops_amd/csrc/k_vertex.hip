@@ -1316,10 +1316,10 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   } else if (mixed && c->pairs.nblocks > 1) {
     // FP32 staged form over several pair blocks (Optabs: both band
     // directions): one launch per pair block, each staging only its own band
-    // ranges, so the chunk can be larger (at configs[4] 16 and 32 plane waves
-    // instead of 8 for the 40 bands of both directions together: vertices
-    // 205 -> 158 ms per 6 q; in FP64 the LDS allows 16 either way, and the
-    // two launches measured 3% slower)
+    // ranges, so the chunk can be larger (on the 36x36 k-grid with 16 + 24
+    // bands, 16 and 32 plane waves instead of 8 for the 40 bands of both
+    // directions together: vertices 205 -> 158 ms per 6 q; in FP64 the LDS
+    // allows 16 either way, and the two launches measured 3% slower)
     int b0 = 0;
     while (b0 < nvb) {
       int b1 = b0;
