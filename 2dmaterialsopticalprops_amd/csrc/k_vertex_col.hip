@@ -82,6 +82,7 @@ struct ColF64 {
   static __device__ inline int out_im(int p) { return p + 2; }
   static __device__ inline cplx make(real x, real y) { return make_double2(x, y); }
   static __device__ inline acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
+  static constexpr bool split = false;
 };
 
 struct ColF32 {
@@ -96,7 +97,33 @@ struct ColF32 {
   static __device__ inline int out_im(int p) { return 2 * p + 1; }
   static __device__ inline cplx make(real x, real y) { return make_float2(x, y); }
   static __device__ inline acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
+  static constexpr bool split = false;
 };
+
+// FP32 vertices on the BF16 matrix cores (charge vertices): every staged
+// value is split as hi = bf16(x), lo = bf16(x - hi) when it is written to
+// LDS, and a product is hi.hi + hi.lo + lo.hi with FP32 accumulation
+// (relative error of one product below 2^-16; the dropped lo.lo term is
+// below 2^-18).  Rows and result layout as ColF32; K of one MFMA is 16
+// positions x (Re, Im): K index 8 kg + 2 i + c <-> position 16 t + 4 kg + i,
+// component c, so a lane reads 4 consecutive positions of one band of A
+// (one 16-byte LDS read per part) and, per tile, 4 k+q values whose hi and
+// lo words lie side by side (one 8-byte LDS read each: 7% faster than hi and
+// lo in separate arrays at N_G = 51).
+struct ColBf3 : ColF32 {
+  static constexpr bool split = true;
+};
+
+typedef uint32_t rpa_u4 __attribute__((ext_vector_type(4)));
+typedef int rpa_i4 __attribute__((ext_vector_type(4)));
+
+// hi and lo BF16 parts of (Re, Im), Re in the low half of each word
+__device__ inline void split_pair(float2 z, uint32_t &h, uint32_t &l) {
+  const __bf16 hx = (__bf16)z.x, hy = (__bf16)z.y;
+  const __bf16 lx = (__bf16)(z.x - (float)hx), ly = (__bf16)(z.y - (float)hy);
+  h = (uint32_t)__builtin_bit_cast(uint16_t, hx) | ((uint32_t)__builtin_bit_cast(uint16_t, hy) << 16);
+  l = (uint32_t)__builtin_bit_cast(uint16_t, lx) | ((uint32_t)__builtin_bit_cast(uint16_t, ly) << 16);
+}
 
 // Staging of one chunk in registers (loaded one chunk ahead, stored to LDS
 // after the barrier).  Thread t: C1 rows of positions t/8 + 32u, band t%8;
@@ -128,11 +155,16 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
   // steps per pass of the product loop: two in FP32 (FP64 would pass 256
   // registers and halve the waves per SIMD)
   constexpr int SU = (sizeof(real) == 4) ? 2 : 1;
+  constexpr bool SPLIT = P::split;
+  static_assert(!SPLIT || KIND == 0, "the split form runs charge vertices");
+  constexpr int KCA = KC + 4;  // split form: A row stride (words), 16-byte aligned rows
   extern __shared__ __align__(16) unsigned char col_raw[];
   cplx *sA = (cplx *)col_raw;         // [KC][8]   C1 of position p, band n
-  int *sP = (int *)(sA + KC * 8);     // [KC]      B row of position p
+  uint32_t *sAw = (uint32_t *)col_raw;  // split: [hi, lo][8 bands n][KCA] BF16 pairs of C1
+  int *sP = SPLIT ? (int *)(sAw + 2 * 8 * KCA) : (int *)(sA + KC * 8);  // [KC] B row of position p
   real *sBeta = (real *)(sP + KC);    // [2][KC]   beta of position p (current vertices)
   cplx *sB = (cplx *)(sBeta + 2 * KC);  // [rows][ms] C2 of the targets, bands m
+  uint32_t *sBw = (uint32_t *)(sP + KC);  // split: [rows][ms][hi, lo] BF16 pairs of C2
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int L = blockIdx.x;
@@ -208,6 +240,30 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
   // staging struct in scratch memory for double2)
   auto keep = [](bool ok, cplx z) { return P::make(ok ? z.x : (real)0, ok ? z.y : (real)0); };
   auto store = [&]() {
+    if constexpr (SPLIT) {
+#pragma unroll
+      for (int u = 0; u < KC / 32; ++u) {
+        uint32_t h, l;
+        split_pair(keep((st.amask >> u) & 1u, st.a[u]), h, l);
+        sAw[sa_n * KCA + sa_p + 32 * u] = h;
+        sAw[(8 + sa_n) * KCA + sa_p + 32 * u] = l;
+      }
+      if (tid < KC) sP[tid] = tid < st.np ? st.prow : 0;
+#pragma unroll
+      for (int u = 0; u < NRU; ++u) {
+        const int rr = sb_r + 64 * u;
+        if (rr < st.nr) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (sb_m + e < a.ms) {
+              uint32_t h, l;
+              split_pair(keep((st.bmask >> (4 * u + e)) & 1u, st.b[u][e]), h, l);
+              *(uint2 *)&sBw[2 * (rr * a.ms + sb_m + e)] = make_uint2(h, l);
+            }
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < KC / 32; ++u) sA[(sa_p + 32 * u) * 8 + sa_n] = keep((st.amask >> u) & 1u, st.a[u]);
     const bool pok = tid < st.np;
@@ -268,7 +324,45 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
     // every wave runs TPW tiles; tiles past nt read row group 0 and their
     // results are dropped (no branch in the loop, so the LDS reads of all
     // tiles are issued ahead of the products)
-    if constexpr (SU == 2) {
+    if constexpr (SPLIT) {
+      // 16 positions per step, three BF16 products per tile.  Positions past
+      // the chunk (to the next multiple of 16 <= KC) have zero A parts and
+      // B row 0.  The A row of an Im row is (-Im, Re) of C1: the two halves
+      // of each word swapped and the sign of the new low half flipped.
+      const int nsteps16 = (a.chunk[ch + 1] - a.chunk[ch] + 15) >> 4;
+      const uint32_t *aph = sAw + band_r * KCA, *apl = aph + 8 * KCA;
+      for (int t = 0; t < nsteps16; ++t) {
+        const int p0 = 16 * t + 4 * kg;
+        const rpa_i4 pr = *(const rpa_i4 *)&sP[p0];
+        rpa_u4 ah = *(const rpa_u4 *)&aph[p0];
+        rpa_u4 al = *(const rpa_u4 *)&apl[p0];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t sh = __builtin_amdgcn_alignbit(ah[i], ah[i], 16) ^ 0x8000u;
+          const uint32_t sl = __builtin_amdgcn_alignbit(al[i], al[i], 16) ^ 0x8000u;
+          ah[i] = im_row ? sh : ah[i];
+          al[i] = im_row ? sl : al[i];
+        }
+        const rpa_bf8 a_h = __builtin_bit_cast(rpa_bf8, ah), a_l = __builtin_bit_cast(rpa_bf8, al);
+        int rb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rb[i] = pr[i] * a.ms + lane_off;
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+          rpa_u4 bh, bl;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const uint2 v = *(const uint2 *)&sBw[2 * (rb[i] + toff[tt])];
+            bh[i] = v.x;
+            bl[i] = v.y;
+          }
+          const rpa_bf8 b_h = __builtin_bit_cast(rpa_bf8, bh), b_l = __builtin_bit_cast(rpa_bf8, bl);
+          acc[0][tt] = mfma_bf16_16x16x32(a_l, b_h, acc[0][tt]);
+          acc[0][tt] = mfma_bf16_16x16x32(a_h, b_l, acc[0][tt]);
+          acc[0][tt] = mfma_bf16_16x16x32(a_h, b_h, acc[0][tt]);
+        }
+      }
+    } else if constexpr (SU == 2) {
       // two steps per pass: the LDS reads of both steps' A values and B rows
       // are issued together, so one latency chain feeds 2 x TPW x NV x 2
       // products.  An odd last step reads positions past the chunk, whose A
@@ -465,6 +559,14 @@ static void launch_col_of(rpa_ctx *c, const ColArgs &a, int kind, bool pipe, int
     launch_col_p<P, 0, KC, NRU, true>(c, a, tpw, grid);
   else
     launch_col_p<P, 0, KC, NRU, false>(c, a, tpw, grid);
+}
+
+// Mixed charge vertices run the split BF16 form unless RPA_VERTEX_COL_SPLIT=0
+// asks for the FP32 matrix cores.
+static bool col_split(bool mixed, int kind) {
+  if (!mixed || kind != 0) return false;
+  const char *env = std::getenv("RPA_VERTEX_COL_SPLIT");
+  return !(env && env[0] == '0');
 }
 
 // 1 when RPA_VERTEX_FORM=col asks for this form regardless of shape
@@ -720,9 +822,14 @@ int build_col_plan(rpa_ctx *c) {
   P.pol1 = pol1;
   P.pol2 = pol2;
   P.lds = fixed + (size_t)rows_used * ms * csz;
+  // the split form keeps hi and lo words of each staged value (the bytes of
+  // one complex FP32 value) and no beta rows
+  P.split = col_split(mixed, kind);
   if (P.lds > 64 * 1024) {
-    const void *fn = mixed ? col_kernel_of<ColF32, kKcF32, kNruF32>(kind, pipe, tpw)
-                           : col_kernel_of<ColF64, kKcF64, kNruF64>(kind, pipe, tpw);
+    const void *fn = P.split ? (pipe ? col_kernel<ColBf3, 0, kKcF32, kNruF32, true>(tpw)
+                                     : col_kernel<ColBf3, 0, kKcF32, kNruF32, false>(tpw))
+                     : mixed ? col_kernel_of<ColF32, kKcF32, kNruF32>(kind, pipe, tpw)
+                             : col_kernel_of<ColF64, kKcF64, kNruF64>(kind, pipe, tpw);
     RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds));
   }
   P.ok = true;
@@ -770,7 +877,12 @@ int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, void *
   const long long kslots = (c->ntot + 7) / 8 * 8;
   const dim3 grid((unsigned)(kslots * a.blocks));
   a.ms = P.ms;
-  if (mixed)
+  if (P.split) {
+    if (P.pipe)
+      launch_col_p<ColBf3, 0, kKcF32, kNruF32, true>(c, a, P.tpw, grid);
+    else
+      launch_col_p<ColBf3, 0, kKcF32, kNruF32, false>(c, a, P.tpw, grid);
+  } else if (mixed)
     launch_col_of<ColF32, kKcF32, kNruF32>(c, a, P.kind, P.pipe, P.tpw, grid);
   else
     launch_col_of<ColF64, kKcF64, kNruF64>(c, a, P.kind, P.pipe, P.tpw, grid);

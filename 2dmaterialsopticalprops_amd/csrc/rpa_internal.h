@@ -85,6 +85,7 @@ struct ColPlan {
   int npos = 0, nchunk = 0, ncls = 0, nrows = 0;  // nrows: B rows of all classes (one G0)
   int kc = 0, tpw = 0, ms = 0, kind = 0, pol1 = 0, pol2 = 0;
   bool pipe = false;
+  bool split = false;  // mixed charge vertices: BF16 hi/lo split products
   size_t lds = 0;
   int *d_slot = nullptr, *d_prow = nullptr, *d_chunk = nullptr, *d_crow = nullptr;
   int *d_bmap = nullptr, *d_shift = nullptr, *d_cls = nullptr;

@@ -20,6 +20,12 @@ pytestmark = pytest.mark.gpu
 
 TOL_V = 1e-12
 TOL = 1e-10
+# Mixed vertices: FP32 matrix-core forms 2e-6; the column form of the charge
+# vertices splits FP32 inputs into BF16 hi + lo and multiplies hi.hi + hi.lo
+# + lo.hi, each product within 2^-16 of the FP32 one (measured 4.8e-6 on
+# cfg0), so 2e-5 there.
+TOL_V_MIXED = 2e-6
+TOL_V_SPLIT = 2e-5
 
 
 def _runner(sys_, **kw):
@@ -292,19 +298,26 @@ def test_many_band_blocks_optabs_full_pairs():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("form", ["col", "lds", "gather", "staged"])
+@pytest.mark.parametrize("form", ["col", "col-fp32", "lds", "gather", "staged"])
 @pytest.mark.parametrize("mode", ["direct", "igk"])
 def test_vertex_kernel_forms(precision, form, mode, monkeypatch):
     """Every vertex kernel form on the Sloss charge vertices of cfg0 (the
-    column form applies to the direct map; with igk the shape rule picks)."""
-    monkeypatch.setenv("RPA_VERTEX_FORM", form)
+    column form applies to the direct map; with igk the shape rule picks).
+    col-fp32 is the column form on the FP32 matrix cores instead of the
+    split BF16 products (same as col in FP64)."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", form.split("-")[0])
+    monkeypatch.setenv("RPA_VERTEX_COL_SPLIT", "0" if form == "col-fp32" else "1")
     s = H.system("cfg0")
     r = host.Runner(s, host.RunConfig(program="sloss", vertex_map=mode, precision=precision))
     pb = _oracle_problem(s, r, 0 if mode == "direct" else 1)
     nv, nb = s.cfg.nval, s.cfg.nband
     ref = orc.vertices(pb, 0, s.gcar, 0, nv, nv, nb)
     got, _ = r.vertices()
-    assert H.rel_max(got.T, ref) < (TOL_V if precision == "fp64" else 2e-6)
+    split = precision == "mixed" and form == "col" and mode == "direct"
+    tol = TOL_V if precision == "fp64" else TOL_V_SPLIT if split else TOL_V_MIXED
+    err = H.rel_max(got.T, ref)
+    print(f"{precision} {form} {mode} vertices rel max err {err:.2e} (tol {tol:.0e})")
+    assert err < tol
     r.close()
 
 
@@ -324,7 +337,7 @@ def test_column_form_current_vertices(precision, pol, monkeypatch):
     nk, tv = len(r.K1), nv * (nb - nv)
     g1 = g1.T.reshape(nk, 2 * tv, r.nlf)
     g2 = g2.T.reshape(nk, 2 * tv, r.nlf)
-    tol = TOL_V if precision == "fp64" else 2e-6
+    tol = TOL_V if precision == "fp64" else TOL_V_MIXED
     e1 = H.rel_max(g1[:, :tv], a1.reshape(nk, tv, r.nlf))
     e2 = H.rel_max(g2[:, :tv], a2.reshape(nk, tv, r.nlf))
     print(f"{precision} {pol} column-form current vertices rel max err {e1:.2e} {e2:.2e}")
