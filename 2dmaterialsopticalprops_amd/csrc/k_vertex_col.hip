@@ -104,12 +104,16 @@ struct ColF32 {
 // value is split as hi = bf16(x), lo = bf16(x - hi) when it is written to
 // LDS, and a product is hi.hi + hi.lo + lo.hi with FP32 accumulation
 // (relative error of one product below 2^-16; the dropped lo.lo term is
-// below 2^-18).  Rows and result layout as ColF32; K of one MFMA is 16
-// positions x (Re, Im): K index 8 kg + 2 i + c <-> position 16 t + 4 kg + i,
-// component c, so a lane reads 4 consecutive positions of one band of A
-// (one 16-byte LDS read per part) and, per tile, 4 k+q values whose hi and
-// lo words lie side by side (one 8-byte LDS read each: 7% faster than hi and
-// lo in separate arrays at N_G = 51).
+// below 2^-18).  Rows and result layout as ColF32; K of one MFMA is 8
+// positions x (Re, Im) x (hi, lo): K index 8 kg + 4 j + 2 h + c <-> position
+// 8 t + 2 kg + j, part h, component c.  The hi and lo words of a staged k+q
+// value lie side by side, so a lane's B operand per tile is two 8-byte LDS
+// reads; A holds (hi, hi) for the hi.hi + hi.lo products and (lo, 0) for
+// lo.hi, so a 16-position step costs 4 MFMAs where an operand layout with
+// K = 16 positions x (Re, Im) costs 3, but that one needs 8 register moves
+// per tile to gather hi and lo words into separate operands (measured at
+// N_G = 51: 1.04 ms with this layout and batched reads, 1.16 ms with that
+// one).
 struct ColBf3 : ColF32 {
   static constexpr bool split = true;
 };
@@ -127,7 +131,9 @@ __device__ inline void split_pair(float2 z, uint32_t &h, uint32_t &l) {
 
 // Staging of one chunk in registers (loaded one chunk ahead, stored to LDS
 // after the barrier).  Thread t: C1 rows of positions t/8 + 32u, band t%8;
-// k+q rows t/4 + 64u, bands m 4(t%4)..+3; B row of position t (t < KC).
+// k+q rows t/4 + 64u, bands m t%4 + 4e (16 consecutive lanes store to 16
+// distinct bank pairs: 2% faster than bands 4(t%4)..+3 at N_G = 51, half
+// the LDS bank conflicts); B row of position t (t < KC).
 // Every load is issued unconditionally from a clamped, in-bounds address and
 // its presence kept in a mask; the zero fill of absent values happens in the
 // LDS store.  (A conditional load joined with a zero fill makes the compiler
@@ -147,7 +153,7 @@ struct ColStage {
 // (fewer waves, hidden latency) instead of loaded between the barriers.
 // KIND: 0 charge, 1 current (one polarisation), 2 current (pol yz: two).
 template <class P, int KIND, int TPW, int KC, int NRU, bool PIPE>
-__global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
+__device__ __forceinline__ void col_body(const ColArgs &a) {
   using real = typename P::real;
   using cplx = typename P::cplx;
   using acc_t = typename P::acc;
@@ -190,7 +196,7 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
 
   // ---- staging roles
   const int sa_p = tid >> 3, sa_n = tid & 7;       // C1: position sa_p + 32u, band sa_n
-  const int sb_r = tid >> 2, sb_m = 4 * (tid & 3);  // k+q: row sb_r + 64u, bands sb_m..+3
+  const int sb_r = tid >> 2, sb_m = tid & 3;  // k+q: row sb_r + 64u, bands sb_m + 4e
   ColStage<P, KC, NRU> st;
   auto load = [&](int ch) {
     const int p0 = a.chunk[ch], np = a.chunk[ch + 1] - p0;
@@ -228,8 +234,8 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
         const int row = ok ? jj[u] : 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const bool eok = sb_m + e < vb.ncols;
-          st.b[u][e] = c2[(size_t)row * a.nband + (eok ? sb_m + e : 0)];
+          const bool eok = sb_m + 4 * e < vb.ncols;
+          st.b[u][e] = c2[(size_t)row * a.nband + (eok ? sb_m + 4 * e : 0)];
           bm |= (ok && eok) ? (1u << (4 * u + e)) : 0u;
         }
       }
@@ -255,10 +261,10 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
         if (rr < st.nr) {
 #pragma unroll
           for (int e = 0; e < 4; ++e)
-            if (sb_m + e < a.ms) {
+            if (sb_m + 4 * e < a.ms) {
               uint32_t h, l;
               split_pair(keep((st.bmask >> (4 * u + e)) & 1u, st.b[u][e]), h, l);
-              *(uint2 *)&sBw[2 * (rr * a.ms + sb_m + e)] = make_uint2(h, l);
+              *(uint2 *)&sBw[2 * (rr * a.ms + sb_m + 4 * e)] = make_uint2(h, l);
             }
         }
       }
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
       if (rr < st.nr) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-          if (sb_m + e < a.ms) sB[rr * a.ms + sb_m + e] = keep((st.bmask >> (4 * u + e)) & 1u, st.b[u][e]);
+          if (sb_m + 4 * e < a.ms) sB[rr * a.ms + sb_m + 4 * e] = keep((st.bmask >> (4 * u + e)) & 1u, st.b[u][e]);
       }
     }
   };
@@ -325,41 +331,59 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
     // results are dropped (no branch in the loop, so the LDS reads of all
     // tiles are issued ahead of the products)
     if constexpr (SPLIT) {
-      // 16 positions per step, three BF16 products per tile.  Positions past
-      // the chunk (to the next multiple of 16 <= KC) have zero A parts and
-      // B row 0.  The A row of an Im row is (-Im, Re) of C1: the two halves
-      // of each word swapped and the sign of the new low half flipped.
-      const int nsteps16 = (a.chunk[ch + 1] - a.chunk[ch] + 15) >> 4;
+      // 8 positions per block, K = (position, Re/Im, hi/lo): a lane's B
+      // operand is the (hi, lo) word pairs of its 2 positions, two 8-byte
+      // LDS reads straight into the operand registers.  A1 = (hi, hi) of C1
+      // against (hi, lo) gives hi.hi + hi.lo, A2 = (lo, 0) gives lo.hi.  The
+      // B values of all tiles are read before the products (the scheduling
+      // barrier keeps the compiler from sinking each read to its product,
+      // which left one LDS latency per tile exposed); the next block's rows
+      // and A parts are read first, since the LDS returns in order.
+      // Positions past the chunk (to the next multiple of 8 <= KC) have zero
+      // A parts and B row 0.  The A row of an Im row is (-Im, Re) of C1: the
+      // halves of each word swapped and the sign of the new low half flipped.
+      const int nblk = (a.chunk[ch + 1] - a.chunk[ch] + 7) >> 3;
       const uint32_t *aph = sAw + band_r * KCA, *apl = aph + 8 * KCA;
-      for (int t = 0; t < nsteps16; ++t) {
-        const int p0 = 16 * t + 4 * kg;
-        const rpa_i4 pr = *(const rpa_i4 *)&sP[p0];
-        rpa_u4 ah = *(const rpa_u4 *)&aph[p0];
-        rpa_u4 al = *(const rpa_u4 *)&apl[p0];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t sh = __builtin_amdgcn_alignbit(ah[i], ah[i], 16) ^ 0x8000u;
-          const uint32_t sl = __builtin_amdgcn_alignbit(al[i], al[i], 16) ^ 0x8000u;
-          ah[i] = im_row ? sh : ah[i];
-          al[i] = im_row ? sl : al[i];
+      auto rot = [&](uint32_t w) { return im_row ? (__builtin_amdgcn_alignbit(w, w, 16) ^ 0x8000u) : w; };
+      int2 prn = *(const int2 *)&sP[2 * kg];
+      uint2 ahn = *(const uint2 *)&aph[2 * kg];
+      uint2 aln = *(const uint2 *)&apl[2 * kg];
+      for (int t = 0; t < nblk; ++t) {
+        const int2 pr = prn;
+        uint2 ah = ahn, al = aln;
+        const int rb0 = pr.x * a.ms + lane_off, rb1 = pr.y * a.ms + lane_off;
+        // the next block's rows and A parts first: the LDS returns in order,
+        // so waiting for the B values below never waits for these
+        {
+          const int q1 = min(8 * (t + 1), KC - 8) + 2 * kg;
+          prn = *(const int2 *)&sP[q1];
+          ahn = *(const uint2 *)&aph[q1];
+          aln = *(const uint2 *)&apl[q1];
         }
-        const rpa_bf8 a_h = __builtin_bit_cast(rpa_bf8, ah), a_l = __builtin_bit_cast(rpa_bf8, al);
-        int rb[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) rb[i] = pr[i] * a.ms + lane_off;
+        rpa_u4 b[TPW];
 #pragma unroll
         for (int tt = 0; tt < TPW; ++tt) {
-          rpa_u4 bh, bl;
+          const uint2 v0 = *(const uint2 *)&sBw[2 * (rb0 + toff[tt])];
+          const uint2 v1 = *(const uint2 *)&sBw[2 * (rb1 + toff[tt])];
+          b[tt][0] = v0.x;
+          b[tt][1] = v0.y;
+          b[tt][2] = v1.x;
+          b[tt][3] = v1.y;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        ah.x = rot(ah.x);
+        ah.y = rot(ah.y);
+        al.x = rot(al.x);
+        al.y = rot(al.y);
+        rpa_u4 a1, a2;
+        a1[0] = ah.x; a1[1] = ah.x; a1[2] = ah.y; a1[3] = ah.y;
+        a2[0] = al.x; a2[1] = 0u; a2[2] = al.y; a2[3] = 0u;
+        const rpa_bf8 a_1 = __builtin_bit_cast(rpa_bf8, a1), a_2 = __builtin_bit_cast(rpa_bf8, a2);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const uint2 v = *(const uint2 *)&sBw[2 * (rb[i] + toff[tt])];
-            bh[i] = v.x;
-            bl[i] = v.y;
-          }
-          const rpa_bf8 b_h = __builtin_bit_cast(rpa_bf8, bh), b_l = __builtin_bit_cast(rpa_bf8, bl);
-          acc[0][tt] = mfma_bf16_16x16x32(a_l, b_h, acc[0][tt]);
-          acc[0][tt] = mfma_bf16_16x16x32(a_h, b_l, acc[0][tt]);
-          acc[0][tt] = mfma_bf16_16x16x32(a_h, b_h, acc[0][tt]);
+        for (int tt = 0; tt < TPW; ++tt) {
+          const rpa_bf8 bb = __builtin_bit_cast(rpa_bf8, b[tt]);
+          acc[0][tt] = mfma_bf16_16x16x32(a_2, bb, acc[0][tt]);
+          acc[0][tt] = mfma_bf16_16x16x32(a_1, bb, acc[0][tt]);
         }
       }
     } else if constexpr (SU == 2) {
@@ -511,6 +535,19 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
   }
 }
 
+template <class P, int KIND, int TPW, int KC, int NRU, bool PIPE>
+__global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
+  col_body<P, KIND, TPW, KC, NRU, PIPE>(a);
+}
+
+// The split form (mixed charge vertices) with two waves per SIMD held: the
+// batched B values of all tiles would otherwise take the register count past
+// 256 and one wave per SIMD.
+template <int TPW, int KC, int NRU, bool PIPE>
+__global__ __launch_bounds__(256, 2) void k_vertex_col_split(ColArgs a) {
+  col_body<ColBf3, 0, TPW, KC, NRU, PIPE>(a);
+}
+
 template <class P, int KIND, int KC, int NRU, bool PIPE>
 static const void *col_kernel(int tpw) {
   switch (tpw) {
@@ -537,6 +574,35 @@ static void launch_col_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
     case 10: hipLaunchKernelGGL((k_vertex_col<P, KIND, 10, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
     case 13: hipLaunchKernelGGL((k_vertex_col<P, KIND, 13, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
     default: hipLaunchKernelGGL((k_vertex_col<P, KIND, 16, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+  }
+}
+
+template <int KC, int NRU, bool PIPE>
+static const void *split_kernel(int tpw) {
+  switch (tpw) {
+    case 1: return (const void *)k_vertex_col_split<1, KC, NRU, PIPE>;
+    case 2: return (const void *)k_vertex_col_split<2, KC, NRU, PIPE>;
+    case 4: return (const void *)k_vertex_col_split<4, KC, NRU, PIPE>;
+    case 6: return (const void *)k_vertex_col_split<6, KC, NRU, PIPE>;
+    case 8: return (const void *)k_vertex_col_split<8, KC, NRU, PIPE>;
+    case 10: return (const void *)k_vertex_col_split<10, KC, NRU, PIPE>;
+    case 13: return (const void *)k_vertex_col_split<13, KC, NRU, PIPE>;
+    default: return (const void *)k_vertex_col_split<16, KC, NRU, PIPE>;
+  }
+}
+
+template <int KC, int NRU, bool PIPE>
+static void launch_split_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
+  const size_t lds = c->col.lds;
+  switch (tpw) {
+    case 1: hipLaunchKernelGGL((k_vertex_col_split<1, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_vertex_col_split<2, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_vertex_col_split<4, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 6: hipLaunchKernelGGL((k_vertex_col_split<6, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 8: hipLaunchKernelGGL((k_vertex_col_split<8, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 10: hipLaunchKernelGGL((k_vertex_col_split<10, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    case 13: hipLaunchKernelGGL((k_vertex_col_split<13, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_vertex_col_split<16, KC, NRU, PIPE>), grid, dim3(256), lds, c->stream, a); break;
   }
 }
 
@@ -698,8 +764,12 @@ int build_col_plan(rpa_ctx *c) {
   // distinct LDS banks
   const int ms = (nmg == 1) ? 4 : (nmg <= 3) ? 12 : 20;
   // loading the next chunk during the products measured faster in both
-  // precisions at N_G = 51 (FP32 3.2 vs 4.6 ms, FP64 5.3 vs 5.4 ms)
-  bool pipe = true;
+  // precisions at N_G = 51 (FP32 3.2 vs 4.6 ms, FP64 5.3 vs 5.4 ms); the
+  // split form reads every tile's B values before its products, and the
+  // staging registers on top spill (1.28 vs 1.04 ms loading between the
+  // barriers)
+  const bool split = col_split(mixed, kind);
+  bool pipe = !split;
   if (const char *env = std::getenv("RPA_VERTEX_COL_PIPE")) pipe = env[0] == '1';
 
   // chunks: consecutive segments, at most kc positions and rmax rows
@@ -824,10 +894,10 @@ int build_col_plan(rpa_ctx *c) {
   P.lds = fixed + (size_t)rows_used * ms * csz;
   // the split form keeps hi and lo words of each staged value (the bytes of
   // one complex FP32 value) and no beta rows
-  P.split = col_split(mixed, kind);
+  P.split = split;
   if (P.lds > 64 * 1024) {
-    const void *fn = P.split ? (pipe ? col_kernel<ColBf3, 0, kKcF32, kNruF32, true>(tpw)
-                                     : col_kernel<ColBf3, 0, kKcF32, kNruF32, false>(tpw))
+    const void *fn = P.split ? (pipe ? split_kernel<kKcF32, kNruF32, true>(tpw)
+                                     : split_kernel<kKcF32, kNruF32, false>(tpw))
                      : mixed ? col_kernel_of<ColF32, kKcF32, kNruF32>(kind, pipe, tpw)
                              : col_kernel_of<ColF64, kKcF64, kNruF64>(kind, pipe, tpw);
     RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)P.lds));
@@ -879,9 +949,9 @@ int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, void *
   a.ms = P.ms;
   if (P.split) {
     if (P.pipe)
-      launch_col_p<ColBf3, 0, kKcF32, kNruF32, true>(c, a, P.tpw, grid);
+      launch_split_p<kKcF32, kNruF32, true>(c, a, P.tpw, grid);
     else
-      launch_col_p<ColBf3, 0, kKcF32, kNruF32, false>(c, a, P.tpw, grid);
+      launch_split_p<kKcF32, kNruF32, false>(c, a, P.tpw, grid);
   } else if (mixed)
     launch_col_of<ColF32, kKcF32, kNruF32>(c, a, P.kind, P.pipe, P.tpw, grid);
   else

@@ -12,8 +12,9 @@ workloads run eagerly (each q changes shapes).
 Default workload (production size): 64x64x1 k-grid (4096 k), 20 bands
 (8 valence / 12 conduction), ~1500 plane waves per k, 51 local fields,
 701 frequencies over 0-30 eV, eta = 50 meV, q = one k-grid step.  value is
-the mixed precision chain (complex FP32 vertices, BF16 hi/lo split products
-into FP32 for S0, FP64 weights, chunk reduction, KK and Dyson/W); the FP64
+the mixed precision chain (complex FP32 vertices, on this workload as BF16
+hi/lo split products into FP32; BF16 hi/lo split products into FP32 for S0;
+FP64 weights, chunk reduction, KK and Dyson/W); the FP64
 chain is timed on the same inputs next to it, and both are compared with the
 FP64 oracle on a k-subset.
 
@@ -147,7 +148,7 @@ def time_chain(r, steps, warmup):
 
 
 KERNEL_NAME = {  # (precision, stage) -> substrings of the kernel name in the rocprof tables
-    ("mixed", "vertices"): ("k_vertex_col<", "ColF32"), ("fp64", "vertices"): ("k_vertex_col<", "ColF64"),
+    ("mixed", "vertices"): ("k_vertex_col_split<",), ("fp64", "vertices"): ("k_vertex_col<", "ColF64"),
     ("mixed", "accumulate"): ("k_accum_mx<",), ("fp64", "accumulate"): ("k_accum<",),
 }
 
@@ -178,13 +179,16 @@ def hbm_traffic(workload, prec, stage):
 def vertex_kernel(r, prec, lf_idx=None):
     """Name of the vertex kernel the library picks for the runner's local
     fields: the column form when the in-plane classes hold at least 12 fields
-    on average (current vertices: one class only), else the gather-direct
-    FP32 form (Sloss) or the staged form."""
+    on average (current vertices: one class only; mixed charge vertices on
+    the BF16 hi/lo split products), else the gather-direct FP32 form (Sloss)
+    or the staged form."""
     m = r.sys.mill_g[np.asarray(r.lf_idx if lf_idx is None else lf_idx) - 1]
     ncls = len({(int(x[0]), int(x[1])) for x in m})
     optabs = r.rc.program == "optabs"
     if r.rc.vertex_map == "direct" and len(m) >= 12 * ncls and not (optabs and ncls > 1):
-        return "k_vertex_col<ColF32>" if prec == "mixed" else "k_vertex_col<ColF64>"
+        if prec == "fp64":
+            return "k_vertex_col<ColF64>"
+        return "k_vertex_col<ColF32>" if optabs else "k_vertex_col_split"
     if prec == "mixed" and not optabs:
         nc = r.cfg.nband - r.cfg.nval  # Sloss band block: 8 valence rows x up to 16 conduction columns
         return "k_vertex_gd<PrecF32> (two local fields per wave)" if nc <= 8 else "k_vertex_gd<PrecF32>"
@@ -217,8 +221,11 @@ def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None, vbytes
     """Roofline of the dominant matrix-core kernel of one run."""
     acc_ms, vtx_ms = res["stages"]["accumulate"], res["stages"]["vertices"]
     if prec == "mixed":
-        names = ("k_accum_mx (S0, BF16 hi/lo x3 into FP32)", f"{vname or 'k_vertex_col<ColF32>'} (vertices, FP32)")
-        peaks = (BF16X3_PEAK_TFLOPS, FP32_PEAK_TFLOPS)
+        vname = vname or "k_vertex_col_split"
+        split = "col_split" in vname
+        names = ("k_accum_mx (S0, BF16 hi/lo x3 into FP32)",
+                 f"{vname} (vertices, {'BF16 hi/lo x3 into FP32' if split else 'FP32'})")
+        peaks = (BF16X3_PEAK_TFLOPS, BF16X3_PEAK_TFLOPS if split else FP32_PEAK_TFLOPS)
     else:
         names = ("k_accum (S0, FP64)", f"{vname or 'k_vertex_col<ColF64>'} (vertices, FP64)")
         peaks = (FP64_PEAK_TFLOPS, FP64_PEAK_TFLOPS)
