@@ -334,33 +334,28 @@ __device__ __forceinline__ void col_body(const ColArgs &a) {
       // 8 positions per block, K = (position, Re/Im, hi/lo): a lane's B
       // operand is the (hi, lo) word pairs of its 2 positions, two 8-byte
       // LDS reads straight into the operand registers.  A1 = (hi, hi) of C1
-      // against (hi, lo) gives hi.hi + hi.lo, A2 = (lo, 0) gives lo.hi.  The
-      // B values of all tiles are read before the products (the scheduling
-      // barrier keeps the compiler from sinking each read to its product,
-      // which left one LDS latency per tile exposed); the next block's rows
-      // and A parts are read first, since the LDS returns in order.
-      // Positions past the chunk (to the next multiple of 8 <= KC) have zero
-      // A parts and B row 0.  The A row of an Im row is (-Im, Re) of C1: the
-      // halves of each word swapped and the sign of the new low half flipped.
+      // against (hi, lo) gives hi.hi + hi.lo, A2 = (lo, 0) gives lo.hi.
+      // Software pipelined: after a tile's two products its registers take
+      // the next block's B values of that tile, so the reads are in flight
+      // during the other tiles' products (the group barriers keep the
+      // compiler from sinking each read to its product, which left one LDS
+      // latency per tile exposed); rows and A parts are read two blocks
+      // ahead.  Positions past the chunk (to the next multiple of 8 <= KC)
+      // have zero A parts and B row 0; reads past the last block use clamped
+      // positions and are never consumed.  The A row of an Im row is
+      // (-Im, Re) of C1: the halves of each word swapped and the sign of the
+      // new low half flipped.
       const int nblk = (a.chunk[ch + 1] - a.chunk[ch] + 7) >> 3;
       const uint32_t *aph = sAw + band_r * KCA, *apl = aph + 8 * KCA;
       auto rot = [&](uint32_t w) { return im_row ? (__builtin_amdgcn_alignbit(w, w, 16) ^ 0x8000u) : w; };
-      int2 prn = *(const int2 *)&sP[2 * kg];
-      uint2 ahn = *(const uint2 *)&aph[2 * kg];
-      uint2 aln = *(const uint2 *)&apl[2 * kg];
-      for (int t = 0; t < nblk; ++t) {
-        const int2 pr = prn;
-        uint2 ah = ahn, al = aln;
+      auto qpos = [&](int blk) { return min(8 * blk, KC - 8) + 2 * kg; };
+      // block 0: rows, A parts and B values; block 1: rows and A parts
+      uint2 ah = *(const uint2 *)&aph[qpos(0)];
+      uint2 al = *(const uint2 *)&apl[qpos(0)];
+      rpa_u4 b[TPW];
+      {
+        const int2 pr = *(const int2 *)&sP[qpos(0)];
         const int rb0 = pr.x * a.ms + lane_off, rb1 = pr.y * a.ms + lane_off;
-        // the next block's rows and A parts first: the LDS returns in order,
-        // so waiting for the B values below never waits for these
-        {
-          const int q1 = min(8 * (t + 1), KC - 8) + 2 * kg;
-          prn = *(const int2 *)&sP[q1];
-          ahn = *(const uint2 *)&aph[q1];
-          aln = *(const uint2 *)&apl[q1];
-        }
-        rpa_u4 b[TPW];
 #pragma unroll
         for (int tt = 0; tt < TPW; ++tt) {
           const uint2 v0 = *(const uint2 *)&sBw[2 * (rb0 + toff[tt])];
@@ -370,20 +365,40 @@ __device__ __forceinline__ void col_body(const ColArgs &a) {
           b[tt][2] = v1.x;
           b[tt][3] = v1.y;
         }
-        __builtin_amdgcn_sched_barrier(0);
-        ah.x = rot(ah.x);
-        ah.y = rot(ah.y);
-        al.x = rot(al.x);
-        al.y = rot(al.y);
+      }
+      int2 prn = *(const int2 *)&sP[qpos(1)];
+      uint2 ahn = *(const uint2 *)&aph[qpos(1)];
+      uint2 aln = *(const uint2 *)&apl[qpos(1)];
+      for (int t = 0; t < nblk; ++t) {
         rpa_u4 a1, a2;
-        a1[0] = ah.x; a1[1] = ah.x; a1[2] = ah.y; a1[3] = ah.y;
-        a2[0] = al.x; a2[1] = 0u; a2[2] = al.y; a2[3] = 0u;
+        {
+          const uint32_t hx = rot(ah.x), hy = rot(ah.y), lx = rot(al.x), ly = rot(al.y);
+          a1[0] = hx; a1[1] = hx; a1[2] = hy; a1[3] = hy;
+          a2[0] = lx; a2[1] = 0u; a2[2] = ly; a2[3] = 0u;
+        }
         const rpa_bf8 a_1 = __builtin_bit_cast(rpa_bf8, a1), a_2 = __builtin_bit_cast(rpa_bf8, a2);
+        const int rb0 = prn.x * a.ms + lane_off, rb1 = prn.y * a.ms + lane_off;
+        ah = ahn;
+        al = aln;
+        prn = *(const int2 *)&sP[qpos(t + 2)];
+        ahn = *(const uint2 *)&aph[qpos(t + 2)];
+        aln = *(const uint2 *)&apl[qpos(t + 2)];
+        // each tile's products, then the next block's B values of that tile
+        // into the same registers: the reads are in flight during the
+        // remaining tiles' products
 #pragma unroll
         for (int tt = 0; tt < TPW; ++tt) {
           const rpa_bf8 bb = __builtin_bit_cast(rpa_bf8, b[tt]);
           acc[0][tt] = mfma_bf16_16x16x32(a_2, bb, acc[0][tt]);
           acc[0][tt] = mfma_bf16_16x16x32(a_1, bb, acc[0][tt]);
+          const uint2 v0 = *(const uint2 *)&sBw[2 * (rb0 + toff[tt])];
+          const uint2 v1 = *(const uint2 *)&sBw[2 * (rb1 + toff[tt])];
+          b[tt][0] = v0.x;
+          b[tt][1] = v0.y;
+          b[tt][2] = v1.x;
+          b[tt][3] = v1.y;
+          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+          __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
         }
       }
     } else if constexpr (SU == 2) {
