@@ -182,6 +182,30 @@ def test_column_form_multi_class(precision, pipe, monkeypatch):
     r.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fp32"])
+def test_column_form_widest_tiles(precision, monkeypatch):
+    """The column form at the largest tile count per wave: 16 conduction
+    bands (4 groups of matrix-core columns) and 61 local fields along z
+    (16 shift groups), so each wave runs 16 tiles, the largest template
+    instance of every precision.  Vertices against the oracle (FP64 1e-10;
+    mixed split BF16 2e-5, FP32 matrix cores 2e-6 max-norm)."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "col")
+    monkeypatch.setenv("RPA_VERTEX_COL_SPLIT", "0" if precision == "mixed-fp32" else "1")
+    tol = {"fp64": 1e-10, "mixed": 2e-5, "mixed-fp32": 2e-6}[precision]
+    s = H.system("tiny", ncond=16, nlf_target=64, ngk_target=250.0)
+    r = _runner(s, program="sloss", precision=precision.split("-")[0])
+    lf = s.mill_g[np.asarray(r.lf_idx) - 1]
+    assert r.nlf == 61 and len({(int(x[0]), int(x[1])) for x in lf}) == 1
+    nv, nb = s.cfg.nval, s.cfg.nband
+    pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+    ref = orc.vertices(pb, 0, s.gcar, 0, nv, nv, nb)
+    got, _ = r.vertices()
+    err = H.rel_max(got.T, ref)
+    print(f"{precision} N_G={r.nlf} column-form vertices, 16 tiles per wave: rel max err {err:.2e}")
+    assert err < tol
+    r.close()
+
+
 @pytest.mark.parametrize("nlf_target", [60, 70])
 def test_dyson_both_lds_layouts(nlf_target):
     """N_G up to 64 runs the column form (eps in LDS, chi0 and V from L2);
