@@ -21,6 +21,9 @@
 // each walks every position of the chunk, loading the position's A value and
 // B row offset once and reusing them for all of its tiles.  The summation
 // runs over positions in column order instead of the reference's i order.
+// Mixed charge vertices run the same tiles on the BF16 matrix cores as hi/lo
+// split products (ColBf3, k_vertex_col_split); FP64, and FP32 current
+// vertices, on the FP64 / FP32 16x16x4 MFMAs.
 //
 // Current vertices (Optabs, Optabs.f90:1412-1466) use the same correlation:
 // with 0.5 J(G,i) = alpha(G) + beta(i), alpha = 0.5 Gcar (q + 2k + Glf(G))_pol
