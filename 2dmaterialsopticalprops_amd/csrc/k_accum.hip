@@ -488,7 +488,7 @@ struct AccMxArgs {
   const int *items;
   const int *nitems;
   int nlf, nw, ncol, ncol_pad, npb, nib;
-  double *part;
+  float *part;  // FP32 chunk partials (the FP32 accumulators as they are)
 };
 
 // IPW local fields G per wave (4*IPW per workgroup), NCB blocks of 16 G'.
@@ -660,8 +660,9 @@ __global__ __launch_bounds__(256, (NCB <= 4) ? 2 : 1) void k_accum_mx(AccMxArgs 
         }
       }
     }
-    // partials in the FP64 layout of k_accum: part[item][col/256][row][col%256]
-    double *dst = a.part + (size_t)item * a.npb * (kTileW * kColsWG);
+    // partials in the layout of k_accum, part[item][col/256][row][col%256],
+    // in FP32: the reduction reads half the bytes and sums the same values
+    float *dst = a.part + (size_t)item * a.npb * (kTileW * kColsWG);
 #pragma unroll
     for (int q = 0; q < IPW; ++q) {
       const int i = ig0 + i0 + q;
@@ -675,24 +676,25 @@ __global__ __launch_bounds__(256, (NCB <= 4) ? 2 : 1) void k_accum_mx(AccMxArgs 
 #pragma unroll
           for (int s = 0; s < NB; ++s) {
             const int col = CPLX ? 2 * (i * a.nlf + j) + s : i * a.nlf + j;
-            double *d = dst + (size_t)(col / kColsWG) * (kTileW * kColsWG) + col % kColsWG;
+            float *d = dst + (size_t)(col / kColsWG) * (kTileW * kColsWG) + col % kColsWG;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) d[(16 * rb2 + 4 * kg + e) * kColsWG] = (double)acc[s][q][rb2][cb][e];
+            for (int e = 0; e < 4; ++e) d[(16 * rb2 + 4 * kg + e) * kColsWG] = acc[s][q][rb2][cb][e];
           }
         }
     }
     if (ib == 0) {  // padded columns stay zero
       for (int x = tid; x < kTileW * (a.ncol_pad - a.ncol); x += 256) {
         const int row = x / (a.ncol_pad - a.ncol), col = a.ncol + x % (a.ncol_pad - a.ncol);
-        dst[(size_t)(col / kColsWG) * (kTileW * kColsWG) + row * kColsWG + col % kColsWG] = 0.0;
+        dst[(size_t)(col / kColsWG) * (kTileW * kColsWG) + row * kColsWG + col % kColsWG] = 0.f;
       }
     }
   }
 }
 
-// S0[w][col] = sum over the chunks of tile w/32 of their partials (Neumaier).
-__global__ void k_reduce(const double *part, const int *first_item, int npb, int nw, int ncol_pad,
-                         double *s0) {
+// S0[w][col] = sum over the chunks of tile w/32 of their partials (Neumaier,
+// in FP64; the partials are FP64 from k_accum, FP32 from k_accum_mx).
+template <class T>
+__global__ void k_reduce(const T *part, const int *first_item, int npb, int nw, int ncol_pad, double *s0) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)nw * ncol_pad) return;
   const int w = (int)(idx / ncol_pad), col = (int)(idx - (long long)w * ncol_pad);
@@ -701,7 +703,7 @@ __global__ void k_reduce(const double *part, const int *first_item, int npb, int
   const int i0 = first_item[b], i1 = first_item[b + 1];
   double s = 0.0, comp = 0.0;
   for (int it = i0; it < i1; ++it) {
-    double v = part[((size_t)it * npb + pbk) * (kTileW * kColsWG) + row * kColsWG + cc];
+    const double v = (double)part[((size_t)it * npb + pbk) * (kTileW * kColsWG) + row * kColsWG + cc];
     double tt = s + v;
     if (fabs(s) >= fabs(v))
       comp += (s - tt) + v;
@@ -842,7 +844,7 @@ int launch_accumulate(rpa_ctx *c) {
     a.ncol = c->ncol;
     a.ncol_pad = c->ncol_pad;
     a.npb = npb;
-    a.part = c->d_part;
+    a.part = (float *)c->d_part;
     const int e = cplx ? launch_mx_ncb<true>(c, a, (long long)c->max_items)
                        : launch_mx_ncb<false>(c, a, (long long)c->max_items);
     if (e) return e;
@@ -892,9 +894,12 @@ int launch_accumulate(rpa_ctx *c) {
   }
   if (c->profiling) RPA_HIP(c, hipEventRecord(c->prof_ev[3], c->stream));
   const long long n = (long long)c->nw * c->ncol_pad;
-  hipLaunchKernelGGL(k_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                     (const double *)c->d_part, (const int *)c->d_first_item, npb, c->nw,
-                     c->ncol_pad, c->d_s0);
+  if (mixed)
+    hipLaunchKernelGGL(k_reduce<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       (const float *)c->d_part, (const int *)c->d_first_item, npb, c->nw, c->ncol_pad, c->d_s0);
+  else
+    hipLaunchKernelGGL(k_reduce<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       (const double *)c->d_part, (const int *)c->d_first_item, npb, c->nw, c->ncol_pad, c->d_s0);
   if (cplx) {
     const int nn = c->nlf * c->nlf;
     const dim3 g((nn + 255) / 256);
