@@ -137,18 +137,26 @@ __global__ __launch_bounds__(256) void k_tile_hist(const int *key, long long ntr
 }
 
 // Exclusive scan of bcnt (key-major) in place by one workgroup; toff[key] =
-// first sorted position of key, toff[nkeys] = total.  Tiles of 1024
-// consecutive counts (coalesced loads), a shuffle scan per wave and the 16
-// wave totals through LDS; the running total of earlier tiles is carried.
+// first sorted position of key, toff[nkeys] = total.  Tiles of 4096
+// consecutive counts, four per thread (coalesced loads), a shuffle scan of
+// the thread totals per wave and the 16 wave totals through LDS; the running
+// total of earlier tiles is carried.  (One count per thread: 0.040 ms for
+// the 35k counts of a 393k-transition run, the tile loop's latency.)
 __global__ __launch_bounds__(1024) void k_tile_scan(int *bcnt, long long n, int nkeys, int nblk,
                                                      int *toff) {
   __shared__ int wsum[16];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int carry = 0;  // total of the earlier tiles
-  for (long long base = 0; base < n; base += 1024) {
-    const long long i = base + tid;
-    const int v = (i < n) ? bcnt[i] : 0;
-    int x = v;  // inclusive scan within the wave
+  for (long long base = 0; base < n; base += 4096) {
+    const long long i0 = base + 4 * tid;
+    int v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = bcnt[i0 + q < n ? i0 + q : n - 1];  // clamped: no conditional load
+      v[q] = (i0 + q < n) ? c : 0;
+    }
+    const int ts = (v[0] + v[1]) + (v[2] + v[3]);
+    int x = ts;  // inclusive scan of the thread totals within the wave
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int y = __shfl_up(x, off);
@@ -163,10 +171,15 @@ __global__ __launch_bounds__(1024) void k_tile_scan(int *bcnt, long long n, int 
       wpre += (w < wv) ? t : 0;
       total += t;
     }
-    if (i < n) {
-      const int e = carry + wpre + x - v;
-      bcnt[i] = e;
-      if (i % nblk == 0) toff[i / nblk] = e;  // first block of key i / nblk
+    int e = carry + wpre + x - ts;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const long long i = i0 + q;
+      if (i < n) {
+        bcnt[i] = e;
+        if (i % nblk == 0) toff[i / nblk] = e;  // first block of key i / nblk
+      }
+      e += v[q];
     }
     carry += total;
     __syncthreads();  // wsum is rewritten by the next tile
