@@ -485,7 +485,17 @@ __global__ __launch_bounds__(256) void k_mx_split(const float2 *y, long long nt,
   const long long t0 = (long long)blockIdx.x * 64;
   const int nrow = (int)min(64LL, nt - t0);
   const float2 *src = y + t0 * nlf;
-  for (int x = threadIdx.x; x < nrow * nlf; x += 256) ytile[x] = src[x];
+  // eight loads per thread in flight, from clamped addresses (a load per
+  // iteration followed by its LDS store waited a full latency each time)
+  const int n = nrow * nlf;
+  for (int x0 = 0; x0 < n; x0 += 256 * 8) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = src[min(x0 + (int)threadIdx.x + 256 * u, n - 1)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (x0 + (int)threadIdx.x + 256 * u < n) ytile[x0 + threadIdx.x + 256 * u] = v[u];
+  }
   __syncthreads();
   for (int x = threadIdx.x; x < 64 * nlf; x += 256) {
     const int j = x >> 6, tt = x & 63;
