@@ -143,6 +143,7 @@ __device__ inline void split_pair(float2 z, uint32_t &h, uint32_t &l) {
 // wait for each load before the next one issues.)
 template <class P, int KC, int NRU>
 struct ColStage {
+  static_assert(4 * NRU <= 32 && KC / 32 <= 32, "the 32-bit presence masks hold 4 NRU and KC / 32 bits");
   typename P::cplx a[KC / 32];
   typename P::cplx b[NRU][4];
   int prow;
