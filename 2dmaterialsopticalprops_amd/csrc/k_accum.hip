@@ -847,6 +847,31 @@ static int launch_mx_ncb(rpa_ctx *c, AccMxArgs &a, long long items) {
   return 0;
 }
 
+// FP64 k_accum stages kKS vertices of every local field (both vertex sets
+// for Optabs) next to the weight rows: 4352 + 512 Nlf bytes for Optabs, past
+// the default 64 KB from Nlf = 120.  Outside any stream capture (prepare):
+// raise the instance's dynamic-LDS limit to what the current Nlf needs.
+static size_t accum_lds(const rpa_ctx *c) {
+  const bool cplx = (c->d.program == RPA_OPTABS);
+  return sizeof(double) * kTileW * (kKS + 1) + sizeof(double2) * kKS * c->nlf * (cplx ? 2 : 1);
+}
+
+int configure_accumulate(rpa_ctx *c) {
+  if (c->d.precision == RPA_MIXED) return 0;
+  const size_t lds = accum_lds(c);
+  if (lds <= 64 * 1024 || lds <= c->accum_lds_attr) return 0;
+  if (lds > 160 * 1024) return fail(c, RPA_E_LIMIT, "Nlf too large for the accumulation staging");
+  const int ru = (kKS * c->nlf + 255) / 256;
+  const bool cplx = (c->d.program == RPA_OPTABS);
+  const void *fn = cplx ? (ru <= 2 ? (const void *)k_accum<true, 2> : ru <= 4 ? (const void *)k_accum<true, 4>
+                                                                              : (const void *)k_accum<true, 8>)
+                        : (ru <= 2 ? (const void *)k_accum<false, 2> : ru <= 4 ? (const void *)k_accum<false, 4>
+                                                                               : (const void *)k_accum<false, 8>);
+  RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  c->accum_lds_attr = lds;
+  return 0;
+}
+
 int launch_accumulate(rpa_ctx *c) {
   const bool cplx = (c->d.program == RPA_OPTABS);
   const bool mixed = (c->d.precision == RPA_MIXED);
@@ -901,7 +926,9 @@ int launch_accumulate(rpa_ctx *c) {
     a.cut = c->wcut;
     a.den = den;
     a.part = c->d_part;
-    const size_t lds = sizeof(double) * kTileW * (kKS + 1) + sizeof(double2) * kKS * c->nlf * (cplx ? 2 : 1);
+    const size_t lds = accum_lds(c);
+    if (lds > 64 * 1024 && lds > c->accum_lds_attr)
+      return fail(c, RPA_E_STATE, "accumulation LDS limit not configured");
     long long work = (long long)c->max_items * a.npb;
     int grid = (int)std::min<long long>(std::max<long long>((work + 7) / 8 * 8, 8), 4096);
     const int ru = (kKS * c->nlf + 255) / 256;

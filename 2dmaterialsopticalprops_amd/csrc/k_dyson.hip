@@ -474,6 +474,12 @@ int configure_dyson(rpa_ctx *c) {
   return 0;
 }
 
+// Clears the singular-matrix flag ahead of each Dyson launch.  A kernel, not
+// a memset, so a captured chain holds no memset node.
+__global__ void k_clear_status(int *status) {
+  if (threadIdx.x == 0) status[0] = 0;
+}
+
 int launch_dyson(rpa_ctx *c) {
   DysonArgs a{};
   a.nlf = c->nlf;
@@ -490,6 +496,7 @@ int launch_dyson(rpa_ctx *c) {
   if (lds + kDysonStatic > kLdsMax) return fail(c, RPA_E_LIMIT, "Nlf too large for the in-LDS Dyson solve (Nlf <= 81)");
   if (lds > 64 * 1024 && lds > c->dyson_lds_attr[st])
     return fail(c, RPA_E_STATE, "Dyson LDS limit not configured");
+  hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, c->stream, c->d_status);
   if (dyson_stage(c->nlf))
     hipLaunchKernelGGL(k_dyson_col, dim3(a.nfreq), dim3(256), lds, c->stream, a);
   else

@@ -1195,6 +1195,8 @@ static void launch_prec(rpa_ctx *c, int kind, const VertexArgs &a, dim3 grid, si
     launch_kind<P, 2>(c, a, grid, lds);
 }
 
+static int launch_block_group(rpa_ctx *c, VertexArgs a, int kind, bool whole);
+
 int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   VertexArgs a{};
   const bool mixed = (c->d.precision == RPA_MIXED);
@@ -1236,17 +1238,50 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
     }
   }
   a.pairs = c->pairs;
-  int nvb = 0;
+  a.perm = perm;
+  a.rho1 = rho1;
+  a.rho2 = rho2;
+  a.kslots = (c->ntot + 7) / 8 * 8;
+  a.groups = (c->nlf + 3) / 4;
+  if ((long long)a.kslots * a.groups <= 0) return 0;
+  // band blocks of 8 bands n by 16 bands m over every pair block; the kernels
+  // take at most kVertexMaxBlocks of them per launch, so longer lists (many
+  // bands with all pairs: finite T or metals in Optabs) run as several
+  // launches of consecutive blocks of one pair block each
+  std::vector<VBlock> all;
   for (int pbi = 0; pbi < c->pairs.nblocks; ++pbi) {
     const PairBlock &pb = c->pairs.b[pbi];
     for (int n0 = pb.n_lo; n0 < pb.n_hi; n0 += 8)
-      for (int m0 = pb.m_lo; m0 < pb.m_hi; m0 += 16) {
-        if (nvb >= kVertexMaxBlocks)
-          return fail(c, RPA_E_LIMIT, "too many band blocks for the vertex kernel");
-        a.vb[nvb++] = VBlock{n0, std::min(8, pb.n_hi - n0), m0, std::min(16, pb.m_hi - m0), pbi};
-      }
+      for (int m0 = pb.m_lo; m0 < pb.m_hi; m0 += 16)
+        all.push_back(VBlock{n0, std::min(8, pb.n_hi - n0), m0, std::min(16, pb.m_hi - m0), pbi});
   }
-  a.nvb = nvb;
+  const std::string form = vertex_form();
+  const bool col = c->col.ok && (form.empty() || form == "col");
+  if (!col)
+    if (int e = launch_direct_map(c)) return e;
+  const bool whole = all.size() <= (size_t)kVertexMaxBlocks;
+  size_t b0 = 0;
+  while (b0 < all.size()) {
+    size_t b1 = b0;
+    while (b1 < all.size() && b1 - b0 < (size_t)kVertexMaxBlocks && (whole || all[b1].pb == all[b0].pb)) ++b1;
+    VertexArgs ga = a;
+    ga.nvb = (int)(b1 - b0);
+    for (int b = 0; b < ga.nvb; ++b) ga.vb[b] = all[b0 + b];
+    if (col) {
+      if (int e = launch_vertices_col(c, ga.vb, ga.nvb, rho1, rho2, perm)) return e;
+    } else if (int e = launch_block_group(c, ga, kind, whole)) {
+      return e;
+    }
+    b0 = b1;
+  }
+  return 0;
+}
+
+// One launch (or one per pair block) of the gather forms over the band
+// blocks a.vb[0 .. a.nvb); `whole` when they are every block of the q.
+static int launch_block_group(rpa_ctx *c, VertexArgs a, int kind, bool whole) {
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  const int nvb = a.nvb;
   int nlo = c->d.nband, nhi = 0, mlo = c->d.nband, mhi = 0;
   for (int b = 0; b < nvb; ++b) {
     nlo = std::min(nlo, a.vb[b].n0);
@@ -1281,18 +1316,10 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   if (!fits(ch)) return fail(c, RPA_E_LIMIT, "band count too large for the vertex staging");
   a.ch = ch;
   const size_t lds = lds_of(ch);
-  a.groups = (c->nlf + 3) / 4;
-  a.kslots = (c->ntot + 7) / 8 * 8;
-  a.perm = perm;
-  a.rho1 = rho1;
-  a.rho2 = rho2;
   const long long nblocks = (long long)a.kslots * a.groups;
-  if (nblocks <= 0) return 0;
   const std::string form = vertex_form();
-  if (c->col.ok && (form.empty() || form == "col")) return launch_vertices_col(c, a.vb, a.nvb, rho1, rho2, perm);
-  if (int e = launch_direct_map(c)) return e;
   size_t lds_bytes = 0;
-  if (lds_form(c, lds_bytes)) {
+  if (whole && lds_form(c, lds_bytes)) {
     // one workgroup per (k, group of four bands m), all local fields
     const int nmg = (c->pairs.b[0].m_hi - c->pairs.b[0].m_lo + 3) / 4;
     const dim3 g((unsigned)((long long)a.kslots * nmg));

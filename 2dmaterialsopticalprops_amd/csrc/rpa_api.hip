@@ -157,11 +157,11 @@ static int prepare(rpa_ctx *c) {
     e = ensure(c, c->d_dmap, c->cap_dmap, (size_t)c->ng0 * c->nlf * p.ngd);
   if (e) return e;
   RPA_HIP(c, hipMemsetAsync(c->d_qeff, 0, sizeof(double) * 2 * nn, c->stream));
-  RPA_HIP(c, hipMemsetAsync(c->d_status, 0, sizeof(int), c->stream));
   if (int e2 = launch_kstencil(c)) return e2;
   if (c->d.program == RPA_SLOSS)
     if (int e2 = configure_dyson(c)) return e2;
   if (int e2 = configure_vertex(c)) return e2;
+  if (int e2 = configure_accumulate(c)) return e2;
   RPA_HIP(c, hipStreamSynchronize(c->stream));
   c->prepared = true;
   return 0;
@@ -203,6 +203,24 @@ static int collect_stage_times(rpa_ctx *c) {
 
 static int sync(rpa_ctx *c) {
   RPA_HIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// The Dyson kernels raise d_status when a pivot is zero (the reference's
+// gjel prints 'singular matrix in gjel', Chi_RPA/gjel.f:59-61, and goes on
+// with the division).  The flag is cleared by a kernel ahead of every Dyson
+// launch; after a run it is copied to pinned memory on the stream and read
+// once the stream has drained.  The results stay on the device, so a caller
+// can still fetch them after RPA_E_SINGULAR.
+static int queue_status(rpa_ctx *c) {
+  if (c->d.program != RPA_SLOSS) return 0;
+  RPA_HIP(c, hipMemcpyAsync(c->h_status, c->d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+static int check_status(rpa_ctx *c) {
+  if (c->d.program == RPA_SLOSS && c->h_status[0] != 0)
+    return fail(c, RPA_E_SINGULAR, "singular matrix in the Dyson step (eps = I - chi0 V has a zero pivot)");
   return 0;
 }
 
@@ -325,6 +343,8 @@ int rpa_ctx_create(const rpa_dims *dims, rpa_ctx **out) {
     e = hipMalloc((void **)&c->d_evc32, sizeof(float2) * nel * d.nband);
     if (e == hipSuccess) e = hipMemset(c->d_evc32, 0, sizeof(float2) * nel * d.nband);
   }
+  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_status, sizeof(int), hipHostMallocDefault);
+  if (e == hipSuccess) c->h_status[0] = 0;
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_ngk, sizeof(int) * d.nki);
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_E, sizeof(double) * d.nki * d.nband);
   if (e != hipSuccess) {
@@ -359,6 +379,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  if (c->h_status) (void)hipHostFree(c->h_status);
   delete c;
   return RPA_OK;
 }
@@ -403,6 +424,14 @@ int rpa_set_gvectors(rpa_ctx *c, int ngm, const int *mill, const double *kc) {
   RPA_HIP(c, hipMalloc((void **)&c->d_gbox, sizeof(int) * box.size()));
   RPA_HIP(c, hipMemcpy(c->d_mill_g, keys.data(), sizeof(int) * ngm, hipMemcpyHostToDevice));
   RPA_HIP(c, hipMemcpy(c->d_gbox, box.data(), sizeof(int) * box.size(), hipMemcpyHostToDevice));
+  // local fields and the k+q map index the G list: both must be set again
+  // for the new list before the next run
+  c->nlf = 0;
+  c->lf_idx.clear();
+  c->parg.clear();
+  c->have_q = false;
+  c->ng0 = 0;
+  c->g0_list.clear();
   c->prepared = false;
   invalidate_graph(c);
   return RPA_OK;
@@ -613,17 +642,15 @@ int rpa_dyson_w(rpa_ctx *c, double *epsinv_out, double *chi_out, double *W_out) 
   (void)hipSetDevice(c->d.device);
   if (int e = launch_coulomb(c)) return e;
   if (int e = launch_dyson(c)) return e;
+  if (int e = queue_status(c)) return e;
   if (int e = sync(c)) return e;
-  int status = 0;
-  if (int e = d2h(c, &status, (const int *)c->d_status, 1)) return e;
-  if (status) return fail(c, RPA_E_SINGULAR, "singular matrix in the Dyson step");
   if (epsinv_out)
     if (int e = copy_freq_mats(c, c->d_epsinv, epsinv_out)) return e;
   if (chi_out)
     if (int e = copy_freq_mats(c, c->d_chi, chi_out)) return e;
   if (W_out)
     if (int e = copy_freq_mats(c, c->d_W, W_out)) return e;
-  return RPA_OK;
+  return check_status(c);
 }
 
 int rpa_epilogue(rpa_ctx *c, double *loss, double *pi11, double *sig) {
@@ -650,13 +677,14 @@ int rpa_run(rpa_ctx *c) {
   RPA_HIP(c, hipEventRecord(c->ev0, c->stream));
   if (int e = chain(c)) return e;
   RPA_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (int e = queue_status(c)) return e;
   if (int e = sync(c)) return e;
   float ms = 0.f;
   RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   if (c->profiling)
     if (int e = collect_stage_times(c)) return e;
-  return RPA_OK;
+  return check_status(c);
 }
 
 int rpa_load_s0(rpa_ctx *c, const double *S0, const double *qeff) {
@@ -779,11 +807,12 @@ int rpa_run_graph(rpa_ctx *c) {
   RPA_HIP(c, hipEventRecord(c->ev0, c->stream));
   RPA_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
   RPA_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (int e = queue_status(c)) return e;
   if (int e = sync(c)) return e;
   float ms = 0.f;
   RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
-  return RPA_OK;
+  return check_status(c);
 }
 
 int rpa_get_vertices(rpa_ctx *c, double *rho1, double *rho2, long long *ntrans_out) {

@@ -121,6 +121,7 @@ struct rpa_ctx {
   void *d_evc = nullptr;
   float2 *d_evc32 = nullptr;  // mixed precision: FP32 copy of the basis
   size_t dyson_lds_attr[2] = {0, 0};  // dynamic LDS k_dyson<false/true> is allowed
+  size_t accum_lds_attr = 0;          // dynamic LDS the FP64 k_accum instance is allowed
   int *d_mill_k = nullptr;
   int16_t *d_kbox = nullptr;
   int *d_ngk = nullptr;
@@ -208,6 +209,7 @@ struct rpa_ctx {
   double *d_epsinv = nullptr, *d_chi = nullptr, *d_W = nullptr;
   double *d_loss = nullptr, *d_pi11 = nullptr, *d_sig = nullptr;
   int *d_status = nullptr;
+  int *h_status = nullptr;  // pinned copy of d_status after a run
 
   // graph
   hipGraphExec_t graph_exec = nullptr;
@@ -273,6 +275,7 @@ int launch_kk(rpa_ctx *c);
 int launch_coulomb(rpa_ctx *c);
 int launch_dyson(rpa_ctx *c);
 int configure_dyson(rpa_ctx *c);
+int configure_accumulate(rpa_ctx *c);
 int configure_vertex(rpa_ctx *c);
 int launch_epilogue(rpa_ctx *c);
 

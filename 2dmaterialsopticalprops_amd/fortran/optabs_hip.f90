@@ -7,7 +7,13 @@
 !   calc = 3  both, the second part reading the files the first one wrote
 !             (Optabs.f90:513-614), plus Info_{pol}.out.
 !
-! Extra namelist entry: &system vertex_map = 'direct' | 'igk' ['direct'].
+! Extra namelist entries (absent from the reference, defaults in brackets):
+!   &system vertex_map = 'direct' | 'igk' ['direct']
+!   &system precision = 'fp64' | 'mixed'  ['fp64'] arithmetic of the device
+!           chain: 'fp64' throughout, or complex FP32 vertices and BF16 hi/lo
+!           split-product S0 with FP64 weights, reduction, KK and Pi.
+! Each q of calc = 1 / 3 runs as one replay of the captured device chain
+! (rpa_run_graph).
 program optabs_hip
   use iso_c_binding
   use rpa_hip
@@ -16,7 +22,7 @@ program optabs_hip
 
   character(len=100) :: rundir = './', savedir = './', scf_file = 'scf.out', band_file = 'bands.dat'
   character(len=3) :: lf = 'z', pol = 'xx'
-  character(len=8) :: vertex_map = 'direct'
+  character(len=8) :: vertex_map = 'direct', precision = 'fp64'
   integer :: calc = 3, jump = 1, qmin = 1, qmax = 1
   real(dp) :: omin = 0.0_dp, omax = 1.0_dp, eta_intra = 0.01_dp, eta_inter = 0.05_dp, &
               Lor_cut = 1.0e-3_dp, df_cut = 1.0e-3_dp
@@ -26,7 +32,7 @@ program optabs_hip
   integer :: Nthreads = 1
   namelist /directories/ rundir, savedir, scf_file, band_file
   namelist /system/ lf, pol, calc, jump, omin, omax, qmin, qmax, eta_intra, eta_inter, Lor_cut, &
-                    df_cut, vertex_map
+                    df_cut, vertex_map, precision
   namelist /config/ NG, NGd, NkI, Nband, Nval, NelQE, no, Nlfd
   namelist /parameters/ Efermi, dGW, eps, T, Ecut, a0, c0, Vcell
   namelist /parallel/ Nthreads
@@ -97,7 +103,7 @@ program optabs_hip
     call wfc_size(trim(savedir), ik, igwx, nbnd)
     npwx = max(npwx, igwx)
   end do
-  dims = rpa_dims_t(NkI, Nband, npwx, Nlfd, no, RPA_OPTABS, RPA_FP64, 0)
+  dims = rpa_dims_t(NkI, Nband, npwx, Nlfd, no, RPA_OPTABS, rpa_precision_of(precision), 0)
   call rpa_check(rpa_ctx_create(dims, ctx), c_null_ptr, 'rpa_ctx_create')
   call rpa_check(rpa_set_gvectors(ctx, ngm, mill, kc), ctx, 'rpa_set_gvectors')
   allocate(millk(3, npwx), evc(npwx, Nband))
@@ -141,7 +147,7 @@ program optabs_hip
     call rpa_check(rpa_set_q(ctx, ntot, K1, K2, iG0, ktot, q), ctx, 'rpa_set_q')
 
     if (calc /= 2) then
-      call rpa_check(rpa_run(ctx), ctx, 'rpa_run')
+      call rpa_check(rpa_run_graph(ctx), ctx, 'rpa_run_graph')
       write(*, '(a,i0,a,f10.3,a)') ' q ', iq, ': device chain ', rpa_last_run_ms(ctx), ' ms'
       call rpa_check(rpa_get_result(ctx, RPA_RES_S0, c_loc(S0)), ctx, 'S0')
       call rpa_check(rpa_get_result(ctx, RPA_RES_QEFF, c_loc(Qeff)), ctx, 'Qeff')

@@ -9,7 +9,7 @@ module rpa_hip
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: RPA_OK = 0
+  integer(c_int), parameter :: RPA_OK = 0, RPA_E_SINGULAR = 6
   integer(c_int), parameter :: RPA_SLOSS = 0, RPA_OPTABS = 1
   integer(c_int), parameter :: RPA_FP64 = 0, RPA_MIXED = 1
   integer(c_int), parameter :: RPA_MAP_DIRECT = 0, RPA_MAP_IGK = 1
@@ -184,6 +184,21 @@ contains
   end function rpa_last_error
 
   ! Fail-stop on a nonzero return code, as notifications.error does.
+  ! The &system entry precision = 'fp64' | 'mixed' as enum rpa_precision;
+  ! anything else stops with a message.
+  integer(c_int) function rpa_precision_of(name) result(pr)
+    character(len=*), intent(in) :: name
+    select case (trim(adjustl(name)))
+    case ('fp64', 'FP64')
+      pr = RPA_FP64
+    case ('mixed', 'MIXED')
+      pr = RPA_MIXED
+    case default
+      write(*, '(a)') 'ERROR: precision must be ''fp64'' or ''mixed'', got '''//trim(name)//''''
+      error stop 1
+    end select
+  end function rpa_precision_of
+
   subroutine rpa_check(rc, ctx, what)
     integer(c_int), intent(in) :: rc
     type(c_ptr), intent(in) :: ctx

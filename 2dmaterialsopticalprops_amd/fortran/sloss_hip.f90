@@ -10,6 +10,13 @@
 !   &system vertex_map = 'direct' | 'igk'  ['direct'] coefficient-to-G map:
 !           'direct' is the reference drivers' first-NGd convention
 !           (Sloss.f90:907-946), 'igk' the Modules/vertices.f90 _v2 one.
+!   &system precision = 'fp64' | 'mixed'  ['fp64'] arithmetic of the device
+!           chain: 'fp64' throughout, or complex FP32 vertices and BF16 hi/lo
+!           split-product S0 with FP64 weights, reduction, KK and Dyson/W.
+!
+! Each q runs as one replay of the captured device chain (rpa_run_graph).
+! A zero pivot in eps = I - chi0 V prints the reference's 'singular matrix
+! in gjel' (Chi_RPA/gjel.f:59-61) and the run goes on, as the reference's.
 program sloss_hip
   use iso_c_binding
   use rpa_hip
@@ -18,7 +25,7 @@ program sloss_hip
 
   character(len=100) :: rundir = './', savedir = './', scf_file = 'scf.out', band_file = 'bands.dat'
   character(len=3) :: lf = 'z'
-  character(len=8) :: vertex_map = 'direct'
+  character(len=8) :: vertex_map = 'direct', precision = 'fp64'
   integer :: jump = 1, qmin = 1, qmax = 1
   real(dp) :: omin = 0.0_dp, omax = 1.0_dp
   integer :: NkI = 0, Nband = 0, Nocc = 0, NelQE = 0, NGd = 0, NG = 0, no = 0, nq = 1, Nlfd = 50
@@ -26,14 +33,14 @@ program sloss_hip
               eta = 0.05_dp, Ecut = 0.0_dp, Vcell = 1.0_dp
   integer :: Nthreads = 1
   namelist /directories/ rundir, savedir, scf_file, band_file
-  namelist /system/ lf, jump, omin, omax, qmin, qmax, vertex_map
+  namelist /system/ lf, jump, omin, omax, qmin, qmax, vertex_map, precision
   namelist /config/ NkI, Nband, Nocc, NelQE, NGd, NG, no, nq, Nlfd
   namelist /parameters/ Efermi, a0, c0, eps, T, eta, Ecut, Vcell
   namelist /parallel/ Nthreads
 
   real(dp), parameter :: pi = 4.0_dp * atan(1.0_dp)
   character(len=256) :: cfgfile
-  integer :: u, ios, nsym, ngm, ik, iq, io, igwx, nbnd, npwx, ntot, nlf, i
+  integer :: u, ios, nsym, ngm, ik, iq, io, igwx, nbnd, npwx, ntot, nlf, i, rc
   real(dp) :: R(48, 3, 3), kc(3, 3), gcar, domega, nel, q(3), oi, fact
   real(dp) :: kks, skk, wind, windkk, absq, err
   real(dp), allocatable :: ki(:, :), E(:, :), ktot(:, :), g(:, :), glf(:, :), V(:, :), loss(:), &
@@ -90,7 +97,7 @@ program sloss_hip
     call wfc_size(trim(savedir), ik, igwx, nbnd)
     npwx = max(npwx, igwx)
   end do
-  dims = rpa_dims_t(NkI, Nband, npwx, Nlfd, no, RPA_SLOSS, RPA_FP64, 0)
+  dims = rpa_dims_t(NkI, Nband, npwx, Nlfd, no, RPA_SLOSS, rpa_precision_of(precision), 0)
   call rpa_check(rpa_ctx_create(dims, ctx), c_null_ptr, 'rpa_ctx_create')
   call rpa_check(rpa_set_gvectors(ctx, ngm, mill, kc), ctx, 'rpa_set_gvectors')
   allocate(millk(3, npwx), evc(npwx, Nband))
@@ -124,7 +131,12 @@ program sloss_hip
     call write_info(q)
     call rpa_check(rpa_find_kq(ntot, ktot, NkI, ki, ngm, g, q, eps, K1, K2, iG0), ctx, 'findKQinBZ')
     call rpa_check(rpa_set_q(ctx, ntot, K1, K2, iG0, ktot, q), ctx, 'rpa_set_q')
-    call rpa_check(rpa_run(ctx), ctx, 'rpa_run')
+    rc = rpa_run_graph(ctx)
+    if (rc == RPA_E_SINGULAR) then
+      print *, 'singular matrix in gjel'
+    else
+      call rpa_check(rc, ctx, 'rpa_run_graph')
+    end if
     write(*, '(a,i0,a,f10.3,a)') ' q ', iq, ': device chain ', rpa_last_run_ms(ctx), ' ms'
     call rpa_check(rpa_get_result(ctx, RPA_RES_W, c_loc(WT)), ctx, 'W')
     call rpa_check(rpa_get_result(ctx, RPA_RES_V, c_loc(Vt)), ctx, 'V')

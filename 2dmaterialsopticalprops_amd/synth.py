@@ -373,6 +373,7 @@ def write_config(path: str, sys: SynthSystem, program: str, savedir: str,
  Lor_cut = {x.get("lor_cut", 1.0e-3):.17e}
  df_cut = {x.get("df_cut", 1.0e-3):.17e}
  vertex_map = '{x.get("vertex_map", "direct")}'
+ precision = '{x.get("precision", "fp64")}'
 /
 &config
  NG = {len(sys.mill_g)}
@@ -413,6 +414,7 @@ def write_config(path: str, sys: SynthSystem, program: str, savedir: str,
  qmin = {cfg.iq}
  qmax = {cfg.iq}
  vertex_map = '{x.get("vertex_map", "direct")}'
+ precision = '{x.get("precision", "fp64")}'
 /
 &config
  NkI = {nk}

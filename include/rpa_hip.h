@@ -109,7 +109,10 @@ double rpa_last_run_ms(const rpa_ctx *ctx);
 /* ---- inputs ---------------------------------------------------------- */
 /* Global (density) G list in Miller indices mill(3,ngm), G(1) = 0, and the
  * reciprocal axes KC(3,3) in 2pi/a0 units.  Replaces loadG_QE6 + loadKC
- * (Optabs_qe6.f90:1122-1183, Optabs.f90:652-680; Modules/io_qe.f90:11-71). */
+ * (Optabs_qe6.f90:1122-1183, Optabs.f90:652-680; Modules/io_qe.f90:11-71).
+ * Local fields and the k+q map index this list, so after a new list both
+ * must be set again (rpa_set_local_fields, rpa_set_q) before the next run;
+ * until then the device calls return RPA_E_STATE.                          */
 int rpa_set_gvectors(rpa_ctx *ctx, int ngm, const int *mill, const double *kc);
 
 /* Bloch coefficients of IBZ k-point ik (1-based): the wfc{ik}.dat Miller list
@@ -172,7 +175,11 @@ int rpa_epilogue(rpa_ctx *ctx, double *loss, double *pi11, double *sig);
 /* The whole chain for the current q on the context's stream: vertices ->
  * S0 -> KK -> (Sloss: V -> Dyson/W -> loss) / (Optabs: Pi assembly).
  * rpa_run_graph captures the same launches in a hipGraph once per shape and
- * replays it.                                                              */
+ * replays it.  When eps = I - chi0 V has a zero pivot at some frequency
+ * (the reference's gjel prints 'singular matrix in gjel' and divides by it,
+ * Chi_RPA/gjel.f:59-62), the run completes and returns RPA_E_SINGULAR; its
+ * results, non-finite at that frequency, stay readable by rpa_get_result.
+ * rpa_dyson_w reports the same condition the same way.                     */
 int rpa_run(rpa_ctx *ctx);
 int rpa_run_graph(rpa_ctx *ctx);
 

@@ -5,7 +5,8 @@ oracle on the same inputs.
 Tolerances follow the file formats: values written with full precision
 (Loss_Qi es24.15, list-directed doubles) to 1e-10 relative; W_Qi (10F15.5)
 to its 5 decimals; Corrfun/Qeff (2F15.10) to their 1e-10 absolute
-quantisation.
+quantisation.  The mixed-precision runs (namelist precision = 'mixed') are
+held to the mixed chain's bounds: eps^-1 and W to 1e-6, Pi to 5e-6.
 """
 
 import importlib
@@ -143,3 +144,44 @@ def test_optabs_driver_outputs(tmp_path, temp):
     if temp > 0:
         assert H.rel_max(f402[:, 1], sa[1:]) < 1e-10
     assert os.path.exists(run / "Info_xx.out")
+
+
+def test_sloss_driver_mixed_precision(tmp_path):
+    """precision = 'mixed' in &system: the loss and W traces within the mixed
+    chain's bound of the FP64 oracle."""
+    s = H.system("tiny")
+    cfg = s.cfg
+    run = tmp_path / "sloss_mixed"
+    synth.write_all(str(run), s, "sloss", extra={"precision": "mixed"})
+    _run(_exe("sloss_hip.x"), run)
+    dw, S0, V, E, W = _sloss_oracle(s)
+    loss = _table(run / f"Loss_Q{cfg.iq}")
+    assert H.rel_max(loss[:, 1], -E[:, 0, 0].imag) < 1e-6
+    re_w = _table(run / "fort.10008")
+    im_w = _table(run / "fort.20008")
+    assert H.rel_max(re_w[:, 1] + 1j * im_w[:, 1], W[:, 0, 0]) < 1e-6
+
+
+def test_optabs_driver_mixed_precision(tmp_path):
+    s = H.system("tiny")
+    cfg = s.cfg
+    run = tmp_path / "optabs_mixed"
+    synth.write_all(str(run), s, "optabs", extra={"pol": "xx", "calc": 3, "precision": "mixed"})
+    _run(_exe("optabs_hip.x"), run)
+    pb = H.setup(s, "optabs")
+    dw = H.optabs_grid(cfg)
+    S0, Q = orc.optabs_s0(pb, cfg.nval, cfg.no, 1, dw, cfg.eta_ev / H.HARTREE, 1e-3, 1e-3, 0.0, 0.0,
+                          s.vcell, s.gcar)
+    Pi0 = orc.kk_optabs(S0, dw)
+    pi11, si, sa = orc.pi_assembly(Pi0, Q, dw, 0.01 / H.HARTREE, cfg.c)
+    pr = _table(run / "Pi_RPA_xx")
+    assert H.rel_max(pr[:, 1] + 1j * pr[:, 2], pi11) < 5e-6
+
+
+def test_driver_rejects_unknown_precision(tmp_path):
+    s = H.system("tiny")
+    run = tmp_path / "sloss_bad"
+    synth.write_all(str(run), s, "sloss", extra={"precision": "fp16"})
+    r = subprocess.run([_exe("sloss_hip.x")], cwd=run, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "precision must be" in r.stdout + r.stderr
