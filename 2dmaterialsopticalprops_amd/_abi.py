@@ -27,7 +27,8 @@ EXPORTS = [
     "rpa_coulomb", "rpa_dyson_w", "rpa_epilogue", "rpa_run", "rpa_run_graph",
     "rpa_get_vertices", "rpa_gen_fbz", "rpa_find_min_q", "rpa_find_kq", "rpa_gen_glf",
     "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms", "rpa_get_result",
-    "rpa_load_s0", "rpa_w_kk",
+    "rpa_load_s0", "rpa_w_kk", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
+    "rpa_batch_result",
 ]
 # enum rpa_result
 RES = {"S0": 0, "QEFF": 1, "CHI0": 2, "V": 3, "EPSINV": 4, "CHI": 5, "W": 6, "LOSS": 7,
@@ -98,6 +99,10 @@ def lib() -> C.CDLL:
         L.rpa_get_result.argtypes = [P, C.c_int, P]
         L.rpa_load_s0.argtypes = [P, P, P]
         L.rpa_w_kk.argtypes = [P, P, P]
+        L.rpa_set_autotune.argtypes = [P, C.c_int]
+        L.rpa_batch_set.argtypes = [P, C.c_int, P, P, P, C.c_int, P, P, P, P, P]
+        L.rpa_batch_run.argtypes = [P]
+        L.rpa_batch_result.argtypes = [P, C.c_int, C.c_int, P]
         _lib = L
     return _lib
 
@@ -313,6 +318,38 @@ class Context:
         w1, w2 = np.zeros(nf), np.zeros(nf)
         self._c(lib().rpa_w_kk(self.h, _ptr(w1), _ptr(w2)))
         return w1, w2
+
+    def set_autotune(self, on=True):
+        self._c(lib().rpa_set_autotune(self.h, 1 if on else 0))
+
+    def batch_set(self, points, ktot):
+        """points: dicts with lf_idx, parg, K1, K2, iG0 and q (host.Runner.q_point)."""
+        self.batch_nlf = [len(p["lf_idx"]) for p in points]
+        nl = np.ascontiguousarray(self.batch_nlf, dtype=np.int32)
+        li = np.ascontiguousarray(np.concatenate([p["lf_idx"] for p in points]), dtype=np.int32)
+        pg = np.ascontiguousarray(np.concatenate([p["parg"] for p in points]), dtype=np.int32)
+        K1 = np.ascontiguousarray(np.concatenate([p["K1"] for p in points]), dtype=np.int32)
+        K2 = np.ascontiguousarray(np.concatenate([p["K2"] for p in points]), dtype=np.int32)
+        iG0 = np.ascontiguousarray(np.concatenate([p["iG0"] for p in points]), dtype=np.int32)
+        kt = np.ascontiguousarray(ktot, dtype=np.float64)
+        qq = np.ascontiguousarray(np.stack([p["q"] for p in points]), dtype=np.float64)
+        ntot = len(points[0]["K1"])
+        self._c(lib().rpa_batch_set(self.h, len(points), _ptr(nl), _ptr(li), _ptr(pg), ntot, _ptr(K1), _ptr(K2),
+                                    _ptr(iG0), _ptr(kt), _ptr(qq)))
+
+    def batch_run(self):
+        self._c(lib().rpa_batch_run(self.h))
+        return lib().rpa_last_run_ms(self.h)
+
+    def batch_result(self, iq, name):
+        nlf, nf = self.batch_nlf[iq], self.no - 1
+        shapes = {"V": ((nlf, nlf), np.float64), "EPSINV": ((nf, nlf, nlf), np.complex128),
+                  "W": ((nf, nlf, nlf), np.complex128), "LOSS": ((nf,), np.float64),
+                  "QEFF": ((nlf, nlf), np.complex128), "PI11": ((nf,), np.complex128), "SIG": ((nf, 2), np.float64)}
+        shp, dt = shapes[name]
+        out = np.zeros(shp, dtype=dt, order="F")
+        self._c(lib().rpa_batch_result(self.h, iq, RES[name], _ptr(out)))
+        return out
 
     def get_vertices(self, both=False):
         n = C.c_longlong()

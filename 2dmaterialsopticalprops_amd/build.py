@@ -16,6 +16,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "librpa_hip.so")
+HBM_LIB = os.path.join(PKG, "libhbm_copy.so")  # measurement only: reference copy kernel
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
 
@@ -53,6 +54,10 @@ def build_lib(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # the reference copy kernel the benchmark scales HBM counters with
+    tmp = HBM_LIB + ".tmp"
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-x", "hip", os.path.join(CSRC, "hbm_copy.hip"), "-o", tmp])
+    os.replace(tmp, HBM_LIB)
     return LIB
 
 

@@ -16,6 +16,9 @@
 //                loss = -Im eps^-1(1,1)
 //  k_pi        Optabs Pi assembly (Optabs.f90:575-611): Pi_tot(1,1) and the
 //              conductivities of fort.401 / fort.402.
+#include <cstdlib>
+#include <string>
+
 #include "rpa_internal.h"
 
 namespace rpa {
@@ -62,26 +65,31 @@ struct DysonArgs {
   int *status;
 };
 
-// General form (N_G up to the LDS limit): eps and V in LDS, chi0 from L2.
+// General form (N_G > 64): the working matrix eps -> eps^-1 lives in LDS when
+// its 16 N_G^2 bytes fit (N_G <= 96), else in place in the eps^-1 output of
+// the frequency (GLOBAL: every access goes through the L2; the barriers order
+// the workgroup's global accesses as they order its LDS ones).  chi0 and V
+// are read from the L2.
+template <bool GLOBAL>
 __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
-  extern __shared__ double2 sa[];  // [nlf][nlf] eps, then eps^-1; then V (real) behind it
+  extern __shared__ double2 lds_eps[];  // [nlf][nlf] eps, then eps^-1 (LDS form)
   __shared__ double2 fcol[kMaxNlf];
   __shared__ int piv[kMaxNlf];
   __shared__ double red_v[4];
   __shared__ int red_i[4];
   const int N = a.nlf, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double *sv = (double *)(sa + N * N);  // [nlf][nlf] V
+  const double *V = a.V;
   const int f = blockIdx.x;
   const double2 *X0 = a.chi0 + (size_t)f * N * N;
-  for (int x = tid; x < N * N; x += 256) sv[x] = a.V[x];
-  __syncthreads();
+  double2 *Einv = a.epsinv + (size_t)f * N * N;
+  double2 *sa = GLOBAL ? Einv : lds_eps;
   // eps = I - chi0 V
   for (int x = tid; x < N * N; x += 256) {
     int i = x / N, j = x - i * N;
     double2 s = make_double2(i == j ? 1.0 : 0.0, 0.0);
     for (int k = 0; k < N; ++k) {
       double2 c = X0[i * N + k];
-      double v = sv[k * N + j];
+      double v = V[k * N + j];
       s.x -= c.x * v;
       s.y -= c.y * v;
     }
@@ -166,13 +174,12 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
       }
     __syncthreads();
   }
-  double2 *Einv = a.epsinv + (size_t)f * N * N;
   double2 *Ch = a.chi + (size_t)f * N * N;
   double2 *Wo = a.W + (size_t)f * N * N;
   if (tid == 0) a.loss[f] = -sa[0].y;
   for (int x = tid; x < N * N; x += 256) {
     int i = x / N, j = x - i * N;
-    Einv[x] = sa[x];
+    if (!GLOBAL) Einv[x] = sa[x];
     // chi = eps^-1 chi0
     double2 s = make_double2(0.0, 0.0);
     for (int k = 0; k < N; ++k) {
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(256) void k_dyson(DysonArgs a) {
     // W = V + V (chi V) = V eps^-1, since chi V = eps^-1 chi0 V = eps^-1 (I - eps)
     double2 w = make_double2(0.0, 0.0);
     for (int k = 0; k < N; ++k) {
-      double v = sv[i * N + k];
+      double v = V[i * N + k];
       double2 e = sa[k * N + j];
       w.x += v * e.x;
       w.y += v * e.y;
@@ -408,6 +415,312 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
   }
 }
 
+// Mixed-precision Dyson step (N_G <= 64): eps = I - chi0 V in FP64, its
+// inverse X by Gauss-Jordan with row pivoting on the FP32 copy of eps, then
+// one FP64 refinement step
+//     R = I - eps X   (FP64 residual; eps staged from the chi output buffer)
+//     X1 = X + X R    (the correction X R in FP32: R is of the order of the
+//                      FP32 rounding, so its FP32 product errs by ~1e-14)
+// and, when max |R| exceeds kRefineAgain, a second one on X1 (the error
+// after a step is about |R|^2, so 1e-8 at the threshold).  chi = X1 chi0, W = V X1 and the loss follow as
+// in k_dyson_col.  Layout as k_dyson_col: lane = column j, rows wave + 4u.
+// LDS: chi0 of the frequency (FP64) while eps is formed, then X and R
+// (FP32), then X1 (FP64); eps itself waits in the chi output of the
+// frequency, which chi overwrites last.
+constexpr double kRefineAgain = 1.0e-4;
+constexpr int kMxKB = 8;  // eps columns staged per pass of the residual product
+
+__global__ __launch_bounds__(256, 3) void k_dyson_mx(DysonArgs a) {
+  extern __shared__ double2 sa[];  // chi0 [N][N] FP64 | X [N][N], R [N][N] FP32 | X1 [N][N] FP64
+  __shared__ float2 fcol[64];
+  __shared__ int piv[64];
+  __shared__ double2 seps[64][kMxKB];  // eps(:, k0 .. k0+KB-1) of the residual product
+  __shared__ float redmax[4];
+  float2 *sx = (float2 *)sa;
+  const int N = a.nlf, tid = threadIdx.x, j = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int f = blockIdx.x;
+  const double2 *X0 = a.chi0 + (size_t)f * N * N;
+  const double *V = a.V;
+  double2 *Ch = a.chi + (size_t)f * N * N;  // eps until chi is written
+  const bool jok = j < N;
+  const int jc = jok ? j : N - 1;
+  double2 acc[kDyRows];
+  auto ic = [&](int u) { return min(wave + 4 * u, N - 1); };
+
+  // eps = I - chi0 V (chi0 staged to LDS in bursts, V four k ahead), as k_dyson_col
+  {
+    const int nn = N * N;
+    for (int x0 = 0; x0 < nn; x0 += 1024) {
+      double2 t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = X0[min(x0 + tid + 256 * u, nn - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (x0 + tid + 256 * u < nn) sa[x0 + tid + 256 * u] = t[u];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k0 = 0; k0 < N; k0 += 4) {
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = V[min(k0 + q, N - 1) * N + jc];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k0 + q < N) {  // uniform
+#pragma unroll
+        for (int u = 0; u < kDyRows; ++u) {
+          const double2 c = sa[ic(u) * N + k0 + q];
+          acc[u].x += c.x * v[q];
+          acc[u].y += c.y * v[q];
+        }
+      }
+    }
+  }
+  __syncthreads();  // every wave has read chi0
+  if (jok) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (i < N) {
+        const double2 e = make_double2((i == j ? 1.0 : 0.0) - acc[u].x, -acc[u].y);
+        Ch[i * N + j] = e;
+        sx[i * N + j] = make_float2((float)e.x, (float)e.y);
+      }
+    }
+  }
+  __syncthreads();
+
+  // FP32 Gauss-Jordan inversion with row pivoting, as in k_dyson_col
+  for (int k = 0; k < N; ++k) {
+    if (wave == 0) {
+      const float2 z = sx[jc * N + k];
+      const float mine = (j >= k && jok) ? fabsf(z.x) + fabsf(z.y) : -1.0f;
+      float best = mine;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) best = fmaxf(best, __shfl_xor(best, off));
+      int p = __ffsll((unsigned long long)__ballot(mine == best)) - 1;
+      if (!(best > 0.0f)) {  // singular: flag it and go on without a swap
+        if (j == 0) atomicOr(a.status, 1);
+        p = k;
+      }
+      if (j == 0) piv[k] = p;
+      const float2 rk = sx[k * N + jc];
+      const float2 rp = sx[p * N + jc];
+      const float2 pz = sx[p * N + k];
+      const float2 okk = sx[k * N + k];
+      // 1 / pz, scaled
+      const float sc = fmaxf(fabsf(pz.x), fabsf(pz.y));
+      const float pr = pz.x / sc, pi = pz.y / sc, pd = pr * pr + pi * pi;
+      const float2 inv = make_float2(pr / (pd * sc), -pi / (pd * sc));
+      if (jok) {
+        sx[k * N + j] = (j == k) ? inv : make_float2(rp.x * inv.x - rp.y * inv.y, rp.x * inv.y + rp.y * inv.x);
+        if (p != k && j != k) sx[p * N + j] = rk;
+        if (j != k) {
+          const bool sw = j == p;
+          fcol[j] = make_float2(sw ? okk.x : z.x, sw ? okk.y : z.y);
+          sx[j * N + k] = make_float2(0.0f, 0.0f);
+        }
+      }
+    }
+    __syncthreads();
+    {
+      const float2 r = sx[k * N + jc];
+#pragma unroll
+      for (int h = 0; h < kDyRows; h += 8) {
+        float2 fk[8], v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          fk[u] = fcol[ic(h + u)];
+          v[u] = sx[ic(h + u) * N + jc];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          v[u].x -= fk[u].x * r.x - fk[u].y * r.y;
+          v[u].y -= fk[u].x * r.y + fk[u].y * r.x;
+        }
+        if (jok) {
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int i = wave + 4 * (h + u);
+            if (i < N && i != k) sx[i * N + j] = v[u];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges
+  int cj = jc;
+  for (int k = 0; k < N; ++k) {
+    const int pk = piv[k];
+    cj = (cj == k) ? pk : (cj == pk) ? k : cj;
+  }
+  // X (FP64) waits in the eps^-1 output of the frequency, each lane holding
+  // its own entries there (rows wave + 4u, column j), between the phases
+  double2 *Einv = a.epsinv + (size_t)f * N * N;
+  {
+    float2 t[kDyRows];
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) t[u] = sx[ic(u) * N + cj];
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (jok && i < N) Einv[i * N + j] = make_double2((double)t[u].x, (double)t[u].y);
+    }
+  }
+  __syncthreads();  // every wave has read the Gauss-Jordan result
+
+  // refinement steps: R = I - eps X with X read from LDS in FP64, then
+  // X += X R with X and R in FP32 (LDS).  A second step runs when
+  // max |R| > kRefineAgain (uniform across the workgroup).
+  double2 *sxd = sa;            // X in FP64 [N][N] during the residual
+  float2 *sr = sx + N * N;      // R in FP32 [N][N] during the correction
+  for (int step = 0; step < 2; ++step) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (jok && i < N) sxd[i * N + j] = Einv[i * N + j];
+    }
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < N; k0 += kMxKB) {
+      // eps(:, k0 .. k0+KB-1) from the chi buffer into LDS
+      for (int x = tid; x < N * kMxKB; x += 256) {
+        const int i = x / kMxKB, kk = x - i * kMxKB;
+        seps[i][kk] = (k0 + kk < N) ? Ch[i * N + k0 + kk] : make_double2(0.0, 0.0);
+      }
+      __syncthreads();  // (first pass: also X in LDS)
+#pragma unroll 2
+      for (int kk = 0; kk < kMxKB; ++kk) {
+        if (k0 + kk < N) {  // uniform
+          const double2 xv = sxd[(k0 + kk) * N + jc];
+#pragma unroll
+          for (int u = 0; u < kDyRows; ++u) {
+            const double2 e = seps[ic(u)][kk];
+            acc[u].x += e.x * xv.x - e.y * xv.y;
+            acc[u].y += e.x * xv.y + e.y * xv.x;
+          }
+        }
+      }
+      __syncthreads();  // seps and X read before the next pass writes them
+    }
+    // R = I - eps X and X, both in FP32, to LDS; the largest |R| component
+    float rmax = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (jok && i < N) {
+        const double rr = (i == j ? 1.0 : 0.0) - acc[u].x, ri = -acc[u].y;
+        rmax = fmaxf(rmax, fmaxf(fabsf((float)rr), fabsf((float)ri)));
+        sr[i * N + j] = make_float2((float)rr, (float)ri);
+        const double2 xo = Einv[i * N + j];
+        sx[i * N + j] = make_float2((float)xo.x, (float)xo.y);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, off));
+    if (j == 0) redmax[wave] = rmax;
+    __syncthreads();
+    // X += X R (FP32 products, added in FP64), in two halves of the rows
+#pragma unroll
+    for (int h = 0; h < kDyRows; h += 8) {
+      float2 cr[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cr[u] = make_float2(0.0f, 0.0f);
+      for (int k = 0; k < N; ++k) {
+        const float2 rv = sr[k * N + jc];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const float2 xv = sx[ic(h + u) * N + k];
+          cr[u].x += xv.x * rv.x - xv.y * rv.y;
+          cr[u].y += xv.x * rv.y + xv.y * rv.x;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = wave + 4 * (h + u);
+        if (jok && i < N) {
+          double2 xo = Einv[i * N + j];
+          xo.x += (double)cr[u].x;
+          xo.y += (double)cr[u].y;
+          Einv[i * N + j] = xo;
+        }
+      }
+    }
+    const float rall = fmaxf(fmaxf(redmax[0], redmax[1]), fmaxf(redmax[2], redmax[3]));
+    __syncthreads();  // X, R and redmax read by every wave
+    if (!(rall > (float)kRefineAgain)) break;  // uniform
+  }
+
+  // X1 (FP64) to LDS for chi and W
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) {
+    const int i = wave + 4 * u;
+    if (jok && i < N) sa[i * N + j] = Einv[i * N + j];
+  }
+  __syncthreads();
+  if (tid == 0) a.loss[f] = -sa[0].y;
+  // chi = X1 chi0 (chi0 rows read four k ahead)
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k0 = 0; k0 < N; k0 += 4) {
+    double2 c[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[q] = X0[min(k0 + q, N - 1) * N + jc];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (k0 + q < N) {  // uniform
+#pragma unroll
+        for (int u = 0; u < kDyRows; ++u) {
+          const double2 e = sa[ic(u) * N + k0 + q];
+          acc[u].x += e.x * c[q].x - e.y * c[q].y;
+          acc[u].y += e.x * c[q].y + e.y * c[q].x;
+        }
+      }
+    }
+  }
+  __syncthreads();  // eps in the chi buffer is read by nobody any more
+  if (jok) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (i < N) Ch[i * N + j] = acc[u];
+    }
+  }
+  // W = V X1
+  double2 *Wo = a.W + (size_t)f * N * N;
+#pragma unroll
+  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k0 = 0; k0 < N; k0 += 2) {
+    double vv[2][kDyRows];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int u = 0; u < kDyRows; ++u) vv[q][u] = V[ic(u) * N + min(k0 + q, N - 1)];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      if (k0 + q < N) {  // uniform
+        const double2 e = sa[(k0 + q) * N + jc];
+#pragma unroll
+        for (int u = 0; u < kDyRows; ++u) {
+          acc[u].x += vv[q][u] * e.x;
+          acc[u].y += vv[q][u] * e.y;
+        }
+      }
+    }
+  }
+  if (jok) {
+#pragma unroll
+    for (int u = 0; u < kDyRows; ++u) {
+      const int i = wave + 4 * u;
+      if (i < N) Wo[i * N + j] = acc[u];
+    }
+  }
+}
+
 // Optabs Pi assembly per io (Optabs.f90:575-611).  Pi_inter is read after each
 // (iG,jG) update in the reference, so for Nlf > 1 it holds Pi_tot(1,1)
 // including the intraband term (Optabs.f90:594).
@@ -449,28 +762,58 @@ int launch_coulomb(rpa_ctx *c) {
   return 0;
 }
 
-// Done once per context outside any stream capture: the in-LDS solve holds
-// Nlf^2 complex values, beyond the default 64 KB when Nlf > 64.
-// static LDS of k_dyson (fcol, piv, reductions) next to the dynamic part
-constexpr size_t kDysonStatic = (sizeof(double2) + sizeof(int)) * kMaxNlf + 64;
+// Which Dyson kernel serves the current N_G:
+//   N_G <= 64   k_dyson_col (FP64 Gauss-Jordan), or k_dyson_mx (FP32
+//               Gauss-Jordan + FP64 refinement) on the mixed path with
+//               RPA_DYSON=mixed
+//   N_G <= 100  k_dyson<false>: eps in LDS (16 N_G^2 bytes)
+//   N_G <= 128  k_dyson<true>:  eps in place in the eps^-1 output (L2)
+enum DysonForm { kDyCol = 0, kDyMx = 1, kDyLds = 2, kDyGlobal = 3 };
 constexpr size_t kLdsMax = 160 * 1024;
+// static LDS next to the dynamic part: k_dyson (fcol, piv, reductions),
+// k_dyson_mx (fcol, piv, staged eps columns, reductions)
+constexpr size_t kDysonStatic = (sizeof(double2) + sizeof(int)) * kMaxNlf + 64;
+constexpr size_t kDysonMxStatic = (sizeof(float2) + sizeof(int)) * 64 + sizeof(double2) * 64 * kMxKB + 64;
 
-// k_dyson_col (N_G <= 64): eps only in LDS; k_dyson: eps and V
-static bool dyson_stage(int nlf) { return nlf <= 64; }
+// The mixed path keeps the FP64 Gauss-Jordan by default: at N_G = 51 the
+// FP32 Gauss-Jordan with its FP64 refinement took 0.32 ms for the 700
+// frequencies against 0.24 ms (the refinement adds an FP64 product as large
+// as the elimination saves).  RPA_DYSON=mixed selects it.
+static bool mixed_dyson(const rpa_ctx *c) {
+  if (c->d.precision != RPA_MIXED) return false;
+  const char *env = std::getenv("RPA_DYSON");
+  return env && std::string(env) == "mixed";
+}
 
-static size_t dyson_lds(int nlf) {
-  return (dyson_stage(nlf) ? sizeof(double2) : sizeof(double2) + sizeof(double)) * nlf * nlf;
+static int dyson_form(const rpa_ctx *c) {
+  const int n = c->nlf;
+  if (n <= 64) return mixed_dyson(c) ? kDyMx : kDyCol;
+  if (sizeof(double2) * n * n + kDysonStatic <= kLdsMax) return kDyLds;
+  return kDyGlobal;
+}
+
+static size_t dyson_lds(int form, int nlf) {
+  return form == kDyGlobal ? 0 : sizeof(double2) * (size_t)nlf * nlf;
+}
+
+static const void *dyson_kernel(int form) {
+  switch (form) {
+    case kDyCol: return (const void *)k_dyson_col;
+    case kDyMx: return (const void *)k_dyson_mx;
+    case kDyLds: return (const void *)k_dyson<false>;
+    default: return (const void *)k_dyson<true>;
+  }
 }
 
 // Outside any stream capture (prepare): allow the kernel the dynamic LDS of
 // the current N_G when that exceeds the default 64 KB.
 int configure_dyson(rpa_ctx *c) {
-  const size_t lds = dyson_lds(c->nlf);
-  const int st = dyson_stage(c->nlf) ? 1 : 0;
-  if (lds <= 64 * 1024 || lds + kDysonStatic > kLdsMax || lds <= c->dyson_lds_attr[st]) return 0;
-  const void *fn = st ? (const void *)k_dyson_col : (const void *)k_dyson;
-  RPA_HIP(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  c->dyson_lds_attr[st] = lds;
+  const int form = dyson_form(c);
+  const size_t lds = dyson_lds(form, c->nlf);
+  const size_t st = form == kDyMx ? kDysonMxStatic : kDysonStatic;
+  if (lds + st <= 64 * 1024 || lds <= c->dyson_lds_attr[form]) return 0;
+  RPA_HIP(c, hipFuncSetAttribute(dyson_kernel(form), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  c->dyson_lds_attr[form] = lds;
   return 0;
 }
 
@@ -491,16 +834,19 @@ int launch_dyson(rpa_ctx *c) {
   a.W = (double2 *)c->d_W;
   a.loss = c->d_loss;
   a.status = c->d_status;
-  const size_t lds = dyson_lds(c->nlf);
-  const int st = dyson_stage(c->nlf) ? 1 : 0;
-  if (lds + kDysonStatic > kLdsMax) return fail(c, RPA_E_LIMIT, "Nlf too large for the in-LDS Dyson solve (Nlf <= 81)");
-  if (lds > 64 * 1024 && lds > c->dyson_lds_attr[st])
+  if (c->nlf > kMaxNlf) return fail(c, RPA_E_LIMIT, "Nlf too large for the Dyson step");
+  const int form = dyson_form(c);
+  const size_t lds = dyson_lds(form, c->nlf);
+  const size_t st = form == kDyMx ? kDysonMxStatic : kDysonStatic;
+  if (lds + st > 64 * 1024 && lds > c->dyson_lds_attr[form])
     return fail(c, RPA_E_STATE, "Dyson LDS limit not configured");
   hipLaunchKernelGGL(k_clear_status, dim3(1), dim3(64), 0, c->stream, c->d_status);
-  if (dyson_stage(c->nlf))
-    hipLaunchKernelGGL(k_dyson_col, dim3(a.nfreq), dim3(256), lds, c->stream, a);
-  else
-    hipLaunchKernelGGL(k_dyson, dim3(a.nfreq), dim3(256), lds, c->stream, a);
+  switch (form) {
+    case kDyCol: hipLaunchKernelGGL(k_dyson_col, dim3(a.nfreq), dim3(256), lds, c->stream, a); break;
+    case kDyMx: hipLaunchKernelGGL(k_dyson_mx, dim3(a.nfreq), dim3(256), lds, c->stream, a); break;
+    case kDyLds: hipLaunchKernelGGL(k_dyson<false>, dim3(a.nfreq), dim3(256), lds, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_dyson<true>, dim3(a.nfreq), dim3(256), 0, c->stream, a); break;
+  }
   RPA_HIP(c, hipGetLastError());
   return 0;
 }

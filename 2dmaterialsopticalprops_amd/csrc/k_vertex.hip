@@ -1123,10 +1123,11 @@ static void launch_lds_p(rpa_ctx *c, const VertexArgs &a, dim3 grid, size_t lds,
 
 // Whether the LDS-resident kernel serves this context, and its LDS bytes.
 // Vertex form asked for by RPA_VERTEX_FORM ("col", "gather", "staged",
-// "lds"); empty: chosen by shape.
-static std::string vertex_form() {
+// "lds"), else the one the autotuner chose for the current q; empty: chosen
+// by shape.
+static std::string vertex_form(const rpa_ctx *c) {
   const char *env = std::getenv("RPA_VERTEX_FORM");
-  return env ? std::string(env) : std::string();
+  return env ? std::string(env) : c->vform;
 }
 
 static bool lds_form(const rpa_ctx *c, size_t &bytes) {
@@ -1134,7 +1135,7 @@ static bool lds_form(const rpa_ctx *c, size_t &bytes) {
   const PairBlock &pb = c->pairs.b[0];
   if (pb.n_hi - pb.n_lo > 8 || c->nlf > 4 * kLdsMaxGQ) return false;
   // on request only: measured at N_G = 51 slower than the other forms
-  if (vertex_form() != "lds") return false;
+  if (vertex_form(c) != "lds") return false;
   const bool mixed = (c->d.precision == RPA_MIXED);
   const LdsLayout L = mixed ? lds_layout<PrecF32>(c->d.npwx, c->nlf) : lds_layout<PrecF64>(c->d.npwx, c->nlf);
   bytes = L.bytes;
@@ -1143,7 +1144,7 @@ static bool lds_form(const rpa_ctx *c, size_t &bytes) {
 
 // Outside any stream capture: allow the LDS-resident kernel its dynamic LDS.
 int configure_vertex(rpa_ctx *c) {
-  const std::string form = vertex_form();
+  const std::string form = vertex_form(c);
   if (form.empty() || form == "col") {
     if (int e = build_col_plan(c)) return e;
   } else {
@@ -1255,7 +1256,7 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
       for (int m0 = pb.m_lo; m0 < pb.m_hi; m0 += 16)
         all.push_back(VBlock{n0, std::min(8, pb.n_hi - n0), m0, std::min(16, pb.m_hi - m0), pbi});
   }
-  const std::string form = vertex_form();
+  const std::string form = vertex_form(c);
   const bool col = c->col.ok && (form.empty() || form == "col");
   if (!col)
     if (int e = launch_direct_map(c)) return e;
@@ -1308,16 +1309,15 @@ static int launch_block_group(rpa_ctx *c, VertexArgs a, int kind, bool whole) {
     return a.nna <= (256 / ch) * kPfC1 && a.nmb <= (64 / ch) * kPfB && lds_of(ch) <= 64 * 1024;
   };
   int ch = 32;
-  if (const char *env = std::getenv("RPA_VERTEX_CH")) {
-    const int v = std::atoi(env);
-    ch = v >= 32 ? 32 : v >= 16 ? 16 : 8;
-  }
+  const char *env_ch = std::getenv("RPA_VERTEX_CH");
+  const int vch = env_ch ? std::atoi(env_ch) : c->vch;
+  if (vch > 0) ch = vch >= 32 ? 32 : vch >= 16 ? 16 : 8;
   while (ch > 8 && !fits(ch)) ch /= 2;
   if (!fits(ch)) return fail(c, RPA_E_LIMIT, "band count too large for the vertex staging");
   a.ch = ch;
   const size_t lds = lds_of(ch);
   const long long nblocks = (long long)a.kslots * a.groups;
-  const std::string form = vertex_form();
+  const std::string form = vertex_form(c);
   size_t lds_bytes = 0;
   if (whole && lds_form(c, lds_bytes)) {
     // one workgroup per (k, group of four bands m), all local fields
@@ -1373,7 +1373,7 @@ static int launch_block_group(rpa_ctx *c, VertexArgs a, int kind, bool whole) {
       auto sfits = [&](int chs) {
         return sa.nna <= (256 / chs) * kPfC1 && sa.nmb <= (64 / chs) * kPfB && slds(chs) <= 64 * 1024;
       };
-      int sch = std::getenv("RPA_VERTEX_CH") ? ch : 32;  // RPA_VERTEX_CH caps the chunk
+      int sch = vch > 0 ? ch : 32;  // RPA_VERTEX_CH (or the autotuner) caps the chunk
       while (sch > 8 && !sfits(sch)) sch /= 2;
       if (!sfits(sch)) return fail(c, RPA_E_LIMIT, "band count too large for the vertex staging");
       sa.ch = sch;
@@ -1388,6 +1388,74 @@ static int launch_block_group(rpa_ctx *c, VertexArgs a, int kind, bool whole) {
   }
   RPA_HIP(c, hipGetLastError());
   return 0;
+}
+
+}  // namespace rpa
+
+namespace rpa {
+
+// Autotuner of the vertex kernel: on the first q of each shape it times every
+// form that serves the shape -- the column correlation, the FP32
+// gather-direct form (mixed precision), and the staged form with its largest
+// chunk and with chunks of 16 plane waves -- on the q's own transitions and
+// keeps the fastest for the life of the context.  In mixed precision the
+// column form (BF16 split products there) is a candidate only where the
+// shape rule would take it.  The shape key is (local
+// fields, in-plane classes of them, band-pair blocks, program, precision,
+// vertex map); the choice replaces the fixed shape rules of the forms.
+int autotune_vertex(rpa_ctx *c) {
+  std::vector<std::pair<int, int>> cls;
+  for (int g = 0; g < c->nlf; ++g) {
+    const int *m = &c->mill_g[3 * (size_t)(c->lf_idx[g] - 1)];
+    if (std::find(cls.begin(), cls.end(), std::make_pair(m[0], m[1])) == cls.end()) cls.push_back({m[0], m[1]});
+  }
+  std::string key = std::to_string(c->nlf) + "/" + std::to_string(cls.size()) + "/" + std::to_string(c->pairs.tpk) +
+                    "/" + std::to_string(c->pairs.nblocks) + "/" + std::to_string(c->d.program) + "/" +
+                    std::to_string(c->d.precision) + "/" + std::to_string(c->p.vertex_map) + "/" +
+                    std::to_string(c->p.pol);
+  for (const auto &kv : c->vform_cache)
+    if (kv.first == key) {
+      c->vform = kv.second.first;
+      c->vch = kv.second.second;
+      return configure_vertex(c);
+    }
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  std::vector<std::pair<std::string, int>> cand = {{"staged", 0}, {"staged", 16}};
+  // the column form of mixed charge vertices runs BF16 hi/lo split products
+  // (each within 2^-16 of FP32): a candidate only where the shape rule takes
+  // it too (at least 12 local fields per in-plane class), so the tuner never
+  // trades accuracy for speed; in FP64 every form gives the same accuracy
+  if (!mixed || c->nlf >= 12 * (int)cls.size()) cand.insert(cand.begin(), {"col", 0});
+  if (mixed && c->d.program == RPA_SLOSS) cand.insert(cand.begin() + 1, {"gather", 0});
+  if (int e = launch_transitions(c)) return e;
+  float best = 1e30f;
+  std::pair<std::string, int> choice{"", 0};
+  for (const auto &cd : cand) {
+    c->vform = cd.first;
+    c->vch = cd.second;
+    if (int e = configure_vertex(c)) return e;
+    if (cd.first == "col" && !c->col.ok) continue;
+    // one launch to warm up, two timed
+    if (launch_vertices(c, c->d_rho1, c->d_rho2, c->d_perm) != 0) continue;  // the form does not take this shape
+    RPA_HIP(c, hipEventRecord(c->ev0, c->stream));
+    int e = launch_vertices(c, c->d_rho1, c->d_rho2, c->d_perm);
+    if (!e) e = launch_vertices(c, c->d_rho1, c->d_rho2, c->d_perm);
+    RPA_HIP(c, hipEventRecord(c->ev1, c->stream));
+    RPA_HIP(c, hipStreamSynchronize(c->stream));
+    if (e) continue;
+    float ms = 0.f;
+    RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (ms < best) {
+      best = ms;
+      choice = cd;
+    }
+  }
+  c->err.clear();
+  if (choice.first.empty()) return fail(c, RPA_E_LIMIT, "no vertex kernel form takes this shape");
+  c->vform_cache.push_back({key, choice});
+  c->vform = choice.first;
+  c->vch = choice.second;
+  return configure_vertex(c);
 }
 
 }  // namespace rpa

@@ -654,10 +654,11 @@ static bool col_split(bool mixed, int kind) {
   return !(env && env[0] == '0');
 }
 
-// 1 when RPA_VERTEX_FORM=col asks for this form regardless of shape
-static int vertex_form_forced() {
+// 1 when RPA_VERTEX_FORM=col (or the autotuner) asks for this form
+// regardless of shape
+static int vertex_form_forced(const rpa_ctx *c) {
   const char *env = std::getenv("RPA_VERTEX_FORM");
-  return (env && std::string(env) == "col") ? 1 : 0;
+  return (env ? std::string(env) == "col" : c->vform == "col") ? 1 : 0;
 }
 
 static int tpw_of(int tiles) {
@@ -742,13 +743,13 @@ int build_col_plan(rpa_ctx *c) {
   // many classes of few fields (|q+G| sets in all directions) stage a k+q
   // row set per class for little reuse; there the gather forms are faster
   // (measured at 4-9 fields per class)
-  if (c->nlf < 12 * ncls && vertex_form_forced() != 1) return 0;
+  if (c->nlf < 12 * ncls && vertex_form_forced(c) != 1) return 0;
   // current vertices run two (yz: three) accumulator sets where the staged
   // form applies J(G, i) to each term and runs one (two): with local fields
   // in several classes the staged form measured faster (2.2x in FP64, 1.8x
   // in FP32 at N_G = 61-81 in 4-7 classes), with one class the column form
   // (FP64 14.2 vs 16.1 ms at N_G = 31)
-  if (kind >= 1 && ncls > 1 && vertex_form_forced() != 1) return 0;
+  if (kind >= 1 && ncls > 1 && vertex_form_forced(c) != 1) return 0;
   std::vector<int> shift((size_t)ncls * kColMaxShift, -1), info((size_t)ncls * kColInfo, 0);
   int spanmax = 0, padmax = 0, tilemax = 0;
   for (int u = 0; u < ncls; ++u) {

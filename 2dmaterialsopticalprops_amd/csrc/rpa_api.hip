@@ -26,6 +26,42 @@ static void invalidate_graph(rpa_ctx *c) {
   c->graph_valid = false;
 }
 
+// A change of the inputs every q shares (G list, basis, bands, parameters,
+// profiling): the batch graph and every q's preparation are stale.
+static void invalidate_batch(rpa_ctx *c) {
+  if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
+  c->batch_exec = nullptr;
+  if (c->slots)
+    for (rpa_qslot &s : *c->slots) s.prepared = false;
+}
+
+// The device buffers of the current q's fields (a slot swapped in).
+void free_q_buffers(rpa_ctx *c) {
+  void *bufs[] = {c->d_lf_key, c->d_glf, c->d_parg, c->d_k1, c->d_k2, c->d_m0, c->d_ktot, c->d_g0keys,
+                  c->d_g0idx, c->d_dmap, c->d_V, c->d_epsinv, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
+                  c->d_qeff, c->d_status};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  free_col_plan(c);
+  rpa_qslot empty;
+  swap_qslot(c, empty);  // every q field back to its initial value
+}
+
+static void free_slots(rpa_ctx *c) {
+  if (c->batch_exec) (void)hipGraphExecDestroy(c->batch_exec);
+  c->batch_exec = nullptr;
+  if (!c->slots) return;
+  for (rpa_qslot &s : *c->slots) {
+    swap_qslot(c, s);
+    free_q_buffers(c);
+    swap_qslot(c, s);
+  }
+  delete c->slots;
+  c->slots = nullptr;
+  if (c->h_bstatus) (void)hipHostFree(c->h_bstatus);
+  c->h_bstatus = nullptr;
+}
+
 // Which band pairs can carry weight: Sloss n <= Nocc < m (Sloss.f90:355,378);
 // Optabs every (n,m) with |f_n - f_m| >= df_cut or n == m (Optabs.f90:360).
 // When T = 0 and the first Nval bands lie below E_F at every k, those pairs
@@ -62,7 +98,7 @@ static void build_pairs(rpa_ctx *c) {
   pt.tpk = off;
 }
 
-static int prepare(rpa_ctx *c) {
+int prepare(rpa_ctx *c) {
   if (c->prepared) return 0;
   if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors not called");
   for (int k = 0; k < c->d.nki; ++k)
@@ -160,7 +196,11 @@ static int prepare(rpa_ctx *c) {
   if (int e2 = launch_kstencil(c)) return e2;
   if (c->d.program == RPA_SLOSS)
     if (int e2 = configure_dyson(c)) return e2;
-  if (int e2 = configure_vertex(c)) return e2;
+  if (c->autotune && !std::getenv("RPA_VERTEX_FORM")) {
+    if (int e2 = autotune_vertex(c)) return e2;
+  } else if (int e2 = configure_vertex(c)) {
+    return e2;
+  }
   if (int e2 = configure_accumulate(c)) return e2;
   RPA_HIP(c, hipStreamSynchronize(c->stream));
   c->prepared = true;
@@ -171,7 +211,7 @@ static int prepare(rpa_ctx *c) {
 // With profiling on, events mark the stage boundaries (prof_ev[3] is
 // recorded inside launch_accumulate between the matrix-core kernel and the
 // reduction).
-static int chain(rpa_ctx *c) {
+int chain(rpa_ctx *c) {
   const bool pr = c->profiling;
   if (pr) RPA_HIP(c, hipEventRecord(c->prof_ev[0], c->stream));
   if (int e = launch_transitions(c)) return e;
@@ -362,6 +402,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
   (void)hipSetDevice(c->d.device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   invalidate_graph(c);
+  free_slots(c);
   void *bufs[] = {c->d_mill_g, c->d_gbox, c->d_evc, c->d_mill_k, c->d_kbox, c->d_ngk, c->d_E,
                   c->d_lf_key, c->d_glf, c->d_parg, c->d_k1, c->d_k2, c->d_m0, c->d_ktot,
                   c->d_key, c->d_perm, c->d_bcnt, c->d_toff, c->d_nat, c->d_te, c->d_items,
@@ -434,6 +475,7 @@ int rpa_set_gvectors(rpa_ctx *c, int ngm, const int *mill, const double *kc) {
   c->g0_list.clear();
   c->prepared = false;
   invalidate_graph(c);
+  invalidate_batch(c);
   return RPA_OK;
 }
 
@@ -476,6 +518,7 @@ int rpa_upload_basis(rpa_ctx *c, int ik, int ngk, const int *mill, const double 
   c->kbox_ready = false;
   c->prepared = false;
   invalidate_graph(c);
+  invalidate_batch(c);
   return RPA_OK;
 }
 
@@ -487,6 +530,7 @@ int rpa_set_bands(rpa_ctx *c, const double *E) {
   RPA_HIP(c, hipMemcpy(c->d_E, E, sizeof(double) * n, hipMemcpyHostToDevice));
   c->prepared = false;
   invalidate_graph(c);
+  invalidate_batch(c);
   return RPA_OK;
 }
 
@@ -541,6 +585,7 @@ int rpa_set_params(rpa_ctx *c, const rpa_params *p) {
   c->have_params = true;
   c->prepared = false;
   invalidate_graph(c);
+  invalidate_batch(c);
   return RPA_OK;
 }
 
@@ -776,6 +821,7 @@ int rpa_set_profiling(rpa_ctx *c, int on) {
     for (int i = 0; i < 8; ++i) RPA_HIP(c, hipEventCreate(&c->prof_ev[i]));
   c->profiling = on != 0;
   invalidate_graph(c);  // a captured graph would hold the event nodes
+  invalidate_batch(c);
   return RPA_OK;
 }
 
@@ -813,6 +859,159 @@ int rpa_run_graph(rpa_ctx *c) {
   RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
   return check_status(c);
+}
+
+int rpa_set_autotune(rpa_ctx *c, int on) {
+  if (!c) return RPA_E_ARG;
+  c->autotune = on != 0;
+  c->prepared = false;
+  invalidate_graph(c);
+  invalidate_batch(c);
+  return RPA_OK;
+}
+
+int rpa_batch_set(rpa_ctx *c, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot, const int *K1,
+                  const int *K2, const int *iG0, const double *ktot, const double *q) {
+  if (!c || nq <= 0 || !nlf || !lf_idx || !parg || ntot <= 0 || !K1 || !K2 || !iG0 || !ktot || !q)
+    return fail(c, RPA_E_ARG, "rpa_batch_set: bad arguments");
+  (void)hipSetDevice(c->d.device);
+  free_slots(c);
+  c->slots = new std::vector<rpa_qslot>(nq);
+  if (hipHostMalloc((void **)&c->h_bstatus, sizeof(int) * nq, hipHostMallocDefault) != hipSuccess) {
+    free_slots(c);
+    return fail(c, RPA_E_HIP, "rpa_batch_set: pinned status buffer");
+  }
+  size_t off = 0;
+  for (int iq = 0; iq < nq; ++iq) {
+    rpa_qslot &sl = (*c->slots)[iq];
+    swap_qslot(c, sl);
+    int e = rpa_set_local_fields(c, nlf[iq], lf_idx + off, parg + off);
+    if (!e) e = rpa_set_q(c, ntot, K1 + (size_t)iq * ntot, K2 + (size_t)iq * ntot, iG0 + (size_t)iq * ntot, ktot, q + 3 * (size_t)iq);
+    swap_qslot(c, sl);
+    if (e) {
+      const std::string m = c->err;
+      free_slots(c);
+      return fail(c, e, "rpa_batch_set: q " + std::to_string(iq + 1) + ": " + m);
+    }
+    off += (size_t)nlf[iq];
+  }
+  return RPA_OK;
+}
+
+// Prepares every q of the batch (the autotuner picks each shape's vertex
+// form), then captures the chains of all q, one after the other on the
+// context's stream, in one graph.  Shared work buffers are sized for the
+// largest q before the capture starts, so no launch in the graph sees a
+// buffer move.
+static int batch_prepare(rpa_ctx *c) {
+  std::vector<rpa_qslot> &sl = *c->slots;
+  const bool tune = c->autotune;
+  for (size_t iq = 0; iq < sl.size(); ++iq) {
+    swap_qslot(c, sl[iq]);
+    c->autotune = true;
+    int e = prepare(c);
+    c->autotune = tune;
+    swap_qslot(c, sl[iq]);
+    if (e) return fail(c, e, "q " + std::to_string(iq + 1) + " of the batch: " + c->err);
+  }
+  return 0;
+}
+
+// RPA_BATCH_GRAPH=0 runs the prepared q eagerly, one chain after the other,
+// instead of replaying the captured graph (same kernels, same results).
+static bool batch_graph() {
+  const char *env = std::getenv("RPA_BATCH_GRAPH");
+  return !(env && env[0] == '0');
+}
+
+static int batch_capture(rpa_ctx *c) {
+  std::vector<rpa_qslot> &sl = *c->slots;
+  if (int e = batch_prepare(c)) return e;
+  invalidate_graph(c);
+  hipGraph_t g = nullptr;
+  RPA_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int e = 0;
+  for (size_t iq = 0; iq < sl.size() && !e; ++iq) {
+    swap_qslot(c, sl[iq]);
+    e = chain(c);
+    swap_qslot(c, sl[iq]);
+  }
+  hipError_t he = hipStreamEndCapture(c->stream, &g);
+  if (e) {
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+  }
+  if (he != hipSuccess) return hip_fail(c, he, "hipStreamEndCapture");
+  he = hipGraphInstantiate(&c->batch_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (he != hipSuccess) return hip_fail(c, he, "hipGraphInstantiate");
+  return 0;
+}
+
+int rpa_batch_run(rpa_ctx *c) {
+  if (!c) return RPA_E_ARG;
+  if (!c->slots) return fail(c, RPA_E_STATE, "rpa_batch_set must precede rpa_batch_run");
+  if (c->profiling) return fail(c, RPA_E_STATE, "rpa_batch_run runs without stage profiling");
+  (void)hipSetDevice(c->d.device);
+  std::vector<rpa_qslot> &sl = *c->slots;
+  const bool graph = batch_graph();
+  if (graph && !c->batch_exec)
+    if (int e = batch_capture(c)) return e;
+  if (!graph)
+    for (rpa_qslot &q : sl)
+      if (!q.prepared)
+        if (int e = batch_prepare(c)) return e;
+  RPA_HIP(c, hipEventRecord(c->ev0, c->stream));
+  if (graph) {
+    RPA_HIP(c, hipGraphLaunch(c->batch_exec, c->stream));
+  } else {
+    for (size_t iq = 0; iq < sl.size(); ++iq) {
+      swap_qslot(c, sl[iq]);
+      const int e = chain(c);
+      swap_qslot(c, sl[iq]);
+      if (e) return e;
+    }
+  }
+  RPA_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (c->d.program == RPA_SLOSS)
+    for (size_t iq = 0; iq < sl.size(); ++iq)
+      RPA_HIP(c, hipMemcpyAsync(c->h_bstatus + iq, sl[iq].d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (int e = sync(c)) return e;
+  float ms = 0.f;
+  RPA_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  if (c->d.program == RPA_SLOSS)
+    for (size_t iq = 0; iq < sl.size(); ++iq)
+      if (c->h_bstatus[iq])
+        return fail(c, RPA_E_SINGULAR,
+                    "singular matrix in the Dyson step of q " + std::to_string(iq + 1) + " of the batch");
+  return RPA_OK;
+}
+
+int rpa_batch_result(rpa_ctx *c, int iq, int which, double *out) {
+  if (!c || !out) return RPA_E_ARG;
+  if (!c->slots) return fail(c, RPA_E_STATE, "rpa_batch_set and rpa_batch_run must precede rpa_batch_result");
+  if (iq < 0 || iq >= (int)c->slots->size()) return fail(c, RPA_E_ARG, "rpa_batch_result: q index outside the batch");
+  const bool sloss = c->d.program == RPA_SLOSS;
+  const bool ok = sloss ? (which == RPA_RES_V || which == RPA_RES_EPSINV || which == RPA_RES_W || which == RPA_RES_LOSS)
+                        : (which == RPA_RES_QEFF || which == RPA_RES_PI11 || which == RPA_RES_SIG);
+  if (!ok) return fail(c, RPA_E_ARG, "rpa_batch_result: not a per-q result of this program");
+  (void)hipSetDevice(c->d.device);
+  rpa_qslot &sl = (*c->slots)[iq];
+  swap_qslot(c, sl);
+  const size_t nf = (size_t)(c->d.no - 1);
+  int e = 0;
+  switch (which) {
+    case RPA_RES_V: e = get_square(c, (const double *)c->d_V, 1, out); break;
+    case RPA_RES_EPSINV: e = copy_freq_mats(c, c->d_epsinv, out); break;
+    case RPA_RES_W: e = copy_freq_mats(c, c->d_W, out); break;
+    case RPA_RES_LOSS: e = d2h(c, out, (const double *)c->d_loss, nf); break;
+    case RPA_RES_QEFF: e = get_square(c, (const double *)c->d_qeff, 2, out); break;
+    case RPA_RES_PI11: e = d2h(c, out, (const double *)c->d_pi11, 2 * nf); break;
+    default: e = get_sig(c, out); break;
+  }
+  swap_qslot(c, sl);
+  return e;
 }
 
 int rpa_get_vertices(rpa_ctx *c, double *rho1, double *rho2, long long *ntrans_out) {

@@ -35,7 +35,7 @@ int rpa_gcart(int n, const int *mill, const double *kc, double *g) {
   return RPA_OK;
 }
 
-// genFBZ (Optabs.f90:697-759): apply every point operation to every IBZ k and
+// genFBZ (Optabs.f90:871-933, Sloss.f90:697-759): apply every point operation to every IBZ k and
 // keep the distinct results in order of appearance.
 int rpa_gen_fbz(int nki, int nsym, double eps, const double *ki, const double *R, int *ntot,
                 double *ktot) {

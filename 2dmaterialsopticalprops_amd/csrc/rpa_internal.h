@@ -19,6 +19,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/rpa_hip.h"
@@ -120,7 +121,7 @@ struct rpa_ctx {
   // basis
   void *d_evc = nullptr;
   float2 *d_evc32 = nullptr;  // mixed precision: FP32 copy of the basis
-  size_t dyson_lds_attr[2] = {0, 0};  // dynamic LDS k_dyson<false/true> is allowed
+  size_t dyson_lds_attr[4] = {0, 0, 0, 0};  // dynamic LDS each Dyson kernel form is allowed
   size_t accum_lds_attr = 0;          // dynamic LDS the FP64 k_accum instance is allowed
   int *d_mill_k = nullptr;
   int16_t *d_kbox = nullptr;
@@ -158,6 +159,12 @@ struct rpa_ctx {
   std::vector<int> g0_list;  // distinct iG0 (1-based) in order of appearance
   int *d_g0keys = nullptr, *d_g0idx = nullptr, *d_dmap = nullptr;
   rpa::ColPlan col;          // column-correlation vertex plan
+  // vertex form of the current q ("" = shape rules; "col", "gather",
+  // "staged") and staged chunk (0 = largest that fits), set by the autotuner
+  std::string vform;
+  int vch = 0;
+  bool autotune = false;
+  std::vector<std::pair<std::string, std::pair<std::string, int>>> vform_cache;  // shape key -> choice
   size_t cap_dmap = 0;
   double *d_wkk = nullptr;  // W1(no-1), W2(no-1) of the W Kramers-Kronig check
   size_t cap_wkk = 0;
@@ -211,6 +218,12 @@ struct rpa_ctx {
   int *d_status = nullptr;
   int *h_status = nullptr;  // pinned copy of d_status after a run
 
+  // batched q loop (rpa_batch_*): per-q state, exchanged with the fields
+  // above while a q is prepared, captured or read; one graph for all q
+  std::vector<struct rpa_qslot> *slots = nullptr;
+  hipGraphExec_t batch_exec = nullptr;
+  int *h_bstatus = nullptr;  // pinned: each q's singular flag after a batch run
+
   // graph
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
@@ -226,9 +239,38 @@ struct rpa_ctx {
   size_t cap_qeff = 0, cap_status = 0, cap_rho2 = 0;
 };
 
+// The q-dependent fields of a context: local fields, k+q map, vertex plans,
+// the shapes derived from them and the per-q results.
+#define RPA_Q_FIELDS(X)                                                                                      \
+  X(nlf) X(ngp) X(lf_idx) X(parg) X(d_lf_key) X(d_glf) X(d_parg) X(cap_lf_key) X(cap_glf) X(cap_parg) X(ntot)   \
+  X(K1) X(K2) X(iG0) X(ktot) X(d_k1) X(d_k2) X(d_m0) X(d_ktot) X(cap_k1) X(cap_k2) X(cap_m0) X(cap_ktot)          \
+  X(cap_g0keys) X(cap_g0idx) X(have_q) X(ng0) X(g0_list) X(d_g0keys) X(d_g0idx) X(d_dmap) X(col) X(cap_dmap)       \
+  X(vform) X(vch) X(ncol) X(ncol_pad) X(prepared) X(d_V) X(d_epsinv) X(d_W) X(d_loss) X(d_pi11) X(d_sig)          \
+  X(d_qeff) X(d_status) X(cap_V) X(cap_epsinv) X(cap_W) X(cap_loss) X(cap_pi11) X(cap_sig) X(cap_qeff)           \
+  X(cap_status)
+
+struct rpa_qslot {
+#define RPA_QSLOT_FIELD(f) decltype(rpa_ctx::f) f{};
+  RPA_Q_FIELDS(RPA_QSLOT_FIELD)
+#undef RPA_QSLOT_FIELD
+  double q[3] = {0.0, 0.0, 0.0};
+};
+
+// Exchanges the q-dependent fields of the context with those of a slot.
+inline void swap_qslot(rpa_ctx *c, rpa_qslot &s) {
+#define RPA_QSLOT_SWAP(f) std::swap(c->f, s.f);
+  RPA_Q_FIELDS(RPA_QSLOT_SWAP)
+#undef RPA_QSLOT_SWAP
+  for (int a = 0; a < 3; ++a) std::swap(c->q[a], s.q[a]);
+}
+
 namespace rpa {
 
 int fail(rpa_ctx *c, int code, const std::string &msg);
+int prepare(rpa_ctx *c);
+int chain(rpa_ctx *c);
+void free_q_buffers(rpa_ctx *c);
+int autotune_vertex(rpa_ctx *c);
 int hip_fail(rpa_ctx *c, hipError_t e, const char *what);
 
 #define RPA_HIP(ctx, call)                                 \

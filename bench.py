@@ -36,6 +36,20 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+
+def host_threads():
+    """OpenMP threads for the CPU baseline: every core this process may run
+    on (its affinity set), or OMP_NUM_THREADS where the environment sets it
+    (the GPU box sets the host share there).  Threads are bound close."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else n
+
+
+# OpenMP reads these when the oracle's runtime starts (its first parallel region)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 METRIC = "ω-pts/s for χ(q,ω), mixed vs FP64 GPU at ε⁻¹ rel err ≤1e-6; % HBM/MFMA peak"
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
@@ -74,12 +88,16 @@ def cpu_baseline(sys_, r, threads, target_s=12.0):
     s0(problem(nk))
     t_probe = time.perf_counter() - t0
     per_k = t_probe / nk
-    nk_s = int(min(nk_full, max(nk, target_s / max(per_k, 1e-9))))
+    # three timed repetitions of the sample fit the target; the best counts
+    nk_s = int(min(nk_full, max(nk, target_s / 3.0 / max(per_k, 1e-9))))
     nk_s = max(8, (nk_s // 8) * 8)
     pb = problem(nk_s)
-    t0 = time.perf_counter()
-    S0, Q = s0(pb)
-    t_s0 = time.perf_counter() - t0
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        S0, Q = s0(pb)
+        times.append(time.perf_counter() - t0)
+    t_s0 = min(times)
     t0 = time.perf_counter()
     out = {"S0": S0}
     if optabs:
@@ -93,23 +111,23 @@ def cpu_baseline(sys_, r, threads, target_s=12.0):
         out["W"] = np.array([o[2] for o in outs])
     t_tail = time.perf_counter() - t0
     t_full = t_s0 * nk_full / nk_s + t_tail
-    out.update({"t_full_s": t_full, "nk_sample": nk_s, "t_s0_sample": t_s0, "t_tail": t_tail})
+    out.update({"t_full_s": t_full, "nk_sample": nk_s, "t_s0_sample": t_s0, "t_tail": t_tail,
+                "t_s0_runs": times})
     return out
 
 
 def hbm_calibration(mib):
-    """A plain copy of known size through one elementwise kernel (read n bytes,
-    write n bytes), three times, so HBM counters can be scaled against it."""
-    import torch
-    n = mib * (1 << 20) // 4
-    src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        torch.mul(src, 1.0, out=dst)
-    torch.cuda.synchronize()
-    log(f"[bench] hbm calibration: 3 x elementwise copy of {mib} MiB")
-    del src, dst
-    torch.cuda.empty_cache()
+    """A plain copy of known size (read n bytes, write n bytes) through the
+    package's reference copy kernel (libhbm_copy.so, 16-byte loads and
+    stores), three times, so HBM counters can be scaled against it."""
+    import ctypes
+    path = os.path.join(ROOT, "2dmaterialsopticalprops_amd", "libhbm_copy.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+    rc = ctypes.CDLL(path).hbm_copy_calibration(0, mib)
+    if rc != 0:
+        raise RuntimeError(f"hbm_copy_calibration failed with HIP error {rc}")
+    log(f"[bench] hbm calibration: 3 x k_hbm_copy of {mib} MiB")
 
 
 def time_chain(r, steps, warmup):
@@ -206,15 +224,26 @@ def vertex_kernels(r, prec, plans):
 
 def vertex_operand_bytes(r, prec):
     """Bytes the vertex kernel has to read at least once per k (direct map):
-    C1 of the NGd plane waves in the bands n, and the k+q rows of every
-    distinct target G(i) + Glf(G) in the bands m, summed over the FBZ k."""
-    m = r.sys.mill_g[:r.ngd]
-    lf = r.sys.mill_g[np.asarray(r.lf_idx) - 1]
-    ntarget = len(np.unique((m[:, None, :] + lf[None, :, :]).reshape(-1, 3), axis=0))
+    C1 of the NGd plane waves in the bands n, and, in the bands m, the k+q
+    rows that a term uses: the distinct targets G(i) + Glf(G) + G0 whose
+    global index lies within ngk(K2) (the reference adds a term only then,
+    Sloss.f90:928-948), summed over the FBZ k."""
+    mg = r.sys.mill_g
+    index = {tuple(x): i for i, x in enumerate(mg.tolist())}
+    m = mg[:r.ngd]
+    lf = mg[np.asarray(r.lf_idx) - 1]
+    base = (m[:, None, :] + lf[None, :, :]).reshape(-1, 3)
+    ngk2 = r.sys.ngk[np.asarray(r.K2) - 1]
+    rows = 0
+    for g0 in np.unique(r.iG0):
+        t = np.unique(base + mg[g0 - 1][None, :], axis=0)
+        idx = np.sort(np.array([index.get(tuple(x), 1 << 30) for x in t.tolist()]))
+        sel = np.asarray(r.iG0) == g0
+        rows += int(np.searchsorted(idx, ngk2[sel], side="left").sum())
     nv, nb = r.cfg.nval, r.cfg.nband
     nn, nm = (nb, nb) if r.rc.program == "optabs" else (nv, nb - nv)
     es = 8 if prec == "mixed" else 16
-    return float(len(r.K1)) * (r.ngd * nn + ntarget * nm) * es
+    return float(len(r.K1) * r.ngd * nn + rows * nm) * es
 
 
 def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None, vbytes=None):
@@ -270,23 +299,38 @@ def work_model(r, host, cfg, K1, K2, nlf, ntr):
     return nnz, 2.0 * nlf * nlf * nnz, 8.0 * nlf * r.ngd * ntr
 
 
-def time_q_loop(r, plans, steps, warmup):
-    """Warmup passes, then `steps` timed passes of the q loop (per q the
-    context takes the q point -- local fields, k+q map -- and runs the
-    chain), then `steps` passes with stage events for the breakdown."""
+def time_q_loop(r, plans, steps, warmup, eager=False):
+    """The q loop as one batch (rpa_batch_set once, then one replay of the
+    captured graph of every q's chain per pass; vertex forms from the
+    autotuner): warmup passes, then `steps` timed passes; with eager=True
+    instead per q the context takes the q point and runs the chain.  Then
+    `steps` eager passes with stage events for the breakdown."""
     from importlib import import_module
     host = import_module("2dmaterialsopticalprops_amd.host")
     ctx = r.ctx
-    for _ in range(max(warmup, 1)):
-        for pt in plans:
-            r.set_q_point(pt)
-            ctx.run()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        for pt in plans:
-            r.set_q_point(pt)
-            ctx.run()
-    wall = time.perf_counter() - t0
+    if eager:
+        for _ in range(max(warmup, 1)):
+            for pt in plans:
+                r.set_q_point(pt)
+                ctx.run()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            for pt in plans:
+                r.set_q_point(pt)
+                ctx.run()
+        wall = time.perf_counter() - t0
+    else:
+        tb = time.perf_counter()
+        ctx.batch_set(plans, r.ktot)
+        ctx.batch_run()  # prepares every q (autotuner), captures and replays once
+        log(f"[bench] batch of {len(plans)} q set up and captured in {time.perf_counter() - tb:.1f}s")
+        for _ in range(max(warmup, 1) - 1):
+            ctx.batch_run()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.batch_run()
+        wall = time.perf_counter() - t0
+        ctx.set_autotune(True)  # the stage breakdown below on the forms the batch chose
     ctx.set_profiling(True)
     stages = {k: 0.0 for k in host._abi.STAGES}
     dev_ms = 0.0
@@ -318,7 +362,7 @@ def bench_batched(args, s, cfg, synth, host):
         plans[prec] = [r.q_point(iq, nlf) for iq, nlf in spec]
         log(f"[bench] {prec}: setup+upload {t1 - t0:.1f}s, {len(spec)} q points planned "
             f"({time.perf_counter() - t1:.1f}s), N_G {spec[0][1]}..{spec[-1][1]}")
-        res = time_q_loop(r, plans[prec], args.steps, args.warmup)
+        res = time_q_loop(r, plans[prec], args.steps, args.warmup, eager=args.eager_q_loop)
         log(f"[bench] {prec}: {res['ms_step']:.3f} ms per q loop (device {res['device_ms']:.3f} ms); "
             f"stages {json.dumps({k: round(v, 4) for k, v in res['stages'].items()})}")
         runs[prec], runners[prec] = res, r
@@ -361,6 +405,9 @@ def bench_batched(args, s, cfg, synth, host):
                          f"({nv}v/{nb - nv}c), ngk {s.ngk.min()}-{s.ngk.max()}, N_G {spec[0][1]}-{spec[-1][1]} "
                          f"by |q+G|, {cfg.no} omega 0-{cfg.omax_ev:g} eV, eta={cfg.eta_ev * 1e3:g} meV"),
             "precision": head, "q_points": len(spec),
+            "timing": ("eager q loop, q by q (rpa_set_local_fields, rpa_set_q, rpa_run)" if args.eager_q_loop else
+                       "one replay per step of the captured graph of all q (rpa_batch_run; vertex forms by the "
+                       "autotuner)"),
             ("omega_points_per_step" if optabs else "w_solves_per_step"): nsolve, "transitions_per_q": ntr,
             "device_ms_per_step": round(runs[head]["device_ms"], 4),
             "stage_ms": {k: round(v, 4) for k, v in runs[head]["stages"].items()},
@@ -380,7 +427,7 @@ def bench_batched(args, s, cfg, synth, host):
         }
         out["mixed_vs_fp64"] = round(value / fv, 2)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         try:
             # 5 q points spanning the N_G range, each timed on a k-subset,
             # then scaled by sum N_G^2 over all q
@@ -431,6 +478,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--nq", type=int, default=None,
                     help="batched-W workloads: use this many of the q points (spread over the N_G range)")
+    ap.add_argument("--eager-q-loop", action="store_true",
+                    help="batched workloads: run the q loop eagerly, q by q, instead of one graph of all q")
     ap.add_argument("--hbm-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (x1.0 elementwise kernel), "
                          "the reference point for FETCH_SIZE / WRITE_SIZE counter scaling")
@@ -509,7 +558,7 @@ def main():
         }
         out["mixed_vs_fp64"] = round(value / fv, 2)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         try:
             cb = cpu_baseline(s, r, threads, args.cpu_seconds)
             out["cpu_baseline"] = {

@@ -202,6 +202,27 @@ int rpa_load_s0(rpa_ctx *ctx, const double *S0, const double *qeff);
  * returns them.  Needs the W of the last run.                               */
 int rpa_w_kk(rpa_ctx *ctx, double *w1, double *w2);
 
+/* Vertex-kernel autotuner (no counterpart in the reference): with on != 0,
+ * the first run of each shape times every vertex kernel form that takes it
+ * and keeps the fastest for the context; off (the default) chooses by fixed
+ * shape rules.  rpa_batch_run tunes always.                                */
+int rpa_set_autotune(rpa_ctx *ctx, int on);
+
+/* Batched q loop: the loop over iq of Sloss.f90:292-534 (Optabs.f90:291-614)
+ * with nq q points at once.  Per q its local fields, nlf[q] entries of lf_idx
+ * and parg (concatenated over q), and its k+q map, K1/K2/iG0 of q at offset
+ * q*ntot, with q(3,nq); ktot(3,ntot) is shared.  rpa_batch_set copies them
+ * to the device.  rpa_batch_run prepares every q (vertex forms by the
+ * autotuner), captures the chains of all q in one hipGraph on its first call
+ * and replays it; it returns RPA_E_SINGULAR naming the first q with a zero
+ * pivot.  rpa_batch_result copies a per-q result of the last batch run (q
+ * 0-based): RPA_RES_V, _EPSINV, _W, _LOSS (Sloss) or RPA_RES_QEFF, _PI11,
+ * _SIG (Optabs), sizes as for rpa_get_result with that q's Nlf.            */
+int rpa_batch_set(rpa_ctx *ctx, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot,
+                  const int *K1, const int *K2, const int *iG0, const double *ktot, const double *q);
+int rpa_batch_run(rpa_ctx *ctx);
+int rpa_batch_result(rpa_ctx *ctx, int iq, int which, double *out);
+
 /* Diagnostics: vertices of every transition in natural order
  * t = ((k*Npairs) + pair) over the program's band-pair blocks, (Nlf, ntrans)
  * complex; ntrans_out receives the count.  rho2 (pol yz) may be NULL.      */
@@ -217,7 +238,7 @@ int rpa_set_profiling(rpa_ctx *ctx, int on);
 int rpa_stage_ms(const rpa_ctx *ctx, double *ms, int n);
 
 /* ---- host setup helpers (the reference's setup subroutines) ------------ */
-/* genFBZ (Optabs.f90:697-759): R(48,3,3); ktot must hold 3*48*nki doubles.   */
+/* genFBZ (Optabs.f90:871-933, Sloss.f90:697-759): R(48,3,3); ktot holds 3*48*nki. */
 int rpa_gen_fbz(int nki, int nsym, double eps, const double *ki, const double *R,
                 int *ntot, double *ktot);
 /* findMinQ (Optabs.f90:743-773). */

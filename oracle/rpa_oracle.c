@@ -84,7 +84,7 @@ void orc_occupation(double ei, double ef, double T, double *expo, double *f) {
   }
 }
 
-/* a3: FBZ generation, Optabs.f90:697-759 (genFBZ, Sloss.f90 same).  R is the
+/* a3: FBZ generation, Optabs.f90:871-933 (genFBZ; Sloss.f90:697-759 the same).  R is the
  * reference's R(48,3,3).  Writing fbz_check.dat is left to the caller.       */
 int orc_gen_fbz(int nki, int nsym, double eps, const double *ki, const double *R,
                 int *ntot, double *ktot) {

@@ -1,0 +1,118 @@
+"""The batched q loop (rpa_batch_set / rpa_batch_run / rpa_batch_result):
+every q of the loop in one captured graph, each q with its own local fields
+and k+q map, vertex forms chosen by the autotuner.
+
+Checks: every q's eps^-1, W and loss against the FP64 oracle (1e-10) and,
+with one vertex form forced for both, bitwise against single-q runs of the
+same context; graph replays with eager work between them give the first
+replay's results bit for bit; the Optabs route's per-q Pi_11.
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+
+import helpers as H
+from oracle import oracle as orc
+
+host = importlib.import_module("2dmaterialsopticalprops_amd.host")
+abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
+pytestmark = pytest.mark.gpu
+
+QS = [(2, 9), (3, 14), (4, 19)]  # (iq, N_G) on the 4x4 grid, local fields by |q+G|
+
+
+def _runner(s, **kw):
+    return host.Runner(s, host.RunConfig(**kw))
+
+
+def _sloss_oracle(s, r, pt):
+    cfg = s.cfg
+    pb = orc.Problem(s, pt["K1"], pt["K2"], pt["iG0"], r.ktot, pt["q"], pt["lf_idx"], map_mode=0, ngd=r.ngd,
+                     E_hartree=r.E)
+    S0 = orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, cfg.eta_ev / H.HARTREE, s.vcell)
+    chi0 = orc.kk_sloss(S0, r.domega)
+    glf = r.g[pt["lf_idx"] - 1]
+    V = orc.genV(r.rc.eps, pt["q"][0] * s.gcar, pt["q"][1] * s.gcar, cfg.c, pt["parg"], glf, s.gcar)
+    outs = [orc.dyson_w(chi0[f], V) for f in range(cfg.no - 1)]
+    return np.array([o[0] for o in outs]), np.array([o[2] for o in outs])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_sloss_q_loop_against_oracle_and_replays(precision):
+    s = H.system("tiny", lf="xyz", nlf_target=20)
+    r = _runner(s, program="sloss", precision=precision, iq=QS[0][0])
+    pts = [r.q_point(iq, n) for iq, n in QS]
+    ctx = r.ctx
+    ctx.batch_set(pts, r.ktot)
+    ctx.batch_run()
+    first = [(ctx.batch_result(j, "EPSINV"), ctx.batch_result(j, "W"), ctx.batch_result(j, "LOSS"))
+             for j in range(len(pts))]
+    tol = 1e-10 if precision == "fp64" else 3e-6
+    for j, pt in enumerate(pts):
+        E, W = _sloss_oracle(s, r, pt)
+        e, w, loss = first[j]
+        assert e.shape == E.shape
+        assert H.rel_max(e, E) < tol and H.rel_max(w, W) < tol
+        assert H.rel_max(loss, -E[:, 0, 0].imag) < tol
+    # replays with the context's own eager work between them
+    for _ in range(2):
+        r.set_q_point(pts[1])
+        ctx.run()
+        ctx.batch_run()
+        for j in range(len(pts)):
+            assert np.array_equal(ctx.batch_result(j, "EPSINV"), first[j][0])
+            assert np.array_equal(ctx.batch_result(j, "W"), first[j][1])
+    r.close()
+
+
+@pytest.mark.parametrize("form", ["staged", "col"])
+def test_batch_equals_single_runs_bitwise(form, monkeypatch):
+    """With the vertex form forced for both, a batch q gives bitwise what a
+    single-q run of the same context gives."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", form)
+    s = H.system("tiny", lf="xyz", nlf_target=20)
+    r = _runner(s, program="sloss", iq=QS[0][0])
+    pts = [r.q_point(iq, n) for iq, n in QS]
+    ctx = r.ctx
+    ctx.batch_set(pts, r.ktot)
+    ctx.batch_run()
+    for j, pt in enumerate(pts):
+        r.set_q_point(pt)
+        ctx.run()
+        assert np.array_equal(ctx.batch_result(j, "W"), ctx.result("W"))
+        assert np.array_equal(ctx.batch_result(j, "LOSS"), ctx.result("LOSS"))
+    r.close()
+
+
+def test_batch_optabs_route():
+    s = H.system("tiny", lf="xyz", nlf_target=20)
+    cfg = s.cfg
+    r = _runner(s, program="optabs", pol="xx", iq=QS[0][0])
+    pts = [r.q_point(iq, n) for iq, n in QS]
+    ctx = r.ctx
+    ctx.batch_set(pts, r.ktot)
+    ctx.batch_run()
+    for j, pt in enumerate(pts):
+        pb = orc.Problem(s, pt["K1"], pt["K2"], pt["iG0"], r.ktot, pt["q"], pt["lf_idx"], map_mode=0,
+                         ngd=r.ngd, E_hartree=r.E)
+        S0, Q = orc.optabs_s0(pb, cfg.nval, cfg.no, 1, r.domega, cfg.eta_ev / H.HARTREE, r.rc.lor_cut,
+                              r.rc.df_cut, 0.0, 0.0, s.vcell, s.gcar)
+        pi0 = orc.kk_optabs(S0, r.domega)
+        pi11, si, _ = orc.pi_assembly(pi0, Q, r.domega, r.rc.eta_intra_ev / H.HARTREE, cfg.c)
+        assert H.rel_max(ctx.batch_result(j, "PI11"), pi11) < 1e-10
+        assert H.rel_max(ctx.batch_result(j, "SIG")[:, 0], si) < 1e-10
+    r.close()
+
+
+def test_autotune_single_q_matches_oracle():
+    s = H.system("cfg0")
+    r = _runner(s, program="sloss")
+    r.ctx.set_autotune(True)
+    out = r.full()
+    pb = orc.Problem(s, r.K1, r.K2, r.iG0, r.ktot, r.q, r.lf_idx, map_mode=0, ngd=r.ngd, E_hartree=r.E)
+    cfg = s.cfg
+    S0 = orc.sloss_s0(pb, cfg.nval, cfg.no, r.domega, cfg.eta_ev / H.HARTREE, s.vcell)
+    assert H.rel_max(out["S0"], S0) < 1e-10
+    r.close()

@@ -206,14 +206,16 @@ def test_column_form_widest_tiles(precision, monkeypatch):
     r.close()
 
 
-@pytest.mark.parametrize("nlf_target", [60, 70])
-def test_dyson_both_lds_layouts(nlf_target):
+@pytest.mark.parametrize("nlf_target", [60, 70, 90, 118])
+def test_dyson_lds_and_global_forms(nlf_target):
     """N_G up to 64 runs the column form (eps in LDS, chi0 and V from L2);
-    larger N_G (to the in-LDS limit of 81) the general form with eps and V in
-    LDS.  Both against the oracle."""
+    up to 100 the general form with eps in LDS; beyond (to the 128 fields
+    the kernels take; the reference's gjel allows 1000) the general form
+    with eps in place in the eps^-1 output.  All against the oracle."""
     s = H.system("tiny", lf="xyz", nlf_target=nlf_target)
     r = _runner(s, program="sloss")
-    assert (r.nlf <= 64) == (nlf_target == 60) and r.nlf <= 81
+    form = "col" if r.nlf <= 64 else "lds" if r.nlf <= 100 else "global"
+    assert form == {60: "col", 70: "lds", 90: "lds", 118: "global"}[nlf_target], r.nlf
     out = r.full()
     S0, chi0, E = _sloss_ref(s, r)
     V = orc.genV(r.rc.eps, r.q[0] * s.gcar, r.q[1] * s.gcar, s.cfg.c, r.parg, r.glf, s.gcar)

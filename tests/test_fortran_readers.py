@@ -86,3 +86,60 @@ def test_numbered_file_names(dump_exe, tmp_path):
                     "-o", exe], check=True)
     out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
     assert out == ["W_Q7", "Kramers-Kron_Q12"]
+
+
+BAND_FIXTURE = os.path.join(os.path.dirname(__file__), "golden", "MoS2_head.band")
+REF_BAND = "/root/reference/W/MoS2.band"
+
+
+def _free_parse(path, nbnd):
+    """k points and energies of a bands.x file by whitespace splitting: a
+    reading that shares nothing with the drivers' fixed-format one."""
+    toks = open(path).read().split("/", 1)[1].split()
+    vals = np.array([float(t) for t in toks])
+    per = 3 + nbnd
+    recs = vals.reshape(-1, per)
+    return recs[:, :3], recs[:, 3:]
+
+
+def _read_bands_fortran(exe_dir, path, nki, nband):
+    src = exe_dir / "rb.f90"
+    src.write_text(
+        "program rb\n use rpa_files\n implicit none\n character(len=512) :: p\n"
+        " integer :: nki, nb, u\n character(len=16) :: a\n real(dp), allocatable :: ki(:,:), E(:,:)\n"
+        " call get_command_argument(1, p)\n call get_command_argument(2, a); read(a, *) nki\n"
+        " call get_command_argument(3, a); read(a, *) nb\n allocate(ki(3, nki), E(nki, nb))\n"
+        " call load_ki_and_e(trim(p), nki, nb, nb, 0.0_dp, ki, E)\n"
+        " open(newunit=u, file='bands.bin', access='stream', form='unformatted', status='replace')\n"
+        " write(u) ki, E\n close(u)\nend program\n")
+    fb = os.path.join(build.BUILD, "fortran")
+    exe = str(exe_dir / "rb.x")
+    subprocess.run([build.FLANG, "-module-dir", fb, "-I", fb, str(src), os.path.join(fb, "rpa_files.o"),
+                    "-o", exe], check=True)
+    subprocess.run([exe, path, str(nki), str(nband)], check=True, cwd=exe_dir, capture_output=True)
+    raw = np.fromfile(exe_dir / "bands.bin", dtype=np.float64)
+    ki = raw[:3 * nki].reshape(nki, 3)
+    E = raw[3 * nki:].reshape(nki, nband, order="F") * H.HARTREE
+    return ki, E
+
+
+def test_band_reader_on_the_reference_band_file(dump_exe, tmp_path):
+    """W/MoS2.band (bands.x, nbnd = 60): the k lines '(10X,3F10.3)' and the
+    energies '(10F8.4)' of Optabs.f90:1333-1339 / Sloss.f90:1506-1511.  The
+    committed fixture holds its first 20 records and its last one; the whole
+    file is read too where the reference tree is present."""
+    ki_w, e_w = _free_parse(BAND_FIXTURE, 60)
+    ki, E = _read_bands_fortran(tmp_path, BAND_FIXTURE, len(ki_w), 60)
+    np.testing.assert_array_equal(ki, ki_w)
+    np.testing.assert_allclose(E, e_w, rtol=1e-15, atol=0)
+    # first record: Gamma, lowest band -14.716 eV, highest 27.421 eV
+    assert ki[0].tolist() == [0.0, 0.0, 0.0] and E[0, 0] == pytest.approx(-14.716, abs=1e-12)
+    assert E[0, 59] == pytest.approx(27.421, abs=1e-12)
+    assert ki[-1].tolist() == [-0.328358, -0.574478, 0.0]
+    assert E[-1, 59] == pytest.approx(30.112, abs=1e-12)
+    if os.path.exists(REF_BAND):
+        kf, ef = _free_parse(REF_BAND, 60)
+        assert len(kf) == 6835
+        kr, er = _read_bands_fortran(tmp_path, REF_BAND, len(kf), 60)
+        np.testing.assert_array_equal(kr, kf)
+        np.testing.assert_allclose(er, ef, rtol=1e-15, atol=0)

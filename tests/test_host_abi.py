@@ -163,3 +163,36 @@ def test_batched_q_points():
     pts = synth.batched_q_points("cfg4")
     assert len(pts) == 24 and pts[0] == (2, 61) and pts[-1] == (25, 81)
     assert synth.BATCHED_Q["cfg4"]["program"] == "optabs"
+
+
+def _lib_gen_fbz(ki, Rs, eps=1e-4):
+    import ctypes as C
+    R = np.zeros((48, 3, 3), order="F")
+    for i, r in enumerate(Rs):
+        R[i] = r
+    kt = np.zeros((48 * len(ki), 3))
+    nt = C.c_int()
+    kk = np.ascontiguousarray(ki, dtype=np.float64)
+    rc = abi.lib().rpa_gen_fbz(len(ki), len(Rs), eps, kk.ctypes.data, R.ctypes.data, C.byref(nt), kt.ctypes.data)
+    assert rc == 0
+    return kt[:nt.value].copy()
+
+
+def test_gen_fbz_and_find_min_q_match_the_pinned_oracle_on_symmetric_inputs():
+    """The cases the oracle is pinned on (mirror-unfolded half grid, sixfold
+    orbits, equal shortest vectors) give the library's host entry points the
+    same answers, bit for bit."""
+    import math
+    kc = synth.reciprocal_axes(4.65, 20.0)
+    n = 8
+    P = np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1]], dtype=np.float64)
+    M = kc @ P @ np.linalg.inv(kc)
+    half = np.array([kc @ np.array([i / n, j / n, 0.0]) for i in range(n) for j in range(n) if i <= j])
+    rots = [np.array([[math.cos(t * math.pi / 3), -math.sin(t * math.pi / 3), 0],
+                      [math.sin(t * math.pi / 3), math.cos(t * math.pi / 3), 0], [0, 0, 1.0]]) for t in range(6)]
+    import test_oracle as T
+    for ki, Rs in ((half, [np.eye(3), M]), (np.array([[0.13, 0.05, 0.0], [0.0, 0.0, 0.0]]), rots)):
+        np.testing.assert_array_equal(_lib_gen_fbz(ki, Rs), T._gen_fbz(ki, Rs))
+    k = np.array([[0, 0, 0], [0.3, 0, 0], [0.1, 0, 0], [0, 0.1, 0]], dtype=np.float64)
+    for order in ([0, 1, 2, 3], [0, 1, 3, 2]):
+        np.testing.assert_array_equal(abi.find_min_q(k[order], 4), orc.find_min_q(k[order], 4))

@@ -12,6 +12,8 @@ Each oracle routine is checked against something that does not restate it:
   - identities: eps^-1 = I + chi V, Hermiticity, LAPACK inverses.
 """
 
+import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -472,3 +474,139 @@ def test_pi_assembly_single_field_closed_form():
     np.testing.assert_allclose(si[1:], (-1j * c0 * inter[1:] / w[1:]).real, rtol=1e-13)
     np.testing.assert_allclose(sa[1:], (-1j * c0 * intra[1:] / w[1:]).real, rtol=1e-13)
     assert si[0] == 0.0 and sa[0] == 0.0
+
+
+# ---------------------------------------------------------------------------
+# findMinQ, genFBZ and the Pi assembly at N_G >= 2: pinned by properties of
+# the k-grid, of the point group and of the reference's statement order.
+
+def _mp_grid(n, kc):
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    kcrys = np.stack([ii.ravel() / n, jj.ravel() / n, np.zeros(n * n)], axis=1)
+    return np.round(H.synth.gcart(kcrys, kc), 6)
+
+
+@pytest.mark.parametrize("n", [4, 12, 48, 64])
+def test_find_min_q_is_the_shortest_grid_step(n):
+    """Optabs.f90:743-773: q = (iq-1) k*, k* the nonzero FBZ vector of least
+    norm (below 1 in 2pi/a0 units), the first in list order among equals.  On
+    an N x N grid of the hexagonal cell the shortest steps are b1/N, b2/N and
+    (b1+b2)/N, all of norm |b|/N."""
+    kc = H.synth.reciprocal_axes(4.65, 20.0)
+    k = _mp_grid(n, kc)
+    norms = np.sqrt(np.einsum("ij,ij->i", k, k))
+    b = np.linalg.norm(kc[:, 0])
+    steps = np.round(np.stack([kc[:, 0], kc[:, 1], kc[:, 0] + kc[:, 1]]) / n, 6)
+    for iq in (1, 2, 5):
+        q = orc.find_min_q(k, iq)
+        if iq == 1:
+            assert not q.any()
+            continue
+        ks = q / (iq - 1)
+        assert abs(np.linalg.norm(ks) - b / n) < 2e-6
+        assert any(np.allclose(ks, s_, rtol=0, atol=1e-12) for s_ in steps)
+        nz = norms[norms > 0]
+        assert np.linalg.norm(ks) <= nz.min() + 1e-15           # no shorter vector exists
+        first = int(np.flatnonzero(norms == np.linalg.norm(ks))[0])
+        np.testing.assert_array_equal(ks, k[first])              # the first of equal norm
+
+
+def test_find_min_q_takes_the_first_of_equal_vectors_in_list_order():
+    k = np.array([[0, 0, 0], [0.3, 0, 0], [0.1, 0, 0], [0, 0.1, 0], [0.5, 0.5, 0]], dtype=np.float64)
+    np.testing.assert_array_equal(orc.find_min_q(k, 3), [0.2, 0, 0])
+    np.testing.assert_array_equal(orc.find_min_q(k[[0, 1, 3, 2, 4]], 3), [0, 0.2, 0])
+    # vectors of norm 1 or more never qualify (kmin starts at 1.0)
+    k2 = np.array([[0, 0, 0], [1.0, 0, 0], [0.9, 0, 0]])
+    np.testing.assert_array_equal(orc.find_min_q(k2, 2), [0.9, 0, 0])
+
+
+def _gen_fbz(ki, Rs, eps=1e-4):
+    nki = len(ki)
+    R = np.zeros((48, 3, 3), order="F")
+    for i, r in enumerate(Rs):
+        R[i] = r
+    kt = np.zeros((3, 48 * nki), order="F")
+    nt = C.c_int()
+    kif = np.asfortranarray(np.asarray(ki, dtype=np.float64).T)
+    rc = orc.lib().orc_gen_fbz(nki, len(Rs), eps, kif, R, C.byref(nt), kt)
+    assert rc == 0
+    return kt[:, :nt.value].T.copy()
+
+
+def test_gen_fbz_identity_keeps_the_list_and_drops_repeats():
+    """Optabs.f90:871-933 with Nsym = 1: the IBZ list itself, in order; a
+    point within eps of an earlier one appears once."""
+    kc = H.synth.reciprocal_axes(4.65, 20.0)
+    k = _mp_grid(6, kc)
+    np.testing.assert_array_equal(_gen_fbz(k, [np.eye(3)]), k)
+    dup = np.vstack([k[:5], k[2:3] + 5e-5, k[5:]])
+    np.testing.assert_array_equal(_gen_fbz(dup, [np.eye(3)]), k)
+
+
+def test_gen_fbz_mirror_unfolds_half_a_grid_to_the_whole_grid():
+    """The swap of crystal coordinates (b1 <-> b2) is a mirror of the
+    hexagonal lattice.  From the half grid i <= j it must give every point of
+    the N x N grid exactly once: the half first, in order, then the mirror
+    images of its off-diagonal points in the same order."""
+    n = 8
+    kc = H.synth.reciprocal_axes(4.65, 20.0)
+    P = np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1]], dtype=np.float64)
+    M = kc @ P @ np.linalg.inv(kc)
+    pairs = [(i, j) for i in range(n) for j in range(n) if i <= j]
+    half = np.array([kc @ np.array([i / n, j / n, 0.0]) for i, j in pairs])
+    fbz = _gen_fbz(half, [np.eye(3), M])
+    full = np.array([kc @ np.array([i / n, j / n, 0.0]) for i in range(n) for j in range(n)])
+    assert len(fbz) == n * n
+    d = np.abs(fbz[:, None, :] - full[None, :, :]).max(axis=2)
+    assert (d.min(axis=1) < 1e-12).all() and (d.min(axis=0) < 1e-12).all()
+    np.testing.assert_array_equal(fbz[:len(half)], half)
+    off = [p for p, (i, j) in enumerate(pairs) if i != j]
+    np.testing.assert_allclose(fbz[len(half):], half[off] @ M.T, rtol=0, atol=1e-15)
+
+
+def test_gen_fbz_orbit_sizes_of_the_sixfold_rotations():
+    """Six rotations about z: a generic k has six distinct images, Gamma one."""
+    Rs = []
+    for t in range(6):
+        c, s_ = math.cos(t * math.pi / 3), math.sin(t * math.pi / 3)
+        Rs.append(np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1.0]]))
+    assert len(_gen_fbz(np.array([[0.13, 0.05, 0.0]]), Rs)) == 6
+    assert len(_gen_fbz(np.array([[0.0, 0.0, 0.0]]), Rs)) == 1
+    assert len(_gen_fbz(np.array([[0.0, 0.0, 0.0], [0.2, 0.0, 0.0]]), Rs)) == 7
+
+
+def test_pi_assembly_two_fields_reads_pi11_after_each_update():
+    """Optabs.f90:575-603 at Nlf = 2: Pi_inter = Pi_tot(1,1) is read after
+    every (iG, jG) update, and Pi_tot(1,1) gets its intraband term at the
+    first update, so fort.401 holds Re(-i c0 [Pi0 - Re Pi0(0) + Qeff
+    omega/(omega + i eta)]_11 / omega), intraband included (at Nlf = 1 it is
+    read before that term).  A pure-Python walk of the reference's loop
+    order gives the same."""
+    rng = np.random.default_rng(12)
+    no, dw, eta_i, c0 = 10, 0.02, 0.005, 15.0
+    pi0 = rng.standard_normal((no - 1, 2, 2)) + 1j * rng.standard_normal((no - 1, 2, 2))
+    qeff = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+    pi11, si, sa = orc.pi_assembly(pi0, qeff, dw, eta_i, c0)
+    w = np.arange(no - 1) * dw
+    intra = qeff[0, 0] * w / (w + 1j * eta_i)
+    tot11 = pi0[:, 0, 0] - pi0[0, 0, 0].real + intra
+    np.testing.assert_allclose(pi11, tot11, rtol=1e-14, atol=1e-15)
+    np.testing.assert_allclose(si[1:], c0 * tot11[1:].imag / w[1:], rtol=1e-13)
+    np.testing.assert_allclose(sa[1:], c0 * intra[1:].imag / w[1:], rtol=1e-13)
+    # the single-field assembly of the same (1,1) data leaves intra out of si
+    _, s1, _ = orc.pi_assembly(pi0[:, :1, :1].copy(), qeff[:1, :1].copy(), dw, eta_i, c0)
+    np.testing.assert_allclose(s1[1:], c0 * (tot11 - intra)[1:].imag / w[1:], rtol=1e-13)
+    # the reference's loop nest, statement by statement
+    dia = np.zeros((2, 2), dtype=complex)
+    for io in range(1, no):
+        oi = (io - 1) * dw
+        tot = np.zeros((2, 2), dtype=complex)
+        for iG in range(2):
+            for jG in range(2):
+                if io == 1:
+                    dia[iG, jG] = -complex(pi0[0, iG, jG].real, 0.0)
+                tot[iG, jG] = pi0[io - 1, iG, jG] + dia[iG, jG]
+                p_inter = tot[0, 0]
+                tot[iG, jG] = tot[iG, jG] + qeff[iG, jG] * oi / (oi + 1j * eta_i)
+        if io > 1:
+            assert si[io - 1] == pytest.approx((-1j * c0 * p_inter / oi).real, rel=1e-13)

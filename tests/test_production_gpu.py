@@ -15,8 +15,10 @@ the FP64 oracle:
   36x36 k, 40 bands, ~3200 plane waves, 2001 spectral bins -> KK to 999
       omega: two q of the Optabs q loop (N_G 61 and 81 by |q+G|), both
       precisions, on the first 32 k of the grid (the oracle's complex S0 at
-      N_G = 81 costs ~1e10 flop per k); Pi_11 to 1e-10 (FP64) and 5e-6
-      (mixed).
+      N_G = 81 costs ~1e10 flop per k); Pi_11 and sigma to 1e-10 (FP64) and
+      1e-5 (mixed: the FP32 current vertices carry errors of 1.5e-6 whose
+      J-weighted terms largely cancel, and 32 k average them little; measured
+      Pi_11 1.9e-6, sigma 5.4e-6, S0 4.1e-6).
 
 Size-independent identities are checked on the full-grid FP64 results:
 eps^-1 = I + chi V and W = V + V chi V (= V eps^-1).
@@ -37,7 +39,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 TOL_EPS_MIXED = 1e-6
 TOL_S0_MIXED = 5e-6
-TOL_PI_MIXED = 5e-6
+TOL_PI_MIXED = 1e-5
 
 
 def _runner(s, **kw):
