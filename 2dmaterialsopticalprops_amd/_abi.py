@@ -27,7 +27,7 @@ EXPORTS = [
     "rpa_coulomb", "rpa_dyson_w", "rpa_epilogue", "rpa_run", "rpa_run_graph",
     "rpa_get_vertices", "rpa_gen_fbz", "rpa_find_min_q", "rpa_find_kq", "rpa_gen_glf",
     "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms", "rpa_get_result",
-    "rpa_load_s0", "rpa_w_kk", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
+    "rpa_load_s0", "rpa_w_kk", "rpa_kk_noalloc", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
     "rpa_batch_result",
 ]
 # enum rpa_result
@@ -99,6 +99,7 @@ def lib() -> C.CDLL:
         L.rpa_get_result.argtypes = [P, C.c_int, P]
         L.rpa_load_s0.argtypes = [P, P, P]
         L.rpa_w_kk.argtypes = [P, P, P]
+        L.rpa_kk_noalloc.argtypes = [P, C.c_void_p]
         L.rpa_set_autotune.argtypes = [P, C.c_int]
         L.rpa_batch_set.argtypes = [P, C.c_int, P, P, P, C.c_int, P, P, P, P, P]
         L.rpa_batch_run.argtypes = [P]

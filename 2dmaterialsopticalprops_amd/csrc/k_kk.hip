@@ -11,6 +11,9 @@
 //      are added in the epilogue.
 // Each stencil entry is evaluated independently with the reference's chain,
 // so the matrix is exactly the coefficients the reference multiplies with.
+#include <mutex>
+#include <string>
+
 #include "rpa_internal.h"
 
 namespace rpa {
@@ -214,6 +217,60 @@ int launch_kk(rpa_ctx *c) {
                      c->d.program, no, c->nlf, c->ncol_pad, (const double *)c->d_x,
                      (const double *)c->d_s0, c->d_chi0);
   RPA_HIP(c, hipGetLastError());
+  return 0;
+}
+
+// ---- Kramers-Kronig without allocation (rpa_kk_noalloc) ------------------
+// The stencil K and the product X = K S0 of this call live in a file-scope
+// workspace that every call on a device shares, instead of the context's
+// buffers; the stencil is rebuilt on each call, since another context (or
+// another shape) may have used the workspace since.  Calls that overlap on
+// one device race on it.
+constexpr size_t kNoAllocDoubles = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices];
+
+// Once per device, with that device current: the device address of the
+// workspace (the array's name is only a host-side stand-in for it).
+int noalloc_init_device(int device) {
+  if (device < 0 || device >= kMaxDevices) return RPA_E_LIMIT;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return 0;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return RPA_E_HIP;
+  g_scratch[device] = (double *)ptr;
+  return 0;
+}
+
+int launch_kk_noalloc(rpa_ctx *c, double *chi0_dev) {
+  const int dev = c->d.device;
+  double *ws = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ws = (dev >= 0 && dev < kMaxDevices) ? g_scratch[dev] : nullptr;
+  }
+  if (!ws) return fail(c, RPA_E_STATE, "rpa_kk_noalloc: no workspace address for this device");
+  const int no = c->d.no, nw = c->nw;
+  const size_t nk = (size_t)(no - 1) * nw, nx = (size_t)(no - 1) * c->ncol_pad;
+  if (nk + nx > kNoAllocDoubles)
+    return fail(c, RPA_E_LIMIT, "rpa_kk_noalloc: stencil and product need " + std::to_string((nk + nx) * 8 >> 20) +
+                                    " MiB, more than the 64 MiB workspace");
+  double *kmat = ws, *x = ws + (nk + 31) / 32 * 32;
+  if ((size_t)(x - ws) + nx > kNoAllocDoubles)
+    return fail(c, RPA_E_LIMIT, "rpa_kk_noalloc: stencil and product exceed the 64 MiB workspace");
+  hipLaunchKernelGGL(k_kstencil, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, c->stream, c->d.program, no, nw,
+                     c->p.domega, kmat);
+  dim3 grid((c->ncol_pad + 63) / 64, (no - 1 + 63) / 64);
+  hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, c->stream, no - 1, c->ncol_pad, nw, (const double *)kmat,
+                     (const double *)c->d_s0, x);
+  const long long n = (long long)(no - 1) * c->nlf * c->nlf;
+  hipLaunchKernelGGL(k_chi0_epilogue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d.program, no,
+                     c->nlf, c->ncol_pad, (const double *)x, (const double *)c->d_s0, chi0_dev);
+  RPA_HIP(c, hipGetLastError());
+  RPA_HIP(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 

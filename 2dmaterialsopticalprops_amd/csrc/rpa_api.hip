@@ -383,6 +383,7 @@ int rpa_ctx_create(const rpa_dims *dims, rpa_ctx **out) {
     e = hipMalloc((void **)&c->d_evc32, sizeof(float2) * nel * d.nband);
     if (e == hipSuccess) e = hipMemset(c->d_evc32, 0, sizeof(float2) * nel * d.nband);
   }
+  if (e == hipSuccess && noalloc_init_device(d.device) != 0) e = hipErrorInvalidDevice;
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_status, sizeof(int), hipHostMallocDefault);
   if (e == hipSuccess) c->h_status[0] = 0;
   if (e == hipSuccess) e = hipMalloc((void **)&c->d_ngk, sizeof(int) * d.nki);
@@ -669,6 +670,13 @@ int rpa_kk(rpa_ctx *c, double *chi0_out) {
   if (int e = sync(c)) return e;
   if (chi0_out) return copy_freq_mats(c, c->d_chi0, chi0_out);
   return RPA_OK;
+}
+
+int rpa_kk_noalloc(rpa_ctx *c, double *chi0_dev) {
+  if (!c || !chi0_dev) return fail(c, RPA_E_ARG, "rpa_kk_noalloc: null argument");
+  if (!c->d_s0) return fail(c, RPA_E_STATE, "rpa_accumulate_s0 (or rpa_run, rpa_load_s0) must precede rpa_kk_noalloc");
+  (void)hipSetDevice(c->d.device);
+  return launch_kk_noalloc(c, chi0_dev);
 }
 
 int rpa_coulomb(rpa_ctx *c, double *V_out) {

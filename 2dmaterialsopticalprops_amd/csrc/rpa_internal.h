@@ -320,6 +320,8 @@ int configure_dyson(rpa_ctx *c);
 int configure_accumulate(rpa_ctx *c);
 int configure_vertex(rpa_ctx *c);
 int launch_epilogue(rpa_ctx *c);
+int noalloc_init_device(int device);
+int launch_kk_noalloc(rpa_ctx *c, double *chi0_dev);
 
 }  // namespace rpa
 

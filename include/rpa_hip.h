@@ -156,6 +156,19 @@ int rpa_accumulate_s0(rpa_ctx *ctx, double *S0_out, double *qeff_out);
  * chi0_out (may be NULL): (no-1,Nlf,Nlf) complex for io = 1..no-1.         */
 int rpa_kk(rpa_ctx *ctx, double *chi0_out);
 
+/* rpa_kk without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * Kramers-Kronig transform of the context's resident S0, written to the
+ * caller's device buffer chi0_dev as [no-1][Nlf][Nlf] complex (row-major in
+ * iG, jG; not the reference layout), stream-synchronised before returning.
+ * The stencil and the product K*S0 live in a 64 MiB workspace of the
+ * library, one per device, shared by every call: shapes needing more,
+ * ((no-1)*nw + (no-1)*ncol) doubles with nw = no (Sloss) or 2no+1 (Optabs)
+ * frequency rows and ncol the padded S0 columns, return RPA_E_LIMIT, and
+ * calls that overlap on one device race.                                  */
+int rpa_kk_noalloc(rpa_ctx *ctx, double *chi0_dev);
+
 /* Slab Coulomb matrix V(Nlf,Nlf) for the current q (genV, Sloss.f90:1173-1208). */
 int rpa_coulomb(rpa_ctx *ctx, double *V_out);
 
