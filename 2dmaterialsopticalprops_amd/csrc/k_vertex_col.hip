@@ -35,6 +35,8 @@
 //
 // Workgroup = (k, in-plane class, band block); workgroups of one k on one XCD.
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cstdlib>
 #include <numeric>
 #include <string>
@@ -47,7 +49,9 @@ constexpr int kColInfo = 2 + kColMaxShift / 4;  // per class: rowoff, nsg, sg li
 constexpr size_t kColLdsBudget = 80 * 1024; // default: two workgroups per CU
 
 struct ColArgs {
-  const void *evc;
+  const void *box[2];  // basis arrays, rows of bw[g] bands (column-box or global order)
+  int bw[2], blo1, nbg;  // band widths, first band of array 1, arrays in use
+  int lim_ngk;           // 1: rows past ngk(K2) read as absent (global order); 0: stored as zero
   const int *ngk, *k1, *k2, *g0idx;
   const int *slot;   // [npos] C1 slot (global index) of position p
   const int *prow;   // [ncls][npos] B row of p relative to its chunk
@@ -187,10 +191,12 @@ __device__ __forceinline__ void col_body(const ColArgs &a) {
   const VBlock vb = a.vb[rest / a.ncls];
 
   const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
-  const int lim2 = a.ngk[K2];
-  const cplx *evc = (const cplx *)a.evc;
-  const cplx *c1 = evc + (size_t)K1 * a.npwx * a.nband + vb.n0;
-  const cplx *c2 = evc + (size_t)K2 * a.npwx * a.nband + vb.m0;
+  const int lim2 = a.lim_ngk ? a.ngk[K2] : 0x7fffffff;
+  const int g1 = (a.nbg == 2 && vb.n0 >= a.blo1) ? 1 : 0;
+  const int g2 = (a.nbg == 2 && vb.m0 >= a.blo1) ? 1 : 0;
+  const int s1 = a.bw[g1], s2 = a.bw[g2];  // row strides of the C1 and C2 arrays
+  const cplx *c1 = (const cplx *)a.box[g1] + (size_t)K1 * a.npwx * s1 + (vb.n0 - (g1 ? a.blo1 : 0));
+  const cplx *c2 = (const cplx *)a.box[g2] + (size_t)K2 * a.npwx * s2 + (vb.m0 - (g2 ? a.blo1 : 0));
   const int *info = a.cls + (size_t)cl * kColInfo;
   const int *bmap = a.bmap + (size_t)a.g0idx[k] * a.nrows + info[0];
   const int *prow = a.prow + (size_t)cl * a.npos;
@@ -224,7 +230,7 @@ __device__ __forceinline__ void col_body(const ColArgs &a) {
     unsigned am = 0;
 #pragma unroll
     for (int u = 0; u < KC / 32; ++u) {
-      st.a[u] = c1[(size_t)slot_u[u] * a.nband + nn];
+      st.a[u] = c1[(size_t)slot_u[u] * s1 + nn];
       am |= (nok && sa_p + 32 * u < np) ? (1u << u) : 0u;
     }
     st.amask = am;
@@ -239,7 +245,7 @@ __device__ __forceinline__ void col_body(const ColArgs &a) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const bool eok = sb_m + 4 * e < vb.ncols;
-          st.b[u][e] = c2[(size_t)row * a.nband + (eok ? sb_m + 4 * e : 0)];
+          st.b[u][e] = c2[(size_t)row * s2 + (eok ? sb_m + 4 * e : 0)];
           bm |= (ok && eok) ? (1u << (4 * u + e)) : 0u;
         }
       }
@@ -680,6 +686,107 @@ void free_col_plan(rpa_ctx *c) {
   P = ColPlan{};
 }
 
+// Column-box basis (see rpa_ctx::d_box): row R of k holds the coefficients of
+// global G index gidx[R] when that index is below ngk(k), else zero.
+template <class T>
+__global__ void k_build_box_k(const T *evc, const int *ngk, const int *gidx, int npwx, int nband, int nrow, int nbg,
+                              int blo1, T *box0, T *box1) {
+  const int k = blockIdx.y;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nrow * nband) return;
+  const int R = idx / nband, b = idx - R * nband;
+  const int g = gidx[R];
+  T v;
+  if (g >= 0 && g < ngk[k]) {
+    v = evc[((size_t)k * npwx + g) * nband + b];
+  } else {
+    v.x = 0;
+    v.y = 0;
+  }
+  if (nbg == 2 && b >= blo1)
+    box1[((size_t)k * npwx + R) * (nband - blo1) + (b - blo1)] = v;
+  else
+    box0[((size_t)k * npwx + R) * (nbg == 2 ? blo1 : nband) + b] = v;
+}
+
+// Builds the column-box basis when it is missing or its band split no longer
+// matches the band blocks; leaves box_ready false (the plan then indexes the
+// global order) when the columns' z ranges need more rows than npwx.
+static int ensure_box(rpa_ctx *c) {
+  const int nb = c->d.nband, nv = c->p.nval;
+  bool two = nv > 0 && nv < nb;
+  for (int i = 0; i < c->pairs.nblocks && two; ++i) {
+    const PairBlock &b = c->pairs.b[i];
+    const bool n_ok = b.n_hi <= nv || b.n_lo >= nv, m_ok = b.m_hi <= nv || b.m_lo >= nv;
+    two = n_ok && m_ok;
+  }
+  const int nbg = two ? 2 : 1, blo1 = two ? nv : nb;
+  if (c->box_ready && c->box_nbg == nbg && c->box_blo1 == blo1) return 0;
+  c->box_ready = false;
+  int npw = 0;
+  for (int k = 0; k < c->d.nki; ++k) npw = std::max(npw, c->ngk[k]);
+  npw = std::min(npw, c->ngm);
+  const int *mg = c->mill_g.data();
+  std::vector<std::array<int, 4>> cols;  // (mx, my, zmin, zmax)
+  {
+    std::vector<int> ord(npw);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) {
+      for (int d = 0; d < 3; ++d)
+        if (mg[3 * x + d] != mg[3 * y + d]) return mg[3 * x + d] < mg[3 * y + d];
+      return false;
+    });
+    for (int i : ord) {
+      const int *m = mg + 3 * i;
+      if (cols.empty() || cols.back()[0] != m[0] || cols.back()[1] != m[1])
+        cols.push_back({m[0], m[1], m[2], m[2]});
+      else
+        cols.back()[3] = m[2];
+    }
+  }
+  std::vector<int> base(cols.size() + 1, 0);
+  for (size_t u = 0; u < cols.size(); ++u) base[u + 1] = base[u] + cols[u][3] - cols[u][2] + 1;
+  const int nrow = base.back();
+  if (nrow > c->d.npwx) return 0;  // holes in the columns: keep the global order
+  std::vector<int> gidx(nrow, -1);
+  c->h_rbox.assign((size_t)c->gbox.size(), -1);
+  for (int i = 0; i < npw; ++i) {
+    const int *m = mg + 3 * i;
+    const auto it = std::lower_bound(cols.begin(), cols.end(), std::array<int, 4>{m[0], m[1], INT_MIN, INT_MIN});
+    const int u = (int)(it - cols.begin());
+    const int R = base[u] + m[2] - cols[u][2];
+    gidx[R] = i;
+    c->h_rbox[c->gbox.index(m[0], m[1], m[2])] = R;
+  }
+  const bool mixed = (c->d.precision == RPA_MIXED);
+  const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
+  const size_t rows = (size_t)c->d.nki * c->d.npwx;
+  int e = 0;
+  unsigned char *b0 = (unsigned char *)c->d_box[0], *b1 = (unsigned char *)c->d_box[1];
+  if (!e) e = ensure(c, b0, c->cap_box[0], rows * (nbg == 2 ? blo1 : nb) * csz);
+  if (!e && nbg == 2) e = ensure(c, b1, c->cap_box[1], rows * (nb - blo1) * csz);
+  c->d_box[0] = b0;
+  c->d_box[1] = b1;
+  if (!e) e = ensure(c, c->d_box_gidx, c->cap_box_gidx, (size_t)nrow);
+  if (e) return e;
+  const Upload up[] = {{c->d_box_gidx, gidx.data(), sizeof(int) * gidx.size()}};
+  if (int e2 = upload(c, up, 1)) return e2;
+  const long long n = (long long)nrow * nb;
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)c->d.nki);  // grid.y: k
+  if (mixed)
+    hipLaunchKernelGGL(k_build_box_k<float2>, grid, dim3(256), 0, c->stream, (const float2 *)c->d_evc32, c->d_ngk,
+                       c->d_box_gidx, c->d.npwx, nb, nrow, nbg, blo1, (float2 *)c->d_box[0], (float2 *)c->d_box[1]);
+  else
+    hipLaunchKernelGGL(k_build_box_k<double2>, grid, dim3(256), 0, c->stream, (const double2 *)c->d_evc, c->d_ngk,
+                       c->d_box_gidx, c->d.npwx, nb, nrow, nbg, blo1, (double2 *)c->d_box[0], (double2 *)c->d_box[1]);
+  RPA_HIP(c, hipGetLastError());
+  c->box_nbg = nbg;
+  c->box_blo1 = blo1;
+  c->box_rows = nrow;
+  c->box_ready = true;
+  return 0;
+}
+
 // The plan of the current q: positions in column order, chunks that fit the
 // LDS budget, in-plane classes of the local fields with their shift slots,
 // and for every distinct G0 the global index of each staged k+q row.
@@ -846,7 +953,14 @@ int build_col_plan(rpa_ctx *c) {
     crow[(size_t)u * (nchunk + 1) + nchunk] = r;
     nrows += r;
   }
-  // target global index of every B row, per distinct G0
+  // the column-box basis, where it can be built (RPA_VERTEX_BOX=0: global order)
+  const char *env_box = std::getenv("RPA_VERTEX_BOX");
+  bool box = !(env_box && env_box[0] == '0');
+  if (box) {
+    if (int e2 = ensure_box(c)) return e2;
+    box = c->box_ready;
+  }
+  // target row of every B row (global index, or box row), per distinct G0
   const int ng0 = (int)c->g0_list.size();
   std::vector<int> bmap((size_t)ng0 * nrows, -1);
   for (int z = 0; z < ng0; ++z) {
@@ -860,14 +974,16 @@ int build_col_plan(rpa_ctx *c) {
           for (int e = 0; e < ext; ++e, ++row) {
             const int bi = c->gbox.index(segs[s].mx + cls[u].lx + o[0], segs[s].my + cls[u].ly + o[1],
                                          segs[s].zlo + cls[u].lzmin + e + o[2]);
-            bmap[(size_t)z * nrows + row] = bi >= 0 ? c->h_gbox[bi] : -1;
+            bmap[(size_t)z * nrows + row] = bi >= 0 ? (box ? c->h_rbox[bi] : c->h_gbox[bi]) : -1;
           }
         }
       }
     }
   }
   std::vector<int> slot(ngd);
-  for (int p = 0; p < ngd; ++p) slot[p] = ord[p];
+  for (int p = 0; p < ngd; ++p) slot[p] = box ? c->h_rbox[c->gbox.index(mg[3 * ord[p]], mg[3 * ord[p] + 1],
+                                                                         mg[3 * ord[p] + 2])]
+                                              : ord[p];
   // beta(i) = Gcar G(i)_pol of every position (current vertices), with the
   // cartesian component summed as the gather forms do
   std::vector<double> beta(2 * (size_t)ngd, 0.0);
@@ -915,6 +1031,7 @@ int build_col_plan(rpa_ctx *c) {
   // the split form keeps hi and lo words of each staged value (the bytes of
   // one complex FP32 value) and no beta rows
   P.split = split;
+  P.box = box;
   if (P.lds > 64 * 1024) {
     const void *fn = P.split ? (pipe ? split_kernel<kKcF32, kNruF32, true>(tpw)
                                      : split_kernel<kKcF32, kNruF32, false>(tpw))
@@ -930,7 +1047,20 @@ int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, void *
   const ColPlan &P = c->col;
   const bool mixed = (c->d.precision == RPA_MIXED);
   ColArgs a{};
-  a.evc = mixed ? (const void *)c->d_evc32 : (const void *)c->d_evc;
+  if (P.box) {
+    a.box[0] = c->d_box[0];
+    a.box[1] = c->d_box[1] ? c->d_box[1] : c->d_box[0];
+    a.nbg = c->box_nbg;
+    a.blo1 = c->box_blo1;
+    a.bw[0] = c->box_nbg == 2 ? c->box_blo1 : c->d.nband;
+    a.bw[1] = c->d.nband - c->box_blo1;
+    a.lim_ngk = 0;
+  } else {
+    a.box[0] = a.box[1] = mixed ? (const void *)c->d_evc32 : (const void *)c->d_evc;
+    a.nbg = 1;
+    a.bw[0] = a.bw[1] = c->d.nband;
+    a.lim_ngk = 1;
+  }
   a.ngk = c->d_ngk;
   a.k1 = c->d_k1;
   a.k2 = c->d_k2;

@@ -411,7 +411,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
                   c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32,
-                  c->d_wmx, c->d_rbt, c->d_wt64};
+                  c->d_wmx, c->d_rbt, c->d_wt64, c->d_box[0], c->d_box[1], c->d_box_gidx};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_col_plan(c);
@@ -474,6 +474,7 @@ int rpa_set_gvectors(rpa_ctx *c, int ngm, const int *mill, const double *kc) {
   c->have_q = false;
   c->ng0 = 0;
   c->g0_list.clear();
+  c->box_ready = false;
   c->prepared = false;
   invalidate_graph(c);
   invalidate_batch(c);
@@ -517,6 +518,7 @@ int rpa_upload_basis(rpa_ctx *c, int ik, int ngk, const int *mill, const double 
   c->ngk[k] = ngk;
   c->uploaded[k] = 1;
   c->kbox_ready = false;
+  c->box_ready = false;
   c->prepared = false;
   invalidate_graph(c);
   invalidate_batch(c);

@@ -87,6 +87,7 @@ struct ColPlan {
   int kc = 0, tpw = 0, ms = 0, kind = 0, pol1 = 0, pol2 = 0;
   bool pipe = false;
   bool split = false;  // mixed charge vertices: BF16 hi/lo split products
+  bool box = false;    // slots and staged rows index the column-box basis
   size_t lds = 0;
   int *d_slot = nullptr, *d_prow = nullptr, *d_chunk = nullptr, *d_crow = nullptr;
   int *d_bmap = nullptr, *d_shift = nullptr, *d_cls = nullptr;
@@ -159,6 +160,19 @@ struct rpa_ctx {
   std::vector<int> g0_list;  // distinct iG0 (1-based) in order of appearance
   int *d_g0keys = nullptr, *d_g0idx = nullptr, *d_dmap = nullptr;
   rpa::ColPlan col;          // column-correlation vertex plan
+  // column-box copy of the basis for the column vertex form: per k the
+  // plane waves of global index below the largest ngk, in (mx, my, z) order
+  // (a column of the Miller lattice is a contiguous run of rows), zero where
+  // the index is past that k's ngk; bands split at Nval into two arrays when
+  // every band block lies on one side, so C1 and C2 reads fetch only their
+  // own bands
+  void *d_box[2] = {nullptr, nullptr};
+  size_t cap_box[2] = {0, 0};
+  int *d_box_gidx = nullptr;
+  size_t cap_box_gidx = 0;
+  std::vector<int> h_rbox;  // global Miller box index -> box row (-1: none)
+  int box_nbg = 0, box_blo1 = 0, box_rows = 0;
+  bool box_ready = false;
   // vertex form of the current q ("" = shape rules; "col", "gather",
   // "staged") and staged chunk (0 = largest that fits), set by the autotuner
   std::string vform;

@@ -293,3 +293,22 @@ def test_staged_form_paired_and_wide_blocks(precision, ncond, monkeypatch):
         print(f"{precision} ncond={ncond} iq={iq} N_G={nlf} staged-form vertices rel max err {err:.2e}")
         assert err < (1e-10 if precision == "fp64" else 2e-6)
     r.close()
+
+
+@pytest.mark.parametrize("program,precision", [("sloss", "fp64"), ("sloss", "mixed"), ("optabs", "fp64"),
+                                               ("optabs", "mixed")])
+def test_column_box_basis_is_bitwise_the_global_order(program, precision, monkeypatch):
+    """The column vertex form reads the basis in column-box order (Miller
+    columns contiguous, bands split at Nval, zeros past each k's ngk) instead
+    of the global order with a per-row ngk test: the staged values are the
+    same, so the vertices are bitwise equal (RPA_VERTEX_BOX=0 reads the
+    global order)."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "col")
+    s = H.system("cfg0")
+    got = {}
+    for box in ("1", "0"):
+        monkeypatch.setenv("RPA_VERTEX_BOX", box)
+        r = _runner(s, program=program, precision=precision, pol="xx")
+        got[box] = r.vertices()[0]
+        r.close()
+    assert np.array_equal(got["1"], got["0"])
