@@ -101,8 +101,10 @@ def batched_q_points(name: str, nq: Optional[int] = None):
     return [pts[j] for j in pick]
 
 
-def reciprocal_axes(a: float, c: float) -> np.ndarray:
-    """KC(3,3) in 2pi/a units, rounded to the 6 decimals scf.out carries.
+def reciprocal_axes(a: float, c: float, decimals: int | None = 6) -> np.ndarray:
+    """KC(3,3) in 2pi/a units, rounded to the 6 decimals scf.out carries
+    (decimals=None: unrounded, so that the hexagonal rotations map the
+    lattice onto itself exactly).
 
     KC[:, m] is b_m (cartesian), as loadKC stores KC(1:3, j) (Optabs.f90:668).
     """
@@ -110,7 +112,7 @@ def reciprocal_axes(a: float, c: float) -> np.ndarray:
     b2 = np.array([0.0, 2.0 / math.sqrt(3.0), 0.0])
     b3 = np.array([0.0, 0.0, a / c])
     kc = np.stack([b1, b2, b3], axis=1)
-    return np.round(kc, 6)
+    return kc if decimals is None else np.round(kc, decimals)
 
 
 def cell_volume(a: float, c: float) -> float:
@@ -452,3 +454,139 @@ def write_all(rundir: str, sys: SynthSystem, program: str, extra: Optional[dict]
     write_scf_out(os.path.join(rundir, "scf.out"), sys)
     write_band_file(os.path.join(rundir, "synthetic.band"), sys)
     write_config(os.path.join(rundir, "config.in"), sys, program, "./save/", extra=extra)
+
+
+# --------------------------------------------------------------------------
+# Point-group test systems (Nsym > 1)
+
+def hex_point_ops(kind: str) -> np.ndarray:
+    """Cartesian operations of the hexagonal lattice, R(48,3,3) with the
+    identity first: 'c2' (identity, 180-degree rotation about z), 'c3', 'c6'
+    (rotations about z), 'c3v' (c3 and the mirrors y -> -y and its images)."""
+    def rot(t):
+        c, s = math.cos(t), math.sin(t)
+        return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+    my = np.diag([1.0, -1.0, 1.0])
+    if kind == "c2":
+        ops = [rot(0.0), rot(math.pi)]
+    elif kind == "c3":
+        ops = [rot(2.0 * math.pi * j / 3.0) for j in range(3)]
+    elif kind == "c6":
+        ops = [rot(math.pi * j / 3.0) for j in range(6)]
+    elif kind == "c3v":
+        r = [rot(2.0 * math.pi * j / 3.0) for j in range(3)]
+        ops = r + [x @ my for x in r]
+    else:
+        raise ValueError(kind)
+    R = np.zeros((48, 3, 3))
+    for i, o in enumerate(ops):
+        R[i] = np.where(np.abs(o) < 1e-15, 0.0, o)
+    return R, len(ops)
+
+
+def make_symmetric_pair(cfg: SynthConfig, kind: str, npw_target: int, ngd_target: int, seed_shift: int = 0):
+    """Two descriptions of one physical system for testing the unfolding:
+
+    ibz: the irreducible k-points of an N x N grid under the point group
+         `kind` (the first point of each orbit in grid order), seeded
+         orthonormal coefficients over the first npw global G, energies from
+         the tight-binding bands of those points;
+    fz:  every point of R(i) kI(j) (the unfolded zone, in the order genFBZ
+         gives it) as a plain Nsym = 1 list, with coefficients C_Rk(R G) =
+         C_k(G) and energies E(Rk) = E(k).
+
+    npw and NGd are rounded up to counts whose G windows every operation maps
+    onto themselves, so rotations keep both coefficient windows.  Returns (ibz, fz, R, nsym, kinfo) with
+    kinfo[j] = (op, IBZ point) of fz point j, both 0-based."""
+    R, nsym = hex_point_ops(kind)
+    a, c = cfg.a, cfg.c
+    kc = reciprocal_axes(a, c, decimals=None)
+    gcar = 2.0 * math.pi / a
+    vol = cell_volume(a, c)
+    kmax = (npw_target * 8.0 * math.pi ** 3 / (vol * 4.0 * math.pi / 3.0)) ** (1.0 / 3.0)
+    mill_g = density_gvectors(kc, a, 2.0 * kmax + 2.0 * gcar)
+    g = gcart(mill_g, kc)
+    g2 = np.einsum("ij,ij->i", g, g)
+    index = {tuple(m): i for i, m in enumerate(mill_g.tolist())}
+    kcinv = np.linalg.inv(kc)
+
+    def rot_index(op, i):  # global index of R(op) G(i), len(g) when outside the list
+        m = np.rint(kcinv @ (R[op] @ g[i])).astype(int)
+        return index.get(tuple(m.tolist()), len(g))
+
+    def closed(n):  # smallest count >= n whose window every operation maps onto itself
+        while any(rot_index(op, i) >= n for op in range(nsym) for i in range(n)):
+            n += 1
+        return n
+    npw, ngd = closed(npw_target), closed(ngd_target)
+
+    n = cfg.nk
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    kcrys = np.stack([ii.ravel() / n, jj.ravel() / n, np.zeros(n * n)], axis=1)
+    kcart = np.round(gcart(kcrys, kc), 6)
+    covered = np.zeros(n * n, bool)
+    reps = []
+    for p in range(n * n):
+        if covered[p]:
+            continue
+        reps.append(p)
+        for op in range(nsym):
+            kr = kcinv @ (R[op] @ kcart[p])
+            q = np.rint(np.mod(kr, 1.0) * n).astype(int) % n
+            covered[q[0] * n + q[1]] = True
+    ki = kcart[reps]
+    e_ibz = np.round(tb_bands(kcrys[reps], cfg), 3)
+    nb = cfg.nband
+    evc_i = np.zeros((len(reps), nb, npw), dtype=np.complex128)
+    for j in range(len(reps)):
+        rng = np.random.default_rng([cfg.seed + seed_shift, j])
+        kg = (ki[j][None, :] + g[:npw]) * gcar
+        env = np.exp(-np.einsum("ij,ij->i", kg, kg) / 8.0)
+        z = (rng.standard_normal((nb, npw)) + 1j * rng.standard_normal((nb, npw))) * env[None, :]
+        for b in range(nb):
+            v = z[b]
+            for p_ in range(b):
+                v = v - np.vdot(z[p_], v) * z[p_]
+            z[b] = v / math.sqrt(np.vdot(v, v).real)
+        evc_i[j] = z
+    # unfolded zone in genFBZ order (operation outer, IBZ point inner; a
+    # point within eps of an earlier one is dropped)
+    kt = []
+    for op in range(nsym):
+        for j in range(len(reps)):
+            v = R[op] @ ki[j]
+            if not any(np.all(np.abs(v - w) <= 1e-4) for w in kt):
+                kt.append(v)
+    kt = np.array(kt)
+    # each point's (operation, IBZ point) by the reference's rule: the first
+    # operation, in order, whose inverse takes it to an IBZ point
+    # (findKinIBZ / findKQinBZ, Optabs.f90:776-869)
+    kinfo = []
+    for v in kt:
+        hit = None
+        for op in range(nsym):
+            w = np.linalg.inv(R[op]) @ v
+            for j in range(len(reps)):
+                if np.all(np.abs(w - ki[j]) <= 1e-4):
+                    hit = (op, j)
+                    break
+            if hit:
+                break
+        kinfo.append(hit)
+    evc_f = np.zeros((len(kt), nb, npw), dtype=np.complex128)
+    for t, (op, j) in enumerate(kinfo):
+        perm = np.array([rot_index(op, i) for i in range(npw)])
+        evc_f[t][:, perm] = evc_i[j]
+    e_fz = e_ibz[[j for _, j in kinfo]]
+
+    def system(kcart_, e_, evc_):
+        nk = len(kcart_)
+        npwx = int((npw + 63) // 64 * 64)
+        ev = np.zeros((nk, nb, npwx), dtype=np.complex128)
+        ev[:, :, :npw] = evc_
+        mk = np.zeros((nk, npwx, 3), dtype=np.int32)
+        mk[:, :npw] = mill_g[:npw]
+        return SynthSystem(cfg=dataclasses.replace(cfg, name=cfg.name + "-" + kind), kc=kc, mill_g=mill_g,
+                           kcart=np.asarray(kcart_), e_ev=e_, ngk=np.full(nk, npw, dtype=np.int32), npwx=npwx,
+                           mill_k=mk, evc=ev, gcutw=float(g2[npw - 1]), ecutwfc_ry=0.0, vcell=vol)
+    return system(ki, e_ibz, evc_i), system(kt, e_fz, evc_f), R, nsym, kinfo, ngd

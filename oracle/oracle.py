@@ -34,6 +34,7 @@ class OrcProblem(C.Structure):
         ("ntot", C.c_int), ("ktot", C.c_void_p), ("K1", C.c_void_p), ("K2", C.c_void_p),
         ("iG0", C.c_void_p), ("q", C.c_double * 3), ("nlf", C.c_int), ("lf_idx", C.c_void_p),
         ("map_mode", C.c_int), ("ngd", C.c_int), ("eps", C.c_double),
+        ("nsym", C.c_int), ("R", C.c_void_p), ("RI", C.c_void_p), ("R1", C.c_void_p), ("R2", C.c_void_p),
     ]
 
 
@@ -89,6 +90,11 @@ def lib():
         L.orc_gfast_linear.argtypes = [C.c_int, P_DBL, C.c_int, P_DBL, C.c_int, P_DBL, P_DBL,
                                        C.c_double, P_INT]
         L.orc_gfast_miller.argtypes = [C.c_int, P_INT, C.c_int, P_INT, C.c_int, P_INT, P_INT, P_INT]
+        L.orc_point_inverse.argtypes = [C.c_int, P_DBL, P_DBL]
+        L.orc_find_kin_ibz.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, P_DBL, P_DBL, P_DBL,
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_find_kq_sym.argtypes = [P_DBL, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, P_DBL, P_DBL,
+                                      P_DBL, P_DBL, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     return _lib
 
 
@@ -164,6 +170,43 @@ def find_kq_all(ktot, ki, g, q, eps):
     return K2, iG0
 
 
+def point_inverse(R, nsym):
+    """PointR's inverses of R(48,3,3) (Chi_RPA/PointR.f90:119-138)."""
+    Rf = F(R, np.float64)
+    RI = np.zeros((48, 3, 3), order="F")
+    lib().orc_point_inverse(nsym, Rf, RI)
+    return RI
+
+
+def find_kin_ibz(ik, ki, k, nsym, RI, eps):
+    kI = F(np.asarray(ki).T, np.float64)
+    a, b = C.c_int(), C.c_int()
+    rc = lib().orc_find_kin_ibz(ik, kI.shape[1], nsym, eps, F(k, np.float64), F(RI, np.float64), kI,
+                                C.byref(a), C.byref(b))
+    if rc:
+        raise RuntimeError(f"k {ik} not in the IBZ")
+    return a.value, b.value
+
+
+def find_kq_sym_all(ktot, ki, g, q, nsym, RI, eps):
+    """(iG0, R2, K2) 1-based for every FBZ k (Optabs.f90:822-869)."""
+    kt = F(np.asarray(ktot).T, np.float64)
+    kI = F(np.asarray(ki).T, np.float64)
+    gg = F(np.asarray(g).T, np.float64)
+    RIf = F(RI, np.float64)
+    ntot = kt.shape[1]
+    out = np.zeros((3, ntot), dtype=np.int32)
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    for ik in range(ntot):
+        kq = F(kt[:, ik] + q, np.float64)
+        rc = lib().orc_find_kq_sym(kq, eps, nsym, kI.shape[1], ntot, gg.shape[1], kt, kI, gg, RIf,
+                                   C.byref(a), C.byref(b), C.byref(c))
+        if rc:
+            raise RuntimeError(f"k+q not found for k={ik + 1} (code {rc})")
+        out[:, ik] = a.value, b.value, c.value
+    return out[0], out[1], out[2]
+
+
 def gk_sort(k, g, gcutw, npwx):
     kk = F(k, np.float64)
     gg = F(np.asarray(g).T, np.float64)
@@ -200,7 +243,7 @@ class Problem:
     """Holds the arrays of one (system, q) and the ctypes view of them."""
 
     def __init__(self, sys, K1, K2, iG0, ktot, q, lf_idx, map_mode=0, ngd=None, eps=1e-4,
-                 E_hartree=None):
+                 E_hartree=None, R=None, RI=None, R1=None, R2=None):
         self.sys = sys
         nki = len(sys.ngk)
         self.ngk = F(sys.ngk, np.int32)
@@ -239,6 +282,16 @@ class Problem:
         s.map_mode = map_mode
         s.ngd = self.ngd
         s.eps = eps
+        # point-group unfolding: R, RI (48,3,3) cartesian, R1/R2 1-based per FBZ k
+        s.nsym = 1
+        if R1 is not None:
+            self.R = F(R, np.float64)
+            self.RI = F(RI, np.float64)
+            self.R1 = np.ascontiguousarray(R1, dtype=np.int32)
+            self.R2 = np.ascontiguousarray(R2, dtype=np.int32)
+            s.nsym = int(max(self.R1.max(), self.R2.max()))
+            s.R, s.RI = self.R.ctypes.data, self.RI.ctypes.data
+            s.R1, s.R2 = self.R1.ctypes.data, self.R2.ctypes.data
         self.s = s
 
     @property

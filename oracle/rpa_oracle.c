@@ -163,6 +163,78 @@ int orc_find_kq(const double kq[3], double eps, int nki, int ntot, int ng,
   return it == 1 ? 1 : 2;
 }
 
+/* a3 with symmetry.  R and RI are (48,3,3) Fortran arrays: R(i,n,m) is
+ * R[i + 48*(n + 3*m)].  PointR inverts each R(i,:,:) with gjel on a complex
+ * 3x3 copy and keeps the real part (Chi_RPA/PointR.f90:119-138).          */
+void orc_point_inverse(int nsym, const double *R, double *RI) {
+  for (int i = 0; i < nsym; ++i) {
+    orc_z T[9], B[9];
+    for (int n = 0; n < 3; ++n)
+      for (int m = 0; m < 3; ++m) {
+        T[n + 3 * m] = R[i + 48 * (n + 3 * m)] + 0.0 * I;
+        B[n + 3 * m] = (n == m) ? 1.0 : 0.0;
+      }
+    orc_gjel(T, 3, 3, B, 3, 3);
+    for (int n = 0; n < 3; ++n)
+      for (int m = 0; m < 3; ++m) RI[i + 48 * (n + 3 * m)] = creal(T[n + 3 * m]);
+  }
+}
+
+static inline double orc_rot(const double *M, int i, int n, const double v[3]) {
+  /* M(i,n,1)*v1 + M(i,n,2)*v2 + M(i,n,3)*v3, left to right as the reference */
+  return (M[i + 48 * (n + 0)] * v[0] + M[i + 48 * (n + 3)] * v[1]) + M[i + 48 * (n + 6)] * v[2];
+}
+
+/* findKinIBZ, Optabs.f90:776-819: the first ik <= NkI are the IBZ list
+ * itself (R1 = 1); the others are searched over operations 2..Nsym.        */
+int orc_find_kin_ibz(int ik, int nki, int nsym, double eps, const double k[3],
+                     const double *RI, const double *ki, int *R1, int *K1) {
+  if (ik <= nki) {
+    *R1 = 1;
+    *K1 = ik;
+    return 0;
+  }
+  for (int i = 1; i < nsym; ++i) {
+    double v0 = orc_rot(RI, i, 0, k), v1 = orc_rot(RI, i, 1, k), v2 = orc_rot(RI, i, 2, k);
+    for (int j = 0; j < nki; ++j)
+      if (fabs(v0 - ki[3 * j]) <= eps && fabs(v1 - ki[3 * j + 1]) <= eps && fabs(v2 - ki[3 * j + 2]) <= eps) {
+        *R1 = i + 1;
+        *K1 = j + 1;
+        return 0;
+      }
+  }
+  return 1;
+}
+
+/* findKQinBZ, Optabs.f90:822-869: the first (iG, jk) with k+q - G(iG) =
+ * ktot(jk), then the first operation i and IBZ point j with RI(i) ktot(jk)
+ * = kI(j); the search over (iG, jk) goes on when none matches.            */
+int orc_find_kq_sym(const double kq[3], double eps, int nsym, int nki, int ntot, int ng,
+                    const double *ktot, const double *ki, const double *g, const double *RI,
+                    int *iG0, int *R2, int *K2) {
+  int it = 1;
+  for (int iG = 0; iG < ng; ++iG)
+    for (int jk = 0; jk < ntot; ++jk) {
+      if (fabs(kq[0] - g[3 * iG] - ktot[3 * jk]) <= eps && fabs(kq[1] - g[3 * iG + 1] - ktot[3 * jk + 1]) <= eps &&
+          fabs(kq[2] - g[3 * iG + 2] - ktot[3 * jk + 2]) <= eps) {
+        it = 2;
+        *iG0 = iG + 1;
+        const double *kt = ktot + 3 * jk;
+        for (int i = 0; i < nsym; ++i) {
+          double v0 = orc_rot(RI, i, 0, kt), v1 = orc_rot(RI, i, 1, kt), v2 = orc_rot(RI, i, 2, kt);
+          for (int j = 0; j < nki; ++j)
+            if (fabs(v0 - ki[3 * j]) <= eps && fabs(v1 - ki[3 * j + 1]) <= eps &&
+                fabs(v2 - ki[3 * j + 2]) <= eps) {
+              *R2 = i + 1;
+              *K2 = j + 1;
+              return 0;
+            }
+        }
+      }
+    }
+  return it == 1 ? 1 : 2;
+}
+
 /* G(n) = sum_m KC(n,m) * mill(m), Optabs_qe6.f90:1162-1167 (loadG_QE6). */
 void orc_gcart(int n, const int *mill, const double *kc, double *g) {
   for (int i = 0; i < n; ++i)
@@ -328,6 +400,102 @@ static int orc_make_pair(const orc_problem *p, int ik, orc_kpair *kp) {
   return 0;
 }
 
+/* a6 with symmetry (Optabs.f90:1405-1449, Sloss.f90:917-946): plane wave i
+ * of C1 carries G'' = R(R1) G(i) (cartesian, in the reference's order), the
+ * target is RI(R2) (G'' + Glf(G) + G(iG0)), matched against the G list of
+ * C2 within eps per component.  The match takes the lattice vector nearest
+ * to the target (its Miller indices rounded through KC^-1) and keeps it when
+ * every component lies within eps: the reference's linear search finds the
+ * same vector whenever eps is below half a lattice spacing.  l1 receives
+ * R(R1) G(i) for the current vertices.                                       */
+static int orc_gfast_sym(const orc_problem *p, int ik, const orc_kpair *kp, int nlf, const int *mlf,
+                         int *gfast, double *l1) {
+  const int r1 = p->R1[ik] - 1, r2 = p->R2[ik] - 1;
+  const double *kc = p->kc;
+  /* KC^-1 by cofactors */
+  double inv[9];
+  {
+    const double a = kc[0], b = kc[3], c = kc[6], d = kc[1], e = kc[4], f = kc[7], g = kc[2], h = kc[5], k = kc[8];
+    const double det = a * (e * k - f * h) - b * (d * k - f * g) + c * (d * h - e * g);
+    inv[0] = (e * k - f * h) / det;
+    inv[3] = (c * h - b * k) / det;
+    inv[6] = (b * f - c * e) / det;
+    inv[1] = (f * g - d * k) / det;
+    inv[4] = (a * k - c * g) / det;
+    inv[7] = (c * d - a * f) / det;
+    inv[2] = (d * h - e * g) / det;
+    inv[5] = (b * g - a * h) / det;
+    inv[8] = (a * e - b * d) / det;
+  }
+  const int n1 = kp->n1, n2 = kp->n2;
+  unsigned cap = 1;
+  while (cap < 2u * (unsigned)n2 + 2u) cap <<= 1;
+  int *slot = (int *)malloc(sizeof(int) * cap);
+  double *glf = (double *)malloc(sizeof(double) * 3 * nlf);
+  if (!slot || !glf) {
+    free(slot);
+    free(glf);
+    return -1;
+  }
+  for (unsigned t = 0; t < cap; ++t) slot[t] = -1;
+  const int *m2 = kp->m2;
+  for (int j = 0; j < n2; ++j) {
+    unsigned hh = orc_hkey(m2[3 * j], m2[3 * j + 1], m2[3 * j + 2]) & (cap - 1);
+    while (slot[hh] >= 0) hh = (hh + 1) & (cap - 1);
+    slot[hh] = j;
+  }
+  orc_gcart(nlf, mlf, kc, glf);
+  double g0[3];
+  orc_gcart(1, kp->m0, kc, g0);
+  for (int i = 0; i < n1; ++i) {
+    double gi[3];
+    orc_gcart(1, kp->m1 + 3 * i, kc, gi);
+    const double k11 = orc_rot(p->R, r1, 0, gi), k22 = orc_rot(p->R, r1, 1, gi), k33 = orc_rot(p->R, r1, 2, gi);
+    l1[3 * i] = k11;
+    l1[3 * i + 1] = k22;
+    l1[3 * i + 2] = k33;
+    for (int iG = 0; iG < nlf; ++iG) {
+      const double t[3] = {k11 + glf[3 * iG] + g0[0], k22 + glf[3 * iG + 1] + g0[1], k33 + glf[3 * iG + 2] + g0[2]};
+      double w[3];
+      for (int n = 0; n < 3; ++n) w[n] = orc_rot(p->RI, r2, n, t);
+      int mm[3];
+      for (int n = 0; n < 3; ++n) mm[n] = (int)lround(inv[n] * w[0] + inv[n + 3] * w[1] + inv[n + 6] * w[2]);
+      unsigned hh = orc_hkey(mm[0], mm[1], mm[2]) & (cap - 1);
+      int hit = -1;
+      while (slot[hh] >= 0) {
+        const int o = slot[hh];
+        if (m2[3 * o] == mm[0] && m2[3 * o + 1] == mm[1] && m2[3 * o + 2] == mm[2]) {
+          hit = o;
+          break;
+        }
+        hh = (hh + 1) & (cap - 1);
+      }
+      if (hit >= 0) {
+        double gj[3];
+        orc_gcart(1, m2 + 3 * hit, kc, gj);
+        if (!(fabs(gj[0] - w[0]) < p->eps && fabs(gj[1] - w[1]) < p->eps && fabs(gj[2] - w[2]) < p->eps)) hit = -1;
+      }
+      gfast[iG * n1 + i] = hit;
+    }
+  }
+  free(slot);
+  free(glf);
+  return 0;
+}
+
+/* The index map of a k pair and, for the current vertices, the cartesian G
+ * of C1's plane waves (rotated by R(R1) under symmetry).                    */
+static int orc_pair_maps(const orc_problem *p, int ik, const orc_kpair *kp, int nlf, const int *mlf, int *gfast,
+                         double *l1) {
+  if (p->R1 && (p->R1[ik] != 1 || p->R2[ik] != 1)) {
+    if (p->map_mode != ORC_MAP_DIRECT) return -5;  /* unfolding follows the drivers' direct map */
+    return orc_gfast_sym(p, ik, kp, nlf, mlf, gfast, l1);
+  }
+  orc_gfast_miller(kp->n1, kp->m1, kp->n2, kp->m2, nlf, mlf, kp->m0, gfast);
+  if (l1) orc_gcart(kp->n1, kp->m1, p->kc, l1);
+  return 0;
+}
+
 static void orc_lf_miller(const orc_problem *p, int *mlf) {
   for (int iG = 0; iG < p->nlf; ++iG)
     for (int c = 0; c < 3; ++c) mlf[3 * iG + c] = p->mill_g[3 * (p->lf_idx[iG] - 1) + c];
@@ -401,8 +569,10 @@ int orc_vertices(const orc_problem *p, int kind, double gcar, int n_lo, int n_hi
         err = -3;
         continue;
       }
-      orc_gfast_miller(kp.n1, kp.m1, kp.n2, kp.m2, nlf, mlf, kp.m0, gfast);
-      orc_gcart(kp.n1, kp.m1, p->kc, l1);
+      if (orc_pair_maps(p, ik, &kp, nlf, mlf, gfast, l1)) {
+        err = -5;
+        continue;
+      }
       for (int n = n_lo; n < n_hi; ++n) {
         const double *c1 = kp.c1base + (size_t)2 * p->npwx * n;
         for (int m = m_lo; m < m_hi; ++m) {
@@ -464,6 +634,7 @@ int orc_sloss_s0(const orc_problem *p, const orc_sloss_params *sp, double *S0,
     double *part = (double *)malloc(sizeof(double) * s0n);
     int nmax = p->ngd > p->npwx ? p->ngd : p->npwx;
     int *gfast = (int *)malloc(sizeof(int) * (size_t)nlf * nmax);
+    double *l1s = (double *)malloc(sizeof(double) * 3 * (size_t)nmax);
     double *rr = (double *)malloc(sizeof(double) * nlf);
     double *ri = (double *)malloc(sizeof(double) * nlf);
 #pragma omp for schedule(static)
@@ -473,7 +644,10 @@ int orc_sloss_s0(const orc_problem *p, const orc_sloss_params *sp, double *S0,
         err = -3;
         continue;
       }
-      orc_gfast_miller(kp.n1, kp.m1, kp.n2, kp.m2, nlf, mlf, kp.m0, gfast);
+      if (orc_pair_maps(p, ik, &kp, nlf, mlf, gfast, l1s)) {
+        err = -5;
+        continue;
+      }
       memset(part, 0, sizeof(double) * s0n);
       for (int n = 0; n < nocc; ++n) {
         const double *c1 = kp.c1base + (size_t)2 * p->npwx * n;
@@ -501,6 +675,7 @@ int orc_sloss_s0(const orc_problem *p, const orc_sloss_params *sp, double *S0,
     }
     free(part);
     free(gfast);
+    free(l1s);
     free(rr);
     free(ri);
   }
@@ -563,8 +738,10 @@ int orc_optabs_s0(const orc_problem *p, const orc_optabs_params *op, orc_z *S0,
         err = -3;
         continue;
       }
-      orc_gfast_miller(kp.n1, kp.m1, kp.n2, kp.m2, nlf, mlf, kp.m0, gfast);
-      orc_gcart(kp.n1, kp.m1, p->kc, l1);
+      if (orc_pair_maps(p, ik, &kp, nlf, mlf, gfast, l1)) {
+        err = -5;
+        continue;
+      }
       const double *kv = p->ktot + 3 * ik;
       memset(part, 0, sizeof(double) * 2 * s0n);
       memset(qpart, 0, sizeof(double) * 2 * nlf * nlf);

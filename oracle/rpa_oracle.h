@@ -50,6 +50,12 @@ typedef struct orc_problem {
   int map_mode;       /* ORC_MAP_*                                             */
   int ngd;            /* NGd (direct mode)                                     */
   double eps;         /* G-vector match tolerance                              */
+  /* point-group unfolding (Nsym > 1); R1 == NULL means the identity for
+   * every k.  R, RI: (48,3,3) cartesian operations and their inverses as
+   * PointR gives them; R1, R2: [ntot] 1-based operation of k and of k+q.    */
+  int nsym;
+  const double *R, *RI;
+  const int *R1, *R2;
 } orc_problem;
 
 /* ---- host setup restatements (a3-a5) ---------------------------------- */
@@ -64,6 +70,17 @@ int  orc_find_kq(const double kq[3], double eps, int nki, int ntot, int ng,
                  const double *ktot, const double *ki, const double *g,
                  int *iG0, int *K2);
 void orc_gcart(int n, const int *mill, const double *kc, double *g);
+/* PointR's inverses (Chi_RPA/PointR.f90:119-138): RI(i) = gjel(R(i)). */
+void orc_point_inverse(int nsym, const double *R, double *RI);
+/* findKinIBZ (Optabs.f90:776-819): ik 1-based FBZ index; R1, K1 1-based.
+ * Returns 1 when k is not in the IBZ (the reference stops).               */
+int  orc_find_kin_ibz(int ik, int nki, int nsym, double eps, const double k[3],
+                      const double *RI, const double *ki, int *R1, int *K1);
+/* findKQinBZ with symmetry (Optabs.f90:822-869): iG0, R2, K2 1-based;
+ * returns 1 (not in the FBZ) or 2 (not in the IBZ) as the reference.       */
+int  orc_find_kq_sym(const double kq[3], double eps, int nsym, int nki, int ntot, int ng,
+                     const double *ktot, const double *ki, const double *g, const double *RI,
+                     int *iG0, int *R2, int *K2);
 int  orc_gk_sort(const double k[3], int ngm, const double *g, double gcutw,
                  int npwx, int *ngk, int *igk);
 

@@ -610,3 +610,70 @@ def test_pi_assembly_two_fields_reads_pi11_after_each_update():
                 tot[iG, jG] = tot[iG, jG] + qeff[iG, jG] * oi / (oi + 1j * eta_i)
         if io > 1:
             assert si[io - 1] == pytest.approx((-1j * c0 * p_inter / oi).real, rel=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# Point-group unfolding (Nsym > 1): the oracle's rotated vertex map and k
+# bookkeeping against the same system written out on the whole unfolded zone
+# with rotated coefficients, C_Rk(R G) = C_k(G), and E(Rk) = E(k), run with
+# Nsym = 1.  The two must agree to rounding: a wrong rotation, inverse,
+# operation choice or index map breaks the equality.
+
+def _sym_setup(kind, nk, nlf_target=9):
+    import dataclasses
+    cfg = dataclasses.replace(H.synth.CONFIGS["tiny"], nk=nk, lf="xyz", nlf_target=nlf_target)
+    ibz, fz, R, nsym, kinfo, ngd = H.synth.make_symmetric_pair(cfg, kind, 90, 60)
+    g = H.synth.gcart(ibz.mill_g, ibz.kc)
+    RI = orc.point_inverse(R, nsym)
+    # genFBZ with the operations
+    kif = np.asfortranarray(ibz.kcart.T)
+    Rf = np.asfortranarray(R)
+    kt = np.zeros((3, 48 * len(ibz.kcart)), order="F")
+    nt = C.c_int()
+    assert orc.lib().orc_gen_fbz(len(ibz.kcart), nsym, 1e-4, kif, Rf, C.byref(nt), kt) == 0
+    ktot = kt[:, :nt.value].T.copy()
+    np.testing.assert_allclose(ktot, fz.kcart, rtol=0, atol=1e-12)
+    R1 = np.zeros(len(ktot), np.int32)
+    K1 = np.zeros(len(ktot), np.int32)
+    for ik in range(len(ktot)):
+        R1[ik], K1[ik] = orc.find_kin_ibz(ik + 1, ibz.kcart, ktot[ik], nsym, RI, 1e-4)
+    assert [(r - 1, k - 1) for r, k in zip(R1, K1)] == kinfo
+    q = orc.find_min_q(ktot, 2)
+    iG0, R2, K2 = orc.find_kq_sym_all(ktot, ibz.kcart, g, q, nsym, RI, 1e-4)
+    ecut = H.synth.local_field_ecut(ibz)
+    rc, nlf, glf, parg, idx = orc.gen_glf("xyz", ecut, ibz.gcar, g, ibz.mill_g, 100)
+    pb_s = orc.Problem(ibz, K1, K2, iG0, ktot, q, idx, ngd=ngd, E_hartree=H.energies(ibz, "sloss"),
+                       R=R, RI=RI, R1=R1, R2=R2)
+    K2f, iG0f = orc.find_kq_all(ktot, ktot, g, q, 1e-4)
+    pb_f = orc.Problem(fz, np.arange(1, len(ktot) + 1, dtype=np.int32), K2f, iG0f, ktot, q, idx, ngd=ngd,
+                       E_hartree=H.energies(fz, "sloss"))
+    # the unfolded k+q point is the same FBZ point in both descriptions
+    assert np.array_equal(iG0, iG0f)
+    return cfg, ibz, fz, pb_s, pb_f, nsym
+
+
+@pytest.mark.parametrize("kind,nk", [("c2", 4), ("c3", 6), ("c6", 6), ("c3v", 6)])
+def test_unfolded_vertices_and_s0_equal_the_whole_zone(kind, nk):
+    cfg, ibz, fz, pb_s, pb_f, nsym = _sym_setup(kind, nk)
+    nv, nb = cfg.nval, cfg.nband
+    assert nsym > 1 and (pb_s.R1 != 1).any() and (pb_s.R2 != 1).any()
+    v_s = orc.vertices(pb_s, 0, ibz.gcar, 0, nv, nv, nb)
+    v_f = orc.vertices(pb_f, 0, fz.gcar, 0, nv, nv, nb)
+    assert np.abs(v_s).max() > 0.1
+    assert H.rel_max(v_s, v_f) < 1e-12
+    dw = H.sloss_grid(cfg)
+    S_s = orc.sloss_s0(pb_s, nv, cfg.no, dw, cfg.eta_ev / H.HARTREE, ibz.vcell)
+    S_f = orc.sloss_s0(pb_f, nv, cfg.no, dw, cfg.eta_ev / H.HARTREE, fz.vcell)
+    assert H.rel_max(S_s, S_f) < 1e-12
+    # current vertices: J carries R(R1) G'' of the unfolded plane wave
+    for pol in (1, 2, 4):
+        a1, a2 = orc.vertices(pb_s, pol, ibz.gcar, 0, nv, nv, nb, both=True)
+        b1, b2 = orc.vertices(pb_f, pol, fz.gcar, 0, nv, nv, nb, both=True)
+        assert H.rel_max(a1, b1) < 1e-12 and H.rel_max(a2, b2) < 1e-12
+
+
+def test_point_inverse_matches_lapack():
+    R, nsym = H.synth.hex_point_ops("c3v")
+    RI = orc.point_inverse(R, nsym)
+    for i in range(nsym):
+        np.testing.assert_allclose(RI[i], np.linalg.inv(R[i]), rtol=0, atol=1e-15)
