@@ -28,7 +28,8 @@ EXPORTS = [
     "rpa_get_vertices", "rpa_gen_fbz", "rpa_find_min_q", "rpa_find_kq", "rpa_gen_glf",
     "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms", "rpa_get_result",
     "rpa_load_s0", "rpa_w_kk", "rpa_kk_noalloc", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
-    "rpa_batch_result",
+    "rpa_batch_result", "rpa_set_symmetry", "rpa_set_q_sym", "rpa_batch_set_sym", "rpa_find_kq_sym",
+    "rpa_point_inverse",
 ]
 # enum rpa_result
 RES = {"S0": 0, "QEFF": 1, "CHI0": 2, "V": 3, "EPSINV": 4, "CHI": 5, "W": 6, "LOSS": 7,
@@ -104,6 +105,12 @@ def lib() -> C.CDLL:
         L.rpa_batch_set.argtypes = [P, C.c_int, P, P, P, C.c_int, P, P, P, P, P]
         L.rpa_batch_run.argtypes = [P]
         L.rpa_batch_result.argtypes = [P, C.c_int, C.c_int, P]
+        L.rpa_set_symmetry.argtypes = [P, C.c_int, P]
+        L.rpa_set_q_sym.argtypes = [P, C.c_int, P, P, P, P, P, P, P]
+        L.rpa_batch_set_sym.argtypes = [P, C.c_int, P, P, P, C.c_int, P, P, P, P, P, P, P]
+        L.rpa_find_kq_sym.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P, P, C.c_int, P, C.c_double,
+                                      P, P, P, P, P]
+        L.rpa_point_inverse.argtypes = [C.c_int, P, P]
         _lib = L
     return _lib
 
@@ -146,6 +153,28 @@ def find_kq(ktot, ki, g, q, eps):
     _check(None, lib().rpa_find_kq(n, _ptr(k), len(kI), _ptr(kI), len(gg), _ptr(gg), _ptr(qq), eps,
                                    _ptr(K1), _ptr(K2), _ptr(iG0)))
     return K1, K2, iG0
+
+
+def point_inverse(R, nsym):
+    """PointR's inverses RI(48,3,3) of R(48,3,3) (numpy arrays indexed [op, i, j])."""
+    Rf = _f(R, np.float64)
+    RI = np.zeros((48, 3, 3), order="F")
+    _check(None, lib().rpa_point_inverse(nsym, _ptr(Rf), _ptr(RI)))
+    return np.ascontiguousarray(RI)
+
+
+def find_kq_sym(ktot, ki, g, q, nsym, RI, eps):
+    """(R1, K1, R2, K2, iG0), 1-based, of every FBZ k with point operations."""
+    k = np.ascontiguousarray(ktot, dtype=np.float64)
+    kI = np.ascontiguousarray(ki, dtype=np.float64)
+    gg = np.ascontiguousarray(g, dtype=np.float64)
+    qq = np.ascontiguousarray(q, dtype=np.float64)
+    RIf = _f(RI, np.float64)
+    n = len(k)
+    R1, K1, R2, K2, iG0 = (np.zeros(n, dtype=np.int32) for _ in range(5))
+    _check(None, lib().rpa_find_kq_sym(n, _ptr(k), len(kI), _ptr(kI), len(gg), _ptr(gg), _ptr(qq), nsym,
+                                       _ptr(RIf), eps, _ptr(R1), _ptr(K1), _ptr(R2), _ptr(K2), _ptr(iG0)))
+    return R1, K1, R2, K2, iG0
 
 
 def gen_glf(lf, ecut, gcar, g, mill, nlfd):
@@ -239,6 +268,17 @@ class Context:
         kt = np.ascontiguousarray(ktot, dtype=np.float64)
         qq = np.ascontiguousarray(q, dtype=np.float64)
         self._c(lib().rpa_set_q(self.h, len(K1), _ptr(K1), _ptr(K2), _ptr(iG0), _ptr(kt), _ptr(qq)))
+
+    def set_symmetry(self, R, nsym):
+        Rf = _f(R, np.float64)
+        self._c(lib().rpa_set_symmetry(self.h, nsym, _ptr(Rf)))
+
+    def set_q_sym(self, K1, K2, iG0, R1, R2, ktot, q):
+        K1, K2, iG0, R1, R2 = (np.ascontiguousarray(x, dtype=np.int32) for x in (K1, K2, iG0, R1, R2))
+        kt = np.ascontiguousarray(ktot, dtype=np.float64)
+        qq = np.ascontiguousarray(q, dtype=np.float64)
+        self._c(lib().rpa_set_q_sym(self.h, len(K1), _ptr(K1), _ptr(K2), _ptr(iG0), _ptr(R1), _ptr(R2), _ptr(kt),
+                                    _ptr(qq)))
 
     # ---- hot path
     def vertices(self):
@@ -335,6 +375,12 @@ class Context:
         kt = np.ascontiguousarray(ktot, dtype=np.float64)
         qq = np.ascontiguousarray(np.stack([p["q"] for p in points]), dtype=np.float64)
         ntot = len(points[0]["K1"])
+        if "R1" in points[0]:
+            R1 = np.ascontiguousarray(np.concatenate([p["R1"] for p in points]), dtype=np.int32)
+            R2 = np.ascontiguousarray(np.concatenate([p["R2"] for p in points]), dtype=np.int32)
+            self._c(lib().rpa_batch_set_sym(self.h, len(points), _ptr(nl), _ptr(li), _ptr(pg), ntot, _ptr(K1),
+                                            _ptr(K2), _ptr(iG0), _ptr(R1), _ptr(R2), _ptr(kt), _ptr(qq)))
+            return
         self._c(lib().rpa_batch_set(self.h, len(points), _ptr(nl), _ptr(li), _ptr(pg), ntot, _ptr(K1), _ptr(K2),
                                     _ptr(iG0), _ptr(kt), _ptr(qq)))
 

@@ -44,7 +44,10 @@ struct VertexArgs {
   const int16_t *kbox;
   const int *gbox;
   const int *dmap;   // direct map [ng0][nlf][ngd] global G index or -1
-  const int *g0idx;  // per k: index of its G0 in the direct map
+  const int *g0idx;  // per k: index of its map combination (R1, R2, G0)
+  const int *csym;   // per combination: A (9) then B (9), row-major
+  const double *crkc;  // per combination: R(R1) KC, column-major
+  int sym;           // some combination rotates: targets A m + B (Glf + G0)
   Box kb, gb;
   const int *ngk;
   const int *k1, *k2, *m0;
@@ -117,9 +120,48 @@ struct PrecF32 {
   }
 };
 
-// Target global index of G(i) + Glf(g) + G0 for every distinct G0 of the q.
+// Point operation of k in the igk map: plane wave s of k has the target
+// Miller index A s + h in the k+q list, h = B (Glf + G0) (A = B = I without
+// operations: s + Glf + G0).  The branch is uniform per k.
+struct IgkOp {
+  int A[9];
+  int h0, h1, h2;
+  bool on;
+  __device__ inline void init(const VertexArgs &a, int k, int l0, int l1, int l2, int d0, int d1, int d2) {
+    on = a.sym != 0;
+    h0 = l0 + d0, h1 = l1 + d1, h2 = l2 + d2;
+    if (on) {
+      const int *cs = a.csym + 18 * a.g0idx[k];
+#pragma unroll
+      for (int x = 0; x < 9; ++x) A[x] = cs[x];
+      const int u0 = h0, u1 = h1, u2 = h2;
+      h0 = cs[9] * u0 + cs[10] * u1 + cs[11] * u2;
+      h1 = cs[12] * u0 + cs[13] * u1 + cs[14] * u2;
+      h2 = cs[15] * u0 + cs[16] * u1 + cs[17] * u2;
+    }
+  }
+  __device__ inline int index(const Box &kb, int s0, int s1, int s2) const {
+    if (!on) return kb.index(s0 + h0, s1 + h1, s2 + h2);
+    return kb.index((A[0] * s0 + A[1] * s1 + A[2] * s2) + h0, (A[3] * s0 + A[4] * s1 + A[5] * s2) + h1,
+                    (A[6] * s0 + A[7] * s1 + A[8] * s2) + h2);
+  }
+};
+
+// Row pol of the matrix that takes a plane wave's Miller index to the
+// G'' of the current vertex J: KC, or R(R1) KC with point operations.
+__device__ inline void j_row(const VertexArgs &a, int k, int pol, double r[3]) {
+  if (a.sym) {
+    const double *m = a.crkc + 9 * a.g0idx[k];
+    r[0] = m[pol], r[1] = m[pol + 3], r[2] = m[pol + 6];
+  } else {
+    r[0] = a.kc[pol], r[1] = a.kc[pol + 3], r[2] = a.kc[pol + 6];
+  }
+}
+
+// Target global index of A G(i) + B (Glf(g) + G0) for every map combination
+// of the q, in Miller indices (A = B = I without point operations).
 __global__ void k_direct_map(const int *mill_g, const int *gbox, Box gb, const int *lf_key,
-                             const int *g0keys, int ng0, int nlf, int ngd, int *dmap) {
+                             const int *g0keys, const int *csym, int ng0, int nlf, int ngd, int *dmap) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)ng0 * nlf * ngd;
   if (idx >= n) return;
@@ -130,7 +172,12 @@ __global__ void k_direct_map(const int *mill_g, const int *gbox, Box gb, const i
   mill_unpack(mill_g[i], s0, s1, s2);
   mill_unpack(lf_key[g], l0, l1, l2);
   mill_unpack(g0keys[z], o0, o1, o2);
-  const int bi = gb.index(s0 + l0 + o0, s1 + l1 + o1, s2 + l2 + o2);
+  const int *A = csym + 18 * z, *B = A + 9;
+  const int u0 = l0 + o0, u1 = l1 + o1, u2 = l2 + o2;
+  const int t0 = (A[0] * s0 + A[1] * s1 + A[2] * s2) + (B[0] * u0 + B[1] * u1 + B[2] * u2);
+  const int t1 = (A[3] * s0 + A[4] * s1 + A[5] * s2) + (B[3] * u0 + B[4] * u1 + B[5] * u2);
+  const int t2 = (A[6] * s0 + A[7] * s1 + A[8] * s2) + (B[6] * u0 + B[7] * u1 + B[8] * u2);
+  const int bi = gb.index(t0, t1, t2);
   dmap[idx] = bi >= 0 ? gbox[bi] : -1;
 }
 
@@ -217,6 +264,11 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   const int *dmap = direct ? a.dmap + ((size_t)a.g0idx[k] * a.nlf + (bg_ok ? bg : 0)) * a.ngd : nullptr;
   int lf0 = 0, lf1 = 0, lf2 = 0;
   if (!direct && bg_ok) mill_unpack(a.lf_key[bg], lf0, lf1, lf2);
+  IgkOp iop;
+  iop.init(a, k, lf0, lf1, lf2, d0, d1, d2);
+  double kr1[3] = {0.0, 0.0, 0.0}, kr2[3] = {0.0, 0.0, 0.0};
+  if (KIND >= 1) j_row(a, k, a.pol1, kr1);
+  if (KIND == 2) j_row(a, k, a.pol2, kr2);
   const int arow = tid / TPA, asub = tid % TPA;  // C1 row c, first band
 
   const int r = lane & 15, q4 = lane >> 4;
@@ -293,13 +345,12 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
       if (bg_ok && i0 + bc < n1) {
         int s0, s1, s2;
         mill_unpack(mkey, s0, s1, s2);
-        // J = Gcar (q + 2k + Glf(G) + 2 G(i))_pol with G(i) = KC mill (left to right)
-        const double g1 = ((0.0 + a.kc[a.pol1 + 0] * (double)s0) + a.kc[a.pol1 + 3] * (double)s1) +
-                          a.kc[a.pol1 + 6] * (double)s2;
+        // J = Gcar (q + 2k + Glf(G) + 2 G(i))_pol with G(i) = KC mill (left to
+        // right; R(R1) G(i) with point operations)
+        const double g1 = ((0.0 + kr1[0] * (double)s0) + kr1[1] * (double)s1) + kr1[2] * (double)s2;
         pj1 = (real)(0.5 * ((jb1 + 2.0 * g1) * a.gcar));
         if (KIND == 2) {
-          const double g2 = ((0.0 + a.kc[a.pol2 + 0] * (double)s0) + a.kc[a.pol2 + 3] * (double)s1) +
-                            a.kc[a.pol2 + 6] * (double)s2;
+          const double g2 = ((0.0 + kr2[0] * (double)s0) + kr2[1] * (double)s1) + kr2[2] * (double)s2;
           pj2 = (real)(0.5 * ((jb2 + 2.0 * g2) * a.gcar));
         }
       }
@@ -319,7 +370,7 @@ __global__ __launch_bounds__(256) void k_vertex(VertexArgs a) {
   auto lookup_igk = [&](int key, bool ok, bool &found) -> int {
     int s0, s1, s2;
     mill_unpack(key, s0, s1, s2);
-    const int bi = a.kb.index(s0 + lf0 + d0, s1 + lf1 + d1, s2 + lf2 + d2);
+    const int bi = iop.index(a.kb, s0, s1, s2);
     found = ok && bi >= 0;
     return (int)box2[found ? bi : 0];
   };
@@ -581,6 +632,11 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
   const int *dmap = direct ? a.dmap + ((size_t)a.g0idx[k] * a.nlf + min(iGr, a.nlf - 1)) * a.ngd : nullptr;
   int lf0 = 0, lf1 = 0, lf2 = 0;
   if (!direct && iGr < a.nlf) mill_unpack(a.lf_key[iGr], lf0, lf1, lf2);
+  IgkOp iop;
+  iop.init(a, k, lf0, lf1, lf2, d0, d1, d2);
+  double kr1[3] = {0.0, 0.0, 0.0}, kr2[3] = {0.0, 0.0, 0.0};
+  if (KIND >= 1) j_row(a, k, a.pol1, kr1);
+  if (KIND == 2) j_row(a, k, a.pol2, kr2);
 
   const int r = lane & 15, kg = lane >> 4;
   const int hc = PAIR ? (r >> 3) : 0;  // field of this matrix-core column
@@ -631,18 +687,18 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
     if (key == kNone) return -1;
     int s0, s1, s2;
     mill_unpack(key, s0, s1, s2);
-    const int bi = a.kb.index(s0 + lf0 + d0, s1 + lf1 + d1, s2 + lf2 + d2);
+    const int bi = iop.index(a.kb, s0, s1, s2);
     return bi >= 0 ? (int)box2[bi] : -1;
   };
   auto row_of = [&](int jraw) -> int { return (jraw >= 0 && jraw < lim2) ? jraw : -1; };
   auto load_mkey = [&](int i) -> int { return (KIND >= 1 && res_ok && i < n1) ? src[i] : 0; };
   // J of plane wave i for current vertices (pol p), lanes < 32
-  auto jweight = [&](int mkey, int i, int pol) -> real {
+  auto jweight = [&](int mkey, int i, int pol, const double kr[3]) -> real {
     if (!(res_ok && i < n1)) return 0;
     int s0, s1, s2;
     mill_unpack(mkey, s0, s1, s2);
     const double jb = (a.q[pol] + 2.0 * kv[pol]) + a.glf[3 * iGr + pol];
-    const double g = ((0.0 + a.kc[pol + 0] * (double)s0) + a.kc[pol + 3] * (double)s1) + a.kc[pol + 6] * (double)s2;
+    const double g = ((0.0 + kr[0] * (double)s0) + kr[1] * (double)s1) + kr[2] * (double)s2;
     return (real)(0.5 * ((jb + 2.0 * g) * a.gcar));
   };
 
@@ -674,8 +730,8 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
     const int j0 = row_of(direct ? key0 : lookup_igk(key0));
     if (KIND >= 1) {
       const int mk0 = load_mkey(lr);
-      jw1 = jweight(mk0, lr, a.pol1);
-      if (KIND == 2) jw2 = jweight(mk0, lr, a.pol2);
+      jw1 = jweight(mk0, lr, a.pol1, kr1);
+      if (KIND == 2) jw2 = jweight(mk0, lr, a.pol2, kr2);
     }
     if (active) load_b(j0);
     const int key1 = load_key(CH + lr);
@@ -702,8 +758,8 @@ __global__ __launch_bounds__(256) void k_vertex_gd(VertexArgs a) {
     if (active && i0 + CH < n1) {
       load_b(row_of(jn));
       if (KIND >= 1) {
-        jw1 = jweight(mkn, i0 + CH + lr, a.pol1);
-        if (KIND == 2) jw2 = jweight(mkn, i0 + CH + lr, a.pol2);
+        jw1 = jweight(mkn, i0 + CH + lr, a.pol1, kr1);
+        if (KIND == 2) jw2 = jweight(mkn, i0 + CH + lr, a.pol2, kr2);
       }
       const int i2 = i0 + 2 * CH + lr;
       if (direct) {
@@ -907,7 +963,9 @@ __global__ __launch_bounds__(256) void k_vertex_lds(VertexArgs a) {
     int s0, s1, s2, l0, l1, l2;
     mill_unpack(v, s0, s1, s2);
     mill_unpack(a.lf_key[g], l0, l1, l2);
-    const int bi = a.kb.index(s0 + l0 + d0, s1 + l1 + d1, s2 + l2 + d2);
+    IgkOp op;
+    op.init(a, k, l0, l1, l2, d0, d1, d2);
+    const int bi = op.index(a.kb, s0, s1, s2);
     const int j = bi >= 0 ? (int)box2[bi] : -1;
     return j >= 0 ? j : zrow;
   };
@@ -1045,8 +1103,8 @@ int launch_direct_map(rpa_ctx *c) {
   if (n <= 0) return 0;
   hipLaunchKernelGGL(k_direct_map, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                      (const int *)c->d_mill_g, (const int *)c->d_gbox, c->gbox,
-                     (const int *)c->d_lf_key, (const int *)c->d_g0keys, c->ng0, c->nlf, c->p.ngd,
-                     c->d_dmap);
+                     (const int *)c->d_lf_key, (const int *)c->d_g0keys, (const int *)c->d_csym, c->ng0, c->nlf,
+                     c->p.ngd, c->d_dmap);
   RPA_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1208,6 +1266,9 @@ int launch_vertices(rpa_ctx *c, void *rho1, void *rho2, const int *perm) {
   a.gbox = c->d_gbox;
   a.dmap = c->d_dmap;
   a.g0idx = c->d_g0idx;
+  a.csym = c->d_csym;
+  a.crkc = c->d_crkc;
+  a.sym = c->qsym ? 1 : 0;
   a.kb = c->kbox;
   a.gb = c->gbox;
   a.ngk = c->d_ngk;
@@ -1412,7 +1473,7 @@ int autotune_vertex(rpa_ctx *c) {
   std::string key = std::to_string(c->nlf) + "/" + std::to_string(cls.size()) + "/" + std::to_string(c->pairs.tpk) +
                     "/" + std::to_string(c->pairs.nblocks) + "/" + std::to_string(c->d.program) + "/" +
                     std::to_string(c->d.precision) + "/" + std::to_string(c->p.vertex_map) + "/" +
-                    std::to_string(c->p.pol);
+                    std::to_string(c->p.pol) + "/" + std::to_string((int)c->qsym);
   for (const auto &kv : c->vform_cache)
     if (kv.first == key) {
       c->vform = kv.second.first;

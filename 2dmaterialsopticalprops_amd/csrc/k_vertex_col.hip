@@ -794,7 +794,8 @@ static int ensure_box(rpa_ctx *c) {
 int build_col_plan(rpa_ctx *c) {
   ColPlan &P = c->col;
   P.ok = false;
-  if (c->p.vertex_map != RPA_MAP_DIRECT) return 0;
+  // the column correlation needs targets G(i) + Glf + G0: no point operations
+  if (c->p.vertex_map != RPA_MAP_DIRECT || c->qsym) return 0;
   const bool mixed = (c->d.precision == RPA_MIXED);
   const int kc = mixed ? kKcF32 : kKcF64;
   const size_t csz = mixed ? sizeof(float2) : sizeof(double2);

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <unordered_map>
 
 #include "rpa_internal.h"
 
@@ -39,7 +40,7 @@ static void invalidate_batch(rpa_ctx *c) {
 void free_q_buffers(rpa_ctx *c) {
   void *bufs[] = {c->d_lf_key, c->d_glf, c->d_parg, c->d_k1, c->d_k2, c->d_m0, c->d_ktot, c->d_g0keys,
                   c->d_g0idx, c->d_dmap, c->d_V, c->d_epsinv, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
-                  c->d_qeff, c->d_status};
+                  c->d_qeff, c->d_status, c->d_csym, c->d_crkc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_col_plan(c);
@@ -411,7 +412,8 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_s0, c->d_part, c->d_qeff, c->d_kmat, c->d_x, c->d_chi0, c->d_V,
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
                   c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32,
-                  c->d_wmx, c->d_rbt, c->d_wt64, c->d_box[0], c->d_box[1], c->d_box_gidx};
+                  c->d_wmx, c->d_rbt, c->d_wt64, c->d_box[0], c->d_box[1], c->d_box_gidx,
+                  c->d_csym, c->d_crkc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_col_plan(c);
@@ -475,6 +477,11 @@ int rpa_set_gvectors(rpa_ctx *c, int ngm, const int *mill, const double *kc) {
   c->ng0 = 0;
   c->g0_list.clear();
   c->box_ready = false;
+  // the point operations' Miller forms depend on KC
+  c->nsym = 1;
+  c->op_m.clear();
+  c->op_minv.clear();
+  c->op_rkc.clear();
   c->prepared = false;
   invalidate_graph(c);
   invalidate_batch(c);
@@ -592,24 +599,130 @@ int rpa_set_params(rpa_ctx *c, const rpa_params *p) {
   return RPA_OK;
 }
 
-int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0, const double *ktot,
-              const double *q) {
-  if (!c || ntot <= 0 || !K1 || !K2 || !iG0 || !ktot || !q) return fail(c, RPA_E_ARG, "rpa_set_q: bad arguments");
+// Point operations R(48,3,3) (Chi_RPA/PointR.f90:41-138).  Each must map the
+// reciprocal lattice onto itself: its Miller form KC^-1 R KC, and that of its
+// inverse RI (rpa_point_inverse, PointR's gjel), are integer matrices within
+// 1e-3.  The reference finds RI(R2) (R(R1) G + Glf + G0) in the G list by a
+// tolerance search; for a lattice operation that is the vector of integer
+// Miller index A m + B (Glf + G0) used here, A = M(RI(R2)) M(R(R1)),
+// B = M(RI(R2)).
+int rpa_set_symmetry(rpa_ctx *c, int nsym, const double *R) {
+  if (!c || nsym < 1 || nsym > 48 || !R) return fail(c, RPA_E_ARG, "rpa_set_symmetry: bad arguments");
+  if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors must precede rpa_set_symmetry");
+  std::vector<double> RI(48 * 9, 0.0);
+  if (rpa_point_inverse(nsym, R, RI.data()) != RPA_OK)
+    return fail(c, RPA_E_ARG, "a point operation is singular (gjel)");
+  // KC^-1 by cofactors, column-major like KC
+  const double *K = c->kc;
+  auto at = [&](int i, int j) { return K[i + 3 * j]; };
+  const double det = at(0, 0) * (at(1, 1) * at(2, 2) - at(1, 2) * at(2, 1)) -
+                     at(0, 1) * (at(1, 0) * at(2, 2) - at(1, 2) * at(2, 0)) +
+                     at(0, 2) * (at(1, 0) * at(2, 1) - at(1, 1) * at(2, 0));
+  if (!(std::fabs(det) > 0.0)) return fail(c, RPA_E_ARG, "rpa_set_symmetry: KC is singular");
+  double Ki[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const int i1 = (j + 1) % 3, i2 = (j + 2) % 3, j1 = (i + 1) % 3, j2 = (i + 2) % 3;
+      Ki[i + 3 * j] = (at(i1, j1) * at(i2, j2) - at(i1, j2) * at(i2, j1)) / det;
+    }
+  // Miller form (row-major) of the cartesian operation Q(op) (R or RI)
+  auto miller = [&](const double *Q, int op, int *out) -> bool {
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double t = 0.0;
+        for (int l = 0; l < 3; ++l)
+          for (int n = 0; n < 3; ++n) t += Ki[i + 3 * l] * Q[op + 48 * (l + 3 * n)] * K[n + 3 * j];
+        const double r = std::nearbyint(t);
+        if (std::fabs(t - r) > 1e-3) return false;
+        out[3 * i + j] = (int)r;
+      }
+    return true;
+  };
+  std::vector<int> m(9 * (size_t)nsym), mi(9 * (size_t)nsym);
+  std::vector<double> rkc(9 * (size_t)nsym);
+  for (int op = 0; op < nsym; ++op) {
+    if (!miller(R, op, &m[9 * op]) || !miller(RI.data(), op, &mi[9 * op]))
+      return fail(c, RPA_E_ARG,
+                  "point operation " + std::to_string(op + 1) + " does not map the reciprocal lattice onto itself");
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        int t = 0;
+        for (int l = 0; l < 3; ++l) t += m[9 * op + 3 * i + l] * mi[9 * op + 3 * l + j];
+        if (t != (i == j ? 1 : 0))
+          return fail(c, RPA_E_ARG, "point operation " + std::to_string(op + 1) + ": RI is not its inverse");
+      }
+    // R(op) KC, column-major, sums left to right
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        rkc[9 * op + i + 3 * j] = (R[op + 48 * i] * K[3 * j] + R[op + 48 * (i + 3)] * K[1 + 3 * j]) +
+                                  R[op + 48 * (i + 6)] * K[2 + 3 * j];
+  }
+  c->nsym = nsym;
+  c->op_m = m;
+  c->op_minv = mi;
+  c->op_rkc = rkc;
+  // the k+q map refers to operations by number: set it again
+  c->have_q = false;
+  c->prepared = false;
+  invalidate_graph(c);
+  invalidate_batch(c);
+  return RPA_OK;
+}
+
+// The k+q map of a q: K1, K2, iG0 and, with point operations, R1 and R2 of
+// every FBZ k (1-based; R1 = R2 = null: the identity).
+static int set_q_impl(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0, const int *R1,
+                      const int *R2, const double *ktot, const double *q) {
+  if (!c || ntot <= 0 || !K1 || !K2 || !iG0 || !ktot || !q || (!R1) != (!R2))
+    return fail(c, RPA_E_ARG, "rpa_set_q: bad arguments");
   if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors must precede rpa_set_q");
   (void)hipSetDevice(c->d.device);
-  std::vector<int> m0(ntot), g0idx(ntot), g0keys;
-  std::vector<int> g0seen;  // distinct iG0 in order of appearance
+  const bool ops = R1 != nullptr;
+  if (ops && c->op_m.empty()) return fail(c, RPA_E_STATE, "rpa_set_symmetry must precede rpa_set_q_sym");
+  auto is_identity = [&](int op) {  // 0-based
+    if (!ops) return true;
+    for (int x = 0; x < 9; ++x) {
+      const int e = (x % 4 == 0) ? 1 : 0;
+      if (c->op_m[9 * op + x] != e || c->op_minv[9 * op + x] != e || c->op_rkc[9 * op + x] != c->kc[x]) return false;
+    }
+    return true;
+  };
+  std::vector<int> m0(ntot), g0idx(ntot), g0keys, csym;
+  std::vector<double> crkc;
+  std::vector<int> g0seen;  // iG0 of each combination
+  std::unordered_map<long long, int> combo;
+  bool qsym = false;
   for (int k = 0; k < ntot; ++k) {
     if (K1[k] < 1 || K1[k] > c->d.nki || K2[k] < 1 || K2[k] > c->d.nki)
       return fail(c, RPA_E_ARG, "K1/K2 outside 1..NkI");
     if (iG0[k] < 1 || iG0[k] > c->ngm) return fail(c, RPA_E_ARG, "iG0 outside 1..NG");
+    const int r1 = ops ? R1[k] - 1 : 0, r2 = ops ? R2[k] - 1 : 0;
+    if (r1 < 0 || r1 >= c->nsym || r2 < 0 || r2 >= c->nsym) return fail(c, RPA_E_ARG, "R1/R2 outside 1..Nsym");
     const int *m = &c->mill_g[3 * (size_t)(iG0[k] - 1)];
     m0[k] = mill_pack(m[0], m[1], m[2]);
-    int z = 0;
-    while (z < (int)g0seen.size() && g0seen[z] != iG0[k]) ++z;
-    if (z == (int)g0seen.size()) {
+    const long long key = ((long long)r1 * 48 + r2) * (long long)c->ngm + (iG0[k] - 1);
+    auto it = combo.find(key);
+    int z;
+    if (it == combo.end()) {
+      z = (int)g0seen.size();
+      combo.emplace(key, z);
       g0seen.push_back(iG0[k]);
       g0keys.push_back(m0[k]);
+      int A[9], B[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          B[3 * i + j] = ops ? c->op_minv[9 * r2 + 3 * i + j] : (i == j);
+          int t = 0;
+          for (int l = 0; l < 3; ++l)
+            t += (ops ? c->op_minv[9 * r2 + 3 * i + l] : (i == l)) * (ops ? c->op_m[9 * r1 + 3 * l + j] : (l == j));
+          A[3 * i + j] = t;
+        }
+      csym.insert(csym.end(), A, A + 9);
+      csym.insert(csym.end(), B, B + 9);
+      for (int x = 0; x < 9; ++x) crkc.push_back(ops ? c->op_rkc[9 * r1 + x] : c->kc[x]);
+      qsym = qsym || !is_identity(r1) || !is_identity(r2);
+    } else {
+      z = it->second;
     }
     g0idx[k] = z;
   }
@@ -617,6 +730,14 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   c->K1.assign(K1, K1 + ntot);
   c->K2.assign(K2, K2 + ntot);
   c->iG0.assign(iG0, iG0 + ntot);
+  if (ops) {
+    c->kR1.assign(R1, R1 + ntot);
+    c->kR2.assign(R2, R2 + ntot);
+  } else {
+    c->kR1.clear();
+    c->kR2.clear();
+  }
+  c->qsym = qsym;
   c->ktot.assign(ktot, ktot + 3 * (size_t)ntot);
   for (int a = 0; a < 3; ++a) c->q[a] = q[a];
   c->ng0 = (int)g0keys.size();
@@ -628,18 +749,33 @@ int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0
   if (!e) e = ensure(c, c->d_ktot, c->cap_ktot, 3 * nt);
   if (!e) e = ensure(c, c->d_g0keys, c->cap_g0keys, (size_t)c->ng0);
   if (!e) e = ensure(c, c->d_g0idx, c->cap_g0idx, nt);
+  if (!e) e = ensure(c, c->d_csym, c->cap_csym, 18 * (size_t)c->ng0);
+  if (!e) e = ensure(c, c->d_crkc, c->cap_crkc, 9 * (size_t)c->ng0);
   if (e) return e;
   const Upload up[] = {{c->d_k1, K1, sizeof(int) * nt},
                        {c->d_k2, K2, sizeof(int) * nt},
                        {c->d_m0, m0.data(), sizeof(int) * nt},
                        {c->d_ktot, ktot, sizeof(double) * 3 * nt},
                        {c->d_g0keys, g0keys.data(), sizeof(int) * c->ng0},
-                       {c->d_g0idx, g0idx.data(), sizeof(int) * nt}};
-  if (int e2 = upload(c, up, 6)) return e2;
+                       {c->d_g0idx, g0idx.data(), sizeof(int) * nt},
+                       {c->d_csym, csym.data(), sizeof(int) * csym.size()},
+                       {c->d_crkc, crkc.data(), sizeof(double) * crkc.size()}};
+  if (int e2 = upload(c, up, 8)) return e2;
   c->have_q = true;
   c->prepared = false;
   invalidate_graph(c);
   return RPA_OK;
+}
+
+int rpa_set_q(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0, const double *ktot,
+              const double *q) {
+  return set_q_impl(c, ntot, K1, K2, iG0, nullptr, nullptr, ktot, q);
+}
+
+int rpa_set_q_sym(rpa_ctx *c, int ntot, const int *K1, const int *K2, const int *iG0, const int *R1,
+                  const int *R2, const double *ktot, const double *q) {
+  if (!R1 || !R2) return fail(c, RPA_E_ARG, "rpa_set_q_sym: R1 and R2 are required");
+  return set_q_impl(c, ntot, K1, K2, iG0, R1, R2, ktot, q);
 }
 
 int rpa_vertices(rpa_ctx *c) {
@@ -880,8 +1016,9 @@ int rpa_set_autotune(rpa_ctx *c, int on) {
   return RPA_OK;
 }
 
-int rpa_batch_set(rpa_ctx *c, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot, const int *K1,
-                  const int *K2, const int *iG0, const double *ktot, const double *q) {
+static int batch_set_impl(rpa_ctx *c, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot,
+                          const int *K1, const int *K2, const int *iG0, const int *R1, const int *R2,
+                          const double *ktot, const double *q) {
   if (!c || nq <= 0 || !nlf || !lf_idx || !parg || ntot <= 0 || !K1 || !K2 || !iG0 || !ktot || !q)
     return fail(c, RPA_E_ARG, "rpa_batch_set: bad arguments");
   (void)hipSetDevice(c->d.device);
@@ -896,7 +1033,10 @@ int rpa_batch_set(rpa_ctx *c, int nq, const int *nlf, const int *lf_idx, const i
     rpa_qslot &sl = (*c->slots)[iq];
     swap_qslot(c, sl);
     int e = rpa_set_local_fields(c, nlf[iq], lf_idx + off, parg + off);
-    if (!e) e = rpa_set_q(c, ntot, K1 + (size_t)iq * ntot, K2 + (size_t)iq * ntot, iG0 + (size_t)iq * ntot, ktot, q + 3 * (size_t)iq);
+    const size_t o = (size_t)iq * ntot;
+    if (!e)
+      e = set_q_impl(c, ntot, K1 + o, K2 + o, iG0 + o, R1 ? R1 + o : nullptr, R2 ? R2 + o : nullptr, ktot,
+                     q + 3 * (size_t)iq);
     swap_qslot(c, sl);
     if (e) {
       const std::string m = c->err;
@@ -906,6 +1046,18 @@ int rpa_batch_set(rpa_ctx *c, int nq, const int *nlf, const int *lf_idx, const i
     off += (size_t)nlf[iq];
   }
   return RPA_OK;
+}
+
+int rpa_batch_set(rpa_ctx *c, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot, const int *K1,
+                  const int *K2, const int *iG0, const double *ktot, const double *q) {
+  return batch_set_impl(c, nq, nlf, lf_idx, parg, ntot, K1, K2, iG0, nullptr, nullptr, ktot, q);
+}
+
+int rpa_batch_set_sym(rpa_ctx *c, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot,
+                      const int *K1, const int *K2, const int *iG0, const int *R1, const int *R2,
+                      const double *ktot, const double *q) {
+  if (!R1 || !R2) return fail(c, RPA_E_ARG, "rpa_batch_set_sym: R1 and R2 are required");
+  return batch_set_impl(c, nq, nlf, lf_idx, parg, ntot, K1, K2, iG0, R1, R2, ktot, q);
 }
 
 // Prepares every q of the batch (the autotuner picks each shape's vertex

@@ -118,6 +118,132 @@ int rpa_find_kq(int ntot, const double *ktot, int nki, const double *ki, int ng,
   return RPA_OK;
 }
 
+// PointR (Chi_RPA/PointR.f90:119-138): RI(i) = R(i)^-1 for every operation
+// by Gauss-Jordan elimination with full pivoting in gjel's order
+// (Chi_RPA/gjel.f: the pivot is the last entry of largest modulus in a
+// row-by-row scan of the rows and columns not yet used; rows are swapped to
+// put it on the diagonal, columns restored at the end).  PointR runs it on a
+// complex copy whose imaginary parts stay zero, so real arithmetic gives the
+// same values.  A zero pivot (gjel's 'singular matrix') is an error here.
+int rpa_point_inverse(int nsym, const double *R, double *RI) {
+  if (nsym <= 0 || nsym > 48 || !R || !RI) return RPA_E_ARG;
+  for (int op = 0; op < nsym; ++op) {
+    double a[3][3];
+    for (int n = 0; n < 3; ++n)
+      for (int m = 0; m < 3; ++m) a[n][m] = R[op + 48 * (n + 3 * m)];
+    int used[3] = {0, 0, 0}, prow[3], pcol[3];
+    for (int step = 0; step < 3; ++step) {
+      double big = 0.0;
+      int ir = -1, ic = -1;
+      for (int j = 0; j < 3; ++j) {
+        if (used[j]) continue;
+        for (int k = 0; k < 3; ++k)
+          if (!used[k] && std::fabs(a[j][k]) >= big) {
+            big = std::fabs(a[j][k]);
+            ir = j;
+            ic = k;
+          }
+      }
+      if (ic < 0 || a[ir][ic] == 0.0) return RPA_E_SINGULAR;
+      used[ic] = 1;
+      if (ir != ic)
+        for (int l = 0; l < 3; ++l) std::swap(a[ir][l], a[ic][l]);
+      prow[step] = ir;
+      pcol[step] = ic;
+      const double inv = 1.0 / a[ic][ic];
+      a[ic][ic] = 1.0;
+      for (int l = 0; l < 3; ++l) a[ic][l] *= inv;
+      for (int r = 0; r < 3; ++r) {
+        if (r == ic) continue;
+        const double f = a[r][ic];
+        a[r][ic] = 0.0;
+        for (int l = 0; l < 3; ++l) a[r][l] -= a[ic][l] * f;
+      }
+    }
+    for (int step = 2; step >= 0; --step)
+      if (prow[step] != pcol[step])
+        for (int k = 0; k < 3; ++k) std::swap(a[k][prow[step]], a[k][pcol[step]]);
+    for (int n = 0; n < 3; ++n)
+      for (int m = 0; m < 3; ++m) RI[op + 48 * (n + 3 * m)] = a[n][m];
+  }
+  return RPA_OK;
+}
+
+namespace {
+// RI(op) v accumulated left to right, as the reference's sums
+inline void rot3(const double *RI, int op, const double *v, double *w) {
+  for (int n = 0; n < 3; ++n)
+    w[n] = (RI[op + 48 * n] * v[0] + RI[op + 48 * (n + 3)] * v[1]) + RI[op + 48 * (n + 6)] * v[2];
+}
+}  // namespace
+
+// findKinIBZ + findKQinBZ with the point operations, for every FBZ k
+// (Optabs.f90:306-320, 776-869):
+//   findKinIBZ: k <= NkI is the IBZ point itself (R1 = 1); otherwise the first
+//     operation i = 2..Nsym and IBZ point j with RI(i) k = kI(j) within eps;
+//   findKQinBZ: the first (G0, k') in G-major order with k + q - G0 = k'
+//     within eps for which some operation i = 1..Nsym and IBZ point j give
+//     RI(i) k' = kI(j) (the first such i, then j).
+// RPA_E_NOTFOUND at the reference's stops ('Can not find wave vector ...').
+int rpa_find_kq_sym(int ntot, const double *ktot, int nki, const double *ki, int ng, const double *g,
+                    const double *q, int nsym, const double *RI, double eps, int *R1, int *K1, int *R2,
+                    int *K2, int *iG0) {
+  if (ntot <= 0 || nki <= 0 || ng <= 0 || nsym <= 0 || nsym > 48 || !ktot || !ki || !g || !q || !RI ||
+      !R1 || !K1 || !R2 || !K2 || !iG0)
+    return RPA_E_ARG;
+  auto in_ibz = [&](const double *v) -> int {
+    for (int j = 0; j < nki; ++j)
+      if (near3(v, ki + 3 * j, eps)) return j;
+    return -1;
+  };
+  for (int k = 0; k < ntot; ++k) {
+    const double *kv = ktot + 3 * k;
+    if (k < nki) {
+      R1[k] = 1;
+      K1[k] = k + 1;
+    } else {
+      int hit = -1;
+      for (int op = 1; op < nsym && hit < 0; ++op) {
+        double w[3];
+        rot3(RI, op, kv, w);
+        const int j = in_ibz(w);
+        if (j >= 0) {
+          hit = op;
+          R1[k] = op + 1;
+          K1[k] = j + 1;
+        }
+      }
+      if (hit < 0) return RPA_E_NOTFOUND;  // 'Can not find wave vector K= .. in I.B.Z.'
+    }
+    const double kq[3] = {kv[0] + q[0], kv[1] + q[1], kv[2] + q[2]};
+    int state = 1;  // 1: no k' in the FBZ, 2: k' but not in the IBZ, 3: found
+    for (int ig = 0; ig < ng && state < 3; ++ig) {
+      const double *G = g + 3 * ig;
+      for (int jk = 0; jk < ntot && state < 3; ++jk) {
+        const double *kk = ktot + 3 * jk;
+        if (!(std::fabs(kq[0] - G[0] - kk[0]) <= eps && std::fabs(kq[1] - G[1] - kk[1]) <= eps &&
+              std::fabs(kq[2] - G[2] - kk[2]) <= eps))
+          continue;
+        state = 2;
+        iG0[k] = ig + 1;
+        for (int op = 0; op < nsym; ++op) {
+          double w[3];
+          rot3(RI, op, kk, w);
+          const int j = in_ibz(w);
+          if (j >= 0) {
+            R2[k] = op + 1;
+            K2[k] = j + 1;
+            state = 3;
+            break;
+          }
+        }
+      }
+    }
+    if (state != 3) return RPA_E_NOTFOUND;  // 'Can not find wave vector K+Q'
+  }
+  return RPA_OK;
+}
+
 // genGlf / genGlfandParity (Optabs.f90:1016-1059, Sloss.f90:824-881):
 // 'z' keeps G with Gx = Gy = 0 and Gcar^2 Gz^2 / 2 <= Ecut, 'xyz' keeps
 // Gcar^2 |G|^2 / 2 <= Ecut; parity +1 for even Miller z.

@@ -118,6 +118,11 @@ struct rpa_ctx {
   int *d_gbox = nullptr;    // global box -> index
   rpa::Box gbox{};
   std::vector<int> h_gbox;  // host copy of the global box
+  // point operations (rpa_set_symmetry), per operation: the Miller forms of
+  // R and of its inverse RI (row-major, m' = M m) and R KC (column-major)
+  int nsym = 1;
+  std::vector<int> op_m, op_minv;
+  std::vector<double> op_rkc;
 
   // basis
   void *d_evc = nullptr;
@@ -155,10 +160,19 @@ struct rpa_ctx {
   double *d_ktot = nullptr;
   size_t cap_k1 = 0, cap_k2 = 0, cap_m0 = 0, cap_ktot = 0, cap_g0keys = 0, cap_g0idx = 0;
   bool have_q = false;
-  // distinct G0 of the q (direct vertex map): keys, per-k index, map table
+  // map combinations of the q: the distinct (R1, R2, iG0) of its k, in order
+  // of appearance (with the identity only, the distinct G0): keys, per-k
+  // index, direct map table, and per combination the integer operation of
+  // the target, A = M(RI(R2)) M(R(R1)) and B = M(RI(R2)) (target Miller
+  // A m + B (Glf + G0)), and R(R1) KC for the current vertices
   int ng0 = 0;
-  std::vector<int> g0_list;  // distinct iG0 (1-based) in order of appearance
+  std::vector<int> g0_list;  // iG0 (1-based) of each combination
+  std::vector<int> kR1, kR2;  // per k, 1-based operations (empty: identity)
   int *d_g0keys = nullptr, *d_g0idx = nullptr, *d_dmap = nullptr;
+  int *d_csym = nullptr;     // [ng0][18]: A then B, row-major
+  double *d_crkc = nullptr;  // [ng0][9]
+  size_t cap_csym = 0, cap_crkc = 0;
+  bool qsym = false;         // some combination rotates (A or B not the identity)
   rpa::ColPlan col;          // column-correlation vertex plan
   // column-box copy of the basis for the column vertex form: per k the
   // plane waves of global index below the largest ngk, in (mx, my, z) order
@@ -259,6 +273,7 @@ struct rpa_ctx {
   X(nlf) X(ngp) X(lf_idx) X(parg) X(d_lf_key) X(d_glf) X(d_parg) X(cap_lf_key) X(cap_glf) X(cap_parg) X(ntot)   \
   X(K1) X(K2) X(iG0) X(ktot) X(d_k1) X(d_k2) X(d_m0) X(d_ktot) X(cap_k1) X(cap_k2) X(cap_m0) X(cap_ktot)          \
   X(cap_g0keys) X(cap_g0idx) X(have_q) X(ng0) X(g0_list) X(d_g0keys) X(d_g0idx) X(d_dmap) X(col) X(cap_dmap)       \
+  X(kR1) X(kR2) X(d_csym) X(d_crkc) X(cap_csym) X(cap_crkc) X(qsym)                                              \
   X(vform) X(vch) X(ncol) X(ncol_pad) X(prepared) X(d_V) X(d_epsinv) X(d_W) X(d_loss) X(d_pi11) X(d_sig)          \
   X(d_qeff) X(d_status) X(cap_V) X(cap_epsinv) X(cap_W) X(cap_loss) X(cap_pi11) X(cap_sig) X(cap_qeff)           \
   X(cap_status)

@@ -41,9 +41,9 @@ program optabs_hip
   character(len=256) :: cfgfile
   character(len=100) :: dato1, dato2, dato3
   integer :: u, ios, nsym, ngm, ik, iq, io, iG, jG, igwx, nbnd, npwx, ntot, nlf
-  real(dp) :: R(48, 3, 3), kc(3, 3), gcar, domega, nel, q(3), o, oi
+  real(dp) :: R(48, 3, 3), RI(48, 3, 3), q0(3), kc(3, 3), gcar, domega, nel, q(3), o, oi
   real(dp), allocatable :: ki(:, :), E(:, :), ktot(:, :), g(:, :), glf(:, :)
-  integer, allocatable :: mill(:, :), millk(:, :), parg(:), lf_idx(:), K1(:), K2(:), iG0(:)
+  integer, allocatable :: mill(:, :), millk(:, :), parg(:), lf_idx(:), K1(:), K2(:), iG0(:), R1(:), R2(:)
   complex(dp), allocatable :: evc(:, :)
   complex(dp), allocatable, target :: S0(:, :, :), Qeff(:, :), pi11(:)
   real(dp), allocatable, target :: sig(:, :)
@@ -81,21 +81,24 @@ program optabs_hip
   domega = (omax - omin) / (no - 1)
 
   call point_r(trim(rundir)//trim(scf_file), nsym, R)
-  if (nsym /= 1) then
-    write(*, '(a,i0,a)') 'ERROR: Nsym = ', nsym, ': symmetry unfolding is not supported, '// &
-      'run QE without symmetry (nosym, noinv)'
-    error stop 1
-  end if
+  RI = 0.0_dp
+  call rpa_check(rpa_point_inverse(nsym, R, RI), c_null_ptr, 'PointR')
   allocate(ki(3, NkI), E(NkI, Nband), ktot(3, 48 * NkI))
   call load_ki_and_e(trim(rundir)//trim(band_file), NkI, Nband, Nval, dGW, ki, E)
   call rpa_check(rpa_gen_fbz(NkI, nsym, eps, ki, R, ntot, ktot), c_null_ptr, 'genFBZ')
-  call check_fbz_integration(NkI, Nband, ntot, Efermi, E, nel)
   call load_kc(trim(rundir)//trim(scf_file), kc)
   call load_g(trim(savedir), ngm, mill)
   allocate(g(3, ngm), glf(3, Nlfd), parg(Nlfd), lf_idx(Nlfd))
   call rpa_check(rpa_gcart(ngm, mill, kc, g), c_null_ptr, 'G vectors')
   call rpa_check(rpa_gen_glf(trim(lf)//c_null_char, Ecut, gcar, ngm, g, mill, Nlfd, nlf, glf, parg, &
                              lf_idx), c_null_ptr, 'genGlf')
+  ! findKinIBZ for every FBZ k (K1 does not depend on q), then the FBZ
+  ! electron count over those IBZ points
+  allocate(R1(ntot), K1(ntot), R2(ntot), K2(ntot), iG0(ntot))
+  q0 = 0.0_dp
+  call rpa_check(rpa_find_kq_sym(ntot, ktot, NkI, ki, ngm, g, q0, nsym, RI, eps, R1, K1, R2, K2, iG0), &
+                 c_null_ptr, 'findKinIBZ')
+  call check_fbz_integration(NkI, Nband, ntot, K1, Efermi, E, nel)
   write(*, '(a,i0,a,i0)') ' Nlf: ', nlf, ' Nlfd: ', Nlfd
 
   npwx = 0
@@ -106,6 +109,7 @@ program optabs_hip
   dims = rpa_dims_t(NkI, Nband, npwx, Nlfd, no, RPA_OPTABS, rpa_precision_of(precision), 0)
   call rpa_check(rpa_ctx_create(dims, ctx), c_null_ptr, 'rpa_ctx_create')
   call rpa_check(rpa_set_gvectors(ctx, ngm, mill, kc), ctx, 'rpa_set_gvectors')
+  if (nsym > 1) call rpa_check(rpa_set_symmetry(ctx, nsym, R), ctx, 'rpa_set_symmetry')
   allocate(millk(3, npwx), evc(npwx, Nband))
   do ik = 1, NkI
     call load_cs_full(trim(savedir), ik, Nband, millk, evc, igwx)
@@ -138,13 +142,19 @@ program optabs_hip
   p%eta_intra = eta_intra
   call rpa_check(rpa_set_params(ctx, p), ctx, 'rpa_set_params')
 
-  allocate(K1(ntot), K2(ntot), iG0(ntot), S0(-no:no, nlf, nlf), Qeff(nlf, nlf), pi11(no - 1), &
+  allocate(S0(-no:no, nlf, nlf), Qeff(nlf, nlf), pi11(no - 1), &
            sig(no - 1, 2))
   do iq = qmin, qmax
     call rpa_check(rpa_find_min_q(ntot, ktot, iq, q), ctx, 'findMinQ')
     call write_info(q)
-    call rpa_check(rpa_find_kq(ntot, ktot, NkI, ki, ngm, g, q, eps, K1, K2, iG0), ctx, 'findKQinBZ')
-    call rpa_check(rpa_set_q(ctx, ntot, K1, K2, iG0, ktot, q), ctx, 'rpa_set_q')
+    if (nsym > 1) then
+      call rpa_check(rpa_find_kq_sym(ntot, ktot, NkI, ki, ngm, g, q, nsym, RI, eps, R1, K1, R2, K2, iG0), &
+                     ctx, 'findKQinBZ')
+      call rpa_check(rpa_set_q_sym(ctx, ntot, K1, K2, iG0, R1, R2, ktot, q), ctx, 'rpa_set_q_sym')
+    else
+      call rpa_check(rpa_find_kq(ntot, ktot, NkI, ki, ngm, g, q, eps, K1, K2, iG0), ctx, 'findKQinBZ')
+      call rpa_check(rpa_set_q(ctx, ntot, K1, K2, iG0, ktot, q), ctx, 'rpa_set_q')
+    end if
 
     if (calc /= 2) then
       call rpa_check(rpa_run_graph(ctx), ctx, 'rpa_run_graph')

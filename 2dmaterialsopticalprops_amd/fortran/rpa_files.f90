@@ -171,18 +171,18 @@ contains
     close(u)
   end subroutine load_cs_full
 
-  ! Electron count of the FBZ sum, Nel = 2 * #(E < Efermi) / Ntot
-  ! (checkFBZintegration, Sloss.f90:761-822; with Nsym = 1 every FBZ k is its
-  ! own IBZ point).
-  subroutine check_fbz_integration(nki, nband, ntot, efermi, E, nel)
-    integer, intent(in) :: nki, nband, ntot
+  ! Electron count of the FBZ sum, Nel = 2 * #(E(K1(k), n) < Efermi) / Ntot
+  ! over the FBZ k and their IBZ points K1 (checkFBZintegration,
+  ! Sloss.f90:761-822, with findKinIBZ's K1).
+  subroutine check_fbz_integration(nki, nband, ntot, K1, efermi, E, nel)
+    integer, intent(in) :: nki, nband, ntot, K1(ntot)
     real(dp), intent(in) :: efermi, E(nki, nband)
     real(dp), intent(out) :: nel
     integer :: ik, n
     nel = 0.0_dp
     do ik = 1, ntot
       do n = 1, nband
-        if (E(ik, n) < efermi) nel = nel + 1.0_dp
+        if (E(K1(ik), n) < efermi) nel = nel + 1.0_dp
       end do
     end do
     nel = 2.0_dp * nel / ntot

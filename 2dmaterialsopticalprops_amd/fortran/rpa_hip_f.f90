@@ -83,6 +83,19 @@ module rpa_hip
       integer(c_int), intent(in) :: K1(*), K2(*), iG0(*)
       real(c_double), intent(in) :: ktot(3, *), q(3)
     end function
+    integer(c_int) function rpa_set_q_sym(ctx, ntot, K1, K2, iG0, R1, R2, ktot, q) bind(C, name="rpa_set_q_sym")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ntot
+      integer(c_int), intent(in) :: K1(*), K2(*), iG0(*), R1(*), R2(*)
+      real(c_double), intent(in) :: ktot(3, *), q(3)
+    end function
+    integer(c_int) function rpa_set_symmetry(ctx, nsym, R) bind(C, name="rpa_set_symmetry")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsym
+      real(c_double), intent(in) :: R(48, 3, 3)
+    end function
     integer(c_int) function rpa_run(ctx) bind(C, name="rpa_run")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -135,6 +148,20 @@ module rpa_hip
       real(c_double), intent(in) :: ktot(3, *), ki(3, *), g(3, *), q(3)
       real(c_double), value :: eps
       integer(c_int), intent(out) :: K1(*), K2(*), iG0(*)
+    end function
+    integer(c_int) function rpa_find_kq_sym(ntot, ktot, nki, ki, ng, g, q, nsym, RI, eps, R1, K1, R2, K2, iG0) &
+        bind(C, name="rpa_find_kq_sym")
+      import :: c_int, c_double
+      integer(c_int), value :: ntot, nki, ng, nsym
+      real(c_double), intent(in) :: ktot(3, *), ki(3, *), g(3, *), q(3), RI(48, 3, 3)
+      real(c_double), value :: eps
+      integer(c_int), intent(out) :: R1(*), K1(*), R2(*), K2(*), iG0(*)
+    end function
+    integer(c_int) function rpa_point_inverse(nsym, R, RI) bind(C, name="rpa_point_inverse")
+      import :: c_int, c_double
+      integer(c_int), value :: nsym
+      real(c_double), intent(in) :: R(48, 3, 3)
+      real(c_double), intent(out) :: RI(48, 3, 3)
     end function
     integer(c_int) function rpa_gen_glf(lf, ecut, gcar, ng, g, mill, nlfd, nlf, glf, parg, lf_idx) &
         bind(C, name="rpa_gen_glf")

@@ -41,11 +41,11 @@ program sloss_hip
   real(dp), parameter :: pi = 4.0_dp * atan(1.0_dp)
   character(len=256) :: cfgfile
   integer :: u, ios, nsym, ngm, ik, iq, io, igwx, nbnd, npwx, ntot, nlf, i, rc
-  real(dp) :: R(48, 3, 3), kc(3, 3), gcar, domega, nel, q(3), oi, fact
+  real(dp) :: R(48, 3, 3), RI(48, 3, 3), q0(3), kc(3, 3), gcar, domega, nel, q(3), oi, fact
   real(dp) :: kks, skk, wind, windkk, absq, err
   real(dp), allocatable :: ki(:, :), E(:, :), ktot(:, :), g(:, :), glf(:, :), V(:, :), loss(:), &
                            w1(:), w2(:)
-  integer, allocatable :: mill(:, :), millk(:, :), parg(:), lf_idx(:), K1(:), K2(:), iG0(:)
+  integer, allocatable :: mill(:, :), millk(:, :), parg(:), lf_idx(:), K1(:), K2(:), iG0(:), R1(:), R2(:)
   complex(dp), allocatable, target :: evc(:, :), WT(:, :, :)
   real(dp), allocatable, target :: Vt(:, :), losst(:)
   type(c_ptr) :: ctx
@@ -74,15 +74,11 @@ program sloss_hip
   domega = (omax - omin) / (no - 1)
 
   call point_r(trim(rundir)//trim(scf_file), nsym, R)
-  if (nsym /= 1) then
-    write(*, '(a,i0,a)') 'ERROR: Nsym = ', nsym, ': symmetry unfolding is not supported, '// &
-      'run QE without symmetry (nosym, noinv)'
-    error stop 1
-  end if
+  RI = 0.0_dp
+  call rpa_check(rpa_point_inverse(nsym, R, RI), c_null_ptr, 'PointR')
   allocate(ki(3, NkI), E(NkI, Nband), ktot(3, 48 * NkI))
   call load_ki_and_e(trim(rundir)//trim(band_file), NkI, Nband, Nocc, 1.0_dp / hartree, ki, E)
   call rpa_check(rpa_gen_fbz(NkI, nsym, eps, ki, R, ntot, ktot), c_null_ptr, 'genFBZ')
-  call check_fbz_integration(NkI, Nband, ntot, Efermi, E, nel)
   call load_kc(trim(rundir)//trim(scf_file), kc)
   call load_g(trim(savedir), ngm, mill)
   if (ngm /= NG) write(*, '(a,i0,a,i0)') 'note: NG = ', NG, ' in config, charge-density.dat has ', ngm
@@ -90,6 +86,13 @@ program sloss_hip
   call rpa_check(rpa_gcart(ngm, mill, kc, g), c_null_ptr, 'G vectors')
   call rpa_check(rpa_gen_glf(trim(lf)//c_null_char, Ecut, gcar, ngm, g, mill, Nlfd, nlf, glf, parg, &
                              lf_idx), c_null_ptr, 'genGlfandParity')
+  ! findKinIBZ for every FBZ k (K1 does not depend on q), then the FBZ
+  ! electron count over those IBZ points
+  allocate(R1(ntot), K1(ntot), R2(ntot), K2(ntot), iG0(ntot))
+  q0 = 0.0_dp
+  call rpa_check(rpa_find_kq_sym(ntot, ktot, NkI, ki, ngm, g, q0, nsym, RI, eps, R1, K1, R2, K2, iG0), &
+                 c_null_ptr, 'findKinIBZ')
+  call check_fbz_integration(NkI, Nband, ntot, K1, Efermi, E, nel)
   write(*, '(a,i0,a,i0)') ' Nlf: ', nlf, ' Nlfd: ', Nlfd
 
   npwx = 0
@@ -100,6 +103,7 @@ program sloss_hip
   dims = rpa_dims_t(NkI, Nband, npwx, Nlfd, no, RPA_SLOSS, rpa_precision_of(precision), 0)
   call rpa_check(rpa_ctx_create(dims, ctx), c_null_ptr, 'rpa_ctx_create')
   call rpa_check(rpa_set_gvectors(ctx, ngm, mill, kc), ctx, 'rpa_set_gvectors')
+  if (nsym > 1) call rpa_check(rpa_set_symmetry(ctx, nsym, R), ctx, 'rpa_set_symmetry')
   allocate(millk(3, npwx), evc(npwx, Nband))
   do ik = 1, NkI
     call load_cs_full(trim(savedir), ik, Nband, millk, evc, igwx)
@@ -124,13 +128,19 @@ program sloss_hip
   p%eta_intra = 0.0_dp
   call rpa_check(rpa_set_params(ctx, p), ctx, 'rpa_set_params')
 
-  allocate(K1(ntot), K2(ntot), iG0(ntot), WT(no - 1, nlf, nlf), Vt(nlf, nlf), losst(no - 1), &
+  allocate(WT(no - 1, nlf, nlf), Vt(nlf, nlf), losst(no - 1), &
            w1(no - 1), w2(no - 1))
   do iq = qmin, qmax
     call rpa_check(rpa_find_min_q(ntot, ktot, iq, q), ctx, 'findMinQ')
     call write_info(q)
-    call rpa_check(rpa_find_kq(ntot, ktot, NkI, ki, ngm, g, q, eps, K1, K2, iG0), ctx, 'findKQinBZ')
-    call rpa_check(rpa_set_q(ctx, ntot, K1, K2, iG0, ktot, q), ctx, 'rpa_set_q')
+    if (nsym > 1) then
+      call rpa_check(rpa_find_kq_sym(ntot, ktot, NkI, ki, ngm, g, q, nsym, RI, eps, R1, K1, R2, K2, iG0), &
+                     ctx, 'findKQinBZ')
+      call rpa_check(rpa_set_q_sym(ctx, ntot, K1, K2, iG0, R1, R2, ktot, q), ctx, 'rpa_set_q_sym')
+    else
+      call rpa_check(rpa_find_kq(ntot, ktot, NkI, ki, ngm, g, q, eps, K1, K2, iG0), ctx, 'findKQinBZ')
+      call rpa_check(rpa_set_q(ctx, ntot, K1, K2, iG0, ktot, q), ctx, 'rpa_set_q')
+    end if
     rc = rpa_run_graph(ctx)
     if (rc == RPA_E_SINGULAR) then
       print *, 'singular matrix in gjel'

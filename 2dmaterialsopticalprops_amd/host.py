@@ -132,22 +132,33 @@ def frequency_step(cfg, rc: RunConfig) -> float:
 
 
 class Runner:
-    """Sets up one synthetic system for one q on the device."""
+    """Sets up one synthetic system for one q on the device.
 
-    def __init__(self, sys_, rc: RunConfig, ecut_lf: Optional[float] = None):
+    ops = (R, nsym): point operations R[op, i, j] (cartesian, identity first)
+    of an irreducible k list; the FBZ is unfolded from it (genFBZ) and every
+    k carries its operations R1, R2 (findKinIBZ / findKQinBZ).  Without ops
+    the k list is the whole zone (Nsym = 1)."""
+
+    def __init__(self, sys_, rc: RunConfig, ecut_lf: Optional[float] = None, ops=None):
         from . import synth
         cfg = sys_.cfg
         self.sys, self.rc, self.cfg = sys_, rc, cfg
         prog = RPA_SLOSS if rc.program == "sloss" else RPA_OPTABS
         nki = len(sys_.ngk)
-        # FBZ from the IBZ list with the identity (genFBZ), q, k+q map
-        R = np.zeros((48, 3, 3), order="F")
-        R[0] = np.eye(3)
+        # FBZ from the IBZ list with the operations (genFBZ), q, k+q map
+        if ops is None:
+            R = np.zeros((48, 3, 3))
+            R[0] = np.eye(3)
+            nsym = 1
+        else:
+            R, nsym = np.asarray(ops[0], dtype=np.float64), int(ops[1])
+        self.R, self.nsym = R, nsym
+        Rf = np.asfortranarray(R)
         ktot = np.zeros((48 * nki, 3))
         import ctypes as C
         nt = C.c_int()
         ki = np.ascontiguousarray(sys_.kcart, dtype=np.float64)
-        rcode = _abi.lib().rpa_gen_fbz(nki, 1, rc.eps, ki.ctypes.data, R.ctypes.data, C.byref(nt),
+        rcode = _abi.lib().rpa_gen_fbz(nki, nsym, rc.eps, ki.ctypes.data, Rf.ctypes.data, C.byref(nt),
                                        ktot.ctypes.data)
         if rcode:
             raise _abi.RpaError(rcode, "rpa_gen_fbz")
@@ -155,7 +166,13 @@ class Runner:
         iq = cfg.iq if rc.iq is None else rc.iq
         self.q = _abi.find_min_q(self.ktot, iq)
         self.g = _abi.gcart(sys_.mill_g, sys_.kc)
-        self.K1, self.K2, self.iG0 = _abi.find_kq(self.ktot, ki, self.g, self.q, rc.eps)
+        self.RI = _abi.point_inverse(R, nsym) if nsym > 1 else None
+        self.R1 = self.R2 = None
+        if nsym > 1:
+            self.R1, self.K1, self.R2, self.K2, self.iG0 = _abi.find_kq_sym(self.ktot, ki, self.g, self.q, nsym,
+                                                                            self.RI, rc.eps)
+        else:
+            self.K1, self.K2, self.iG0 = _abi.find_kq(self.ktot, ki, self.g, self.q, rc.eps)
         self.ecut_lf = synth.local_field_ecut(sys_) if ecut_lf is None else ecut_lf
         nlfd = max(cfg.nlf_target, 1) + 10
         code, nlf, glf, parg, idx = _abi.gen_glf(cfg.lf, self.ecut_lf, sys_.gcar, self.g, sys_.mill_g, nlfd)
@@ -169,6 +186,8 @@ class Runner:
                                 device=rc.device)
         ctx = self.ctx
         ctx.set_gvectors(sys_.mill_g, sys_.kc)
+        if nsym > 1:
+            ctx.set_symmetry(R, nsym)
         for ik in range(nki):
             n = int(sys_.ngk[ik])
             ctx.upload_basis(ik + 1, sys_.mill_k[ik, :n], sys_.evc[ik, :, :n].T)
@@ -183,19 +202,30 @@ class Runner:
                        domega=self.domega, eta=eta, lor_cut=rc.lor_cut, df_cut=rc.df_cut,
                        efermi=rc.efermi_ev / HARTREE, temp=rc.temp_ev / HARTREE,
                        vcell=sys_.vcell, c0=cfg.c, eta_intra=rc.eta_intra_ev / HARTREE)
-        ctx.set_q(self.K1, self.K2, self.iG0, self.ktot, self.q)
+        self._set_q(self.K1, self.K2, self.iG0, self.R1, self.R2, self.q)
+
+    def _set_q(self, K1, K2, iG0, R1, R2, q):
+        if R1 is None:
+            self.ctx.set_q(K1, K2, iG0, self.ktot, q)
+        else:
+            self.ctx.set_q_sym(K1, K2, iG0, R1, R2, self.ktot, q)
 
     def q_point(self, iq: int, nlf: Optional[int] = None) -> dict:
         """Host setup of one q of the Sloss q loop (findMinQ, findKQinBZ,
         Sloss.f90:292-299): the k+q map and, with nlf given, the nlf local
         fields of smallest |q+G|.  Nothing is sent to the device."""
         q = _abi.find_min_q(self.ktot, iq)
-        K1, K2, iG0 = _abi.find_kq(self.ktot, self.sys.kcart, self.g, q, self.rc.eps)
-        if nlf is None:
-            idx, parg = self.lf_idx, self.parg
+        pt = {"iq": iq, "q": q}
+        if self.nsym > 1:
+            pt["R1"], pt["K1"], pt["R2"], pt["K2"], pt["iG0"] = _abi.find_kq_sym(
+                self.ktot, self.sys.kcart, self.g, q, self.nsym, self.RI, self.rc.eps)
         else:
-            idx, parg = local_fields_by_qg(self.g, self.sys.mill_g, self.sys.gcar, q, nlf)
-        return {"iq": iq, "q": q, "K1": K1, "K2": K2, "iG0": iG0, "lf_idx": idx, "parg": parg}
+            pt["K1"], pt["K2"], pt["iG0"] = _abi.find_kq(self.ktot, self.sys.kcart, self.g, q, self.rc.eps)
+        if nlf is None:
+            pt["lf_idx"], pt["parg"] = self.lf_idx, self.parg
+        else:
+            pt["lf_idx"], pt["parg"] = local_fields_by_qg(self.g, self.sys.mill_g, self.sys.gcar, q, nlf)
+        return pt
 
     def set_q_point(self, pt: dict) -> None:
         """Make pt (from q_point) the context's current q."""
@@ -203,8 +233,9 @@ class Runner:
             self.ctx.set_local_fields(pt["lf_idx"], pt["parg"])
             self.lf_idx, self.parg, self.nlf = pt["lf_idx"], pt["parg"], len(pt["lf_idx"])
             self.glf = self.g[self.lf_idx - 1].copy()      # Glf(:, n) = G(:, iG)
-        self.ctx.set_q(pt["K1"], pt["K2"], pt["iG0"], self.ktot, pt["q"])
+        self._set_q(pt["K1"], pt["K2"], pt["iG0"], pt.get("R1"), pt.get("R2"), pt["q"])
         self.q, self.K1, self.K2, self.iG0 = pt["q"], pt["K1"], pt["K2"], pt["iG0"]
+        self.R1, self.R2 = pt.get("R1"), pt.get("R2")
 
     def close(self):
         self.ctx.close()

@@ -301,7 +301,11 @@ def write_band_file(path: str, sys: SynthSystem) -> None:
                 f.write("".join(f"{x:8.3f}" for x in e[s:s + 10]) + "\n")
 
 
-def write_scf_out(path: str, sys: SynthSystem) -> None:
+def write_scf_out(path: str, sys: SynthSystem, ops=None) -> None:
+    """scf.out with the reciprocal axes (6 decimals) and the point operations:
+    the identity, or ops = (R, nsym) with R[op] cartesian, each written as
+    pw.x does (the crystal form, then the cartesian rows to 7 decimals, the
+    values from column 20 as PointR reads them)."""
     kc = sys.kc
     lines = [
         "",
@@ -319,21 +323,25 @@ def write_scf_out(path: str, sys: SynthSystem) -> None:
         "        X1            14.00    95.95000     X1( 1.00)",
         "        X2             6.00    32.06000     X2( 1.00)",
         "",
-        "     1 Sym. Ops. (no inversion) found",
-        "",
-        "                                    s                        frac. trans.",
-        "",
-        "      isym =  1     identity",
-        "",
-        " cryst.   s( 1) = (     1          0          0      )",
-        "                  (     0          1          0      )",
-        "                  (     0          0          1      )",
-        "",
-        " cart.    s( 1) = (  1.0000000  0.0000000  0.0000000 )",
-        "                 (  0.0000000  1.0000000  0.0000000 )",
-        "                 (  0.0000000  0.0000000  1.0000000 )",
-        "",
     ]
+    if ops is None:
+        R, nsym = np.eye(3)[None], 1
+    else:
+        R, nsym = np.asarray(ops[0]), int(ops[1])
+    kci = np.linalg.inv(kc)
+    lines += [f"     {nsym} Sym. Ops. (no inversion) found", "",
+              "                                    s                        frac. trans.", ""]
+    for op in range(nsym):
+        cr = np.rint(kci @ R[op] @ kc).astype(int).T  # crystal form (rows of the direct-lattice operation)
+        lines += [f"      isym = {op + 1:2d}     {'identity' if op == 0 else 'operation'}", ""]
+        for i in range(3):
+            head = f" cryst.   s({op + 1:2d}) = (" if i == 0 else " " * 18 + "("
+            lines.append(head + "".join(f"{cr[i, j]:6d}     " for j in range(3)) + " )")
+        lines.append("")
+        for i in range(3):
+            head = f" cart.    s({op + 1:2d}) = (" if i == 0 else " " * 18 + "("
+            lines.append(head + "".join(f"{R[op][i, j]:11.7f}" for j in range(3)) + " )")
+        lines.append("")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -447,11 +455,12 @@ def write_config(path: str, sys: SynthSystem, program: str, savedir: str,
         f.write(text)
 
 
-def write_all(rundir: str, sys: SynthSystem, program: str, extra: Optional[dict] = None) -> None:
-    """Complete run directory: save/, scf.out, synthetic.band, config.in."""
+def write_all(rundir: str, sys: SynthSystem, program: str, extra: Optional[dict] = None, ops=None) -> None:
+    """Complete run directory: save/, scf.out, synthetic.band, config.in
+    (ops = (R, nsym): the point operations of an irreducible k list)."""
     os.makedirs(rundir, exist_ok=True)
     write_save(os.path.join(rundir, "save"), sys)
-    write_scf_out(os.path.join(rundir, "scf.out"), sys)
+    write_scf_out(os.path.join(rundir, "scf.out"), sys, ops)
     write_band_file(os.path.join(rundir, "synthetic.band"), sys)
     write_config(os.path.join(rundir, "config.in"), sys, program, "./save/", extra=extra)
 

@@ -112,8 +112,17 @@ double rpa_last_run_ms(const rpa_ctx *ctx);
  * (Optabs_qe6.f90:1122-1183, Optabs.f90:652-680; Modules/io_qe.f90:11-71).
  * Local fields and the k+q map index this list, so after a new list both
  * must be set again (rpa_set_local_fields, rpa_set_q) before the next run;
- * until then the device calls return RPA_E_STATE.                          */
+ * until then the device calls return RPA_E_STATE.  A new list also clears
+ * the point operations (rpa_set_symmetry), whose Miller forms depend on KC. */
 int rpa_set_gvectors(rpa_ctx *ctx, int ngm, const int *mill, const double *kc);
+
+/* Point operations of the crystal, R(48,3,3) cartesian with the identity
+ * first, as PointR reads them from scf.out (Chi_RPA/PointR.f90:41-138); the
+ * inverses RI are computed as PointR does (rpa_point_inverse).  Needs
+ * rpa_set_gvectors first; each operation must map the reciprocal lattice onto
+ * itself (integer Miller form KC^-1 R KC within 1e-3), else RPA_E_ARG.  The
+ * k+q map must be set again afterwards (rpa_set_q_sym).                     */
+int rpa_set_symmetry(rpa_ctx *ctx, int nsym, const double *R);
 
 /* Bloch coefficients of IBZ k-point ik (1-based): the wfc{ik}.dat Miller list
  * mill(3,ngk) and evc(ngk,nband) complex128.  Replaces loadCsQE6_full /
@@ -137,6 +146,15 @@ int rpa_set_params(rpa_ctx *ctx, const rpa_params *p);
  * (findKinIBZ / findKQinBZ, Optabs.f90:306-320, Sloss.f90:328-343).       */
 int rpa_set_q(rpa_ctx *ctx, int ntot, const int *K1, const int *K2, const int *iG0,
               const double *ktot, const double *q);
+
+/* The same with point operations (Nsym > 1, after rpa_set_symmetry): R1(ntot)
+ * and R2(ntot), 1-based, the operations with k = R(R1) kI(K1) and
+ * k+q - G0 = R(R2) kI(K2) (findKinIBZ / findKQinBZ, Optabs.f90:776-869).
+ * The vertices then take C1 at G'' and C2 at RI(R2) (R(R1) G'' + Glf + G0),
+ * and the current vertex J has R(R1) G'' (Optabs.f90:1399-1449,
+ * Sloss.f90:907-946, Modules/vertices.f90:88-189,293-394).               */
+int rpa_set_q_sym(rpa_ctx *ctx, int ntot, const int *K1, const int *K2, const int *iG0,
+                  const int *R1, const int *R2, const double *ktot, const double *q);
 
 /* ---- the hot path (device) -------------------------------------------- */
 /* Plane-wave vertices rho (Sloss genMnmK1K2, Sloss.f90:883-952) or current
@@ -233,6 +251,11 @@ int rpa_set_autotune(rpa_ctx *ctx, int on);
  * _SIG (Optabs), sizes as for rpa_get_result with that q's Nlf.            */
 int rpa_batch_set(rpa_ctx *ctx, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot,
                   const int *K1, const int *K2, const int *iG0, const double *ktot, const double *q);
+/* The same with point operations: R1, R2 of q at offset q*ntot, as for
+ * rpa_set_q_sym.                                                           */
+int rpa_batch_set_sym(rpa_ctx *ctx, int nq, const int *nlf, const int *lf_idx, const int *parg, int ntot,
+                      const int *K1, const int *K2, const int *iG0, const int *R1, const int *R2,
+                      const double *ktot, const double *q);
 int rpa_batch_run(rpa_ctx *ctx);
 int rpa_batch_result(rpa_ctx *ctx, int iq, int which, double *out);
 
@@ -260,6 +283,17 @@ int rpa_find_min_q(int ntot, const double *ktot, int iq, double *q);
  * Nsym = 1: K1(ntot), K2(ntot), iG0(ntot) 1-based.  g is G(3,ng) cartesian. */
 int rpa_find_kq(int ntot, const double *ktot, int nki, const double *ki, int ng,
                 const double *g, const double *q, double eps, int *K1, int *K2, int *iG0);
+/* findKinIBZ + findKQinBZ with point operations for every FBZ k
+ * (Optabs.f90:306-320, 776-869): R1, K1, R2, K2, iG0 (ntot each, 1-based);
+ * RI(48,3,3) from rpa_point_inverse.  RPA_E_NOTFOUND where the reference
+ * stops ('Can not find wave vector ...').                                  */
+int rpa_find_kq_sym(int ntot, const double *ktot, int nki, const double *ki, int ng, const double *g,
+                    const double *q, int nsym, const double *RI, double eps, int *R1, int *K1, int *R2,
+                    int *K2, int *iG0);
+/* PointR's inverses RI(48,3,3) of R(48,3,3) by gjel's Gauss-Jordan
+ * elimination (Chi_RPA/PointR.f90:119-138, Chi_RPA/gjel.f); RPA_E_SINGULAR
+ * on a zero pivot.                                                         */
+int rpa_point_inverse(int nsym, const double *R, double *RI);
 /* genGlf / genGlfandParity: lf "z" or "xyz"; outputs glf(3,nlfd), parg,
  * lf_idx (1-based).  Returns RPA_E_LIMIT when Nlf > Nlfd.                  */
 int rpa_gen_glf(const char *lf, double ecut, double gcar, int ng, const double *g,

@@ -400,7 +400,8 @@ static int orc_make_pair(const orc_problem *p, int ik, orc_kpair *kp) {
   return 0;
 }
 
-/* a6 with symmetry (Optabs.f90:1405-1449, Sloss.f90:917-946): plane wave i
+/* a6 with symmetry (Optabs.f90:1405-1449, Sloss.f90:917-946; the igk map of
+ * Modules/vertices.f90:130-160, 343-370 the same on the k lists): plane wave i
  * of C1 carries G'' = R(R1) G(i) (cartesian, in the reference's order), the
  * target is RI(R2) (G'' + Glf(G) + G(iG0)), matched against the G list of
  * C2 within eps per component.  The match takes the lattice vector nearest
@@ -487,10 +488,8 @@ static int orc_gfast_sym(const orc_problem *p, int ik, const orc_kpair *kp, int 
  * of C1's plane waves (rotated by R(R1) under symmetry).                    */
 static int orc_pair_maps(const orc_problem *p, int ik, const orc_kpair *kp, int nlf, const int *mlf, int *gfast,
                          double *l1) {
-  if (p->R1 && (p->R1[ik] != 1 || p->R2[ik] != 1)) {
-    if (p->map_mode != ORC_MAP_DIRECT) return -5;  /* unfolding follows the drivers' direct map */
+  if (p->R1 && (p->R1[ik] != 1 || p->R2[ik] != 1))  /* both maps: kp holds the mode's lists */
     return orc_gfast_sym(p, ik, kp, nlf, mlf, gfast, l1);
-  }
   orc_gfast_miller(kp->n1, kp->m1, kp->n2, kp->m2, nlf, mlf, kp->m0, gfast);
   if (l1) orc_gcart(kp->n1, kp->m1, p->kc, l1);
   return 0;

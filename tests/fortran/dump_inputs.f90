@@ -9,7 +9,7 @@ program dump_inputs
   integer :: i, nki, nband, nfill, nsym, ngm, ik, igwx, nbnd, u, npwx
   real(dp) :: shift, R(48, 3, 3), kc(3, 3), nel
   real(dp), allocatable :: ki(:, :), E(:, :)
-  integer, allocatable :: mill(:, :), millk(:, :)
+  integer, allocatable :: mill(:, :), millk(:, :), k1(:)
   complex(dp), allocatable :: evc(:, :)
   do i = 1, 6
     call get_command_argument(i, a(i))
@@ -22,10 +22,12 @@ program dump_inputs
   call load_kc(trim(a(1))//'scf.out', kc)
   allocate(ki(3, nki), E(nki, nband))
   call load_ki_and_e(trim(a(1))//'synthetic.band', nki, nband, nfill, shift, ki, E)
-  call check_fbz_integration(nki, nband, nki, 0.0_dp, E, nel)
+  allocate(k1(nki))
+  k1 = [(ik, ik = 1, nki)]
+  call check_fbz_integration(nki, nband, nki, k1, 0.0_dp, E, nel)
   call load_g(trim(a(2)), ngm, mill)
   open(newunit=u, file='dump.bin', access='stream', form='unformatted', status='replace')
-  write(u) nsym, R(1, :, :), kc, ki, E, nel, ngm, mill
+  write(u) nsym, R(1:nsym, :, :), kc, ki, E, nel, ngm, mill
   npwx = 0
   do ik = 1, nki
     call wfc_size(trim(a(2)), ik, igwx, nbnd)

@@ -188,3 +188,102 @@ def test_driver_rejects_unknown_precision(tmp_path):
     r = subprocess.run([_exe("sloss_hip.x")], cwd=run, capture_output=True, text=True, timeout=120)
     assert r.returncode != 0
     assert "precision must be" in r.stdout + r.stderr
+
+
+def _as_read(ibz, R):
+    """The symmetric system's inputs as the drivers parse them: KC and k to the
+    6 decimals of scf.out and the band file, R to the 7 of scf.out."""
+    import dataclasses
+    six = np.vectorize(lambda x: float(f"{x:.6f}"))
+    s2 = dataclasses.replace(ibz, kc=six(ibz.kc), kcart=six(ibz.kcart))
+    R2 = np.vectorize(lambda x: float(f"{x:.7f}"))(R)
+    return s2, R2
+
+
+def _sym_oracle(s2, R2, nsym, program, ecut, eps=1e-4):
+    """Oracle-side setup with point operations: genFBZ, PointR, findMinQ,
+    findKinIBZ / findKQinBZ over the operations, genGlf."""
+    import ctypes as C
+    cfg = s2.cfg
+    g = synth.gcart(s2.mill_g, s2.kc)
+    kt = np.zeros((3, 48 * len(s2.kcart)), order="F")
+    nt = C.c_int()
+    assert orc.lib().orc_gen_fbz(len(s2.kcart), nsym, eps, np.asfortranarray(s2.kcart.T), np.asfortranarray(R2),
+                                 C.byref(nt), kt) == 0
+    ktot = kt[:, :nt.value].T.copy()
+    RI = orc.point_inverse(R2, nsym)
+    q = orc.find_min_q(ktot, cfg.iq)
+    R1 = np.zeros(len(ktot), np.int32)
+    K1 = np.zeros(len(ktot), np.int32)
+    for ik in range(len(ktot)):
+        R1[ik], K1[ik] = orc.find_kin_ibz(ik + 1, s2.kcart, ktot[ik], nsym, RI, eps)
+    iG0, R2k, K2 = orc.find_kq_sym_all(ktot, s2.kcart, g, q, nsym, RI, eps)
+    rc, nlf, glf, parg, idx = orc.gen_glf(cfg.lf, ecut, s2.gcar, g, s2.mill_g, 1000)
+    assert rc == 0
+    pb = orc.Problem(s2, K1, K2, iG0, ktot, q, idx, map_mode=0, ngd=int(s2.ngk.min()),
+                     E_hartree=H.energies(s2, program), R=R2, RI=RI, R1=R1, R2=R2k, eps=eps)
+    pb.glf, pb.parg = glf, parg
+    assert (R1 != 1).any() and (R2k != 1).any()
+    return pb
+
+
+def _symmetric_pair(kind):
+    import dataclasses
+    cfg = dataclasses.replace(synth.CONFIGS["tiny"], nk=6, lf="xyz", nlf_target=9)
+    ibz, fz, R, nsym, kinfo, ngd = synth.make_symmetric_pair(cfg, kind, 90, 60)
+    assert ngd <= int(ibz.ngk.min())
+    return ibz, R, nsym
+
+
+def test_sloss_driver_unfolds_point_operations(tmp_path):
+    """Nsym > 1 in scf.out: the driver inverts the operations (PointR), finds
+    R1/K1 and R2/K2 of every FBZ k and runs the unfolded chain; its loss and
+    W against the oracle on the inputs as the driver reads them."""
+    ibz, R, nsym = _symmetric_pair("c6")
+    cfg = ibz.cfg
+    run = tmp_path / "sloss_sym"
+    synth.write_all(str(run), ibz, "sloss", ops=(R, nsym))
+    out = _run(_exe("sloss_hip.x"), run)
+    s2, R2 = _as_read(ibz, R)
+    pb = _sym_oracle(s2, R2, nsym, "sloss", synth.local_field_ecut(ibz))
+    dw = H.sloss_grid(cfg)
+    S0 = orc.sloss_s0(pb, cfg.nval, cfg.no, dw, cfg.eta_ev / H.HARTREE, s2.vcell)
+    chi0 = orc.kk_sloss(S0, dw)
+    V = orc.genV(1e-4, pb.q[0] * s2.gcar, pb.q[1] * s2.gcar, cfg.c, pb.parg, pb.glf, s2.gcar)
+    outs = [orc.dyson_w(chi0[f], V) for f in range(cfg.no - 1)]
+    E = np.array([o[0] for o in outs])
+    W = np.array([o[2] for o in outs])
+    loss = _table(run / f"Loss_Q{cfg.iq}")
+    assert H.rel_max(loss[:, 1], -E[:, 0, 0].imag) < 1e-10
+    re_w = _table(run / "fort.10008")
+    im_w = _table(run / "fort.20008")
+    assert H.rel_max(re_w[:, 1] + 1j * im_w[:, 1], W[:, 0, 0]) < 1e-10
+    info = open(run / "Info").read()
+    assert re.search(r"Number of point symmetry operation is\s+%d" % nsym, info), out
+
+
+def test_optabs_driver_unfolds_point_operations(tmp_path):
+    ibz, R, nsym = _symmetric_pair("c3v")
+    cfg = ibz.cfg
+    run = tmp_path / "optabs_sym"
+    synth.write_all(str(run), ibz, "optabs", extra={"pol": "yy", "calc": 3}, ops=(R, nsym))
+    _run(_exe("optabs_hip.x"), run)
+    s2, R2 = _as_read(ibz, R)
+    pb = _sym_oracle(s2, R2, nsym, "optabs", synth.local_field_ecut(ibz))
+    dw = H.optabs_grid(cfg)
+    S0, Q = orc.optabs_s0(pb, cfg.nval, cfg.no, 2, dw, cfg.eta_ev / H.HARTREE, 1e-3, 1e-3, 0.0, 0.0,
+                          s2.vcell, s2.gcar)
+    nlf = S0.shape[1]
+    lines = open(run / "Corrfun_yy").read().splitlines()
+    got = np.array([_floats(l) for k, l in enumerate(lines) if k % (nlf * nlf + 1)])
+    got = (got[:, 0] + 1j * got[:, 1]).reshape(2 * cfg.no + 1, nlf, nlf)
+    quant = 0.51e-10  # half the last decimal of F15.10, per component
+    assert np.abs((got - S0).view(np.float64)).max() <= quant + 1e-12 * np.abs(S0).max()
+    gq = _table(run / "Qeff_yy")
+    gq = (gq[:, 0] + 1j * gq[:, 1]).reshape(nlf, nlf)
+    assert np.abs((gq - Q).view(np.float64)).max() <= quant + 1e-12 * np.abs(Q).max()
+    # the second part from the files, as the reference does
+    Pi0 = orc.kk_optabs(got, dw)
+    pi11, si, sa = orc.pi_assembly(Pi0, gq, dw, 0.01 / H.HARTREE, cfg.c)
+    pr = _table(run / "Pi_RPA_yy")
+    assert H.rel_max(pr[:, 1] + 1j * pr[:, 2], pi11) < 1e-10

@@ -28,12 +28,13 @@ def dump_exe(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("program", ["sloss", "optabs"])
-def test_readers_return_the_written_inputs(dump_exe, tmp_path, program):
+@pytest.mark.parametrize("program,ops", [("sloss", None), ("optabs", None), ("sloss", "c3v")])
+def test_readers_return_the_written_inputs(dump_exe, tmp_path, program, ops):
     s = H.system("tiny")
     cfg = s.cfg
     run = tmp_path / "run"
-    synth.write_all(str(run), s, program)
+    R_w, nsym_w = synth.hex_point_ops(ops) if ops else (np.eye(3)[None], 1)
+    synth.write_all(str(run), s, program, ops=(R_w, nsym_w) if ops else None)
     nki, nb, nv = len(s.ngk), cfg.nband, cfg.nval
     shift = 1.0 / H.HARTREE if program == "sloss" else 0.0
     subprocess.run([dump_exe, str(run) + "/", str(run / "save"), str(nki), str(nb), str(nv),
@@ -48,15 +49,17 @@ def test_readers_return_the_written_inputs(dump_exe, tmp_path, program):
         return a
 
     nsym = take(np.int32, 1)[0]
-    R = take(np.float64, 9).reshape(3, 3, order="F")
+    R = take(np.float64, 9 * nsym).reshape(nsym, 3, 3, order="F")
     kc = take(np.float64, 9).reshape(3, 3, order="F")
     ki = take(np.float64, 3 * nki).reshape(nki, 3)
     E = take(np.float64, nki * nb).reshape(nki, nb, order="F")
     nel = take(np.float64, 1)[0]
     ngm = take(np.int32, 1)[0]
     mill = take(np.int32, 3 * ngm).reshape(ngm, 3)
-    assert nsym == 1
-    np.testing.assert_array_equal(R, np.eye(3))
+    assert nsym == nsym_w
+    # the cartesian rows of scf.out carry 7 decimals
+    np.testing.assert_allclose(R, R_w[:nsym], rtol=0, atol=5.1e-8)
+    np.testing.assert_array_equal(R[0], np.eye(3))
     np.testing.assert_array_equal(kc, s.kc)
     np.testing.assert_allclose(ki, s.kcart, rtol=0, atol=5e-7)
     # the band file holds meV-rounded eV values (F8.3), read back exactly

@@ -196,3 +196,32 @@ def test_gen_fbz_and_find_min_q_match_the_pinned_oracle_on_symmetric_inputs():
     k = np.array([[0, 0, 0], [0.3, 0, 0], [0.1, 0, 0], [0, 0.1, 0]], dtype=np.float64)
     for order in ([0, 1, 2, 3], [0, 1, 3, 2]):
         np.testing.assert_array_equal(abi.find_min_q(k[order], 4), orc.find_min_q(k[order], 4))
+
+
+@pytest.mark.parametrize("kind", ["c2", "c3", "c6", "c3v"])
+def test_point_inverse_and_find_kq_sym_match_the_pinned_oracle(kind):
+    """PointR's inverses and the k bookkeeping with point operations
+    (findKinIBZ / findKQinBZ over the operations) equal the oracle's, which
+    is pinned against the same system written out on the whole zone."""
+    import test_oracle as T
+    cfg, ibz, fz, pb_s, pb_f, nsym = T._sym_setup(kind, 6)
+    R, _ = synth.hex_point_ops(kind)
+    RI = abi.point_inverse(R, nsym)
+    np.testing.assert_array_equal(RI[:nsym], orc.point_inverse(R, nsym)[:nsym])
+    g = synth.gcart(ibz.mill_g, ibz.kc)
+    R1, K1, R2, K2, iG0 = abi.find_kq_sym(pb_s.ktot, ibz.kcart, g, pb_s.q, nsym, RI, 1e-4)
+    for a, b in ((R1, pb_s.R1), (K1, pb_s.K1), (R2, pb_s.R2), (K2, pb_s.K2), (iG0, pb_s.iG0)):
+        np.testing.assert_array_equal(a, b)
+    # a k+q off the grid: the reference's stop
+    with pytest.raises(abi.RpaError) as e:
+        abi.find_kq_sym(pb_s.ktot, ibz.kcart, g, pb_s.q + np.array([0.0123, 0.0, 0.0]), nsym, RI, 1e-4)
+    assert e.value.code == abi.RPA_E_NOTFOUND
+
+
+def test_point_inverse_reports_a_singular_operation():
+    R = np.zeros((48, 3, 3))
+    R[0] = np.eye(3)
+    R[1] = np.diag([1.0, 1.0, 0.0])
+    with pytest.raises(abi.RpaError) as e:
+        abi.point_inverse(R, 2)
+    assert e.value.code == abi.RPA_E_SINGULAR

@@ -619,7 +619,7 @@ def test_pi_assembly_two_fields_reads_pi11_after_each_update():
 # Nsym = 1.  The two must agree to rounding: a wrong rotation, inverse,
 # operation choice or index map breaks the equality.
 
-def _sym_setup(kind, nk, nlf_target=9):
+def _sym_setup(kind, nk, nlf_target=9, map_mode=0):
     import dataclasses
     cfg = dataclasses.replace(H.synth.CONFIGS["tiny"], nk=nk, lf="xyz", nlf_target=nlf_target)
     ibz, fz, R, nsym, kinfo, ngd = H.synth.make_symmetric_pair(cfg, kind, 90, 60)
@@ -642,19 +642,20 @@ def _sym_setup(kind, nk, nlf_target=9):
     iG0, R2, K2 = orc.find_kq_sym_all(ktot, ibz.kcart, g, q, nsym, RI, 1e-4)
     ecut = H.synth.local_field_ecut(ibz)
     rc, nlf, glf, parg, idx = orc.gen_glf("xyz", ecut, ibz.gcar, g, ibz.mill_g, 100)
-    pb_s = orc.Problem(ibz, K1, K2, iG0, ktot, q, idx, ngd=ngd, E_hartree=H.energies(ibz, "sloss"),
-                       R=R, RI=RI, R1=R1, R2=R2)
+    pb_s = orc.Problem(ibz, K1, K2, iG0, ktot, q, idx, map_mode=map_mode, ngd=ngd,
+                       E_hartree=H.energies(ibz, "sloss"), R=R, RI=RI, R1=R1, R2=R2)
     K2f, iG0f = orc.find_kq_all(ktot, ktot, g, q, 1e-4)
-    pb_f = orc.Problem(fz, np.arange(1, len(ktot) + 1, dtype=np.int32), K2f, iG0f, ktot, q, idx, ngd=ngd,
-                       E_hartree=H.energies(fz, "sloss"))
+    pb_f = orc.Problem(fz, np.arange(1, len(ktot) + 1, dtype=np.int32), K2f, iG0f, ktot, q, idx,
+                       map_mode=map_mode, ngd=ngd, E_hartree=H.energies(fz, "sloss"))
     # the unfolded k+q point is the same FBZ point in both descriptions
     assert np.array_equal(iG0, iG0f)
     return cfg, ibz, fz, pb_s, pb_f, nsym
 
 
-@pytest.mark.parametrize("kind,nk", [("c2", 4), ("c3", 6), ("c6", 6), ("c3v", 6)])
-def test_unfolded_vertices_and_s0_equal_the_whole_zone(kind, nk):
-    cfg, ibz, fz, pb_s, pb_f, nsym = _sym_setup(kind, nk)
+@pytest.mark.parametrize("kind,nk,map_mode", [("c2", 4, 0), ("c3", 6, 0), ("c6", 6, 0), ("c3v", 6, 0),
+                                               ("c3v", 6, 1)])
+def test_unfolded_vertices_and_s0_equal_the_whole_zone(kind, nk, map_mode):
+    cfg, ibz, fz, pb_s, pb_f, nsym = _sym_setup(kind, nk, map_mode=map_mode)
     nv, nb = cfg.nval, cfg.nband
     assert nsym > 1 and (pb_s.R1 != 1).any() and (pb_s.R2 != 1).any()
     v_s = orc.vertices(pb_s, 0, ibz.gcar, 0, nv, nv, nb)
