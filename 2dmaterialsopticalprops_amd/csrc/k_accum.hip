@@ -721,6 +721,307 @@ __global__ __launch_bounds__(256, (NCB <= 4) ? 2 : 1) void k_accum_mx(AccMxArgs 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision, home panels.  A transition's Lorentzian window starts in
+// its home tile b and spans at most 2 hw / domega + 5 rows, so it lies in
+// the row panel [32 b, 32 b + 16 NRB).  One work item is a chunk of up to
+// kChunkHp transitions of one home tile with that whole panel, so every
+// transition enters exactly one item and its products
+//   P_t(G, G') = x_t(G) conj y_t(G')     (Re for Sloss; Re and Im for Optabs)
+// are formed once:  S0(w, col) += sum_t W(w, t) P_t(col),  A = W (panel rows
+// x transitions), B = P (transitions x columns), K = transitions, on the BF16
+// matrix cores as hi*hi + hi*lo + lo*hi with FP32 accumulation over the chunk.
+// Columns come in blocks of 16 with G fixed and G' = G'0 .. G'0 + 15.  The
+// real S0 of Sloss is symmetric, so only G' >= G is formed (blocks G'0 = G +
+// 16 j, G-major); the complex S0 of Optabs takes every G' (G'0 = 16 j).
+constexpr int kHpRowA = 40;     // bf16 per staged weight row (32 + pad: conflict-free 16-byte reads)
+constexpr int kHpRowX = 34;     // float2 per staged vertex row (32 + pad)
+
+// (G, G'0) of column block bl (hp_nblocks gives the count)
+__device__ inline void hp_block(int bl, int nlf, bool cplx, int &G, int &G0) {
+  if (cplx) {
+    const int ncb = (nlf + 15) / 16;
+    G = bl / ncb;
+    G0 = 16 * (bl - G * ncb);
+    return;
+  }
+  int g = 0, rem = bl;
+  while (g < nlf) {
+    const int nb = (nlf - g + 15) / 16;
+    if (rem < nb) break;
+    rem -= nb;
+    ++g;
+  }
+  G = g;
+  G0 = g + 16 * rem;
+}
+
+// W of every item and stage as BF16 hi and lo parts:
+// tab[(((item * nst + s) * 2 + hl) * rows + row) * 32 + tt] = part hl of
+// W(32 b + row, t_lo + 32 s + tt), zero past t_hi and past nw.  The weight is
+// the reference's FP64 arithmetic and cutoff test, rounded to FP32 and split.
+// (Formed inside the panel kernel instead, from the energies, the weights
+// made that kernel VALU-bound: 0.26 vs 0.23 ms with this table at N_G = 51.)
+__global__ __launch_bounds__(256) void k_wsplit(const double *te, const int *items, const int *nitems, int nst,
+                                                int rows, int program, int shift, double domega, double eta,
+                                                double cut, double den, int nw, uint16_t *tab) {
+  const int item = blockIdx.x / nst, s = blockIdx.x - item * nst;
+  if (item >= nitems[0]) return;
+  const int b = items[3 * item], t_lo = items[3 * item + 1], t_hi = items[3 * item + 2];
+  const int t0 = t_lo + s * kHpKS;
+  if (t0 >= t_hi) return;
+  const int tt = threadIdx.x & 31, rg = threadIdx.x >> 5;
+  const int t = t0 + tt;
+  const bool tok = t < t_hi;
+  const double *e = te + 3 * (size_t)(tok ? t : t_lo);
+  const double e1 = e[0], e2 = e[1], z = e[2];
+  uint16_t *hi = tab + ((size_t)item * nst + s) * 2 * rows * kHpKS;
+  uint16_t *lo = hi + (size_t)rows * kHpKS;
+  for (int row = rg; row < rows; row += 8) {
+    const int w = b * kTileW + row;
+    double v = 0.0;
+    if (tok && w < nw) v = weight(program, w, shift, domega, eta, cut, den, e1, e2, z);
+    __bf16 h, l;
+    split_bf16((float)v, h, l);
+    hi[row * kHpKS + tt] = __builtin_bit_cast(uint16_t, h);
+    lo[row * kHpKS + tt] = __builtin_bit_cast(uint16_t, l);
+  }
+}
+
+struct AccHpArgs {
+  const uint16_t *tab;  // weights, k_wsplit
+  const float2 *x, *y;  // vertices at sorted positions, [t][nlf]; y == x unless two sets
+  const int *items, *nitems;
+  int nlf, nbl, ngroup, nst;
+  float *part;  // [item][nbl][16 NRB rows][16][NS]
+};
+
+// One work unit = (item, group of WPG NCW column blocks): wave w takes
+// blocks (WPG group + w) NCW .. + NCW - 1.  The next stage's weights and
+// vertices are loaded into registers during the products, every load from a
+// clamped in-bounds address with its validity applied when the values are
+// stored (a conditional load joined with a zero fill makes the compiler wait
+// for each load before the next issues).  XU: staged vertex values per
+// thread and set (32 nlf <= 64 WPG XU).
+template <bool CPLX, bool TWO, int NRB, int NCW, int XU, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_accum_hp(AccHpArgs a) {
+  constexpr int NS = CPLX ? 2 : 1;
+  constexpr int ROWS = 16 * NRB;
+  constexpr int NT = 64 * WPG;
+  constexpr int NAU = (ROWS * 8 + NT - 1) / NT;  // 16-byte weight units per thread
+  extern __shared__ __align__(16) unsigned char hp_raw[];
+  uint16_t *sA = (uint16_t *)hp_raw;                                        // [2][ROWS][kHpRowA]
+  float2 *sX = (float2 *)(hp_raw + sizeof(uint16_t) * 2 * ROWS * kHpRowA);  // [nlf + 1][kHpRowX]
+  float2 *sY = TWO ? sX + (size_t)(a.nlf + 1) * kHpRowX : sX;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  const int nlf = a.nlf;
+  // row nlf of the staged vertices stays zero: columns G' >= nlf read it
+  for (int x = tid; x < kHpRowX; x += NT) {
+    sX[(size_t)nlf * kHpRowX + x] = make_float2(0.f, 0.f);
+    if (TWO) sY[(size_t)nlf * kHpRowX + x] = make_float2(0.f, 0.f);
+  }
+  // staging assignment, fixed for the kernel: weight unit q = tid + NT u ->
+  // LDS offset; vertex value e = tid + NT u of a stage -> (tt, G) -> offset
+  int ao[NAU];
+#pragma unroll
+  for (int u = 0; u < NAU; ++u) {
+    const int q = min(tid + NT * u, ROWS * 8 - 1);
+    const int hl = q / (ROWS * 4), row = (q >> 2) % ROWS, part = q & 3;
+    ao[u] = (hl * ROWS + row) * kHpRowA + 8 * part;
+  }
+  const int nxe = kHpKS * nlf;  // vertex values per stage and set
+  int xo[XU], xt[XU];
+#pragma unroll
+  for (int u = 0; u < XU; ++u) {
+    const int e = tid + NT * u;
+    const int tt = e / nlf, g = e - tt * nlf;
+    xt[u] = e < nxe ? tt : kHpKS;  // kHpKS: not a value of the stage
+    xo[u] = e < nxe ? g * kHpRowX + tt : 0;
+  }
+  const int xcd = blockIdx.x & 7, nsl = gridDim.x >> 3;
+  const int nix = (a.nitems[0] - xcd + 7) / 8;
+  const int total = nix * a.ngroup;
+
+  for (int wk = blockIdx.x >> 3; wk < total; wk += nsl) {
+    const int il = wk / a.ngroup, grp = wk - il * a.ngroup;
+    const int item = 8 * il + xcd;
+    const int t_lo = a.items[3 * item + 1], t_hi = a.items[3 * item + 2];
+    const int nstg = (t_hi - t_lo + kHpKS - 1) / kHpKS;
+    int Gb[NCW], Pb[NCW];
+    bool okb[NCW];
+#pragma unroll
+    for (int cb = 0; cb < NCW; ++cb) {
+      const int bl = (WPG * grp + wave) * NCW + cb;
+      okb[cb] = bl < a.nbl;
+      hp_block(okb[cb] ? bl : 0, nlf, CPLX, Gb[cb], Pb[cb]);
+    }
+    // 16-byte units as a native vector type: an array of HIP's uint4 struct
+    // here was kept in scratch memory
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 *tsrc = (const u32x4 *)(a.tab + (size_t)item * a.nst * 2 * ROWS * kHpKS);
+
+    rpa_f4 acc[NRB][NCW][NS];
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < NCW; ++cb)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[rb][cb][s] = rpa_f4{0.f, 0.f, 0.f, 0.f};
+
+    u32x4 ra[NAU];
+    float2 rx[XU], ry[XU];
+    int nvl = 0;  // valid transitions of the stage in registers
+    auto load = [&](int st) {
+      const u32x4 *src = tsrc + (size_t)st * (ROWS * 8);
+#pragma unroll
+      for (int u = 0; u < NAU; ++u) ra[u] = src[min(tid + NT * u, ROWS * 8 - 1)];
+      const int t0 = t_lo + st * kHpKS;
+      nvl = min(kHpKS, t_hi - t0);
+      const int last = nvl * nlf - 1;  // the stage's last valid value
+      const float2 *xs = a.x + (size_t)t0 * nlf;
+      const float2 *ys = a.y + (size_t)t0 * nlf;
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        const int e = min(tid + NT * u, last);
+        rx[u] = xs[e];
+        if (TWO) ry[u] = ys[e];
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int u = 0; u < NAU; ++u)
+        if (tid + NT * u < ROWS * 8) *(u32x4 *)(sA + ao[u]) = ra[u];
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        if (xt[u] < kHpKS) {
+          const bool ok = xt[u] < nvl;
+          sX[xo[u]] = ok ? rx[u] : make_float2(0.f, 0.f);
+          if (TWO) sY[xo[u]] = ok ? ry[u] : make_float2(0.f, 0.f);
+        }
+      }
+    };
+
+    load(0);
+    for (int st = 0; st < nstg; ++st) {
+      __syncthreads();  // the previous stage's reads are done
+      store();
+      __syncthreads();
+      if (st + 1 < nstg) load(st + 1);
+      // B fragments of every block: P = x conj y for the lane's column and
+      // the transitions 8 kg .. 8 kg + 7, split into BF16 hi and lo
+      rpa_bf8 bh[NCW], bl[NCW], ch[NCW], cl[NCW];
+#pragma unroll
+      for (int cb = 0; cb < NCW; ++cb) {
+        const int gp = Pb[cb] + r;
+        const float2 *xrow = sX + (size_t)Gb[cb] * kHpRowX + 8 * kg;
+        const float2 *yrow = sY + (size_t)(gp < nlf ? gp : nlf) * kHpRowX + 8 * kg;
+#pragma unroll
+        for (int v = 0; v < 8; v += 2) {
+          const float4 p = *(const float4 *)(xrow + v), q = *(const float4 *)(yrow + v);
+          const float xr[2] = {p.x, p.z}, xi[2] = {p.y, p.w}, yr[2] = {q.x, q.z}, yi[2] = {q.y, q.w};
+#pragma unroll
+          for (int h2 = 0; h2 < 2; ++h2) {
+            __bf16 h, l;
+            split_bf16(xr[h2] * yr[h2] + xi[h2] * yi[h2], h, l);  // Re(x conj y)
+            bh[cb][v + h2] = h;
+            bl[cb][v + h2] = l;
+            if (CPLX) {
+              split_bf16(xi[h2] * yr[h2] - xr[h2] * yi[h2], h, l);  // Im(x conj y)
+              ch[cb][v + h2] = h;
+              cl[cb][v + h2] = l;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) {
+        const rpa_bf8 ah = *(const rpa_bf8 *)(sA + (16 * rb + r) * kHpRowA + 8 * kg);
+        const rpa_bf8 al = *(const rpa_bf8 *)(sA + (ROWS + 16 * rb + r) * kHpRowA + 8 * kg);
+#pragma unroll
+        for (int cb = 0; cb < NCW; ++cb) {
+          rpa_f4 c = acc[rb][cb][0];
+          c = mfma_bf16_16x16x32(ah, bh[cb], c);
+          c = mfma_bf16_16x16x32(ah, bl[cb], c);
+          c = mfma_bf16_16x16x32(al, bh[cb], c);
+          acc[rb][cb][0] = c;
+          if (CPLX) {
+            rpa_f4 d = acc[rb][cb][NS - 1];
+            d = mfma_bf16_16x16x32(ah, ch[cb], d);
+            d = mfma_bf16_16x16x32(ah, cl[cb], d);
+            d = mfma_bf16_16x16x32(al, ch[cb], d);
+            acc[rb][cb][NS - 1] = d;
+          }
+        }
+      }
+    }
+    float *dst = a.part + (size_t)item * a.nbl * ROWS * 16 * NS;
+#pragma unroll
+    for (int cb = 0; cb < NCW; ++cb) {
+      if (!okb[cb]) continue;
+      const int bl = (WPG * grp + wave) * NCW + cb;
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = 16 * rb + 4 * kg + e;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) dst[(((size_t)bl * ROWS + row) * 16 + r) * NS + s] = acc[rb][cb][s][e];
+        }
+    }
+  }
+}
+
+// S0[w][col] from the panels: the items of every home tile b whose panel
+// [32 b, 32 b + rows) holds row w, in tile then item order, summed in FP64
+// (Neumaier).  Thread = (w, column block, column r, part s); the symmetric
+// real S0 writes (G, G') and (G', G).  Columns past ncol are set to zero.
+__global__ void k_reduce_hp(const float *part, const int *first_item, int rows, int nlf, int nbl, int cplx,
+                            int nw, int ncol, int ncol_pad, double *s0) {
+  const int NS = cplx ? 2 : 1;
+  const long long per_w = (long long)nbl * 16 * NS;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long npad = (long long)nw * (ncol_pad - ncol);
+  if (idx >= (long long)nw * per_w) {
+    const long long z = idx - (long long)nw * per_w;
+    if (z < npad) {
+      const int w = (int)(z / (ncol_pad - ncol)), c = ncol + (int)(z - (long long)w * (ncol_pad - ncol));
+      s0[(size_t)w * ncol_pad + c] = 0.0;
+    }
+    return;
+  }
+  const int w = (int)(idx / per_w);
+  const int rest = (int)(idx - (long long)w * per_w);
+  const int bl = rest / (16 * NS), rr = (rest / NS) % 16, s = rest % NS;
+  int G, G0;
+  hp_block(bl, nlf, cplx != 0, G, G0);
+  const int gp = G0 + rr;
+  if (gp >= nlf) return;
+  const int b_hi = w / kTileW;
+  const int b_lo = w >= rows ? (w - rows) / kTileW + 1 : 0;
+  double sum = 0.0, comp = 0.0;
+  for (int b = b_lo; b <= b_hi; ++b) {
+    const int row = w - b * kTileW;
+    for (int it = first_item[b]; it < first_item[b + 1]; ++it) {
+      const double v = (double)part[((((size_t)it * nbl + bl) * rows + row) * 16 + rr) * NS + s];
+      const double tt = sum + v;
+      if (fabs(sum) >= fabs(v))
+        comp += (sum - tt) + v;
+      else
+        comp += (v - tt) + sum;
+      sum = tt;
+    }
+  }
+  const double val = sum + comp;
+  double *row_out = s0 + (size_t)w * ncol_pad;
+  if (cplx) {
+    row_out[2 * ((size_t)G * nlf + gp) + s] = val;
+  } else {
+    row_out[(size_t)G * nlf + gp] = val;
+    if (gp != G) row_out[(size_t)gp * nlf + G] = val;
+  }
+}
+
 // S0[w][col] = sum over the chunks of tile w/32 of their partials (Neumaier,
 // in FP64; the partials are FP64 from k_accum, FP32 from k_accum_mx).
 template <class T>
@@ -818,7 +1119,7 @@ int launch_transitions(rpa_ctx *c) {
   sa.te = c->d_te;
   hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(256), 0, c->stream, sa);
   hipLaunchKernelGGL(k_worklist, dim3(1), dim3(1024), 0, c->stream, (const int *)c->d_toff,
-                     c->ntiles, c->span, c->d.precision == RPA_MIXED ? kChunkTMx : kChunkT,
+                     c->ntiles, c->wl_span, c->wl_chunk,
                      c->d_items, c->d_first_item, c->d_nitems);
   RPA_HIP(c, hipGetLastError());
   return 0;
@@ -847,6 +1148,76 @@ static int launch_mx_ncb(rpa_ctx *c, AccMxArgs &a, long long items) {
   return 0;
 }
 
+// Home-panel launch: NRB row blocks, XU staged vertex values per thread.
+// Column blocks per wave: as many as fit 256 registers without spills
+// (measured at N_G = 51, 112 panel rows: 2 blocks 0.25 ms, 3 blocks 0.205
+// ms; workgroups of 4 waves, two per CU, were no faster than one of 8); the
+// complex S0 keeps two accumulator sets per block.
+constexpr int kHpWaves = 8;
+constexpr int hp_ncw(bool cplx, int nrb, int xu) {
+  return cplx ? ((nrb <= 4 && xu == 4) ? 2 : 1) : ((nrb >= 7 && xu == 8) ? 2 : 3);
+}
+template <bool CPLX, bool TWO, int NRB>
+static const void *hp_kernel(int nlf) {
+  if (nlf <= 64) return (const void *)k_accum_hp<CPLX, TWO, NRB, hp_ncw(CPLX, NRB, 4), 4, kHpWaves>;
+  return (const void *)k_accum_hp<CPLX, TWO, NRB, hp_ncw(CPLX, NRB, 8), 8, kHpWaves>;
+}
+template <bool CPLX, bool TWO>
+static const void *hp_kernel_nrb(int nrb, int nlf) {
+  switch (nrb) {
+    case 4: return hp_kernel<CPLX, TWO, 4>(nlf);
+    case 5: return hp_kernel<CPLX, TWO, 5>(nlf);
+    case 6: return hp_kernel<CPLX, TWO, 6>(nlf);
+    case 7: return hp_kernel<CPLX, TWO, 7>(nlf);
+    default: return hp_kernel<CPLX, TWO, 8>(nlf);
+  }
+}
+static const void *hp_kernel_of(const rpa_ctx *c) {
+  const bool cplx = (c->d.program == RPA_OPTABS);
+  const bool two = cplx && c->d_rho2 != nullptr;
+  if (!cplx) return hp_kernel_nrb<false, false>(c->hp_nrb, c->nlf);
+  return two ? hp_kernel_nrb<true, true>(c->hp_nrb, c->nlf) : hp_kernel_nrb<true, false>(c->hp_nrb, c->nlf);
+}
+static size_t hp_lds(const rpa_ctx *c) {
+  const bool two = (c->d.program == RPA_OPTABS) && c->d_rho2 != nullptr;
+  return sizeof(uint16_t) * 2 * 16 * (size_t)c->hp_nrb * kHpRowA +
+         sizeof(float2) * (size_t)(c->nlf + 1) * kHpRowX * (two ? 2 : 1);
+}
+
+static int launch_hp(rpa_ctx *c, int shift, double den) {
+  const bool cplx = (c->d.program == RPA_OPTABS);
+  const int rows = 16 * c->hp_nrb, nst = kChunkHp / kHpKS;
+  hipLaunchKernelGGL(k_wsplit, dim3((unsigned)(c->max_items * nst)), dim3(256), 0, c->stream,
+                     (const double *)c->d_te, (const int *)c->d_items, (const int *)c->d_nitems, nst, rows,
+                     c->d.program, shift, c->p.domega, c->p.eta, c->wcut, den, c->nw, c->d_wtab);
+  AccHpArgs a{};
+  a.tab = c->d_wtab;
+  a.x = (const float2 *)c->d_rho1;
+  a.y = (const float2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
+  a.items = c->d_items;
+  a.nitems = c->d_nitems;
+  a.nlf = c->nlf;
+  a.nbl = hp_nblocks(c->nlf, cplx);
+  const int ncw = hp_ncw(cplx, c->hp_nrb, c->nlf <= 64 ? 4 : 8);
+  a.ngroup = (a.nbl + kHpWaves * ncw - 1) / (kHpWaves * ncw);
+  a.nst = nst;
+  a.part = (float *)c->d_part;
+  const size_t lds = hp_lds(c);
+  if (lds > 64 * 1024 && lds > c->accum_lds_attr)
+    return fail(c, RPA_E_STATE, "accumulation LDS limit not configured");
+  const long long units = (long long)c->max_items * a.ngroup;
+  const int grid = (int)std::min<long long>(std::max<long long>((units + 7) / 8 * 8, 8), 8192);
+  const void *fn = hp_kernel_of(c);
+  void *args[] = {&a};
+  RPA_HIP(c, hipLaunchKernel(fn, dim3(grid), dim3(64 * kHpWaves), args, lds, c->stream));
+  if (c->profiling) RPA_HIP(c, hipEventRecord(c->prof_ev[3], c->stream));
+  const long long n = (long long)c->nw * a.nbl * 16 * (cplx ? 2 : 1) + (long long)c->nw * (c->ncol_pad - c->ncol);
+  hipLaunchKernelGGL(k_reduce_hp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                     (const float *)c->d_part, (const int *)c->d_first_item, rows, c->nlf, a.nbl, cplx ? 1 : 0,
+                     c->nw, c->ncol, c->ncol_pad, c->d_s0);
+  return 0;
+}
+
 // FP64 k_accum stages kKS vertices of every local field (both vertex sets
 // for Optabs) next to the weight rows: 4352 + 512 Nlf bytes for Optabs, past
 // the default 64 KB from Nlf = 120.  Outside any stream capture (prepare):
@@ -857,6 +1228,14 @@ static size_t accum_lds(const rpa_ctx *c) {
 }
 
 int configure_accumulate(rpa_ctx *c) {
+  if (c->hp_nrb) {  // the instance depends on the q's Nlf: set its limit every time
+    const size_t lds = hp_lds(c);
+    if (lds <= 64 * 1024) return 0;
+    if (lds > 160 * 1024) return fail(c, RPA_E_LIMIT, "Nlf too large for the accumulation staging");
+    RPA_HIP(c, hipFuncSetAttribute(hp_kernel_of(c), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    c->accum_lds_attr = std::max(c->accum_lds_attr, lds);
+    return 0;
+  }
   if (c->d.precision == RPA_MIXED) return 0;
   const size_t lds = accum_lds(c);
   if (lds <= 64 * 1024 || lds <= c->accum_lds_attr) return 0;
@@ -878,7 +1257,9 @@ int launch_accumulate(rpa_ctx *c) {
   const int npb = c->ncol_pad / kColsWG;
   const double den = (M_PI * (double)c->ntot) * c->p.vcell;
   const int shift = cplx ? c->d.no : 0;
-  if (mixed) {
+  if (mixed && c->hp_nrb) {
+    if (int e = launch_hp(c, shift, den)) return e;
+  } else if (mixed) {
     if (c->nlf > 96) return fail(c, RPA_E_LIMIT, "mixed precision accumulation supports Nlf <= 96");
     const long long nt = std::max<long long>(c->ntrans, 1);
     const float2 *y = (const float2 *)(c->d_rho2 ? c->d_rho2 : c->d_rho1);
@@ -949,9 +1330,11 @@ int launch_accumulate(rpa_ctx *c) {
         hipLaunchKernelGGL((k_accum<false, 8>), dim3(grid), dim3(256), lds, c->stream, a);
     }
   }
-  if (c->profiling) RPA_HIP(c, hipEventRecord(c->prof_ev[3], c->stream));
+  if (!c->hp_nrb && c->profiling) RPA_HIP(c, hipEventRecord(c->prof_ev[3], c->stream));
   const long long n = (long long)c->nw * c->ncol_pad;
-  if (mixed)
+  if (mixed && c->hp_nrb) {
+    // reduced in launch_hp
+  } else if (mixed)
     hipLaunchKernelGGL(k_reduce<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
                        (const float *)c->d_part, (const int *)c->d_first_item, npb, c->nw, c->ncol_pad, c->d_s0);
   else

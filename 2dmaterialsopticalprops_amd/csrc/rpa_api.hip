@@ -99,6 +99,12 @@ static void build_pairs(rpa_ctx *c) {
   pt.tpk = off;
 }
 
+// An environment switch set to "0".
+static bool env_off(const char *name) {
+  const char *v = std::getenv(name);
+  return v && v[0] == '0';
+}
+
 int prepare(rpa_ctx *c) {
   if (c->prepared) return 0;
   if (c->ngm == 0) return fail(c, RPA_E_STATE, "rpa_set_gvectors not called");
@@ -142,8 +148,20 @@ int prepare(rpa_ctx *c) {
   c->ntrans = (long long)c->ntot * c->pairs.tpk;
   const size_t nt = (size_t)std::max<long long>(c->ntrans, 1);
   const size_t nblk = (nt + 255) / 256;
-  const size_t chunk = (c->d.precision == RPA_MIXED) ? kChunkTMx : kChunkT;
-  c->max_items = (size_t)(c->span + 1) * ((nt + chunk - 1) / chunk) + c->ntiles + 1;
+  // mixed S0 in home panels when every window fits a panel of at most
+  // kHpMaxNrb row blocks: a window starts in its home tile (offset <= 31)
+  // and spans at most floor(2 hw / domega) + 5 rows (+1 for the rounding
+  // of its two ends)
+  c->hp_nrb = 0;
+  if (c->d.precision == RPA_MIXED && !std::isinf(hw) && !env_off("RPA_ACCUM_HP")) {
+    const double rows = (kTileW - 1) + std::floor(2.0 * hw / p.domega) + 6.0;
+    const int nrb = (int)std::ceil(rows / 16.0);
+    if (nrb <= kHpMaxNrb) c->hp_nrb = std::max(nrb, 4);
+  }
+  c->wl_span = c->hp_nrb ? 0 : c->span;
+  c->wl_chunk = c->hp_nrb ? kChunkHp : (c->d.precision == RPA_MIXED) ? kChunkTMx : kChunkT;
+  const size_t chunk = (size_t)c->wl_chunk;
+  c->max_items = (size_t)(c->wl_span + 1) * ((nt + chunk - 1) / chunk) + c->ntiles + 1;
   const bool two = (c->d.program == RPA_OPTABS && p.pol == RPA_POL_YZ);
   c->ncol = (c->d.program == RPA_OPTABS ? 2 : 1) * c->nlf * c->nlf;
   c->ncol_pad = (c->ncol + kColsWG - 1) / kColsWG * kColsWG;
@@ -171,10 +189,18 @@ int prepare(rpa_ctx *c) {
     c->cap_rho2 = 0;
   }
   if (!e) e = ensure(c, c->d_s0, c->cap_s0, (size_t)c->nw * c->ncol_pad);
-  if (!e) e = ensure(c, c->d_part, c->cap_part, c->max_items * npb * kTileW * kColsWG);
-  if (!e && c->d.precision == RPA_MIXED) {
-    e = ensure(c, c->d_wmx, c->cap_wmx, c->max_items * kTileW * kChunkTMx);
-    if (!e) e = ensure(c, c->d_rbt, c->cap_rbt, 2 * (size_t)c->nlf * nt);
+  if (c->hp_nrb) {
+    const bool cplx = (c->d.program == RPA_OPTABS);
+    const size_t rows = 16 * (size_t)c->hp_nrb;
+    const size_t nbl = (size_t)hp_nblocks(c->nlf, cplx);
+    if (!e) e = ensure(c, c->d_part, c->cap_part, c->max_items * nbl * rows * 16 * (cplx ? 2 : 1) / 2 + 1);
+    if (!e) e = ensure(c, c->d_wtab, c->cap_wtab, c->max_items * (kChunkHp / kHpKS) * 2 * rows * kHpKS);
+  } else {
+    if (!e) e = ensure(c, c->d_part, c->cap_part, c->max_items * npb * kTileW * kColsWG);
+    if (!e && c->d.precision == RPA_MIXED) {
+      e = ensure(c, c->d_wmx, c->cap_wmx, c->max_items * kTileW * kChunkTMx);
+      if (!e) e = ensure(c, c->d_rbt, c->cap_rbt, 2 * (size_t)c->nlf * nt);
+    }
   }
   if (!e && c->d.precision != RPA_MIXED) e = ensure(c, c->d_wt64, c->cap_wt64, c->max_items * kTileW * kChunkT);
   if (!e) e = ensure(c, c->d_kmat, c->cap_kmat, nf * c->nw);
@@ -413,7 +439,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
                   c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32,
                   c->d_wmx, c->d_rbt, c->d_wt64, c->d_box[0], c->d_box[1], c->d_box_gidx,
-                  c->d_csym, c->d_crkc};
+                  c->d_csym, c->d_crkc, c->d_wtab};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_col_plan(c);

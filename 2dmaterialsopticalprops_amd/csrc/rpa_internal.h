@@ -62,6 +62,19 @@ constexpr int kColsWG = 256;   // S0 columns per accumulation work item
 constexpr int kChunkT = 4096;    // transitions per accumulation work item (FP64)
 constexpr int kChunkTMx = 1024;  // the same for the mixed path (FP32 partial sums)
 constexpr int kMaxNlf = 128;   // local fields supported by the kernels
+constexpr int kChunkHp = 1024;  // mixed S0 in home panels: transitions per item
+constexpr int kHpKS = 32;       // the same: transitions per stage (K of one MFMA)
+constexpr int kHpMaxNrb = 8;    // the same: panel row blocks of 16 compiled in
+
+// Column blocks of 16 of the home-panel S0 form: G fixed, G' = G'0 .. G'0 +
+// 15; the symmetric real S0 takes G'0 = G + 16 j (G-major), the complex S0
+// G'0 = 16 j.  Count of all blocks:
+inline int hp_nblocks(int nlf, bool cplx) {
+  if (cplx) return nlf * ((nlf + 15) / 16);
+  int n = 0;
+  for (int g = 0; g < nlf; ++g) n += (nlf - g + 15) / 16;
+  return n;
+}
 
 struct PairTable {
   int nblocks;
@@ -211,6 +224,10 @@ struct rpa_ctx {
   int *d_toff = nullptr;     // tile offsets [ntiles+1]
   int *d_nat = nullptr;      // sorted -> natural
   double *d_te = nullptr;    // sorted (E1, E2, df) triples
+  int wl_span = 0, wl_chunk = 0;  // worklist: tiles behind a tile's home, transitions per item
+  int hp_nrb = 0;            // mixed S0 in home panels of 16 hp_nrb rows (0: the tile form)
+  uint16_t *d_wtab = nullptr;  // home panels: BF16 hi/lo weights [item][stage][2][rows][32]
+  size_t cap_wtab = 0;
   int *d_items = nullptr;    // accumulation work items
   int *d_nitems = nullptr;   // [0] = item count, [1] = total transitions kept
   int *d_first_item = nullptr;
