@@ -14,7 +14,9 @@
 !           chain: 'fp64' throughout, or complex FP32 vertices and BF16 hi/lo
 !           split-product S0 with FP64 weights, reduction, KK and Dyson/W.
 !
-! Each q runs as one replay of the captured device chain (rpa_run_graph).
+! Each q runs the device chain with rpa_run (its kernels on one stream; measured
+! faster than a replay of the captured chain, rpa_run_graph, which gives the
+! same results bit for bit).
 ! A zero pivot in eps = I - chi0 V prints the reference's 'singular matrix
 ! in gjel' (Chi_RPA/gjel.f:59-61) and the run goes on, as the reference's.
 program sloss_hip
@@ -141,11 +143,11 @@ program sloss_hip
       call rpa_check(rpa_find_kq(ntot, ktot, NkI, ki, ngm, g, q, eps, K1, K2, iG0), ctx, 'findKQinBZ')
       call rpa_check(rpa_set_q(ctx, ntot, K1, K2, iG0, ktot, q), ctx, 'rpa_set_q')
     end if
-    rc = rpa_run_graph(ctx)
+    rc = rpa_run(ctx)
     if (rc == RPA_E_SINGULAR) then
       print *, 'singular matrix in gjel'
     else
-      call rpa_check(rc, ctx, 'rpa_run_graph')
+      call rpa_check(rc, ctx, 'rpa_run')
     end if
     write(*, '(a,i0,a,f10.3,a)') ' q ', iq, ': device chain ', rpa_last_run_ms(ctx), ' ms'
     call rpa_check(rpa_get_result(ctx, RPA_RES_W, c_loc(WT)), ctx, 'W')
