@@ -47,6 +47,9 @@ namespace rpa {
 
 constexpr int kColInfo = 2 + kColMaxShift / 4;  // per class: rowoff, nsg, sg list
 constexpr size_t kColLdsBudget = 80 * 1024; // default: two workgroups per CU
+// positions per chunk and staged k+q rows per thread (rows <= 64 NRU)
+constexpr int kKcF32 = 128, kKcF64 = 64;
+constexpr int kNruF32 = 8, kNruF64 = 4;
 
 struct ColArgs {
   const void *box[2];  // basis arrays, rows of bw[g] bands (column-box or global order)
@@ -75,6 +78,10 @@ struct ColArgs {
   double q[3];
   double gcar;
   int pol1, pol2;
+  // LDS DMA form: the k+q box array as (hi, lo) BF16 words, its row stride
+  // in values, a 16-byte zero area
+  const uint32_t *boxw, *zero16;
+  int bww;
 };
 
 struct ColF64 {
@@ -156,6 +163,167 @@ struct ColStage {
   unsigned bmask;  // bit 4u + e: k+q value (u, e) present
   int np, nr;      // positions and B rows of the chunk
 };
+
+// Products of one staged chunk in the split form (mixed charge vertices):
+// 8 positions per block, K = (position, Re/Im, hi/lo): a lane's B operand is
+// the (hi, lo) word pairs of its 2 positions, two 8-byte LDS reads straight
+// into the operand registers.  A1 = (hi, hi) of C1 against (hi, lo) gives
+// hi.hi + hi.lo, A2 = (lo, 0) gives lo.hi.  Software pipelined: after a
+// tile's two products its registers take the next block's B values of that
+// tile, so the reads are in flight during the other tiles' products (the
+// group barriers keep the compiler from sinking each read to its product,
+// which left one LDS latency per tile exposed); rows and A parts are read two
+// blocks ahead.  Positions past the chunk (to the next multiple of 8 <= KC)
+// have zero A parts and B row 0; reads past the last block use clamped
+// positions and are never consumed.  The A row of an Im row is (-Im, Re) of
+// C1: the halves of each word swapped and the sign of the new low half
+// flipped.  sAw: [hi, lo][8 bands n][KC + 4] words; sP: [KC] B row of each
+// position; sBw: [rows][ms][hi, lo] words.
+template <int TPW, int KC>
+__device__ __forceinline__ void split_products(rpa_f4 (&acc)[TPW], const uint32_t *sAw, const int *sP,
+                                               const uint32_t *sBw, int nblk, int ms, int lane_off,
+                                               const int (&toff)[TPW], int band_r, bool im_row, int kg) {
+  constexpr int KCA = KC + 4;
+  const uint32_t *aph = sAw + band_r * KCA, *apl = aph + 8 * KCA;
+  auto rot = [&](uint32_t w) { return im_row ? (__builtin_amdgcn_alignbit(w, w, 16) ^ 0x8000u) : w; };
+  auto qpos = [&](int blk) { return min(8 * blk, KC - 8) + 2 * kg; };
+  // block 0: rows, A parts and B values; block 1: rows and A parts
+  uint2 ah = *(const uint2 *)&aph[qpos(0)];
+  uint2 al = *(const uint2 *)&apl[qpos(0)];
+  rpa_u4 b[TPW];
+  {
+    const int2 pr = *(const int2 *)&sP[qpos(0)];
+    const int rb0 = pr.x * ms + lane_off, rb1 = pr.y * ms + lane_off;
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) {
+      const uint2 v0 = *(const uint2 *)&sBw[2 * (rb0 + toff[tt])];
+      const uint2 v1 = *(const uint2 *)&sBw[2 * (rb1 + toff[tt])];
+      b[tt][0] = v0.x;
+      b[tt][1] = v0.y;
+      b[tt][2] = v1.x;
+      b[tt][3] = v1.y;
+    }
+  }
+  int2 prn = *(const int2 *)&sP[qpos(1)];
+  uint2 ahn = *(const uint2 *)&aph[qpos(1)];
+  uint2 aln = *(const uint2 *)&apl[qpos(1)];
+  for (int t = 0; t < nblk; ++t) {
+    rpa_u4 a1, a2;
+    {
+      const uint32_t hx = rot(ah.x), hy = rot(ah.y), lx = rot(al.x), ly = rot(al.y);
+      a1[0] = hx; a1[1] = hx; a1[2] = hy; a1[3] = hy;
+      a2[0] = lx; a2[1] = 0u; a2[2] = ly; a2[3] = 0u;
+    }
+    const rpa_bf8 a_1 = __builtin_bit_cast(rpa_bf8, a1), a_2 = __builtin_bit_cast(rpa_bf8, a2);
+    const int rb0 = prn.x * ms + lane_off, rb1 = prn.y * ms + lane_off;
+    ah = ahn;
+    al = aln;
+    prn = *(const int2 *)&sP[qpos(t + 2)];
+    ahn = *(const uint2 *)&aph[qpos(t + 2)];
+    aln = *(const uint2 *)&apl[qpos(t + 2)];
+    // each tile's products, then the next block's B values of that tile
+    // into the same registers: the reads are in flight during the
+    // remaining tiles' products
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) {
+      const rpa_bf8 bb = __builtin_bit_cast(rpa_bf8, b[tt]);
+      acc[tt] = mfma_bf16_16x16x32(a_2, bb, acc[tt]);
+      acc[tt] = mfma_bf16_16x16x32(a_1, bb, acc[tt]);
+      const uint2 v0 = *(const uint2 *)&sBw[2 * (rb0 + toff[tt])];
+      const uint2 v1 = *(const uint2 *)&sBw[2 * (rb1 + toff[tt])];
+      b[tt][0] = v0.x;
+      b[tt][1] = v0.y;
+      b[tt][2] = v1.x;
+      b[tt][3] = v1.y;
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
+    }
+  }
+}
+
+// Writes the tiles of one workgroup: tile (sg, mg), lane column r -> shift
+// 4 sg + (r & 3), band m 4 mg + (r >> 2), rows as the precision's layout;
+// current vertices add alpha(G) rho.
+template <class P, int KIND, int TPW, int NV>
+__device__ __forceinline__ void col_write(const ColArgs &a, const typename P::acc (&acc)[NV][TPW], int k, int cl,
+                                          const VBlock &vb, int wave, int r, int kg, const int *info, int nt,
+                                          int nmg) {
+  using real = typename P::real;
+  using cplx = typename P::cplx;
+  // The output slots (perm lookups) of all tiles are loaded first, from
+  // clamped addresses, so that the lookups are in flight together.
+  const PairBlock &pbk = a.pairs.b[vb.pb];
+  const int nm = pbk.m_hi - pbk.m_lo;
+  cplx *rho = (cplx *)a.rho, *rho2 = (cplx *)a.rho2;
+  const int *shift = a.shift + (size_t)cl * kColMaxShift;
+  const double *kv = a.ktot + 3 * (size_t)k;
+  // (slot indices fit 31 bits: perm holds them as int, and the vertex buffer
+  // of 2^31 transitions would exceed device memory)
+  int out_g[TPW];
+  int out_pos[TPW][2];
+  int sgv[TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int T = wave + 4 * tt;
+    sgv[tt] = info[2 + (T < nt ? T : 0) / nmg];
+  }
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) out_g[tt] = shift[4 * sgv[tt] + (r & 3)];
+  long long idx[TPW][2];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int T = wave + 4 * tt;
+    const int mo = 4 * ((T < nt ? T : 0) % nmg) + (r >> 2);
+    const bool ok = T < nt && out_g[tt] >= 0 && mo < vb.ncols;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int onr = P::out_band(kg, p);
+      const bool v = ok && onr < vb.nrows;
+      idx[tt][p] = v ? (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] + (long long)onr * nm + mo +
+                           (long long)(vb.n0 - pbk.n_lo) * nm + (vb.m0 - pbk.m_lo)
+                     : -1;
+    }
+  }
+  if (a.perm) {
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int pv = a.perm[idx[tt][p] >= 0 ? idx[tt][p] : 0];
+        out_pos[tt][p] = idx[tt][p] >= 0 ? pv : -1;
+      }
+  } else {
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p) out_pos[tt][p] = (int)idx[tt][p];
+  }
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int iG = out_g[tt];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const long long pos = out_pos[tt][p];
+      if (pos < 0) continue;
+      const real xr = acc[0][tt][P::out_re(p)], xi = acc[0][tt][P::out_im(p)];
+      if (KIND == 0) {
+        rho[pos * a.nlf + iG] = P::make(xr, xi);
+      } else {
+        // j = alpha(G) rho + rho_beta, alpha = 0.5 Gcar (q + 2k + Glf(G))_pol
+        const real al1 = (real)(0.5 * (((a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * iG + a.pol1]) * a.gcar));
+        const cplx j1 = P::make(al1 * xr + acc[1][tt][P::out_re(p)], al1 * xi + acc[1][tt][P::out_im(p)]);
+        rho[pos * a.nlf + iG] = j1;
+        if (KIND == 2) {
+          const real al2 = (real)(0.5 * (((a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * iG + a.pol2]) * a.gcar));
+          rho2[pos * a.nlf + iG] = P::make(al2 * xr + acc[NV - 1][tt][P::out_re(p)],
+                                           al2 * xi + acc[NV - 1][tt][P::out_im(p)]);
+        } else if (rho2) {
+          rho2[pos * a.nlf + iG] = j1;
+        }
+      }
+    }
+  }
+}
 
 // PIPE: the next chunk is loaded into registers while the current one runs
 // (fewer waves, hidden latency) instead of loaded between the barriers.
@@ -341,76 +509,8 @@ __device__ __forceinline__ void col_body(const ColArgs &a) {
     // results are dropped (no branch in the loop, so the LDS reads of all
     // tiles are issued ahead of the products)
     if constexpr (SPLIT) {
-      // 8 positions per block, K = (position, Re/Im, hi/lo): a lane's B
-      // operand is the (hi, lo) word pairs of its 2 positions, two 8-byte
-      // LDS reads straight into the operand registers.  A1 = (hi, hi) of C1
-      // against (hi, lo) gives hi.hi + hi.lo, A2 = (lo, 0) gives lo.hi.
-      // Software pipelined: after a tile's two products its registers take
-      // the next block's B values of that tile, so the reads are in flight
-      // during the other tiles' products (the group barriers keep the
-      // compiler from sinking each read to its product, which left one LDS
-      // latency per tile exposed); rows and A parts are read two blocks
-      // ahead.  Positions past the chunk (to the next multiple of 8 <= KC)
-      // have zero A parts and B row 0; reads past the last block use clamped
-      // positions and are never consumed.  The A row of an Im row is
-      // (-Im, Re) of C1: the halves of each word swapped and the sign of the
-      // new low half flipped.
-      const int nblk = (a.chunk[ch + 1] - a.chunk[ch] + 7) >> 3;
-      const uint32_t *aph = sAw + band_r * KCA, *apl = aph + 8 * KCA;
-      auto rot = [&](uint32_t w) { return im_row ? (__builtin_amdgcn_alignbit(w, w, 16) ^ 0x8000u) : w; };
-      auto qpos = [&](int blk) { return min(8 * blk, KC - 8) + 2 * kg; };
-      // block 0: rows, A parts and B values; block 1: rows and A parts
-      uint2 ah = *(const uint2 *)&aph[qpos(0)];
-      uint2 al = *(const uint2 *)&apl[qpos(0)];
-      rpa_u4 b[TPW];
-      {
-        const int2 pr = *(const int2 *)&sP[qpos(0)];
-        const int rb0 = pr.x * a.ms + lane_off, rb1 = pr.y * a.ms + lane_off;
-#pragma unroll
-        for (int tt = 0; tt < TPW; ++tt) {
-          const uint2 v0 = *(const uint2 *)&sBw[2 * (rb0 + toff[tt])];
-          const uint2 v1 = *(const uint2 *)&sBw[2 * (rb1 + toff[tt])];
-          b[tt][0] = v0.x;
-          b[tt][1] = v0.y;
-          b[tt][2] = v1.x;
-          b[tt][3] = v1.y;
-        }
-      }
-      int2 prn = *(const int2 *)&sP[qpos(1)];
-      uint2 ahn = *(const uint2 *)&aph[qpos(1)];
-      uint2 aln = *(const uint2 *)&apl[qpos(1)];
-      for (int t = 0; t < nblk; ++t) {
-        rpa_u4 a1, a2;
-        {
-          const uint32_t hx = rot(ah.x), hy = rot(ah.y), lx = rot(al.x), ly = rot(al.y);
-          a1[0] = hx; a1[1] = hx; a1[2] = hy; a1[3] = hy;
-          a2[0] = lx; a2[1] = 0u; a2[2] = ly; a2[3] = 0u;
-        }
-        const rpa_bf8 a_1 = __builtin_bit_cast(rpa_bf8, a1), a_2 = __builtin_bit_cast(rpa_bf8, a2);
-        const int rb0 = prn.x * a.ms + lane_off, rb1 = prn.y * a.ms + lane_off;
-        ah = ahn;
-        al = aln;
-        prn = *(const int2 *)&sP[qpos(t + 2)];
-        ahn = *(const uint2 *)&aph[qpos(t + 2)];
-        aln = *(const uint2 *)&apl[qpos(t + 2)];
-        // each tile's products, then the next block's B values of that tile
-        // into the same registers: the reads are in flight during the
-        // remaining tiles' products
-#pragma unroll
-        for (int tt = 0; tt < TPW; ++tt) {
-          const rpa_bf8 bb = __builtin_bit_cast(rpa_bf8, b[tt]);
-          acc[0][tt] = mfma_bf16_16x16x32(a_2, bb, acc[0][tt]);
-          acc[0][tt] = mfma_bf16_16x16x32(a_1, bb, acc[0][tt]);
-          const uint2 v0 = *(const uint2 *)&sBw[2 * (rb0 + toff[tt])];
-          const uint2 v1 = *(const uint2 *)&sBw[2 * (rb1 + toff[tt])];
-          b[tt][0] = v0.x;
-          b[tt][1] = v0.y;
-          b[tt][2] = v1.x;
-          b[tt][3] = v1.y;
-          __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-          __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-        }
-      }
+      split_products<TPW, KC>(acc[0], sAw, sP, sBw, (a.chunk[ch + 1] - a.chunk[ch] + 7) >> 3, a.ms, lane_off, toff,
+                              band_r, im_row, kg);
     } else if constexpr (SU == 2) {
       // two steps per pass: the LDS reads of both steps' A values and B rows
       // are issued together, so one latency chain feeds 2 x TPW x NV x 2
@@ -484,80 +584,7 @@ __device__ __forceinline__ void col_body(const ColArgs &a) {
     }
   }
 
-  // results: tile (sg, mg), lane column r -> shift 4 sg + (r & 3), band m 4 mg + (r >> 2).
-  // The output slots (perm lookups) of all tiles are loaded first, from
-  // clamped addresses, so that the lookups are in flight together.
-  const PairBlock &pbk = a.pairs.b[vb.pb];
-  const int nm = pbk.m_hi - pbk.m_lo;
-  cplx *rho = (cplx *)a.rho, *rho2 = (cplx *)a.rho2;
-  const int *shift = a.shift + (size_t)cl * kColMaxShift;
-  const double *kv = a.ktot + 3 * (size_t)k;
-  // (slot indices fit 31 bits: perm holds them as int, and the vertex buffer
-  // of 2^31 transitions would exceed device memory)
-  int out_g[TPW];
-  int out_pos[TPW][2];
-  int sgv[TPW];
-#pragma unroll
-  for (int tt = 0; tt < TPW; ++tt) {
-    const int T = wave + 4 * tt;
-    sgv[tt] = info[2 + (T < nt ? T : 0) / nmg];
-  }
-#pragma unroll
-  for (int tt = 0; tt < TPW; ++tt) out_g[tt] = shift[4 * sgv[tt] + (r & 3)];
-  long long idx[TPW][2];
-#pragma unroll
-  for (int tt = 0; tt < TPW; ++tt) {
-    const int T = wave + 4 * tt;
-    const int mo = 4 * ((T < nt ? T : 0) % nmg) + (r >> 2);
-    const bool ok = T < nt && out_g[tt] >= 0 && mo < vb.ncols;
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int onr = P::out_band(kg, p);
-      const bool v = ok && onr < vb.nrows;
-      idx[tt][p] = v ? (long long)k * a.pairs.tpk + a.pairs.off[vb.pb] + (long long)onr * nm + mo +
-                           (long long)(vb.n0 - pbk.n_lo) * nm + (vb.m0 - pbk.m_lo)
-                     : -1;
-    }
-  }
-  if (a.perm) {
-#pragma unroll
-    for (int tt = 0; tt < TPW; ++tt)
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        const int pv = a.perm[idx[tt][p] >= 0 ? idx[tt][p] : 0];
-        out_pos[tt][p] = idx[tt][p] >= 0 ? pv : -1;
-      }
-  } else {
-#pragma unroll
-    for (int tt = 0; tt < TPW; ++tt)
-#pragma unroll
-      for (int p = 0; p < 2; ++p) out_pos[tt][p] = (int)idx[tt][p];
-  }
-#pragma unroll
-  for (int tt = 0; tt < TPW; ++tt) {
-    const int iG = out_g[tt];
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const long long pos = out_pos[tt][p];
-      if (pos < 0) continue;
-      const real xr = acc[0][tt][P::out_re(p)], xi = acc[0][tt][P::out_im(p)];
-      if (KIND == 0) {
-        rho[pos * a.nlf + iG] = P::make(xr, xi);
-      } else {
-        // j = alpha(G) rho + rho_beta, alpha = 0.5 Gcar (q + 2k + Glf(G))_pol
-        const real al1 = (real)(0.5 * (((a.q[a.pol1] + 2.0 * kv[a.pol1]) + a.glf[3 * iG + a.pol1]) * a.gcar));
-        const cplx j1 = P::make(al1 * xr + acc[1][tt][P::out_re(p)], al1 * xi + acc[1][tt][P::out_im(p)]);
-        rho[pos * a.nlf + iG] = j1;
-        if (KIND == 2) {
-          const real al2 = (real)(0.5 * (((a.q[a.pol2] + 2.0 * kv[a.pol2]) + a.glf[3 * iG + a.pol2]) * a.gcar));
-          rho2[pos * a.nlf + iG] = P::make(al2 * xr + acc[NV - 1][tt][P::out_re(p)],
-                                           al2 * xi + acc[NV - 1][tt][P::out_im(p)]);
-        } else if (rho2) {
-          rho2[pos * a.nlf + iG] = j1;
-        }
-      }
-    }
-  }
+  col_write<P, KIND, TPW, NV>(a, acc, k, cl, vb, wave, r, kg, info, nt, nmg);
 }
 
 template <class P, int KIND, int TPW, int KC, int NRU, bool PIPE>
@@ -571,6 +598,168 @@ __global__ __launch_bounds__(256) void k_vertex_col(ColArgs a) {
 template <int TPW, int KC, int NRU, bool PIPE>
 __global__ __launch_bounds__(256, 2) void k_vertex_col_split(ColArgs a) {
   col_body<ColBf3, 0, TPW, KC, NRU, PIPE>(a);
+}
+
+// The split form with the k+q rows staged by LDS DMA (global_load_lds, 16
+// bytes a lane).  The k+q side reads a copy of its box array already split
+// into (hi, lo) BF16 words, so a staged row passes through no registers and
+// no conversion.  Each chunk's rows are requested one chunk ahead into the
+// other of two LDS buffers and land while the current chunk's products run;
+// the C1 values and position rows of the next chunk come one chunk ahead in
+// registers, and the slot and row indices they are addressed by two chunks
+// ahead, so no load of an iteration waits on another.  The two buffers are
+// distinct LDS variables and each half of the unrolled loop names its own,
+// so the compiler's wait for LDS DMA stores stays off the other buffer's
+// reads.  Unit u (16 bytes: values 2 part, 2 part + 1 of a row) of a chunk
+// is row u / (ms / 2), part u % (ms / 2): a buffer holds [rows][ms][hi, lo]
+// words as the register-staged form does, and the 64 lanes of one request
+// fill 64 consecutive units.  Absent rows read a zero area; values of a row
+// past the block's bands come from the next bands or row of the copy and
+// meet only output columns that are dropped.  Products and results as the
+// split form, so the results are bitwise those of k_vertex_col_split with
+// the same chunks.
+constexpr int kColDmaBuf = 8 * 256;  // 16-byte units per LDS buffer (32 KB)
+constexpr int kColDmaChunks = 255;   // chunks of one plan
+
+template <int TPW>
+__global__ __launch_bounds__(256, 2) void k_vertex_col_dma(ColArgs a) {
+  constexpr int KC = kKcF32, KCA = KC + 4, NA = KC / 32, NBJ = kColDmaBuf / 256;
+  __shared__ __align__(16) uint32_t sAw[2 * 8 * KCA];
+  __shared__ __align__(16) int sP[KC];
+  __shared__ __align__(16) uint32_t sB0[4 * kColDmaBuf];
+  __shared__ __align__(16) uint32_t sB1[4 * kColDmaBuf];
+  __shared__ int sCh[kColDmaChunks + 1], sCr[kColDmaChunks + 1];  // first position, first row of each chunk
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int L = blockIdx.x;
+  const int xcd = L & 7;
+  const int slot = L >> 3;
+  const int k = xcd + 8 * (slot / a.blocks);
+  if (k >= a.ntot) return;  // whole workgroup leaves together
+  const int rest = slot % a.blocks;
+  const int cl = rest % a.ncls;
+  const VBlock vb = a.vb[rest / a.ncls];
+
+  const int K1 = a.k1[k] - 1, K2 = a.k2[k] - 1;
+  const int g1 = (a.nbg == 2 && vb.n0 >= a.blo1) ? 1 : 0;
+  const int s1 = a.bw[g1];
+  const float2 *c1 = (const float2 *)a.box[g1] + (size_t)K1 * a.npwx * s1 + (vb.n0 - (g1 ? a.blo1 : 0));
+  const uint32_t *c2w = a.boxw + 2 * ((size_t)K2 * a.npwx * a.bww + (vb.m0 - (a.nbg == 2 ? a.blo1 : 0)));
+  const int *info = a.cls + (size_t)cl * kColInfo;
+  const int *bmap = a.bmap + (size_t)a.g0idx[k] * a.nrows + info[0];
+  const int *prow = a.prow + (size_t)cl * a.npos;
+  const int *crow = a.crow + (size_t)cl * (a.nchunk + 1);
+  const int nmg = (vb.ncols + 3) >> 2;
+  const int nt = info[1] * nmg;
+  const int hu = a.ms >> 1;  // 16-byte units per staged row
+  // chunk bounds in LDS: read in the loop without waiting on the loads in
+  // flight (a global load of them would)
+  for (int i = tid; i <= a.nchunk; i += 256) {
+    sCh[i] = a.chunk[i];
+    sCr[i] = crow[i];
+  }
+  __syncthreads();
+
+  // ---- staging roles: C1 positions sa_p + 32 u, band sa_n; DMA unit tid + 256 j
+  const int sa_p = tid >> 3, sa_n = tid & 7;
+  const bool nok = sa_n < vb.nrows;
+  const int nn = nok ? sa_n : 0;
+  int urow[NBJ];
+#pragma unroll
+  for (int j = 0; j < NBJ; ++j) urow[j] = (tid + 256 * j) / hu;
+  int sl[NA];          // C1 slots of the next chunk whose values are loaded
+  int bm[NBJ];         // box rows of the next chunk whose rows are requested
+  float2 av[NA];       // C1 values of the chunk stored next
+  unsigned am = 0;     // bit u: av[u] present
+  int pr = 0, npc = 0; // row of position tid, positions of that chunk
+  auto load_idx = [&](int ch) {
+    const int p0 = sCh[ch], np = sCh[ch + 1] - p0;
+    const int r0 = sCr[ch], nr = sCr[ch + 1] - r0;
+#pragma unroll
+    for (int u = 0; u < NA; ++u) sl[u] = a.slot[p0 + min(sa_p + 32 * u, np - 1)];
+#pragma unroll
+    for (int j = 0; j < NBJ; ++j) bm[j] = bmap[r0 + min(urow[j], nr - 1)];
+  };
+  auto load_a = [&](int ch) {
+    const int p0 = sCh[ch], np = sCh[ch + 1] - p0;
+    unsigned m = 0;
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      av[u] = c1[(size_t)sl[u] * s1 + nn];
+      m |= (nok && sa_p + 32 * u < np) ? (1u << u) : 0u;
+    }
+    am = m;
+    pr = prow[p0 + min(tid, np - 1)];
+    npc = np;
+  };
+  // requests of whole waves past the chunk's units are skipped; lanes of the
+  // last request past them fill units of the buffer that no read reaches
+  auto dma = [&](int ch, uint32_t *buf) {
+    const int nu = (sCr[ch + 1] - sCr[ch]) * hu;
+#pragma unroll
+    for (int j = 0; j < NBJ; ++j) {
+      const int ub = 256 * j + 64 * wave;
+      if (ub < nu) {
+        const bool ok = ub + lane < nu && bm[j] >= 0;
+        const int part = tid + 256 * j - urow[j] * hu;
+        const uint32_t *src = ok ? c2w + 2 * ((size_t)bm[j] * a.bww + 2 * part) : a.zero16;
+        __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                         (void __attribute__((address_space(3))) *)(buf + 4 * ub), 16, 0, 0);
+      }
+    }
+  };
+  auto store_a = [&]() {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      uint32_t h, l;
+      const bool ok = (am >> u) & 1u;
+      split_pair(make_float2(ok ? av[u].x : 0.f, ok ? av[u].y : 0.f), h, l);
+      sAw[sa_n * KCA + sa_p + 32 * u] = h;
+      sAw[(8 + sa_n) * KCA + sa_p + 32 * u] = l;
+    }
+    if (tid < KC) sP[tid] = tid < npc ? pr : 0;
+  };
+
+  // ---- matrix-core roles (as the split form)
+  const int r = lane & 15, kg = lane >> 4;
+  const int band_r = r >> 1;
+  const bool im_row = (r & 1) != 0;
+  const int lane_off = (r & 3) * a.ms + (r >> 2);
+  int toff[TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) {
+    const int T = wave + 4 * tt;
+    toff[tt] = (T < nt) ? 4 * info[2 + T / nmg] * a.ms + 4 * (T % nmg) : 0;
+  }
+  rpa_f4 acc[1][TPW];
+#pragma unroll
+  for (int tt = 0; tt < TPW; ++tt) acc[0][tt] = rpa_f4{0.f, 0.f, 0.f, 0.f};
+
+  // one chunk: its C1 values into LDS, the next chunk's requests and loads,
+  // then the products from buffer cur while the next chunk lands in nxt
+  auto step = [&](int ch, const uint32_t *cur, uint32_t *nxt) {
+    __builtin_amdgcn_s_waitcnt(0);  // this chunk's rows and values have landed
+    __syncthreads();                // ... in every wave; the last products are done
+    store_a();
+    __syncthreads();
+    const int np = npc;
+    if (ch + 1 < a.nchunk) {
+      dma(ch + 1, nxt);
+      load_a(ch + 1);
+      if (ch + 2 < a.nchunk) load_idx(ch + 2);
+    }
+    split_products<TPW, KC>(acc[0], sAw, sP, cur, (np + 7) >> 3, a.ms, lane_off, toff, band_r, im_row, kg);
+  };
+  load_idx(0);
+  dma(0, sB0);
+  load_a(0);
+  if (a.nchunk > 1) load_idx(1);
+  for (int ch = 0; ch < a.nchunk; ch += 2) {
+    step(ch, sB0, sB1);
+    if (ch + 1 < a.nchunk) step(ch + 1, sB1, sB0);
+  }
+  col_write<ColF32, 0, TPW, 1>(a, acc, k, cl, vb, wave, r, kg, info, nt, nmg);
 }
 
 template <class P, int KIND, int KC, int NRU, bool PIPE>
@@ -631,6 +820,19 @@ static void launch_split_p(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
   }
 }
 
+static void launch_dma(rpa_ctx *c, const ColArgs &a, int tpw, dim3 grid) {
+  switch (tpw) {
+    case 1: hipLaunchKernelGGL((k_vertex_col_dma<1>), grid, dim3(256), 0, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_vertex_col_dma<2>), grid, dim3(256), 0, c->stream, a); break;
+    case 4: hipLaunchKernelGGL((k_vertex_col_dma<4>), grid, dim3(256), 0, c->stream, a); break;
+    case 6: hipLaunchKernelGGL((k_vertex_col_dma<6>), grid, dim3(256), 0, c->stream, a); break;
+    case 8: hipLaunchKernelGGL((k_vertex_col_dma<8>), grid, dim3(256), 0, c->stream, a); break;
+    case 10: hipLaunchKernelGGL((k_vertex_col_dma<10>), grid, dim3(256), 0, c->stream, a); break;
+    case 13: hipLaunchKernelGGL((k_vertex_col_dma<13>), grid, dim3(256), 0, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_vertex_col_dma<16>), grid, dim3(256), 0, c->stream, a); break;
+  }
+}
+
 // Kernel of a (precision, kind, staging) choice; current vertices always
 // load the next chunk during the products.
 template <class P, int KC, int NRU>
@@ -674,9 +876,6 @@ static int tpw_of(int tiles) {
   return 0;
 }
 
-// positions per chunk and staged k+q rows per thread (rows <= 64 NRU)
-constexpr int kKcF32 = 128, kKcF64 = 64;
-constexpr int kNruF32 = 8, kNruF64 = 4;
 
 void free_col_plan(rpa_ctx *c) {
   ColPlan &P = c->col;
@@ -723,6 +922,7 @@ static int ensure_box(rpa_ctx *c) {
   const int nbg = two ? 2 : 1, blo1 = two ? nv : nb;
   if (c->box_ready && c->box_nbg == nbg && c->box_blo1 == blo1) return 0;
   c->box_ready = false;
+  c->boxw_ready = false;
   int npw = 0;
   for (int k = 0; k < c->d.nki; ++k) npw = std::max(npw, c->ngk[k]);
   npw = std::min(npw, c->ngm);
@@ -784,6 +984,45 @@ static int ensure_box(rpa_ctx *c) {
   c->box_blo1 = blo1;
   c->box_rows = nrow;
   c->box_ready = true;
+  return 0;
+}
+
+// Split copy of a box array of bw bands: value j of row (k, R) is the (hi,
+// lo) word pair 2 (row bww + j), + 1; zero for j >= bw and for rows past the
+// box (R >= nrow); nv values in all, the last ntail of them a zero tail.
+__global__ void k_build_boxw(const float2 *box, int npwx, int nrow, int bw, int bww, long long nrows_all,
+                             long long nv, uint32_t *out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  float2 v = make_float2(0.f, 0.f);
+  if (i < nrows_all * bww) {
+    const long long row = i / bww;
+    const int j = (int)(i - row * bww);
+    if (j < bw && (int)(row % npwx) < nrow) v = box[row * bw + j];
+  }
+  uint32_t h, l;
+  split_pair(v, h, l);
+  *(uint2 *)&out[2 * i] = make_uint2(h, l);
+}
+
+// Builds the split copy of the k+q side's box array (array 1 when the bands
+// are split, else array 0) once per box.  The tail of 32 zero values covers
+// reads of up to 20 values past the last row and holds the 16-byte zero area
+// that absent rows are read from.
+static int ensure_boxw(rpa_ctx *c) {
+  if (c->boxw_ready) return 0;
+  const int g = c->box_nbg == 2 ? 1 : 0;
+  const int bw = g ? c->d.nband - c->box_blo1 : c->d.nband;
+  const int bww = (bw + 1) & ~1;
+  const long long rows = (long long)c->d.nki * c->d.npwx;
+  const long long nv = rows * bww + 32;
+  if (int e = ensure(c, c->d_boxw, c->cap_boxw, 2 * (size_t)nv)) return e;
+  hipLaunchKernelGGL(k_build_boxw, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream,
+                     (const float2 *)c->d_box[g], c->d.npwx, c->box_rows, bw, bww, rows, nv, c->d_boxw);
+  RPA_HIP(c, hipGetLastError());
+  c->boxw_bww = bww;
+  c->boxw_zero = 2 * (size_t)(rows * bww + 28);
+  c->boxw_ready = true;
   return 0;
 }
 
@@ -900,12 +1139,25 @@ int build_col_plan(rpa_ctx *c) {
   bool pipe = !split;
   if (const char *env = std::getenv("RPA_VERTEX_COL_PIPE")) pipe = env[0] == '1';
 
-  // chunks: consecutive segments, at most kc positions and rmax rows
+  // chunks: consecutive segments, at most kc positions and rlim rows
   const size_t fixed = (size_t)kc * (8 * csz + sizeof(int) + 2 * rsz);
   size_t budget = kColLdsBudget;
   if (const char *env = std::getenv("RPA_VERTEX_COL_LDS")) budget = std::max<size_t>(fixed + 4096, 1024 * (size_t)std::atoi(env));
   budget = std::min<size_t>(budget, 160 * 1024);
   const int rmax = std::min((int)((budget - fixed) / (ms * csz)), 64 * (mixed ? kNruF32 : kNruF64));
+  // the column-box basis, where it can be built (RPA_VERTEX_BOX=0: global order)
+  const char *env_box = std::getenv("RPA_VERTEX_BOX");
+  bool box = !(env_box && env_box[0] == '0');
+  if (box) {
+    if (int e2 = ensure_box(c)) return e2;
+    box = c->box_ready;
+  }
+  // the split form stages its k+q rows by LDS DMA from the split box copy
+  // (RPA_VERTEX_COL_DMA=0: through registers); its chunks fit one DMA buffer
+  // whichever way they are staged, so both stagings sum alike
+  const char *env_dma = std::getenv("RPA_VERTEX_COL_DMA");
+  bool dma = split && box && !(env_dma && env_dma[0] == '0');
+  const int rlim = split ? std::min(rmax, kColDmaBuf / (ms / 2)) : rmax;
   std::vector<int> chunk{0};
   std::vector<std::pair<int, int>> chunk_segs;  // [first seg, end seg)
   {
@@ -913,8 +1165,8 @@ int build_col_plan(rpa_ctx *c) {
     for (int s = 0; s < (int)segs.size(); ++s) {
       const int cnt = segs[s].p1 - segs[s].p0;
       const int ext = segs[s].zhi - segs[s].zlo + 1 + spanmax;
-      if (ext + padmax > rmax) return 0;
-      if (s > s0 && (npos + cnt > kc || rows + ext > rmax)) {
+      if (ext + padmax > rlim) return 0;
+      if (s > s0 && (npos + cnt > kc || rows + ext > rlim)) {
         chunk_segs.push_back({s0, s});
         chunk.push_back(segs[s].p0);
         s0 = s;
@@ -927,6 +1179,9 @@ int build_col_plan(rpa_ctx *c) {
     chunk_segs.push_back({s0, (int)segs.size()});
     chunk.push_back(ngd);
   }
+  dma = dma && (int)chunk_segs.size() <= kColDmaChunks;
+  if (dma)
+    if (int e2 = ensure_boxw(c)) return e2;
   const int nchunk = (int)chunk_segs.size();
 
   // per class: B rows of each chunk, chunk-relative row of each position
@@ -953,13 +1208,6 @@ int build_col_plan(rpa_ctx *c) {
     }
     crow[(size_t)u * (nchunk + 1) + nchunk] = r;
     nrows += r;
-  }
-  // the column-box basis, where it can be built (RPA_VERTEX_BOX=0: global order)
-  const char *env_box = std::getenv("RPA_VERTEX_BOX");
-  bool box = !(env_box && env_box[0] == '0');
-  if (box) {
-    if (int e2 = ensure_box(c)) return e2;
-    box = c->box_ready;
   }
   // target row of every B row (global index, or box row), per distinct G0
   const int ng0 = (int)c->g0_list.size();
@@ -1033,6 +1281,7 @@ int build_col_plan(rpa_ctx *c) {
   // one complex FP32 value) and no beta rows
   P.split = split;
   P.box = box;
+  P.dma = dma;
   if (P.lds > 64 * 1024) {
     const void *fn = P.split ? (pipe ? split_kernel<kKcF32, kNruF32, true>(tpw)
                                      : split_kernel<kKcF32, kNruF32, false>(tpw))
@@ -1098,7 +1347,19 @@ int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, void *
   const long long kslots = (c->ntot + 7) / 8 * 8;
   const dim3 grid((unsigned)(kslots * a.blocks));
   a.ms = P.ms;
-  if (P.split) {
+  // the DMA form takes the k+q rows of every block from the split copy at
+  // 16-byte aligned values
+  bool dma = P.dma && c->boxw_ready;
+  for (int b = 0; b < nvb && dma; ++b) {
+    const int m_off = vb[b].m0 - (c->box_nbg == 2 ? c->box_blo1 : 0);
+    dma = m_off >= 0 && (m_off & 1) == 0;
+  }
+  if (dma) {
+    a.boxw = c->d_boxw;
+    a.zero16 = c->d_boxw + c->boxw_zero;
+    a.bww = c->boxw_bww;
+    launch_dma(c, a, P.tpw, grid);
+  } else if (P.split) {
     if (P.pipe)
       launch_split_p<kKcF32, kNruF32, true>(c, a, P.tpw, grid);
     else

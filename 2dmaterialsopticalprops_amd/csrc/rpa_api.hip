@@ -439,7 +439,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
                   c->d_epsinv, c->d_chi, c->d_W, c->d_loss, c->d_pi11, c->d_sig,
                   c->d_status, c->d_g0keys, c->d_g0idx, c->d_dmap, c->d_wkk, c->d_evc32,
                   c->d_wmx, c->d_rbt, c->d_wt64, c->d_box[0], c->d_box[1], c->d_box_gidx,
-                  c->d_csym, c->d_crkc, c->d_wtab};
+                  c->d_csym, c->d_crkc, c->d_wtab, c->d_boxw};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   free_col_plan(c);

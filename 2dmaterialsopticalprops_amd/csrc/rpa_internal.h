@@ -101,6 +101,7 @@ struct ColPlan {
   bool pipe = false;
   bool split = false;  // mixed charge vertices: BF16 hi/lo split products
   bool box = false;    // slots and staged rows index the column-box basis
+  bool dma = false;    // split form with the k+q rows staged by LDS DMA
   size_t lds = 0;
   int *d_slot = nullptr, *d_prow = nullptr, *d_chunk = nullptr, *d_crow = nullptr;
   int *d_bmap = nullptr, *d_shift = nullptr, *d_cls = nullptr;
@@ -200,6 +201,14 @@ struct rpa_ctx {
   std::vector<int> h_rbox;  // global Miller box index -> box row (-1: none)
   int box_nbg = 0, box_blo1 = 0, box_rows = 0;
   bool box_ready = false;
+  // mixed precision: the k+q side's box array (array 1 when the bands are
+  // split, else array 0) as BF16 (hi, lo) word pairs per value, rows of
+  // boxw_bww values (even), a zero tail past the last row; read by LDS DMA
+  uint32_t *d_boxw = nullptr;
+  size_t cap_boxw = 0;
+  int boxw_bww = 0;
+  size_t boxw_zero = 0;  // word offset of a 16-byte zero area
+  bool boxw_ready = false;
   // vertex form of the current q ("" = shape rules; "col", "gather",
   // "staged") and staged chunk (0 = largest that fits), set by the autotuner
   std::string vform;

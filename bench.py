@@ -171,7 +171,7 @@ def time_chain(r, steps, warmup):
 
 
 KERNEL_NAME = {  # (precision, stage) -> substrings of the kernel name in the rocprof tables
-    ("mixed", "vertices"): ("k_vertex_col_split<",), ("fp64", "vertices"): ("k_vertex_col<", "ColF64"),
+    ("mixed", "vertices"): ("k_vertex_col_dma<",), ("fp64", "vertices"): ("k_vertex_col<", "ColF64"),
     ("mixed", "accumulate"): ("k_accum_hp<",), ("fp64", "accumulate"): ("k_accum<",),
 }
 
@@ -215,7 +215,7 @@ def vertex_kernel(r, prec, lf_idx=None):
     """Name of the vertex kernel the library picks for the runner's local
     fields: the column form when the in-plane classes hold at least 12 fields
     on average (current vertices: one class only; mixed charge vertices on
-    the BF16 hi/lo split products), else the gather-direct FP32 form (Sloss)
+    the BF16 hi/lo split products, k+q rows staged by LDS DMA), else the gather-direct FP32 form (Sloss)
     or the staged form."""
     m = r.sys.mill_g[np.asarray(r.lf_idx if lf_idx is None else lf_idx) - 1]
     ncls = len({(int(x[0]), int(x[1])) for x in m})
@@ -223,7 +223,7 @@ def vertex_kernel(r, prec, lf_idx=None):
     if r.rc.vertex_map == "direct" and len(m) >= 12 * ncls and not (optabs and ncls > 1):
         if prec == "fp64":
             return "k_vertex_col<ColF64>"
-        return "k_vertex_col<ColF32>" if optabs else "k_vertex_col_split"
+        return "k_vertex_col<ColF32>" if optabs else "k_vertex_col_dma"
     if prec == "mixed" and not optabs:
         nc = r.cfg.nband - r.cfg.nval  # Sloss band block: 8 valence rows x up to 16 conduction columns
         return "k_vertex_gd<PrecF32> (two local fields per wave)" if nc <= 8 else "k_vertex_gd<PrecF32>"
@@ -267,8 +267,8 @@ def roofline_of(res, prec, flops_acc, flops_vtx, workload="", vname=None, vbytes
     """Roofline of the dominant matrix-core kernel of one run."""
     acc_ms, vtx_ms = res["stages"]["accumulate"], res["stages"]["vertices"]
     if prec == "mixed":
-        vname = vname or "k_vertex_col_split"
-        split = "col_split" in vname
+        vname = vname or "k_vertex_col_dma"
+        split = "col_dma" in vname or "col_split" in vname
         names = (f"{sform} (S0, BF16 hi/lo x3 into FP32, with its weight prepass)",
                  f"{vname} (vertices, {'BF16 hi/lo x3 into FP32' if split else 'FP32'})")
         peaks = (BF16X3_PEAK_TFLOPS, BF16X3_PEAK_TFLOPS if split else FP32_PEAK_TFLOPS)

@@ -312,3 +312,33 @@ def test_column_box_basis_is_bitwise_the_global_order(program, precision, monkey
         got[box] = r.vertices()[0]
         r.close()
     assert np.array_equal(got["1"], got["0"])
+
+
+@pytest.mark.parametrize("case", ["cfg0", "multi-class", "widest"])
+def test_split_column_form_dma_staging_is_bitwise_the_register_staging(case, monkeypatch):
+    """Mixed charge vertices of the split column form stage the k+q rows by
+    LDS DMA from a BF16 hi/lo copy of the box basis; RPA_VERTEX_COL_DMA=0
+    stages them through registers and splits them there.  The chunks are the
+    same and so are the staged words, so the vertices are bitwise equal: on
+    the benchmark-like system, on several in-plane classes with G0 wraps, and
+    at 16 tiles per wave (16 conduction bands, the widest staged rows)."""
+    monkeypatch.setenv("RPA_VERTEX_FORM", "col")
+    if case == "cfg0":
+        s, qs = H.system("cfg0"), [None]
+    elif case == "multi-class":
+        s, qs = H.system("tiny", lf="xyz", nlf_target=9), [(2, 9), (4, 14), (6, 5)]
+    else:
+        s, qs = H.system("tiny", ncond=16, nlf_target=64, ngk_target=250.0), [None]
+    got = {}
+    for dma in ("1", "0"):
+        monkeypatch.setenv("RPA_VERTEX_COL_DMA", dma)
+        r = _runner(s, program="sloss", precision="mixed")
+        got[dma] = []
+        for q in qs:
+            if q is not None:
+                r.set_q_point(r.q_point(*q))
+            got[dma].append(r.vertices()[0])
+        r.close()
+    for a, b in zip(got["1"], got["0"]):
+        assert np.isfinite(a).all()
+        assert np.array_equal(a, b)
