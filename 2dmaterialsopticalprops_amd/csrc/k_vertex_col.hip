@@ -78,9 +78,10 @@ struct ColArgs {
   double q[3];
   double gcar;
   int pol1, pol2;
-  // LDS DMA form: the k+q box array as (hi, lo) BF16 words, its row stride
-  // in values, a 16-byte zero area
-  const uint32_t *boxw, *zero16;
+  // LDS DMA form: the k+q box array as (hi, lo) BF16 words, its values and
+  // row stride in values
+  const uint32_t *boxw;
+  long long boxw_nv;
   int bww;
 };
 
@@ -179,10 +180,17 @@ struct ColStage {
 // C1: the halves of each word swapped and the sign of the new low half
 // flipped.  sAw: [hi, lo][8 bands n][KC + 4] words; sP: [KC] B row of each
 // position; sBw: [rows][ms][hi, lo] words.
-template <int TPW, int KC>
+// After block t < nissue the caller's issue(t) runs (the next chunk's
+// requests, spread over the products); the rest after the last block.
+struct NoIssue {
+  __device__ void operator()(int) const {}
+};
+
+template <int TPW, int KC, class Issue = NoIssue>
 __device__ __forceinline__ void split_products(rpa_f4 (&acc)[TPW], const uint32_t *sAw, const int *sP,
                                                const uint32_t *sBw, int nblk, int ms, int lane_off,
-                                               const int (&toff)[TPW], int band_r, bool im_row, int kg) {
+                                               const int (&toff)[TPW], int band_r, bool im_row, int kg,
+                                               Issue &&issue = Issue{}, int nissue = 0) {
   constexpr int KCA = KC + 4;
   const uint32_t *aph = sAw + band_r * KCA, *apl = aph + 8 * KCA;
   auto rot = [&](uint32_t w) { return im_row ? (__builtin_amdgcn_alignbit(w, w, 16) ^ 0x8000u) : w; };
@@ -207,6 +215,10 @@ __device__ __forceinline__ void split_products(rpa_f4 (&acc)[TPW], const uint32_
   int2 prn = *(const int2 *)&sP[qpos(1)];
   uint2 ahn = *(const uint2 *)&aph[qpos(1)];
   uint2 aln = *(const uint2 *)&apl[qpos(1)];
+  // all of them landed (lgkmcnt(0)): the loop's own waits then cover one
+  // pass of reads.  Entered with these in flight, in whatever order the
+  // compiler issued them, the loop began with a full LDS wait on every pass.
+  __builtin_amdgcn_s_waitcnt(0xc07f);
   for (int t = 0; t < nblk; ++t) {
     rpa_u4 a1, a2;
     {
@@ -238,7 +250,9 @@ __device__ __forceinline__ void split_products(rpa_f4 (&acc)[TPW], const uint32_
       __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
       __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
     }
+    if (t < nissue) issue(t);
   }
+  for (int t = nblk; t < nissue; ++t) issue(t);
 }
 
 // Writes the tiles of one workgroup: tile (sg, mg), lane column r -> shift
@@ -600,35 +614,42 @@ __global__ __launch_bounds__(256, 2) void k_vertex_col_split(ColArgs a) {
   col_body<ColBf3, 0, TPW, KC, NRU, PIPE>(a);
 }
 
-// The split form with the k+q rows staged by LDS DMA (global_load_lds, 16
-// bytes a lane).  The k+q side reads a copy of its box array already split
-// into (hi, lo) BF16 words, so a staged row passes through no registers and
-// no conversion.  Each chunk's rows are requested one chunk ahead into the
-// other of two LDS buffers and land while the current chunk's products run;
-// the C1 values and position rows of the next chunk come one chunk ahead in
-// registers, and the slot and row indices they are addressed by two chunks
-// ahead, so no load of an iteration waits on another.  The two buffers are
-// distinct LDS variables and each half of the unrolled loop names its own,
-// so the compiler's wait for LDS DMA stores stays off the other buffer's
-// reads.  Unit u (16 bytes: values 2 part, 2 part + 1 of a row) of a chunk
-// is row u / (ms / 2), part u % (ms / 2): a buffer holds [rows][ms][hi, lo]
-// words as the register-staged form does, and the 64 lanes of one request
-// fill 64 consecutive units.  Absent rows read a zero area; values of a row
-// past the block's bands come from the next bands or row of the copy and
-// meet only output columns that are dropped.  Products and results as the
-// split form, so the results are bitwise those of k_vertex_col_split with
-// the same chunks.
-constexpr int kColDmaBuf = 8 * 256;  // 16-byte units per LDS buffer (32 KB)
-constexpr int kColDmaChunks = 255;   // chunks of one plan
+// The split form with the k+q rows staged by LDS DMA (buffer loads to LDS,
+// 16 bytes a lane).  The k+q side reads a copy of its box array already
+// split into (hi, lo) BF16 words, so a staged row passes through no
+// registers and no conversion.  Each chunk's rows are requested one chunk
+// ahead into the other of two LDS buffers and land while the current chunk's
+// products run; the C1 values and position rows of the next chunk come one
+// chunk ahead in registers, and the slot and row indices they are addressed
+// by two chunks ahead, so no load of an iteration waits on another.
+//   * Buffer loads, not global ones: the compiler takes a global load to LDS
+//     for a flat access that may also count on the LDS counter, and while
+//     one is in flight it waits for every LDS read before any product (one
+//     full LDS drain per block of 8 positions).
+//   * The chunk bounds sit in registers, entry lane + 64 q of register q,
+//     read with readlane: an LDS or global read of them in the loop waited
+//     behind the other workgroup's product reads or the loads in flight.
+//   * The two buffers are distinct LDS variables and each half of the
+//     unrolled loop names its own, so the compiler's wait for LDS DMA stores
+//     stays off the other buffer's reads.
+// Unit u (16 bytes: values 2 part, 2 part + 1 of a row) of a chunk is row
+// u / (ms / 2), part u % (ms / 2): a buffer holds [rows][ms][hi, lo] words as
+// the register-staged form does, and the 64 lanes of one request fill 64
+// consecutive units.  Absent rows read past the buffer resource's range,
+// which loads zeros; values of a row past the block's bands come from the
+// next bands or row of the copy and meet only output columns that are
+// dropped.  Products and results as the split form, so the results are
+// bitwise those of k_vertex_col_split with the same chunks.
+constexpr int kColDmaUnits = 8 * 256;  // 16-byte units per LDS buffer (32 KB)
+constexpr int kColDmaChunks = 255;     // chunks of one plan (4 bound registers)
 
 template <int TPW>
 __global__ __launch_bounds__(256, 2) void k_vertex_col_dma(ColArgs a) {
-  constexpr int KC = kKcF32, KCA = KC + 4, NA = KC / 32, NBJ = kColDmaBuf / 256;
+  constexpr int KC = kKcF32, KCA = KC + 4, NA = KC / 32, NBJ = kColDmaUnits / 256;
   __shared__ __align__(16) uint32_t sAw[2 * 8 * KCA];
   __shared__ __align__(16) int sP[KC];
-  __shared__ __align__(16) uint32_t sB0[4 * kColDmaBuf];
-  __shared__ __align__(16) uint32_t sB1[4 * kColDmaBuf];
-  __shared__ int sCh[kColDmaChunks + 1], sCr[kColDmaChunks + 1];  // first position, first row of each chunk
+  __shared__ __align__(16) uint32_t sB0[4 * kColDmaUnits];
+  __shared__ __align__(16) uint32_t sB1[4 * kColDmaUnits];
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -645,7 +666,12 @@ __global__ __launch_bounds__(256, 2) void k_vertex_col_dma(ColArgs a) {
   const int g1 = (a.nbg == 2 && vb.n0 >= a.blo1) ? 1 : 0;
   const int s1 = a.bw[g1];
   const float2 *c1 = (const float2 *)a.box[g1] + (size_t)K1 * a.npwx * s1 + (vb.n0 - (g1 ? a.blo1 : 0));
-  const uint32_t *c2w = a.boxw + 2 * ((size_t)K2 * a.npwx * a.bww + (vb.m0 - (a.nbg == 2 ? a.blo1 : 0)));
+  // k+q rows: a buffer resource from this k's first row at the block's first
+  // band, its range the rest of the copy (at most 2^31 - 1 bytes; the rows
+  // read lie within the first npwx rows and 20 values past them)
+  const long long c2off = (long long)K2 * a.npwx * a.bww + (vb.m0 - (a.nbg == 2 ? a.blo1 : 0));
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(a.boxw + 2 * c2off), (short)0, (int)min(8 * (a.boxw_nv - c2off), 0x7fffffffLL), 0x00020000);
   const int *info = a.cls + (size_t)cl * kColInfo;
   const int *bmap = a.bmap + (size_t)a.g0idx[k] * a.nrows + info[0];
   const int *prow = a.prow + (size_t)cl * a.npos;
@@ -653,13 +679,20 @@ __global__ __launch_bounds__(256, 2) void k_vertex_col_dma(ColArgs a) {
   const int nmg = (vb.ncols + 3) >> 2;
   const int nt = info[1] * nmg;
   const int hu = a.ms >> 1;  // 16-byte units per staged row
-  // chunk bounds in LDS: read in the loop without waiting on the loads in
-  // flight (a global load of them would)
-  for (int i = tid; i <= a.nchunk; i += 256) {
-    sCh[i] = a.chunk[i];
-    sCr[i] = crow[i];
+
+  // chunk bounds: entry i of chunk / crow in lane i & 63 of register i >> 6
+  int chv[4], crv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = min(lane + 64 * q, a.nchunk);
+    chv[q] = a.chunk[i];
+    crv[q] = crow[i];
   }
-  __syncthreads();
+  auto bound = [&](const int (&v)[4], int i) {
+    const int q = i >> 6;
+    const int x = q == 0 ? v[0] : q == 1 ? v[1] : q == 2 ? v[2] : v[3];
+    return __builtin_amdgcn_readlane(x, i & 63);
+  };
 
   // ---- staging roles: C1 positions sa_p + 32 u, band sa_n; DMA unit tid + 256 j
   const int sa_p = tid >> 3, sa_n = tid & 7;
@@ -674,15 +707,15 @@ __global__ __launch_bounds__(256, 2) void k_vertex_col_dma(ColArgs a) {
   unsigned am = 0;     // bit u: av[u] present
   int pr = 0, npc = 0; // row of position tid, positions of that chunk
   auto load_idx = [&](int ch) {
-    const int p0 = sCh[ch], np = sCh[ch + 1] - p0;
-    const int r0 = sCr[ch], nr = sCr[ch + 1] - r0;
+    const int p0 = bound(chv, ch), np = bound(chv, ch + 1) - p0;
+    const int r0 = bound(crv, ch), nr = bound(crv, ch + 1) - r0;
 #pragma unroll
     for (int u = 0; u < NA; ++u) sl[u] = a.slot[p0 + min(sa_p + 32 * u, np - 1)];
 #pragma unroll
     for (int j = 0; j < NBJ; ++j) bm[j] = bmap[r0 + min(urow[j], nr - 1)];
   };
   auto load_a = [&](int ch) {
-    const int p0 = sCh[ch], np = sCh[ch + 1] - p0;
+    const int p0 = bound(chv, ch), np = bound(chv, ch + 1) - p0;
     unsigned m = 0;
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
@@ -693,22 +726,24 @@ __global__ __launch_bounds__(256, 2) void k_vertex_col_dma(ColArgs a) {
     pr = prow[p0 + min(tid, np - 1)];
     npc = np;
   };
-  // requests of whole waves past the chunk's units are skipped; lanes of the
-  // last request past them fill units of the buffer that no read reaches
-  auto dma = [&](int ch, uint32_t *buf) {
-    const int nu = (sCr[ch + 1] - sCr[ch]) * hu;
+  // requests j of this wave with (j >> 1) == i, i.e. two of them (of a
+  // chunk of nu units); requests of whole waves past the units are skipped,
+  // lanes of the last request past them fill units of the buffer that no
+  // read reaches
+  auto dma = [&](int i, int nu, uint32_t *buf) {
 #pragma unroll
     for (int j = 0; j < NBJ; ++j) {
       const int ub = 256 * j + 64 * wave;
-      if (ub < nu) {
+      if ((j >> 1) == i && ub < nu) {
         const bool ok = ub + lane < nu && bm[j] >= 0;
         const int part = tid + 256 * j - urow[j] * hu;
-        const uint32_t *src = ok ? c2w + 2 * ((size_t)bm[j] * a.bww + 2 * part) : a.zero16;
-        __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
-                                         (void __attribute__((address_space(3))) *)(buf + 4 * ub), 16, 0, 0);
+        const int off = ok ? 8 * (bm[j] * a.bww + 2 * part) : (int)0x80000000;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (void __attribute__((address_space(3))) *)(buf + 4 * ub), 16,
+                                                 off, 0, 0, 0);
       }
     }
   };
+  auto units = [&](int ch) { return (bound(crv, ch + 1) - bound(crv, ch)) * hu; };
   auto store_a = [&]() {
 #pragma unroll
     for (int u = 0; u < NA; ++u) {
@@ -736,23 +771,33 @@ __global__ __launch_bounds__(256, 2) void k_vertex_col_dma(ColArgs a) {
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt) acc[0][tt] = rpa_f4{0.f, 0.f, 0.f, 0.f};
 
-  // one chunk: its C1 values into LDS, the next chunk's requests and loads,
-  // then the products from buffer cur while the next chunk lands in nxt
+  // one chunk: its C1 values into LDS, then the products from buffer cur
+  // with the next chunk's requests and loads spread over their first blocks
+  // (two requests a block, then the C1 loads, then the indices): issued
+  // together they stalled every wave of the CU on the memory pipeline
+  // (about 4700 cycles a chunk at N_G = 51)
   auto step = [&](int ch, const uint32_t *cur, uint32_t *nxt) {
     __builtin_amdgcn_s_waitcnt(0);  // this chunk's rows and values have landed
     __syncthreads();                // ... in every wave; the last products are done
     store_a();
     __syncthreads();
     const int np = npc;
-    if (ch + 1 < a.nchunk) {
-      dma(ch + 1, nxt);
-      load_a(ch + 1);
-      if (ch + 2 < a.nchunk) load_idx(ch + 2);
-    }
-    split_products<TPW, KC>(acc[0], sAw, sP, cur, (np + 7) >> 3, a.ms, lane_off, toff, band_r, im_row, kg);
+    const bool more = ch + 1 < a.nchunk, more2 = ch + 2 < a.nchunk;
+    const int nu = more ? units(ch + 1) : 0;
+    auto issue = [&](int i) {
+      if (i < NBJ / 2) {
+        dma(i, nu, nxt);
+      } else if (i == NBJ / 2) {
+        if (more) load_a(ch + 1);
+      } else if (more2) {
+        load_idx(ch + 2);
+      }
+    };
+    split_products<TPW, KC>(acc[0], sAw, sP, cur, (np + 7) >> 3, a.ms, lane_off, toff, band_r, im_row, kg, issue,
+                            NBJ / 2 + 2);
   };
   load_idx(0);
-  dma(0, sB0);
+  for (int i = 0; i < NBJ / 2; ++i) dma(i, units(0), sB0);
   load_a(0);
   if (a.nchunk > 1) load_idx(1);
   for (int ch = 0; ch < a.nchunk; ch += 2) {
@@ -1007,8 +1052,7 @@ __global__ void k_build_boxw(const float2 *box, int npwx, int nrow, int bw, int 
 
 // Builds the split copy of the k+q side's box array (array 1 when the bands
 // are split, else array 0) once per box.  The tail of 32 zero values covers
-// reads of up to 20 values past the last row and holds the 16-byte zero area
-// that absent rows are read from.
+// reads of up to 20 values past the last row.
 static int ensure_boxw(rpa_ctx *c) {
   if (c->boxw_ready) return 0;
   const int g = c->box_nbg == 2 ? 1 : 0;
@@ -1021,7 +1065,7 @@ static int ensure_boxw(rpa_ctx *c) {
                      (const float2 *)c->d_box[g], c->d.npwx, c->box_rows, bw, bww, rows, nv, c->d_boxw);
   RPA_HIP(c, hipGetLastError());
   c->boxw_bww = bww;
-  c->boxw_zero = 2 * (size_t)(rows * bww + 28);
+  c->boxw_nv = nv;
   c->boxw_ready = true;
   return 0;
 }
@@ -1157,7 +1201,7 @@ int build_col_plan(rpa_ctx *c) {
   // whichever way they are staged, so both stagings sum alike
   const char *env_dma = std::getenv("RPA_VERTEX_COL_DMA");
   bool dma = split && box && !(env_dma && env_dma[0] == '0');
-  const int rlim = split ? std::min(rmax, kColDmaBuf / (ms / 2)) : rmax;
+  const int rlim = split ? std::min(rmax, kColDmaUnits / (ms / 2)) : rmax;
   std::vector<int> chunk{0};
   std::vector<std::pair<int, int>> chunk_segs;  // [first seg, end seg)
   {
@@ -1356,7 +1400,7 @@ int launch_vertices_col(rpa_ctx *c, const VBlock *vb, int nvb, void *rho, void *
   }
   if (dma) {
     a.boxw = c->d_boxw;
-    a.zero16 = c->d_boxw + c->boxw_zero;
+    a.boxw_nv = c->boxw_nv;
     a.bww = c->boxw_bww;
     launch_dma(c, a, P.tpw, grid);
   } else if (P.split) {

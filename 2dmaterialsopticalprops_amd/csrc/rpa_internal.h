@@ -207,7 +207,7 @@ struct rpa_ctx {
   uint32_t *d_boxw = nullptr;
   size_t cap_boxw = 0;
   int boxw_bww = 0;
-  size_t boxw_zero = 0;  // word offset of a 16-byte zero area
+  long long boxw_nv = 0;  // values, tail included
   bool boxw_ready = false;
   // vertex form of the current q ("" = shape rules; "col", "gather",
   // "staged") and staged chunk (0 = largest that fits), set by the autotuner
