@@ -223,7 +223,6 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
   const double *V = a.V;
   const bool jok = j < N;
   const int jc = jok ? j : N - 1;  // lanes past N work on a valid column, results dropped
-  double2 acc[kDyRows];
   auto ic = [&](int u) { return min(wave + 4 * u, N - 1); };
 
   // eps = I - chi0 V.  chi0 of this frequency is first copied to LDS in
@@ -242,30 +241,48 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
     }
   }
   __syncthreads();
+  // The three O(N^3) products run on the FP64 matrix cores: wave w takes the
+  // 16 x 16 tiles w, w + 4, ... of the (N padded to 16) result, K in steps of
+  // 4, each operand one value per lane (A[row lane & 15][k lane >> 4],
+  // B[k lane >> 4][column lane & 15]), read from clamped addresses and zeroed
+  // past N; D[4 e + (lane >> 4)][lane & 15] in register e.  (As one FMA per
+  // LDS broadcast on the vector units the products took 92 of the kernel's
+  // 204 us per frequency at N_G = 51.)
+  const int nT = (N + 15) >> 4, ntile = nT * nT, nk4 = (N + 3) >> 2;
+  const int lr = j & 15, lk = j >> 4;
+  // eps = I - chi0 V: A = Re, Im chi0 (LDS), B = V (L2)
+  rpa_d4 er[4], ei[4];
 #pragma unroll
-  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
-  for (int k0 = 0; k0 < N; k0 += 4) {
-    double v[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = V[min(k0 + q, N - 1) * N + jc];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (k0 + q < N) {  // uniform
-#pragma unroll
-        for (int u = 0; u < kDyRows; ++u) {
-          const double2 c = sa[ic(u) * N + k0 + q];
-          acc[u].x += c.x * v[q];
-          acc[u].y += c.y * v[q];
-        }
+  for (int t = 0; t < 4; ++t) {
+    er[t] = rpa_d4{0.0, 0.0, 0.0, 0.0};
+    ei[t] = rpa_d4{0.0, 0.0, 0.0, 0.0};
+    const int tile = wave + 4 * t;
+    if (tile < ntile) {  // uniform
+      const int row = 16 * (tile / nT) + lr, col = 16 * (tile % nT) + lr;
+      const int rc = min(row, N - 1), cc = min(col, N - 1);
+#pragma unroll 4
+      for (int kk = 0; kk < nk4; ++kk) {
+        const int k = 4 * kk + lk, kc = min(k, N - 1);
+        const double2 x = sa[rc * N + kc];
+        const double v = V[kc * N + cc];
+        const bool ok = row < N && k < N;  // B past N meets only dropped columns
+        const double ax = ok ? x.x : 0.0, ay = ok ? x.y : 0.0;
+        er[t] = mfma_f64_16x16x4(ax, v, er[t]);
+        ei[t] = mfma_f64_16x16x4(ay, v, ei[t]);
       }
     }
   }
   __syncthreads();  // every wave has read chi0
-  if (jok) {
 #pragma unroll
-    for (int u = 0; u < kDyRows; ++u) {
-      const int i = wave + 4 * u;
-      if (i < N) sa[i * N + j] = make_double2((i == j ? 1.0 : 0.0) - acc[u].x, -acc[u].y);
+  for (int t = 0; t < 4; ++t) {
+    const int tile = wave + 4 * t;
+    if (tile < ntile) {
+      const int col = 16 * (tile % nT) + lr;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = 16 * (tile / nT) + 4 * e + lk;
+        if (i < N && col < N) sa[i * N + col] = make_double2((i == col ? 1.0 : 0.0) - er[t][e], -ei[t][e]);
+      }
     }
   }
   __syncthreads();
@@ -341,8 +358,12 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
     const int pk = piv[k];
     cj = (cj == k) ? pk : (cj == pk) ? k : cj;
   }
+  // (the values move as native vectors: an array of HIP's double2 struct,
+  // only copied, went to scratch memory)
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  dv2 pv[kDyRows];
 #pragma unroll
-  for (int u = 0; u < kDyRows; ++u) acc[u] = sa[ic(u) * N + cj];
+  for (int u = 0; u < kDyRows; ++u) pv[u] = *(const dv2 *)&sa[ic(u) * N + cj];
   __syncthreads();
   double2 *Einv = a.epsinv + (size_t)f * N * N;
   double2 *Ch = a.chi + (size_t)f * N * N;
@@ -352,65 +373,50 @@ __global__ __launch_bounds__(256, 3) void k_dyson_col(DysonArgs a) {
     for (int u = 0; u < kDyRows; ++u) {
       const int i = wave + 4 * u;
       if (i < N) {
-        sa[i * N + j] = acc[u];
-        Einv[i * N + j] = acc[u];
+        *(dv2 *)&sa[i * N + j] = pv[u];
+        *(dv2 *)&Einv[i * N + j] = pv[u];
       }
     }
   }
   __syncthreads();
   if (tid == 0) a.loss[f] = -sa[0].y;
-  // chi = eps^-1 chi0 (chi0 rows read four k ahead)
+  // chi = eps^-1 chi0: A = eps^-1 (LDS), B = chi0 (L2); then W = V eps^-1
+  // (= V + V chi V): A = V (L2), B = eps^-1 (LDS)
 #pragma unroll
-  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
-  for (int k0 = 0; k0 < N; k0 += 4) {
-    double2 c[4];
+  for (int t = 0; t < 4; ++t) {
+    const int tile = wave + 4 * t;
+    if (tile < ntile) {  // uniform
+      const int row = 16 * (tile / nT) + lr, col = 16 * (tile % nT) + lr;
+      const int rc = min(row, N - 1), cc = min(col, N - 1);
+      rpa_d4 cr = {0.0, 0.0, 0.0, 0.0}, ci = {0.0, 0.0, 0.0, 0.0};
+      rpa_d4 wr = {0.0, 0.0, 0.0, 0.0}, wi = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+      for (int kk = 0; kk < nk4; ++kk) {
+        const int k = 4 * kk + lk, kc = min(k, N - 1);
+        const bool oka = row < N && k < N, okb = k < N && col < N;
+        const double2 e = sa[rc * N + kc];
+        const double2 c = X0[kc * N + cc];
+        const double v = V[rc * N + kc];
+        const double2 eb = sa[kc * N + cc];
+        const double ex = oka ? e.x : 0.0, ey = oka ? e.y : 0.0;
+        const double cx = okb ? c.x : 0.0, cy = okb ? c.y : 0.0;
+        const double va = oka ? v : 0.0;
+        const double bx = okb ? eb.x : 0.0, by = okb ? eb.y : 0.0;
+        cr = mfma_f64_16x16x4(ex, cx, cr);
+        cr = mfma_f64_16x16x4(-ey, cy, cr);
+        ci = mfma_f64_16x16x4(ex, cy, ci);
+        ci = mfma_f64_16x16x4(ey, cx, ci);
+        wr = mfma_f64_16x16x4(va, bx, wr);
+        wi = mfma_f64_16x16x4(va, by, wi);
+      }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) c[q] = X0[min(k0 + q, N - 1) * N + jc];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (k0 + q < N) {  // uniform
-#pragma unroll
-        for (int u = 0; u < kDyRows; ++u) {
-          const double2 e = sa[ic(u) * N + k0 + q];
-          acc[u].x += e.x * c[q].x - e.y * c[q].y;
-          acc[u].y += e.x * c[q].y + e.y * c[q].x;
+      for (int q = 0; q < 4; ++q) {
+        const int i = 16 * (tile / nT) + 4 * q + lk;
+        if (i < N && col < N) {
+          Ch[i * N + col] = make_double2(cr[q], ci[q]);
+          Wo[i * N + col] = make_double2(wr[q], wi[q]);
         }
       }
-    }
-  }
-  if (jok) {
-#pragma unroll
-    for (int u = 0; u < kDyRows; ++u) {
-      const int i = wave + 4 * u;
-      if (i < N) Ch[i * N + j] = acc[u];
-    }
-  }
-  // W = V eps^-1 (= V + V chi V); the V elements of two k are read together
-#pragma unroll
-  for (int u = 0; u < kDyRows; ++u) acc[u] = make_double2(0.0, 0.0);
-  for (int k0 = 0; k0 < N; k0 += 2) {
-    double vv[2][kDyRows];
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-      for (int u = 0; u < kDyRows; ++u) vv[q][u] = V[ic(u) * N + min(k0 + q, N - 1)];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      if (k0 + q < N) {  // uniform
-        const double2 e = sa[(k0 + q) * N + jc];
-#pragma unroll
-        for (int u = 0; u < kDyRows; ++u) {
-          acc[u].x += vv[q][u] * e.x;
-          acc[u].y += vv[q][u] * e.y;
-        }
-      }
-    }
-  }
-  if (jok) {
-#pragma unroll
-    for (int u = 0; u < kDyRows; ++u) {
-      const int i = wave + 4 * u;
-      if (i < N) Wo[i * N + j] = acc[u];
     }
   }
 }
