@@ -140,21 +140,27 @@ __global__ __launch_bounds__(256) void k_tile_hist(const int *key, long long ntr
 // first sorted position of key, toff[nkeys] = total.  Tiles of 4096
 // consecutive counts, four per thread (coalesced loads), a shuffle scan of
 // the thread totals per wave and the 16 wave totals through LDS; the running
-// total of earlier tiles is carried.  (One count per thread: 0.040 ms for
-// the 35k counts of a 393k-transition run, the tile loop's latency.)
+// total of earlier tiles is carried.  The next tile's counts are loaded
+// while the current one is scanned.  (One count per thread: 0.040 ms for the
+// 35k counts of a 393k-transition run; four per thread, each tile's loads
+// waited on before its scan: 0.025 ms.)
 __global__ __launch_bounds__(1024) void k_tile_scan(int *bcnt, long long n, int nkeys, int nblk,
                                                      int *toff) {
   __shared__ int wsum[16];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   int carry = 0;  // total of the earlier tiles
+  int vn[4];
+  auto load = [&](long long base) {  // clamped: no conditional load
+#pragma unroll
+    for (int q = 0; q < 4; ++q) vn[q] = bcnt[min(base + 4 * tid + q, n - 1)];
+  };
+  load(0);
   for (long long base = 0; base < n; base += 4096) {
     const long long i0 = base + 4 * tid;
     int v[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int c = bcnt[i0 + q < n ? i0 + q : n - 1];  // clamped: no conditional load
-      v[q] = (i0 + q < n) ? c : 0;
-    }
+    for (int q = 0; q < 4; ++q) v[q] = (i0 + q < n) ? vn[q] : 0;
+    load(base + 4096);  // past the end: clamped loads, never used
     const int ts = (v[0] + v[1]) + (v[2] + v[3]);
     int x = ts;  // inclusive scan of the thread totals within the wave
 #pragma unroll
