@@ -783,8 +783,9 @@ constexpr size_t kDysonMxStatic = (sizeof(float2) + sizeof(int)) * 64 + sizeof(d
 
 // The mixed path keeps the FP64 Gauss-Jordan by default: at N_G = 51 the
 // FP32 Gauss-Jordan with its FP64 refinement took 0.32 ms for the 700
-// frequencies against 0.24 ms (the refinement adds an FP64 product as large
-// as the elimination saves).  RPA_DYSON=mixed selects it.
+// frequencies against 0.24 ms with the FP64 form's products on the vector
+// units (the refinement adds an FP64 product as large as the elimination
+// saves), 0.19 ms with them on the matrix cores.  RPA_DYSON=mixed selects it.
 static bool mixed_dyson(const rpa_ctx *c) {
   if (c->d.precision != RPA_MIXED) return false;
   const char *env = std::getenv("RPA_DYSON");
