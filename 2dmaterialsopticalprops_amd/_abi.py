@@ -29,7 +29,7 @@ EXPORTS = [
     "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms", "rpa_get_result",
     "rpa_load_s0", "rpa_w_kk", "rpa_kk_noalloc", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
     "rpa_batch_result", "rpa_set_symmetry", "rpa_set_q_sym", "rpa_batch_set_sym", "rpa_find_kq_sym",
-    "rpa_point_inverse",
+    "rpa_point_inverse", "rpa_interpolate",
 ]
 # enum rpa_result
 RES = {"S0": 0, "QEFF": 1, "CHI0": 2, "V": 3, "EPSINV": 4, "CHI": 5, "W": 6, "LOSS": 7,
@@ -48,6 +48,12 @@ class RpaParams(C.Structure):
                 ("eta", C.c_double), ("lor_cut", C.c_double), ("df_cut", C.c_double),
                 ("efermi", C.c_double), ("temp", C.c_double), ("vcell", C.c_double),
                 ("c0", C.c_double), ("eta_intra", C.c_double)]
+
+
+class RpaInterpParams(C.Structure):
+    _fields_ = [("no", C.c_int), ("nlf", C.c_int), ("no_interp", C.c_int), ("all", C.c_int),
+                ("device", C.c_int), ("domega", C.c_double), ("domega_tot", C.c_double),
+                ("eta_intra", C.c_double)]
 
 
 class RpaError(RuntimeError):
@@ -111,6 +117,7 @@ def lib() -> C.CDLL:
         L.rpa_find_kq_sym.argtypes = [C.c_int, P, C.c_int, P, C.c_int, P, P, C.c_int, P, C.c_double,
                                       P, P, P, P, P]
         L.rpa_point_inverse.argtypes = [C.c_int, P, P]
+        L.rpa_interpolate.argtypes = [C.POINTER(RpaInterpParams), P, P, P, P]
         _lib = L
     return _lib
 
@@ -197,6 +204,30 @@ def gk_sort(k, g, gcutw, npwx):
     n = C.c_int()
     rc = lib().rpa_gk_sort(_ptr(kk), len(gg), _ptr(gg), gcutw, npwx, C.byref(n), _ptr(igk))
     return rc, igk[:n.value].copy()
+
+
+def interpolate(S0, qeff, no_interp, domega, domega_tot, eta_intra, all_pairs=True, device=0):
+    """interpolate.x on the device (rpa_interpolate): S0(-No:No,Nlf,Nlf) and
+    Qeff(Nlf,Nlf) as read from Corrfun_{pol} / Qeff_{pol} ->
+    (Pi_tot_interp(No_tot,Nlf,Nlf), or its (1,1) column (No_tot) when
+    all_pairs is False, and the intraband column (No_tot))."""
+    S0f = _f(S0, np.complex128)
+    qf = _f(qeff, np.complex128)
+    if S0f.ndim != 3 or S0f.shape[1] != S0f.shape[2] or S0f.shape[0] % 2 != 1:
+        raise ValueError("S0 must be (2 No + 1, Nlf, Nlf)")
+    nlf = S0f.shape[1]
+    if qf.shape != (nlf, nlf):
+        raise ValueError("Qeff must be (Nlf, Nlf)")
+    no = (S0f.shape[0] - 1) // 2
+    no_tot = max(no_interp - 1, 0) * max(no - 2, 0)
+    shape = (no_tot, nlf, nlf) if all_pairs else (no_tot,)
+    out = np.zeros(shape, dtype=np.complex128, order="F")
+    intra = np.zeros(no_tot, dtype=np.complex128)
+    p = RpaInterpParams(no, nlf, no_interp, 1 if all_pairs else 0, device, domega, domega_tot, eta_intra)
+    rc = lib().rpa_interpolate(C.byref(p), _ptr(S0f), _ptr(qf), _ptr(out), _ptr(intra))
+    if rc != RPA_OK:
+        raise RpaError(rc, lib().rpa_last_error(None).decode())
+    return out, intra
 
 
 def _check(ctx, rc):

@@ -220,6 +220,20 @@ int launch_kk(rpa_ctx *c) {
   return 0;
 }
 
+hipError_t kk_optabs_columns(hipStream_t s, int no, int nlf, int ncol, double dw, const double *s0, double *kmat,
+                             double *x, double *chi0) {
+  const int nw = 2 * no + 1;
+  const long long nk = (long long)(no - 1) * nw;
+  hipLaunchKernelGGL(k_kstencil, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, (int)RPA_OPTABS, no, nw, dw,
+                     kmat);
+  dim3 grid((ncol + 63) / 64, (no - 1 + 63) / 64);
+  hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, s, no - 1, ncol, nw, (const double *)kmat, s0, x);
+  const long long n = (long long)(no - 1) * nlf * nlf;
+  hipLaunchKernelGGL(k_chi0_epilogue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int)RPA_OPTABS, no, nlf,
+                     ncol, (const double *)x, s0, chi0);
+  return hipGetLastError();
+}
+
 // ---- Kramers-Kronig without allocation (rpa_kk_noalloc) ------------------
 // The stencil K and the product X = K S0 of this call live in a file-scope
 // workspace that every call on a device shares, instead of the context's

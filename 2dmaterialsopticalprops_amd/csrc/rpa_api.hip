@@ -10,6 +10,11 @@
 
 namespace rpa {
 
+// the calls that take no context (rpa_interpolate) keep their message per thread
+static thread_local std::string g_free_err = "null context";
+
+void set_free_error(const std::string &msg) { g_free_err = msg; }
+
 int fail(rpa_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
   return code;
@@ -454,7 +459,7 @@ int rpa_ctx_destroy(rpa_ctx *c) {
   return RPA_OK;
 }
 
-const char *rpa_last_error(const rpa_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *rpa_last_error(const rpa_ctx *c) { return c ? c->err.c_str() : g_free_err.c_str(); }
 
 double rpa_last_run_ms(const rpa_ctx *c) { return c ? c->last_ms : 0.0; }
 

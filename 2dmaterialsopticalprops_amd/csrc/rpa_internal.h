@@ -377,6 +377,14 @@ int configure_vertex(rpa_ctx *c);
 int launch_epilogue(rpa_ctx *c);
 int noalloc_init_device(int device);
 int launch_kk_noalloc(rpa_ctx *c, double *chi0_dev);
+// The Optabs Kramers-Kronig transform (stencil, product, epilogue) of ncol
+// = 2 nlf^2 interleaved complex S0 columns s0[2no+1][ncol] on a stream, with
+// no context: kmat holds (no-1)(2no+1) and x (no-1) ncol doubles, chi0 gets
+// [no-1][nlf*nlf] complex.
+hipError_t kk_optabs_columns(hipStream_t s, int no, int nlf, int ncol, double dw, const double *s0, double *kmat,
+                             double *x, double *chi0);
+// The message of the last failed call that takes no context (this thread).
+void set_free_error(const std::string &msg);
 
 }  // namespace rpa
 

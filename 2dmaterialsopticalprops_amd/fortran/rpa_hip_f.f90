@@ -28,6 +28,11 @@ module rpa_hip
     real(c_double) :: eps, gcar, domega, eta, lor_cut, df_cut, efermi, temp, vcell, c0, eta_intra
   end type rpa_params_t
 
+  type, bind(C) :: rpa_interp_params_t
+    integer(c_int) :: no, nlf, no_interp, all, device
+    real(c_double) :: domega, domega_tot, eta_intra
+  end type rpa_interp_params_t
+
   interface
     integer(c_int) function rpa_ctx_create(dims, ctx) bind(C, name="rpa_ctx_create")
       import :: c_int, c_ptr, rpa_dims_t
@@ -185,6 +190,11 @@ module rpa_hip
     integer(c_long) function c_strlen(s) bind(C, name="strlen")
       import :: c_long, c_ptr
       type(c_ptr), value :: s
+    end function
+    integer(c_int) function rpa_interpolate(p, S0, qeff, pi_dense, pi_intra) bind(C, name="rpa_interpolate")
+      import :: c_int, c_ptr, rpa_interp_params_t
+      type(rpa_interp_params_t), intent(in) :: p
+      type(c_ptr), value :: S0, qeff, pi_dense, pi_intra
     end function
   end interface
 

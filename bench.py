@@ -500,7 +500,11 @@ def main():
     ap.add_argument("--hbm-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (x1.0 elementwise kernel), "
                          "the reference point for FETCH_SIZE / WRITE_SIZE counter scaling")
+    ap.add_argument("--interpolate", action="store_true",
+                    help="time interpolate.x (rpa_interpolate) at No=2001, Nlf=65, No_interp=10 instead")
     args = ap.parse_args()
+    if args.interpolate:
+        return bench_interpolate(args)
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
               "(q points would shard across GPUs with no exchange)", file=sys.stderr)
@@ -593,6 +597,52 @@ def main():
     for rr in runners.values():
         rr.close()
     print(json.dumps(out))
+
+
+def bench_interpolate(args):
+    """interpolate.x (Chi_RPA/interpolate.f90) at the size its own comments
+    name (No = 2001 coarse frequencies, No_interp = 10, Nlf = 65, every
+    (iG, jG) pair written): one step = one rpa_interpolate call, host S0 and
+    Qeff in, the dense Pi_tot_interp(17991, 65, 65) back on the host (the
+    call's copies and device buffers included; the kernels' own times come
+    from the kernel trace)."""
+    abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
+    no, nlf, nint = 2001, 65, 10
+    hartree = 2.0 * 13.6056923
+    omax = 50.0 / hartree
+    dw, dwt, eta = omax / (no - 1), omax / ((nint - 1) * (no - 2)), 0.025 / hartree
+    rng = np.random.default_rng(1)
+    shape = (2 * no + 1, nlf, nlf)
+    s0 = np.asfortranarray(1e-2 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)))
+    qeff = np.asfortranarray(1e-2 * (rng.standard_normal((nlf, nlf)) + 1j * rng.standard_normal((nlf, nlf))))
+    no_tot = (nint - 1) * (no - 2)
+    for _ in range(args.warmup):
+        abi.interpolate(s0, qeff, nint, dw, dwt, eta)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, _ = abi.interpolate(s0, qeff, nint, dw, dwt, eta)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    vals = no_tot * nlf * nlf
+    res = {"metric": "interpolate.x dense Pi values per second (host in, host out)",
+           "value": round(vals / (ms * 1e-3), 1), "unit": "(point, pair) values/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"interpolate.x: No={no}, No_interp={nint} ({no_tot} dense points), Nlf={nlf}, "
+                                  f"all pairs", "output_bytes": vals * 16}}
+    if not args.no_cpu:
+        orc = importlib.import_module("oracle.oracle")
+        k = 3  # local fields of the sample: k*k of the nlf*nlf independent pairs
+        t0 = time.perf_counter()
+        pi0 = orc.kk_optabs(np.asfortranarray(s0[:, :k, :k]), dw)
+        orc.interpolate(pi0, np.asfortranarray(qeff[:k, :k]), nint, dwt, eta)
+        t = (time.perf_counter() - t0) * (nlf * nlf) / (k * k)
+        res["cpu_baseline"] = {"value": round(vals / t, 1), "unit": "(point, pair) values/s",
+                               "cores": min(k * k, host_threads()), "kind": "port",
+                               "sample": f"oracle on {k * k} of {nlf * nlf} pairs, scaled"}
+        ref = orc.interpolate(pi0, np.asfortranarray(qeff[:k, :k]), nint, dwt, eta)[0]
+        den = np.abs(ref).max()
+        res["max_rel_err_vs_oracle_sample"] = float(np.abs(out[:, :k, :k] - ref).max() / den)
+    print(json.dumps(res))
 
 
 def program_name(p):

@@ -304,6 +304,35 @@ int rpa_gk_sort(const double *k, int ngm, const double *g, double gcutw, int npw
 /* G(3,n) = KC * mill (Optabs_qe6.f90:1162-1167). */
 int rpa_gcart(int n, const int *mill, const double *kc, double *g);
 
+/* ---- interpolate.x (Chi_RPA/interpolate.f90) ----------------------------- */
+typedef struct rpa_interp_params {
+  int no;            /* No: S0 on -No..No, Pi_tot on io = 1..No-1              */
+  int nlf;           /* Nlf                                                     */
+  int no_interp;     /* No_interp: No_interp-1 dense points per coarse interval,
+                        No_tot = (No_interp-1)(No-2) (interpolate.f90:54)      */
+  int all;           /* write_GiGj: 1 = Pi for every (iG,jG), 0 = (1,1) only    */
+  int device;        /* HIP device                                              */
+  double domega;     /* (omax-omin)/(No-1), Hartree (interpolate.f90:60)        */
+  double domega_tot; /* (omax-omin)/No_tot, Hartree (interpolate.f90:61)        */
+  double eta_intra;  /* Hartree                                                 */
+} rpa_interp_params;
+
+/* The whole computation of interpolate.x after its file reads
+ * (interpolate.f90:103-200; the contained genReChi0/genImChi0,
+ * interpolate.f90:242-464, are Optabs' K1 stencil): from S0(-No:No,Nlf,Nlf)
+ * and Qeff(Nlf,Nlf) (complex, column-major, as read from Corrfun_{pol} and
+ * Qeff_{pol}) the interpolated response on the dense grid
+ *   all = 1: pi_dense = Pi_tot_interp(No_tot,Nlf,Nlf) complex
+ *   all = 0: pi_dense = Pi_tot_interp(No_tot,1,1)     complex
+ * and, when pi_intra is not NULL, pi_intra(No_tot) = Qeff(1,1) oi/(oi + i
+ * eta_intra) with oi = c domega_tot (the Pi_RPA_{pol}_dense_intra column).
+ * Needs no context: it creates its own stream and device buffers and releases
+ * them before returning; rpa_last_error(NULL) gives the message of a failed
+ * call on the calling thread.  No_interp = 1 gives No_tot = 0 and returns
+ * with nothing written, as the reference's loops do.                       */
+int rpa_interpolate(const rpa_interp_params *p, const double *S0, const double *qeff, double *pi_dense,
+                    double *pi_intra);
+
 #ifdef __cplusplus
 }
 #endif

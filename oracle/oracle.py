@@ -75,6 +75,8 @@ def lib():
         L.orc_dyson_w.argtypes = [C.c_int, P_CPLX, P_DBL, P_CPLX, P_CPLX, P_CPLX]
         L.orc_pi_assembly.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       P_CPLX, P_CPLX, P_CPLX, P_DBL, P_DBL]
+        L.orc_interpolate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      P_CPLX, P_CPLX, P_CPLX, P_CPLX]
         L.orc_gen_glf.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_int, P_DBL, P_INT,
                                   C.c_int, C.POINTER(C.c_int), P_DBL, P_INT, P_INT]
         L.orc_occupation.argtypes = [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
@@ -397,3 +399,18 @@ def pi_assembly(Pi0, Qeff, domega, eta_intra, c0):
     lib().orc_pi_assembly(no, nlf, domega, eta_intra, c0, F(Pi0, np.complex128),
                           F(Qeff, np.complex128), pi11, si, sa)
     return pi11, si, sa
+
+
+def interpolate(Pi0, Qeff, no_interp, domega_tot, eta_intra):
+    """interpolate.x after the KK (Chi_RPA/interpolate.f90:103-186):
+    Pi_tot_interp(No_tot,Nlf,Nlf) and the intraband column (No_tot) from
+    Pi0 = kk_optabs(S0, domega)."""
+    no = Pi0.shape[0] + 1
+    nlf = Pi0.shape[1]
+    no_tot = (no_interp - 1) * (no - 2)
+    out = np.zeros((max(no_tot, 1), nlf, nlf), dtype=np.complex128, order="F")
+    intra = np.zeros(max(no_tot, 1), dtype=np.complex128, order="F")
+    if no_tot > 0:
+        lib().orc_interpolate(no, nlf, no_interp, domega_tot, eta_intra, F(Pi0, np.complex128),
+                              F(Qeff, np.complex128), out, intra)
+    return out[:no_tot], intra[:no_tot]

@@ -1272,6 +1272,41 @@ void orc_dyson_w(int nlf, const orc_z *chi0, const double *V, orc_z *epsinv,
   free(c);
 }
 
+/* f2: interpolate.x, Chi_RPA/interpolate.f90:103-186.  Pi0(no-1,Nlf,Nlf) is
+ * (ReChi0, ImChi0) of every io = 1..No-1: interpolate.f90's contained
+ * genReChi0/genImChi0 (:242-464) are Optabs' K1, i.e. orc_kk_optabs.
+ * Pi_tot(io) = Pi0(io) + Pi_dia with Pi_dia = -Re Pi0(1) (:116-123); then
+ * for io = 1..No-2, jo = 1..No_interp-1 (:166-181):
+ *   counter += 1, oi = counter domega_tot,
+ *   pi_interp(counter) = Pi_tot(io) + (Pi_tot(io+1) - Pi_tot(io))/No_interp jo
+ *                        + Qeff oi/(oi + i eta_intra)
+ * pi_interp(No_tot,Nlf,Nlf); pi_intra(No_tot) = Qeff(1,1) oi/(oi + i eta).  */
+void orc_interpolate(int no, int nlf, int no_interp, double domega_tot, double eta_intra,
+                     const orc_z *Pi0, const orc_z *Qeff, orc_z *pi_interp, orc_z *pi_intra) {
+  const size_t nn = (size_t)nlf * nlf, nf = (size_t)(no - 1);
+  const size_t no_tot = (size_t)(no_interp - 1) * (size_t)(no - 2);
+  orc_z *tot = (orc_z *)malloc(sizeof(orc_z) * nf * nn);
+  for (size_t x = 0; x < nn; ++x) {
+    orc_z dia = -(creal(Pi0[nf * x]) + I * 0.0);
+    for (size_t io = 0; io < nf; ++io) tot[io + nf * x] = Pi0[io + nf * x] + dia;
+  }
+  size_t counter = 0;
+  for (int io = 1; io <= no - 2; ++io)
+    for (int jo = 1; jo <= no_interp - 1; ++jo) {
+      counter += 1;
+      double oi = (double)counter * domega_tot;
+      orc_z den = oi + I * eta_intra;
+      for (size_t x = 0; x < nn; ++x) {
+        orc_z a = tot[(io - 1) + nf * x], b = tot[io + nf * x];
+        orc_z tmp = (b - a) / (double)no_interp;
+        orc_z v = a + tmp * (double)jo;
+        pi_interp[(counter - 1) + no_tot * x] = v + Qeff[x] * oi / den;
+      }
+      if (pi_intra) pi_intra[counter - 1] = Qeff[0] * oi / den;
+    }
+  free(tot);
+}
+
 /* a13: Pi assembly, Optabs.f90:575-611.  Pi0(no-1,Nlf,Nlf) from orc_kk_optabs;
  * pi11[io-1] = Pi_tot(1,1); sig_inter/sig_intra = Re(-i c0 Pi/oi) (io > 1,
  * entry 0 left 0).  Pi_inter is read back after each (iG,jG) update exactly as

@@ -132,6 +132,10 @@ void orc_pi_assembly(int no, int nlf, double domega, double eta_intra, double c0
                      const orc_z *Pi0, const orc_z *Qeff, orc_z *pi11,
                      double *sig_inter, double *sig_intra);
 
+/* ---- f2: interpolate.x dense-grid response ------------------------------ */
+void orc_interpolate(int no, int nlf, int no_interp, double domega_tot, double eta_intra,
+                     const orc_z *Pi0, const orc_z *Qeff, orc_z *pi_interp, orc_z *pi_intra);
+
 #ifdef __cplusplus
 }
 #endif

@@ -225,3 +225,22 @@ def test_point_inverse_reports_a_singular_operation():
     with pytest.raises(abi.RpaError) as e:
         abi.point_inverse(R, 2)
     assert e.value.code == abi.RPA_E_SINGULAR
+
+
+def test_interpolate_argument_checks_need_no_device():
+    """rpa_interpolate rejects bad sizes before any device call, with its
+    message under rpa_last_error(NULL); No_interp = 1 leaves No_tot = 0 and
+    returns with nothing to write (interpolate.f90:54, 166-169)."""
+    s0 = np.zeros((7, 2, 2), dtype=np.complex128)  # No = 3
+    q = np.zeros((2, 2), dtype=np.complex128)
+    with pytest.raises(abi.RpaError) as e:
+        abi.interpolate(s0, q, 10, 0.01, 0.001, 0.001)
+    assert e.value.code == abi.RPA_E_ARG and "No must be at least 4" in str(e.value)
+    s0 = np.zeros((13, 2, 2), dtype=np.complex128)  # No = 6
+    with pytest.raises(abi.RpaError) as e:
+        abi.interpolate(s0, q, 10, 0.0, 0.001, 0.001)
+    assert e.value.code == abi.RPA_E_ARG and "domega" in str(e.value)
+    with pytest.raises(ValueError):
+        abi.interpolate(s0, np.zeros((3, 3)), 10, 0.01, 0.001, 0.001)
+    out, intra = abi.interpolate(s0, q, 1, 0.01, 0.001, 0.001)
+    assert out.shape == (0, 2, 2) and intra.shape == (0,)

@@ -678,3 +678,60 @@ def test_point_inverse_matches_lapack():
     RI = orc.point_inverse(R, nsym)
     for i in range(nsym):
         np.testing.assert_allclose(RI[i], np.linalg.inv(R[i]), rtol=0, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# interpolate.x (Chi_RPA/interpolate.f90:103-186)
+
+def _dense_by_np_interp(pi0, qeff, no_interp, dwt, eta):
+    """The dense grid written another way: Pi_tot = Pi0 - Re Pi0(1) on the
+    coarse nodes x = 0..No-2, np.interp at x = (io-1) + jo/No_interp, plus
+    Qeff oi/(oi + i eta) with oi = counter * domega_tot."""
+    no = pi0.shape[0] + 1
+    tot = pi0 - pi0[0].real[None]
+    x = np.arange(no - 1, dtype=np.float64)
+    t = np.array([(io - 1) + jo / no_interp for io in range(1, no - 1) for jo in range(1, no_interp)])
+    oi = np.arange(1, len(t) + 1) * dwt
+    nlf = pi0.shape[1]
+    out = np.zeros((len(t), nlf, nlf), dtype=np.complex128)
+    for i in range(nlf):
+        for j in range(nlf):
+            out[:, i, j] = (np.interp(t, x, tot[:, i, j].real) + 1j * np.interp(t, x, tot[:, i, j].imag)
+                            + qeff[i, j] * oi / (oi + 1j * eta))
+    return out, qeff[0, 0] * oi / (oi + 1j * eta)
+
+
+@pytest.mark.parametrize("no,nlf,no_interp", [(9, 3, 10), (17, 2, 2), (6, 1, 5), (40, 4, 7)])
+def test_interpolate_is_linear_interpolation_of_the_kk_response(no, nlf, no_interp):
+    """orc_interpolate against np.interp on Pi0 - Re Pi0(io=1) plus the
+    intraband closed form; a divisor No_interp - 1, a counter starting at 0,
+    Pi_dia taken at another io or Qeff(1,1) on every pair fails it."""
+    rng = np.random.default_rng(no * 100 + nlf)
+    dw, dwt, eta = 0.013, 0.0021, 0.004
+    s0 = (rng.standard_normal((2 * no + 1, nlf, nlf)) + 1j * rng.standard_normal((2 * no + 1, nlf, nlf)))
+    qeff = rng.standard_normal((nlf, nlf)) + 1j * rng.standard_normal((nlf, nlf))
+    pi0 = orc.kk_optabs(s0, dw)
+    got, intra = orc.interpolate(pi0, qeff, no_interp, dwt, eta)
+    want, want_intra = _dense_by_np_interp(pi0, qeff, no_interp, dwt, eta)
+    assert got.shape == ((no_interp - 1) * (no - 2), nlf, nlf)
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13 * scale)
+    np.testing.assert_allclose(intra, want_intra, rtol=1e-14, atol=0)
+
+
+def test_interpolate_one_point_per_interval_is_empty_and_the_static_real_part_vanishes():
+    """No_interp = 1 gives No_tot = 0 (the reference's jo loop is empty);
+    Re Pi_tot(io = 1) = 0 by Pi_dia, so with Qeff = 0 the first No_interp - 1
+    dense points have real part Re Pi_tot(2) * jo / No_interp exactly up to
+    rounding."""
+    rng = np.random.default_rng(5)
+    no, nlf = 8, 2
+    s0 = rng.standard_normal((2 * no + 1, nlf, nlf)) + 1j * rng.standard_normal((2 * no + 1, nlf, nlf))
+    pi0 = orc.kk_optabs(s0, 0.02)
+    got, intra = orc.interpolate(pi0, np.zeros((nlf, nlf)), 1, 0.01, 0.003)
+    assert got.shape[0] == 0 and intra.shape[0] == 0
+    n = 4
+    got, _ = orc.interpolate(pi0, np.zeros((nlf, nlf)), n, 0.01, 0.003)
+    re2 = (pi0[1] - pi0[0]).real
+    for jo in range(1, n):
+        np.testing.assert_allclose(got[jo - 1].real, re2 * jo / n, rtol=1e-14, atol=1e-15)
