@@ -29,7 +29,7 @@ EXPORTS = [
     "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms", "rpa_get_result",
     "rpa_load_s0", "rpa_w_kk", "rpa_kk_noalloc", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
     "rpa_batch_result", "rpa_set_symmetry", "rpa_set_q_sym", "rpa_batch_set_sym", "rpa_find_kq_sym",
-    "rpa_point_inverse", "rpa_interpolate",
+    "rpa_point_inverse", "rpa_interpolate", "rpa_photon",
 ]
 # enum rpa_result
 RES = {"S0": 0, "QEFF": 1, "CHI0": 2, "V": 3, "EPSINV": 4, "CHI": 5, "W": 6, "LOSS": 7,
@@ -54,6 +54,11 @@ class RpaInterpParams(C.Structure):
     _fields_ = [("no", C.c_int), ("nlf", C.c_int), ("no_interp", C.c_int), ("all", C.c_int),
                 ("device", C.c_int), ("domega", C.c_double), ("domega_tot", C.c_double),
                 ("eta_intra", C.c_double)]
+
+
+class RpaPhotonParams(C.Structure):
+    _fields_ = [("no", C.c_int), ("nlf", C.c_int), ("nline", C.c_int), ("device", C.c_int),
+                ("omin", C.c_double), ("domega", C.c_double), ("eta", C.c_double), ("c0", C.c_double)]
 
 
 class RpaError(RuntimeError):
@@ -118,6 +123,7 @@ def lib() -> C.CDLL:
                                       P, P, P, P, P]
         L.rpa_point_inverse.argtypes = [C.c_int, P, P]
         L.rpa_interpolate.argtypes = [C.POINTER(RpaInterpParams), P, P, P, P]
+        L.rpa_photon.argtypes = [C.POINTER(RpaPhotonParams), P, P, P, P, P, P, P, P, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -228,6 +234,42 @@ def interpolate(S0, qeff, no_interp, domega, domega_tot, eta_intra, all_pairs=Tr
     if rc != RPA_OK:
         raise RpaError(rc, lib().rpa_last_error(None).decode())
     return out, intra
+
+
+def photon(pixx0, piyy0, pizz0, glfz, parg, lines, omin, domega, eta, c0, piyz0=None, pizy0=None,
+           full=False, device=0):
+    """Photon.x's per-point work on the device (rpa_photon).  Pi0 arrays are
+    (No, Nlf, Nlf) complex as loadPi0 reads them; lines is a list of
+    (q, theta, qmode) with qmode 1 for q = gamma o sin(theta).  Returns a dict
+    with pi11 (No-1, nline, 3), psum (No-1, nline, 3), spectra (No-1, nline, 2)
+    and, with full=True, pi (Nlf, Nlf, 5, No-1, nline)."""
+    arrs = [_f(a, np.complex128) if a is not None else None for a in (pixx0, piyy0, pizz0, piyz0, pizy0)]
+    no, nlf = arrs[0].shape[0], arrs[0].shape[1]
+    for a in arrs:
+        if a is not None and a.shape != (no, nlf, nlf):
+            raise ValueError("Pi0 arrays must all be (No, Nlf, Nlf)")
+    gz = _f(glfz, np.float64)
+    pg = _f(parg, np.int32)
+    if gz.shape != (nlf,) or pg.shape != (nlf,):
+        raise ValueError("glfz and parg must have Nlf entries")
+    nline = len(lines)
+    ql = np.array([l[0] for l in lines], dtype=np.float64)
+    tl = np.array([l[1] for l in lines], dtype=np.float64)
+    qm = np.array([l[2] for l in lines], dtype=np.int32)
+    nf = no - 1
+    pi11 = np.zeros((3, nf, nline), dtype=np.complex128, order="F")
+    psum = np.zeros((3, nf, nline), dtype=np.complex128, order="F")
+    spec = np.zeros((2, nf, nline), order="F")
+    pif = np.zeros((nlf, nlf, 5, nf, nline), dtype=np.complex128, order="F") if full else None
+    p = RpaPhotonParams(no, nlf, nline, device, omin, domega, eta, c0)
+    rc = lib().rpa_photon(C.byref(p), _ptr(gz), _ptr(pg), *[_ptr(a) for a in arrs], _ptr(ql), _ptr(tl),
+                          _ptr(qm), _ptr(pi11), _ptr(psum), _ptr(spec), _ptr(pif))
+    if rc != RPA_OK:
+        raise RpaError(rc, lib().rpa_last_error(None).decode())
+    out = {"pi11": np.moveaxis(pi11, 0, -1), "psum": np.moveaxis(psum, 0, -1), "spectra": np.moveaxis(spec, 0, -1)}
+    if full:
+        out["pi"] = pif
+    return out
 
 
 def _check(ctx, rc):

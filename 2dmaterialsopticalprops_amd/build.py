@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["rpa_api.hip", "k_vertex.hip", "k_vertex_col.hip", "k_accum.hip", "k_kk.hip", "k_dyson.hip", "k_interp.hip",
+SOURCES = ["rpa_api.hip", "k_vertex.hip", "k_vertex_col.hip", "k_accum.hip", "k_kk.hip", "k_dyson.hip", "k_interp.hip", "k_photon.hip",
            "rpa_host.cpp"]
 
 
@@ -66,7 +66,7 @@ FORTRAN = os.path.join(PKG, "fortran")
 BIN = os.path.join(PKG, "bin")
 F_MODULES = ["rpa_hip_f.f90", "rpa_files.f90"]
 F_PROGRAMS = {"sloss_hip.x": "sloss_hip.f90", "optabs_hip.x": "optabs_hip.f90",
-              "interpolate_hip.x": "interpolate_hip.f90"}
+              "interpolate_hip.x": "interpolate_hip.f90", "photon_hip.x": "photon_hip.f90"}
 F_FLAGS = ["-O2"]
 
 

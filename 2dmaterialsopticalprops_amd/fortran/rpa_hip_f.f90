@@ -28,6 +28,11 @@ module rpa_hip
     real(c_double) :: eps, gcar, domega, eta, lor_cut, df_cut, efermi, temp, vcell, c0, eta_intra
   end type rpa_params_t
 
+  type, bind(C) :: rpa_photon_params_t
+    integer(c_int) :: no, nlf, nline, device
+    real(c_double) :: omin, domega, eta, c0
+  end type rpa_photon_params_t
+
   type, bind(C) :: rpa_interp_params_t
     integer(c_int) :: no, nlf, no_interp, all, device
     real(c_double) :: domega, domega_tot, eta_intra
@@ -195,6 +200,14 @@ module rpa_hip
       import :: c_int, c_ptr, rpa_interp_params_t
       type(rpa_interp_params_t), intent(in) :: p
       type(c_ptr), value :: S0, qeff, pi_dense, pi_intra
+    end function
+    integer(c_int) function rpa_photon(p, glfz, parg, pixx0, piyy0, pizz0, piyz0, pizy0, q_line, theta_line, &
+                                       qmode_line, pi11, psum, spectra, pi_full) bind(C, name="rpa_photon")
+      import :: c_int, c_double, c_ptr, rpa_photon_params_t
+      type(rpa_photon_params_t), intent(in) :: p
+      real(c_double), intent(in) :: glfz(*), q_line(*), theta_line(*)
+      integer(c_int), intent(in) :: parg(*), qmode_line(*)
+      type(c_ptr), value :: pixx0, piyy0, pizz0, piyz0, pizy0, pi11, psum, spectra, pi_full
     end function
   end interface
 

@@ -502,9 +502,13 @@ def main():
                          "the reference point for FETCH_SIZE / WRITE_SIZE counter scaling")
     ap.add_argument("--interpolate", action="store_true",
                     help="time interpolate.x (rpa_interpolate) at No=2001, Nlf=65, No_interp=10 instead")
+    ap.add_argument("--photon", action="store_true",
+                    help="time Photon.x (rpa_photon) at No=2001, Nlf=33, 10 q lines instead")
     args = ap.parse_args()
     if args.interpolate:
         return bench_interpolate(args)
+    if args.photon:
+        return bench_photon(args)
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path "
               "(q points would shard across GPUs with no exchange)", file=sys.stderr)
@@ -642,6 +646,58 @@ def bench_interpolate(args):
         ref = orc.interpolate(pi0, np.asfortranarray(qeff[:k, :k]), nint, dwt, eta)[0]
         den = np.abs(ref).max()
         res["max_rel_err_vs_oracle_sample"] = float(np.abs(out[:, :k, :k] - ref).max() / den)
+    print(json.dumps(res))
+
+
+def bench_photon(args):
+    """Photon.x (BSE/Photonew.f90) at the size of its own inputs: No = 2001
+    frequencies of Pi0_xx/yy/zz (0.5-50 eV), Nlf = 33 local fields along z
+    (the reference's 5 Ha graphene count), 10 q lines: 20000 (q, omega)
+    points, each an N_G and a 2 N_G complex inversion with their products.
+    One step = one rpa_photon call, host Pi0 in, the per-point outputs back."""
+    abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
+    no, nlf, nq, c0 = 2001, 33, 10, 32.5411
+    hartree = 2.0 * 13.6056923
+    omin = 0.5 / hartree
+    dw = (50.0 / hartree) / (no - 1)
+    eta = 0.05 / hartree
+    ms = np.array([0] + [(k + 1) // 2 * (1 if k % 2 else -1) for k in range(1, nlf)])
+    gz = 2 * np.pi * ms / c0
+    par = np.where(ms % 2 == 0, 1, -1).astype(np.int32)
+    rng = np.random.default_rng(2)
+    comps = []
+    for _ in range(3):
+        a = (0.02 / nlf) * (rng.standard_normal((no, nlf, nlf)) + 1j * rng.standard_normal((no, nlf, nlf)))
+        comps.append(np.asfortranarray(a + np.swapaxes(a, 1, 2)))
+    lines = [(iq * 0.002, 0.0, 0) for iq in range(nq)]
+    for _ in range(args.warmup):
+        abi.photon(comps[0], comps[1], comps[2], gz, par, lines, omin, dw, eta, c0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = abi.photon(comps[0], comps[1], comps[2], gz, par, lines, omin, dw, eta, c0)
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    npts = (no - 1) * nq
+    res = {"metric": "Photon.x (q, omega) points per second (host in, host out)",
+           "value": round(npts / (ms_step * 1e-3), 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic",
+           "config": {"workload": f"Photon.x: No={no}, Nlf={nlf} (TS {nlf}x{nlf}, TP {2 * nlf}x{2 * nlf}), "
+                                  f"{nq} q lines, {npts} points"}}
+    if not args.no_cpu:
+        orc = importlib.import_module("oracle.oracle")
+        nsample = 200
+        pts = [(l, f) for l in range(nq) for f in range(0, no - 1, (no - 1) * nq // nsample)][:nsample]
+        t0 = time.perf_counter()
+        worst = 0.0
+        for l, f in pts:
+            pi0 = np.zeros((nlf, nlf, 5), dtype=np.complex128, order="F")
+            pi0[:, :, 0], pi0[:, :, 1], pi0[:, :, 4] = comps[0][f], comps[1][f], comps[2][f]
+            _, sums, spec, _ = orc.photon_point(lines[l][0], omin + f * dw, eta, c0, 0.0, par, gz, pi0)
+            worst = max(worst, abs(out["psum"][f, l, 1] - sums[1]) / max(abs(sums[1]), 1e-300))
+        t = (time.perf_counter() - t0) * npts / len(pts)
+        res["cpu_baseline"] = {"value": round(npts / t, 1), "unit": "points/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle on {len(pts)} of {npts} points, scaled"}
+        res["max_rel_err_vs_oracle_sample_sPiyy"] = float(worst)
     print(json.dumps(res))
 
 

@@ -333,6 +333,42 @@ typedef struct rpa_interp_params {
 int rpa_interpolate(const rpa_interp_params *p, const double *S0, const double *qeff, double *pi_dense,
                     double *pi_intra);
 
+/* ---- Photon.x (BSE/Photonew.f90) ------------------------------------------ */
+typedef struct rpa_photon_params {
+  int no;         /* No: frequencies of the Pi0 arrays; io = 1..No-1 are solved  */
+  int nlf;        /* Nlf (<= 64)                                                */
+  int nline;      /* spectrum lines (the reference's (theta, q) iterations)     */
+  int device;     /* HIP device                                                 */
+  double omin;    /* Hartree; o = omin + (io-1) domega (Photonew.f90:237)        */
+  double domega;  /* (omax-omin)/(No-1), Hartree                                 */
+  double eta;     /* Hartree                                                     */
+  double c0;      /* slab height, a.u.                                           */
+} rpa_photon_params;
+
+/* The per-point work of Photon.x's angle / q / frequency loops
+ * (Photonew.f90:205-330): for line l and io = 1..No-1, with oi = o + i eta,
+ * q = q_line[l] (qmode_line[l] = 0) or gamma o sin(theta_line[l])
+ * (qmode_line[l] = 1, Photonew.f90:240-249) and beta = sqrt(gamma^2 oi^2 - q^2):
+ * genD0, TS = I - Pixx0 Dxx0 and TP = I - Pi0_p D0_p (genTS/genTP, 890-966),
+ * their inverses (invert, 388-462), the screened blocks (genScreenedPi,
+ * 744-787), the slab sums (sumIntScreenedPi, 826-888) and the p-mode
+ * absorption and reflection (genSpectra, 968-1064).  Inputs: glfz(Nlf) =
+ * Glf(3,:) in a.u.^-1, parg(Nlf) = +-1, Pi0 arrays (No,Nlf,Nlf) complex as
+ * loadPi0 reads them (665-696); piyz0 / pizy0 may be NULL (zero, as loadPi0
+ * sets them).  Outputs per point p = io + (No-1)(l-1), component fastest:
+ *   pi11(3, No-1, nline)   complex  screened Pixx(1,1), Piyy(1,1), Pizz(1,1)
+ *   psum(3, No-1, nline)   complex  sPixx, sPiyy, sPizz
+ *   spectra(2, No-1, nline) real    A_p, R_p (transmission is 1 - A_p - R_p)
+ *   pi_full (may be NULL): (Nlf, Nlf, 5, No-1, nline) complex, the screened
+ *            Pixx, Piyy, Piyz, Pizy, Pizz.
+ * A zero pivot in TS or TP returns RPA_E_SINGULAR after the whole run (the
+ * reference stops, Photonew.f90:421-425).  No context: rpa_last_error(NULL)
+ * gives the message.                                                        */
+int rpa_photon(const rpa_photon_params *p, const double *glfz, const int *parg, const double *pixx0,
+               const double *piyy0, const double *pizz0, const double *piyz0, const double *pizy0,
+               const double *q_line, const double *theta_line, const int *qmode_line, double *pi11, double *psum,
+               double *spectra, double *pi_full);
+
 #ifdef __cplusplus
 }
 #endif

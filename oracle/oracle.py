@@ -75,6 +75,11 @@ def lib():
         L.orc_dyson_w.argtypes = [C.c_int, P_CPLX, P_DBL, P_CPLX, P_CPLX, P_CPLX]
         L.orc_pi_assembly.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       P_CPLX, P_CPLX, P_CPLX, P_DBL, P_DBL]
+        L.orc_lu_invert.argtypes = [P_CPLX, C.c_int]
+        L.orc_photon_d0.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, P_INT,
+                                    P_DBL, P_CPLX]
+        L.orc_photon_point.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_int, P_INT, P_DBL, P_CPLX, P_CPLX, P_CPLX, P_DBL]
         L.orc_interpolate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                       P_CPLX, P_CPLX, P_CPLX, P_CPLX]
         L.orc_gen_glf.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_int, P_DBL, P_INT,
@@ -414,3 +419,32 @@ def interpolate(Pi0, Qeff, no_interp, domega_tot, eta_intra):
         lib().orc_interpolate(no, nlf, no_interp, domega_tot, eta_intra, F(Pi0, np.complex128),
                               F(Qeff, np.complex128), out, intra)
     return out[:no_tot], intra[:no_tot]
+
+
+def lu_invert(a):
+    """zgetrf + zgetri as Photon's invert uses them (BSE/Photonew.f90:388-462);
+    returns (inverse, info)."""
+    a = F(a, np.complex128).copy(order="F")
+    info = lib().orc_lu_invert(a, a.shape[0])
+    return a, info
+
+
+def photon_d0(q, o, eta, c0, parg, glfz):
+    """genD0 (BSE/Photonew.f90:618-663): (Dxx0, Dyy0, Dzz0, Dyz0) as (N,N,4)."""
+    n = len(glfz)
+    D = np.zeros((n, n, 4), dtype=np.complex128, order="F")
+    lib().orc_photon_d0(q, o, eta, c0, n, F(parg, np.int32), F(glfz, np.float64), D)
+    return D
+
+
+def photon_point(q, o, eta, c0, theta, parg, glfz, pi0):
+    """One point of Photon's loops (BSE/Photonew.f90:229-330): pi0 (N,N,5) =
+    xx, yy, yz, zy, zz -> (screened (N,N,5), sums (5: xx, yy, zz, yz, zy),
+    (A_p, R_p), status)."""
+    n = len(glfz)
+    pi = np.zeros((n, n, 5), dtype=np.complex128, order="F")
+    sums = np.zeros(5, dtype=np.complex128)
+    spec = np.zeros(2)
+    st = lib().orc_photon_point(q, o, eta, c0, theta, n, F(parg, np.int32), F(glfz, np.float64),
+                                F(pi0, np.complex128), pi, sums, spec)
+    return pi, sums, spec, st

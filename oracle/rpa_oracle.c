@@ -1272,6 +1272,194 @@ void orc_dyson_w(int nlf, const orc_z *chi0, const double *V, orc_z *epsinv,
   free(c);
 }
 
+/* f3: Photon.x.  zgetrf (partial pivoting on |Re| + |Im|, the first
+ * maximum) then the inverse column by column from the LU factors: the same
+ * matrix zgetri returns, in another operation order.                       */
+int orc_lu_invert(orc_z *a, int n) {
+#define A_(i, j) a[(size_t)(i) + (size_t)n * (j)]
+  int *ip = (int *)malloc(sizeof(int) * (size_t)n);
+  int info = 0;
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double best = -1.0;
+    for (int i = k; i < n; ++i) {
+      double m = fabs(creal(A_(i, k))) + fabs(cimag(A_(i, k)));
+      if (m > best) {
+        best = m;
+        p = i;
+      }
+    }
+    ip[k] = p;
+    if (best == 0.0) {
+      if (!info) info = k + 1;
+      continue;
+    }
+    if (p != k)
+      for (int j = 0; j < n; ++j) {
+        orc_z t = A_(k, j);
+        A_(k, j) = A_(p, j);
+        A_(p, j) = t;
+      }
+    for (int i = k + 1; i < n; ++i) {
+      A_(i, k) = A_(i, k) / A_(k, k);
+      for (int j = k + 1; j < n; ++j) A_(i, j) = A_(i, j) - A_(i, k) * A_(k, j);
+    }
+  }
+  if (!info) {
+    orc_z *lu = (orc_z *)malloc(sizeof(orc_z) * (size_t)n * n);
+    memcpy(lu, a, sizeof(orc_z) * (size_t)n * n);
+    for (int c = 0; c < n; ++c) {
+      orc_z *x = a + (size_t)n * c; /* column c of the inverse: solve A x = e_c */
+      for (int i = 0; i < n; ++i) x[i] = (i == c) ? 1.0 : 0.0;
+      for (int k = 0; k < n; ++k)
+        if (ip[k] != k) {
+          orc_z t = x[k];
+          x[k] = x[ip[k]];
+          x[ip[k]] = t;
+        }
+      for (int i = 0; i < n; ++i)
+        for (int k = 0; k < i; ++k) x[i] = x[i] - lu[i + (size_t)n * k] * x[k];
+      for (int i = n - 1; i >= 0; --i) {
+        for (int k = i + 1; k < n; ++k) x[i] = x[i] - lu[i + (size_t)n * k] * x[k];
+        x[i] = x[i] / lu[i + (size_t)n * i];
+      }
+    }
+    free(lu);
+  }
+  free(ip);
+  return info;
+#undef A_
+}
+
+static const double ORC_GAMMA = 1.0 / 137.0; /* Photonew.f90:38 */
+
+/* genD0, Photonew.f90:618-663, in its statement order */
+void orc_photon_d0(double q, double o, double eta, double c0, int nlf, const int *parg,
+                   const double *glfz, orc_z *D) {
+  const size_t nn = (size_t)nlf * nlf;
+  orc_z oi = o + I * eta;
+  orc_z beta = csqrt(ORC_GAMMA * ORC_GAMMA * oi * oi - q * q + 0.0 * I);
+  for (int i = 0; i < nlf; ++i) {
+    orc_z aii = parg[i] * (1.0 - cexp(I * beta * c0)) / (beta * beta - glfz[i] * glfz[i]);
+    for (int j = 0; j < nlf; ++j) {
+      orc_z ajj = aii * parg[j] / (beta * beta - glfz[j] * glfz[j]);
+      orc_z aij = 2.0 * (beta * beta + glfz[i] * glfz[j]) * ajj / c0;
+      orc_z bij = 2.0 * beta * (glfz[i] + glfz[j]) * ajj / c0;
+      orc_z d1 = 0.0, d2 = 0.0;
+      if (i == j) {
+        d1 = 2.0 * I * beta / (beta * beta - glfz[i] * glfz[i]);
+        d2 = 2.0 * I * glfz[i] / (beta * beta - glfz[i] * glfz[i]);
+      }
+      size_t x = (size_t)i + (size_t)nlf * j;
+      D[x] = 2.0 * ORC_PI * I * ORC_GAMMA * ORC_GAMMA * (aij + d1) / beta;
+      D[nn + x] = 2.0 * ORC_PI * I * beta * (aij + d1) / (oi * oi);
+      D[3 * nn + x] = -2.0 * ORC_PI * I * q * (bij + d2) / (oi * oi);
+      D[2 * nn + x] = 2.0 * ORC_PI * I * q * q * (aij + d1) / (beta * oi * oi);
+    }
+    D[2 * nn + (size_t)i * (nlf + 1)] -= 4.0 * ORC_PI / (oi * oi);
+  }
+}
+
+int orc_photon_point(double q, double o, double eta, double c0, double theta, int nlf,
+                     const int *parg, const double *glfz, const orc_z *pi0, orc_z *pi,
+                     orc_z *sums, double *spec) {
+  const int N = nlf, M = 2 * nlf;
+  const size_t nn = (size_t)N * N;
+  orc_z *D = (orc_z *)malloc(sizeof(orc_z) * 4 * nn);
+  orc_z *TS = (orc_z *)malloc(sizeof(orc_z) * nn);
+  orc_z *TP = (orc_z *)malloc(sizeof(orc_z) * 4 * nn);
+  orc_photon_d0(q, o, eta, c0, N, parg, glfz, D);
+  const orc_z *Dxx = D, *Dyy = D + nn, *Dzz = D + 2 * nn, *Dyz = D + 3 * nn, *Dzy = Dyz;
+  const orc_z *Pxx = pi0, *Pyy = pi0 + nn, *Pyz = pi0 + 2 * nn, *Pzy = pi0 + 3 * nn, *Pzz = pi0 + 4 * nn;
+#define E_(A, i, j) (A)[(size_t)(i) + (size_t)N * (j)]
+  /* genTS (890-917) */
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      orc_z s = 0.0;
+      for (int k = 0; k < N; ++k) s += E_(Pxx, i, k) * E_(Dxx, k, j);
+      E_(TS, i, j) = (i == j ? 1.0 : 0.0) - s;
+    }
+  int st_s = orc_lu_invert(TS, N);
+  /* genTP (919-966) */
+  for (int I2 = 0; I2 < M; ++I2)
+    for (int J2 = 0; J2 < M; ++J2) {
+      orc_z s = 0.0;
+      int i = I2 % N, j = J2 % N;
+      for (int k = 0; k < N; ++k) {
+        if (I2 < N && J2 < N)
+          s += E_(Pyy, i, k) * E_(Dyy, k, j) + E_(Pyz, i, k) * E_(Dzy, k, j);
+        else if (I2 < N)
+          s += E_(Pyy, i, k) * E_(Dyz, k, j) + E_(Pyz, i, k) * E_(Dzz, k, j);
+        else if (J2 < N)
+          s += E_(Pzy, i, k) * E_(Dyy, k, j) + E_(Pzz, i, k) * E_(Dzy, k, j);
+        else
+          s += E_(Pzy, i, k) * E_(Dyz, k, j) + E_(Pzz, i, k) * E_(Dzz, k, j);
+      }
+      TP[(size_t)I2 + (size_t)M * J2] = (I2 == J2 ? 1.0 : 0.0) - s;
+    }
+  int st_p = orc_lu_invert(TP, M);
+#define T_(i, j) TP[(size_t)(i) + (size_t)M * (j)]
+  /* genScreenedPi (744-787) */
+  orc_z *Xx = pi, *Yy = pi + nn, *Yz = pi + 2 * nn, *Zy = pi + 3 * nn, *Zz = pi + 4 * nn;
+  for (size_t x = 0; x < 5 * nn; ++x) pi[x] = 0.0;
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j)
+      for (int k = 0; k < N; ++k) {
+        E_(Xx, i, j) += E_(TS, i, k) * E_(Pxx, k, j);
+        E_(Yy, i, j) += T_(i, k) * E_(Pyy, k, j) + T_(i, k + N) * E_(Pzy, k, j);
+        E_(Yz, i, j) += T_(i, k) * E_(Pyz, k, j) + T_(i, k + N) * E_(Pzz, k, j);
+        E_(Zy, i, j) += T_(i + N, k) * E_(Pyy, k, j) + T_(i + N, k + N) * E_(Pzy, k, j);
+        E_(Zz, i, j) += T_(i + N, k) * E_(Pyz, k, j) + T_(i + N, k + N) * E_(Pzz, k, j);
+      }
+  /* sumIntScreenedPi (826-888): order xx, yy, zz, yz, zy */
+  const orc_z *mats[5] = {Xx, Yy, Zz, Yz, Zy};
+  for (int m = 0; m < 5; ++m) sums[m] = 0.0;
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      orc_z w;
+      if (i == 0 && j != 0)
+        w = (-I * (-1.0 + cexp(I * (glfz[j] * c0 / 2)))) * c0 / (2 * glfz[j]);
+      else if (i != 0 && j == 0)
+        w = (-I * (1.0 - cexp(-I * (glfz[i] * c0 / 2)))) * c0 / (2 * glfz[i]);
+      else if (i == 0 && j == 0)
+        w = c0 * c0 / 4;
+      else
+        w = -(-1.0 + cexp(I * (glfz[i] * c0 / 2))) * (-1.0 + cexp(I * (glfz[j] * c0 / 2))) /
+            (glfz[i] * glfz[j] * cexp(I * (glfz[i] * c0 / 2)));
+      /* the last branch subtracts P (.)(.)/(.) in the reference: w carries the sign */
+      for (int m = 0; m < 5; ++m) sums[m] += E_(mats[m], i, j) * w;
+    }
+  for (int m = 0; m < 5; ++m) sums[m] = 1 / c0 * sums[m];
+  /* genSpectra (968-1064), p mode */
+  orc_z oi = o + I * eta;
+  orc_z beta = csqrt(ORC_GAMMA * ORC_GAMMA * oi * oi - q * q + 0.0 * I);
+  orc_z dyy_r = 0.0, dzz_r = 0.0, dyy_a = 0.0, dzz_a = 0.0, dyz_a = 0.0;
+  for (int i = 0; i < N; ++i) {
+    double ff1 = creal((2.0 * parg[i] / sqrt(c0)) * csin(beta * c0 / 2.0) / (beta + glfz[i]));
+    for (int j = 0; j < N; ++j) {
+      double ff2 = creal((2.0 * parg[j] / sqrt(c0)) * csin(beta * c0 / 2.0) / (beta + glfz[j]));
+      double gf2 = creal((2.0 * parg[j] / sqrt(c0)) * csin(beta * c0 / 2.0) / (beta - glfz[j]));
+      dyy_r += ff1 * E_(Yy, i, j) * gf2;
+      dzz_r += ff1 * E_(Zz, i, j) * gf2;
+      dyy_a += ff1 * E_(Yy, i, j) * ff2;
+      dzz_a += ff1 * E_(Zz, i, j) * ff2;
+      dyz_a += ff1 * E_(Yz, i, j) * ff2;
+    }
+  }
+  orc_z dyy0 = 2 * ORC_PI * ORC_GAMMA * I / oi;
+  orc_z ref_p = dyy0 * (dyy_r * cos(theta) - dzz_r * sin(theta) * sin(theta) / cos(theta));
+  orc_z abso_p = cos(theta) * dyy_a * cos(theta) + sin(theta) * dzz_a * sin(theta) -
+                 2.0 * cos(theta) * dyz_a * sin(theta);
+  spec[0] = (4.0 * ORC_PI * ORC_GAMMA / o) * cimag(abso_p);
+  spec[1] = creal(ref_p * conj(ref_p));
+#undef T_
+#undef E_
+  free(D);
+  free(TS);
+  free(TP);
+  return st_s ? st_s : (st_p ? -st_p : 0);
+}
+
 /* f2: interpolate.x, Chi_RPA/interpolate.f90:103-186.  Pi0(no-1,Nlf,Nlf) is
  * (ReChi0, ImChi0) of every io = 1..No-1: interpolate.f90's contained
  * genReChi0/genImChi0 (:242-464) are Optabs' K1, i.e. orc_kk_optabs.

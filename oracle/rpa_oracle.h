@@ -132,6 +132,23 @@ void orc_pi_assembly(int no, int nlf, double domega, double eta_intra, double c0
                      const orc_z *Pi0, const orc_z *Qeff, orc_z *pi11,
                      double *sig_inter, double *sig_intra);
 
+/* ---- f3: Photon.x per (line, frequency) point ----------------------------- */
+/* LU with partial pivoting and the inverse from it (zgetrf + zgetri, the
+ * reference's invert, BSE/Photonew.f90:388-462); a is n x n column-major,
+ * overwritten by its inverse.  Returns 0, or k > 0 for a zero pivot at k.  */
+int  orc_lu_invert(orc_z *a, int n);
+/* genD0 (Photonew.f90:618-663): D(N,N,4) = Dxx0, Dyy0, Dzz0, Dyz0 (= Dzy0). */
+void orc_photon_d0(double q, double o, double eta, double c0, int nlf, const int *parg,
+                   const double *glfz, orc_z *D);
+/* One point of the angle / q / frequency loop (Photonew.f90:229-330):
+ * pi0(N,N,5) = Pixx0, Piyy0, Piyz0, Pizy0, Pizz0 at this frequency;
+ * pi(N,N,5) the screened blocks, sums(5) the sumIntScreenedPi values
+ * (xx, yy, zz, yz, zy), spec(2) = A_p, R_p of genSpectra.  Returns the
+ * orc_lu_invert status of TS, then of TP.                                 */
+int  orc_photon_point(double q, double o, double eta, double c0, double theta, int nlf,
+                      const int *parg, const double *glfz, const orc_z *pi0, orc_z *pi,
+                      orc_z *sums, double *spec);
+
 /* ---- f2: interpolate.x dense-grid response ------------------------------ */
 void orc_interpolate(int no, int nlf, int no_interp, double domega_tot, double eta_intra,
                      const orc_z *Pi0, const orc_z *Qeff, orc_z *pi_interp, orc_z *pi_intra);

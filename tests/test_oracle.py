@@ -735,3 +735,114 @@ def test_interpolate_one_point_per_interval_is_empty_and_the_static_real_part_va
     re2 = (pi0[1] - pi0[0]).real
     for jo in range(1, n):
         np.testing.assert_allclose(got[jo - 1].real, re2 * jo / n, rtol=1e-14, atol=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# Photon.x (BSE/Photonew.f90)
+
+def test_lu_invert_matches_lapack_and_reports_a_zero_pivot():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 7, 33, 66):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        inv, info = orc.lu_invert(a)
+        assert info == 0
+        np.testing.assert_allclose(inv, np.linalg.inv(a), rtol=0, atol=1e-12 * np.abs(np.linalg.inv(a)).max())
+    a = np.ones((3, 3), dtype=np.complex128)
+    _, info = orc.lu_invert(a)
+    assert info == 2
+
+
+def _slab_integrals(beta, gi, gj, c0):
+    """(1/c0) int int over the slab of e^{-i Gi z} K(z - z') e^{i Gj z'} for
+    K = e^{i beta |z - z'|} and K = sign(z - z') e^{i beta |z - z'|}, the inner
+    integral split at z' = z."""
+    from scipy import integrate
+    h = c0 / 2
+
+    def inner(z, sgn):
+        def f(zp, part):
+            v = np.exp(1j * beta * abs(z - zp)) * np.exp(1j * gj * zp) * (np.sign(z - zp) if sgn else 1.0)
+            return v.real if part == 0 else v.imag
+        r = sum(integrate.quad(f, a, b, args=(0,), epsabs=1e-13, epsrel=1e-12)[0] for a, b in ((-h, z), (z, h)))
+        i = sum(integrate.quad(f, a, b, args=(1,), epsabs=1e-13, epsrel=1e-12)[0] for a, b in ((-h, z), (z, h)))
+        return r + 1j * i
+
+    out = []
+    for sgn in (False, True):
+        def g(z, part):
+            v = np.exp(-1j * gi * z) * inner(z, sgn)
+            return v.real if part == 0 else v.imag
+        r = integrate.quad(g, -h, h, args=(0,), epsabs=1e-12, epsrel=1e-11, limit=200)[0]
+        i = integrate.quad(g, -h, h, args=(1,), epsabs=1e-12, epsrel=1e-11, limit=200)[0]
+        out.append((r + 1j * i) / c0)
+    return out
+
+
+def test_photon_d0_is_the_slab_average_of_the_free_propagator():
+    """genD0's A + diagonal1 and B + diagonal2 (Photonew.f90:632-643) are the
+    slab double integrals of e^{i beta|z-z'|} and sign(z-z') e^{i beta|z-z'|}
+    between plane waves of Gz = 2 pi m / c0, parity (-1)^m; Dxx0, Dyy0, Dyz0
+    and Dzz0 are them times the propagator's prefactors.  A swapped index, a
+    wrong parity or sign fails it."""
+    c0, o, eta, q = 7.3, 0.35, 0.01, 0.02
+    ms = np.array([0, 1, -1, 2])
+    gz = 2 * np.pi * ms / c0
+    par = np.where(ms % 2 == 0, 1, -1).astype(np.int32)
+    D = orc.photon_d0(q, o, eta, c0, par, gz)
+    gam = 1 / 137.0
+    oi = o + 1j * eta
+    beta = np.sqrt(gam ** 2 * oi ** 2 - q ** 2 + 0j)
+    a_d1 = D[:, :, 0] * beta / (2j * np.pi * gam ** 2)
+    b_d2 = D[:, :, 3] * oi ** 2 / (-2j * np.pi * q)
+    for i, j in [(0, 0), (1, 1), (0, 1), (1, 2), (3, 1), (2, 0)]:
+        ia, ib = _slab_integrals(beta, gz[i], gz[j], c0)
+        assert abs(a_d1[i, j] - ia) < 1e-8 * max(abs(ia), 1e-3), (i, j, a_d1[i, j], ia)
+        assert abs(b_d2[i, j] - ib) < 1e-8 * max(abs(ib), 1e-3), (i, j, b_d2[i, j], ib)
+    np.testing.assert_allclose(D[:, :, 1], 2j * np.pi * beta * a_d1 / oi ** 2, rtol=1e-13)
+    np.testing.assert_allclose(D[:, :, 2], 2j * np.pi * q ** 2 * a_d1 / (beta * oi ** 2)
+                               - np.eye(4) * 4 * np.pi / oi ** 2, rtol=1e-13, atol=1e-13)
+
+
+def _photon_inputs(n, seed, scale=0.2):
+    rng = np.random.default_rng(seed)
+    c0 = 9.0
+    ms = np.array([0] + [(k + 1) // 2 * (1 if k % 2 else -1) for k in range(1, n)])
+    gz = 2 * np.pi * ms / c0
+    par = np.where(ms % 2 == 0, 1, -1).astype(np.int32)
+    pi0 = scale * (rng.standard_normal((n, n, 5)) + 1j * rng.standard_normal((n, n, 5)))
+    return c0, gz, par, np.asfortranarray(pi0)
+
+
+@pytest.mark.parametrize("n", [1, 3, 6])
+def test_photon_point_solves_the_dyson_equation_and_sums_the_slab_integral(n):
+    """The screened blocks satisfy Pi = Pi0 + Pi0 D0 Pi (s mode with Dxx0, p
+    mode with the 2x2 block propagator), the reference's TS/TP inversion
+    written as a numpy solve; the sums equal (1/c0) sum Pi(G,G') int_0^{c0/2}
+    e^{-iGz} dz int_0^{c0/2} e^{iG'z'} dz' by quadrature; with Pi0 = 0 the
+    absorption and reflection vanish."""
+    from scipy import integrate
+    c0, gz, par, pi0 = _photon_inputs(n, seed=n)
+    q, o, eta, theta = 0.03, 0.4, 0.02, 0.0
+    pi, sums, spec, st = orc.photon_point(q, o, eta, c0, theta, par, gz, pi0)
+    assert st == 0
+    D = orc.photon_d0(q, o, eta, c0, par, gz)
+    Dxx, Dyy, Dzz, Dyz = (D[:, :, k] for k in range(4))
+    xx0, yy0, yz0, zy0, zz0 = (pi0[:, :, k] for k in range(5))
+    np.testing.assert_allclose(pi[:, :, 0], xx0 + xx0 @ Dxx @ pi[:, :, 0], rtol=0, atol=1e-12)
+    P0 = np.block([[yy0, yz0], [zy0, zz0]])
+    Dp = np.block([[Dyy, Dyz], [Dyz, Dzz]])
+    Pp = np.block([[pi[:, :, 1], pi[:, :, 2]], [pi[:, :, 3], pi[:, :, 4]]])
+    np.testing.assert_allclose(Pp, P0 + P0 @ Dp @ Pp, rtol=0, atol=1e-12)
+    h = c0 / 2
+
+    def cquad(f):
+        return (integrate.quad(lambda z: f(z).real, 0, h, epsabs=1e-13)[0]
+                + 1j * integrate.quad(lambda z: f(z).imag, 0, h, epsabs=1e-13)[0])
+
+    left = np.array([cquad(lambda z, g=g: np.exp(-1j * g * z)) for g in gz])
+    right = np.array([cquad(lambda z, g=g: np.exp(1j * g * z)) for g in gz])
+    for m, k in zip(range(5), (0, 1, 4, 2, 3)):  # sums: xx, yy, zz, yz, zy
+        want = left @ pi[:, :, k] @ right / c0
+        assert abs(sums[m] - want) < 1e-11 * max(1.0, abs(want)), (m, sums[m], want)
+    _, _, spec0, _ = orc.photon_point(q, o, eta, c0, theta, par, gz, np.zeros_like(pi0))
+    assert np.all(spec0 == 0.0)

@@ -1,0 +1,189 @@
+"""Photon.x on the GPU (rpa_photon, photon_hip.x) against the oracle
+(orc_photon_point, the reference's loops of BSE/Photonew.f90:229-330 with its
+O(N^3) products and an LU inverse).
+
+The device builds TS and TP from the rank-two form of D0 and inverts them by
+Gauss-Jordan; the oracle multiplies the explicit D0 and inverts by LU, so the
+two agree to rounding times the condition of T: 1e-10 of the largest value.
+The p-mode reflection is a squared modulus and the absorption a difference of
+such products, so they are compared to 1e-9 absolute on values of order 1e-3
+to 1.  Driver outputs are list-directed doubles: 1e-9.
+"""
+
+import importlib
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import helpers as H
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
+build = importlib.import_module("2dmaterialsopticalprops_amd.build")
+synth = H.synth
+
+GAMMA = 1.0 / 137.0
+HARTREE = 2.0 * 13.6056923
+
+
+def _fields(n, c0):
+    ms = np.array([0] + [(k + 1) // 2 * (1 if k % 2 else -1) for k in range(1, n)])
+    return 2 * np.pi * ms / c0, np.where(ms % 2 == 0, 1, -1).astype(np.int32)
+
+
+def _pi0(no, n, seed, scale, general=False):
+    rng = np.random.default_rng(seed)
+    comps = []
+    for c in range(5):
+        if c in (2, 3) and not general:
+            comps.append(np.zeros((no, n, n), dtype=np.complex128))
+            continue
+        a = scale * (rng.standard_normal((no, n, n)) + 1j * rng.standard_normal((no, n, n)))
+        comps.append(np.asfortranarray(a + np.swapaxes(a, 1, 2)))  # symmetric in (G, G')
+    return comps  # xx, yy, yz, zy, zz
+
+
+def _oracle(comps, gz, par, lines, omin, dw, eta, c0):
+    no, n = comps[0].shape[0], comps[0].shape[1]
+    nf, nl = no - 1, len(lines)
+    out = {"pi11": np.zeros((nf, nl, 3), complex), "psum": np.zeros((nf, nl, 3), complex),
+           "spectra": np.zeros((nf, nl, 2)), "pi": np.zeros((n, n, 5, nf, nl), complex)}
+    for l, (q0, th, mode) in enumerate(lines):
+        for f in range(nf):
+            o = omin + f * dw
+            q = GAMMA * o * np.sin(th) if mode else q0
+            pi0 = np.asfortranarray(np.stack([c[f] for c in comps], axis=-1))
+            pi, sums, spec, st = orc.photon_point(q, o, eta, c0, th, par, gz, pi0)
+            assert st == 0
+            out["pi"][:, :, :, f, l] = pi
+            out["pi11"][f, l] = [pi[0, 0, 0], pi[0, 0, 1], pi[0, 0, 4]]
+            out["psum"][f, l] = [sums[0], sums[1], sums[2]]
+            out["spectra"][f, l] = spec
+    return out
+
+
+CASES = [  # (No, Nlf, general Piyz0/Pizy0, lines)
+    (12, 1, False, [(0.0, 0.0, 0), (0.01, 0.0, 0)]),
+    (10, 3, False, [(0.004, 0.0, 0), (0.0, 0.4, 1), (0.0, 1.1, 1)]),
+    (8, 7, True, [(0.02, 0.0, 0)]),
+    (6, 21, False, [(0.01, 0.0, 0), (0.0, 0.7, 1)]),
+    (4, 50, True, [(0.01, 0.3, 0)]),  # TP (100 x 100) past the LDS form
+]
+
+
+@pytest.mark.parametrize("no,n,general,lines", CASES, ids=[f"no{c[0]}-n{c[1]}" for c in CASES])
+def test_photon_matches_oracle(no, n, general, lines):
+    print(f"[photon] no={no} nlf={n} general={general} lines={len(lines)}", flush=True)
+    c0 = 12.0
+    gz, par = _fields(n, c0)
+    comps = _pi0(no, n, seed=no * 10 + n, scale=0.05 / n, general=general)
+    omin, dw, eta = 0.5 / HARTREE, 0.4 / HARTREE, 0.05 / HARTREE
+    got = abi.photon(comps[0], comps[1], comps[4], gz, par, lines, omin, dw, eta, c0,
+                     piyz0=comps[2] if general else None, pizy0=comps[3] if general else None, full=True)
+    want = _oracle(comps, gz, par, lines, omin, dw, eta, c0)
+    for key in ("pi11", "psum", "pi"):
+        assert np.isfinite(got[key]).all(), key
+        assert H.rel_max(got[key], want[key]) < 1e-10, key
+    np.testing.assert_allclose(got["spectra"], want["spectra"], rtol=1e-9, atol=1e-9)
+
+
+def test_photon_vacuum_gives_no_absorption_or_reflection():
+    no, n, c0 = 6, 4, 10.0
+    gz, par = _fields(n, c0)
+    z = np.zeros((no, n, n), dtype=np.complex128)
+    got = abi.photon(z, z, z, gz, par, [(0.01, 0.0, 0), (0.0, 0.5, 1)], 0.02, 0.01, 0.002, c0, full=True)
+    assert np.all(got["pi"] == 0) and np.all(got["spectra"] == 0) and np.all(got["psum"] == 0)
+
+
+# ---------------------------------------------------------------------------
+# the drop-in program
+
+def _floats(line):
+    pat = r"[-+]?(?:\d+\.\d*|\.\d+)(?:[eEdD][-+]?\d+)?|[-+]?\d+[eEdD][-+]?\d+"
+    return [float(x.replace("D", "E").replace("d", "e")) for x in re.findall(pat, line)]
+
+
+def _table(path):
+    return np.array([_floats(ln) for ln in open(path).read().splitlines() if ln.strip()])
+
+
+@pytest.mark.parametrize("mode", ["q", "theta"])
+def test_photon_driver_matches_oracle(tmp_path, mode):
+    exe = os.path.join(build.BIN, "photon_hip.x")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} not built (run __graft_entry__.build())")
+    run = str(tmp_path)
+    s = H.system("tiny")
+    os.makedirs(os.path.join(run, "save"))
+    synth.write_charge_density(os.path.join(run, "save", "charge-density.dat"), s)
+    synth.write_scf_out(os.path.join(run, "scf.out"), s)
+    a0, c0 = s.cfg.a, s.cfg.c
+    gcar = 2 * np.pi / a0
+    ecut = 0.5 * gcar ** 2 * (2.2 * a0 / c0) ** 2  # Gz up to m = 2
+    g = synth.gcart(s.mill_g, s.kc)
+    rc, nlf, glf, par, _ = abi.gen_glf("z", ecut, gcar, g, s.mill_g, 20)
+    assert rc == 0 and nlf >= 3
+    gz = gcar * glf[:, 2]
+    no = 7
+    comps = _pi0(no, nlf, seed=3, scale=0.02 / nlf)
+    omin_ev, omax_ev, eta_ev = 0.5, 6.0, 0.05
+    for pol, arr in (("xx", comps[0]), ("yy", comps[1]), ("zz", comps[4])):
+        with open(os.path.join(run, f"Pi_{pol}"), "w") as f:
+            for io in range(no):
+                for i in range(nlf):
+                    for j in range(nlf):
+                        v = arr[io, i, j]
+                        f.write(f" {io * 0.1:.6f} ({v.real:.17e},{v.imag:.17e})\n")
+    if mode == "q":
+        sysnl = "qmin = 1, qmax = 3, theta_min = 1, theta_max = 1, dq = 0.004"
+    else:
+        sysnl = "qmin = 1, qmax = 1, theta_min = 1, theta_max = 3, dq = 0.004"
+    with open(os.path.join(run, "config.in"), "w") as f:
+        f.write(f"&directories\n rundir = '.', savedir = './save', scf_file = 'scf.out',\n"
+                f" rpa_xx_file = 'Pi_xx', rpa_yy_file = 'Pi_yy', rpa_zz_file = 'Pi_zz'\n/\n"
+                f"&system\n lf = 'z', {sysnl}, omin = {omin_ev}, omax = {omax_ev}, eta = {eta_ev},"
+                f" Ecut = {ecut!r}\n/\n"
+                f"&config\n No = {no}, NG = {len(s.mill_g)}, NGd = 1, NkI = 1, Nlfd = {nlf}\n/\n"
+                f"&parameters\n a0 = {a0!r}, c0 = {c0!r}, Vcell = 1.0\n/\n&parallel\n Nthreads = 1\n/\n")
+    print(f"[photon] driver mode={mode} nlf={nlf} in {run}", flush=True)
+    r = subprocess.run([exe], cwd=run, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    eta = eta_ev / HARTREE
+    omin = omin_ev / HARTREE
+    omax = omax_ev / HARTREE + omin
+    dw = (omax - omin) / (no - 1)
+    if mode == "q":
+        lines = [((iq - 1) * 0.004, 0.0, 0) for iq in (1, 2, 3)]
+        ids = [(iq, 1) for iq in (1, 2, 3)]
+        spec_id = [f"_q#{iq}" for iq in (1, 2, 3)]
+    else:
+        dth = np.pi / (2.0 * 2)
+        lines = [(0.0, (it - 1) * dth, 1 if it != 1 else 0) for it in (1, 2, 3)]
+        ids = [(1, it) for it in (1, 2, 3)]
+        spec_id = [f"_theta=#{it}" for it in (1, 2, 3)]
+    want = _oracle(comps, gz, par, lines, omin, dw, eta, c0)
+    o = omin + np.arange(no - 1) * dw
+    for l, (iq, it) in enumerate(ids):
+        tag = f"_q#{iq}_theta#{it}"
+        for name, k in (("pixx", 0), ("piyy", 1), ("pizz", 2)):
+            t = _table(os.path.join(run, name + tag))
+            np.testing.assert_allclose(t[:, 0], o * HARTREE, rtol=1e-12)
+            assert H.rel_max(t[:, 1] + 1j * t[:, 2], want["pi11"][:, l, k]) < 1e-9, name
+            t = _table(os.path.join(run, name + "_sumGiGj" + tag))
+            assert H.rel_max(t[:, 1] + 1j * t[:, 2], want["psum"][:, l, k]) < 1e-9, name + "_sum"
+        t = _table(os.path.join(run, "sigmaSc_yy" + tag))
+        sig = -1j * 4.0 * c0 * want["pi11"][:, l, 1] / o
+        assert H.rel_max(t[:, 1] + 1j * t[:, 2], sig) < 1e-9
+        t = _table(os.path.join(run, "pizz0" + tag))
+        np.testing.assert_allclose(t[:, 1] + 1j * t[:, 2], comps[4][:no - 1, 0, 0], rtol=1e-15)
+        a = _table(os.path.join(run, "absorption" + spec_id[l]))
+        rf = _table(os.path.join(run, "reflection" + spec_id[l]))
+        tr = _table(os.path.join(run, "transmission" + spec_id[l]))
+        # theta = pi/2 on the last angle line: R_p divides by cos(theta) ~ 6e-17
+        np.testing.assert_allclose(a[:, 1], want["spectra"][:, l, 0], rtol=1e-8, atol=1e-9)
+        np.testing.assert_allclose(rf[:, 1], want["spectra"][:, l, 1], rtol=1e-8, atol=1e-9)
+        np.testing.assert_allclose(tr[:, 1], 1 - a[:, 1] - rf[:, 1], rtol=0, atol=1e-12)
