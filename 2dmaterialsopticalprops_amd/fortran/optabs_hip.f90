@@ -8,6 +8,9 @@
 !             (Optabs.f90:513-614), plus Info_{pol}.out.
 !
 ! Extra namelist entries (absent from the reference, defaults in brackets):
+!   &directories xml_file = ''  QE's data-file-schema.xml (path under rundir):
+!           when given, the operations and reciprocal axes come from it
+!           (Modules/io_xml.f90) instead of scf_file.
 !   &system vertex_map = 'direct' | 'igk' ['direct']
 !   &system precision = 'fp64' | 'mixed'  ['fp64'] arithmetic of the device
 !           chain: 'fp64' throughout, or complex FP32 vertices and BF16 hi/lo
@@ -20,7 +23,8 @@ program optabs_hip
   use rpa_files
   implicit none
 
-  character(len=100) :: rundir = './', savedir = './', scf_file = 'scf.out', band_file = 'bands.dat'
+  character(len=100) :: rundir = './', savedir = './', scf_file = 'scf.out', band_file = 'bands.dat', &
+                        xml_file = ''
   character(len=3) :: lf = 'z', pol = 'xx'
   character(len=8) :: vertex_map = 'direct', precision = 'fp64'
   integer :: calc = 3, jump = 1, qmin = 1, qmax = 1
@@ -30,7 +34,7 @@ program optabs_hip
   real(dp) :: Efermi = 0.0_dp, dGW = 0.0_dp, eps = 1.0e-4_dp, T = 0.0_dp, Ecut = 0.0_dp, &
               a0 = 1.0_dp, c0 = 1.0_dp, Vcell = 1.0_dp
   integer :: Nthreads = 1
-  namelist /directories/ rundir, savedir, scf_file, band_file
+  namelist /directories/ rundir, savedir, scf_file, band_file, xml_file
   namelist /system/ lf, pol, calc, jump, omin, omax, qmin, qmax, eta_intra, eta_inter, Lor_cut, &
                     df_cut, vertex_map, precision
   namelist /config/ NG, NGd, NkI, Nband, Nval, NelQE, no, Nlfd
@@ -48,6 +52,7 @@ program optabs_hip
   complex(dp), allocatable, target :: S0(:, :, :), Qeff(:, :), pi11(:)
   real(dp), allocatable, target :: sig(:, :)
   type(c_ptr) :: ctx
+  type(qe_xml) :: qx
   type(rpa_dims_t) :: dims
   type(rpa_params_t) :: p
 
@@ -80,13 +85,23 @@ program optabs_hip
   omax = omax / hartree + omin
   domega = (omax - omin) / (no - 1)
 
-  call point_r(trim(rundir)//trim(scf_file), nsym, R)
+  if (len_trim(xml_file) > 0) then
+    ! operations and reciprocal axes from data-file-schema.xml (io_xml.f90)
+    call load_xml_qe(trim(rundir)//trim(xml_file), qx)
+    call xml_point_ops(qx, nsym, R)
+  else
+    call point_r(trim(rundir)//trim(scf_file), nsym, R)
+  end if
   RI = 0.0_dp
   call rpa_check(rpa_point_inverse(nsym, R, RI), c_null_ptr, 'PointR')
   allocate(ki(3, NkI), E(NkI, Nband), ktot(3, 48 * NkI))
   call load_ki_and_e(trim(rundir)//trim(band_file), NkI, Nband, Nval, dGW, ki, E)
   call rpa_check(rpa_gen_fbz(NkI, nsym, eps, ki, R, ntot, ktot), c_null_ptr, 'genFBZ')
-  call load_kc(trim(rundir)//trim(scf_file), kc)
+  if (len_trim(xml_file) > 0) then
+    call xml_kc(qx, kc)
+  else
+    call load_kc(trim(rundir)//trim(scf_file), kc)
+  end if
   call load_g(trim(savedir), ngm, mill)
   allocate(g(3, ngm), glf(3, Nlfd), parg(Nlfd), lf_idx(Nlfd))
   call rpa_check(rpa_gcart(ngm, mill, kc, g), c_null_ptr, 'G vectors')

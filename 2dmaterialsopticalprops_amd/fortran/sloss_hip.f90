@@ -7,6 +7,9 @@
 ! spectrum -Im eps^-1_00(q, omega) and the run time of the device chain.
 !
 ! Extra namelist entries (absent from the reference, defaults in brackets):
+!   &directories xml_file = ''  QE's data-file-schema.xml (path under rundir):
+!           when given, the operations and reciprocal axes come from it
+!           (Modules/io_xml.f90) instead of scf_file.
 !   &system vertex_map = 'direct' | 'igk'  ['direct'] coefficient-to-G map:
 !           'direct' is the reference drivers' first-NGd convention
 !           (Sloss.f90:907-946), 'igk' the Modules/vertices.f90 _v2 one.
@@ -25,7 +28,8 @@ program sloss_hip
   use rpa_files
   implicit none
 
-  character(len=100) :: rundir = './', savedir = './', scf_file = 'scf.out', band_file = 'bands.dat'
+  character(len=100) :: rundir = './', savedir = './', scf_file = 'scf.out', band_file = 'bands.dat', &
+                        xml_file = ''
   character(len=3) :: lf = 'z'
   character(len=8) :: vertex_map = 'direct', precision = 'fp64'
   integer :: jump = 1, qmin = 1, qmax = 1
@@ -34,7 +38,7 @@ program sloss_hip
   real(dp) :: Efermi = 0.0_dp, a0 = 1.0_dp, c0 = 1.0_dp, eps = 1.0e-4_dp, T = 0.0_dp, &
               eta = 0.05_dp, Ecut = 0.0_dp, Vcell = 1.0_dp
   integer :: Nthreads = 1
-  namelist /directories/ rundir, savedir, scf_file, band_file
+  namelist /directories/ rundir, savedir, scf_file, band_file, xml_file
   namelist /system/ lf, jump, omin, omax, qmin, qmax, vertex_map, precision
   namelist /config/ NkI, Nband, Nocc, NelQE, NGd, NG, no, nq, Nlfd
   namelist /parameters/ Efermi, a0, c0, eps, T, eta, Ecut, Vcell
@@ -51,6 +55,7 @@ program sloss_hip
   complex(dp), allocatable, target :: evc(:, :), WT(:, :, :)
   real(dp), allocatable, target :: Vt(:, :), losst(:)
   type(c_ptr) :: ctx
+  type(qe_xml) :: qx
   type(rpa_dims_t) :: dims
   type(rpa_params_t) :: p
 
@@ -75,13 +80,23 @@ program sloss_hip
   gcar = 2.0_dp * pi / a0
   domega = (omax - omin) / (no - 1)
 
-  call point_r(trim(rundir)//trim(scf_file), nsym, R)
+  if (len_trim(xml_file) > 0) then
+    ! operations and reciprocal axes from data-file-schema.xml (io_xml.f90)
+    call load_xml_qe(trim(rundir)//trim(xml_file), qx)
+    call xml_point_ops(qx, nsym, R)
+  else
+    call point_r(trim(rundir)//trim(scf_file), nsym, R)
+  end if
   RI = 0.0_dp
   call rpa_check(rpa_point_inverse(nsym, R, RI), c_null_ptr, 'PointR')
   allocate(ki(3, NkI), E(NkI, Nband), ktot(3, 48 * NkI))
   call load_ki_and_e(trim(rundir)//trim(band_file), NkI, Nband, Nocc, 1.0_dp / hartree, ki, E)
   call rpa_check(rpa_gen_fbz(NkI, nsym, eps, ki, R, ntot, ktot), c_null_ptr, 'genFBZ')
-  call load_kc(trim(rundir)//trim(scf_file), kc)
+  if (len_trim(xml_file) > 0) then
+    call xml_kc(qx, kc)
+  else
+    call load_kc(trim(rundir)//trim(scf_file), kc)
+  end if
   call load_g(trim(savedir), ngm, mill)
   if (ngm /= NG) write(*, '(a,i0,a,i0)') 'note: NG = ', NG, ' in config, charge-density.dat has ', ngm
   allocate(g(3, ngm), glf(3, Nlfd), parg(Nlfd), lf_idx(Nlfd))

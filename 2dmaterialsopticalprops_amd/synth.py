@@ -346,6 +346,116 @@ def write_scf_out(path: str, sys: SynthSystem, ops=None) -> None:
         f.write("\n".join(lines) + "\n")
 
 
+def xml_crystal_ops(kc: np.ndarray, R: np.ndarray, nsym: int) -> np.ndarray:
+    """Integer crystal forms s = KC^-1 R KC of cartesian operations: s acts on
+    reciprocal crystal components, the way pw.x rotates k-points."""
+    kci = np.linalg.inv(kc)
+    return np.stack([np.rint(kci @ R[op] @ kc) for op in range(nsym)])
+
+
+def write_data_file_schema(path: str, sys: SynthSystem, ops=None, noinv: bool = True,
+                           force_symmorphic: bool = True, efermi_ha: float = 0.0) -> None:
+    """A pw.x data-file-schema.xml with the elements io_xml reads: the input
+    cell and Monkhorst-Pack grid, symmetry flags and smearing, the output
+    symmetries (each <rotation> as three lines, the columns of s), the
+    reciprocal lattice (the scf.out axes sys.kc), and per IBZ k-point its
+    cartesian k (2pi/alat), npw, eigenvalues (Hartree) and occupations, five
+    values per line."""
+    cfg = sys.cfg
+    kc = sys.kc
+    at = np.linalg.inv(kc).T * cfg.a  # direct lattice vectors (columns), bohr
+    if ops is None:
+        R, nsym = np.eye(3)[None], 1
+    else:
+        R, nsym = np.asarray(ops[0]), int(ops[1])
+    S = xml_crystal_ops(kc, R, nsym)
+    nk, nb = sys.e_ev.shape
+    e_ha = sys.e_ev / HARTREE
+    occ = (e_ha < efermi_ha).astype(float)
+
+    def vec(v):
+        return " ".join(repr(float(x)) for x in v)
+
+    def block(vals):
+        out = []
+        for i in range(0, len(vals), 5):
+            out.append("  " + " ".join(f"{x:.15e}" for x in vals[i:i + 5]))
+        return out
+
+    L = ['<?xml version="1.0" encoding="UTF-8"?>',
+         '<qes:espresso Units="Hartree atomic units">',
+         '  <input>',
+         f'    <atomic_structure nat="2" alat="{cfg.a!r}" bravais_index="4">',
+         '      <cell>']
+    L += [f'        <a{i + 1}>{vec(at[:, i])}</a{i + 1}>' for i in range(3)]
+    L += ['      </cell>',
+          '    </atomic_structure>',
+          '    <basis>',
+          '      <ecutwfc>1.0e1</ecutwfc>',
+          '      <ecutrho>4.0e1</ecutrho>',
+          '    </basis>',
+          '    <bands>',
+          '      <smearing degauss="0.0">gaussian</smearing>',
+          '      <occupations>fixed</occupations>',
+          '    </bands>',
+          '    <k_points_IBZ>',
+          f'      <monkhorst_pack nk1="{cfg.nk}" nk2="{cfg.nk}" nk3="1" k1="0" k2="0" k3="0">Monkhorst-Pack'
+          '</monkhorst_pack>',
+          '    </k_points_IBZ>',
+          '    <symmetry_flags>',
+          '      <nosym>false</nosym>',
+          f'      <noinv>{"true" if noinv else "false"}</noinv>',
+          '      <no_t_rev>false</no_t_rev>',
+          f'      <force_symmorphic>{"true" if force_symmorphic else "false"}</force_symmorphic>',
+          '    </symmetry_flags>',
+          '  </input>',
+          '  <output>',
+          '    <symmetries>',
+          f'      <nsym>{nsym}</nsym>',
+          f'      <nrot>{nsym}</nrot>',
+          '      <space_group>0</space_group>']
+    for op in range(nsym):
+        L += ['      <symmetry>',
+              '        <info name="crystal_symmetry">crystal_symmetry</info>',
+              '        <rotation rank="2" dims="3 3" order="F">']
+        L += ["  " + " ".join(f"{S[op][i, c]:.15e}" for i in range(3)) for c in range(3)]
+        L += ['        </rotation>',
+              '      </symmetry>']
+    L += ['    </symmetries>',
+          '    <basis_set>',
+          '      <ecutwfc>1.0e1</ecutwfc>',
+          '      <ecutrho>4.0e1</ecutrho>',
+          '      <reciprocal_lattice>']
+    L += [f'        <b{j + 1}>{vec(kc[:, j])}</b{j + 1}>' for j in range(3)]
+    L += ['      </reciprocal_lattice>',
+          '    </basis_set>',
+          '    <band_structure>',
+          f'      <nbnd>{nb}</nbnd>',
+          f'      <nelec>{2.0 * cfg.nval!r}</nelec>',
+          f'      <fermi_energy>{efermi_ha!r}</fermi_energy>',
+          '      <starting_k_points>',
+          f'        <monkhorst_pack nk1="{cfg.nk}" nk2="{cfg.nk}" nk3="1" k1="0" k2="0" k3="0">Monkhorst-Pack'
+          '</monkhorst_pack>',
+          '      </starting_k_points>',
+          f'      <nks>{nk}</nks>']
+    for ik in range(nk):
+        L += ['      <ks_energies>',
+              f'        <k_point weight="{2.0 / nk!r}">{vec(sys.kcart[ik])}</k_point>',
+              f'        <npw>{int(sys.ngk[ik])}</npw>',
+              f'        <eigenvalues size="{nb}">']
+        L += block(e_ha[ik])
+        L += ['        </eigenvalues>',
+              f'        <occupations size="{nb}">']
+        L += block(occ[ik])
+        L += ['        </occupations>',
+              '      </ks_energies>']
+    L += ['    </band_structure>',
+          '  </output>',
+          '</qes:espresso>']
+    with open(path, "w") as f:
+        f.write("\n".join(L) + "\n")
+
+
 def write_save(dirpath: str, sys: SynthSystem) -> None:
     os.makedirs(dirpath, exist_ok=True)
     write_charge_density(os.path.join(dirpath, "charge-density.dat"), sys)

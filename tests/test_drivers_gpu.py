@@ -287,3 +287,40 @@ def test_optabs_driver_unfolds_point_operations(tmp_path):
     pi11, si, sa = orc.pi_assembly(Pi0, gq, dw, 0.01 / H.HARTREE, cfg.c)
     pr = _table(run / "Pi_RPA_yy")
     assert H.rel_max(pr[:, 1] + 1j * pr[:, 2], pi11) < 1e-10
+
+
+@pytest.mark.parametrize("program,kind", [("sloss", "c6"), ("optabs", "c3v")])
+def test_drivers_take_operations_and_axes_from_the_xml_file(tmp_path, program, kind):
+    """&directories xml_file: the operations and reciprocal axes come from a
+    data-file-schema.xml (Modules/io_xml.f90) instead of scf.out, whose
+    operation and axes lines are removed.  The same system run both ways:
+    Sloss's loss and W, Optabs' Corrfun/Pi, equal within the difference of
+    the two sources' operations (the xml rebuilds them from integer crystal
+    forms and the 6-decimal axes, scf.out has 7-decimal cartesian rows)."""
+    ibz, R, nsym = _symmetric_pair(kind)
+    cfg = ibz.cfg
+    extra = {"pol": "yy", "calc": 3} if program == "optabs" else None
+    outs = {}
+    for how in ("scf", "xml"):
+        run = tmp_path / how
+        synth.write_all(str(run), ibz, program, extra=extra, ops=(R, nsym))
+        if how == "xml":
+            synth.write_data_file_schema(str(run / "data-file-schema.xml"), ibz, ops=(R, nsym))
+            cfgtxt = open(run / "config.in").read().replace("&directories", "&directories\n xml_file = "
+                                                              "'data-file-schema.xml',", 1)
+            open(run / "config.in", "w").write(cfgtxt)
+            scf = open(run / "scf.out").read().splitlines()
+            keep = [l for l in scf if "reciprocal axes" not in l and " b(" not in l and "Sym. Ops" not in l
+                    and "cryst." not in l and "cart." not in l]
+            open(run / "scf.out", "w").write("\n".join(keep) + "\n")
+        print(f"[drivers] {program} {kind} operations from {how}", flush=True)
+        _run(_exe(f"{program}_hip.x"), run)
+        if program == "sloss":
+            outs[how] = (_table(run / f"Loss_Q{cfg.iq}")[:, 1], _table(run / "fort.10008")[:, 1],
+                         _table(run / "fort.20008")[:, 1])
+        else:
+            outs[how] = (_table(run / "Pi_RPA_yy")[:, 1:3],)
+    # the two operation sets differ by ~1e-6 (the 6-decimal axes in KC s KC^-1):
+    # the unfolded k list, q and the current vertices R(R1) G follow them
+    for a, b in zip(outs["xml"], outs["scf"]):
+        assert H.rel_max(a, b) < 2e-5

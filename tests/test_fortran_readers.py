@@ -146,3 +146,88 @@ def test_band_reader_on_the_reference_band_file(dump_exe, tmp_path):
         kr, er = _read_bands_fortran(tmp_path, REF_BAND, len(kf), 60)
         np.testing.assert_array_equal(kr, kf)
         np.testing.assert_allclose(er, ef, rtol=1e-15, atol=0)
+
+
+@pytest.fixture(scope="module")
+def xml_exe(tmp_path_factory):
+    if not os.path.exists(build.FLANG):
+        pytest.skip("flang not available")
+    build.build_fortran()
+    d = tmp_path_factory.mktemp("fxml")
+    fb = os.path.join(build.BUILD, "fortran")
+    exe = str(d / "dump_xml.x")
+    src = os.path.join(os.path.dirname(__file__), "fortran", "dump_xml.f90")
+    subprocess.run([build.FLANG, "-O1", "-module-dir", fb, "-I", fb, src,
+                    os.path.join(fb, "rpa_files.o"), "-o", exe], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("ops", [None, "c3v", "c6"])
+def test_xml_reader_returns_the_written_schema(xml_exe, tmp_path, ops):
+    """data-file-schema.xml (Modules/io_xml.f90:24-235): counts, cutoffs,
+    lattices (a_i, b_i as rows), alat = a(1,1), MP grid, every <rotation>
+    (the file's lines as columns), k-points, npw, eigenvalues and
+    occupations (five a line, Hartree); the operations in cartesian form
+    equal those the file was written from, and the axes equal scf.out's."""
+    s = H.system("tiny")
+    R_w, nsym_w = synth.hex_point_ops(ops) if ops else (np.eye(3)[None], 1)
+    path = tmp_path / "data-file-schema.xml"
+    ef = 0.1
+    synth.write_data_file_schema(str(path), s, ops=(R_w, nsym_w), efermi_ha=ef)
+    subprocess.run([xml_exe, str(path)], check=True, cwd=tmp_path, capture_output=True)
+    raw = (tmp_path / "dump_xml.bin").read_bytes()
+    off = 0
+
+    def take(dtype, n):
+        nonlocal off
+        a = np.frombuffer(raw, dtype=dtype, count=n, offset=off)
+        off += a.nbytes
+        return a
+
+    nband, nki, nrot, nsym = take(np.int32, 4)
+    nmp = take(np.int32, 3)
+    ecutwfc, ecutrho, alat, temp, efermi, nelec = take(np.float64, 6)
+    a = take(np.float64, 9).reshape(3, 3, order="F")
+    b = take(np.float64, 9).reshape(3, 3, order="F")
+    S = take(np.float64, 9 * nrot).reshape(nrot, 3, 3).transpose(0, 2, 1)  # S[op][i, j] = s(i, j)
+    ki = take(np.float64, 3 * nki).reshape(nki, 3)
+    npw = take(np.int32, nki)
+    eig = take(np.float64, nband * nki).reshape(nki, nband)
+    occ = take(np.float64, nband * nki).reshape(nki, nband)
+    nsym_c = take(np.int32, 1)[0]
+    Rc = take(np.float64, 9 * nsym_c).reshape(nsym_c, 3, 3, order="F")
+    kc = take(np.float64, 9).reshape(3, 3, order="F")
+    assert off == len(raw)
+    nk_w, nb_w = s.e_ev.shape
+    assert (nband, nki, nrot, nsym, nsym_c) == (nb_w, nk_w, nsym_w, nsym_w, nsym_w)
+    assert list(nmp) == [s.cfg.nk, s.cfg.nk, 1]
+    assert (ecutwfc, ecutrho, temp, efermi, nelec) == (10.0, 40.0, 0.0, ef, 2.0 * s.cfg.nval)
+    at = np.linalg.inv(s.kc).T * s.cfg.a
+    np.testing.assert_array_equal(a, at.T)
+    assert alat == at[0, 0]
+    np.testing.assert_array_equal(b, s.kc.T)
+    np.testing.assert_array_equal(kc, s.kc)
+    np.testing.assert_array_equal(S, synth.xml_crystal_ops(s.kc, R_w, nsym_w))
+    # KC s KC^-1 with the 6-decimal axes of the synthetic system (a pw.x file
+    # carries them to full precision)
+    np.testing.assert_allclose(Rc, R_w[:nsym_w], rtol=0, atol=2e-6)
+    kci = np.linalg.inv(s.kc)
+    np.testing.assert_array_equal(np.stack([np.rint(kci @ Rc[op] @ s.kc) for op in range(nsym_w)]),
+                                  synth.xml_crystal_ops(s.kc, R_w, nsym_w))
+    np.testing.assert_array_equal(ki, s.kcart)
+    np.testing.assert_array_equal(npw, s.ngk)
+    np.testing.assert_allclose(eig, s.e_ev / H.HARTREE, rtol=1e-15)
+    np.testing.assert_array_equal(occ, (s.e_ev / H.HARTREE < ef).astype(float))
+
+
+@pytest.mark.parametrize("flag", ["noinv", "force_symmorphic"])
+def test_xml_reader_stops_on_time_reversal_or_fractional_translations(xml_exe, tmp_path, flag):
+    """io_xml.f90:228-231: a run with time reversal or fractional
+    translations is refused."""
+    s = H.system("tiny")
+    path = tmp_path / "data-file-schema.xml"
+    synth.write_data_file_schema(str(path), s, noinv=(flag != "noinv"),
+                                 force_symmorphic=(flag != "force_symmorphic"))
+    r = subprocess.run([xml_exe, str(path)], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode != 0
+    assert ("time reversal" if flag == "noinv" else "fractionary translation") in r.stdout
