@@ -29,7 +29,8 @@ EXPORTS = [
     "rpa_gk_sort", "rpa_gcart", "rpa_set_profiling", "rpa_stage_ms", "rpa_get_result",
     "rpa_load_s0", "rpa_w_kk", "rpa_kk_noalloc", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
     "rpa_batch_result", "rpa_set_symmetry", "rpa_set_q_sym", "rpa_batch_set_sym", "rpa_find_kq_sym",
-    "rpa_point_inverse", "rpa_interpolate", "rpa_photon",
+    "rpa_point_inverse", "rpa_interpolate", "rpa_photon", "rpa_gamma_w", "rpa_gamma_propagator",
+    "rpa_ladder",
 ]
 # enum rpa_result
 RES = {"S0": 0, "QEFF": 1, "CHI0": 2, "V": 3, "EPSINV": 4, "CHI": 5, "W": 6, "LOSS": 7,
@@ -59,6 +60,12 @@ class RpaInterpParams(C.Structure):
 class RpaPhotonParams(C.Structure):
     _fields_ = [("no", C.c_int), ("nlf", C.c_int), ("nline", C.c_int), ("device", C.c_int),
                 ("omin", C.c_double), ("domega", C.c_double), ("eta", C.c_double), ("c0", C.c_double)]
+
+
+class RpaLadderParams(C.Structure):
+    _fields_ = [("nk", C.c_int), ("no", C.c_int), ("ntot", C.c_int), ("device", C.c_int),
+                ("omin", C.c_double), ("domega", C.c_double), ("eta", C.c_double), ("vcell", C.c_double),
+                ("valley", C.c_double)]
 
 
 class RpaError(RuntimeError):
@@ -123,6 +130,9 @@ def lib() -> C.CDLL:
                                       P, P, P, P, P]
         L.rpa_point_inverse.argtypes = [C.c_int, P, P]
         L.rpa_interpolate.argtypes = [C.POINTER(RpaInterpParams), P, P, P, P]
+        L.rpa_ladder.argtypes = [C.POINTER(RpaLadderParams), P, P, P, P, P]
+        L.rpa_gamma_w.argtypes = [P, P, P]
+        L.rpa_gamma_propagator.argtypes = [C.c_int, C.c_int, C.c_double, P, P, P, P, P, C.c_int]
         L.rpa_photon.argtypes = [C.POINTER(RpaPhotonParams), P, P, P, P, P, P, P, P, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -234,6 +244,41 @@ def interpolate(S0, qeff, no_interp, domega, domega_tot, eta_intra, all_pairs=Tr
     if rc != RPA_OK:
         raise RpaError(rc, lib().rpa_last_error(None).decode())
     return out, intra
+
+
+def ladder(fockk, delta, current, no, omin, domega, eta, ntot, vcell, valley, device=0):
+    """Chi_ladd's frequency loop on the device (rpa_ladder): fockk (nk, nk),
+    delta (nk), current (2, nk) -> chi_ladd, chi_para (2, no) complex."""
+    F = _f(fockk, np.complex128)
+    d = _f(delta, np.float64)
+    j = _f(current, np.complex128)
+    nk = F.shape[0]
+    if F.shape != (nk, nk) or d.shape != (nk,) or j.shape != (2, nk):
+        raise ValueError("fockk (nk, nk), delta (nk) and current (2, nk) expected")
+    cl = np.zeros((2, no), dtype=np.complex128, order="F")
+    cp = np.zeros((2, no), dtype=np.complex128, order="F")
+    p = RpaLadderParams(nk, no, ntot, device, omin, domega, eta, vcell, valley)
+    rc = lib().rpa_ladder(C.byref(p), _ptr(F), _ptr(d), _ptr(j), _ptr(cl), _ptr(cp))
+    if rc != RPA_OK:
+        raise RpaError(rc, lib().rpa_last_error(None).decode())
+    return cl, cp
+
+
+def gamma_propagator(D, domega, device=0):
+    """genGammaPropagator on the device for the columns of D (No, nser):
+    returns (Gammap, Gammam) (No-1, nser) complex and (KKS, SKK) (nser)."""
+    Df = _f(D, np.complex128)
+    if Df.ndim == 1:
+        Df = Df.reshape(-1, 1, order="F")
+    no, nser = Df.shape
+    gp = np.zeros((no - 1, nser), dtype=np.complex128, order="F")
+    gm = np.zeros_like(gp)
+    kks = np.zeros(nser)
+    skk = np.zeros(nser)
+    rc = lib().rpa_gamma_propagator(no, nser, domega, _ptr(Df), _ptr(gp), _ptr(gm), _ptr(kks), _ptr(skk), device)
+    if rc != RPA_OK:
+        raise RpaError(rc, lib().rpa_last_error(None).decode())
+    return gp, gm, kks, skk
 
 
 def photon(pixx0, piyy0, pizz0, glfz, parg, lines, omin, domega, eta, c0, piyz0=None, pizy0=None,
@@ -432,6 +477,14 @@ class Context:
         w1, w2 = np.zeros(nf), np.zeros(nf)
         self._c(lib().rpa_w_kk(self.h, _ptr(w1), _ptr(w2)))
         return w1, w2
+
+    def gamma_w(self):
+        """GW propagators (Gammap, Gammam) (no-1, nlf, nlf) of the resident W."""
+        shape = (self.no - 1, self.nlf, self.nlf)
+        gp = np.zeros(shape, dtype=np.complex128, order="F")
+        gm = np.zeros(shape, dtype=np.complex128, order="F")
+        self._c(lib().rpa_gamma_w(self.h, _ptr(gp), _ptr(gm)))
+        return gp, gm
 
     def set_autotune(self, on=True):
         self._c(lib().rpa_set_autotune(self.h, 1 if on else 0))

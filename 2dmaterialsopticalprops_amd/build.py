@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["rpa_api.hip", "k_vertex.hip", "k_vertex_col.hip", "k_accum.hip", "k_kk.hip", "k_dyson.hip", "k_interp.hip", "k_photon.hip",
+SOURCES = ["rpa_api.hip", "k_vertex.hip", "k_vertex_col.hip", "k_accum.hip", "k_kk.hip", "k_dyson.hip", "k_interp.hip", "k_photon.hip", "k_ladder.hip",
            "rpa_host.cpp"]
 
 

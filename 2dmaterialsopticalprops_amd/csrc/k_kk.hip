@@ -288,6 +288,111 @@ int launch_kk_noalloc(rpa_ctx *c, double *chi0_dev) {
   return 0;
 }
 
+// ---- GW propagators Gamma+- (the Kramers-Kronig of a real spectrum S) ----
+// Stencil rows io' = io-1 = 0..no-2 give W1(io), rows no-1+io' give W2(io):
+//   io = 1:  W1 = -sum_{jo>=2} f_jo S(jo), W2 = -W1 (static limit)
+//   io >= 2: W1 = sum_jo c-(io,jo) S(jo), W2 = sum_jo c+(io,jo) S(jo)
+// with c- of the Sloss stencil K2 (genW1W2, Sloss.f90:1062-1171) or of the
+// Optabs stencil K1 (genGammaPropagator, Modules/propagators.f90:35-92,
+// whose chain of ifs gives K1's weights for No >= 4).
+__global__ void k_gamma_stencil(int kind, int no, double dw, double *K) {
+#pragma clang fp contract(off)
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 2LL * (no - 1) * no) return;
+  const int r = (int)(idx / no), jo = (int)(idx - (long long)r * no) + 1;
+  const bool w2 = r >= no - 1;
+  const int io = (w2 ? r - (no - 1) : r) + 1;
+  const double oi = (double)(io - 1) * dw, oj = (double)(jo - 1) * dw;
+  double c;
+  if (io == 1) {
+    c = (jo >= 2) ? -f_static(jo, no, oj, dw) : 0.0;
+    if (w2) c = -c;
+  } else if (!w2) {
+    c = (kind == RPA_KK_SLOSS) ? k2_cminus(io, jo, oi, oj, dw) : k1_cminus(io, jo, no, oi, oj, dw);
+  } else {
+    c = c_plus(jo, no, oi, oj, dw);
+  }
+  K[idx] = c;
+}
+
+// X = [W1; W2] (rows io', series columns) -> Gamma+ = (W1, -pi S(io)),
+// Gamma- = (-W2, 0) as [io'][series] complex
+__global__ void k_gamma_epilogue(int no, int nser, double pi_s, const double *X, const double *S, double *gp,
+                                 double *gm) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)(no - 1) * nser) return;
+  const long long io0 = idx / nser, ser = idx - io0 * nser;
+  const double w1 = X[idx], w2 = X[(size_t)(no - 1) * nser + idx];
+  gp[2 * idx] = w1;
+  gp[2 * idx + 1] = -pi_s * S[idx];
+  gm[2 * idx] = -w2;
+  gm[2 * idx + 1] = 0.0;
+  (void)io0;
+  (void)ser;
+}
+
+// genGammaPropagator's Kramers-Kronig check per series (propagators.f90:97-105):
+// KKS = sum_io f (W1 - W2 - Re D(io))^2, SKK = sum_io f Re D(io)^2, f = domega
+// (half at io = 1 and No-1), io = 1..No-1 in order
+__global__ void k_gamma_check(int no, int nser, double dw, const double *X, const double *dre, double *kks,
+                              double *skk) {
+#pragma clang fp contract(off)
+  const int ser = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ser >= nser) return;
+  double a = 0.0, b = 0.0;
+  for (int io = 1; io <= no - 1; ++io) {
+    const size_t x = (size_t)(io - 1) * nser + ser;
+    const double wind = dre[x];
+    const double windkk = X[x] + (-X[(size_t)(no - 1) * nser + x]);
+    double f = dw;
+    if (io == 1 || io == no - 1) f = 0.5 * dw;
+    a = a + f * (windkk - wind) * (windkk - wind);
+    b = b + f * wind * wind;
+  }
+  kks[ser] = a;
+  skk[ser] = b;
+}
+
+// S(jo, pair) of W for the GW propagators (Sloss.f90:489): -(1/pi) Im W for
+// jo = 1..no-1; S(no) keeps chi0's spectral value S0(no), which the
+// reference does not overwrite
+__global__ void k_gamma_s_of_w(int no, int npair, int ncol_pad, const double *W, const double *s0, double *S) {
+#pragma clang fp contract(off)
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)no * npair) return;
+  const int jo = (int)(idx / npair), pr = (int)(idx - (long long)jo * npair);
+  const double pi = 3.14159265358979323846;
+  S[idx] = (jo < no - 1) ? -((1.0 / pi) * W[2 * ((size_t)jo * npair + pr) + 1]) : s0[(size_t)(no - 1) * ncol_pad + pr];
+}
+
+// stencil, product and epilogue of the Gamma propagators for nser series
+// S[no][nser] on a stream; ws holds 2(no-1)no + 2(no-1)nser doubles
+static hipError_t gamma_series(hipStream_t st, int kind, int no, int nser, double dw, double pi_s, const double *S,
+                               double *ws, double *gp, double *gm, double **x_out) {
+  double *K = ws, *X = ws + 2 * (size_t)(no - 1) * no;
+  const long long nk = 2LL * (no - 1) * no;
+  hipLaunchKernelGGL(k_gamma_stencil, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, st, kind, no, dw, K);
+  dim3 grid((nser + 63) / 64, (2 * (no - 1) + 63) / 64);
+  hipLaunchKernelGGL(k_gemm_f64, grid, dim3(256), 0, st, 2 * (no - 1), nser, no, (const double *)K, S, X);
+  const long long n = (long long)(no - 1) * nser;
+  hipLaunchKernelGGL(k_gamma_epilogue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, no, nser, pi_s,
+                     (const double *)X, S, gp, gm);
+  if (x_out) *x_out = X;
+  return hipGetLastError();
+}
+
+size_t gamma_ws_doubles(int no, int nser) { return 2 * (size_t)(no - 1) * no + 2 * (size_t)(no - 1) * nser; }
+
+int launch_gamma_w(rpa_ctx *c, double *ws, double *S, double *gp, double *gm) {
+  const int no = c->d.no, np = c->nlf * c->nlf;
+  const long long n = (long long)no * np;
+  hipLaunchKernelGGL(k_gamma_s_of_w, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, no, np,
+                     c->ncol_pad, (const double *)c->d_W, (const double *)c->d_s0, S);
+  RPA_HIP(c, gamma_series(c->stream, RPA_KK_SLOSS, no, np, c->p.domega, 3.14159265358979323846, S, ws, gp, gm,
+                          nullptr));
+  return 0;
+}
+
 // Kramers-Kronig self-check of the screened interaction (Sloss.f90:489-534,
 // genW1W2 1062-1171) at (G, G') = (1, 1): the spectral function
 // S_W(io) = -(1/pi) Im W(io,1,1) for io = 1..no-1 (S0(no,1,1) keeps its
@@ -334,3 +439,72 @@ int launch_w_kk(rpa_ctx *c) {
 }
 
 }  // namespace rpa
+
+using namespace rpa;
+
+extern "C" int rpa_gamma_propagator(int no, int nser, double domega, const double *D, double *gammap, double *gammam,
+                                    double *kks, double *skk, int device) {
+  auto err = [](int code, const std::string &m) {
+    set_free_error("rpa_gamma_propagator: " + m);
+    return code;
+  };
+  if (!D || !gammap || !gammam || !kks || !skk) return err(RPA_E_ARG, "null argument");
+  if (no < 4 || nser < 1) return err(RPA_E_ARG, "No must be at least 4 and the series at least one");
+  if (!(domega > 0.0)) return err(RPA_E_ARG, "domega must be positive");
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return err(RPA_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  // S = -(1/pi) Im D and Re D, [jo][series], with propagators.f90's pi
+  const double pi_s = 3.141592654;
+  const size_t nS = (size_t)no * nser, nG = (size_t)(no - 1) * nser;
+  std::vector<double> hs(nS), hr(nS), hg(2 * nG), hm(2 * nG);
+  for (int s = 0; s < nser; ++s)
+    for (int jo = 0; jo < no; ++jo) {
+      const double *d = D + 2 * ((size_t)jo + (size_t)no * s);
+      hs[(size_t)jo * nser + s] = -((1.0 / pi_s) * d[1]);
+      hr[(size_t)jo * nser + s] = d[0];
+    }
+  hipStream_t st = nullptr;
+  double *S = nullptr, *R = nullptr, *ws = nullptr, *gp = nullptr, *gm = nullptr, *a = nullptr, *b = nullptr;
+  auto cleanup = [&]() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (double *p : {S, R, ws, gp, gm, a, b})
+      if (p) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  double *X = nullptr;
+  if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc((void **)&S, sizeof(double) * nS)) != hipSuccess ||
+      (e = hipMalloc((void **)&R, sizeof(double) * nS)) != hipSuccess ||
+      (e = hipMalloc((void **)&ws, sizeof(double) * gamma_ws_doubles(no, nser))) != hipSuccess ||
+      (e = hipMalloc((void **)&gp, sizeof(double) * 2 * nG)) != hipSuccess ||
+      (e = hipMalloc((void **)&gm, sizeof(double) * 2 * nG)) != hipSuccess ||
+      (e = hipMalloc((void **)&a, sizeof(double) * nser)) != hipSuccess ||
+      (e = hipMalloc((void **)&b, sizeof(double) * nser)) != hipSuccess ||
+      (e = hipMemcpyAsync(S, hs.data(), sizeof(double) * nS, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(R, hr.data(), sizeof(double) * nS, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = gamma_series(st, RPA_KK_OPTABS, no, nser, domega, pi_s, S, ws, gp, gm, &X)) != hipSuccess) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(k_gamma_check, dim3((unsigned)((nser + 255) / 256)), dim3(256), 0, st, no, nser, domega,
+                     (const double *)X, (const double *)R, a, b);
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(hg.data(), gp, sizeof(double) * 2 * nG, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(hm.data(), gm, sizeof(double) * 2 * nG, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(kks, a, sizeof(double) * nser, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(skk, b, sizeof(double) * nser, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
+  }
+  cleanup();
+  // [io'][series] -> (No-1, nser) column-major
+  for (int s = 0; s < nser; ++s)
+    for (int i = 0; i < no - 1; ++i)
+      for (int r = 0; r < 2; ++r) {
+        gammap[2 * ((size_t)i + (size_t)(no - 1) * s) + r] = hg[2 * ((size_t)i * nser + s) + r];
+        gammam[2 * ((size_t)i + (size_t)(no - 1) * s) + r] = hm[2 * ((size_t)i * nser + s) + r];
+      }
+  return RPA_OK;
+}
+

@@ -22,6 +22,7 @@
 // thread computes.
 #include <string>
 
+#include "dev_linalg.h"
 #include "rpa_internal.h"
 
 namespace rpa {
@@ -37,46 +38,6 @@ inline size_t photon_lds_bytes(int N, bool global) {
   const size_t M = 2 * (size_t)N;
   return (global ? 0 : sizeof(double2) * M * M) + sizeof(double2) * 17 * N + sizeof(double) * 3 * N +
          sizeof(int) * 2 * N;
-}
-
-__device__ inline double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ inline double2 c_mul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ inline double2 c_scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
-__device__ inline double2 c_conj(double2 a) { return make_double2(a.x, -a.y); }
-// a / b by Smith's method
-__device__ inline double2 c_div(double2 a, double2 b) {
-  if (fabs(b.x) >= fabs(b.y)) {
-    const double r = b.y / b.x, d = b.x + b.y * r;
-    return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
-  }
-  const double r = b.x / b.y, d = b.x * r + b.y;
-  return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
-}
-// principal square root (branch cut on the negative real axis)
-__device__ inline double2 c_sqrt(double2 z) {
-  if (z.x == 0.0 && z.y == 0.0) return make_double2(0.0, z.y);
-  const double t = sqrt(0.5 * (fabs(z.x) + hypot(z.x, z.y)));
-  if (z.x >= 0.0) return make_double2(t, z.y / (2.0 * t));
-  return make_double2(fabs(z.y) / (2.0 * t), copysign(t, z.y));
-}
-__device__ inline double2 c_expi(double x) {  // exp(i x)
-  double s, c;
-  sincos(x, &s, &c);
-  return make_double2(c, s);
-}
-__device__ inline double2 c_exp(double2 z) {
-  const double m = exp(z.x);
-  double s, c;
-  sincos(z.y, &s, &c);
-  return make_double2(m * c, m * s);
-}
-__device__ inline double2 c_sin(double2 z) {  // sin(x + iy) = sin x cosh y + i cos x sinh y
-  double s, c;
-  sincos(z.x, &s, &c);
-  return make_double2(s * cosh(z.y), c * sinh(z.y));
 }
 
 }  // namespace
@@ -96,99 +57,6 @@ struct PhotonArgs {
   double *spec;         // [point][2] A_p, R_p
   int *status;
 };
-
-// In-place Gauss-Jordan inversion of the n x n matrix sa (row-major) with
-// row pivoting on |Re| + |Im| (lowest row on ties); a zero pivot sets the
-// status flag and goes on without a swap.
-template <bool GLOBAL>
-__device__ void gj_invert(double2 *sa, int n, double2 *fcol, int *piv, double *red_v, int *red_i, int *status) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int k = 0; k < n; ++k) {
-    double best = -1.0;
-    int bi = n;
-    for (int i = k + tid; i < n; i += 256) {
-      const double2 z = sa[i * n + k];
-      const double m = fabs(z.x) + fabs(z.y);
-      if (m > best) {
-        best = m;
-        bi = i;
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
-    if (lane == 0) {
-      red_v[wave] = best;
-      red_i[wave] = bi;
-    }
-    __syncthreads();
-    double pv = red_v[0];
-    int p = red_i[0];
-    for (int w = 1; w < 4; ++w)
-      if (red_v[w] > pv || (red_v[w] == pv && red_i[w] < p)) {
-        pv = red_v[w];
-        p = red_i[w];
-      }
-    if (!(pv > 0.0)) {
-      if (tid == 0) atomicOr(status, 1);
-      p = k;
-    }
-    if (tid == 0) piv[k] = p;
-    if (p != k)
-      for (int j = tid; j < n; j += 256) {
-        const double2 t = sa[k * n + j];
-        sa[k * n + j] = sa[p * n + j];
-        sa[p * n + j] = t;
-      }
-    __syncthreads();
-    const double2 inv = c_div(make_double2(1.0, 0.0), sa[k * n + k]);
-    __syncthreads();
-    for (int j = tid; j < n; j += 256) sa[k * n + j] = (j == k) ? inv : c_mul(sa[k * n + j], inv);
-    for (int i = tid; i < n; i += 256) fcol[i] = sa[i * n + k];
-    __syncthreads();
-    for (int i = tid; i < n; i += 256)
-      if (i != k) sa[i * n + k] = make_double2(0.0, 0.0);
-    __syncthreads();
-    for (int x = tid; x < n * n; x += 256) {
-      const int i = x / n, j = x - i * n;
-      if (i != k) sa[x] = c_sub(sa[x], c_mul(fcol[i], sa[k * n + j]));
-    }
-    __syncthreads();
-  }
-  for (int k = n - 1; k >= 0; --k) {  // row interchanges undone as column interchanges
-    const int p = piv[k];
-    if (p != k)
-      for (int i = tid; i < n; i += 256) {
-        const double2 t = sa[i * n + k];
-        sa[i * n + k] = sa[i * n + p];
-        sa[i * n + p] = t;
-      }
-    __syncthreads();
-  }
-}
-
-// NV doubles summed over the workgroup into red[4 NV ..] (thread 0 reads them)
-template <int NV>
-__device__ inline void block_sum(const double (&v)[NV], double *red) {
-  constexpr int nv = NV;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-  for (int e = 0; e < nv; ++e) {
-    double s = v[e];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) red[wave * nv + e] = s;
-  }
-  __syncthreads();
-  if (tid < nv) red[4 * nv + tid] = red[tid] + red[nv + tid] + red[2 * nv + tid] + red[3 * nv + tid];
-  __syncthreads();
-}
 
 template <bool GLOBAL>
 __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {

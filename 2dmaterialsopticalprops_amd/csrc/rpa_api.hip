@@ -991,6 +991,34 @@ int rpa_w_kk(rpa_ctx *c, double *w1, double *w2) {
   return d2h(c, w2, (const double *)c->d_wkk + nf, nf);
 }
 
+int rpa_gamma_w(rpa_ctx *c, double *gammap, double *gammam) {
+  if (!c || !gammap || !gammam) return fail(c, RPA_E_ARG, "rpa_gamma_w: null argument");
+  if (c->d.program != RPA_SLOSS) return fail(c, RPA_E_STATE, "W belongs to the Sloss path");
+  if (!c->d_W) return fail(c, RPA_E_STATE, "rpa_run or rpa_dyson_w must precede rpa_gamma_w");
+  (void)hipSetDevice(c->d.device);
+  const int no = c->d.no, np = c->nlf * c->nlf;
+  const size_t nG = (size_t)(no - 1) * np;
+  double *ws = nullptr, *S = nullptr, *gp = nullptr, *gm = nullptr;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    for (double *p : {ws, S, gp, gm})
+      if (p) (void)hipFree(p);
+  };
+  hipError_t e;
+  if ((e = hipMalloc((void **)&ws, sizeof(double) * gamma_ws_doubles(no, np))) != hipSuccess ||
+      (e = hipMalloc((void **)&S, sizeof(double) * (size_t)no * np)) != hipSuccess ||
+      (e = hipMalloc((void **)&gp, sizeof(double) * 2 * nG)) != hipSuccess ||
+      (e = hipMalloc((void **)&gm, sizeof(double) * 2 * nG)) != hipSuccess) {
+    release();
+    return hip_fail(c, e, "hipMalloc");
+  }
+  int r = launch_gamma_w(c, ws, S, gp, gm);
+  if (!r) r = copy_freq_mats(c, gp, gammap);
+  if (!r) r = copy_freq_mats(c, gm, gammam);
+  release();
+  return r;
+}
+
 int rpa_set_profiling(rpa_ctx *c, int on) {
   if (!c) return RPA_E_ARG;
   (void)hipSetDevice(c->d.device);

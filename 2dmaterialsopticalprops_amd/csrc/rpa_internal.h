@@ -383,6 +383,10 @@ int launch_kk_noalloc(rpa_ctx *c, double *chi0_dev);
 // [no-1][nlf*nlf] complex.
 hipError_t kk_optabs_columns(hipStream_t s, int no, int nlf, int ncol, double dw, const double *s0, double *kmat,
                              double *x, double *chi0);
+// GW propagators of the resident W (all pairs): ws holds gamma_ws_doubles,
+// S no * nlf^2 doubles, gp / gm [no-1][nlf^2] complex.
+size_t gamma_ws_doubles(int no, int nser);
+int launch_gamma_w(rpa_ctx *c, double *ws, double *S, double *gp, double *gm);
 // The message of the last failed call that takes no context (this thread).
 void set_free_error(const std::string &msg);
 

@@ -504,9 +504,17 @@ def main():
                     help="time interpolate.x (rpa_interpolate) at No=2001, Nlf=65, No_interp=10 instead")
     ap.add_argument("--photon", action="store_true",
                     help="time Photon.x (rpa_photon) at No=2001, Nlf=33, 10 q lines instead")
+    ap.add_argument("--gamma", action="store_true",
+                    help="time the GW propagators (rpa_gamma_propagator): 701 omega, 26010 series")
+    ap.add_argument("--ladder", action="store_true",
+                    help="time the ladder frequency loop (rpa_ladder): 512 k, 401 omega")
     args = ap.parse_args()
     if args.interpolate:
         return bench_interpolate(args)
+    if args.gamma:
+        return bench_gamma(args)
+    if args.ladder:
+        return bench_ladder(args)
     if args.photon:
         return bench_photon(args)
     if args.gpus != 1:
@@ -698,6 +706,74 @@ def bench_photon(args):
         res["cpu_baseline"] = {"value": round(npts / t, 1), "unit": "points/s", "cores": 1, "kind": "port",
                                "sample": f"oracle on {len(pts)} of {npts} points, scaled"}
         res["max_rel_err_vs_oracle_sample_sPiyy"] = float(worst)
+    print(json.dumps(res))
+
+
+def bench_gamma(args):
+    """GW propagators (Modules/propagators.f90 genGammaPropagator) for the
+    screened interaction of ten q at the default workload's shape: 701
+    frequencies, N_G = 51, so 10 x 51^2 = 26010 series per call."""
+    abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
+    no, nser, dw = 701, 26010, (30.0 / (2.0 * 13.6056923)) / 700
+    rng = np.random.default_rng(4)
+    D = np.asfortranarray(rng.standard_normal((no, nser)) + 1j * rng.standard_normal((no, nser)))
+    for _ in range(args.warmup):
+        abi.gamma_propagator(D, dw)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gp, gm, kks, skk = abi.gamma_propagator(D, dw)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    res = {"metric": "GW propagator series per second (host in, host out)", "value": round(nser / (ms * 1e-3), 1),
+           "unit": "series/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"genGammaPropagator: No={no}, {nser} series (10 q x 51^2 pairs)"}}
+    if not args.no_cpu:
+        orc = importlib.import_module("oracle.oracle")
+        ns = 100
+        t0 = time.perf_counter()
+        worst = 0.0
+        for s in range(ns):
+            wp, _, _, _ = orc.gamma_propagator(D[:, s], dw)
+            worst = max(worst, float(np.abs(gp[:, s] - wp).max() / np.abs(wp).max()))
+        t = (time.perf_counter() - t0) * nser / ns
+        res["cpu_baseline"] = {"value": round(nser / t, 1), "unit": "series/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle on {ns} of {nser} series, scaled"}
+        res["max_rel_err_vs_oracle_sample"] = worst
+    print(json.dumps(res))
+
+
+def bench_ladder(args):
+    """The ladder frequency loop of Chi_ladd (Chi_ladd_up/BSE.f90:1116-1176):
+    a 512 x 512 Fock kernel, 401 frequencies (the reference's no), one
+    inversion of I - L0 F / NV per frequency."""
+    abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
+    nk, no, ntot, vcell, valley = 512, 401, 2601, 922.0586, 2.0
+    omin, dw, eta = 0.0, (4.0 / (2.0 * 13.6056923)) / 400, 0.05 / (2.0 * 13.6056923)
+    nv = ntot * vcell
+    rng = np.random.default_rng(5)
+    F = np.asfortranarray(0.3 * nv / np.sqrt(nk) * 0.05 * (rng.standard_normal((nk, nk))
+                                                           + 1j * rng.standard_normal((nk, nk))))
+    delta = rng.uniform(0.02, 0.1, nk)
+    cur = np.asfortranarray(rng.standard_normal((2, nk)) + 1j * rng.standard_normal((2, nk)))
+    for _ in range(args.warmup):
+        abi.ladder(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cl, cp = abi.ladder(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley)
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    res = {"metric": "ladder frequencies per second (host in, host out)", "value": round(no / (ms * 1e-3), 2),
+           "unit": "omega/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"Chi_ladd frequency loop: nkbz={nk}, no={no}"}}
+    if not args.no_cpu:
+        orc = importlib.import_module("oracle.oracle")
+        nsmp = 2
+        t0 = time.perf_counter()
+        wl, _ = orc.ladder(F, delta, cur, nsmp, omin, dw, eta, ntot, vcell, valley)
+        t = (time.perf_counter() - t0) * no / nsmp
+        res["cpu_baseline"] = {"value": round(no / t, 3), "unit": "omega/s", "cores": 1, "kind": "port",
+                               "sample": f"oracle on {nsmp} of {no} frequencies, scaled"}
+        res["max_rel_err_vs_oracle_sample"] = float(np.abs(cl[:, :nsmp] - wl).max() / np.abs(wl).max())
     print(json.dumps(res))
 
 

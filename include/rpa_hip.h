@@ -304,6 +304,53 @@ int rpa_gk_sort(const double *k, int ngm, const double *g, double gcutw, int npw
 /* G(3,n) = KC * mill (Optabs_qe6.f90:1162-1167). */
 int rpa_gcart(int n, const int *mill, const double *kc, double *g);
 
+/* ---- GW propagators ---------------------------------------------------------- */
+/* The GW correlation propagators of the screened interaction for every
+ * (G, G') pair (Sloss.f90:489-512, genW1W2 1062-1171): with S the spectral
+ * function -(1/pi) Im WT(io,G,G') for io = 1..no-1 (S(no) keeps chi0's
+ * S0(no), which the reference does not overwrite),
+ *   Gammap(io,G,G') = (W1, -pi S(io)),  Gammam(io,G,G') = (-W2, 0),
+ * W1 / W2 the K2 Kramers-Kronig sums of S.  Outputs (no-1,Nlf,Nlf) complex.
+ * Needs the resident W of rpa_run / rpa_dyson_w (Sloss).  The reference
+ * forms these matrices per frequency and keeps only (1,1) (rpa_w_kk).       */
+int rpa_gamma_w(rpa_ctx *ctx, double *gammap, double *gammam);
+
+/* genGammaPropagator (Modules/propagators.f90:11-108) for nser independent
+ * series D(No, nser) complex: S = -(1/pi) Im D with the module's
+ * pi = 3.141592654, Gammap(io) = (W1, -pi S(io)), Gammam(io) = (-W2, 0) for
+ * io = 1..No-1 (K1 weights), and the Kramers-Kronig check sums
+ * kks = sum f (Re Gammap + Re Gammam - Re D)^2, skk = sum f (Re D)^2 per
+ * series.  Outputs gammap, gammam (No-1, nser) complex, kks, skk (nser).
+ * No context; rpa_last_error(NULL) gives the message.                      */
+int rpa_gamma_propagator(int no, int nser, double domega, const double *D, double *gammap, double *gammam,
+                         double *kks, double *skk, int device);
+
+/* ---- ladder (BSE) current-current response --------------------------------- */
+typedef struct rpa_ladder_params {
+  int nk;         /* nkbz: k-points of the Fock kernel (<= 1024)               */
+  int no;         /* frequencies io = 1..no                                     */
+  int ntot;       /* Ntot (the kernel and currents are normalised by Ntot Vcell) */
+  int device;     /* HIP device                                                  */
+  double omin;    /* Hartree; o = omin + (io-1) domega                           */
+  double domega;  /* Hartree                                                     */
+  double eta;     /* Hartree                                                     */
+  double vcell;   /* a.u.^3                                                      */
+  double valley;  /* valley factor                                               */
+} rpa_ladder_params;
+
+/* The frequency loop of Chi_ladd (Chi_ladd_up/BSE.f90:1116-1176): for
+ * io = 1..no, L0(k) = 1/(o + Delta(k) + i eta), fock = I - L0 FockK/(Ntot
+ * Vcell) inverted (gjel), lladd = fock^-1 L0 FockK L0, and
+ *   chi_ladd(pol,io) = -valley sum j(pol,k) lladd(k,k') conj(j(pol,k'))/(Ntot Vcell)^2
+ *   chi_para(pol,io) =  valley sum j(pol,k) (o/Delta(k)) L0(k) conj(j(pol,k))/(Ntot Vcell)
+ * Inputs: fockk(nk,nk) complex (the FockK file), delta(nk), current(2,nk)
+ * complex.  Outputs chi_ladd(2,no), chi_para(2,no) complex.  FP64 throughout
+ * (the reference keeps FockK, lladd, the currents and the sums in COMPLEX*8).
+ * A zero pivot returns RPA_E_SINGULAR after the run.  No context:
+ * rpa_last_error(NULL) gives the message.                                   */
+int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const double *delta, const double *current,
+               double *chi_ladd, double *chi_para);
+
 /* ---- interpolate.x (Chi_RPA/interpolate.f90) ----------------------------- */
 typedef struct rpa_interp_params {
   int no;            /* No: S0 on -No..No, Pi_tot on io = 1..No-1              */

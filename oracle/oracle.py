@@ -75,6 +75,10 @@ def lib():
         L.orc_dyson_w.argtypes = [C.c_int, P_CPLX, P_DBL, P_CPLX, P_CPLX, P_CPLX]
         L.orc_pi_assembly.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       P_CPLX, P_CPLX, P_CPLX, P_DBL, P_DBL]
+        L.orc_ladder.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                                 C.c_double, P_CPLX, P_DBL, P_CPLX, P_CPLX, P_CPLX]
+        L.orc_gamma_propagator.argtypes = [C.c_int, C.c_double, P_CPLX, P_CPLX, P_CPLX,
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.orc_lu_invert.argtypes = [P_CPLX, C.c_int]
         L.orc_photon_d0.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, P_INT,
                                     P_DBL, P_CPLX]
@@ -448,3 +452,26 @@ def photon_point(q, o, eta, c0, theta, parg, glfz, pi0):
     st = lib().orc_photon_point(q, o, eta, c0, theta, n, F(parg, np.int32), F(glfz, np.float64),
                                 F(pi0, np.complex128), pi, sums, spec)
     return pi, sums, spec, st
+
+
+def gamma_propagator(D, domega):
+    """genGammaPropagator (Modules/propagators.f90:11-108) of one series D(No):
+    (Gammap (No-1), Gammam (No-1), KKS, SKK)."""
+    D = F(D, np.complex128)
+    no = len(D)
+    gp = np.zeros(no - 1, dtype=np.complex128, order="F")
+    gm = np.zeros(no - 1, dtype=np.complex128, order="F")
+    a, b = C.c_double(), C.c_double()
+    lib().orc_gamma_propagator(no, domega, D, gp, gm, C.byref(a), C.byref(b))
+    return gp, gm, a.value, b.value
+
+
+def ladder(fockk, delta, current, no, omin, domega, eta, ntot, vcell, valley):
+    """Chi_ladd's frequency loop (Chi_ladd_up/BSE.f90:1116-1176):
+    (chi_ladd, chi_para) (2, no) complex."""
+    nk = len(delta)
+    cl = np.zeros((2, no), dtype=np.complex128, order="F")
+    cp = np.zeros((2, no), dtype=np.complex128, order="F")
+    lib().orc_ladder(nk, no, omin, domega, eta, ntot, vcell, valley, F(fockk, np.complex128),
+                     F(delta, np.float64), F(current, np.complex128), cl, cp)
+    return cl, cp

@@ -1272,6 +1272,134 @@ void orc_dyson_w(int nlf, const orc_z *chi0, const double *V, orc_z *epsinv,
   free(c);
 }
 
+/* f4: the ladder frequency loop of Chi_ladd, Chi_ladd_up/BSE.f90:1116-1176,
+ * in its statement order (fock and xi0, gjel, lladd = fock^-1 xi0, the two
+ * closures); the reference's COMPLEX*8 arrays are FP64 here.              */
+void orc_ladder(int nk, int no, double omin, double domega, double eta, int ntot, double vcell,
+                double valley, const orc_z *fockk, const double *delta, const orc_z *current,
+                orc_z *chi_ladd, orc_z *chi_para) {
+  const size_t nn = (size_t)nk * nk;
+  orc_z *fock = (orc_z *)malloc(sizeof(orc_z) * nn);
+  orc_z *xi0 = (orc_z *)malloc(sizeof(orc_z) * nn);
+  orc_z *lladd = (orc_z *)malloc(sizeof(orc_z) * nn);
+  orc_z *unit = (orc_z *)malloc(sizeof(orc_z) * nn);
+#define M_(A, i, j) (A)[(size_t)(i) + (size_t)nk * (j)]
+  for (int io = 1; io <= no; ++io) {
+    double o = omin + (io - 1) * domega;
+    for (int i = 0; i < nk; ++i)
+      for (int j = 0; j < nk; ++j) M_(unit, i, j) = (i == j) ? 1.0 : 0.0;
+    for (int i = 0; i < nk; ++i) {
+      orc_z l0i = 1.0 / (o + delta[i] + I * eta);
+      for (int j = 0; j < nk; ++j) {
+        orc_z l0j = 1.0 / (o + delta[j] + I * eta);
+        M_(fock, i, j) = M_(unit, i, j) - l0i * M_(fockk, i, j) / (ntot * vcell);
+        M_(xi0, i, j) = l0i * M_(fockk, i, j) * l0j;
+      }
+    }
+    orc_gjel(fock, nk, nk, unit, nk, nk);
+    for (int i = 0; i < nk; ++i)
+      for (int j = 0; j < nk; ++j) {
+        orc_z s = 0.0;
+        for (int k = 0; k < nk; ++k) s += M_(fock, i, k) * M_(xi0, k, j);
+        M_(lladd, i, j) = s;
+      }
+    for (int pol = 0; pol < 2; ++pol) {
+      orc_z cl = 0.0, cp = 0.0;
+      for (int i = 0; i < nk; ++i) {
+        orc_z l0i = (o / delta[i]) * (1.0 / (o + delta[i] + I * eta));
+        orc_z j1 = current[pol + 2 * (size_t)i];
+        for (int j = 0; j < nk; ++j) {
+          orc_z j2 = conj(current[pol + 2 * (size_t)j]);
+          cl = cl - valley * j1 * M_(lladd, i, j) * j2 / (ntot * vcell * ntot * vcell);
+        }
+        cp = cp + valley * j1 * l0i * conj(j1) / (ntot * vcell);
+      }
+      chi_ladd[pol + 2 * (size_t)(io - 1)] = cl;
+      chi_para[pol + 2 * (size_t)(io - 1)] = cp;
+    }
+  }
+#undef M_
+  free(fock);
+  free(xi0);
+  free(lladd);
+  free(unit);
+}
+
+/* f4: genGammaPropagator, Modules/propagators.f90:11-108, its statement
+ * order: the weights as the module's chain of plain ifs sets them (later
+ * ifs override earlier ones), pi = 3.141592654 as the module declares it. */
+void orc_gamma_propagator(int no, double domega, const orc_z *D, orc_z *gp, orc_z *gm,
+                          double *kks, double *skk) {
+  const double pi = 3.141592654;
+  double *S = (double *)malloc(sizeof(double) * (size_t)no);
+  for (int j = 0; j < no; ++j) S[j] = -(1.0 / pi) * cimag(D[j]);
+  double KKS = 0.0, SKK = 0.0, fact = 0.0;
+  for (int io = 1; io <= no - 1; ++io) {
+    double oi = (io - 1) * domega, W1 = 0.0, W2 = 0.0;
+    if (io == 1) {
+      for (int jo = 2; jo <= no; ++jo) {
+        double oj = (jo - 1) * domega;
+        fact = domega / oj;
+        if (jo == 2) fact = 3.0 / 2.0;
+        if (jo == no) fact = 0.5 * domega / oj;
+        W1 = W1 - fact * S[jo - 1];
+      }
+      W2 = -W1;
+    } else if (io == 2) {
+      for (int jo = 1; jo <= no; ++jo) {
+        double oj = (jo - 1) * domega;
+        if (jo != io) fact = domega / (oi - oj);
+        if (jo == 1) fact = 1.0;
+        if (jo == 2) fact = 0.0;
+        if (jo == 3) fact = -3.0 / 2.0;
+        if (jo == no) fact = 0.5 * domega / (oi - oj);
+        W1 = W1 + fact * S[jo - 1];
+        fact = domega / (oi + oj);
+        if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+        W2 = W2 + fact * S[jo - 1];
+      }
+    } else if (io == no - 1) {
+      for (int jo = 1; jo <= no; ++jo) {
+        double oj = (jo - 1) * domega;
+        if (jo != io) fact = domega / (oi - oj);
+        if (jo == 1) fact = 0.5 * domega / (oi - oj);
+        if (jo == no - 2) fact = 3.0 / 2.0;
+        if (jo == no - 1) fact = 0.0;
+        if (jo == no) fact = -1.0;
+        W1 = W1 + fact * S[jo - 1];
+        fact = domega / (oi + oj);
+        if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+        W2 = W2 + fact * S[jo - 1];
+      }
+    } else {
+      for (int jo = 1; jo <= no; ++jo) {
+        double oj = (jo - 1) * domega;
+        if (jo != io) fact = domega / (oi - oj);
+        if (jo == 1) fact = 0.5 * domega / (oi - oj);
+        if (jo == io - 1) fact = 3.0 / 2.0;
+        if (jo == io) fact = 0.0;
+        if (jo == io + 1) fact = -3.0 / 2.0;
+        if (jo == no) fact = 0.5 * domega / (oi - oj);
+        W1 = W1 + fact * S[jo - 1];
+        fact = domega / (oi + oj);
+        if (jo == 1 || jo == no) fact = 0.5 * domega / (oi + oj);
+        W2 = W2 + fact * S[jo - 1];
+      }
+    }
+    gp[io - 1] = W1 + I * (-pi * S[io - 1]);
+    gm[io - 1] = -W2 + I * 0.0;
+    double wind = creal(D[io - 1]);
+    double windkk = creal(gp[io - 1]) + creal(gm[io - 1]);
+    fact = domega;
+    if (io == 1 || io == no - 1) fact = 0.5 * domega;
+    KKS = KKS + fact * (windkk - wind) * (windkk - wind);
+    SKK = SKK + fact * wind * wind;
+  }
+  *kks = KKS;
+  *skk = SKK;
+  free(S);
+}
+
 /* f3: Photon.x.  zgetrf (partial pivoting on |Re| + |Im|, the first
  * maximum) then the inverse column by column from the LU factors: the same
  * matrix zgetri returns, in another operation order.                       */

@@ -132,6 +132,19 @@ void orc_pi_assembly(int no, int nlf, double domega, double eta_intra, double c0
                      const orc_z *Pi0, const orc_z *Qeff, orc_z *pi11,
                      double *sig_inter, double *sig_intra);
 
+/* ---- f4: ladder (BSE) frequency loop, Chi_ladd_up/BSE.f90:1116-1176 ---------- */
+/* fockk(nk,nk), current(2,nk) complex, delta(nk) -> chi_ladd(2,no),
+ * chi_para(2,no) complex; FP64 throughout, fock inverted by orc_gjel.      */
+void orc_ladder(int nk, int no, double omin, double domega, double eta, int ntot, double vcell,
+                double valley, const orc_z *fockk, const double *delta, const orc_z *current,
+                orc_z *chi_ladd, orc_z *chi_para);
+
+/* ---- f4: GW propagators ------------------------------------------------------ */
+/* genGammaPropagator (Modules/propagators.f90:11-108) for one series D(No):
+ * gp, gm (No-1) complex, *kks, *skk the Kramers-Kronig check sums.        */
+void orc_gamma_propagator(int no, double domega, const orc_z *D, orc_z *gp, orc_z *gm,
+                          double *kks, double *skk);
+
 /* ---- f3: Photon.x per (line, frequency) point ----------------------------- */
 /* LU with partial pivoting and the inverse from it (zgetrf + zgetri, the
  * reference's invert, BSE/Photonew.f90:388-462); a is n x n column-major,

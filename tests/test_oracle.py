@@ -846,3 +846,86 @@ def test_photon_point_solves_the_dyson_equation_and_sums_the_slab_integral(n):
         assert abs(sums[m] - want) < 1e-11 * max(1.0, abs(want)), (m, sums[m], want)
     _, _, spec0, _ = orc.photon_point(q, o, eta, c0, theta, par, gz, np.zeros_like(pi0))
     assert np.all(spec0 == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# GW propagators (Modules/propagators.f90)
+
+@pytest.mark.parametrize("no", [5, 6, 9, 17, 40])
+def test_gamma_propagator_weights_are_the_pinned_k1_stencil(no):
+    """genGammaPropagator's W1 and W2 are the K1 sums (the module's chain of
+    plain ifs): a real spectrum S with S(1) = 0 placed on the positive
+    branch of an Optabs S0(-no:no) gives Re K1[S0] = W1, on the negative
+    branch Re K1[S0] = W2 (orc_kk_optabs is pinned by the reference's
+    golden K1 tables); Im Gamma+ = -pi S(io) with the module's pi =
+    3.141592654, Gamma- real."""
+    rng = np.random.default_rng(no)
+    dw = 0.037
+    S = rng.standard_normal(no)
+    S[0] = 0.0
+    D = -3.141592654 * S * 1j + rng.standard_normal(no)  # Im D = -pi S
+    gp, gm, _, _ = orc.gamma_propagator(D, dw)
+    np.testing.assert_allclose(-(1.0 / 3.141592654) * D.imag, S, rtol=1e-15)
+    pos = np.zeros((2 * no + 1, 1, 1), dtype=np.complex128)
+    neg = np.zeros_like(pos)
+    for jo in range(1, no + 1):
+        pos[no + jo - 1, 0, 0] = S[jo - 1]
+        neg[no - (jo - 1), 0, 0] = S[jo - 1]
+    w1 = orc.kk_optabs(pos, dw)[:, 0, 0].real
+    w2 = orc.kk_optabs(neg, dw)[:, 0, 0].real
+    scale = np.abs(S).max() * no
+    np.testing.assert_allclose(gp.real, w1, rtol=0, atol=1e-13 * scale)
+    np.testing.assert_allclose(gm.real, -w2, rtol=0, atol=1e-13 * scale)
+    np.testing.assert_array_equal(gp.imag, -3.141592654 * (-(1.0 / 3.141592654) * D.imag[:no - 1]))
+    assert np.all(gm.imag == 0.0)
+
+
+def test_gamma_propagator_reconstructs_a_causal_response():
+    """For a retarded D(w) = 1/(w - w0 + i g) - 1/(w + w0 + i g) on a fine
+    grid, Re Gamma+ + Re Gamma- = W1 - W2 reconstructs Re D (the module's own
+    Kramers-Kronig check: KKS/SKK small), and Re Gamma+(0) = Re D(0)/2 (the
+    static row, W2 = -W1); a sign error in either sum breaks both."""
+    no, dw, w0, g = 2001, 0.002, 1.0, 0.05
+    w = np.arange(no) * dw
+    D = 1.0 / (w - w0 + 1j * g) - 1.0 / (w + w0 + 1j * g)
+    gp, gm, kks, skk = orc.gamma_propagator(D, dw)
+    assert kks / skk < 1e-3
+    rec = gp.real + gm.real
+    assert np.abs(rec - D.real[:no - 1]).max() < 0.02 * np.abs(D.real).max()
+    assert gp[0].real == pytest.approx(D[0].real / 2, rel=1e-2)
+
+
+# ---------------------------------------------------------------------------
+# ladder (BSE) frequency loop (Chi_ladd_up/BSE.f90:1116-1176)
+
+def test_ladder_is_the_summed_neumann_series():
+    """lladd = (I - L0 F/NV)^-1 L0 F L0 is the ladder series
+    sum_m (L0 F / NV)^m L0 F L0: chi_ladd from the oracle's gjel inversion
+    equals the series summed with numpy for a non-symmetric kernel (a
+    transposed kernel or L0 on the wrong side fails it); F = 0 gives no
+    ladder term and the bubble sum_k |j|^2 (o/Delta) L0 valley / NV."""
+    rng = np.random.default_rng(7)
+    nk, no, ntot, vcell, valley = 9, 5, 16, 2.5, 2.0
+    omin, dw, eta = 0.1, 0.03, 0.02
+    nv = ntot * vcell
+    F = 0.05 * nv * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk)))
+    delta = rng.uniform(0.5, 1.5, nk)
+    cur = rng.standard_normal((2, nk)) + 1j * rng.standard_normal((2, nk))
+    cl, cp = orc.ladder(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley)
+    for io in range(no):
+        o = omin + io * dw
+        L0 = np.diag(1.0 / (o + delta + 1j * eta))
+        A = L0 @ F / nv
+        term = L0 @ F @ L0
+        lad = np.zeros_like(term)
+        for _ in range(80):
+            lad += term
+            term = A @ term
+        for pol in range(2):
+            want = -valley * cur[pol] @ lad @ cur[pol].conj() / nv ** 2
+            assert abs(cl[pol, io] - want) < 1e-12 * abs(want)
+            bub = valley * np.sum(np.abs(cur[pol]) ** 2 * (o / delta) / (o + delta + 1j * eta)) / nv
+            assert abs(cp[pol, io] - bub) < 1e-13 * abs(bub)
+    cl0, cp0 = orc.ladder(np.zeros((nk, nk)), delta, cur, no, omin, dw, eta, ntot, vcell, valley)
+    assert np.all(cl0 == 0.0)
+    np.testing.assert_allclose(cp0, cp, rtol=1e-15)
