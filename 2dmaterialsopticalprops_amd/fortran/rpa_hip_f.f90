@@ -28,6 +28,11 @@ module rpa_hip
     real(c_double) :: eps, gcar, domega, eta, lor_cut, df_cut, efermi, temp, vcell, c0, eta_intra
   end type rpa_params_t
 
+  type, bind(C) :: rpa_ladder_params_t
+    integer(c_int) :: nk, no, ntot, device
+    real(c_double) :: omin, domega, eta, vcell, valley
+  end type rpa_ladder_params_t
+
   type, bind(C) :: rpa_photon_params_t
     integer(c_int) :: no, nlf, nline, device
     real(c_double) :: omin, domega, eta, c0
@@ -200,6 +205,24 @@ module rpa_hip
       import :: c_int, c_ptr, rpa_interp_params_t
       type(rpa_interp_params_t), intent(in) :: p
       type(c_ptr), value :: S0, qeff, pi_dense, pi_intra
+    end function
+    integer(c_int) function rpa_gamma_w(ctx, gammap, gammam) bind(C, name="rpa_gamma_w")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, gammap, gammam
+    end function
+    integer(c_int) function rpa_gamma_propagator(no, nser, domega, D, gammap, gammam, kks, skk, device) &
+        bind(C, name="rpa_gamma_propagator")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: no, nser, device
+      real(c_double), value :: domega
+      type(c_ptr), value :: D, gammap, gammam
+      real(c_double), intent(out) :: kks(*), skk(*)
+    end function
+    integer(c_int) function rpa_ladder(p, fockk, delta, current, chi_ladd, chi_para) bind(C, name="rpa_ladder")
+      import :: c_int, c_double, c_ptr, rpa_ladder_params_t
+      type(rpa_ladder_params_t), intent(in) :: p
+      real(c_double), intent(in) :: delta(*)
+      type(c_ptr), value :: fockk, current, chi_ladd, chi_para
     end function
     integer(c_int) function rpa_photon(p, glfz, parg, pixx0, piyy0, pizz0, piyz0, pizy0, q_line, theta_line, &
                                        qmode_line, pi11, psum, spectra, pi_full) bind(C, name="rpa_photon")
