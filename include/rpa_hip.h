@@ -340,7 +340,8 @@ typedef struct rpa_ladder_params {
 
 /* The frequency loop of Chi_ladd (Chi_ladd_up/BSE.f90:1116-1176): for
  * io = 1..no, L0(k) = 1/(o + Delta(k) + i eta), fock = I - L0 FockK/(Ntot
- * Vcell) inverted (gjel), lladd = fock^-1 L0 FockK L0, and
+ * Vcell), lladd = fock^-1 L0 FockK L0 (gjel in the reference; here the two
+ * closures are solved for directly), and
  *   chi_ladd(pol,io) = -valley sum j(pol,k) lladd(k,k') conj(j(pol,k'))/(Ntot Vcell)^2
  *   chi_para(pol,io) =  valley sum j(pol,k) (o/Delta(k)) L0(k) conj(j(pol,k))/(Ntot Vcell)
  * Inputs: fockk(nk,nk) complex (the FockK file), delta(nk), current(2,nk)
