@@ -78,16 +78,25 @@ __global__ __launch_bounds__(256) void k_pi_dense(int no_interp, long long no_to
   }
 }
 
+// S0(-No:No, pairs) as the caller holds it (frequency fastest) -> the KK's
+// frequency rows s0[w][pair] (interleaved complex)
+__global__ void k_interp_pack(long long nw, int npair, const double2 *__restrict__ raw, double2 *__restrict__ s0) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nw * npair) return;
+  const long long w = t % nw, pr = t / nw;
+  s0[w * npair + pr] = raw[t];
+}
+
 namespace {
 
 // device buffers and the stream of one call, released on every return path
 struct InterpCall {
   hipStream_t s = nullptr;
   double *s0 = nullptr, *kmat = nullptr, *x = nullptr, *chi0 = nullptr, *q = nullptr, *out = nullptr,
-         *intra = nullptr;
+         *intra = nullptr, *raw = nullptr;
   ~InterpCall() {
     if (s) (void)hipStreamSynchronize(s);
-    for (double *p : {s0, kmat, x, chi0, q, out, intra})
+    for (double *p : {s0, kmat, x, chi0, q, out, intra, raw})
       if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
   }
@@ -114,6 +123,7 @@ extern "C" int rpa_interpolate(const rpa_interp_params *p, const double *S0, con
   const int no = p->no, nlf = p->nlf, nint = p->no_interp;
   if (no < 4) return err(RPA_E_ARG, "No must be at least 4");
   if (nlf < 1) return err(RPA_E_ARG, "Nlf must be positive");
+  if (nlf > 4096) return err(RPA_E_LIMIT, "Nlf above 4096");
   if (nint < 1) return err(RPA_E_ARG, "No_interp must be positive");
   if (!(p->domega > 0.0) || !(p->domega_tot > 0.0) || !(p->eta_intra >= 0.0))
     return err(RPA_E_ARG, "domega and domega_tot must be positive, eta_intra not negative");
@@ -126,7 +136,6 @@ extern "C" int rpa_interpolate(const rpa_interp_params *p, const double *S0, con
 
   hipError_t e = hipSetDevice(p->device);
   if (e != hipSuccess) return hip_err(e, "hipSetDevice");
-  std::vector<double> h(nw * ncol);  // outlives the call's stream (declared first)
   InterpCall call;
   if ((e = hipStreamCreateWithFlags(&call.s, hipStreamNonBlocking)) != hipSuccess)
     return hip_err(e, "hipStreamCreateWithFlags");
@@ -135,19 +144,22 @@ extern "C" int rpa_interpolate(const rpa_interp_params *p, const double *S0, con
     double **ptr;
     size_t n;
   } bufs[] = {{&call.s0, nw * ncol},  {&call.kmat, nf * nw},   {&call.x, nf * ncol},       {&call.chi0, nf * ncol},
-              {&call.q, (size_t)ncol}, {&call.out, n_out},     {&call.intra, 2 * (size_t)no_tot}};
+              {&call.q, (size_t)ncol}, {&call.out, n_out},     {&call.intra, 2 * (size_t)no_tot},
+              {&call.raw, nw * ncol}};
   for (auto &b : bufs)
     if ((e = hipMalloc((void **)b.ptr, sizeof(double) * b.n)) != hipSuccess) return hip_err(e, "hipMalloc");
 
   // S0(-No:No, Nlf, Nlf) and Qeff(Nlf, Nlf), column-major complex: column
-  // pair = iG + Nlf jG of the device rows is the reference's own order
-  for (size_t w = 0; w < nw; ++w)
-    for (int pr = 0; pr < npair; ++pr) {
-      h[w * ncol + 2 * pr] = S0[2 * (w + nw * pr)];
-      h[w * ncol + 2 * pr + 1] = S0[2 * (w + nw * pr) + 1];
-    }
-  if ((e = hipMemcpyAsync(call.s0, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, call.s)) != hipSuccess)
+  // pair = iG + Nlf jG of the device rows is the reference's own order (the
+  // first npair columns: all of them, or (1,1) alone)
+  if ((e = hipMemcpyAsync(call.raw, S0, sizeof(double) * nw * ncol, hipMemcpyHostToDevice, call.s)) != hipSuccess)
     return hip_err(e, "hipMemcpyAsync");
+  {
+    const long long n = (long long)nw * npair;
+    hipLaunchKernelGGL(k_interp_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, call.s, (long long)nw, npair,
+                       (const double2 *)call.raw, (double2 *)call.s0);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_err(e, "k_interp_pack launch");
+  }
   if ((e = hipMemcpyAsync(call.q, qeff, sizeof(double) * ncol, hipMemcpyHostToDevice, call.s)) != hipSuccess)
     return hip_err(e, "hipMemcpyAsync");
   if ((e = kk_optabs_columns(call.s, no, p->all ? nlf : 1, ncol, p->domega, call.s0, call.kmat, call.x, call.chi0)) !=

@@ -353,6 +353,29 @@ __global__ void k_gamma_check(int no, int nser, double dw, const double *X, cons
   skk[ser] = b;
 }
 
+// D(No, nser) column-major (frequency fastest) -> S[jo][series] = -(1/pi) Im D
+// and R[jo][series] = Re D
+__global__ void k_gamma_in(int no, int nser, double pi_s, const double2 *__restrict__ D, double *__restrict__ S,
+                           double *__restrict__ R) {
+#pragma clang fp contract(off)
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)no * nser) return;
+  const int jo = (int)(t % no);
+  const long long s = t / no;
+  const double2 d = D[t];
+  S[(size_t)jo * nser + s] = -((1.0 / pi_s) * d.y);
+  R[(size_t)jo * nser + s] = d.x;
+}
+
+// [io'][series] complex -> (No-1, nser) column-major
+__global__ void k_gamma_out(int nf, int nser, const double2 *__restrict__ in, double2 *__restrict__ out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nf * nser) return;
+  const int i = (int)(t % nf);
+  const long long s = t / nf;
+  out[t] = in[(size_t)i * nser + s];
+}
+
 // S(jo, pair) of W for the GW propagators (Sloss.f90:489): -(1/pi) Im W for
 // jo = 1..no-1; S(no) keeps chi0's spectral value S0(no), which the
 // reference does not overwrite
@@ -453,44 +476,54 @@ extern "C" int rpa_gamma_propagator(int no, int nser, double domega, const doubl
   if (!(domega > 0.0)) return err(RPA_E_ARG, "domega must be positive");
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return err(RPA_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-  // S = -(1/pi) Im D and Re D, [jo][series], with propagators.f90's pi
-  const double pi_s = 3.141592654;
+  const double pi_s = 3.141592654;  // propagators.f90's pi
   const size_t nS = (size_t)no * nser, nG = (size_t)(no - 1) * nser;
-  std::vector<double> hs(nS), hr(nS), hg(2 * nG), hm(2 * nG);
-  for (int s = 0; s < nser; ++s)
-    for (int jo = 0; jo < no; ++jo) {
-      const double *d = D + 2 * ((size_t)jo + (size_t)no * s);
-      hs[(size_t)jo * nser + s] = -((1.0 / pi_s) * d[1]);
-      hr[(size_t)jo * nser + s] = d[0];
-    }
   hipStream_t st = nullptr;
-  double *S = nullptr, *R = nullptr, *ws = nullptr, *gp = nullptr, *gm = nullptr, *a = nullptr, *b = nullptr;
+  double *S = nullptr, *R = nullptr, *ws = nullptr, *gp = nullptr, *gm = nullptr, *a = nullptr, *b = nullptr,
+         *dD = nullptr, *go = nullptr;
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
-    for (double *p : {S, R, ws, gp, gm, a, b})
+    for (double *p : {S, R, ws, gp, gm, a, b, dD, go})
       if (p) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
   };
   double *X = nullptr;
+  const long long nin = (long long)nS, nout = (long long)nG;
   if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc((void **)&dD, sizeof(double) * 2 * nS)) != hipSuccess ||
       (e = hipMalloc((void **)&S, sizeof(double) * nS)) != hipSuccess ||
       (e = hipMalloc((void **)&R, sizeof(double) * nS)) != hipSuccess ||
       (e = hipMalloc((void **)&ws, sizeof(double) * gamma_ws_doubles(no, nser))) != hipSuccess ||
       (e = hipMalloc((void **)&gp, sizeof(double) * 2 * nG)) != hipSuccess ||
       (e = hipMalloc((void **)&gm, sizeof(double) * 2 * nG)) != hipSuccess ||
+      (e = hipMalloc((void **)&go, sizeof(double) * 2 * nG)) != hipSuccess ||
       (e = hipMalloc((void **)&a, sizeof(double) * nser)) != hipSuccess ||
       (e = hipMalloc((void **)&b, sizeof(double) * nser)) != hipSuccess ||
-      (e = hipMemcpyAsync(S, hs.data(), sizeof(double) * nS, hipMemcpyHostToDevice, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(R, hr.data(), sizeof(double) * nS, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(dD, D, sizeof(double) * 2 * nS, hipMemcpyHostToDevice, st)) != hipSuccess) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(k_gamma_in, dim3((unsigned)((nin + 255) / 256)), dim3(256), 0, st, no, nser, pi_s,
+                     (const double2 *)dD, S, R);
+  if ((e = hipGetLastError()) != hipSuccess ||
       (e = gamma_series(st, RPA_KK_OPTABS, no, nser, domega, pi_s, S, ws, gp, gm, &X)) != hipSuccess) {
     cleanup();
     return err(RPA_E_HIP, hipGetErrorString(e));
   }
   hipLaunchKernelGGL(k_gamma_check, dim3((unsigned)((nser + 255) / 256)), dim3(256), 0, st, no, nser, domega,
                      (const double *)X, (const double *)R, a, b);
+  const dim3 go_grid((unsigned)((nout + 255) / 256));
+  hipLaunchKernelGGL(k_gamma_out, go_grid, dim3(256), 0, st, no - 1, nser, (const double2 *)gp, (double2 *)go);
   if ((e = hipGetLastError()) != hipSuccess ||
-      (e = hipMemcpyAsync(hg.data(), gp, sizeof(double) * 2 * nG, hipMemcpyDeviceToHost, st)) != hipSuccess ||
-      (e = hipMemcpyAsync(hm.data(), gm, sizeof(double) * 2 * nG, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(gammap, go, sizeof(double) * 2 * nG, hipMemcpyDeviceToHost, st)) != hipSuccess) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
+  }
+  // Gamma+ in [io'][series] has been read by the first transposition (stream
+  // order), so its buffer takes the transposed Gamma-
+  hipLaunchKernelGGL(k_gamma_out, go_grid, dim3(256), 0, st, no - 1, nser, (const double2 *)gm, (double2 *)gp);
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(gammam, gp, sizeof(double) * 2 * nG, hipMemcpyDeviceToHost, st)) != hipSuccess ||
       (e = hipMemcpyAsync(kks, a, sizeof(double) * nser, hipMemcpyDeviceToHost, st)) != hipSuccess ||
       (e = hipMemcpyAsync(skk, b, sizeof(double) * nser, hipMemcpyDeviceToHost, st)) != hipSuccess ||
       (e = hipStreamSynchronize(st)) != hipSuccess) {
@@ -498,13 +531,5 @@ extern "C" int rpa_gamma_propagator(int no, int nser, double domega, const doubl
     return err(RPA_E_HIP, hipGetErrorString(e));
   }
   cleanup();
-  // [io'][series] -> (No-1, nser) column-major
-  for (int s = 0; s < nser; ++s)
-    for (int i = 0; i < no - 1; ++i)
-      for (int r = 0; r < 2; ++r) {
-        gammap[2 * ((size_t)i + (size_t)(no - 1) * s) + r] = hg[2 * ((size_t)i * nser + s) + r];
-        gammam[2 * ((size_t)i + (size_t)(no - 1) * s) + r] = hm[2 * ((size_t)i * nser + s) + r];
-      }
   return RPA_OK;
 }
-

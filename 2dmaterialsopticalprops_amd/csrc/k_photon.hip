@@ -256,16 +256,31 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
   }
 }
 
+// One Pi0 component as the reference holds it, Pi0(No, Nlf, Nlf) column-major
+// (frequency fastest), into the blocks P0[io][c][iG][jG] of io = 1..No-1:
+// reads follow the frequencies, writes are scattered.
+__global__ void k_photon_pack(int c, int no, int nf, int N, const double2 *__restrict__ raw,
+                              double2 *__restrict__ P0) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long NN = (long long)N * N;
+  if (t >= (long long)nf * NN) return;
+  const int f = (int)(t % nf);
+  const long long pr = t / nf;  // iG + N jG
+  const int i = (int)(pr % N), j = (int)(pr / N);
+  P0[(((size_t)f * 5 + c) * N + i) * N + j] = raw[(size_t)f + (size_t)no * pr];
+}
+
 namespace {
 
 struct PhotonCall {
   hipStream_t s = nullptr;
+  double2 *raw = nullptr;
   double2 *P0 = nullptr, *W = nullptr, *T = nullptr, *pi11 = nullptr, *psum = nullptr;
   double *gz = nullptr, *ql = nullptr, *tl = nullptr, *spec = nullptr;
   int *par = nullptr, *qm = nullptr, *status = nullptr;
   ~PhotonCall() {
     if (s) (void)hipStreamSynchronize(s);
-    for (void *p : {(void *)P0, (void *)W, (void *)T, (void *)pi11, (void *)psum, (void *)gz, (void *)ql, (void *)tl,
+    for (void *p : {(void *)raw, (void *)P0, (void *)W, (void *)T, (void *)pi11, (void *)psum, (void *)gz, (void *)ql, (void *)tl,
                     (void *)spec, (void *)par, (void *)qm, (void *)status})
       if (p) (void)hipFree(p);
     if (s) (void)hipStreamDestroy(s);
@@ -314,25 +329,11 @@ extern "C" int rpa_photon(const rpa_photon_params *p, const double *glfz, const 
 
   hipError_t e = hipSetDevice(p->device);
   if (e != hipSuccess) return phip(e, "hipSetDevice");
-  // Pi0 blocks [io][5][N][N], from the reference's Pi0(No, Nlf, Nlf); Piyz0 and
-  // Pizy0 are zero when not given (loadPi0, Photonew.f90:693-694)
-  std::vector<double> h((size_t)nf * 5 * NN * 2, 0.0);
-  const double *src[5] = {pixx0, piyy0, piyz0, pizy0, pizz0};
-  for (int c = 0; c < 5; ++c) {
-    if (!src[c]) continue;
-    for (int f = 0; f < nf; ++f)
-      for (int i = 0; i < N; ++i)
-        for (int j = 0; j < N; ++j) {
-          const size_t s = 2 * ((size_t)f + (size_t)no * (i + (size_t)N * j));
-          const size_t d = 2 * ((((size_t)f * 5 + c) * N + i) * N + j);
-          h[d] = src[c][s];
-          h[d + 1] = src[c][s + 1];
-        }
-  }
   PhotonCall call;
   if ((e = hipStreamCreateWithFlags(&call.s, hipStreamNonBlocking)) != hipSuccess)
     return phip(e, "hipStreamCreateWithFlags");
-  if ((e = dalloc(&call.P0, (size_t)nf * 5 * NN)) != hipSuccess || (e = dalloc(&call.gz, N)) != hipSuccess ||
+  if ((e = dalloc(&call.P0, (size_t)nf * 5 * NN)) != hipSuccess || (e = dalloc(&call.raw, (size_t)no * NN)) != hipSuccess ||
+      (e = dalloc(&call.gz, N)) != hipSuccess ||
       (e = dalloc(&call.par, N)) != hipSuccess || (e = dalloc(&call.ql, nline)) != hipSuccess ||
       (e = dalloc(&call.tl, nline)) != hipSuccess || (e = dalloc(&call.qm, nline)) != hipSuccess ||
       (e = dalloc(&call.pi11, (size_t)npts * 3)) != hipSuccess ||
@@ -345,12 +346,28 @@ extern "C" int rpa_photon(const rpa_photon_params *p, const double *glfz, const 
     void *d;
     const void *s;
     size_t b;
-  } up[] = {{call.P0, h.data(), sizeof(double) * h.size()}, {call.gz, glfz, sizeof(double) * N},
+  } up[] = {{call.gz, glfz, sizeof(double) * N},
             {call.par, parg, sizeof(int) * N},             {call.ql, q_line, sizeof(double) * nline},
             {call.tl, theta_line, sizeof(double) * nline}, {call.qm, qmode_line, sizeof(int) * nline}};
   for (auto &u : up)
     if ((e = hipMemcpyAsync(u.d, u.s, u.b, hipMemcpyHostToDevice, call.s)) != hipSuccess)
       return phip(e, "hipMemcpyAsync");
+  // Pi0 blocks [io][5][N][N] from the reference's Pi0(No, Nlf, Nlf), one
+  // component at a time through the staging buffer; Piyz0 and Pizy0 are zero
+  // when not given (loadPi0, Photonew.f90:693-694)
+  if ((e = hipMemsetAsync(call.P0, 0, sizeof(double2) * (size_t)nf * 5 * NN, call.s)) != hipSuccess)
+    return phip(e, "hipMemsetAsync");
+  const double *src[5] = {pixx0, piyy0, piyz0, pizy0, pizz0};
+  for (int c = 0; c < 5; ++c) {
+    if (!src[c]) continue;
+    if ((e = hipMemcpyAsync(call.raw, src[c], sizeof(double2) * (size_t)no * NN, hipMemcpyHostToDevice, call.s)) !=
+        hipSuccess)
+      return phip(e, "hipMemcpyAsync");
+    const long long n = (long long)nf * NN;
+    hipLaunchKernelGGL(k_photon_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, call.s, c, no, nf, N,
+                       (const double2 *)call.raw, call.P0);
+    if ((e = hipGetLastError()) != hipSuccess) return phip(e, "k_photon_pack launch");
+  }
   if ((e = hipMemsetAsync(call.status, 0, sizeof(int), call.s)) != hipSuccess) return phip(e, "hipMemsetAsync");
 
   PhotonArgs a{};
