@@ -1,5 +1,6 @@
-// Complex FP64 helpers and a workgroup Gauss-Jordan inversion shared by the
-// per-point solvers of the downstream programs (k_photon.hip).
+// Complex FP64 helpers, a workgroup Gauss-Jordan inversion and a Gaussian
+// solve shared by the per-point solvers of the downstream programs
+// (k_photon.hip, k_ladder.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -119,6 +120,91 @@ __device__ void gj_invert(double2 *sa, int n, double2 *fcol, int *piv, double *r
         sa[i * n + k] = sa[i * n + p];
         sa[i * n + p] = t;
       }
+    __syncthreads();
+  }
+}
+
+// Solves A z = y in place for nr right-hand sides y[nr][n] (row r at y + r n)
+// by Gaussian elimination with row pivoting on |Re| + |Im| (lowest row on
+// ties), A n x n row-major in sa: per step the rows below the pivot lose
+// their column-k entries (the multipliers are not kept) and the right-hand
+// sides follow, then back substitution column by column; z overwrites y.
+// A zero pivot sets the status flag and goes on without a swap.  n^3/3
+// multiply-adds and traffic for A where its inverse takes n^3.
+template <bool GLOBAL>
+__device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fcol, double *red_v, int *red_i,
+                            int *status) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int k = 0; k < n; ++k) {
+    double best = -1.0;
+    int bi = n;
+    for (int i = k + tid; i < n; i += 256) {
+      const double2 z = sa[(size_t)i * n + k];
+      const double m = fabs(z.x) + fabs(z.y);
+      if (m > best) {
+        best = m;
+        bi = i;
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      red_v[wave] = best;
+      red_i[wave] = bi;
+    }
+    __syncthreads();
+    double pv = red_v[0];
+    int p = red_i[0];
+    for (int w = 1; w < 4; ++w)
+      if (red_v[w] > pv || (red_v[w] == pv && red_i[w] < p)) {
+        pv = red_v[w];
+        p = red_i[w];
+      }
+    if (!(pv > 0.0)) {
+      if (tid == 0) atomicOr(status, 1);
+      p = k;
+    }
+    if (p != k) {
+      for (int j = k + tid; j < n; j += 256) {
+        const double2 t = sa[(size_t)k * n + j];
+        sa[(size_t)k * n + j] = sa[(size_t)p * n + j];
+        sa[(size_t)p * n + j] = t;
+      }
+      for (int r = tid; r < nr; r += 256) {
+        const double2 t = y[r * n + k];
+        y[r * n + k] = y[r * n + p];
+        y[r * n + p] = t;
+      }
+    }
+    __syncthreads();
+    const double2 inv = c_div(make_double2(1.0, 0.0), sa[(size_t)k * n + k]);
+    for (int i = k + 1 + tid; i < n; i += 256) fcol[i] = c_mul(sa[(size_t)i * n + k], inv);
+    __syncthreads();
+    const int m = n - 1 - k;  // the trailing block is m x m
+    for (int x = tid; x < m * m; x += 256) {
+      const int i = k + 1 + x / m, j = k + 1 + x % m;
+      sa[(size_t)i * n + j] = c_sub(sa[(size_t)i * n + j], c_mul(fcol[i], sa[(size_t)k * n + j]));
+    }
+    for (int x = tid; x < nr * m; x += 256) {
+      const int r = x / m, i = k + 1 + x % m;
+      y[r * n + i] = c_sub(y[r * n + i], c_mul(fcol[i], y[r * n + k]));
+    }
+    __syncthreads();
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    for (int r = tid; r < nr; r += 256) y[r * n + i] = c_div(y[r * n + i], sa[(size_t)i * n + i]);
+    __syncthreads();
+    for (int x = tid; x < nr * i; x += 256) {
+      const int r = x / i, j = x % i;
+      y[r * n + j] = c_sub(y[r * n + j], c_mul(sa[(size_t)j * n + i], y[r * n + i]));
+    }
     __syncthreads();
   }
 }

@@ -37,89 +37,6 @@ struct LadderArgs {
   int *status;
 };
 
-// Solves fock z = y for the two right-hand sides y[2][n] (LDS) by Gaussian
-// elimination with row pivoting on |Re| + |Im| (lowest row on ties): per
-// step the rows below the pivot lose their column-k entries (the
-// multipliers are not kept), then back substitution column by column; z
-// overwrites y.  A zero pivot sets the status flag and goes on without a
-// swap.  n^3/3 multiply-adds and traffic where the inverse takes n^3.
-template <bool GLOBAL>
-__device__ void gauss_solve2(double2 *sa, int n, double2 *y, double2 *fcol, double *red_v, int *red_i, int *status) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int k = 0; k < n; ++k) {
-    double best = -1.0;
-    int bi = n;
-    for (int i = k + tid; i < n; i += 256) {
-      const double2 z = sa[(size_t)i * n + k];
-      const double m = fabs(z.x) + fabs(z.y);
-      if (m > best) {
-        best = m;
-        bi = i;
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ob > best || (ob == best && oi < bi)) {
-        best = ob;
-        bi = oi;
-      }
-    }
-    if (lane == 0) {
-      red_v[wave] = best;
-      red_i[wave] = bi;
-    }
-    __syncthreads();
-    double pv = red_v[0];
-    int p = red_i[0];
-    for (int w = 1; w < 4; ++w)
-      if (red_v[w] > pv || (red_v[w] == pv && red_i[w] < p)) {
-        pv = red_v[w];
-        p = red_i[w];
-      }
-    if (!(pv > 0.0)) {
-      if (tid == 0) atomicOr(status, 1);
-      p = k;
-    }
-    if (p != k) {
-      for (int j = k + tid; j < n; j += 256) {
-        const double2 t = sa[(size_t)k * n + j];
-        sa[(size_t)k * n + j] = sa[(size_t)p * n + j];
-        sa[(size_t)p * n + j] = t;
-      }
-      if (tid < 2) {
-        const double2 t = y[tid * n + k];
-        y[tid * n + k] = y[tid * n + p];
-        y[tid * n + p] = t;
-      }
-    }
-    __syncthreads();
-    const double2 inv = c_div(make_double2(1.0, 0.0), sa[(size_t)k * n + k]);
-    for (int i = k + 1 + tid; i < n; i += 256) fcol[i] = c_mul(sa[(size_t)i * n + k], inv);
-    __syncthreads();
-    const int m = n - 1 - k;  // the trailing block is m x m
-    for (int x = tid; x < m * m; x += 256) {
-      const int i = k + 1 + x / m, j = k + 1 + x % m;
-      sa[(size_t)i * n + j] = c_sub(sa[(size_t)i * n + j], c_mul(fcol[i], sa[(size_t)k * n + j]));
-    }
-    for (int x = tid; x < 2 * m; x += 256) {
-      const int r = x / m, i = k + 1 + x % m;
-      y[r * n + i] = c_sub(y[r * n + i], c_mul(fcol[i], y[r * n + k]));
-    }
-    __syncthreads();
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    if (tid < 2) y[tid * n + i] = c_div(y[tid * n + i], sa[(size_t)i * n + i]);
-    __syncthreads();
-    for (int x = tid; x < 2 * i; x += 256) {
-      const int r = x / i, j = x % i;
-      y[r * n + j] = c_sub(y[r * n + j], c_mul(sa[(size_t)j * n + i], y[r * n + i]));
-    }
-    __syncthreads();
-  }
-}
-
 // dynamic LDS: fock (LDS form), then L0 [n], y [2][n], fcol [n] (double2), piv [n] (int)
 inline size_t ladder_lds_bytes(int n, bool global) {
   return (global ? 0 : sizeof(double2) * (size_t)n * n) + sizeof(double2) * 4 * (size_t)n + sizeof(int) * n;
@@ -160,7 +77,7 @@ __global__ __launch_bounds__(256) void k_ladder(LadderArgs a) {
   }
   __syncthreads();
   (void)piv;
-  gauss_solve2<GLOBAL>(sa, n, y, fcol, red_v, red_i, a.status);
+  gauss_solve<GLOBAL>(sa, n, y, 2, fcol, red_v, red_i, a.status);
   // chi_ladd(pol) = -valley j^T fock^-1 y / NV^2 (z = fock^-1 y now in y),
   // chi_para from the diagonal
   double acc[8];

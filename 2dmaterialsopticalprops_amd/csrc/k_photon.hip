@@ -33,10 +33,10 @@ constexpr int kPhMaxNlf = 64;     // local fields per point (TP is 2N x 2N)
 constexpr int kPhLdsNlf = 44;     // TP in LDS up to here (64 N^2 bytes)
 constexpr double kGamma = 1.0 / 137.0;  // Photonew.f90:38
 
-// dynamic LDS of k_photon: T (LDS form) + 17 N double2 + 3 N double + 2 N int
+// dynamic LDS of k_photon: T (LDS form) + 19 N double2 + 3 N double + 2 N int
 inline size_t photon_lds_bytes(int N, bool global) {
   const size_t M = 2 * (size_t)N;
-  return (global ? 0 : sizeof(double2) * M * M) + sizeof(double2) * 17 * N + sizeof(double) * 3 * N +
+  return (global ? 0 : sizeof(double2) * M * M) + sizeof(double2) * 19 * N + sizeof(double) * 3 * N +
          sizeof(int) * 2 * N;
 }
 
@@ -72,10 +72,14 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
   const int line = p / a.nf, io = p - line * a.nf + 1;
   double2 *sa = GLOBAL ? a.T + (size_t)lp * M * M : lds_dyn;
   double2 *tab = lds_dyn + (GLOBAL ? 0 : (size_t)M * M);
-  double2 *s_aii = tab, *s_pd = tab + N, *s_d1 = tab + 2 * N, *s_d2 = tab + 3 * N, *s_eg = tab + 4 * N;
+  // [0,N) s_eg; [N,15N) the tables of the T builds, then (solve form) the
+  // eight p-mode right-hand sides [8][2N] in [N,17N); [15N,17N) the two
+  // s-mode right-hand sides; [17N,19N) fcol
+  double2 *s_eg = tab, *s_aii = tab + N, *s_pd = tab + 2 * N, *s_d1 = tab + 3 * N, *s_d2 = tab + 4 * N;
   double2 *s_u = tab + 5 * N, *s_v = tab + 10 * N;  // [5][N]
-  double2 *fcol = tab + 15 * N;                      // [2N]
-  double *s_gz = (double *)(tab + 17 * N), *s_ff = s_gz + N, *s_gf = s_gz + 2 * N;
+  double2 *ys = tab + 15 * N, *yp = tab + N;
+  double2 *fcol = tab + 17 * N;                      // [2N]
+  double *s_gz = (double *)(tab + 19 * N), *s_ff = s_gz + N, *s_gf = s_gz + 2 * N;
   int *piv = (int *)(s_gz + 3 * N);                  // [2N]
 
   // ---- the point: frequency, q, beta (Photonew.f90:229-249) -------------
@@ -155,6 +159,16 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
     const double2 num = c_mul(make_double2(s_eg[i].x - 1.0, s_eg[i].y), make_double2(s_eg[j].x - 1.0, s_eg[j].y));
     return c_scale(c_div(num, c_scale(s_eg[i], s_gz[i] * s_gz[j])), -1.0);
   };
+  // the weights factor as w(i,j) = us(i) vs(j): us(i) = int_0^{c0/2} e^{-i Gz_i z} dz,
+  // vs(j) = int_0^{c0/2} e^{i Gz_j z} dz (c0/2 at the first field)
+  auto vslab = [&](int j) {
+    if (j == 0) return make_double2(c0 / 2.0, 0.0);
+    return make_double2(s_eg[j].y / s_gz[j], -(s_eg[j].x - 1.0) / s_gz[j]);
+  };
+  auto uslab = [&](int i) {
+    if (i == 0) return make_double2(c0 / 2.0, 0.0);
+    return make_double2(s_eg[i].y / s_gz[i], (s_eg[i].x - 1.0) / s_gz[i]);
+  };
   double acc[20];
 #pragma unroll
   for (int e = 0; e < 20; ++e) acc[e] = 0.0;
@@ -166,17 +180,41 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
     sa[x] = make_double2((i == j ? 1.0 : 0.0) - t.x, -t.y);
   }
   __syncthreads();
-  gj_invert<GLOBAL>(sa, N, fcol, piv, red_v, red_i, a.status);
   double2 *Wp = a.W ? a.W + (size_t)lp * 5 * N * N : nullptr;
-  for (int x = tid; x < N * N; x += 256) {
-    const int i = x / N, j = x - i * N;
-    double2 s = make_double2(0.0, 0.0);
-    for (int k = 0; k < N; ++k) s = c_add(s, c_mul(sa[i * N + k], P(0, k, j)));
-    if (Wp) Wp[x] = s;
-    if (x == 0) s_11[0] = s;
-    const double2 ws = c_mul(s, wsum(i, j));
-    acc[0] += ws.x;
-    acc[1] += ws.y;
+  if (!Wp) {
+    // solve form: every output is x^T T^-1 Pi0 y, so TS z = Pi0 y for y = e_1
+    // (Pixx(1,1) = z_1) and y = vs (sPixx = us^T z / c0), vs and us the slab
+    // integrals of the plane waves (the rank-one form of the sum's weights)
+    for (int x = tid; x < 2 * N; x += 256) {
+      const int r = x / N, i = x - r * N;
+      double2 b = make_double2(0.0, 0.0);
+      if (r == 0) {
+        b = P(0, i, 0);
+      } else {
+        for (int j = 0; j < N; ++j) b = c_add(b, c_mul(P(0, i, j), vslab(j)));
+      }
+      ys[x] = b;
+    }
+    __syncthreads();
+    gauss_solve<GLOBAL>(sa, N, ys, 2, fcol, red_v, red_i, a.status);
+    if (tid == 0) s_11[0] = ys[0];
+    for (int i = tid; i < N; i += 256) {
+      const double2 t = c_mul(uslab(i), ys[N + i]);
+      acc[0] += t.x;
+      acc[1] += t.y;
+    }
+  } else {
+    gj_invert<GLOBAL>(sa, N, fcol, piv, red_v, red_i, a.status);
+    for (int x = tid; x < N * N; x += 256) {
+      const int i = x / N, j = x - i * N;
+      double2 s = make_double2(0.0, 0.0);
+      for (int k = 0; k < N; ++k) s = c_add(s, c_mul(sa[i * N + k], P(0, k, j)));
+      Wp[x] = s;
+      if (x == 0) s_11[0] = s;
+      const double2 ws = c_mul(s, wsum(i, j));
+      acc[0] += ws.x;
+      acc[1] += ws.y;
+    }
   }
   __syncthreads();
 
@@ -199,34 +237,72 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
     sa[x] = make_double2((I == J ? 1.0 : 0.0) - t.x, -t.y);
   }
   __syncthreads();
-  gj_invert<GLOBAL>(sa, M, fcol, piv, red_v, red_i, a.status);
-  // [Piyy Piyz; Pizy Pizz] = TP^-1 [Piyy0 Piyz0; Pizy0 Pizz0]
-  for (int x = tid; x < M * M; x += 256) {
-    const int I = x / M, J = x - I * M;
-    const int j = J % N, cj = J < N ? 0 : 1;  // column block: y or z
-    double2 s = make_double2(0.0, 0.0);
-    for (int k = 0; k < N; ++k) s = c_add(s, c_mul(sa[I * M + k], P(1 + cj, k, j)));       // rows of Piy.0
-    for (int k = 0; k < N; ++k) s = c_add(s, c_mul(sa[I * M + N + k], P(3 + cj, k, j)));   // rows of Piz.0
-    const int i = I % N;
-    const int c = (I < N ? 1 : 3) + cj;  // 1 yy, 2 yz, 3 zy, 4 zz
-    if (Wp) Wp[((size_t)c * N + i) * N + j] = s;
-    if (i == 0 && j == 0 && (c == 1 || c == 4)) s_11[c == 1 ? 1 : 2] = s;
-    const double2 ws = c_mul(s, wsum(i, j));
-    // genSpectra sums (1016-1030): reflection Ff Pi Gf, absorption Ff Pi Ff
-    const double fi = s_ff[i], gj = s_gf[j], fj = s_ff[j];
-    if (c == 1) {
-      acc[2] += ws.x, acc[3] += ws.y;
-      acc[10] += fi * s.x * gj, acc[11] += fi * s.y * gj;    // Dyy_r
-      acc[14] += fi * s.x * fj, acc[15] += fi * s.y * fj;    // Dyy_a
-    } else if (c == 2) {
-      acc[4] += ws.x, acc[5] += ws.y;
-      acc[18] += fi * s.x * fj, acc[19] += fi * s.y * fj;    // Dyz_a
-    } else if (c == 3) {
-      acc[6] += ws.x, acc[7] += ws.y;
-    } else {
-      acc[8] += ws.x, acc[9] += ws.y;
-      acc[12] += fi * s.x * gj, acc[13] += fi * s.y * gj;    // Dzz_r
-      acc[16] += fi * s.x * fj, acc[17] += fi * s.y * fj;    // Dzz_a
+  if (!Wp) {
+    // solve form: TP z = B0 y with B0 = [Piyy0 Piyz0; Pizy0 Pizz0] restricted to
+    // its y or z columns (blk) and y one of e_1, vs, Gf, Ff (vec): r = 2 vec + blk
+    for (int x = tid; x < 8 * M; x += 256) {
+      const int r = x / M, K = x - r * M;
+      const int blk = r & 1, vec = r >> 1;
+      const int c = (K < N ? 1 : 3) + blk, kk = K < N ? K : K - N;
+      double2 b = make_double2(0.0, 0.0);
+      if (vec == 0) {
+        b = P(c, kk, 0);
+      } else {
+        for (int j = 0; j < N; ++j) {
+          const double2 pij = P(c, kk, j);
+          b = c_add(b, vec == 1 ? c_mul(pij, vslab(j)) : c_scale(pij, vec == 2 ? s_gf[j] : s_ff[j]));
+        }
+      }
+      yp[x] = b;
+    }
+    __syncthreads();
+    gauss_solve<GLOBAL>(sa, M, yp, 8, fcol, red_v, red_i, a.status);
+    if (tid == 0) {
+      s_11[1] = yp[0];      // Piyy(1,1)
+      s_11[2] = yp[M + N];  // Pizz(1,1)
+    }
+    for (int i = tid; i < N; i += 256) {
+      const double2 us = uslab(i);
+      const double fi = s_ff[i];
+      const double2 syy = c_mul(us, yp[2 * M + i]), szz = c_mul(us, yp[3 * M + N + i]);
+      acc[2] += syy.x, acc[3] += syy.y;
+      acc[8] += szz.x, acc[9] += szz.y;
+      acc[10] += fi * yp[4 * M + i].x, acc[11] += fi * yp[4 * M + i].y;          // Dyy_r
+      acc[12] += fi * yp[5 * M + N + i].x, acc[13] += fi * yp[5 * M + N + i].y;  // Dzz_r
+      acc[14] += fi * yp[6 * M + i].x, acc[15] += fi * yp[6 * M + i].y;          // Dyy_a
+      acc[16] += fi * yp[7 * M + N + i].x, acc[17] += fi * yp[7 * M + N + i].y;  // Dzz_a
+      acc[18] += fi * yp[7 * M + i].x, acc[19] += fi * yp[7 * M + i].y;          // Dyz_a
+    }
+  } else {
+    gj_invert<GLOBAL>(sa, M, fcol, piv, red_v, red_i, a.status);
+    // [Piyy Piyz; Pizy Pizz] = TP^-1 [Piyy0 Piyz0; Pizy0 Pizz0]
+    for (int x = tid; x < M * M; x += 256) {
+      const int I = x / M, J = x - I * M;
+      const int j = J % N, cj = J < N ? 0 : 1;  // column block: y or z
+      double2 s = make_double2(0.0, 0.0);
+      for (int k = 0; k < N; ++k) s = c_add(s, c_mul(sa[I * M + k], P(1 + cj, k, j)));       // rows of Piy.0
+      for (int k = 0; k < N; ++k) s = c_add(s, c_mul(sa[I * M + N + k], P(3 + cj, k, j)));   // rows of Piz.0
+      const int i = I % N;
+      const int c = (I < N ? 1 : 3) + cj;  // 1 yy, 2 yz, 3 zy, 4 zz
+      Wp[((size_t)c * N + i) * N + j] = s;
+      if (i == 0 && j == 0 && (c == 1 || c == 4)) s_11[c == 1 ? 1 : 2] = s;
+      const double2 ws = c_mul(s, wsum(i, j));
+      // genSpectra sums (1016-1030): reflection Ff Pi Gf, absorption Ff Pi Ff
+      const double fi = s_ff[i], gj = s_gf[j], fj = s_ff[j];
+      if (c == 1) {
+        acc[2] += ws.x, acc[3] += ws.y;
+        acc[10] += fi * s.x * gj, acc[11] += fi * s.y * gj;    // Dyy_r
+        acc[14] += fi * s.x * fj, acc[15] += fi * s.y * fj;    // Dyy_a
+      } else if (c == 2) {
+        acc[4] += ws.x, acc[5] += ws.y;
+        acc[18] += fi * s.x * fj, acc[19] += fi * s.y * fj;    // Dyz_a
+      } else if (c == 3) {
+        acc[6] += ws.x, acc[7] += ws.y;
+      } else {
+        acc[8] += ws.x, acc[9] += ws.y;
+        acc[12] += fi * s.x * gj, acc[13] += fi * s.y * gj;    // Dzz_r
+        acc[16] += fi * s.x * fj, acc[17] += fi * s.y * fj;    // Dzz_a
+      }
     }
   }
   __syncthreads();
