@@ -30,7 +30,7 @@ EXPORTS = [
     "rpa_load_s0", "rpa_w_kk", "rpa_kk_noalloc", "rpa_set_autotune", "rpa_batch_set", "rpa_batch_run",
     "rpa_batch_result", "rpa_set_symmetry", "rpa_set_q_sym", "rpa_batch_set_sym", "rpa_find_kq_sym",
     "rpa_point_inverse", "rpa_interpolate", "rpa_photon", "rpa_gamma_w", "rpa_gamma_propagator",
-    "rpa_ladder",
+    "rpa_ladder", "rpa_fock_kernel",
 ]
 # enum rpa_result
 RES = {"S0": 0, "QEFF": 1, "CHI0": 2, "V": 3, "EPSINV": 4, "CHI": 5, "W": 6, "LOSS": 7,
@@ -60,6 +60,12 @@ class RpaInterpParams(C.Structure):
 class RpaPhotonParams(C.Structure):
     _fields_ = [("no", C.c_int), ("nlf", C.c_int), ("nline", C.c_int), ("device", C.c_int),
                 ("omin", C.c_double), ("domega", C.c_double), ("eta", C.c_double), ("c0", C.c_double)]
+
+
+class RpaFockParams(C.Structure):
+    _fields_ = [("nk", C.c_int), ("nlf", C.c_int), ("nq", C.c_int), ("ntot", C.c_int), ("device", C.c_int),
+                ("pad", C.c_int), ("gcar", C.c_double), ("qmin", C.c_double), ("c0", C.c_double),
+                ("vcell", C.c_double)]
 
 
 class RpaLadderParams(C.Structure):
@@ -131,6 +137,7 @@ def lib() -> C.CDLL:
         L.rpa_point_inverse.argtypes = [C.c_int, P, P]
         L.rpa_interpolate.argtypes = [C.POINTER(RpaInterpParams), P, P, P, P]
         L.rpa_ladder.argtypes = [C.POINTER(RpaLadderParams), P, P, P, P, P]
+        L.rpa_fock_kernel.argtypes = [C.POINTER(RpaFockParams), P, P, P, P]
         L.rpa_gamma_w.argtypes = [P, P, P]
         L.rpa_gamma_propagator.argtypes = [C.c_int, C.c_int, C.c_double, P, P, P, P, P, C.c_int]
         L.rpa_photon.argtypes = [C.POINTER(RpaPhotonParams), P, P, P, P, P, P, P, P, P, P, P, P, P, P]
@@ -244,6 +251,24 @@ def interpolate(S0, qeff, no_interp, domega, domega_tot, eta_intra, all_pairs=Tr
     if rc != RPA_OK:
         raise RpaError(rc, lib().rpa_last_error(None).decode())
     return out, intra
+
+
+def fock_kernel(kbz, wt, mnm, ntot, gcar, qmin, c0, vcell, device=0):
+    """The Fock kernel of Chi_ladd on the device (rpa_fock_kernel): kbz (3, nk),
+    wt (nq, nlf, nlf), mnm (2, nlf, nk, nk) -> fockk (nk, nk) complex."""
+    k = _f(kbz, np.float64)
+    w = _f(wt, np.complex128)
+    m = _f(mnm, np.complex128)
+    nk = k.shape[1]
+    nq, nlf = w.shape[0], w.shape[1]
+    if k.shape != (3, nk) or w.shape != (nq, nlf, nlf) or m.shape != (2, nlf, nk, nk):
+        raise ValueError("kbz (3, nk), wt (nq, nlf, nlf) and mnm (2, nlf, nk, nk) expected")
+    out = np.zeros((nk, nk), dtype=np.complex128, order="F")
+    p = RpaFockParams(nk, nlf, nq, ntot, device, 0, gcar, qmin, c0, vcell)
+    rc = lib().rpa_fock_kernel(C.byref(p), _ptr(k), _ptr(w), _ptr(m), _ptr(out))
+    if rc != RPA_OK:
+        raise RpaError(rc, lib().rpa_last_error(None).decode())
+    return out
 
 
 def ladder(fockk, delta, current, no, omin, domega, eta, ntot, vcell, valley, device=0):

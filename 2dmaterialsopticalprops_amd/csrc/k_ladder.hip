@@ -13,6 +13,7 @@
 // to 88 k, past that in a per-frequency global slice), and chi_ladd = -c
 // j^T z: neither the inverse nor the reference's n^3 product lladd is formed.
 // One workgroup per frequency.
+#include <cmath>
 #include <string>
 
 #include "dev_linalg.h"
@@ -104,9 +105,112 @@ __global__ __launch_bounds__(256) void k_ladder(LadderArgs a) {
   }
 }
 
+// Fock kernel of Chi_ladd (Chi_ladd_up/BSE.f90:804-816): one wave per pair
+// (k, k') of the kernel's k-points, fockk(k,k') = -sum_{G,G'} W(iq,G,G')
+// conj(M1(G)) M2(G') with iq = int(Gcar |k' - k|_xy / qmin) + 1, and on the
+// diagonal the Q = 0 term -Vcell Ntot ln(1 + alpha epsq) / alpha.  Lane j
+// takes columns G' = j, j + 64, ...: t = sum_G conj(M1(G)) W(iq,G,G'), then
+// the wave sums t M2(G').  Pairs whose iq falls outside 2..nq get 0 (the
+// reference reads W only for those and leaves such entries unset).
+__global__ __launch_bounds__(256) void k_fock(int nk, int nlf, int nq, double gcar, double qmin, double diag,
+                                              const double *__restrict__ kbz, const double2 *__restrict__ W,
+                                              const double2 *__restrict__ M, double2 *__restrict__ fockk) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const long long pair = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pair >= (long long)nk * nk) return;
+  const int k1 = (int)(pair % nk), k2 = (int)(pair / nk);
+  double2 f = make_double2(0.0, 0.0);
+  if (k1 == k2) {
+    f = make_double2(diag, 0.0);
+  } else {
+    const double qx = kbz[3 * k2] - kbz[3 * k1], qy = kbz[3 * k2 + 1] - kbz[3 * k1 + 1];
+    const double q = gcar * sqrt(qx * qx + qy * qy);
+    const int iq = (int)(q / qmin) + 1;
+    if (iq >= 2 && iq <= nq) {
+      const double2 *m = M + 2 * (size_t)nlf * pair;  // [ni][G] of the pair
+      const double2 *w = W + (size_t)(iq - 1) * nlf * nlf;  // [G][G']
+      double2 acc = make_double2(0.0, 0.0);
+      for (int j = lane; j < nlf; j += 64) {
+        double2 t = make_double2(0.0, 0.0);
+        for (int g = 0; g < nlf; ++g) t = c_add(t, c_mul(c_conj(m[2 * g]), w[(size_t)g * nlf + j]));
+        acc = c_add(acc, c_mul(t, m[2 * j + 1]));
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        acc.x += __shfl_xor(acc.x, off);
+        acc.y += __shfl_xor(acc.y, off);
+      }
+      f = make_double2(-acc.x, -acc.y);
+    }
+  }
+  if (lane == 0) fockk[pair] = f;
+}
+
 }  // namespace rpa
 
 using namespace rpa;
+
+extern "C" int rpa_fock_kernel(const rpa_fock_params *p, const double *kbz, const double *wt, const double *mnm,
+                               double *fockk) {
+  auto err = [](int code, const std::string &m) {
+    set_free_error("rpa_fock_kernel: " + m);
+    return code;
+  };
+  if (!p || !kbz || !wt || !mnm || !fockk) return err(RPA_E_ARG, "null argument");
+  const int nk = p->nk, N = p->nlf, nq = p->nq;
+  if (nk < 1 || N < 1 || nq < 2) return err(RPA_E_ARG, "nkbz and Nlf must be positive and nq at least 2");
+  if (N > kMaxNlf) return err(RPA_E_LIMIT, "Nlf above " + std::to_string(kMaxNlf));
+  if (!(p->qmin > 0.0) || !(p->c0 > 0.0) || !(p->vcell > 0.0) || p->ntot < 1)
+    return err(RPA_E_ARG, "qmin, c0, Vcell and Ntot must be positive");
+  const size_t NN = (size_t)N * N, npair = (size_t)nk * nk;
+  // alpha from W(q = qmin) at G = G' = 0 (BSE.f90:542), epsq (:305), the
+  // diagonal Q = 0 term (:816)
+  const double pi = 3.141592654;  // BSE.f90's pi
+  const double vq = (2.0 * pi) / p->qmin;
+  const double w11 = wt[2 * (size_t)1];  // Re wt(2,1,1): (nq, Nlf, Nlf) column-major, iq = 2 first
+  const double alpha = (p->c0 * vq / w11 - 1.0) / p->qmin;
+  const double epsq = std::sqrt(4.0 * pi * p->c0 / (p->vcell * p->ntot));
+  const double diag = -p->vcell * p->ntot * std::log(1.0 + alpha * epsq) / alpha;
+  // W(nq, Nlf, Nlf) -> [iq][G][G']
+  std::vector<double> hw(2 * (size_t)nq * NN);
+  for (int q = 0; q < nq; ++q)
+    for (int g = 0; g < N; ++g)
+      for (int h = 0; h < N; ++h)
+        for (int r = 0; r < 2; ++r)
+          hw[2 * (((size_t)q * N + g) * N + h) + r] = wt[2 * ((size_t)q + (size_t)nq * (g + (size_t)N * h)) + r];
+  hipError_t e = hipSetDevice(p->device);
+  if (e != hipSuccess) return err(RPA_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  hipStream_t st = nullptr;
+  double *dk = nullptr, *dw = nullptr, *dm = nullptr, *df = nullptr;
+  auto cleanup = [&]() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (double *q : {dk, dw, dm, df})
+      if (q) (void)hipFree(q);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc((void **)&dk, sizeof(double) * 3 * nk)) != hipSuccess ||
+      (e = hipMalloc((void **)&dw, sizeof(double) * hw.size())) != hipSuccess ||
+      (e = hipMalloc((void **)&dm, sizeof(double) * 4 * N * npair)) != hipSuccess ||
+      (e = hipMalloc((void **)&df, sizeof(double) * 2 * npair)) != hipSuccess ||
+      (e = hipMemcpyAsync(dk, kbz, sizeof(double) * 3 * nk, hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(dw, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(dm, mnm, sizeof(double) * 4 * N * npair, hipMemcpyHostToDevice, st)) != hipSuccess) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(k_fock, dim3((unsigned)((npair + 3) / 4)), dim3(256), 0, st, nk, N, nq, p->gcar, p->qmin, diag,
+                     (const double *)dk, (const double2 *)dw, (const double2 *)dm, (double2 *)df);
+  if ((e = hipGetLastError()) != hipSuccess ||
+      (e = hipMemcpyAsync(fockk, df, sizeof(double) * 2 * npair, hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
+  }
+  cleanup();
+  return RPA_OK;
+}
 
 extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const double *delta,
                           const double *current, double *chi_ladd, double *chi_para) {

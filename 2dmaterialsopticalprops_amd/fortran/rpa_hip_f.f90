@@ -28,6 +28,11 @@ module rpa_hip
     real(c_double) :: eps, gcar, domega, eta, lor_cut, df_cut, efermi, temp, vcell, c0, eta_intra
   end type rpa_params_t
 
+  type, bind(C) :: rpa_fock_params_t
+    integer(c_int) :: nk, nlf, nq, ntot, device, pad
+    real(c_double) :: gcar, qmin, c0, vcell
+  end type rpa_fock_params_t
+
   type, bind(C) :: rpa_ladder_params_t
     integer(c_int) :: nk, no, ntot, device
     real(c_double) :: omin, domega, eta, vcell, valley
@@ -217,6 +222,12 @@ module rpa_hip
       real(c_double), value :: domega
       type(c_ptr), value :: D, gammap, gammam
       real(c_double), intent(out) :: kks(*), skk(*)
+    end function
+    integer(c_int) function rpa_fock_kernel(p, kbz, wt, mnm, fockk) bind(C, name="rpa_fock_kernel")
+      import :: c_int, c_double, c_ptr, rpa_fock_params_t
+      type(rpa_fock_params_t), intent(in) :: p
+      real(c_double), intent(in) :: kbz(3, *)
+      type(c_ptr), value :: wt, mnm, fockk
     end function
     integer(c_int) function rpa_ladder(p, fockk, delta, current, chi_ladd, chi_para) bind(C, name="rpa_ladder")
       import :: c_int, c_double, c_ptr, rpa_ladder_params_t

@@ -507,7 +507,8 @@ def main():
     ap.add_argument("--gamma", action="store_true",
                     help="time the GW propagators (rpa_gamma_propagator): 701 omega, 26010 series")
     ap.add_argument("--ladder", action="store_true",
-                    help="time the ladder frequency loop (rpa_ladder): 512 k, 401 omega")
+                    help="time the ladder frequency loop (rpa_ladder): 512 k, 401 omega, and the Fock kernel "
+                         "(rpa_fock_kernel) of the same 512 k")
     args = ap.parse_args()
     if args.interpolate:
         return bench_interpolate(args)
@@ -774,6 +775,26 @@ def bench_ladder(args):
         res["cpu_baseline"] = {"value": round(no / t, 3), "unit": "omega/s", "cores": 1, "kind": "port",
                                "sample": f"oracle on {nsmp} of {no} frequencies, scaled"}
         res["max_rel_err_vs_oracle_sample"] = float(np.abs(cl[:, :nsmp] - wl).max() / np.abs(wl).max())
+    # the Fock kernel of the same k-points (Chi_ladd_up/BSE.f90:804-816): 512^2 pairs,
+    # N_G = 33, 121 W tables
+    nlf, nq, qmin = 33, 121, 0.0025
+    kbz = np.asfortranarray(np.vstack([rng.uniform(0, 0.05, nk), rng.uniform(0, 0.05, nk), np.zeros(nk)]))
+    wt = np.asfortranarray(rng.standard_normal((nq, nlf, nlf)) + 1j * rng.standard_normal((nq, nlf, nlf)))
+    wt[1, 0, 0] = abs(wt[1, 0, 0]) + 100.0
+    mnm = np.asfortranarray(rng.standard_normal((2, nlf, nk, nk)) + 1j * rng.standard_normal((2, nlf, nk, nk)))
+    gcar = 2 * np.pi / 5.9715
+    abi.fock_kernel(kbz, wt, mnm, ntot, gcar, qmin, 29.8575, vcell)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fk = abi.fock_kernel(kbz, wt, mnm, ntot, gcar, qmin, 29.8575, vcell)
+    res["fock_kernel"] = {"ms_per_call": round((time.perf_counter() - t0) * 1e3 / args.steps, 3),
+                          "workload": f"{nk}^2 pairs, N_G={nlf}, {nq} W tables (host in, host out)"}
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        sub = np.asfortranarray(mnm[:, :, :64, :64])
+        wf = orc.fock_kernel(kbz[:, :64], wt, sub, nq, ntot, gcar, qmin, 29.8575, vcell)
+        res["fock_kernel"]["cpu_oracle_ms_scaled"] = round((time.perf_counter() - t0) * 1e3 * (nk / 64) ** 2, 1)
+        res["fock_kernel"]["max_rel_err_vs_oracle_64x64"] = float(np.abs(fk[:64, :64] - wf).max() / np.abs(wf).max())
     print(json.dumps(res))
 
 

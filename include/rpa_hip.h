@@ -326,6 +326,32 @@ int rpa_gamma_propagator(int no, int nser, double domega, const double *D, doubl
                          double *kks, double *skk, int device);
 
 /* ---- ladder (BSE) current-current response --------------------------------- */
+typedef struct rpa_fock_params {
+  int nk;         /* nkbz: k-points of the kernel (the circles about K and K')  */
+  int nlf;        /* Nlf                                                        */
+  int nq;         /* W_Qi tables for iq = 2..nq (q = (iq-1) qmin)                */
+  int ntot;       /* Ntot                                                       */
+  int device;     /* HIP device                                                 */
+  int pad;
+  double gcar;    /* 2 pi / a0                                                  */
+  double qmin;    /* the W table's q step, a.u.                                  */
+  double c0;      /* a.u.                                                        */
+  double vcell;   /* a.u.^3                                                      */
+} rpa_fock_params;
+
+/* The Fock kernel of Chi_ladd (Chi_ladd_up/BSE.f90:804-816) for every pair of
+ * its k-points: fockk(k,k') = -sum W(iq,G,G') conj(M(1,G)) M(2,G') with
+ * iq = int(Gcar |kbz(k') - kbz(k)|_xy / qmin) + 1, and on the diagonal
+ * -Vcell Ntot ln(1 + alpha epsq)/alpha, alpha = (c0 (2 pi/qmin)/Re W(2,1,1) - 1)/qmin
+ * (:542), epsq = sqrt(4 pi c0/(Vcell Ntot)) (:305).  Inputs: kbz(3,nk) in
+ * 2pi/a0, wt(nq,Nlf,Nlf) complex (the W_Qi tables; iq = 1 unused),
+ * mnm(2,Nlf,nk,nk) complex: the charge vertices of bands Nval and Nval+1 of
+ * each pair (the reference's mnmk1k2, :700-795).  Output fockk(nk,nk)
+ * complex (the FockK file).  Pairs with iq outside 2..nq get 0.  No context.
+ */
+int rpa_fock_kernel(const rpa_fock_params *p, const double *kbz, const double *wt, const double *mnm,
+                    double *fockk);
+
 typedef struct rpa_ladder_params {
   int nk;         /* nkbz: k-points of the Fock kernel (<= 1024)               */
   int no;         /* frequencies io = 1..no                                     */

@@ -75,6 +75,8 @@ def lib():
         L.orc_dyson_w.argtypes = [C.c_int, P_CPLX, P_DBL, P_CPLX, P_CPLX, P_CPLX]
         L.orc_pi_assembly.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       P_CPLX, P_CPLX, P_CPLX, P_DBL, P_DBL]
+        L.orc_fock_kernel.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, P_DBL, P_CPLX, P_CPLX, P_CPLX]
         L.orc_ladder.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
                                  C.c_double, P_CPLX, P_DBL, P_CPLX, P_CPLX, P_CPLX]
         L.orc_gamma_propagator.argtypes = [C.c_int, C.c_double, P_CPLX, P_CPLX, P_CPLX,
@@ -475,3 +477,15 @@ def ladder(fockk, delta, current, no, omin, domega, eta, ntot, vcell, valley):
     lib().orc_ladder(nk, no, omin, domega, eta, ntot, vcell, valley, F(fockk, np.complex128),
                      F(delta, np.float64), F(current, np.complex128), cl, cp)
     return cl, cp
+
+
+def fock_kernel(kbz, wt, mnm, nq, ntot, gcar, qmin, c0, vcell):
+    """The Fock kernel of Chi_ladd (Chi_ladd_up/BSE.f90:305,542,804-816):
+    kbz (3, nk), wt (nq, nlf, nlf), mnm (2, nlf, nk, nk) -> fockk (nk, nk)."""
+    kbz = F(kbz, np.float64)
+    nk = kbz.shape[1]
+    nlf = wt.shape[1]
+    out = np.zeros((nk, nk), dtype=np.complex128, order="F")
+    lib().orc_fock_kernel(nk, nlf, nq, ntot, gcar, qmin, c0, vcell, kbz, F(wt, np.complex128),
+                          F(mnm, np.complex128), out)
+    return out

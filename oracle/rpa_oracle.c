@@ -1272,6 +1272,42 @@ void orc_dyson_w(int nlf, const orc_z *chi0, const double *V, orc_z *epsinv,
   free(c);
 }
 
+/* f4: the Fock kernel of Chi_ladd, Chi_ladd_up/BSE.f90 in its statement
+ * order: epsq (:305), alpha from wt(2,1,1) (:542), per pair (ik1, ik2) the
+ * transfer q and iq (:672-676, pairs past nq skipped and left 0 here), the
+ * contraction and the Q = 0 diagonal (:804-816).  wt(nq,Nlf,Nlf),
+ * mnm(2,Nlf,nk,nk), fockk(nk,nk), all column-major.                       */
+void orc_fock_kernel(int nk, int nlf, int nq, int ntot, double gcar, double qmin, double c0,
+                     double vcell, const double *kbz, const orc_z *wt, const orc_z *mnm,
+                     orc_z *fockk) {
+  const double pi = 3.141592654;
+  double epsq = sqrt(4.0 * pi * c0 / (vcell * ntot));
+  double vq = (2.0 * pi) / qmin;
+  double alpha = (c0 * vq / creal(wt[1]) - 1.0) / qmin;
+  for (int ik1 = 0; ik1 < nk; ++ik1)
+    for (int ik2 = 0; ik2 < nk; ++ik2) {
+      size_t f = (size_t)ik1 + (size_t)nk * ik2;
+      fockk[f] = 0.0;
+      double qx = kbz[3 * ik2] - kbz[3 * ik1], qy = kbz[3 * ik2 + 1] - kbz[3 * ik1 + 1];
+      double q = gcar * sqrt(qx * qx + qy * qy);
+      int iq = (int)(q / qmin) + 1;
+      if (iq > nq) continue;
+      if (ik1 != ik2) {
+        if (iq < 2) continue; /* wt(1,:,:) is never read by the reference */
+        const orc_z *m = mnm + 2 * (size_t)nlf * f;
+        orc_z s = 0.0;
+        for (int ig = 0; ig < nlf; ++ig)
+          for (int jg = 0; jg < nlf; ++jg) {
+            orc_z w = wt[(size_t)(iq - 1) + (size_t)nq * (ig + (size_t)nlf * jg)];
+            s = s - w * conj(m[2 * ig]) * m[2 * jg + 1];
+          }
+        fockk[f] = s;
+      } else {
+        fockk[f] = -vcell * ntot * log(1.0 + alpha * epsq) / alpha;
+      }
+    }
+}
+
 /* f4: the ladder frequency loop of Chi_ladd, Chi_ladd_up/BSE.f90:1116-1176,
  * in its statement order (fock and xi0, gjel, lladd = fock^-1 xi0, the two
  * closures); the reference's COMPLEX*8 arrays are FP64 here.              */

@@ -132,6 +132,11 @@ void orc_pi_assembly(int no, int nlf, double domega, double eta_intra, double c0
                      const orc_z *Pi0, const orc_z *Qeff, orc_z *pi11,
                      double *sig_inter, double *sig_intra);
 
+/* ---- f4: Fock kernel of Chi_ladd, Chi_ladd_up/BSE.f90:305, 542, 804-816 ------- */
+void orc_fock_kernel(int nk, int nlf, int nq, int ntot, double gcar, double qmin, double c0,
+                     double vcell, const double *kbz, const orc_z *wt, const orc_z *mnm,
+                     orc_z *fockk);
+
 /* ---- f4: ladder (BSE) frequency loop, Chi_ladd_up/BSE.f90:1116-1176 ---------- */
 /* fockk(nk,nk), current(2,nk) complex, delta(nk) -> chi_ladd(2,no),
  * chi_para(2,no) complex; FP64 throughout, fock inverted by orc_gjel.      */

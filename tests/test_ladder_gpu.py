@@ -43,3 +43,20 @@ def test_ladder_argument_checks():
     with pytest.raises(abi.RpaError) as e:
         abi.ladder(np.zeros((1025, 1025)), np.ones(1025), np.ones((2, 1025)), 1, 0.1, 0.1, 0.01, 3, 1.0, 2.0)
     assert e.value.code == abi.RPA_E_LIMIT
+
+
+@pytest.mark.parametrize("nk,nlf", [(3, 1), (20, 7), (12, 50), (6, 70)])
+def test_fock_kernel_matches_oracle(nk, nlf):
+    print(f"[fock] nk={nk} nlf={nlf}", flush=True)
+    rng = np.random.default_rng(nk * 100 + nlf)
+    nq, ntot, c0, vcell = 6, 400, 20.0, 800.0
+    gcar, qmin = 2 * np.pi / 4.65, 0.01
+    kbz = np.asfortranarray(np.vstack([rng.uniform(0, 0.012, nk), rng.uniform(0, 0.012, nk), np.zeros(nk)]))
+    wt = np.asfortranarray(rng.standard_normal((nq, nlf, nlf)) + 1j * rng.standard_normal((nq, nlf, nlf)))
+    wt[1, 0, 0] = abs(wt[1, 0, 0]) + 50.0
+    mnm = np.asfortranarray(rng.standard_normal((2, nlf, nk, nk)) + 1j * rng.standard_normal((2, nlf, nk, nk)))
+    got = abi.fock_kernel(kbz, wt, mnm, ntot, gcar, qmin, c0, vcell)
+    want = orc.fock_kernel(kbz, wt, mnm, nq, ntot, gcar, qmin, c0, vcell)
+    assert np.isfinite(got).all()
+    assert H.rel_max(got, want) < 1e-13
+    assert np.array_equal(got == 0, want == 0)

@@ -929,3 +929,36 @@ def test_ladder_is_the_summed_neumann_series():
     cl0, cp0 = orc.ladder(np.zeros((nk, nk)), delta, cur, no, omin, dw, eta, ntot, vcell, valley)
     assert np.all(cl0 == 0.0)
     np.testing.assert_allclose(cp0, cp, rtol=1e-15)
+
+
+def test_fock_kernel_contraction_diagonal_and_transfer_rule():
+    """Chi_ladd's Fock kernel (Chi_ladd_up/BSE.f90:305,542,672-676,804-816):
+    off the diagonal -M1^H W(iq) M2 with a non-symmetric W (numpy: a swapped
+    (G, G') order or the conjugate on the wrong vertex fails it); on the
+    diagonal -Vcell Ntot ln(1 + alpha epsq)/alpha, with alpha recovered from
+    a W(2,1,1) built from a chosen alpha; pairs past nq, or below qmin, 0."""
+    rng = np.random.default_rng(9)
+    nk, nlf, nq, ntot, c0, vcell = 6, 3, 5, 100, 20.0, 800.0
+    gcar, qmin, pi = 2 * np.pi / 4.65, 0.01, 3.141592654
+    kbz = np.zeros((3, nk))
+    kbz[0] = np.array([0.0, 0.0017, 0.0040, 0.0061, 0.0125, 0.0003])  # 2pi/a0 units
+    wt = rng.standard_normal((nq, nlf, nlf)) + 1j * rng.standard_normal((nq, nlf, nlf))
+    alpha0 = 3.7
+    wt[1, 0, 0] = c0 * (2 * pi / qmin) / (1 + alpha0 * qmin) + 0.4j
+    mnm = rng.standard_normal((2, nlf, nk, nk)) + 1j * rng.standard_normal((2, nlf, nk, nk))
+    f = orc.fock_kernel(kbz, wt, mnm, nq, ntot, gcar, qmin, c0, vcell)
+    epsq = np.sqrt(4 * pi * c0 / (vcell * ntot))
+    diag = -vcell * ntot * np.log(1 + alpha0 * epsq) / alpha0
+    for k1 in range(nk):
+        for k2 in range(nk):
+            if k1 == k2:
+                assert f[k1, k2] == pytest.approx(diag, rel=1e-12)
+                continue
+            q = gcar * abs(kbz[0, k2] - kbz[0, k1])
+            iq = int(q / qmin) + 1
+            if iq < 2 or iq > nq:
+                assert f[k1, k2] == 0
+            else:
+                want = -(mnm[0, :, k1, k2].conj() @ wt[iq - 1] @ mnm[1, :, k1, k2])
+                assert abs(f[k1, k2] - want) < 1e-13 * abs(want)
+    assert (np.abs(f) > 0).sum() > nk + 4  # both kinds of pairs occur
