@@ -9,10 +9,29 @@
 //   chi_para(pol) =  valley sum_k j(k) (o/Delta(k)) L0(k) conj(j(k)) / NV
 // for the two current components j = current(pol, :).  The closing vector
 // y = xi0 conj(j) = L0 F (L0 conj(j)) is formed first (O(n^2)), then
-// fock z = y is solved by Gaussian elimination with row pivoting (in LDS up
-// to 88 k, past that in a per-frequency global slice), and chi_ladd = -c
-// j^T z: neither the inverse nor the reference's n^3 product lladd is formed.
-// One workgroup per frequency.
+// fock z = y is solved with row pivoting, and chi_ladd = -c j^T z: neither
+// the inverse nor the reference's n^3 product lladd is formed.
+//
+// Up to 88 k-points one workgroup per frequency eliminates in LDS.  Past
+// that the solve is blocked over all frequencies of a chunk at once, each
+// fock in its own slice [n][ld] with y in columns n, n+1:
+//   k_lb_form    fock and y per frequency;
+//   k_lb_panel   per panel of NB columns, one workgroup per frequency: the
+//                panel's active rows in registers (one per thread), NB
+//                pivot steps (|Re| + |Im|, lowest row on ties) that choose
+//                rows without moving them, the multipliers written in place,
+//                the active-row list compacted, and U12 = L11^-1 A12 on the
+//                pivot rows for every column right of the panel;
+//   k_lb_update  the trailing rows lose L21 U12, a batched complex GEMM on
+//                the FP64 matrix cores (64 x 64 tiles, frequency in z);
+//   k_lb_back    back substitution over the pivot order in blocks of NB
+//                (the block's sums against the solved part by waves, its
+//                triangle by one wave), then chi_ladd and chi_para.
+// The trailing block is read and written once per panel, n^3 / (3 NB)
+// complex values per frequency, where the unblocked solve streamed it once
+// per pivot through one compute unit.
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <string>
 
@@ -32,18 +51,45 @@ struct LadderArgs {
   const double2 *F;      // [k][k'] Fock kernel (row-major)
   const double *delta;   // [n]
   const double2 *cur;    // [2][n] current(pol, k)
-  double2 *T;            // GLOBAL form: [frequency in chunk][n][n]
   double2 *chi_ladd;     // [io][2]
   double2 *chi_para;     // [io][2]
   int *status;
 };
 
-// dynamic LDS: fock (LDS form), then L0 [n], y [2][n], fcol [n] (double2), piv [n] (int)
-inline size_t ladder_lds_bytes(int n, bool global) {
-  return (global ? 0 : sizeof(double2) * (size_t)n * n) + sizeof(double2) * 4 * (size_t)n + sizeof(int) * n;
+// dynamic LDS: fock, then L0 [n], y [2][n], fcol [n] (double2)
+inline size_t ladder_lds_bytes(int n) { return sizeof(double2) * ((size_t)n * n + 4 * (size_t)n); }
+
+// chi_ladd(pol) = -valley j^T z / NV^2 and chi_para(pol) from the diagonal
+// (z = fock^-1 y in z[2][n]), written by thread 0 of the workgroup
+__device__ void ladder_close(const double2 *cur, const double *delta, const double2 *z, int n, double o, double eta,
+                             double nv, double valley, int io, double2 *chi_ladd, double2 *chi_para, double *red) {
+  const int tid = threadIdx.x;
+  double acc[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) acc[e] = 0.0;
+  for (int k = tid; k < n; k += 256) {
+    const double2 z0 = z[k], z1 = z[n + k];
+    const double2 j0 = cur[k], j1 = cur[n + k];
+    const double2 t0 = c_mul(j0, z0), t1 = c_mul(j1, z1);
+    acc[0] += t0.x, acc[1] += t0.y, acc[2] += t1.x, acc[3] += t1.y;
+    const double2 l0 = c_div(make_double2(1.0, 0.0), make_double2(o + delta[k], eta));
+    const double2 lp = c_scale(l0, o / delta[k]);  // (o / Delta) L0
+    const double2 p0 = c_mul(c_mul(j0, lp), c_conj(j0)), p1 = c_mul(c_mul(j1, lp), c_conj(j1));
+    acc[4] += p0.x, acc[5] += p0.y, acc[6] += p1.x, acc[7] += p1.y;
+  }
+  block_sum<8>(acc, red);
+  if (tid == 0) {
+    const double *r = red + 32;
+    const double inv_nv = 1.0 / nv;
+    const double cl = -valley * inv_nv * inv_nv, cp = valley * inv_nv;
+    double2 *ol = chi_ladd + 2 * (size_t)(io - 1), *op = chi_para + 2 * (size_t)(io - 1);
+    ol[0] = make_double2(cl * r[0], cl * r[1]);
+    ol[1] = make_double2(cl * r[2], cl * r[3]);
+    op[0] = make_double2(cp * r[4], cp * r[5]);
+    op[1] = make_double2(cp * r[6], cp * r[7]);
+  }
 }
 
-template <bool GLOBAL>
 __global__ __launch_bounds__(256) void k_ladder(LadderArgs a) {
   extern __shared__ double2 lds_dyn[];
   __shared__ double red_v[4];
@@ -51,10 +97,9 @@ __global__ __launch_bounds__(256) void k_ladder(LadderArgs a) {
   __shared__ double red[5 * 8];
   const int n = a.n, tid = threadIdx.x;
   const int lf = blockIdx.x, io = a.f0 + lf + 1;
-  double2 *sa = GLOBAL ? a.T + (size_t)lf * n * n : lds_dyn;
-  double2 *tab = lds_dyn + (GLOBAL ? 0 : (size_t)n * n);
+  double2 *sa = lds_dyn;
+  double2 *tab = lds_dyn + (size_t)n * n;
   double2 *l0 = tab, *y = tab + n, *fcol = tab + 3 * n;  // y: [2][n]
-  int *piv = (int *)(tab + 4 * n);
   const double o = a.omin + (double)(io - 1) * a.dw;
   for (int k = tid; k < n; k += 256) l0[k] = c_div(make_double2(1.0, 0.0), make_double2(o + a.delta[k], a.eta));
   __syncthreads();
@@ -77,32 +122,342 @@ __global__ __launch_bounds__(256) void k_ladder(LadderArgs a) {
     sa[x] = make_double2((k == k2 ? 1.0 : 0.0) - t.x, -t.y);
   }
   __syncthreads();
-  (void)piv;
-  gauss_solve<GLOBAL>(sa, n, y, 2, fcol, red_v, red_i, a.status);
-  // chi_ladd(pol) = -valley j^T fock^-1 y / NV^2 (z = fock^-1 y now in y),
-  // chi_para from the diagonal
-  double acc[8];
-#pragma unroll
-  for (int e = 0; e < 8; ++e) acc[e] = 0.0;
+  gauss_solve<false>(sa, n, y, 2, fcol, red_v, red_i, a.status);
+  ladder_close(a.cur, a.delta, y, n, o, a.eta, a.nv, a.valley, io, a.chi_ladd, a.chi_para, red);
+}
+
+// ---------------------------------------------------------------------------
+// blocked form (more than 88 k-points)
+
+struct LbArgs {
+  int n, ld, f0;
+  double omin, dw, eta, nv, valley;
+  const double2 *F;      // [k][k'] Fock kernel (row-major)
+  const double *delta;   // [n]
+  const double2 *cur;    // [2][n]
+  double2 *T;            // [frequency in chunk][n][ld]: fock | y0 | y1 | 0
+  int *act;              // [frequency][n] rows not yet chosen as pivots, in row order
+  int *piv;              // [frequency][n] pivot row of each column
+  double2 *chi_ladd;     // [io][2]
+  double2 *chi_para;     // [io][2]
+  int *status;
+};
+
+// fock = I - L0 F / NV and y = L0 F (L0 conj(j)): rows k0 .. k0 + 15 of one
+// frequency per workgroup (blockIdx.y the frequency), one wave per row
+constexpr int kLbFormRows = 16;
+__global__ __launch_bounds__(256) void k_lb_form(LbArgs a) {
+  extern __shared__ double2 lds_dyn[];  // L0 conj(j) [2][n]
+  const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lf = blockIdx.y, io = a.f0 + lf + 1, k0 = blockIdx.x * kLbFormRows;
+  double2 *T = a.T + (size_t)lf * n * ld;
+  double2 *lc = lds_dyn;
+  const double o = a.omin + (double)(io - 1) * a.dw;
   for (int k = tid; k < n; k += 256) {
-    const double2 z0 = y[k], z1 = y[n + k];
-    const double2 j0 = a.cur[k], j1 = a.cur[n + k];
-    const double2 t0 = c_mul(j0, z0), t1 = c_mul(j1, z1);
-    acc[0] += t0.x, acc[1] += t0.y, acc[2] += t1.x, acc[3] += t1.y;
-    const double2 lp = c_scale(l0[k], o / a.delta[k]);  // (o / Delta) L0
-    const double2 p0 = c_mul(c_mul(j0, lp), c_conj(j0)), p1 = c_mul(c_mul(j1, lp), c_conj(j1));
-    acc[4] += p0.x, acc[5] += p0.y, acc[6] += p1.x, acc[7] += p1.y;
+    const double2 l = c_div(make_double2(1.0, 0.0), make_double2(o + a.delta[k], a.eta));
+    lc[k] = c_mul(l, c_conj(a.cur[k]));
+    lc[n + k] = c_mul(l, c_conj(a.cur[n + k]));
   }
-  block_sum<8>(acc, red);
-  if (tid == 0) {
-    const double *r = red + 32;
-    const double cl = -a.valley * inv_nv * inv_nv, cp = a.valley * inv_nv;
-    double2 *ol = a.chi_ladd + 2 * (size_t)(io - 1), *op = a.chi_para + 2 * (size_t)(io - 1);
-    ol[0] = make_double2(cl * r[0], cl * r[1]);
-    ol[1] = make_double2(cl * r[2], cl * r[3]);
-    op[0] = make_double2(cp * r[4], cp * r[5]);
-    op[1] = make_double2(cp * r[6], cp * r[7]);
+  if (blockIdx.x == 0)
+    for (int k = tid; k < n; k += 256) a.act[(size_t)lf * n + k] = k;
+  __syncthreads();
+  const double inv_nv = 1.0 / a.nv;
+  for (int k = k0 + wave; k < min(k0 + kLbFormRows, n); k += 4) {
+    const double2 *row = a.F + (size_t)k * n;
+    double2 *out = T + (size_t)k * ld;
+    const double2 lk = c_div(make_double2(1.0, 0.0), make_double2(o + a.delta[k], a.eta));
+    double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
+    for (int k2 = lane; k2 < n; k2 += 64) {
+      const double2 f = row[k2];
+      s0 = c_add(s0, c_mul(f, lc[k2]));
+      s1 = c_add(s1, c_mul(f, lc[n + k2]));
+      const double2 t = c_scale(c_mul(lk, f), inv_nv);
+      out[k2] = make_double2((k == k2 ? 1.0 : 0.0) - t.x, -t.y);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      s0.x += __shfl_xor(s0.x, off);
+      s0.y += __shfl_xor(s0.y, off);
+      s1.x += __shfl_xor(s1.x, off);
+      s1.y += __shfl_xor(s1.y, off);
+    }
+    if (lane == 0) {
+      out[n] = c_mul(lk, s0);
+      out[n + 1] = c_mul(lk, s1);
+    }
+    if (lane >= 2 && n + lane < ld) out[n + lane] = make_double2(0.0, 0.0);
   }
+}
+
+// One panel p0 .. p0 + w - 1 (w = min(NB, n - p0)) of every frequency: NT
+// threads, thread t holds active row act[t] (n - p0 <= NT).
+template <int NB, int NT>
+__global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
+  constexpr int NW = NT / 64;
+  // per step parity: each wave's best row (value, row index, its panel values)
+  __shared__ double2 s_cand[2][NW][NB];
+  __shared__ double s_rv[2][NW];
+  __shared__ int s_ri[2][NW];
+  __shared__ double2 s_l11[NB][NB];
+  __shared__ int s_piv[NB];
+  __shared__ int s_cnt[NW];
+  const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lf = blockIdx.x;
+  const int m = n - p0, w = min(NB, m);
+  double2 *T = a.T + (size_t)lf * n * ld;
+  int *act = a.act + (size_t)lf * n;
+  const bool mine = tid < m;
+  const int row = mine ? act[tid] : 0;
+  bool live = mine;
+  double2 v[NB];
+  {
+    const double2 *src = T + (size_t)row * ld + p0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      const double2 x = src[min(c, w - 1)];
+      v[c] = (mine && c < w) ? x : make_double2(0.0, 0.0);
+    }
+  }
+  // One barrier per pivot step: the waves publish their candidates in the
+  // slot of the step's parity, so a step's writes never meet the reads of
+  // the step before (every thread has left it at the barrier in between).
+#pragma clang loop unroll(full)  // v[] stays in registers
+  for (int j = 0; j < NB; ++j) {
+    if (j < w) {  // uniform over the workgroup
+      const int ph = j & 1;
+      double best = -1.0;
+      int bi = INT_MAX;
+      if (live) {
+        const double mm = fabs(v[j].x) + fabs(v[j].y);
+        best = (mm >= 0.0) ? mm : 0.0;  // a NaN entry competes as zero
+        bi = row;
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (live && row == bi) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) s_cand[ph][wave][c] = v[c];
+      }
+      if (lane == 0) {
+        s_rv[ph][wave] = best;
+        s_ri[ph][wave] = bi;
+      }
+      __syncthreads();
+      double pv = s_rv[ph][0];
+      int p = s_ri[ph][0], qw = 0;
+#pragma unroll
+      for (int q = 1; q < NW; ++q)
+        if (s_rv[ph][q] > pv || (s_rv[ph][q] == pv && s_ri[ph][q] < p)) {
+          pv = s_rv[ph][q];
+          p = s_ri[ph][q];
+          qw = q;
+        }
+      const double2 *prow = s_cand[ph][qw];
+      if (live && row == p) {
+#pragma unroll
+        for (int c = 0; c < j; ++c) s_l11[j][c] = v[c];
+        s_piv[j] = p;
+        live = false;
+      }
+      if (tid == 0 && !(pv > 0.0)) atomicOr(a.status, 1);
+      if (live) {
+        const double2 l = c_mul(v[j], c_div(make_double2(1.0, 0.0), prow[j]));
+        v[j] = l;
+#pragma unroll
+        for (int c = j + 1; c < NB; ++c) v[c] = c_sub(v[c], c_mul(l, prow[c]));
+      }
+    }
+  }
+  // the panel in place: pivot rows hold their U11 row (and L11 multipliers
+  // left of it), the other rows their L21 multipliers
+  if (mine) {
+    double2 *dst = T + (size_t)row * ld + p0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c < w) dst[c] = v[c];
+  }
+  // rows still active, in row-list order
+  const unsigned long long bal = __ballot(live);
+  if (lane == 0) s_cnt[wave] = __popcll(bal);
+  __syncthreads();  // also: every thread has read its act entry and s_l11 / s_piv are complete
+  int off = __popcll(bal & ((1ull << lane) - 1ull));
+  for (int q = 0; q < wave; ++q) off += s_cnt[q];
+  if (live) act[off] = row;
+  if (tid < w) a.piv[(size_t)lf * n + p0 + tid] = s_piv[tid];
+  // U12 = L11^-1 A12 on the pivot rows, one column per thread (the host
+  // launches NT >= ld - NB threads)
+  const int c = p0 + w + tid;
+  if (c < ld) {
+    double2 x[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) x[j] = (j < w) ? T[(size_t)s_piv[j] * ld + c] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int j = 1; j < NB; ++j) {
+      if (j < w) {
+#pragma unroll
+        for (int i = 0; i < j; ++i) x[j] = c_sub(x[j], c_mul(s_l11[j][i], x[i]));
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      if (j < w) T[(size_t)s_piv[j] * ld + c] = x[j];
+  }
+}
+
+// Trailing update after panel p0 (width w): for the m = n - p0 - w active
+// rows r and columns c in p0 + w .. ld - 1,
+//   T[r][c] -= sum_{j<w} T[r][p0 + j] T[piv(p0 + j)][c]
+// 64 x 64 complex tiles, frequency in blockIdx.z; each wave 32 x 32 as
+// 2 x 2 blocks of 16 x 16, four real products per complex one.  The L and
+// U operands pass through LDS in steps of 16 (34 KB, four workgroups per
+// compute unit); the tile of T itself goes straight to the accumulators.
+template <int NB>
+__global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
+  constexpr int KH = NB < 16 ? NB : 16;
+  __shared__ double2 sL[64][KH + 1];
+  __shared__ double2 sU[KH][65];
+  __shared__ int s_row[64];
+  __shared__ int s_pr[NB];
+  const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lf = blockIdx.z;
+  const int m = n - p0 - w, c0 = p0 + w + blockIdx.x * 64, r0 = blockIdx.y * 64;
+  double2 *T = a.T + (size_t)lf * n * ld;
+  const int *act = a.act + (size_t)lf * n, *piv = a.piv + (size_t)lf * n + p0;
+  const int r = lane & 15, q4 = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  if (tid < 64) s_row[tid] = act[min(r0 + tid, m - 1)];
+  if (tid < NB) s_pr[tid] = piv[min(tid, w - 1)];
+  __syncthreads();
+  rpa_d4 ar[2][2], ai[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int rr = wm + 16 * i + 4 * e + q4, cc = min(c0 + wn + 16 * j + r, ld - 1);
+        const double2 cv = T[(size_t)s_row[rr] * ld + cc];
+        ar[i][j][e] = cv.x;
+        ai[i][j][e] = cv.y;
+      }
+#pragma unroll
+  for (int h = 0; h < NB / KH; ++h) {
+    if (h > 0) __syncthreads();  // the previous step's operand reads are done
+    for (int x = tid; x < 64 * KH; x += 256) {
+      const int rr = x / KH, k = x - rr * KH, kg = h * KH + k;
+      const double2 l = T[(size_t)s_row[rr] * ld + p0 + min(kg, w - 1)];
+      sL[rr][k] = (kg < w) ? l : make_double2(0.0, 0.0);
+    }
+    for (int x = tid; x < KH * 64; x += 256) {
+      const int k = x >> 6, cc = x & 63, kg = h * KH + k;
+      const double2 u = T[(size_t)s_pr[kg] * ld + min(c0 + cc, ld - 1)];
+      sU[k][cc] = (kg < w) ? u : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KH / 4; ++kk) {
+      const int k = 4 * kk + q4;
+      double2 lv[2], uv[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) lv[i] = sL[wm + 16 * i + r][k];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) uv[j] = sU[k][wn + 16 * j + r];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          ar[i][j] = mfma_f64_16x16x4(-lv[i].x, uv[j].x, ar[i][j]);  // -= Lr Ur
+          ar[i][j] = mfma_f64_16x16x4(lv[i].y, uv[j].y, ar[i][j]);   // += Li Ui
+          ai[i][j] = mfma_f64_16x16x4(-lv[i].x, uv[j].y, ai[i][j]);  // -= Lr Ui
+          ai[i][j] = mfma_f64_16x16x4(-lv[i].y, uv[j].x, ai[i][j]);  // -= Li Ur
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int rr = wm + 16 * i + 4 * e + q4, cc = c0 + wn + 16 * j + r;
+        if (r0 + rr < m && cc < ld) T[(size_t)s_row[rr] * ld + cc] = make_double2(ar[i][j][e], ai[i][j][e]);
+      }
+}
+
+// Back substitution z_j = (y(P_j) - sum_{c>j} U(P_j, c) z_c) / U(P_j, j) in
+// blocks of NB pivots from the last, then chi_ladd and chi_para.
+template <int NB>
+__global__ __launch_bounds__(256) void k_lb_back(LbArgs a) {
+  extern __shared__ double2 lds_dyn[];  // z [2][n], pivot rows [n] (int)
+  __shared__ double2 s_u[NB][NB + 1];
+  __shared__ double2 s_s[2][NB];
+  __shared__ double red[5 * 8];
+  const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lf = blockIdx.x, io = a.f0 + lf + 1;
+  const double2 *T = a.T + (size_t)lf * n * ld;
+  double2 *z = lds_dyn;
+  int *sp = (int *)(lds_dyn + 2 * (size_t)n);
+  for (int k = tid; k < n; k += 256) sp[k] = a.piv[(size_t)lf * n + k];
+  __syncthreads();
+  for (int jb = ((n - 1) / NB) * NB; jb >= 0; jb -= NB) {
+    const int je = min(jb + NB, n), bw = je - jb;
+    for (int jj = wave; jj < bw; jj += 4) {
+      const double2 *urow = T + (size_t)sp[jb + jj] * ld;
+      double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
+      for (int c = je + lane; c < n; c += 64) {
+        const double2 u = urow[c];
+        s0 = c_add(s0, c_mul(u, z[c]));
+        s1 = c_add(s1, c_mul(u, z[n + c]));
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        s0.x += __shfl_xor(s0.x, off);
+        s0.y += __shfl_xor(s0.y, off);
+        s1.x += __shfl_xor(s1.x, off);
+        s1.y += __shfl_xor(s1.y, off);
+      }
+      if (lane == 0) {
+        s_s[0][jj] = c_sub(urow[n], s0);
+        s_s[1][jj] = c_sub(urow[n + 1], s1);
+      }
+      if (lane < bw) s_u[jj][lane] = urow[jb + lane];
+    }
+    __syncthreads();
+    if (wave == 0) {  // lane = 32 rhs + row of the block
+      const int rr = lane >> 5, i = lane & 31;
+      double2 s = (i < bw) ? s_s[rr][i] : make_double2(0.0, 0.0), zi = make_double2(0.0, 0.0);
+      for (int j = bw - 1; j >= 0; --j) {
+        if (i == j) zi = c_div(s, s_u[i][i]);
+        const double2 zj = make_double2(__shfl(zi.x, 32 * rr + j), __shfl(zi.y, 32 * rr + j));
+        if (i < j) s = c_sub(s, c_mul(s_u[i][j], zj));
+      }
+      if (i < bw) z[(size_t)rr * n + jb + i] = zi;
+    }
+    __syncthreads();
+  }
+  const double o = a.omin + (double)(io - 1) * a.dw;
+  ladder_close(a.cur, a.delta, z, n, o, a.eta, a.nv, a.valley, io, a.chi_ladd, a.chi_para, red);
+}
+
+template <int NB, int NT>
+hipError_t ladder_blocked(hipStream_t st, LbArgs a, int nf) {
+  const int n = a.n, ld = a.ld;
+  if (n > NT || ld - NB > NT || n <= NB) return hipErrorInvalidValue;  // a row and a U12 column per thread
+  hipLaunchKernelGGL(k_lb_form, dim3((n + kLbFormRows - 1) / kLbFormRows, nf), dim3(256), sizeof(double2) * 2 * n,
+                     st, a);
+  for (int p0 = 0; p0 < n; p0 += NB) {
+    const int w = std::min(NB, n - p0), m = n - p0 - w;
+    hipLaunchKernelGGL((k_lb_panel<NB, NT>), dim3(nf), dim3(NT), 0, st, a, p0);
+    if (m > 0)
+      hipLaunchKernelGGL(k_lb_update<NB>, dim3((ld - p0 - w + 63) / 64, (m + 63) / 64, nf), dim3(256), 0, st, a,
+                         p0, w);
+  }
+  hipLaunchKernelGGL(k_lb_back<NB>, dim3(nf), dim3(256), sizeof(double2) * 2 * n + sizeof(int) * n, st, a);
+  return hipGetLastError();
 }
 
 // Fock kernel of Chi_ladd (Chi_ladd_up/BSE.f90:804-816): one wave per pair
@@ -225,10 +580,16 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
   if (!(p->ntot > 0) || !(p->vcell > 0.0)) return err(RPA_E_ARG, "Ntot and Vcell must be positive");
   for (int k = 0; k < n; ++k)
     if (delta[k] == 0.0) return err(RPA_E_ARG, "a zero Delta(k) (chi_para divides by it)");
-  const bool global = n > kLdLdsK;
+  const bool blocked = n > kLdLdsK;
   const size_t nn = (size_t)n * n;
-  // GLOBAL form: at most 1 GiB of per-frequency fock slices per launch
-  const int chunk = global ? (int)std::min<size_t>(no, std::max<size_t>(1, ((size_t)1 << 30) / (16 * nn))) : no;
+  // blocked form: fock | y per frequency in rows of ld (a multiple of 4)
+  // complex values, at most 4 GiB of slices per chunk of frequencies
+  const int ld = (n + 2 + 3) & ~3;
+  const size_t slice = (size_t)n * ld;
+  const int chunk = blocked ? (int)std::min<size_t>(
+                                  std::min<size_t>(no, 65535),
+                                  std::max<size_t>(1, ((size_t)4 << 30) / (sizeof(double2) * slice)))
+                            : no;
   hipError_t e = hipSetDevice(p->device);
   if (e != hipSuccess) return err(RPA_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
   // FockK(nkbz, nkbz) column-major -> [k][k'] rows; current(2, nkbz) -> [pol][k]
@@ -242,14 +603,14 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
   hipStream_t st = nullptr;
   double2 *dF = nullptr, *dc = nullptr, *dT = nullptr, *dl = nullptr, *dp = nullptr;
   double *dd = nullptr;
-  int *ds = nullptr;
+  int *ds = nullptr, *dact = nullptr, *dpiv = nullptr;
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
-    for (void *q : {(void *)dF, (void *)dc, (void *)dT, (void *)dl, (void *)dp, (void *)dd, (void *)ds})
+    for (void *q : {(void *)dF, (void *)dc, (void *)dT, (void *)dl, (void *)dp, (void *)dd, (void *)ds, (void *)dact,
+                    (void *)dpiv})
       if (q) (void)hipFree(q);
     if (st) (void)hipStreamDestroy(st);
   };
-  const size_t lds = ladder_lds_bytes(n, global);
   if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipMalloc((void **)&dF, sizeof(double2) * nn)) != hipSuccess ||
       (e = hipMalloc((void **)&dc, sizeof(double2) * 2 * n)) != hipSuccess ||
@@ -257,41 +618,65 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
       (e = hipMalloc((void **)&dl, sizeof(double2) * 2 * (size_t)no)) != hipSuccess ||
       (e = hipMalloc((void **)&dp, sizeof(double2) * 2 * (size_t)no)) != hipSuccess ||
       (e = hipMalloc((void **)&ds, sizeof(int))) != hipSuccess ||
-      (global && (e = hipMalloc((void **)&dT, sizeof(double2) * nn * chunk)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dT, sizeof(double2) * slice * chunk)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dact, sizeof(int) * (size_t)n * chunk)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dpiv, sizeof(int) * (size_t)n * chunk)) != hipSuccess) ||
       (e = hipMemcpyAsync(dF, hF.data(), sizeof(double) * hF.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(dc, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(dd, delta, sizeof(double) * n, hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemsetAsync(ds, 0, sizeof(int), st)) != hipSuccess ||
-      (e = hipFuncSetAttribute(global ? (const void *)k_ladder<true> : (const void *)k_ladder<false>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)) != hipSuccess) {
+      (!blocked && (e = hipFuncSetAttribute((const void *)k_ladder, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)ladder_lds_bytes(n))) != hipSuccess)) {
     cleanup();
     return err(RPA_E_HIP, hipGetErrorString(e));
   }
-  LadderArgs a{};
-  a.n = n;
-  a.no = no;
-  a.omin = p->omin;
-  a.dw = p->domega;
-  a.eta = p->eta;
-  a.nv = (double)p->ntot * p->vcell;
-  a.valley = p->valley;
-  a.F = dF;
-  a.delta = dd;
-  a.cur = dc;
-  a.T = dT;
-  a.chi_ladd = dl;
-  a.chi_para = dp;
-  a.status = ds;
+  const double nv = (double)p->ntot * p->vcell;
   for (int f0 = 0; f0 < no; f0 += chunk) {
-    a.f0 = f0;
     const int nf = std::min(chunk, no - f0);
-    if (global)
-      hipLaunchKernelGGL(k_ladder<true>, dim3(nf), dim3(256), lds, st, a);
-    else
-      hipLaunchKernelGGL(k_ladder<false>, dim3(nf), dim3(256), lds, st, a);
-    if ((e = hipGetLastError()) != hipSuccess) {
+    if (blocked) {
+      LbArgs b{};
+      b.n = n;
+      b.ld = ld;
+      b.f0 = f0;
+      b.omin = p->omin;
+      b.dw = p->domega;
+      b.eta = p->eta;
+      b.nv = nv;
+      b.valley = p->valley;
+      b.F = dF;
+      b.delta = dd;
+      b.cur = dc;
+      b.T = dT;
+      b.act = dact;
+      b.piv = dpiv;
+      b.chi_ladd = dl;
+      b.chi_para = dp;
+      b.status = ds;
+      // 32-column panels with 512 threads; past 512 k-points 16 columns with
+      // 1024 threads (one active row per thread)
+      e = n <= 512 ? ladder_blocked<32, 512>(st, b, nf) : ladder_blocked<16, 1024>(st, b, nf);
+    } else {
+      LadderArgs a{};
+      a.n = n;
+      a.no = no;
+      a.f0 = f0;
+      a.omin = p->omin;
+      a.dw = p->domega;
+      a.eta = p->eta;
+      a.nv = nv;
+      a.valley = p->valley;
+      a.F = dF;
+      a.delta = dd;
+      a.cur = dc;
+      a.chi_ladd = dl;
+      a.chi_para = dp;
+      a.status = ds;
+      hipLaunchKernelGGL(k_ladder, dim3(nf), dim3(256), ladder_lds_bytes(n), st, a);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
       cleanup();
-      return err(RPA_E_HIP, std::string("k_ladder launch: ") + hipGetErrorString(e));
+      return err(RPA_E_HIP, std::string("ladder launch: ") + hipGetErrorString(e));
     }
   }
   int status = 0;

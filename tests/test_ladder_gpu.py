@@ -1,8 +1,11 @@
 """The ladder (BSE) frequency loop of Chi_ladd on the GPU (rpa_ladder)
 against the oracle (orc_ladder, Chi_ladd_up/BSE.f90:1116-1176 with the
 reference's gjel and its explicit lladd product).  The device solves
-fock z = xi0 conj(j) through a Gauss-Jordan inverse instead of forming
-lladd: 1e-11 of the largest value."""
+fock z = xi0 conj(j) by Gaussian elimination with row pivoting instead of
+forming lladd: in LDS up to 88 k-points, blocked past that (32-column panels
+up to 512 k-points, 16-column panels with a row per thread of 1024 above,
+the trailing updates on the FP64 matrix cores): 1e-11 of the largest value.
+"""
 
 import importlib
 
@@ -16,11 +19,14 @@ pytestmark = pytest.mark.gpu
 abi = importlib.import_module("2dmaterialsopticalprops_amd._abi")
 
 
-@pytest.mark.parametrize("nk", [1, 6, 40, 88, 150])
+# 89: one full panel and a 25-row tail panel; 300: nine panels and a
+# 12-column tail; 512: the largest 32-column case (ld = 516, U12 columns 484
+# of 512 threads); 600: 16-column panels, 1024 threads
+@pytest.mark.parametrize("nk", [1, 6, 40, 88, 89, 150, 300, 512, 600])
 def test_ladder_matches_oracle(nk):
     print(f"[ladder] nk={nk}", flush=True)
     rng = np.random.default_rng(nk)
-    no, ntot, vcell, valley = 9, 4 * nk, 3.0, 2.0
+    no, ntot, vcell, valley = (9 if nk <= 150 else 2 if nk <= 300 else 1), 4 * nk, 3.0, 2.0
     omin, dw, eta = 0.05, 0.04, 0.01
     nv = ntot * vcell
     F = np.asfortranarray(0.3 * nv / np.sqrt(nk) * (rng.standard_normal((nk, nk))
@@ -32,6 +38,22 @@ def test_ladder_matches_oracle(nk):
     assert np.isfinite(cl).all() and np.isfinite(cp).all()
     assert H.rel_max(cl, wl) < 1e-11
     assert H.rel_max(cp, wp) < 1e-13
+
+
+@pytest.mark.parametrize("nk", [40, 150])
+def test_ladder_nan_kernel_reports_singular(nk):
+    """A row of NaN in the Fock kernel leaves no usable pivot for its column
+    (the reference's gjel stops with "singular matrix"): RPA_E_SINGULAR, in
+    the LDS form and in the blocked one."""
+    print(f"[ladder] NaN row nk={nk}", flush=True)
+    rng = np.random.default_rng(3)
+    F = np.asfortranarray(rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk)))
+    F[nk // 2, :] = np.nan
+    delta = rng.uniform(0.4, 2.0, nk)
+    cur = np.ones((2, nk), dtype=complex)
+    with pytest.raises(abi.RpaError) as ei:
+        abi.ladder(F, delta, cur, 3, 0.05, 0.04, 0.01, 4 * nk, 3.0, 2.0)
+    assert ei.value.code == abi.RPA_E_SINGULAR
 
 
 def test_ladder_argument_checks():
