@@ -194,10 +194,10 @@ __global__ __launch_bounds__(256) void k_lb_form(LbArgs a) {
 template <int NB, int NT>
 __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
   constexpr int NW = NT / 64;
-  // per step parity: each wave's best row (value, row index, its panel values)
-  __shared__ double2 s_cand[2][NW][NB];
-  __shared__ double s_rv[2][NW];
-  __shared__ int s_ri[2][NW];
+  __shared__ double2 s_prow[NB];  // the pivot row of the step
+  __shared__ double2 s_inv;       // 1 / pivot
+  __shared__ double s_rv[NW];     // each wave's best row: |Re| + |Im| and row index
+  __shared__ int s_ri[NW];
   __shared__ double2 s_l11[NB][NB];
   __shared__ int s_piv[NB];
   __shared__ int s_cnt[NW];
@@ -217,13 +217,12 @@ __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
       v[c] = (mine && c < w) ? x : make_double2(0.0, 0.0);
     }
   }
-  // One barrier per pivot step: the waves publish their candidates in the
-  // slot of the step's parity, so a step's writes never meet the reads of
-  // the step before (every thread has left it at the barrier in between).
-#pragma clang loop unroll(full)  // v[] stays in registers
+  // Per pivot step: the waves' best rows meet in LDS (barrier), the pivot
+  // row's owner publishes its values and 1 / pivot (barrier), the active
+  // rows eliminate.  Fully unrolled so that v[] stays in registers.
+#pragma clang loop unroll(full)
   for (int j = 0; j < NB; ++j) {
     if (j < w) {  // uniform over the workgroup
-      const int ph = j & 1;
       double best = -1.0;
       int bi = INT_MAX;
       if (live) {
@@ -240,37 +239,35 @@ __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
           bi = oi;
         }
       }
-      if (live && row == bi) {
-#pragma unroll
-        for (int c = 0; c < NB; ++c) s_cand[ph][wave][c] = v[c];
-      }
       if (lane == 0) {
-        s_rv[ph][wave] = best;
-        s_ri[ph][wave] = bi;
+        s_rv[wave] = best;
+        s_ri[wave] = bi;
       }
       __syncthreads();
-      double pv = s_rv[ph][0];
-      int p = s_ri[ph][0], qw = 0;
+      double pv = s_rv[0];
+      int p = s_ri[0];
 #pragma unroll
       for (int q = 1; q < NW; ++q)
-        if (s_rv[ph][q] > pv || (s_rv[ph][q] == pv && s_ri[ph][q] < p)) {
-          pv = s_rv[ph][q];
-          p = s_ri[ph][q];
-          qw = q;
+        if (s_rv[q] > pv || (s_rv[q] == pv && s_ri[q] < p)) {
+          pv = s_rv[q];
+          p = s_ri[q];
         }
-      const double2 *prow = s_cand[ph][qw];
       if (live && row == p) {
 #pragma unroll
+        for (int c = j; c < NB; ++c) s_prow[c] = v[c];
+#pragma unroll
         for (int c = 0; c < j; ++c) s_l11[j][c] = v[c];
+        s_inv = c_div(make_double2(1.0, 0.0), v[j]);
         s_piv[j] = p;
         live = false;
       }
       if (tid == 0 && !(pv > 0.0)) atomicOr(a.status, 1);
+      __syncthreads();
       if (live) {
-        const double2 l = c_mul(v[j], c_div(make_double2(1.0, 0.0), prow[j]));
+        const double2 l = c_mul(v[j], s_inv);
         v[j] = l;
 #pragma unroll
-        for (int c = j + 1; c < NB; ++c) v[c] = c_sub(v[c], c_mul(l, prow[c]));
+        for (int c = j + 1; c < NB; ++c) v[c] = c_sub(v[c], c_mul(l, s_prow[c]));
       }
     }
   }
@@ -389,53 +386,60 @@ __global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
 }
 
 // Back substitution z_j = (y(P_j) - sum_{c>j} U(P_j, c) z_c) / U(P_j, j) in
-// blocks of NB pivots from the last, then chi_ladd and chi_para.
+// blocks of NB pivots from the last, right-looking: the right-hand sides of
+// all pivots stay in LDS; once a block's z is solved (its triangle by one
+// wave with shuffles), every earlier pivot row subtracts that block's
+// columns (a thread per row, its NB values independent loads).  Then
+// chi_ladd and chi_para.
 template <int NB>
 __global__ __launch_bounds__(256) void k_lb_back(LbArgs a) {
-  extern __shared__ double2 lds_dyn[];  // z [2][n], pivot rows [n] (int)
+  extern __shared__ double2 lds_dyn[];  // z [2][n], rhs [2][n], pivot rows [n] (int)
   __shared__ double2 s_u[NB][NB + 1];
-  __shared__ double2 s_s[2][NB];
   __shared__ double red[5 * 8];
   const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lf = blockIdx.x, io = a.f0 + lf + 1;
   const double2 *T = a.T + (size_t)lf * n * ld;
-  double2 *z = lds_dyn;
-  int *sp = (int *)(lds_dyn + 2 * (size_t)n);
-  for (int k = tid; k < n; k += 256) sp[k] = a.piv[(size_t)lf * n + k];
+  double2 *z = lds_dyn, *rhs = lds_dyn + 2 * (size_t)n;
+  int *sp = (int *)(lds_dyn + 4 * (size_t)n);
+  for (int k = tid; k < n; k += 256) {
+    const int pr = a.piv[(size_t)lf * n + k];
+    sp[k] = pr;
+    rhs[k] = T[(size_t)pr * ld + n];
+    rhs[n + k] = T[(size_t)pr * ld + n + 1];
+  }
   __syncthreads();
   for (int jb = ((n - 1) / NB) * NB; jb >= 0; jb -= NB) {
     const int je = min(jb + NB, n), bw = je - jb;
-    for (int jj = wave; jj < bw; jj += 4) {
-      const double2 *urow = T + (size_t)sp[jb + jj] * ld;
-      double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
-      for (int c = je + lane; c < n; c += 64) {
-        const double2 u = urow[c];
-        s0 = c_add(s0, c_mul(u, z[c]));
-        s1 = c_add(s1, c_mul(u, z[n + c]));
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        s0.x += __shfl_xor(s0.x, off);
-        s0.y += __shfl_xor(s0.y, off);
-        s1.x += __shfl_xor(s1.x, off);
-        s1.y += __shfl_xor(s1.y, off);
-      }
-      if (lane == 0) {
-        s_s[0][jj] = c_sub(urow[n], s0);
-        s_s[1][jj] = c_sub(urow[n + 1], s1);
-      }
-      if (lane < bw) s_u[jj][lane] = urow[jb + lane];
+    for (int x = tid; x < bw * bw; x += 256) {
+      const int jj = x / bw, c = x - jj * bw;
+      s_u[jj][c] = T[(size_t)sp[jb + jj] * ld + jb + c];
     }
     __syncthreads();
     if (wave == 0) {  // lane = 32 rhs + row of the block
       const int rr = lane >> 5, i = lane & 31;
-      double2 s = (i < bw) ? s_s[rr][i] : make_double2(0.0, 0.0), zi = make_double2(0.0, 0.0);
+      double2 sv = (i < bw) ? rhs[(size_t)rr * n + jb + i] : make_double2(0.0, 0.0), zi = make_double2(0.0, 0.0);
       for (int j = bw - 1; j >= 0; --j) {
-        if (i == j) zi = c_div(s, s_u[i][i]);
+        if (i == j) zi = c_div(sv, s_u[i][i]);
         const double2 zj = make_double2(__shfl(zi.x, 32 * rr + j), __shfl(zi.y, 32 * rr + j));
-        if (i < j) s = c_sub(s, c_mul(s_u[i][j], zj));
+        if (i < j) sv = c_sub(sv, c_mul(s_u[i][j], zj));
       }
       if (i < bw) z[(size_t)rr * n + jb + i] = zi;
+    }
+    __syncthreads();
+    for (int j = tid; j < jb; j += 256) {
+      const double2 *urow = T + (size_t)sp[j] * ld + jb;
+      double2 u[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) u[c] = urow[min(c, bw - 1)];
+      double2 s0 = make_double2(0.0, 0.0), s1 = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < bw) {
+          s0 = c_add(s0, c_mul(u[c], z[jb + c]));
+          s1 = c_add(s1, c_mul(u[c], z[n + jb + c]));
+        }
+      rhs[j] = c_sub(rhs[j], s0);
+      rhs[n + j] = c_sub(rhs[n + j], s1);
     }
     __syncthreads();
   }
@@ -456,7 +460,11 @@ hipError_t ladder_blocked(hipStream_t st, LbArgs a, int nf) {
       hipLaunchKernelGGL(k_lb_update<NB>, dim3((ld - p0 - w + 63) / 64, (m + 63) / 64, nf), dim3(256), 0, st, a,
                          p0, w);
   }
-  hipLaunchKernelGGL(k_lb_back<NB>, dim3(nf), dim3(256), sizeof(double2) * 2 * n + sizeof(int) * n, st, a);
+  const size_t back_lds = sizeof(double2) * 4 * n + sizeof(int) * n;  // 68 KB at 1024 k-points
+  hipError_t e = hipFuncSetAttribute((const void *)k_lb_back<NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)back_lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lb_back<NB>, dim3(nf), dim3(256), back_lds, st, a);
   return hipGetLastError();
 }
 
@@ -586,6 +594,10 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
   // complex values, at most 4 GiB of slices per chunk of frequencies
   const int ld = (n + 2 + 3) & ~3;
   const size_t slice = (size_t)n * ld;
+  // The blocked solve runs as two independent halves of each chunk's
+  // frequencies on two streams: one half's panels (a serial pivot chain,
+  // one workgroup per compute unit) overlap the other half's trailing
+  // updates (HBM-bound).  Each half keeps its own region of the slices.
   const int chunk = blocked ? (int)std::min<size_t>(
                                   std::min<size_t>(no, 65535),
                                   std::max<size_t>(1, ((size_t)4 << 30) / (sizeof(double2) * slice)))
@@ -600,33 +612,47 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
   for (int pol = 0; pol < 2; ++pol)
     for (int k = 0; k < n; ++k)
       for (int r = 0; r < 2; ++r) hc[2 * ((size_t)pol * n + k) + r] = current[2 * ((size_t)pol + 2 * (size_t)k) + r];
-  hipStream_t st = nullptr;
+  const int half = (chunk + 1) / 2;
+  hipStream_t st = nullptr, st2 = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   double2 *dF = nullptr, *dc = nullptr, *dT = nullptr, *dl = nullptr, *dp = nullptr;
   double *dd = nullptr;
   int *ds = nullptr, *dact = nullptr, *dpiv = nullptr;
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
+    if (st2) (void)hipStreamSynchronize(st2);
     for (void *q : {(void *)dF, (void *)dc, (void *)dT, (void *)dl, (void *)dp, (void *)dd, (void *)ds, (void *)dact,
                     (void *)dpiv})
       if (q) (void)hipFree(q);
+    for (hipEvent_t ev : {ev_in, ev_out})
+      if (ev) (void)hipEventDestroy(ev);
     if (st) (void)hipStreamDestroy(st);
+    if (st2) (void)hipStreamDestroy(st2);
   };
   if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
+      (blocked && (e = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking)) != hipSuccess) ||
+      (blocked && (e = hipEventCreateWithFlags(&ev_in, hipEventDisableTiming)) != hipSuccess) ||
+      (blocked && (e = hipEventCreateWithFlags(&ev_out, hipEventDisableTiming)) != hipSuccess) ||
       (e = hipMalloc((void **)&dF, sizeof(double2) * nn)) != hipSuccess ||
       (e = hipMalloc((void **)&dc, sizeof(double2) * 2 * n)) != hipSuccess ||
       (e = hipMalloc((void **)&dd, sizeof(double) * n)) != hipSuccess ||
       (e = hipMalloc((void **)&dl, sizeof(double2) * 2 * (size_t)no)) != hipSuccess ||
       (e = hipMalloc((void **)&dp, sizeof(double2) * 2 * (size_t)no)) != hipSuccess ||
       (e = hipMalloc((void **)&ds, sizeof(int))) != hipSuccess ||
-      (blocked && (e = hipMalloc((void **)&dT, sizeof(double2) * slice * chunk)) != hipSuccess) ||
-      (blocked && (e = hipMalloc((void **)&dact, sizeof(int) * (size_t)n * chunk)) != hipSuccess) ||
-      (blocked && (e = hipMalloc((void **)&dpiv, sizeof(int) * (size_t)n * chunk)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dT, sizeof(double2) * slice * 2 * half)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dact, sizeof(int) * (size_t)n * 2 * half)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dpiv, sizeof(int) * (size_t)n * 2 * half)) != hipSuccess) ||
       (e = hipMemcpyAsync(dF, hF.data(), sizeof(double) * hF.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(dc, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(dd, delta, sizeof(double) * n, hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemsetAsync(ds, 0, sizeof(int), st)) != hipSuccess ||
       (!blocked && (e = hipFuncSetAttribute((const void *)k_ladder, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)ladder_lds_bytes(n))) != hipSuccess)) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
+  }
+  // the second stream starts after the inputs are up
+  if (blocked && ((e = hipEventRecord(ev_in, st)) != hipSuccess || (e = hipStreamWaitEvent(st2, ev_in, 0)) != hipSuccess)) {
     cleanup();
     return err(RPA_E_HIP, hipGetErrorString(e));
   }
@@ -653,8 +679,22 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
       b.chi_para = dp;
       b.status = ds;
       // 32-column panels with 512 threads; past 512 k-points 16 columns with
-      // 1024 threads (one active row per thread)
-      e = n <= 512 ? ladder_blocked<32, 512>(st, b, nf) : ladder_blocked<16, 1024>(st, b, nf);
+      // 1024 threads (one active row per thread).  Frequencies f0 .. f0 + h - 1
+      // on the first stream in region [0, half), the rest on the second in
+      // region [half, 2 half).
+      auto run = [&](hipStream_t sq, LbArgs q, int cnt) {
+        return n <= 512 ? ladder_blocked<32, 512>(sq, q, cnt) : ladder_blocked<16, 1024>(sq, q, cnt);
+      };
+      const int h = std::min(nf, (nf + 1) / 2);
+      e = run(st, b, h);
+      if (e == hipSuccess && nf > h) {
+        LbArgs b2 = b;
+        b2.f0 = f0 + h;
+        b2.T = dT + slice * half;
+        b2.act = dact + (size_t)n * half;
+        b2.piv = dpiv + (size_t)n * half;
+        e = run(st2, b2, nf - h);
+      }
     } else {
       LadderArgs a{};
       a.n = n;
@@ -678,6 +718,10 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
       cleanup();
       return err(RPA_E_HIP, std::string("ladder launch: ") + hipGetErrorString(e));
     }
+  }
+  if (blocked && ((e = hipEventRecord(ev_out, st2)) != hipSuccess || (e = hipStreamWaitEvent(st, ev_out, 0)) != hipSuccess)) {
+    cleanup();
+    return err(RPA_E_HIP, hipGetErrorString(e));
   }
   int status = 0;
   if ((e = hipMemcpyAsync(chi_ladd, dl, sizeof(double2) * 2 * (size_t)no, hipMemcpyDeviceToHost, st)) !=

@@ -40,6 +40,29 @@ def test_ladder_matches_oracle(nk):
     assert H.rel_max(cp, wp) < 1e-13
 
 
+def test_ladder_chunks_and_streams_agree_bitwise():
+    """1024 k-points (16-column panels, a row per thread of 1024, the widest
+    U12 solve) and 260 frequencies: two chunks of slices (255 + 5), each
+    split over two streams.  Every frequency is solved on its own with fixed
+    summation orders, so each equals a call for that frequency alone, bit
+    for bit (omega values exact in binary)."""
+    nk, no = 1024, 260
+    print(f"[ladder] chunks nk={nk} no={no}", flush=True)
+    rng = np.random.default_rng(11)
+    ntot, vcell, valley = 4 * nk, 3.0, 2.0
+    nv = ntot * vcell
+    F = np.asfortranarray(0.3 * nv / np.sqrt(nk) * (rng.standard_normal((nk, nk))
+                                                    + 1j * rng.standard_normal((nk, nk))))
+    delta = rng.uniform(0.4, 2.0, nk)
+    cur = np.asfortranarray(rng.standard_normal((2, nk)) + 1j * rng.standard_normal((2, nk)))
+    omin, dw, eta = 0.0625, 0.03125, 0.01
+    cl, cp = abi.ladder(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley)
+    assert np.isfinite(cl).all() and np.isfinite(cp).all()
+    for k in (0, 127, 128, 254, 255, 259):
+        c1, p1 = abi.ladder(F, delta, cur, 1, omin + k * dw, dw, eta, ntot, vcell, valley)
+        assert np.array_equal(cl[:, k], c1[:, 0]) and np.array_equal(cp[:, k], p1[:, 0]), k
+
+
 @pytest.mark.parametrize("nk", [40, 150])
 def test_ladder_nan_kernel_reports_singular(nk):
     """A row of NaN in the Fock kernel leaves no usable pivot for its column
