@@ -24,12 +24,15 @@
 //                pivot rows for every column right of the panel;
 //   k_lb_update  the trailing rows lose L21 U12, a batched complex GEMM on
 //                the FP64 matrix cores (64 x 64 tiles, frequency in z);
-//   k_lb_back    back substitution over the pivot order in blocks of NB
-//                (the block's sums against the solved part by waves, its
-//                triangle by one wave), then chi_ladd and chi_para.
+//   k_lb_back    back substitution over the pivot order in blocks of NB,
+//                right-looking (a block's triangle by one wave, then every
+//                earlier pivot row loses the block's columns), then
+//                chi_ladd and chi_para.
 // The trailing block is read and written once per panel, n^3 / (3 NB)
 // complex values per frequency, where the unblocked solve streamed it once
-// per pivot through one compute unit.
+// per pivot through one compute unit.  The frequencies of a chunk run as two
+// independent halves on two streams, so one half's panels (a serial pivot
+// chain) overlap the other half's trailing updates.
 #include <algorithm>
 #include <climits>
 #include <cmath>
