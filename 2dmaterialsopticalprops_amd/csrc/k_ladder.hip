@@ -30,9 +30,9 @@
 //                chi_ladd and chi_para.
 // The trailing block is read and written once per panel, n^3 / (3 NB)
 // complex values per frequency, where the unblocked solve streamed it once
-// per pivot through one compute unit.  The frequencies of a chunk run as two
-// independent halves on two streams, so one half's panels (a serial pivot
-// chain) overlap the other half's trailing updates.
+// per pivot through one compute unit.  The frequencies of a chunk run as
+// two independent parts on two streams, so one part's panels (a serial
+// pivot chain) overlap the other's trailing updates.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -46,6 +46,7 @@ namespace rpa {
 namespace {
 constexpr int kLdMaxK = 1024;  // k-points of the kernel (fock is n x n)
 constexpr int kLdLdsK = 88;    // fock in LDS up to here (16 n^2 bytes)
+constexpr int kLbParts = 2;    // blocked form: frequency parts on their own streams (4 measured slower)
 }  // namespace
 
 struct LadderArgs {
@@ -597,10 +598,10 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
   // complex values, at most 4 GiB of slices per chunk of frequencies
   const int ld = (n + 2 + 3) & ~3;
   const size_t slice = (size_t)n * ld;
-  // The blocked solve runs as two independent halves of each chunk's
-  // frequencies on two streams: one half's panels (a serial pivot chain,
-  // one workgroup per compute unit) overlap the other half's trailing
-  // updates (HBM-bound).  Each half keeps its own region of the slices.
+  // The blocked solve runs as kLbParts independent parts of each chunk's
+  // frequencies on as many streams: some parts' panels (a serial pivot
+  // chain, one workgroup per compute unit) overlap the others' trailing
+  // updates (HBM-bound).  Each part keeps its own region of the slices.
   const int chunk = blocked ? (int)std::min<size_t>(
                                   std::min<size_t>(no, 65535),
                                   std::max<size_t>(1, ((size_t)4 << 30) / (sizeof(double2) * slice)))
@@ -615,36 +616,37 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
   for (int pol = 0; pol < 2; ++pol)
     for (int k = 0; k < n; ++k)
       for (int r = 0; r < 2; ++r) hc[2 * ((size_t)pol * n + k) + r] = current[2 * ((size_t)pol + 2 * (size_t)k) + r];
-  const int half = (chunk + 1) / 2;
-  hipStream_t st = nullptr, st2 = nullptr;
-  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  const int part = (chunk + kLbParts - 1) / kLbParts;  // region per stream, in slices
+  hipStream_t st = nullptr, sq[kLbParts] = {};  // sq[0] is st
+  hipEvent_t ev_in = nullptr, ev_out[kLbParts] = {};
   double2 *dF = nullptr, *dc = nullptr, *dT = nullptr, *dl = nullptr, *dp = nullptr;
   double *dd = nullptr;
   int *ds = nullptr, *dact = nullptr, *dpiv = nullptr;
   auto cleanup = [&]() {
     if (st) (void)hipStreamSynchronize(st);
-    if (st2) (void)hipStreamSynchronize(st2);
+    for (int i = 1; i < kLbParts; ++i)
+      if (sq[i]) (void)hipStreamSynchronize(sq[i]);
     for (void *q : {(void *)dF, (void *)dc, (void *)dT, (void *)dl, (void *)dp, (void *)dd, (void *)ds, (void *)dact,
                     (void *)dpiv})
       if (q) (void)hipFree(q);
-    for (hipEvent_t ev : {ev_in, ev_out})
-      if (ev) (void)hipEventDestroy(ev);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    for (int i = 0; i < kLbParts; ++i)
+      if (ev_out[i]) (void)hipEventDestroy(ev_out[i]);
     if (st) (void)hipStreamDestroy(st);
-    if (st2) (void)hipStreamDestroy(st2);
+    for (int i = 1; i < kLbParts; ++i)
+      if (sq[i]) (void)hipStreamDestroy(sq[i]);
   };
   if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess ||
-      (blocked && (e = hipStreamCreateWithFlags(&st2, hipStreamNonBlocking)) != hipSuccess) ||
       (blocked && (e = hipEventCreateWithFlags(&ev_in, hipEventDisableTiming)) != hipSuccess) ||
-      (blocked && (e = hipEventCreateWithFlags(&ev_out, hipEventDisableTiming)) != hipSuccess) ||
       (e = hipMalloc((void **)&dF, sizeof(double2) * nn)) != hipSuccess ||
       (e = hipMalloc((void **)&dc, sizeof(double2) * 2 * n)) != hipSuccess ||
       (e = hipMalloc((void **)&dd, sizeof(double) * n)) != hipSuccess ||
       (e = hipMalloc((void **)&dl, sizeof(double2) * 2 * (size_t)no)) != hipSuccess ||
       (e = hipMalloc((void **)&dp, sizeof(double2) * 2 * (size_t)no)) != hipSuccess ||
       (e = hipMalloc((void **)&ds, sizeof(int))) != hipSuccess ||
-      (blocked && (e = hipMalloc((void **)&dT, sizeof(double2) * slice * 2 * half)) != hipSuccess) ||
-      (blocked && (e = hipMalloc((void **)&dact, sizeof(int) * (size_t)n * 2 * half)) != hipSuccess) ||
-      (blocked && (e = hipMalloc((void **)&dpiv, sizeof(int) * (size_t)n * 2 * half)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dT, sizeof(double2) * slice * kLbParts * part)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dact, sizeof(int) * (size_t)n * kLbParts * part)) != hipSuccess) ||
+      (blocked && (e = hipMalloc((void **)&dpiv, sizeof(int) * (size_t)n * kLbParts * part)) != hipSuccess) ||
       (e = hipMemcpyAsync(dF, hF.data(), sizeof(double) * hF.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(dc, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice, st)) != hipSuccess ||
       (e = hipMemcpyAsync(dd, delta, sizeof(double) * n, hipMemcpyHostToDevice, st)) != hipSuccess ||
@@ -654,10 +656,18 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
     cleanup();
     return err(RPA_E_HIP, hipGetErrorString(e));
   }
-  // the second stream starts after the inputs are up
-  if (blocked && ((e = hipEventRecord(ev_in, st)) != hipSuccess || (e = hipStreamWaitEvent(st2, ev_in, 0)) != hipSuccess)) {
-    cleanup();
-    return err(RPA_E_HIP, hipGetErrorString(e));
+  // the other streams start after the inputs are up
+  if (blocked) {
+    sq[0] = st;
+    e = hipEventRecord(ev_in, st);
+    for (int i = 1; i < kLbParts && e == hipSuccess; ++i)
+      if ((e = hipStreamCreateWithFlags(&sq[i], hipStreamNonBlocking)) == hipSuccess &&
+          (e = hipEventCreateWithFlags(&ev_out[i], hipEventDisableTiming)) == hipSuccess)
+        e = hipStreamWaitEvent(sq[i], ev_in, 0);
+    if (e != hipSuccess) {
+      cleanup();
+      return err(RPA_E_HIP, hipGetErrorString(e));
+    }
   }
   const double nv = (double)p->ntot * p->vcell;
   for (int f0 = 0; f0 < no; f0 += chunk) {
@@ -682,21 +692,19 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
       b.chi_para = dp;
       b.status = ds;
       // 32-column panels with 512 threads; past 512 k-points 16 columns with
-      // 1024 threads (one active row per thread).  Frequencies f0 .. f0 + h - 1
-      // on the first stream in region [0, half), the rest on the second in
-      // region [half, 2 half).
-      auto run = [&](hipStream_t sq, LbArgs q, int cnt) {
-        return n <= 512 ? ladder_blocked<32, 512>(sq, q, cnt) : ladder_blocked<16, 1024>(sq, q, cnt);
-      };
-      const int h = std::min(nf, (nf + 1) / 2);
-      e = run(st, b, h);
-      if (e == hipSuccess && nf > h) {
-        LbArgs b2 = b;
-        b2.f0 = f0 + h;
-        b2.T = dT + slice * half;
-        b2.act = dact + (size_t)n * half;
-        b2.piv = dpiv + (size_t)n * half;
-        e = run(st2, b2, nf - h);
+      // 1024 threads (one active row per thread).  Part i (frequencies
+      // f0 + i q .. f0 + (i + 1) q - 1, q = ceil(nf / kLbParts) <= part) on
+      // stream i in region [i part, (i + 1) part).
+      const int q = (nf + kLbParts - 1) / kLbParts;
+      for (int i = 0; i < kLbParts && e == hipSuccess; ++i) {
+        const int lo = i * q, cnt = std::min(nf, lo + q) - lo;
+        if (cnt <= 0) break;
+        LbArgs bi = b;
+        bi.f0 = f0 + lo;
+        bi.T = dT + slice * part * i;
+        bi.act = dact + (size_t)n * part * i;
+        bi.piv = dpiv + (size_t)n * part * i;
+        e = n <= 512 ? ladder_blocked<32, 512>(sq[i], bi, cnt) : ladder_blocked<16, 1024>(sq[i], bi, cnt);
       }
     } else {
       LadderArgs a{};
@@ -722,9 +730,13 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
       return err(RPA_E_HIP, std::string("ladder launch: ") + hipGetErrorString(e));
     }
   }
-  if (blocked && ((e = hipEventRecord(ev_out, st2)) != hipSuccess || (e = hipStreamWaitEvent(st, ev_out, 0)) != hipSuccess)) {
-    cleanup();
-    return err(RPA_E_HIP, hipGetErrorString(e));
+  if (blocked) {
+    for (int i = 1; i < kLbParts && e == hipSuccess; ++i)
+      if ((e = hipEventRecord(ev_out[i], sq[i])) == hipSuccess) e = hipStreamWaitEvent(st, ev_out[i], 0);
+    if (e != hipSuccess) {
+      cleanup();
+      return err(RPA_E_HIP, hipGetErrorString(e));
+    }
   }
   int status = 0;
   if ((e = hipMemcpyAsync(chi_ladd, dl, sizeof(double2) * 2 * (size_t)no, hipMemcpyDeviceToHost, st)) !=
