@@ -475,9 +475,11 @@ hipError_t ladder_blocked(hipStream_t st, LbArgs a, int nf) {
 // Fock kernel of Chi_ladd (Chi_ladd_up/BSE.f90:804-816): one wave per pair
 // (k, k') of the kernel's k-points, fockk(k,k') = -sum_{G,G'} W(iq,G,G')
 // conj(M1(G)) M2(G') with iq = int(Gcar |k' - k|_xy / qmin) + 1, and on the
-// diagonal the Q = 0 term -Vcell Ntot ln(1 + alpha epsq) / alpha.  Lane j
-// takes columns G' = j, j + 64, ...: t = sum_G conj(M1(G)) W(iq,G,G'), then
-// the wave sums t M2(G').  Pairs whose iq falls outside 2..nq get 0 (the
+// diagonal the Q = 0 term -Vcell Ntot ln(1 + alpha epsq) / alpha.  Up to 64
+// local fields the lanes walk the W block contiguously over (G, G') and add
+// conj(M1(G)) W M2(G') (M1, M2 one value per lane, shared by shuffles);
+// beyond, lane j takes columns G' = j, j + 64, ...: t = sum_G conj(M1(G))
+// W(iq,G,G'), then t M2(G').  The wave sums the lanes.  Pairs whose iq falls outside 2..nq get 0 (the
 // reference reads W only for those and leaves such entries unset).
 __global__ __launch_bounds__(256) void k_fock(int nk, int nlf, int nq, double gcar, double qmin, double diag,
                                               const double *__restrict__ kbz, const double2 *__restrict__ W,
@@ -498,10 +500,27 @@ __global__ __launch_bounds__(256) void k_fock(int nk, int nlf, int nq, double gc
       const double2 *m = M + 2 * (size_t)nlf * pair;  // [ni][G] of the pair
       const double2 *w = W + (size_t)(iq - 1) * nlf * nlf;  // [G][G']
       double2 acc = make_double2(0.0, 0.0);
-      for (int j = lane; j < nlf; j += 64) {
-        double2 t = make_double2(0.0, 0.0);
-        for (int g = 0; g < nlf; ++g) t = c_add(t, c_mul(c_conj(m[2 * g]), w[(size_t)g * nlf + j]));
-        acc = c_add(acc, c_mul(t, m[2 * j + 1]));
+      if (nlf <= 64) {
+        // one load per lane for M1 and M2 (lane = G), the W block read
+        // contiguously over (G, G') with M1(G), M2(G') taken from their lanes
+        // by shuffles: 2 + nlf^2/64 load instructions per pair instead of
+        // 2 nlf, the address unit being what bounds this kernel
+        const double2 m1 = lane < nlf ? m[2 * lane] : make_double2(0.0, 0.0);
+        const double2 m2 = lane < nlf ? m[2 * lane + 1] : make_double2(0.0, 0.0);
+        const int nn = nlf * nlf;
+        for (int e0 = 0; e0 < nn; e0 += 64) {  // same trip count on every lane (shuffles)
+          const int e = e0 + lane, ec = min(e, nn - 1), g = ec / nlf, j = ec - g * nlf;
+          const double2 wv = w[ec];
+          const double2 a1 = make_double2(__shfl(m1.x, g), __shfl(m1.y, g));
+          const double2 a2 = make_double2(__shfl(m2.x, j), __shfl(m2.y, j));
+          if (e < nn) acc = c_add(acc, c_mul(c_mul(c_conj(a1), wv), a2));
+        }
+      } else {
+        for (int j = lane; j < nlf; j += 64) {
+          double2 t = make_double2(0.0, 0.0);
+          for (int g = 0; g < nlf; ++g) t = c_add(t, c_mul(c_conj(m[2 * g]), w[(size_t)g * nlf + j]));
+          acc = c_add(acc, c_mul(t, m[2 * j + 1]));
+        }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
