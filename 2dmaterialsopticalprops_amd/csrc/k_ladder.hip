@@ -44,7 +44,7 @@
 namespace rpa {
 
 namespace {
-constexpr int kLdMaxK = 1024;  // k-points of the kernel (fock is n x n)
+constexpr int kLdMaxK = 3072;  // k-points of the kernel (fock is n x n); the reference's nkd is 2700
 constexpr int kLdLdsK = 88;    // fock in LDS up to here (16 n^2 bytes)
 constexpr int kLbParts = 2;    // blocked form: frequency parts on their own streams (4 measured slower)
 }  // namespace
@@ -194,8 +194,8 @@ __global__ __launch_bounds__(256) void k_lb_form(LbArgs a) {
 }
 
 // One panel p0 .. p0 + w - 1 (w = min(NB, n - p0)) of every frequency: NT
-// threads, thread t holds active row act[t] (n - p0 <= NT).
-template <int NB, int NT>
+// threads, thread t holds active rows act[t + NT i], i < R (n - p0 <= R NT).
+template <int NB, int NT, int R>
 __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
   constexpr int NW = NT / 64;
   __shared__ double2 s_prow[NB];  // the pivot row of the step
@@ -204,36 +204,44 @@ __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
   __shared__ int s_ri[NW];
   __shared__ double2 s_l11[NB][NB];
   __shared__ int s_piv[NB];
-  __shared__ int s_cnt[NW];
+  __shared__ int s_cnt[R][NW];
   const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lf = blockIdx.x;
   const int m = n - p0, w = min(NB, m);
   double2 *T = a.T + (size_t)lf * n * ld;
   int *act = a.act + (size_t)lf * n;
-  const bool mine = tid < m;
-  const int row = mine ? act[tid] : 0;
-  bool live = mine;
-  double2 v[NB];
-  {
-    const double2 *src = T + (size_t)row * ld + p0;
+  bool mine[R], live[R];
+  int row[R];
+  double2 v[R][NB];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    mine[i] = tid + NT * i < m;
+    row[i] = mine[i] ? act[tid + NT * i] : 0;
+    live[i] = mine[i];
+    const double2 *src = T + (size_t)row[i] * ld + p0;
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
       const double2 x = src[min(c, w - 1)];
-      v[c] = (mine && c < w) ? x : make_double2(0.0, 0.0);
+      v[i][c] = (mine[i] && c < w) ? x : make_double2(0.0, 0.0);
     }
   }
   // Per pivot step: the waves' best rows meet in LDS (barrier), the pivot
   // row's owner publishes its values and 1 / pivot (barrier), the active
-  // rows eliminate.  Fully unrolled so that v[] stays in registers.
+  // rows eliminate.  Fully unrolled so that v[][] stays in registers.
 #pragma clang loop unroll(full)
   for (int j = 0; j < NB; ++j) {
     if (j < w) {  // uniform over the workgroup
       double best = -1.0;
       int bi = INT_MAX;
-      if (live) {
-        const double mm = fabs(v[j].x) + fabs(v[j].y);
-        best = (mm >= 0.0) ? mm : 0.0;  // a NaN entry competes as zero
-        bi = row;
-      }
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+        if (live[i]) {
+          double mm = fabs(v[i][j].x) + fabs(v[i][j].y);
+          mm = (mm >= 0.0) ? mm : 0.0;  // a NaN entry competes as zero
+          if (mm > best || (mm == best && row[i] < bi)) {
+            best = mm;
+            bi = row[i];
+          }
+        }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) {
         const double ob = __shfl_xor(best, off);
@@ -256,45 +264,62 @@ __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
           pv = s_rv[q];
           p = s_ri[q];
         }
-      if (live && row == p) {
 #pragma unroll
-        for (int c = j; c < NB; ++c) s_prow[c] = v[c];
+      for (int i = 0; i < R; ++i)
+        if (live[i] && row[i] == p) {
 #pragma unroll
-        for (int c = 0; c < j; ++c) s_l11[j][c] = v[c];
-        s_inv = c_div(make_double2(1.0, 0.0), v[j]);
-        s_piv[j] = p;
-        live = false;
-      }
+          for (int c = j; c < NB; ++c) s_prow[c] = v[i][c];
+#pragma unroll
+          for (int c = 0; c < j; ++c) s_l11[j][c] = v[i][c];
+          s_inv = c_div(make_double2(1.0, 0.0), v[i][j]);
+          s_piv[j] = p;
+          live[i] = false;
+        }
       if (tid == 0 && !(pv > 0.0)) atomicOr(a.status, 1);
       __syncthreads();
-      if (live) {
-        const double2 l = c_mul(v[j], s_inv);
-        v[j] = l;
 #pragma unroll
-        for (int c = j + 1; c < NB; ++c) v[c] = c_sub(v[c], c_mul(l, s_prow[c]));
-      }
+      for (int i = 0; i < R; ++i)
+        if (live[i]) {
+          const double2 l = c_mul(v[i][j], s_inv);
+          v[i][j] = l;
+#pragma unroll
+          for (int c = j + 1; c < NB; ++c) v[i][c] = c_sub(v[i][c], c_mul(l, s_prow[c]));
+        }
     }
   }
   // the panel in place: pivot rows hold their U11 row (and L11 multipliers
   // left of it), the other rows their L21 multipliers
-  if (mine) {
-    double2 *dst = T + (size_t)row * ld + p0;
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (c < w) dst[c] = v[c];
+  for (int i = 0; i < R; ++i)
+    if (mine[i]) {
+      double2 *dst = T + (size_t)row[i] * ld + p0;
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < w) dst[c] = v[i][c];
+    }
+  // rows still active, in row-list order (slot i of every thread, then i + 1)
+  unsigned long long bal[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    bal[i] = __ballot(live[i]);
+    if (lane == 0) s_cnt[i][wave] = __popcll(bal[i]);
   }
-  // rows still active, in row-list order
-  const unsigned long long bal = __ballot(live);
-  if (lane == 0) s_cnt[wave] = __popcll(bal);
-  __syncthreads();  // also: every thread has read its act entry and s_l11 / s_piv are complete
-  int off = __popcll(bal & ((1ull << lane) - 1ull));
-  for (int q = 0; q < wave; ++q) off += s_cnt[q];
-  if (live) act[off] = row;
+  __syncthreads();  // also: every thread has read its act entries and s_l11 / s_piv are complete
+  int base = 0;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    int off = base + __popcll(bal[i] & ((1ull << lane) - 1ull));
+    for (int q = 0; q < NW; ++q) {
+      if (q < wave) off += s_cnt[i][q];
+      base += s_cnt[i][q];
+    }
+    if (live[i]) act[off] = row[i];
+  }
   if (tid < w) a.piv[(size_t)lf * n + p0 + tid] = s_piv[tid];
-  // U12 = L11^-1 A12 on the pivot rows, one column per thread (the host
-  // launches NT >= ld - NB threads)
-  const int c = p0 + w + tid;
-  if (c < ld) {
+  // U12 = L11^-1 A12 on the pivot rows, a column per thread and pass (one
+  // pass when R = 1: the host launches NT >= ld - NB threads there, and a
+  // loop would let the compiler hold all of L11 in registers)
+  auto solve_col = [&](int c) {
     double2 x[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) x[j] = (j < w) ? T[(size_t)s_piv[j] * ld + c] : make_double2(0.0, 0.0);
@@ -308,6 +333,12 @@ __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
 #pragma unroll
     for (int j = 0; j < NB; ++j)
       if (j < w) T[(size_t)s_piv[j] * ld + c] = x[j];
+  };
+  if constexpr (R == 1) {
+    const int c = p0 + w + tid;
+    if (c < ld) solve_col(c);
+  } else {
+    for (int c = p0 + w + tid; c < ld; c += NT) solve_col(c);
   }
 }
 
@@ -391,20 +422,22 @@ __global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
 
 // Back substitution z_j = (y(P_j) - sum_{c>j} U(P_j, c) z_c) / U(P_j, j) in
 // blocks of NB pivots from the last, right-looking: the right-hand sides of
-// all pivots stay in LDS; once a block's z is solved (its triangle by one
-// wave with shuffles), every earlier pivot row subtracts that block's
-// columns (a thread per row, its NB values independent loads).  Then
-// chi_ladd and chi_para.
+// all pivots stay in LDS, each replaced by its z once solved; once a block's
+// z is solved (its triangle by one wave with shuffles), every earlier pivot
+// row subtracts that block's columns (a thread per row, its NB values
+// independent loads).  Then chi_ladd and chi_para.
 template <int NB>
 __global__ __launch_bounds__(256) void k_lb_back(LbArgs a) {
-  extern __shared__ double2 lds_dyn[];  // z [2][n], rhs [2][n], pivot rows [n] (int)
+  // dynamic LDS: rhs [2][n] (position j holds z_j once its block is
+  // solved: unknown j belongs to pivot step j), pivot rows [n] (int)
+  extern __shared__ double2 lds_dyn[];
   __shared__ double2 s_u[NB][NB + 1];
   __shared__ double red[5 * 8];
   const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lf = blockIdx.x, io = a.f0 + lf + 1;
   const double2 *T = a.T + (size_t)lf * n * ld;
-  double2 *z = lds_dyn, *rhs = lds_dyn + 2 * (size_t)n;
-  int *sp = (int *)(lds_dyn + 4 * (size_t)n);
+  double2 *rhs = lds_dyn, *z = lds_dyn;
+  int *sp = (int *)(lds_dyn + 2 * (size_t)n);
   for (int k = tid; k < n; k += 256) {
     const int pr = a.piv[(size_t)lf * n + k];
     sp[k] = pr;
@@ -451,23 +484,26 @@ __global__ __launch_bounds__(256) void k_lb_back(LbArgs a) {
   ladder_close(a.cur, a.delta, z, n, o, a.eta, a.nv, a.valley, io, a.chi_ladd, a.chi_para, red);
 }
 
-template <int NB, int NT>
+template <int NB, int NT, int R>
 hipError_t ladder_blocked(hipStream_t st, LbArgs a, int nf) {
   const int n = a.n, ld = a.ld;
-  if (n > NT || ld - NB > NT || n <= NB) return hipErrorInvalidValue;  // a row and a U12 column per thread
-  hipLaunchKernelGGL(k_lb_form, dim3((n + kLbFormRows - 1) / kLbFormRows, nf), dim3(256), sizeof(double2) * 2 * n,
-                     st, a);
+  // R active rows per thread; one U12 column per thread when R = 1
+  if (n > R * NT || (R == 1 && ld - NB > NT) || n <= NB) return hipErrorInvalidValue;
+  const size_t form_lds = sizeof(double2) * 2 * n;                    // 96 KB at 3072 k-points
+  const size_t back_lds = sizeof(double2) * 2 * n + sizeof(int) * n;  // 108 KB at 3072 k-points
+  hipError_t e = hipFuncSetAttribute((const void *)k_lb_form, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)form_lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void *)k_lb_back<NB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)back_lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lb_form, dim3((n + kLbFormRows - 1) / kLbFormRows, nf), dim3(256), form_lds, st, a);
   for (int p0 = 0; p0 < n; p0 += NB) {
     const int w = std::min(NB, n - p0), m = n - p0 - w;
-    hipLaunchKernelGGL((k_lb_panel<NB, NT>), dim3(nf), dim3(NT), 0, st, a, p0);
+    hipLaunchKernelGGL((k_lb_panel<NB, NT, R>), dim3(nf), dim3(NT), 0, st, a, p0);
     if (m > 0)
       hipLaunchKernelGGL(k_lb_update<NB>, dim3((ld - p0 - w + 63) / 64, (m + 63) / 64, nf), dim3(256), 0, st, a,
                          p0, w);
   }
-  const size_t back_lds = sizeof(double2) * 4 * n + sizeof(int) * n;  // 68 KB at 1024 k-points
-  hipError_t e = hipFuncSetAttribute((const void *)k_lb_back<NB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)back_lds);
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lb_back<NB>, dim3(nf), dim3(256), back_lds, st, a);
   return hipGetLastError();
 }
@@ -711,7 +747,8 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
       b.chi_para = dp;
       b.status = ds;
       // 32-column panels with 512 threads; past 512 k-points 16 columns with
-      // 1024 threads (one active row per thread).  Part i (frequencies
+      // 1024 threads (one active row per thread); past 1024 8 columns with
+      // 512 threads of four or six active rows (256 registers a lane there).  Part i (frequencies
       // f0 + i q .. f0 + (i + 1) q - 1, q = ceil(nf / kLbParts) <= part) on
       // stream i in region [i part, (i + 1) part).
       const int q = (nf + kLbParts - 1) / kLbParts;
@@ -723,7 +760,10 @@ extern "C" int rpa_ladder(const rpa_ladder_params *p, const double *fockk, const
         bi.T = dT + slice * part * i;
         bi.act = dact + (size_t)n * part * i;
         bi.piv = dpiv + (size_t)n * part * i;
-        e = n <= 512 ? ladder_blocked<32, 512>(sq[i], bi, cnt) : ladder_blocked<16, 1024>(sq[i], bi, cnt);
+        e = n <= 512    ? ladder_blocked<32, 512, 1>(sq[i], bi, cnt)
+            : n <= 1024 ? ladder_blocked<16, 1024, 1>(sq[i], bi, cnt)
+            : n <= 2048 ? ladder_blocked<8, 512, 4>(sq[i], bi, cnt)
+                        : ladder_blocked<8, 512, 6>(sq[i], bi, cnt);
       }
     } else {
       LadderArgs a{};

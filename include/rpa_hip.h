@@ -353,7 +353,7 @@ int rpa_fock_kernel(const rpa_fock_params *p, const double *kbz, const double *w
                     double *fockk);
 
 typedef struct rpa_ladder_params {
-  int nk;         /* nkbz: k-points of the Fock kernel (<= 1024)               */
+  int nk;         /* nkbz: k-points of the Fock kernel (<= 3072)               */
   int no;         /* frequencies io = 1..no                                     */
   int ntot;       /* Ntot (the kernel and currents are normalised by Ntot Vcell) */
   int device;     /* HIP device                                                  */

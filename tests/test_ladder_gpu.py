@@ -3,8 +3,12 @@ against the oracle (orc_ladder, Chi_ladd_up/BSE.f90:1116-1176 with the
 reference's gjel and its explicit lladd product).  The device solves
 fock z = xi0 conj(j) by Gaussian elimination with row pivoting instead of
 forming lladd: in LDS up to 88 k-points, blocked past that (32-column panels
-up to 512 k-points, 16-column panels with a row per thread of 1024 above,
+up to 512 k-points, 16-column panels with a row per thread of 1024 up to
+1024, 8-column panels with four or six rows per thread of 512 up to 3072,
 the trailing updates on the FP64 matrix cores): 1e-11 of the largest value.
+Past 600 k-points the oracle's inverse takes minutes, so the sizes of the
+8-column panels are checked against numpy's LAPACK solve of the same closed
+form, itself checked against the oracle here at 40 k-points.
 """
 
 import importlib
@@ -35,6 +39,50 @@ def test_ladder_matches_oracle(nk):
     cur = np.asfortranarray(rng.standard_normal((2, nk)) + 1j * rng.standard_normal((2, nk)))
     cl, cp = abi.ladder(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley)
     wl, wp = orc.ladder(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley)
+    assert np.isfinite(cl).all() and np.isfinite(cp).all()
+    assert H.rel_max(cl, wl) < 1e-11
+    assert H.rel_max(cp, wp) < 1e-13
+
+
+def _ladder_numpy(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley):
+    """chi_ladd = -valley j^T (I - L0 F / NV)^-1 L0 F L0 conj(j) / NV^2 and
+    chi_para = valley sum j (o / Delta) L0 conj(j) / NV, by LAPACK."""
+    n = len(delta)
+    nv = ntot * vcell
+    cl = np.zeros((2, no), complex)
+    cp = np.zeros((2, no), complex)
+    for io in range(no):
+        o = omin + io * dw
+        L0 = 1.0 / (o + delta + 1j * eta)
+        A = np.eye(n) - (L0[:, None] * F) / nv
+        Y = np.stack([L0 * (F @ (L0 * np.conj(cur[p]))) for p in range(2)], axis=1)
+        Z = np.linalg.solve(A, Y)
+        for p in range(2):
+            cl[p, io] = -valley * np.sum(cur[p] * Z[:, p]) / nv ** 2
+            cp[p, io] = valley * np.sum(cur[p] * (o / delta) * L0 * np.conj(cur[p])) / nv
+    return cl, cp
+
+
+# 1100: 8-column panels, four rows per thread; 2100: six rows per thread
+@pytest.mark.parametrize("nk", [1100, 2100])
+def test_ladder_large_kernels_match_lapack(nk):
+    print(f"[ladder] nk={nk} against LAPACK", flush=True)
+    rng = np.random.default_rng(nk)
+    no, ntot, vcell, valley = 2, 4 * nk, 3.0, 2.0
+    omin, dw, eta = 0.05, 0.04, 0.01
+    nv = ntot * vcell
+    # the closed form against the oracle first, on a small kernel
+    Fs = 0.3 * nv / np.sqrt(40) * (rng.standard_normal((40, 40)) + 1j * rng.standard_normal((40, 40)))
+    ds = rng.uniform(0.4, 2.0, 40)
+    cs = rng.standard_normal((2, 40)) + 1j * rng.standard_normal((2, 40))
+    wl, wp = orc.ladder(np.asfortranarray(Fs), ds, np.asfortranarray(cs), 3, omin, dw, eta, 4 * 40, vcell, valley)
+    nl, npara = _ladder_numpy(Fs, ds, cs, 3, omin, dw, eta, 4 * 40, vcell, valley)
+    assert H.rel_max(nl, wl) < 1e-11 and H.rel_max(npara, wp) < 1e-13
+    F = 0.3 * nv / np.sqrt(nk) * (rng.standard_normal((nk, nk)) + 1j * rng.standard_normal((nk, nk)))
+    delta = rng.uniform(0.4, 2.0, nk)
+    cur = rng.standard_normal((2, nk)) + 1j * rng.standard_normal((2, nk))
+    cl, cp = abi.ladder(np.asfortranarray(F), delta, np.asfortranarray(cur), no, omin, dw, eta, ntot, vcell, valley)
+    wl, wp = _ladder_numpy(F, delta, cur, no, omin, dw, eta, ntot, vcell, valley)
     assert np.isfinite(cl).all() and np.isfinite(cp).all()
     assert H.rel_max(cl, wl) < 1e-11
     assert H.rel_max(cp, wp) < 1e-13
@@ -86,7 +134,7 @@ def test_ladder_argument_checks():
         abi.ladder(F, np.array([1.0, 0.0, 1.0]), cur, 4, 0.1, 0.1, 0.01, 3, 1.0, 2.0)
     assert e.value.code == abi.RPA_E_ARG
     with pytest.raises(abi.RpaError) as e:
-        abi.ladder(np.zeros((1025, 1025)), np.ones(1025), np.ones((2, 1025)), 1, 0.1, 0.1, 0.01, 3, 1.0, 2.0)
+        abi.ladder(np.zeros((3073, 3073)), np.ones(3073), np.ones((2, 3073)), 1, 0.1, 0.1, 0.01, 3, 1.0, 2.0)
     assert e.value.code == abi.RPA_E_LIMIT
 
 
