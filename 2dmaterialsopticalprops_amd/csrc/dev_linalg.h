@@ -48,6 +48,28 @@ __device__ inline double2 c_sin(double2 z) {  // sin(x + iy) = sin x cosh y + i 
 }
 
 
+// (a, b) = (x / d, x % d) for x = x0, x0 + 256, ...: one division at the
+// start, then a step of (256 / d, 256 % d) with a carry.  An integer division
+// by a run-time value costs tens of instructions, several times the complex
+// multiply-add of the element it indexes.
+struct Walk256 {
+  int a, b, qa, qb, d;
+  __device__ Walk256(int x0, int d_) : d(d_) {
+    a = x0 / d;
+    b = x0 - a * d;
+    qa = 256 / d;
+    qb = 256 - qa * d;
+  }
+  __device__ void next() {
+    a += qa;
+    b += qb;
+    if (b >= d) {
+      b -= d;
+      ++a;
+    }
+  }
+};
+
 // In-place Gauss-Jordan inversion of the n x n matrix sa (row-major) with
 // row pivoting on |Re| + |Im| (lowest row on ties); a zero pivot sets the
 // status flag and goes on without a swap.
@@ -106,10 +128,9 @@ __device__ void gj_invert(double2 *sa, int n, double2 *fcol, int *piv, double *r
     for (int i = tid; i < n; i += 256)
       if (i != k) sa[i * n + k] = make_double2(0.0, 0.0);
     __syncthreads();
-    for (int x = tid; x < n * n; x += 256) {
-      const int i = x / n, j = x - i * n;
-      if (i != k) sa[x] = c_sub(sa[x], c_mul(fcol[i], sa[k * n + j]));
-    }
+    Walk256 wk(tid, n);  // (row, column) of x
+    for (int x = tid; x < n * n; x += 256, wk.next())
+      if (wk.a != k) sa[x] = c_sub(sa[x], c_mul(fcol[wk.a], sa[k * n + wk.b]));
     __syncthreads();
   }
   for (int k = n - 1; k >= 0; --k) {  // row interchanges undone as column interchanges
@@ -188,22 +209,29 @@ __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fco
     for (int i = k + 1 + tid; i < n; i += 256) fcol[i] = c_mul(sa[(size_t)i * n + k], inv);
     __syncthreads();
     const int m = n - 1 - k;  // the trailing block is m x m
-    for (int x = tid; x < m * m; x += 256) {
-      const int i = k + 1 + x / m, j = k + 1 + x % m;
-      sa[(size_t)i * n + j] = c_sub(sa[(size_t)i * n + j], c_mul(fcol[i], sa[(size_t)k * n + j]));
-    }
-    for (int x = tid; x < nr * m; x += 256) {
-      const int r = x / m, i = k + 1 + x % m;
-      y[r * n + i] = c_sub(y[r * n + i], c_mul(fcol[i], y[r * n + k]));
+    if (m > 0) {
+      Walk256 wk(tid, m);  // (row, column) of the block
+      for (int x = tid; x < m * m; x += 256, wk.next()) {
+        const int i = k + 1 + wk.a, j = k + 1 + wk.b;
+        sa[(size_t)i * n + j] = c_sub(sa[(size_t)i * n + j], c_mul(fcol[i], sa[(size_t)k * n + j]));
+      }
+      Walk256 wr(tid, m);  // (right-hand side, row)
+      for (int x = tid; x < nr * m; x += 256, wr.next()) {
+        const int r = wr.a, i = k + 1 + wr.b;
+        y[r * n + i] = c_sub(y[r * n + i], c_mul(fcol[i], y[r * n + k]));
+      }
     }
     __syncthreads();
   }
   for (int i = n - 1; i >= 0; --i) {
     for (int r = tid; r < nr; r += 256) y[r * n + i] = c_div(y[r * n + i], sa[(size_t)i * n + i]);
     __syncthreads();
-    for (int x = tid; x < nr * i; x += 256) {
-      const int r = x / i, j = x % i;
-      y[r * n + j] = c_sub(y[r * n + j], c_mul(sa[(size_t)j * n + i], y[r * n + i]));
+    if (i > 0) {
+      Walk256 wb(tid, i);  // (right-hand side, row)
+      for (int x = tid; x < nr * i; x += 256, wb.next()) {
+        const int r = wb.a, j = wb.b;
+        y[r * n + j] = c_sub(y[r * n + j], c_mul(sa[(size_t)j * n + i], y[r * n + i]));
+      }
     }
     __syncthreads();
   }
