@@ -72,10 +72,10 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
   const int line = p / a.nf, io = p - line * a.nf + 1;
   double2 *sa = GLOBAL ? a.T + (size_t)lp * M * M : lds_dyn;
   double2 *tab = lds_dyn + (GLOBAL ? 0 : (size_t)M * M);
-  // [0,N) s_eg; [N,15N) the tables of the T builds, then (solve form) the
+  // [0,N) s_vs; [N,15N) the tables of the T builds, then (solve form) the
   // eight p-mode right-hand sides [8][2N] in [N,17N); [15N,17N) the two
   // s-mode right-hand sides; [17N,19N) fcol
-  double2 *s_eg = tab, *s_aii = tab + N, *s_pd = tab + 2 * N, *s_d1 = tab + 3 * N, *s_d2 = tab + 4 * N;
+  double2 *s_vs = tab, *s_aii = tab + N, *s_pd = tab + 2 * N, *s_d1 = tab + 3 * N, *s_d2 = tab + 4 * N;
   double2 *s_u = tab + 5 * N, *s_v = tab + 10 * N;  // [5][N]
   double2 *ys = tab + 15 * N, *yp = tab + N;
   double2 *fcol = tab + 17 * N;                      // [2N]
@@ -110,7 +110,10 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
     s_pd[i] = c_div(make_double2(pr, 0.0), den);                 // par_j / (beta^2 - Gz_j^2)
     s_d1[i] = c_div(c_mul(make_double2(0.0, 2.0), beta), den);   // diagonal1 (640)
     s_d2[i] = c_div(make_double2(0.0, 2.0 * gz), den);           // diagonal2 (641-642)
-    s_eg[i] = c_expi(gz * c0 / 2.0);                             // exp(i Gz c0/2) (sumIntScreenedPi)
+    // the slab integral vs(i) = int_0^{c0/2} e^{i Gz z} dz (c0/2 at the first
+    // field), once per field: the sums below read it per element
+    const double2 eg = c_expi(gz * c0 / 2.0);
+    s_vs[i] = i == 0 ? make_double2(c0 / 2.0, 0.0) : make_double2(eg.y / gz, -(eg.x - 1.0) / gz);
     // form factors of genSpectra (1009-1013), real parts as the real Ff/Gf take them
     const double2 sn = c_scale(c_sin(c_scale(beta, c0 / 2.0)), 2.0 * pr / sqrt(c0));
     s_ff[i] = c_div(sn, make_double2(beta.x + gz, beta.y)).x;
@@ -145,30 +148,13 @@ __global__ __launch_bounds__(256) void k_photon(PhotonArgs a) {
     return c_add(c_mul(c_mul(s_pd[j], tbc), w), c_mul(pij, s_d2[j]));
   };
   auto P = [&](int c, int i, int j) { return P0[((size_t)c * N + i) * N + j]; };
-  // sumIntScreenedPi weight of (iG, jG) (Photonew.f90:846-879), times 1/c0 at the end
-  auto wsum = [&](int i, int j) {
-    if (i == 0 && j == 0) return make_double2(c0 * c0 / 4.0, 0.0);
-    if (i == 0) {  // -i (-1 + e_j) c0 / (2 Gz_j)
-      const double2 t = make_double2(s_eg[j].x - 1.0, s_eg[j].y);
-      return c_scale(make_double2(t.y, -t.x), c0 / (2.0 * s_gz[j]));
-    }
-    if (j == 0) {  // -i (1 - conj(e_i)) c0 / (2 Gz_i)
-      const double2 t = make_double2(1.0 - s_eg[i].x, s_eg[i].y);
-      return c_scale(make_double2(t.y, -t.x), c0 / (2.0 * s_gz[i]));
-    }
-    const double2 num = c_mul(make_double2(s_eg[i].x - 1.0, s_eg[i].y), make_double2(s_eg[j].x - 1.0, s_eg[j].y));
-    return c_scale(c_div(num, c_scale(s_eg[i], s_gz[i] * s_gz[j])), -1.0);
-  };
-  // the weights factor as w(i,j) = us(i) vs(j): us(i) = int_0^{c0/2} e^{-i Gz_i z} dz,
-  // vs(j) = int_0^{c0/2} e^{i Gz_j z} dz (c0/2 at the first field)
-  auto vslab = [&](int j) {
-    if (j == 0) return make_double2(c0 / 2.0, 0.0);
-    return make_double2(s_eg[j].y / s_gz[j], -(s_eg[j].x - 1.0) / s_gz[j]);
-  };
-  auto uslab = [&](int i) {
-    if (i == 0) return make_double2(c0 / 2.0, 0.0);
-    return make_double2(s_eg[i].y / s_gz[i], (s_eg[i].x - 1.0) / s_gz[i]);
-  };
+  // sumIntScreenedPi weight of (iG, jG) (Photonew.f90:846-879), times 1/c0 at
+  // the end: it factors as w(i,j) = us(i) vs(j) with vs(j) = int_0^{c0/2}
+  // e^{i Gz_j z} dz and us(i) = conj(vs(i)) (c0/2 at the first field), the
+  // reference's four cases (0,0), (0,j), (i,0), (i,j) included
+  auto vslab = [&](int j) { return s_vs[j]; };
+  auto uslab = [&](int i) { return c_conj(s_vs[i]); };
+  auto wsum = [&](int i, int j) { return c_mul(c_conj(s_vs[i]), s_vs[j]); };
   double acc[20];
 #pragma unroll
   for (int e = 0; e < 20; ++e) acc[e] = 0.0;
