@@ -151,7 +151,9 @@ __device__ void gj_invert(double2 *sa, int n, double2 *fcol, int *piv, double *r
 // their column-k entries (the multipliers are not kept) and the right-hand
 // sides follow, then back substitution column by column; z overwrites y.
 // A zero pivot sets the status flag and goes on without a swap.  n^3/3
-// multiply-adds and traffic for A where its inverse takes n^3.
+// multiply-adds and traffic for A where its inverse takes n^3.  Indices are
+// 32-bit (n <= 128 in every caller; a slice's offset is in sa already), so
+// the LDS addresses need no 64-bit arithmetic.
 template <bool GLOBAL>
 __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fcol, double *red_v, int *red_i,
                             int *status) {
@@ -160,7 +162,7 @@ __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fco
     double best = -1.0;
     int bi = n;
     for (int i = k + tid; i < n; i += 256) {
-      const double2 z = sa[(size_t)i * n + k];
+      const double2 z = sa[i * n + k];
       const double m = fabs(z.x) + fabs(z.y);
       if (m > best) {
         best = m;
@@ -194,9 +196,9 @@ __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fco
     }
     if (p != k) {
       for (int j = k + tid; j < n; j += 256) {
-        const double2 t = sa[(size_t)k * n + j];
-        sa[(size_t)k * n + j] = sa[(size_t)p * n + j];
-        sa[(size_t)p * n + j] = t;
+        const double2 t = sa[k * n + j];
+        sa[k * n + j] = sa[p * n + j];
+        sa[p * n + j] = t;
       }
       for (int r = tid; r < nr; r += 256) {
         const double2 t = y[r * n + k];
@@ -205,15 +207,15 @@ __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fco
       }
     }
     __syncthreads();
-    const double2 inv = c_div(make_double2(1.0, 0.0), sa[(size_t)k * n + k]);
-    for (int i = k + 1 + tid; i < n; i += 256) fcol[i] = c_mul(sa[(size_t)i * n + k], inv);
+    const double2 inv = c_div(make_double2(1.0, 0.0), sa[k * n + k]);
+    for (int i = k + 1 + tid; i < n; i += 256) fcol[i] = c_mul(sa[i * n + k], inv);
     __syncthreads();
     const int m = n - 1 - k;  // the trailing block is m x m
     if (m > 0) {
       Walk256 wk(tid, m);  // (row, column) of the block
       for (int x = tid; x < m * m; x += 256, wk.next()) {
         const int i = k + 1 + wk.a, j = k + 1 + wk.b;
-        sa[(size_t)i * n + j] = c_sub(sa[(size_t)i * n + j], c_mul(fcol[i], sa[(size_t)k * n + j]));
+        sa[i * n + j] = c_sub(sa[i * n + j], c_mul(fcol[i], sa[k * n + j]));
       }
       Walk256 wr(tid, m);  // (right-hand side, row)
       for (int x = tid; x < nr * m; x += 256, wr.next()) {
@@ -224,13 +226,13 @@ __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fco
     __syncthreads();
   }
   for (int i = n - 1; i >= 0; --i) {
-    for (int r = tid; r < nr; r += 256) y[r * n + i] = c_div(y[r * n + i], sa[(size_t)i * n + i]);
+    for (int r = tid; r < nr; r += 256) y[r * n + i] = c_div(y[r * n + i], sa[i * n + i]);
     __syncthreads();
     if (i > 0) {
       Walk256 wb(tid, i);  // (right-hand side, row)
       for (int x = tid; x < nr * i; x += 256, wb.next()) {
         const int r = wb.a, j = wb.b;
-        y[r * n + j] = c_sub(y[r * n + j], c_mul(sa[(size_t)j * n + i], y[r * n + i]));
+        y[r * n + j] = c_sub(y[r * n + j], c_mul(sa[j * n + i], y[r * n + i]));
       }
     }
     __syncthreads();
