@@ -40,7 +40,8 @@ sys.path.insert(0, ROOT)
 def host_threads():
     """OpenMP threads for the CPU baseline: every core this process may run
     on (its affinity set), or OMP_NUM_THREADS where the environment sets it
-    (the GPU box sets the host share there).  Threads are bound close."""
+    (a shared host states the cores granted to the process there).  Threads
+    are bound close."""
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     env = os.environ.get("OMP_NUM_THREADS", "")
     return int(env) if env.isdigit() and int(env) > 0 else n
