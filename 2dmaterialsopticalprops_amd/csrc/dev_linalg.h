@@ -157,6 +157,7 @@ __device__ void gj_invert(double2 *sa, int n, double2 *fcol, int *piv, double *r
 template <bool GLOBAL>
 __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fcol, double *red_v, int *red_i,
                             int *status) {
+  __shared__ double2 s_inv;  // 1 / pivot, formed once per step
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int k = 0; k < n; ++k) {
     double best = -1.0;
@@ -194,6 +195,8 @@ __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fco
       if (tid == 0) atomicOr(status, 1);
       p = k;
     }
+    // thread 0 swaps column k itself, so it reads the pivot before any write
+    if (tid == 0) s_inv = c_div(make_double2(1.0, 0.0), sa[p * n + k]);
     if (p != k) {
       for (int j = k + tid; j < n; j += 256) {
         const double2 t = sa[k * n + j];
@@ -207,7 +210,7 @@ __device__ void gauss_solve(double2 *sa, int n, double2 *y, int nr, double2 *fco
       }
     }
     __syncthreads();
-    const double2 inv = c_div(make_double2(1.0, 0.0), sa[k * n + k]);
+    const double2 inv = s_inv;
     for (int i = k + 1 + tid; i < n; i += 256) fcol[i] = c_mul(sa[i * n + k], inv);
     __syncthreads();
     const int m = n - 1 - k;  // the trailing block is m x m
