@@ -376,20 +376,34 @@ __global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
         ar[i][j][e] = cv.x;
         ai[i][j][e] = cv.y;
       }
+  // the next step's L and U values are loaded into registers before this
+  // step's products, so their loads are in flight while the products run
+  constexpr int NU = 64 * KH / 256;
+  double2 rl[NU], ru[NU];
+  auto load_step = [&](int h) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int x = tid + 256 * u;
+      const int rr = x / KH, k = x - rr * KH, kg = h * KH + k;
+      rl[u] = T[(size_t)s_row[rr] * ld + p0 + min(kg, w - 1)];
+      const int k2 = x >> 6, cc = x & 63, kg2 = h * KH + k2;
+      ru[u] = T[(size_t)s_pr[kg2] * ld + min(c0 + cc, ld - 1)];
+    }
+  };
+  load_step(0);
 #pragma unroll
   for (int h = 0; h < NB / KH; ++h) {
     if (h > 0) __syncthreads();  // the previous step's operand reads are done
-    for (int x = tid; x < 64 * KH; x += 256) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int x = tid + 256 * u;
       const int rr = x / KH, k = x - rr * KH, kg = h * KH + k;
-      const double2 l = T[(size_t)s_row[rr] * ld + p0 + min(kg, w - 1)];
-      sL[rr][k] = (kg < w) ? l : make_double2(0.0, 0.0);
-    }
-    for (int x = tid; x < KH * 64; x += 256) {
-      const int k = x >> 6, cc = x & 63, kg = h * KH + k;
-      const double2 u = T[(size_t)s_pr[kg] * ld + min(c0 + cc, ld - 1)];
-      sU[k][cc] = (kg < w) ? u : make_double2(0.0, 0.0);
+      sL[rr][k] = (kg < w) ? rl[u] : make_double2(0.0, 0.0);
+      const int k2 = x >> 6, cc = x & 63, kg2 = h * KH + k2;
+      sU[k2][cc] = (kg2 < w) ? ru[u] : make_double2(0.0, 0.0);
     }
     __syncthreads();
+    if (h + 1 < NB / KH) load_step(h + 1);
 #pragma unroll
     for (int kk = 0; kk < KH / 4; ++kk) {
       const int k = 4 * kk + q4;
