@@ -195,8 +195,10 @@ __global__ __launch_bounds__(256) void k_lb_form(LbArgs a) {
 
 // One panel p0 .. p0 + w - 1 (w = min(NB, n - p0)) of every frequency: NT
 // threads, thread t holds active rows act[t + NT i], i < R (n - p0 <= R NT).
+// mode 0: factor and U12; 1: factor only; 2: U12 only (L11 read back from
+// the pivot rows, after their trailing columns were brought up to date)
 template <int NB, int NT, int R>
-__global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
+__global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0, int mode) {
   constexpr int NW = NT / 64;
   __shared__ double2 s_prow[NB];  // the pivot row of the step
   __shared__ double2 s_inv;       // 1 / pivot
@@ -209,113 +211,124 @@ __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
   const int m = n - p0, w = min(NB, m);
   double2 *T = a.T + (size_t)lf * n * ld;
   int *act = a.act + (size_t)lf * n;
-  bool mine[R], live[R];
-  int row[R];
-  double2 v[R][NB];
+  if (mode != 2) {
+    bool mine[R], live[R];
+    int row[R];
+    double2 v[R][NB];
 #pragma unroll
-  for (int i = 0; i < R; ++i) {
-    mine[i] = tid + NT * i < m;
-    row[i] = mine[i] ? act[tid + NT * i] : 0;
-    live[i] = mine[i];
-    const double2 *src = T + (size_t)row[i] * ld + p0;
+    for (int i = 0; i < R; ++i) {
+      mine[i] = tid + NT * i < m;
+      row[i] = mine[i] ? act[tid + NT * i] : 0;
+      live[i] = mine[i];
+      const double2 *src = T + (size_t)row[i] * ld + p0;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-      const double2 x = src[min(c, w - 1)];
-      v[i][c] = (mine[i] && c < w) ? x : make_double2(0.0, 0.0);
+      for (int c = 0; c < NB; ++c) {
+        const double2 x = src[min(c, w - 1)];
+        v[i][c] = (mine[i] && c < w) ? x : make_double2(0.0, 0.0);
+      }
     }
-  }
-  // Per pivot step: the waves' best rows meet in LDS (barrier), the pivot
-  // row's owner publishes its values and 1 / pivot (barrier), the active
-  // rows eliminate.  Fully unrolled so that v[][] stays in registers.
+    // Per pivot step: the waves' best rows meet in LDS (barrier), the pivot
+    // row's owner publishes its values and 1 / pivot (barrier), the active
+    // rows eliminate.  Fully unrolled so that v[][] stays in registers.
 #pragma clang loop unroll(full)
-  for (int j = 0; j < NB; ++j) {
-    if (j < w) {  // uniform over the workgroup
-      double best = -1.0;
-      int bi = INT_MAX;
+    for (int j = 0; j < NB; ++j) {
+      if (j < w) {  // uniform over the workgroup
+        double best = -1.0;
+        int bi = INT_MAX;
 #pragma unroll
-      for (int i = 0; i < R; ++i)
-        if (live[i]) {
-          double mm = fabs(v[i][j].x) + fabs(v[i][j].y);
-          mm = (mm >= 0.0) ? mm : 0.0;  // a NaN entry competes as zero
-          if (mm > best || (mm == best && row[i] < bi)) {
-            best = mm;
-            bi = row[i];
+        for (int i = 0; i < R; ++i)
+          if (live[i]) {
+            double mm = fabs(v[i][j].x) + fabs(v[i][j].y);
+            mm = (mm >= 0.0) ? mm : 0.0;  // a NaN entry competes as zero
+            if (mm > best || (mm == best && row[i] < bi)) {
+              best = mm;
+              bi = row[i];
+            }
+          }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ob = __shfl_xor(best, off);
+          const int oi = __shfl_xor(bi, off);
+          if (ob > best || (ob == best && oi < bi)) {
+            best = ob;
+            bi = oi;
           }
         }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ob > best || (ob == best && oi < bi)) {
-          best = ob;
-          bi = oi;
+        if (lane == 0) {
+          s_rv[wave] = best;
+          s_ri[wave] = bi;
         }
+        __syncthreads();
+        double pv = s_rv[0];
+        int p = s_ri[0];
+#pragma unroll
+        for (int q = 1; q < NW; ++q)
+          if (s_rv[q] > pv || (s_rv[q] == pv && s_ri[q] < p)) {
+            pv = s_rv[q];
+            p = s_ri[q];
+          }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+          if (live[i] && row[i] == p) {
+#pragma unroll
+            for (int c = j; c < NB; ++c) s_prow[c] = v[i][c];
+#pragma unroll
+            for (int c = 0; c < j; ++c) s_l11[j][c] = v[i][c];
+            s_inv = c_div(make_double2(1.0, 0.0), v[i][j]);
+            s_piv[j] = p;
+            live[i] = false;
+          }
+        if (tid == 0 && !(pv > 0.0)) atomicOr(a.status, 1);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+          if (live[i]) {
+            const double2 l = c_mul(v[i][j], s_inv);
+            v[i][j] = l;
+#pragma unroll
+            for (int c = j + 1; c < NB; ++c) v[i][c] = c_sub(v[i][c], c_mul(l, s_prow[c]));
+          }
       }
-      if (lane == 0) {
-        s_rv[wave] = best;
-        s_ri[wave] = bi;
+    }
+    // the panel in place: pivot rows hold their U11 row (and L11 multipliers
+    // left of it), the other rows their L21 multipliers
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      if (mine[i]) {
+        double2 *dst = T + (size_t)row[i] * ld + p0;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < w) dst[c] = v[i][c];
       }
-      __syncthreads();
-      double pv = s_rv[0];
-      int p = s_ri[0];
+    // rows still active, in row-list order (slot i of every thread, then i + 1)
+    unsigned long long bal[R];
 #pragma unroll
-      for (int q = 1; q < NW; ++q)
-        if (s_rv[q] > pv || (s_rv[q] == pv && s_ri[q] < p)) {
-          pv = s_rv[q];
-          p = s_ri[q];
-        }
-#pragma unroll
-      for (int i = 0; i < R; ++i)
-        if (live[i] && row[i] == p) {
-#pragma unroll
-          for (int c = j; c < NB; ++c) s_prow[c] = v[i][c];
-#pragma unroll
-          for (int c = 0; c < j; ++c) s_l11[j][c] = v[i][c];
-          s_inv = c_div(make_double2(1.0, 0.0), v[i][j]);
-          s_piv[j] = p;
-          live[i] = false;
-        }
-      if (tid == 0 && !(pv > 0.0)) atomicOr(a.status, 1);
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < R; ++i)
-        if (live[i]) {
-          const double2 l = c_mul(v[i][j], s_inv);
-          v[i][j] = l;
-#pragma unroll
-          for (int c = j + 1; c < NB; ++c) v[i][c] = c_sub(v[i][c], c_mul(l, s_prow[c]));
-        }
+    for (int i = 0; i < R; ++i) {
+      bal[i] = __ballot(live[i]);
+      if (lane == 0) s_cnt[i][wave] = __popcll(bal[i]);
     }
-  }
-  // the panel in place: pivot rows hold their U11 row (and L11 multipliers
-  // left of it), the other rows their L21 multipliers
+    __syncthreads();  // also: every thread has read its act entries and s_l11 / s_piv are complete
+    int base = 0;
 #pragma unroll
-  for (int i = 0; i < R; ++i)
-    if (mine[i]) {
-      double2 *dst = T + (size_t)row[i] * ld + p0;
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (c < w) dst[c] = v[i][c];
+    for (int i = 0; i < R; ++i) {
+      int off = base + __popcll(bal[i] & ((1ull << lane) - 1ull));
+      for (int q = 0; q < NW; ++q) {
+        if (q < wave) off += s_cnt[i][q];
+        base += s_cnt[i][q];
+      }
+      if (live[i]) act[off] = row[i];
     }
-  // rows still active, in row-list order (slot i of every thread, then i + 1)
-  unsigned long long bal[R];
-#pragma unroll
-  for (int i = 0; i < R; ++i) {
-    bal[i] = __ballot(live[i]);
-    if (lane == 0) s_cnt[i][wave] = __popcll(bal[i]);
-  }
-  __syncthreads();  // also: every thread has read its act entries and s_l11 / s_piv are complete
-  int base = 0;
-#pragma unroll
-  for (int i = 0; i < R; ++i) {
-    int off = base + __popcll(bal[i] & ((1ull << lane) - 1ull));
-    for (int q = 0; q < NW; ++q) {
-      if (q < wave) off += s_cnt[i][q];
-      base += s_cnt[i][q];
+    if (tid < w) a.piv[(size_t)lf * n + p0 + tid] = s_piv[tid];
+  } else {
+    for (int x = tid; x < NB * NB; x += NT) {
+      const int jj = x / NB, i = x - jj * NB;
+      const double2 t = T[(size_t)a.piv[(size_t)lf * n + p0 + min(jj, w - 1)] * ld + p0 + i];
+      s_l11[jj][i] = (jj < w && i < jj) ? t : make_double2(0.0, 0.0);
     }
-    if (live[i]) act[off] = row[i];
+    if (tid < NB) s_piv[tid] = a.piv[(size_t)lf * n + p0 + min(tid, w - 1)];
+    __syncthreads();
   }
-  if (tid < w) a.piv[(size_t)lf * n + p0 + tid] = s_piv[tid];
+  if (mode == 1) return;
   // U12 = L11^-1 A12 on the pivot rows, a column per thread and pass (one
   // pass when R = 1: the host launches NT >= ld - NB threads there, and a
   // loop would let the compiler hold all of L11 in registers)
@@ -342,24 +355,27 @@ __global__ __launch_bounds__(NT) void k_lb_panel(LbArgs a, int p0) {
   }
 }
 
-// Trailing update after panel p0 (width w): for the m = n - p0 - w active
-// rows r and columns c in p0 + w .. ld - 1,
+// Update with the multipliers of columns p0 .. p0 + w - 1: for m rows r (the
+// active list, or the pivot rows of columns rows_at ..) and columns c in
+// cbeg .. cend - 1,
 //   T[r][c] -= sum_{j<w} T[r][p0 + j] T[piv(p0 + j)][c]
 // 64 x 64 complex tiles, frequency in blockIdx.z; each wave 32 x 32 as
 // 2 x 2 blocks of 16 x 16, four real products per complex one.  The L and
 // U operands pass through LDS in steps of 16 (34 KB, four workgroups per
 // compute unit); the tile of T itself goes straight to the accumulators.
 template <int NB>
-__global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
+__global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w, int rows_at, int m, int cbeg, int cend) {
   constexpr int KH = NB < 16 ? NB : 16;
   __shared__ double2 sL[64][KH + 1];
   __shared__ double2 sU[KH][65];
   __shared__ int s_row[64];
   __shared__ int s_pr[NB];
   const int n = a.n, ld = a.ld, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lf = blockIdx.z;
-  const int m = n - p0 - w, c0 = p0 + w + blockIdx.x * 64, r0 = blockIdx.y * 64;
+  const int c0 = cbeg + blockIdx.x * 64, r0 = blockIdx.y * 64;
   double2 *T = a.T + (size_t)lf * n * ld;
-  const int *act = a.act + (size_t)lf * n, *piv = a.piv + (size_t)lf * n + p0;
+  // rows: the active list, or (rows_at >= 0) the pivot rows of columns rows_at ..
+  const int *act = rows_at < 0 ? a.act + (size_t)lf * n : a.piv + (size_t)lf * n + rows_at;
+  const int *piv = a.piv + (size_t)lf * n + p0;
   const int r = lane & 15, q4 = lane >> 4, wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   if (tid < 64) s_row[tid] = act[min(r0 + tid, m - 1)];
   if (tid < NB) s_pr[tid] = piv[min(tid, w - 1)];
@@ -371,7 +387,7 @@ __global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int rr = wm + 16 * i + 4 * e + q4, cc = min(c0 + wn + 16 * j + r, ld - 1);
+        const int rr = wm + 16 * i + 4 * e + q4, cc = min(c0 + wn + 16 * j + r, cend - 1);
         const double2 cv = T[(size_t)s_row[rr] * ld + cc];
         ar[i][j][e] = cv.x;
         ai[i][j][e] = cv.y;
@@ -387,7 +403,7 @@ __global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
       const int rr = x / KH, k = x - rr * KH, kg = h * KH + k;
       rl[u] = T[(size_t)s_row[rr] * ld + p0 + min(kg, w - 1)];
       const int k2 = x >> 6, cc = x & 63, kg2 = h * KH + k2;
-      ru[u] = T[(size_t)s_pr[kg2] * ld + min(c0 + cc, ld - 1)];
+      ru[u] = T[(size_t)s_pr[kg2] * ld + min(c0 + cc, cend - 1)];
     }
   };
   load_step(0);
@@ -430,7 +446,7 @@ __global__ __launch_bounds__(256) void k_lb_update(LbArgs a, int p0, int w) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int rr = wm + 16 * i + 4 * e + q4, cc = c0 + wn + 16 * j + r;
-        if (r0 + rr < m && cc < ld) T[(size_t)s_row[rr] * ld + cc] = make_double2(ar[i][j][e], ai[i][j][e]);
+        if (r0 + rr < m && cc < cend) T[(size_t)s_row[rr] * ld + cc] = make_double2(ar[i][j][e], ai[i][j][e]);
       }
 }
 
@@ -511,12 +527,27 @@ hipError_t ladder_blocked(hipStream_t st, LbArgs a, int nf) {
     e = hipFuncSetAttribute((const void *)k_lb_back<NB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)back_lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_lb_form, dim3((n + kLbFormRows - 1) / kLbFormRows, nf), dim3(256), form_lds, st, a);
-  for (int p0 = 0; p0 < n; p0 += NB) {
-    const int w = std::min(NB, n - p0), m = n - p0 - w;
-    hipLaunchKernelGGL((k_lb_panel<NB, NT, R>), dim3(nf), dim3(NT), 0, st, a, p0);
-    if (m > 0)
-      hipLaunchKernelGGL(k_lb_update<NB>, dim3((ld - p0 - w + 63) / 64, (m + 63) / 64, nf), dim3(256), 0, st, a,
-                         p0, w);
+  auto update = [&](auto kern, int p0, int w, int rows_at, int m, int cbeg, int cend) {
+    if (m > 0 && cend > cbeg)
+      hipLaunchKernelGGL(kern, dim3((cend - cbeg + 63) / 64, (m + 63) / 64, nf), dim3(256), 0, st, a, p0, w, rows_at, m,
+                         cbeg, cend);
+  };
+  // Panels in pairs: panel p with its U12; panel p+1's columns brought up to
+  // date (narrow update); panel p+1 factored; its pivot rows brought up to
+  // date on the columns right of it; its U12; then one update of the trailing
+  // block with the 2 NB multiplier columns of both, which is read and
+  // written once per pair instead of once per panel.
+  for (int p0 = 0; p0 < n;) {
+    const int w1 = std::min(NB, n - p0), q0 = p0 + w1;
+    hipLaunchKernelGGL((k_lb_panel<NB, NT, R>), dim3(nf), dim3(NT), 0, st, a, p0, 0);
+    if (q0 >= n) break;
+    const int w2 = std::min(NB, n - q0), e0 = q0 + w2;
+    update(k_lb_update<NB>, p0, w1, -1, n - q0, q0, e0);
+    hipLaunchKernelGGL((k_lb_panel<NB, NT, R>), dim3(nf), dim3(NT), 0, st, a, q0, 1);
+    update(k_lb_update<NB>, p0, w1, q0, w2, e0, ld);
+    hipLaunchKernelGGL((k_lb_panel<NB, NT, R>), dim3(nf), dim3(NT), 0, st, a, q0, 2);
+    update(k_lb_update<2 * NB>, p0, w1 + w2, -1, n - e0, e0, ld);
+    p0 = e0;
   }
   hipLaunchKernelGGL(k_lb_back<NB>, dim3(nf), dim3(256), back_lds, st, a);
   return hipGetLastError();
@@ -529,8 +560,9 @@ hipError_t ladder_blocked(hipStream_t st, LbArgs a, int nf) {
 // local fields the lanes walk the W block contiguously over (G, G') and add
 // conj(M1(G)) W M2(G') (M1, M2 one value per lane, shared by shuffles);
 // beyond, lane j takes columns G' = j, j + 64, ...: t = sum_G conj(M1(G))
-// W(iq,G,G'), then t M2(G').  The wave sums the lanes.  Pairs whose iq falls outside 2..nq get 0 (the
-// reference reads W only for those and leaves such entries unset).
+// W(iq,G,G'), then t M2(G').  The wave sums the lanes.  Pairs whose iq
+// falls outside 2..nq get 0 (the reference reads W only for those and leaves
+// such entries unset).
 __global__ __launch_bounds__(256) void k_fock(int nk, int nlf, int nq, double gcar, double qmin, double diag,
                                               const double *__restrict__ kbz, const double2 *__restrict__ W,
                                               const double2 *__restrict__ M, double2 *__restrict__ fockk) {
